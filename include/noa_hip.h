@@ -1,0 +1,165 @@
+/*
+ * noa_hip.h -- C ABI of the MI355X (gfx950) central-slice library.
+ *
+ * Plain C: pointers, sizes and enums only. No HIP, torch or C++ types appear in the signatures, so the
+ * library can be bound from C++, Python (ctypes), or any FFI.
+ *
+ * Each entry point replaces the GPU branch of one noa public function, i.e. the code between the
+ * parameter check and the CUDA iwise launch:
+ *   noa_hip_rasterize_central_slices_3d  <- noa::xform::rasterize_central_slices_3d
+ *                                           (src/noa/xform/FourierProject.hpp:1653-1692, launcher :1260-1317)
+ *   noa_hip_insert_central_slices_3d     <- noa::xform::insert_central_slices_3d
+ *                                           (src/noa/xform/FourierProject.hpp:1770-1815, launcher :1319-1394)
+ *   noa_hip_extract_central_slices_3d    <- noa::xform::extract_central_slices_3d
+ *                                           (src/noa/xform/FourierProject.hpp:1869-1914, launcher :1396-1487)
+ *
+ * Conventions (same as noa):
+ *   - Shapes are BDHW. Array shapes are rfft shapes, i.e. logical_shape.rfft() = (B, D, H, W/2+1)
+ *     (src/noa/runtime/core/Shape.hpp:380). Strides are in elements, not bytes.
+ *   - Slices are (B, 1, H, W/2+1) stacks; a volume is (1, D, H, W/2+1) (FourierProject.hpp:1149-1159).
+ *   - Complex values are interleaved {real, imag} pairs (src/noa/base/Complex.hpp:25-29).
+ *   - Mat33 elements are row-major 3x3 with a 48-byte stride for f32 (alignas(16) Mat, base/Mat.hpp:177)
+ *     and 80 bytes for f64; Quaternions are {z, y, x, w} (xform/core/Quaternion.hpp:15-23);
+ *     Mat22 are row-major 2x2 (16 B for f32, 32 B for f64).
+ *   - Rasterisation and extraction take forward rotations; interpolated insertion takes inverse rotations
+ *     (FourierProject.hpp:1750, :1768).
+ *
+ * Semantics:
+ *   - Rasterisation and insertion ACCUMULATE into the volume (and its weights); extraction OVERWRITES the
+ *     slices and writes 0 beyond the cutoff (FourierProject.hpp:334-340, :516-518, :654-656).
+ *   - All calls are asynchronous on `stream` (a hipStream_t; NULL = the null stream of the current device).
+ *     Buffers are borrowed: the caller keeps them alive until the stream has passed the call.
+ *   - Every call returns NOA_HIP_OK (0) or an error code, and sets a thread-local message readable with
+ *     noa_hip_last_error(). Validation happens before anything is launched.
+ *   - Accumulating outputs must be ordinary device memory (hipMalloc); device-wide float atomics are used.
+ */
+#ifndef NOA_HIP_H
+#define NOA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NOA_HIP_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+    NOA_HIP_OK = 0,
+    NOA_HIP_ERROR_INVALID_ARGUMENT = 1,  /* the check_parameters rules (FourierProject.hpp:1095-1185) */
+    NOA_HIP_ERROR_UNSUPPORTED = 2,       /* a valid noa call this library does not implement */
+    NOA_HIP_ERROR_RUNTIME = 3            /* a HIP runtime error (launch, allocation, ...) */
+};
+
+/* Element types of a view. */
+enum {
+    NOA_HIP_DTYPE_NONE = 0,
+    NOA_HIP_F32 = 1,
+    NOA_HIP_F64 = 2,
+    NOA_HIP_C32 = 3,      /* Complex<f32> */
+    NOA_HIP_C64 = 4,      /* Complex<f64> */
+    NOA_HIP_C16 = 5,      /* {f16 real, f16 imag}: storage-only volume format for extraction (FP32 math) */
+    NOA_HIP_CBF16 = 6     /* {bf16 real, bf16 imag}: storage-only volume format for extraction (FP32 math) */
+};
+
+/* noa::fft::Layout values restricted to half-to-half layouts (src/noa/fft/core/Layout.hpp:39-68). */
+enum {
+    NOA_HIP_H2H = 0,
+    NOA_HIP_HC2H = 4,
+    NOA_HIP_H2HC = 8,
+    NOA_HIP_HC2HC = 12
+};
+
+/* noa::xform::Interp::Method values (src/noa/xform/core/Interp.hpp:17-41). */
+enum {
+    NOA_HIP_INTERP_NEAREST = 0,
+    NOA_HIP_INTERP_LINEAR = 1,
+    NOA_HIP_INTERP_CUBIC = 2,
+    NOA_HIP_INTERP_CUBIC_BSPLINE = 3,
+    NOA_HIP_INTERP_LANCZOS4 = 4,
+    NOA_HIP_INTERP_LANCZOS6 = 5,
+    NOA_HIP_INTERP_LANCZOS8 = 6,
+    NOA_HIP_INTERP_NEAREST_FAST = 100,
+    NOA_HIP_INTERP_LINEAR_FAST = 101,
+    NOA_HIP_INTERP_CUBIC_FAST = 102,
+    NOA_HIP_INTERP_CUBIC_BSPLINE_FAST = 103,
+    NOA_HIP_INTERP_LANCZOS4_FAST = 104,
+    NOA_HIP_INTERP_LANCZOS6_FAST = 105,
+    NOA_HIP_INTERP_LANCZOS8_FAST = 106
+};
+
+/* Transform kinds (the Rotate and Scale template arguments, FourierProject.hpp:1086-1091). */
+enum {
+    NOA_HIP_XFORM_NONE = 0,       /* Empty */
+    NOA_HIP_MAT33_F32 = 1,
+    NOA_HIP_MAT33_F64 = 2,
+    NOA_HIP_QUAT_F32 = 3,
+    NOA_HIP_QUAT_F64 = 4,
+    NOA_HIP_MAT22_F32 = 5,
+    NOA_HIP_MAT22_F64 = 6
+};
+
+/* A strided 4D array, an Empty (data == NULL and is_value == 0), or a constant value (is_value == 1),
+ * which noa accepts for the inputs of rasterisation and insertion (FourierProject.hpp:1063-1078). */
+typedef struct noa_hip_view {
+    void* data;
+    int64_t shape[4];     /* BDHW rfft shape */
+    int64_t strides[4];   /* element strides */
+    int32_t dtype;        /* NOA_HIP_F32, ... */
+    int32_t is_value;     /* 1: every element equals `value` (data is ignored) */
+    double value[2];      /* {real, imag} of the constant value */
+} noa_hip_view;
+
+/* One transform broadcast over the batch (batched == 0; `data` is a HOST pointer to one element, copied at
+ * the call), or one transform per slice (batched == 1; `data` is a DEVICE pointer to `count` contiguous
+ * elements, count == number of slices), as noa's BatchedParameter (src/noa/runtime/core/Utils.hpp:391-413). */
+typedef struct noa_hip_transform {
+    const void* data;
+    int32_t kind;         /* NOA_HIP_MAT33_F32, ... or NOA_HIP_XFORM_NONE */
+    int32_t batched;
+    int64_t count;
+} noa_hip_transform;
+
+/* Arguments of one projection call (one descriptor for the three operators). */
+typedef struct noa_hip_project_args {
+    noa_hip_view input;            /* slices (rasterize/insert) or volume (extract) */
+    noa_hip_view input_weight;     /* optional real weights of the input */
+    noa_hip_view output;           /* volume (rasterize/insert) or slices (extract) */
+    noa_hip_view output_weight;    /* optional real weights of the output */
+    int64_t input_shape[4];        /* logical BDHW shape of the input */
+    int64_t output_shape[4];       /* logical BDHW shape of the output */
+    int64_t target_shape[4];       /* logical BDHW target shape; all zeros = the volume shape */
+    int32_t remap;                 /* NOA_HIP_H2H, NOA_HIP_HC2H, NOA_HIP_H2HC, NOA_HIP_HC2HC */
+    int32_t interp;                /* NOA_HIP_INTERP_*; ignored by rasterisation */
+    noa_hip_transform rotation;    /* fwd (rasterize, extract) or inv (insert) rotations; required */
+    noa_hip_transform scaling;     /* inv (rasterize, extract) or fwd (insert) Mat22 scaling; optional */
+    double fftfreq_cutoff;         /* default 0.5 */
+    double fftfreq_sinc;           /* windowed sinc (insert: along the slice normal; extract: along w); -1 = default */
+    double fftfreq_blackman;       /* -1 = default */
+    double ews_radius[2];          /* HW Ewald-sphere radius in 1/pixels; {0,0} = projections */
+} noa_hip_project_args;
+
+/* Fill `args` with noa's defaults (cutoff 0.5, sinc/blackman -1, LINEAR, H2H, no scaling). */
+void noa_hip_project_args_init(noa_hip_project_args* args);
+
+/* Library version and ABI version. */
+int noa_hip_abi_version(void);
+
+/* Message of the last error raised on the calling thread ("" if none). */
+const char* noa_hip_last_error(void);
+
+/* Replaces the GPU branch of noa::xform::rasterize_central_slices_3d (FourierProject.hpp:1653-1692). */
+int noa_hip_rasterize_central_slices_3d(const noa_hip_project_args* args, void* stream);
+
+/* Replaces the GPU branch of noa::xform::insert_central_slices_3d (FourierProject.hpp:1770-1815). */
+int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* stream);
+
+/* Replaces the GPU branch of noa::xform::extract_central_slices_3d (FourierProject.hpp:1869-1914). */
+int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NOA_HIP_H */

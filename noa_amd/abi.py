@@ -1,0 +1,108 @@
+"""ctypes mirror of include/noa_hip.h (the C ABI of the central-slice library).
+
+Only plain data definitions live here; loading the library is done by :mod:`noa_amd._lib`.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_double, c_int32, c_int64, c_void_p
+
+ABI_VERSION = 1
+
+# Status codes.
+OK = 0
+ERROR_INVALID_ARGUMENT = 1
+ERROR_UNSUPPORTED = 2
+ERROR_RUNTIME = 3
+
+# Element types.
+DTYPE_NONE = 0
+F32 = 1
+F64 = 2
+C32 = 3
+C64 = 4
+C16 = 5
+CBF16 = 6
+
+# Remaps (noa::fft::Layout bits, src/noa/fft/core/Layout.hpp:39-68).
+H2H = 0
+HC2H = 4
+H2HC = 8
+HC2HC = 12
+
+# Transform kinds.
+XFORM_NONE = 0
+MAT33_F32 = 1
+MAT33_F64 = 2
+QUAT_F32 = 3
+QUAT_F64 = 4
+MAT22_F32 = 5
+MAT22_F64 = 6
+
+
+class View(ctypes.Structure):
+    _fields_ = [
+        ("data", c_void_p),
+        ("shape", c_int64 * 4),
+        ("strides", c_int64 * 4),
+        ("dtype", c_int32),
+        ("is_value", c_int32),
+        ("value", c_double * 2),
+    ]
+
+
+class Transform(ctypes.Structure):
+    _fields_ = [
+        ("data", c_void_p),
+        ("kind", c_int32),
+        ("batched", c_int32),
+        ("count", c_int64),
+    ]
+
+
+class ProjectArgs(ctypes.Structure):
+    _fields_ = [
+        ("input", View),
+        ("input_weight", View),
+        ("output", View),
+        ("output_weight", View),
+        ("input_shape", c_int64 * 4),
+        ("output_shape", c_int64 * 4),
+        ("target_shape", c_int64 * 4),
+        ("remap", c_int32),
+        ("interp", c_int32),
+        ("rotation", Transform),
+        ("scaling", Transform),
+        ("fftfreq_cutoff", c_double),
+        ("fftfreq_sinc", c_double),
+        ("fftfreq_blackman", c_double),
+        ("ews_radius", c_double * 2),
+    ]
+
+
+def default_args() -> ProjectArgs:
+    """noa's defaults, as noa_hip_project_args_init."""
+    a = ProjectArgs()
+    a.remap = H2H
+    a.interp = 1  # LINEAR
+    a.fftfreq_cutoff = 0.5
+    a.fftfreq_sinc = -1.0
+    a.fftfreq_blackman = -1.0
+    return a
+
+
+def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
+    """Set the argument/return types of the exported functions."""
+    P = ctypes.POINTER(ProjectArgs)
+    for name in ("noa_hip_rasterize_central_slices_3d", "noa_hip_insert_central_slices_3d",
+                 "noa_hip_extract_central_slices_3d"):
+        f = getattr(lib, name)
+        f.argtypes = [P, c_void_p]
+        f.restype = ctypes.c_int
+    lib.noa_hip_last_error.argtypes = []
+    lib.noa_hip_last_error.restype = ctypes.c_char_p
+    lib.noa_hip_abi_version.argtypes = []
+    lib.noa_hip_abi_version.restype = ctypes.c_int
+    lib.noa_hip_project_args_init.argtypes = [P]
+    lib.noa_hip_project_args_init.restype = None
+    return lib

@@ -1,0 +1,396 @@
+// C ABI of the central-slice library: validation, descriptor flattening and dispatch.
+//
+// The checks restate noa's fourier_projection_check_parameters (src/noa/xform/FourierProject.hpp:1095-1185)
+// with the same rules and close to the same messages, plus what a C boundary must check itself (null
+// pointers, element types, memory kinds, 32-bit in-volume indexing). Nothing is launched unless every check
+// passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/noa_hip.h"
+#include "kernels.hpp"
+
+namespace noa_hip {
+namespace {
+
+thread_local std::string g_last_error;
+
+enum class Kind { RASTERIZE, INSERT, EXTRACT };
+
+std::string shape_str(const int64_t* s) {
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "(%lld,%lld,%lld,%lld)", (long long)s[0], (long long)s[1], (long long)s[2],
+                  (long long)s[3]);
+    return buf;
+}
+
+bool is_real(int dtype) { return dtype == NOA_HIP_F32 || dtype == NOA_HIP_F64; }
+bool is_complex(int dtype) {
+    return dtype == NOA_HIP_C32 || dtype == NOA_HIP_C64 || dtype == NOA_HIP_C16 || dtype == NOA_HIP_CBF16;
+}
+size_t dtype_size(int dtype) {
+    switch (dtype) {
+        case NOA_HIP_F32: return 4;
+        case NOA_HIP_F64: return 8;
+        case NOA_HIP_C32: return 8;
+        case NOA_HIP_C64: return 16;
+        case NOA_HIP_C16: return 4;
+        case NOA_HIP_CBF16: return 4;
+        default: return 0;
+    }
+}
+bool is_empty_view(const noa_hip_view& v) { return v.data == nullptr && !v.is_value; }
+
+void rfft_shape(const int64_t* logical, int64_t* out) {
+    out[0] = logical[0]; out[1] = logical[1]; out[2] = logical[2]; out[3] = logical[3] / 2 + 1;
+}
+bool same_shape(const int64_t* a, const int64_t* b) {
+    return a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && a[3] == b[3];
+}
+
+// Byte range [lo, hi) covered by a view.
+void extent(const noa_hip_view& v, uintptr_t& lo, uintptr_t& hi) {
+    int64_t min_off = 0, max_off = 0;
+    for (int i = 0; i < 4; ++i) {
+        const int64_t span = (v.shape[i] - 1) * v.strides[i];
+        if (span < 0) min_off += span; else max_off += span;
+    }
+    const size_t es = dtype_size(v.dtype);
+    lo = reinterpret_cast<uintptr_t>(v.data) + static_cast<uintptr_t>(min_off * static_cast<int64_t>(es));
+    hi = reinterpret_cast<uintptr_t>(v.data) + static_cast<uintptr_t>((max_off + 1) * static_cast<int64_t>(es));
+}
+bool overlap(const noa_hip_view& a, const noa_hip_view& b) {
+    uintptr_t alo, ahi, blo, bhi;
+    extent(a, alo, ahi);
+    extent(b, blo, bhi);
+    return alo < bhi && blo < ahi;
+}
+
+// The device owning a pointer, or -1 when it is not device-accessible memory of this process.
+int device_of(const void* p, bool* is_managed) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    if (is_managed) *is_managed = attr.isManaged != 0;
+    if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged || attr.isManaged)
+        return attr.device;
+    return -1;
+}
+
+bool fits_i32(const noa_hip_view& v, int first_dim) {
+    int64_t max_off = 0;
+    for (int i = first_dim; i < 4; ++i) max_off += (v.shape[i] - 1) * (v.strides[i] < 0 ? -v.strides[i] : v.strides[i]);
+    return max_off <= 0x7fffffffLL;
+}
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+#define NOA_CHECK(cond, code, ...)                                                   \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            char buf_[512];                                                          \
+            std::snprintf(buf_, sizeof(buf_), __VA_ARGS__);                          \
+            throw Fail{code, buf_};                                                  \
+        }                                                                            \
+    } while (0)
+
+constexpr int INVALID = NOA_HIP_ERROR_INVALID_ARGUMENT;
+constexpr int UNSUPPORTED = NOA_HIP_ERROR_UNSUPPORTED;
+
+Arr to_arr(const noa_hip_view& v, bool volume_dims) {
+    Arr a;
+    a.data = v.data;
+    a.dtype = v.dtype;
+    a.is_value = v.is_value != 0;
+    a.value[0] = v.value[0];
+    a.value[1] = v.value[1];
+    if (volume_dims) { a.s0 = v.strides[1]; a.s1 = v.strides[2]; a.s2 = v.strides[3]; }   // filter {1,2,3}
+    else { a.s0 = v.strides[0]; a.s1 = v.strides[2]; a.s2 = v.strides[3]; }               // filter {0,2,3}
+    if (a.is_value) a.data = nullptr;
+    return a;
+}
+
+Xform to_xform(const noa_hip_transform& t) {
+    Xform x;
+    x.data = t.data;
+    x.kind = t.kind;
+    x.batched = t.batched != 0;
+    if (t.kind != NOA_HIP_XFORM_NONE && !t.batched && t.data) {
+        // Single (broadcast) value: copied from host memory, keeping the stored element order.
+        const unsigned char* p = static_cast<const unsigned char*>(t.data);
+        switch (t.kind) {
+            case NOA_HIP_MAT33_F32: for (int i = 0; i < 9; ++i) { float e; std::memcpy(&e, p + 4 * i, 4); x.single[i] = e; } break;
+            case NOA_HIP_MAT33_F64: for (int i = 0; i < 9; ++i) { double e; std::memcpy(&e, p + 8 * i, 8); x.single[i] = e; } break;
+            case NOA_HIP_QUAT_F32: for (int i = 0; i < 4; ++i) { float e; std::memcpy(&e, p + 4 * i, 4); x.single[i] = e; } break;
+            case NOA_HIP_QUAT_F64: for (int i = 0; i < 4; ++i) { double e; std::memcpy(&e, p + 8 * i, 8); x.single[i] = e; } break;
+            case NOA_HIP_MAT22_F32: for (int i = 0; i < 4; ++i) { float e; std::memcpy(&e, p + 4 * i, 4); x.single[i] = e; } break;
+            case NOA_HIP_MAT22_F64: for (int i = 0; i < 4; ++i) { double e; std::memcpy(&e, p + 8 * i, 8); x.single[i] = e; } break;
+            default: break;
+        }
+    }
+    return x;
+}
+
+void check_view_common(const noa_hip_view& v, const char* name) {
+    NOA_CHECK(dtype_size(v.dtype) != 0, INVALID, "%s: invalid element type %d", name, v.dtype);
+    for (int i = 0; i < 4; ++i)
+        NOA_CHECK(v.shape[i] > 0, INVALID, "Empty array detected (%s:shape=%s)", name, shape_str(v.shape).c_str());
+}
+
+ProjectDesc validate(Kind kind, const noa_hip_project_args& a, int& device) {
+    ProjectDesc d;
+    const bool extract = kind == Kind::EXTRACT;
+
+    // Output (:1109-1113).
+    NOA_CHECK(a.output.data != nullptr && !a.output.is_value, INVALID, "Empty array detected");
+    check_view_common(a.output, "output");
+    int64_t expected[4];
+    rfft_shape(a.output_shape, expected);
+    NOA_CHECK(same_shape(a.output.shape, expected), INVALID,
+              "The shape of the rfft output does not match the expected shape. Got output:shape=%s and expected:shape=%s",
+              shape_str(a.output.shape).c_str(), shape_str(expected).c_str());
+    // Inputs (:1115-1136). Values are allowed for rasterize/insert inputs, not for extraction.
+    // Device checks need the HIP runtime and run last, after every host-only check.
+    auto check_input = [&](const noa_hip_view& v, const char* name, bool is_weight) {
+        if (is_empty_view(v)) return;
+        if (v.is_value) {
+            NOA_CHECK(!extract, INVALID, "%s: a constant value is not a valid input of the extraction", name);
+            if (is_weight) NOA_CHECK(is_real(v.dtype), INVALID, "%s: weights should be real", name);
+            return;
+        }
+        check_view_common(v, name);
+        NOA_CHECK(!overlap(v, a.output), INVALID, "Input and output arrays should not overlap");
+        int64_t exp[4];
+        rfft_shape(a.input_shape, exp);
+        NOA_CHECK(same_shape(v.shape, exp), INVALID,
+                  "The shape of the rfft does not match the expected shape. Got %s:shape=%s and shape=%s", name,
+                  shape_str(v.shape).c_str(), shape_str(exp).c_str());
+        if (is_weight) NOA_CHECK(is_real(v.dtype), INVALID, "%s: weights should be real", name);
+    };
+    check_input(a.input, "input", false);
+    check_input(a.input_weight, "input_weight", true);
+    NOA_CHECK(!is_empty_view(a.input), INVALID, "input: Empty array detected");
+
+    // Output weight (:1138-1147).
+    if (!is_empty_view(a.output_weight)) {
+        NOA_CHECK(!a.output_weight.is_value && a.output_weight.data, INVALID, "Empty array detected");
+        check_view_common(a.output_weight, "output_weight");
+        NOA_CHECK(!overlap(a.output_weight, a.output), INVALID, "Output arrays should not overlap");
+        NOA_CHECK(same_shape(a.output_weight.shape, expected), INVALID,
+                  "The shape of the rfft does not match the expected shape. Got output_weight:shape=%s and shape=%s",
+                  shape_str(a.output_weight.shape).c_str(), shape_str(expected).c_str());
+        NOA_CHECK(is_real(a.output_weight.dtype), INVALID, "output_weight: weights should be real");
+        if (!is_empty_view(a.input_weight) && !a.input_weight.is_value)
+            NOA_CHECK(!overlap(a.input_weight, a.output_weight), INVALID, "Input and output arrays should not overlap");
+        if (!a.input.is_value) NOA_CHECK(!overlap(a.input, a.output_weight), INVALID, "Input and output arrays should not overlap");
+    }
+
+    // Dimensionality (:1149-1164).
+    if (!extract) {
+        NOA_CHECK(a.input_shape[1] == 1, INVALID, "2d input slices are expected, but got shape=%s", shape_str(a.input_shape).c_str());
+        NOA_CHECK(a.output_shape[0] == 1 && a.target_shape[0] <= 1, INVALID,
+                  "A single 3d volume is expected, but got output_shape=%s and target_shape=%s (optional)",
+                  shape_str(a.output_shape).c_str(), shape_str(a.target_shape).c_str());
+    } else {
+        NOA_CHECK(a.input_shape[0] == 1 && a.target_shape[0] <= 1, INVALID,
+                  "A single 3d volume is expected, but got input_shape=%s and target_shape=%s (optional)",
+                  shape_str(a.input_shape).c_str(), shape_str(a.target_shape).c_str());
+        NOA_CHECK(a.output_shape[1] == 1, INVALID, "2d input slices are expected but got shape %s", shape_str(a.output_shape).c_str());
+    }
+    for (int i = 0; i < 4; ++i) {
+        NOA_CHECK(a.input_shape[i] > 0 && a.output_shape[i] > 0, INVALID, "Empty array detected");
+        NOA_CHECK(a.target_shape[i] >= 0, INVALID, "target_shape should be positive or zero");
+    }
+
+    // Remap (requires REMAP.is_hx2hx(), :1659).
+    NOA_CHECK(a.remap == NOA_HIP_H2H || a.remap == NOA_HIP_HC2H || a.remap == NOA_HIP_H2HC || a.remap == NOA_HIP_HC2HC,
+              INVALID, "The remap should be H2H, HC2H, H2HC or HC2HC, but got %d", a.remap);
+
+    // Interpolation.
+    int interp = a.interp >= 100 ? a.interp - 100 : a.interp;
+    NOA_CHECK(interp >= 0 && interp <= 6, INVALID, "Invalid interpolation method %d", a.interp);
+    if (kind != Kind::RASTERIZE)
+        NOA_CHECK(interp == NOA_HIP_INTERP_LINEAR || interp == NOA_HIP_INTERP_NEAREST, UNSUPPORTED,
+                  "Interpolation method %d is not implemented by this library (NEAREST and LINEAR are)", a.interp);
+
+    // Value types: spectrum types (Traits.hpp:87) = c->c, r->r, c->r.
+    const int in = a.input.dtype, out = a.output.dtype;
+    NOA_CHECK((is_complex(in) && is_complex(out)) || (is_real(in) && is_real(out)) || (is_complex(in) && is_real(out)),
+              INVALID, "Invalid input/output types: %d -> %d", in, out);
+    if (in == NOA_HIP_C16 || in == NOA_HIP_CBF16)
+        NOA_CHECK(extract && !a.input.is_value, UNSUPPORTED, "FP16/BF16 storage is only supported for the extracted volume");
+
+    // Transforms (:1166-1178).
+    const int64_t required = extract ? a.output_shape[0] : a.input_shape[0];
+    auto check_transform = [&](const noa_hip_transform& t, const char* name, bool rotation) {
+        if (rotation) {
+            NOA_CHECK(t.kind >= NOA_HIP_MAT33_F32 && t.kind <= NOA_HIP_QUAT_F64, INVALID,
+                      "%s should be Mat33 or Quaternion (f32 or f64), got kind=%d", name, t.kind);
+        } else if (t.kind == NOA_HIP_XFORM_NONE) {
+            return;
+        } else {
+            NOA_CHECK(t.kind == NOA_HIP_MAT22_F32 || t.kind == NOA_HIP_MAT22_F64, INVALID,
+                      "%s should be Mat22 (f32 or f64), got kind=%d", name, t.kind);
+        }
+        NOA_CHECK(t.data != nullptr, INVALID, "%s should not be empty", name);
+        if (t.batched) {
+            NOA_CHECK(t.count == required, INVALID,
+                      "%s should be a contiguous vector with n_slices=%lld elements, but got %lld elements", name,
+                      (long long)required, (long long)t.count);
+            NOA_CHECK(reinterpret_cast<uintptr_t>(t.data) % 16 == 0 || t.kind == NOA_HIP_MAT33_F64, INVALID,
+                      "%s should be 16-byte aligned", name);
+        }
+    };
+    check_transform(a.rotation, "rotation", true);
+    check_transform(a.scaling, "scaling", false);
+    const bool rot64 = a.rotation.kind == NOA_HIP_MAT33_F64 || a.rotation.kind == NOA_HIP_QUAT_F64;
+    if (a.scaling.kind != NOA_HIP_XFORM_NONE) {
+        const bool s64 = a.scaling.kind == NOA_HIP_MAT22_F64;
+        NOA_CHECK(s64 == rot64, INVALID, "The scaling and rotation should have the same precision");
+    }
+
+    // 32-bit in-volume / in-slice indexing (the batch offset of slice stacks is 64-bit).
+    auto check_i32 = [&](const noa_hip_view& v, bool is_volume, const char* name) {
+        if (is_empty_view(v) || v.is_value) return;
+        NOA_CHECK(fits_i32(v, is_volume ? 1 : 2), UNSUPPORTED, "%s: offsets within a %s exceed 32-bit indexing", name,
+                  is_volume ? "volume" : "slice");
+    };
+    check_i32(a.input, extract, "input");
+    check_i32(a.input_weight, extract, "input_weight");
+    check_i32(a.output, !extract, "output");
+    check_i32(a.output_weight, !extract, "output_weight");
+
+    // Weights: real, same precision as the output values.
+    const bool out64 = out == NOA_HIP_F64 || out == NOA_HIP_C64;
+    const int wtype = out64 ? NOA_HIP_F64 : NOA_HIP_F32;
+    if (!is_empty_view(a.input_weight)) NOA_CHECK(a.input_weight.dtype == wtype, UNSUPPORTED, "input_weight: expected %s weights", out64 ? "f64" : "f32");
+    if (!is_empty_view(a.output_weight)) NOA_CHECK(a.output_weight.dtype == wtype, UNSUPPORTED, "output_weight: expected %s weights", out64 ? "f64" : "f32");
+    if (!a.input.is_value) {
+        const bool in64 = in == NOA_HIP_F64 || in == NOA_HIP_C64;
+        NOA_CHECK(in64 == out64 || in == NOA_HIP_C16 || in == NOA_HIP_CBF16, UNSUPPORTED,
+                  "input and output should have the same precision");
+    }
+
+    // Device checks (HIP runtime): every array and batched transform on the output's device.
+    bool managed = false;
+    device = device_of(a.output.data, &managed);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    auto check_device = [&](const noa_hip_view& v, const char* name) {
+        if (is_empty_view(v) || v.is_value) return;
+        const int dev = device_of(v.data, nullptr);
+        NOA_CHECK(dev == device, INVALID, "The arrays should be on the same device, but got %s:device=%d and output:device=%d",
+                  name, dev, device);
+    };
+    check_device(a.input, "input");
+    check_device(a.input_weight, "input_weight");
+    check_device(a.output_weight, "output_weight");
+    if (a.rotation.batched)
+        NOA_CHECK(device_of(a.rotation.data, nullptr) == device, INVALID, "rotation should be on the compute device");
+    if (a.scaling.kind != NOA_HIP_XFORM_NONE && a.scaling.batched)
+        NOA_CHECK(device_of(a.scaling.data, nullptr) == device, INVALID, "scaling should be on the compute device");
+    // Float atomics on managed memory are not device-wide safe without XNACK: refuse for accumulating ops.
+    if (kind == Kind::RASTERIZE)
+        NOA_CHECK(!managed, UNSUPPORTED, "rasterize: the volume should be device (hipMalloc) memory, not managed memory");
+
+    // Flatten.
+    d.input = to_arr(a.input, extract);
+    d.input_weight = to_arr(a.input_weight, extract);
+    d.output = to_arr(a.output, !extract);
+    d.output_weight = to_arr(a.output_weight, !extract);
+    if (is_empty_view(a.input_weight)) { d.input_weight.data = nullptr; d.input_weight.is_value = false; }
+    if (is_empty_view(a.output_weight)) { d.output_weight.data = nullptr; }
+    for (int i = 0; i < 4; ++i) { d.in_shape[i] = a.input_shape[i]; d.out_shape[i] = a.output_shape[i]; }
+    const int64_t* grid = extract ? a.input_shape : a.output_shape;
+    const bool any0 = a.target_shape[0] == 0 || a.target_shape[1] == 0 || a.target_shape[2] == 0 || a.target_shape[3] == 0;
+    for (int i = 0; i < 3; ++i) d.target[i] = any0 ? grid[i + 1] : a.target_shape[i + 1];
+    d.src_centered = (a.remap & 4) != 0;
+    d.dst_centered = (a.remap & 8) != 0;
+    d.interp = interp;
+    d.rot = to_xform(a.rotation);
+    d.scale = to_xform(a.scaling);
+    d.coord_f64 = rot64;
+    d.cutoff = a.fftfreq_cutoff;
+    d.fftfreq_sinc = a.fftfreq_sinc;
+    d.fftfreq_blackman = a.fftfreq_blackman;
+    d.ews_radius[0] = a.ews_radius[0];
+    d.ews_radius[1] = a.ews_radius[1];
+    const bool has_ews = a.ews_radius[0] != 0 || a.ews_radius[1] != 0;
+    bool has_scale = a.scaling.kind != NOA_HIP_XFORM_NONE;
+    if (has_scale && !a.scaling.batched) {   // a single identity Mat22 is no scaling (:1234-1244)
+        const double* s = d.scale.single;
+        has_scale = !(s[0] == 1 && s[1] == 0 && s[2] == 0 && s[3] == 1);
+    }
+    d.general = has_ews || has_scale;
+    if (!has_scale) d.scale.kind = NOA_HIP_XFORM_NONE;
+    return d;
+}
+
+int call(Kind kind, const noa_hip_project_args* args, void* stream) {
+    g_last_error.clear();
+    if (!args) {
+        g_last_error = "null arguments";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    int device = -1;
+    ProjectDesc d;
+    try {
+        d = validate(kind, *args, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    int current = 0;
+    (void)hipGetDevice(&current);
+    if (current != device) (void)hipSetDevice(device);
+    std::string err;
+    int status = NOA_HIP_OK;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (kind) {
+        case Kind::RASTERIZE: status = launch_rasterize(d, s, err); break;
+        case Kind::INSERT: status = launch_insert(d, s, err); break;
+        case Kind::EXTRACT: status = launch_extract(d, s, err); break;
+    }
+    if (current != device) (void)hipSetDevice(current);
+    if (status != NOA_HIP_OK) g_last_error = err;
+    return status;
+}
+
+} // namespace
+} // namespace noa_hip
+
+extern "C" {
+
+void noa_hip_project_args_init(noa_hip_project_args* args) {
+    if (!args) return;
+    std::memset(args, 0, sizeof(*args));
+    args->remap = NOA_HIP_H2H;
+    args->interp = NOA_HIP_INTERP_LINEAR;
+    args->fftfreq_cutoff = 0.5;
+    args->fftfreq_sinc = -1;
+    args->fftfreq_blackman = -1;
+}
+
+int noa_hip_abi_version(void) { return NOA_HIP_ABI_VERSION; }
+
+const char* noa_hip_last_error(void) { return noa_hip::g_last_error.c_str(); }
+
+int noa_hip_rasterize_central_slices_3d(const noa_hip_project_args* args, void* stream) {
+    return noa_hip::call(noa_hip::Kind::RASTERIZE, args, stream);
+}
+int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* stream) {
+    return noa_hip::call(noa_hip::Kind::INSERT, args, stream);
+}
+int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream) {
+    return noa_hip::call(noa_hip::Kind::EXTRACT, args, stream);
+}
+
+} // extern "C"

@@ -1,0 +1,370 @@
+// Device-side building blocks of the central-slice kernels (gfx950).
+//
+// Every expression that changes a result is written in the same evaluation order as noa's operators, so the
+// kernels agree with the CPU path to the last bit wherever no transcendental function is involved (the
+// library is compiled with -ffp-contract=off):
+//   index <-> frequency           src/noa/fft/core/Frequency.hpp:80-145, bounds :258-288
+//   slice <-> grid transforms     src/noa/xform/FourierProject.hpp:41-95
+//   windowed sinc                 src/noa/xform/FourierProject.hpp:97-110, noa::sinc base/Math.hpp:28-29
+//   interpolate_spectrum          src/noa/xform/core/Interpolation.hpp:435-526
+//   Mat*Vec (row dot, left fold)  src/noa/base/Mat.hpp:419-424, base/Vec.hpp:846-855
+//   Quaternion::rotate            src/noa/xform/core/Quaternion.hpp:167-181
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "../../include/noa_hip.h"
+
+namespace noa_hip {
+
+using i32 = int32_t;
+using i64 = int64_t;
+using u32 = uint32_t;
+
+// ---------------------------------------------------------------------------------------------------------
+// Value types: real (float, double) and complex (float2, double2 as {re, im}).
+// ---------------------------------------------------------------------------------------------------------
+template<class T> struct vt;
+template<> struct vt<float>   { using real = float;  static constexpr bool cx = false; };
+template<> struct vt<double>  { using real = double; static constexpr bool cx = false; };
+template<> struct vt<float2>  { using real = float;  static constexpr bool cx = true; };
+template<> struct vt<double2> { using real = double; static constexpr bool cx = true; };
+
+template<class T> __device__ __forceinline__ T vzero() { return T{}; }
+
+__device__ __forceinline__ float   vadd(float a, float b) { return a + b; }
+__device__ __forceinline__ double  vadd(double a, double b) { return a + b; }
+__device__ __forceinline__ float2  vadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 vadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// Complex * real = {s*re, s*im} (base/Complex.hpp:203-216).
+__device__ __forceinline__ float   vmul(float a, float s) { return a * s; }
+__device__ __forceinline__ double  vmul(double a, double s) { return a * s; }
+__device__ __forceinline__ float2  vmul(float2 a, float s) { return make_float2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 vmul(double2 a, double s) { return make_double2(s * a.x, s * a.y); }
+
+template<class T, class R> __device__ __forceinline__ void vimag_mul(T& v, R s) {
+    if constexpr (vt<T>::cx) v.y *= s;
+}
+
+// static_cast between spectrum types; complex -> real is |z|^2 (cast_or_abs_squared, base/Math.hpp:220-225).
+template<class To, class From> __device__ __forceinline__ To vcast(const From& v) {
+    if constexpr (vt<To>::cx && vt<From>::cx) {
+        using R = typename vt<To>::real;
+        To out; out.x = static_cast<R>(v.x); out.y = static_cast<R>(v.y); return out;
+    } else if constexpr (!vt<To>::cx && !vt<From>::cx) {
+        return static_cast<To>(v);
+    } else {
+        static_assert(!vt<To>::cx, "real to complex is not a spectrum type");
+        return static_cast<To>(v.x * v.x + v.y * v.y);
+    }
+}
+
+// Storage -> math conversion for volume reads (FP32 math on FP16/BF16 complex storage).
+template<class Math, class Storage> __device__ __forceinline__ Math load_as(const Storage* p) {
+    if constexpr (__hip_internal::is_same<Storage, __half2>::value) {
+        const float2 f = __half22float2(*p);
+        return f;
+    } else if constexpr (__hip_internal::is_same<Storage, __hip_bfloat162>::value) {
+        const __hip_bfloat162 h = *p;
+        return make_float2(__bfloat162float(h.x), __bfloat162float(h.y));
+    } else {
+        return vcast<Math>(*p);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Index <-> frequency (fft/core/Frequency.hpp:80-87, :115-122).
+// ---------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ i32 index2frequency(bool centered, i32 index, i32 size) {
+    if (centered) return index - size / 2;
+    return index < (size + 1) / 2 ? index : index - size;
+}
+__device__ __forceinline__ i32 frequency2index(bool centered, i32 frequency, i32 size) {
+    if (centered) return frequency + size / 2;
+    return frequency < 0 ? frequency + size : frequency;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Transforms. A transform is read from device memory (batched) or from the kernel arguments (broadcast).
+// ---------------------------------------------------------------------------------------------------------
+template<class C> struct vec3 { C z, y, x; };
+template<class C> struct vec2 { C y, x; };
+
+template<class C>
+struct XformArg {
+    const void* data;   // device pointer when batched
+    i32 kind;           // NOA_HIP_MAT33_F32, ...
+    i32 batched;
+    double single[9];   // broadcast value (Mat33 row-major, Quaternion zyxw, Mat22 row-major)
+};
+
+template<class C> __device__ __forceinline__ vec3<C> matvec3(const C m[9], vec3<C> v) {
+    vec3<C> o;
+    o.z = (m[0] * v.z + m[1] * v.y) + m[2] * v.x;
+    o.y = (m[3] * v.z + m[4] * v.y) + m[5] * v.x;
+    o.x = (m[6] * v.z + m[7] * v.y) + m[8] * v.x;
+    return o;
+}
+
+template<class C> __device__ __forceinline__ vec3<C> quat_rotate(C qz, C qy, C qx, C qw, vec3<C> v) {
+    const C a0 = qw * v.x + qy * v.z - qz * v.y;
+    const C a1 = qw * v.y + qz * v.x - qx * v.z;
+    const C a2 = qw * v.z + qx * v.y - qy * v.x;
+    const C two = 2;
+    vec3<C> o;
+    o.z = v.z + two * (qx * a1 - qy * a0);
+    o.y = v.y + two * (qz * a0 - qx * a2);
+    o.x = v.x + two * (qy * a2 - qz * a1);
+    return o;
+}
+
+// A rotation resolved for one batch index: either 9 matrix coefficients or a quaternion.
+template<class C>
+struct Rot {
+    C m[9];
+    bool quat;
+    __device__ __forceinline__ vec3<C> apply(vec3<C> v) const {
+        return quat ? quat_rotate<C>(m[0], m[1], m[2], m[3], v) : matvec3<C>(m, v);
+    }
+};
+
+template<class C>
+__device__ __forceinline__ Rot<C> load_rotation(const XformArg<C>& a, i64 b) {
+    Rot<C> r;
+    r.quat = a.kind == NOA_HIP_QUAT_F32 || a.kind == NOA_HIP_QUAT_F64;
+    if (!a.batched) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.m[i] = static_cast<C>(a.single[i]);
+        return r;
+    }
+    const unsigned char* base = static_cast<const unsigned char*>(a.data);
+    switch (a.kind) {
+        case NOA_HIP_MAT33_F32: {
+            const float* p = reinterpret_cast<const float*>(base + b * 48);   // alignas(16) Mat33<f32>
+#pragma unroll
+            for (int i = 0; i < 9; ++i) r.m[i] = static_cast<C>(p[i]);
+            break;
+        }
+        case NOA_HIP_MAT33_F64: {
+            const double* p = reinterpret_cast<const double*>(base + b * 80);  // alignas(16) Mat33<f64>
+#pragma unroll
+            for (int i = 0; i < 9; ++i) r.m[i] = static_cast<C>(p[i]);
+            break;
+        }
+        case NOA_HIP_QUAT_F32: {
+            const float4 q = reinterpret_cast<const float4*>(base)[b];
+            r.m[0] = static_cast<C>(q.x); r.m[1] = static_cast<C>(q.y);
+            r.m[2] = static_cast<C>(q.z); r.m[3] = static_cast<C>(q.w);
+            break;
+        }
+        default: {  // NOA_HIP_QUAT_F64
+            const double* p = reinterpret_cast<const double*>(base + b * 32);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) r.m[i] = static_cast<C>(p[i]);
+            break;
+        }
+    }
+    return r;
+}
+
+template<class C>
+struct Scale {
+    C m[4];
+    bool identity;   // Empty: no arithmetic at all
+    __device__ __forceinline__ vec2<C> apply(vec2<C> v) const {
+        if (identity) return v;
+        vec2<C> o;
+        o.y = m[0] * v.y + m[1] * v.x;
+        o.x = m[2] * v.y + m[3] * v.x;
+        return o;
+    }
+};
+
+template<class C>
+__device__ __forceinline__ Scale<C> load_scaling(const XformArg<C>& a, i64 b) {
+    Scale<C> s;
+    s.identity = a.kind == NOA_HIP_XFORM_NONE;
+    if (s.identity) return s;
+    if (!a.batched) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s.m[i] = static_cast<C>(a.single[i]);
+        return s;
+    }
+    if (a.kind == NOA_HIP_MAT22_F32) {
+        const float4 q = reinterpret_cast<const float4*>(a.data)[b];
+        s.m[0] = static_cast<C>(q.x); s.m[1] = static_cast<C>(q.y);
+        s.m[2] = static_cast<C>(q.z); s.m[3] = static_cast<C>(q.w);
+    } else {
+        const double* p = reinterpret_cast<const double*>(a.data) + b * 4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s.m[i] = static_cast<C>(p[i]);
+    }
+    return s;
+}
+
+// fourier_slice2grid (FourierProject.hpp:41-68). GENERAL = scaling or EWS present.
+template<bool GENERAL, class C>
+__device__ __forceinline__ vec3<C> slice2grid(vec2<C> f, const Scale<C>& s, const Rot<C>& r, C ews0, C ews1) {
+    vec3<C> f3;
+    if constexpr (GENERAL) {
+        f = s.apply(f);
+        f3.z = (ews0 * f.y) * f.y + (ews1 * f.x) * f.x;
+    } else {
+        f3.z = 0;
+    }
+    f3.y = f.y;
+    f3.x = f.x;
+    return r.apply(f3);
+}
+
+// fourier_grid2slice (FourierProject.hpp:71-95). Returns fz; f2 receives the 2D frequency.
+template<bool GENERAL, class C>
+__device__ __forceinline__ C grid2slice(vec3<C> f, const Scale<C>& s, const Rot<C>& r, C ews0, C ews1, vec2<C>& f2) {
+    const vec3<C> g = r.apply(f);
+    f2.y = g.y;
+    f2.x = g.x;
+    C fz = g.z;
+    if constexpr (GENERAL) {
+        fz -= (ews0 * f2.y) * f2.y + (ews1 * f2.x) * f2.x;
+        f2 = s.apply(f2);
+    }
+    return fz;
+}
+
+template<class C> __device__ __forceinline__ C dot3(vec3<C> a) { return (a.z * a.z + a.y * a.y) + a.x * a.x; }
+
+// windowed_sinc (FourierProject.hpp:99-110).
+template<class C>
+__device__ __forceinline__ C windowed_sinc(C fftfreq, C fftfreq_sinc, C fftfreq_blackman) {
+    const C PI = static_cast<C>(3.141592653589793238462643383279502884);
+    fftfreq *= PI;
+    const C xs = fftfreq / fftfreq_sinc;
+    const C s = xs == 0 ? C(1) : sin(xs) / xs;
+    const C bc = fftfreq / fftfreq_blackman;
+    const C blackman = static_cast<C>(0.42) + static_cast<C>(0.5) * cos(bc) + static_cast<C>(0.08) * cos(2 * bc);
+    return s * blackman;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Spectrum interpolation (Interpolation.hpp:435-526) for NEAREST and LINEAR, rfft along x.
+// ---------------------------------------------------------------------------------------------------------
+struct Bounds3 { i32 lo_z, hi_z, lo_y, hi_y, hi_x; i32 nz, ny; };
+
+__device__ __forceinline__ Bounds3 make_bounds3(i32 nz, i32 ny, i32 nx) {
+    return {-nz / 2, (nz - 1) / 2, -ny / 2, (ny - 1) / 2, nx / 2, nz, ny};
+}
+
+// 3D: `vol` with strides (sz, sy, sx); Math = float2/float/double2/double, Storage = stored element type.
+template<int INTERP, class Math, class Storage, class C>
+__device__ __forceinline__ Math interp3(const Storage* __restrict__ vol, i32 sz, i32 sy, i32 sx, const Bounds3& bd,
+                                       bool centered, C fz, C fy, C fx) {
+    using R = typename vt<Math>::real;
+    R conjugate = 1;
+    if (fx < 0) { fz *= -1; fy *= -1; fx *= -1; conjugate = -1; }
+    Math value = vzero<Math>();
+    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+        const i32 z = static_cast<i32>(round(fz)), y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
+        if (z >= bd.lo_z && z <= bd.hi_z && y >= bd.lo_y && y <= bd.hi_y && x >= 0 && x <= bd.hi_x) {
+            const i32 iz = frequency2index(centered, z, bd.nz), iy = frequency2index(centered, y, bd.ny);
+            value = vadd(value, vmul(load_as<Math>(vol + (iz * sz + iy * sy + x * sx)), R(1)));
+        }
+        return value;   // no conjugation after the flip for NEAREST (Interpolation.hpp:479-484)
+    } else {
+        const C flz = floor(fz), fly = floor(fy), flx = floor(fx);
+        const i32 z0 = static_cast<i32>(flz), y0 = static_cast<i32>(fly), x0 = static_cast<i32>(flx);
+        const R tz = static_cast<R>(fz - flz), ty = static_cast<R>(fy - fly), tx = static_cast<R>(fx - flx);
+        const R wz[2] = {1 - tz, tz}, wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
+        bool okz[2], oky[2], okx[2];
+        i32 oz[2], oy[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            okz[i] = z0 + i >= bd.lo_z && z0 + i <= bd.hi_z;
+            oky[i] = y0 + i >= bd.lo_y && y0 + i <= bd.hi_y;
+            okx[i] = x0 + i >= 0 && x0 + i <= bd.hi_x;
+            oz[i] = frequency2index(centered, z0 + i, bd.nz) * sz;
+            oy[i] = frequency2index(centered, y0 + i, bd.ny) * sy;
+        }
+#pragma unroll
+        for (int z = 0; z < 2; ++z) {
+            Math vz = vzero<Math>();
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                Math vy = vzero<Math>();
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    if (okz[z] && oky[y] && okx[x])
+                        vy = vadd(vy, vmul(load_as<Math>(vol + (oz[z] + oy[y] + (x0 + x) * sx)), wx[x]));
+                }
+                vz = vadd(vz, vmul(vy, wy[y]));
+            }
+            value = vadd(value, vmul(vz, wz[z]));
+        }
+        vimag_mul(value, conjugate);
+        return value;
+    }
+}
+
+// 2D: slice `b` of a stack with strides (sb, sy, sx); returns the interpolated value.
+// A constant input (noa's AccessorValue) is read through the same code with `is_value` set.
+template<int INTERP, class Math, class C>
+__device__ __forceinline__ Math interp2(const Math* __restrict__ base, i32 sy, i32 sx, i32 ny, i32 nx,
+                                       bool centered, C fy, C fx, bool is_value = false, Math cval = Math{}) {
+    using R = typename vt<Math>::real;
+    R conjugate = 1;
+    if (fx < 0) { fy *= -1; fx *= -1; conjugate = -1; }
+    const i32 lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
+    Math value = vzero<Math>();
+    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+        const i32 y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
+        if (y >= lo_y && y <= hi_y && x >= 0 && x <= hi_x)
+            value = vadd(value, vmul(is_value ? cval : base[frequency2index(centered, y, ny) * sy + x * sx], R(1)));
+        return value;
+    } else {
+        const C fly = floor(fy), flx = floor(fx);
+        const i32 y0 = static_cast<i32>(fly), x0 = static_cast<i32>(flx);
+        const R ty = static_cast<R>(fy - fly), tx = static_cast<R>(fx - flx);
+        const R wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            Math vy = vzero<Math>();
+            const bool oky = y0 + y >= lo_y && y0 + y <= hi_y;
+            const i32 oy = frequency2index(centered, y0 + y, ny) * sy;
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                if (oky && x0 + x >= 0 && x0 + x <= hi_x)
+                    vy = vadd(vy, vmul(is_value ? cval : base[oy + (x0 + x) * sx], wx[x]));
+            }
+            value = vadd(value, vmul(vy, wy[y]));
+        }
+        vimag_mul(value, conjugate);
+        return value;
+    }
+}
+
+// Device-wide float atomics. Outputs are coarse-grained hipMalloc memory: the hardware float atomic is exact.
+__device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(float2* p, float2 v) {
+    unsafeAtomicAdd(&p->x, v.x);
+    unsafeAtomicAdd(&p->y, v.y);
+}
+__device__ __forceinline__ void atomic_add(double2* p, double2 v) {
+    unsafeAtomicAdd(&p->x, v.x);
+    unsafeAtomicAdd(&p->y, v.y);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block i runs on XCD i % 8). Remap the hardware block
+// index so that each XCD processes one contiguous range of logical blocks, which keeps neighbouring tiles (and
+// their volume neighbourhoods) in the same L2.
+__device__ __forceinline__ u32 xcd_swizzle(u32 bid, u32 nblocks) {
+    constexpr u32 NXCD = 8;
+    const u32 q = nblocks / NXCD, r = nblocks % NXCD;
+    const u32 xcd = bid % NXCD, local = bid / NXCD;
+    return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+} // namespace noa_hip

@@ -1,0 +1,206 @@
+// Fourier extraction of central slices from a 3D rfft volume (gather).
+//
+// Replaces the FourierExtract operator (src/noa/xform/FourierProject.hpp:541-725) launched by
+// launch_extract_central_slices_3d (:1396-1487) through the CUDA iwise_{3,4}d_static kernels.
+//
+// Design (gfx950):
+//   - One workgroup of 256 lanes (4 wave64) owns `ppt * 256` consecutive pixels of one slice (row-major
+//     y, u), so stores are fully coalesced along u and a wavefront's gathers walk a line of the volume.
+//   - The slice's rotation (and scaling) is uniform over the workgroup and is read with scalar loads.
+//   - Workgroups are remapped so that each of the 8 XCDs processes a contiguous range of tiles: consecutive
+//     tiles of a slice share one L2, which holds the volume neighbourhood of that slice (the whole volume
+//     stays resident in the 256 MB Infinity Cache).
+//   - The w-windowed-sinc variant (window size > 1, :661-694) loops over the window inside the lane that owns
+//     the pixel, in increasing w, so no zero fill and no atomics are needed; the sum order is that of noa's
+//     serial CPU loop.
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace noa_hip {
+
+template<class C>
+struct ExtractParams {
+    const void* vol; i32 vs0, vs1, vs2;
+    const void* wvol; i32 wvs0, wvs1, wvs2;
+    void* out; i64 os_b; i32 os_y, os_x;
+    void* wout; i64 ws_b; i32 ws_y, ws_x;
+    i32 grid[3];
+    C target[3];
+    C slice_f[2];
+    i32 slice_h, slice_wr, pixels, tiles, ppt;
+    C cutoff_sqd;
+    C ews[2];
+    C fftfreq_sinc, fftfreq_blackman;
+    i32 window;            // blackman window size along w (1 = plain 3D extraction)
+    XformArg<C> rot, scale;
+    bool general, vol_centered, slice_centered, has_wvol, has_wout;
+};
+
+template<class C, class S, class M, class Out, class W, int INTERP>
+__global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) {
+    const u32 lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const i32 b = static_cast<i32>(lb / static_cast<u32>(p.tiles));
+    const i32 tile = static_cast<i32>(lb) - b * p.tiles;
+
+    const Rot<C> rot = load_rotation(p.rot, b);
+    Scale<C> sc;
+    sc.identity = true;
+    if (p.general) sc = load_scaling(p.scale, b);
+    const Bounds3 bd = make_bounds3(p.grid[0], p.grid[1], p.grid[2]);
+
+    const S* __restrict__ vol = static_cast<const S*>(p.vol);
+    const W* __restrict__ wvol = static_cast<const W*>(p.wvol);
+    Out* __restrict__ out = static_cast<Out*>(p.out) + static_cast<i64>(b) * p.os_b;
+    W* __restrict__ wout = p.has_wout ? static_cast<W*>(p.wout) + static_cast<i64>(b) * p.ws_b : nullptr;
+
+    const i32 first = tile * p.ppt * 256 + static_cast<i32>(threadIdx.x);
+    for (i32 k = 0; k < p.ppt; ++k) {
+        const i32 pix = first + k * 256;
+        if (pix >= p.pixels) break;
+        const i32 y = pix / p.slice_wr;
+        const i32 u = pix - y * p.slice_wr;
+
+        // compute_fftfreq_in_volume_ (:699-705)
+        vec2<C> f2;
+        f2.y = static_cast<C>(index2frequency(p.slice_centered, y, p.slice_h)) / p.slice_f[0];
+        f2.x = static_cast<C>(u) / p.slice_f[1];
+        const vec3<C> f = p.general ? slice2grid<true>(f2, sc, rot, p.ews[0], p.ews[1])
+                                    : slice2grid<false>(f2, sc, rot, p.ews[0], p.ews[1]);
+
+        Out value = vzero<Out>();
+        W weight{};
+        if (p.window == 1) {
+            // 3D operator (:632-657): overwrite, 0 beyond the cutoff.
+            if (dot3(f) <= p.cutoff_sqd) {
+                const C fz = f.z * p.target[0], fy = f.y * p.target[1], fx = f.x * p.target[2];
+                value = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx));
+                if (p.has_wout)
+                    weight = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx)
+                                        : W(1);
+            }
+        } else {
+            // 4D operator (:661-694), summed over w in increasing order.
+            using RO = typename vt<Out>::real;
+            for (i32 w = 0; w < p.window; ++w) {
+                const C offset = static_cast<C>(w - p.window / 2) / p.target[0];   // w_index_to_fftfreq_offset
+                vec3<C> fw = f;
+                fw.z += offset;
+                if (dot3(fw) > p.cutoff_sqd) continue;
+                const C fz = fw.z * p.target[0], fy = fw.y * p.target[1], fx = fw.x * p.target[2];
+                const C conv = windowed_sinc<C>(offset, p.fftfreq_sinc, p.fftfreq_blackman);
+                const Out v = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx));
+                value = vadd(value, vmul(v, static_cast<RO>(conv)));
+                if (p.has_wout) {
+                    const W wv = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx)
+                                            : W(1);
+                    weight = weight + wv * static_cast<W>(conv);
+                }
+            }
+        }
+        out[y * p.os_y + u * p.os_x] = value;
+        if (p.has_wout) wout[y * p.ws_y + u * p.ws_x] = weight;
+    }
+}
+
+namespace {
+
+template<class C>
+XformArg<C> to_arg(const Xform& x) {
+    XformArg<C> a{};
+    a.data = x.data;
+    a.kind = x.kind;
+    a.batched = x.batched ? 1 : 0;
+    for (int i = 0; i < 9; ++i) a.single[i] = x.single[i];
+    return a;
+}
+
+// noa's blackman_window_size (FourierProject.hpp:114-137).
+i64 blackman_window_size(double fftfreq_blackman, double spectrum_size) {
+    double samples = spectrum_size * fftfreq_blackman;
+    if (std::abs(samples - 1.) <= 1e-6) samples = std::round(samples);
+    else samples = std::ceil(samples);
+    const i64 n = std::max<i64>(1, static_cast<i64>(samples));
+    return 2 * n + 1 - 2;
+}
+
+template<class C, class S, class M, class Out, class W>
+int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
+    ExtractParams<C> p{};
+    p.vol = d.input.data; p.vs0 = (i32)d.input.s0; p.vs1 = (i32)d.input.s1; p.vs2 = (i32)d.input.s2;
+    p.has_wvol = d.input_weight.data != nullptr;
+    p.wvol = d.input_weight.data; p.wvs0 = (i32)d.input_weight.s0; p.wvs1 = (i32)d.input_weight.s1; p.wvs2 = (i32)d.input_weight.s2;
+    p.out = d.output.data; p.os_b = d.output.s0; p.os_y = (i32)d.output.s1; p.os_x = (i32)d.output.s2;
+    p.has_wout = d.output_weight.data != nullptr;
+    p.wout = d.output_weight.data; p.ws_b = d.output_weight.s0; p.ws_y = (i32)d.output_weight.s1; p.ws_x = (i32)d.output_weight.s2;
+    for (int i = 0; i < 3; ++i) {
+        p.grid[i] = static_cast<i32>(d.in_shape[i + 1]);
+        p.target[i] = static_cast<C>(d.target[i]);
+    }
+    p.slice_f[0] = static_cast<C>(d.out_shape[2]);
+    p.slice_f[1] = static_cast<C>(d.out_shape[3]);
+    p.slice_h = static_cast<i32>(d.out_shape[2]);
+    p.slice_wr = static_cast<i32>(d.out_shape[3] / 2 + 1);
+    p.pixels = p.slice_h * p.slice_wr;
+    C cutoff = std::max(static_cast<C>(d.cutoff), C(0));
+    p.cutoff_sqd = cutoff * cutoff;
+    p.general = d.general;
+    if (d.general && (d.ews_radius[0] != 0 || d.ews_radius[1] != 0)) {
+        p.ews[0] = 1 / (2 * static_cast<C>(d.ews_radius[0]));
+        p.ews[1] = 1 / (2 * static_cast<C>(d.ews_radius[1]));
+    }
+    // Along the w of the grid (:622-626).
+    p.fftfreq_sinc = std::max(static_cast<C>(d.fftfreq_sinc), 1 / p.target[0]);
+    p.fftfreq_blackman = std::max(static_cast<C>(d.fftfreq_blackman), 1 / p.target[0]);
+    p.window = static_cast<i32>(blackman_window_size(static_cast<double>(p.fftfreq_blackman),
+                                                     static_cast<double>(p.target[0])));
+    p.rot = to_arg<C>(d.rot);
+    p.scale = to_arg<C>(d.scale);
+    p.vol_centered = d.src_centered;
+    p.slice_centered = d.dst_centered;
+
+    // Pixels per lane: the largest of {4, 3, 2, 1} that tiles a slice without a partial workgroup, else 4.
+    p.ppt = 4;
+    for (int c : {4, 3, 2, 1}) {
+        if (p.pixels % (256 * c) == 0) { p.ppt = c; break; }
+    }
+    p.tiles = (p.pixels + 256 * p.ppt - 1) / (256 * p.ppt);
+    const i64 n_slices = d.out_shape[0];
+    const i64 blocks = n_slices * p.tiles;
+    if (blocks == 0) return NOA_HIP_OK;
+    if (blocks > 0x7fffffffLL) { err = "too many slices for one launch"; return NOA_HIP_ERROR_UNSUPPORTED; }
+
+    const dim3 grid(static_cast<unsigned>(blocks)), block(256);
+    if (d.interp == NOA_HIP_INTERP_NEAREST)
+        hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_NEAREST>), grid, block, 0, stream, p);
+    else
+        hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_LINEAR>), grid, block, 0, stream, p);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+    return NOA_HIP_OK;
+}
+
+template<class C>
+int dispatch(const ProjectDesc& d, hipStream_t stream, std::string& err) {
+    const int in = d.input.dtype, out = d.output.dtype;
+    if (in == NOA_HIP_C32 && out == NOA_HIP_C32) return run<C, float2, float2, float2, float>(d, stream, err);
+    if (in == NOA_HIP_C32 && out == NOA_HIP_F32) return run<C, float2, float2, float, float>(d, stream, err);
+    if (in == NOA_HIP_F32 && out == NOA_HIP_F32) return run<C, float, float, float, float>(d, stream, err);
+    if (in == NOA_HIP_C16 && out == NOA_HIP_C32) return run<C, __half2, float2, float2, float>(d, stream, err);
+    if (in == NOA_HIP_CBF16 && out == NOA_HIP_C32) return run<C, __hip_bfloat162, float2, float2, float>(d, stream, err);
+    if (in == NOA_HIP_C64 && out == NOA_HIP_C64) return run<C, double2, double2, double2, double>(d, stream, err);
+    if (in == NOA_HIP_C64 && out == NOA_HIP_F64) return run<C, double2, double2, double, double>(d, stream, err);
+    if (in == NOA_HIP_F64 && out == NOA_HIP_F64) return run<C, double, double, double, double>(d, stream, err);
+    err = "extract: unsupported input/output value types";
+    return NOA_HIP_ERROR_UNSUPPORTED;
+}
+
+} // namespace
+
+int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err) {
+    return d.coord_f64 ? dispatch<double>(d, stream, err) : dispatch<float>(d, stream, err);
+}
+
+} // namespace noa_hip

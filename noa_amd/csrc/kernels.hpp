@@ -1,0 +1,52 @@
+// Resolved (validated, flattened) parameters of the three operators and their launchers. The C ABI layer
+// (api.cpp) checks a noa_hip_project_args, fills one of these structures and calls the launcher.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/noa_hip.h"
+
+namespace noa_hip {
+
+using i32 = int32_t;
+using i64 = int64_t;
+
+// A flattened strided array: slices use (batch, y, x) strides, volumes (z, y, x) strides.
+struct Arr {
+    void* data{};
+    i64 s0{}, s1{}, s2{};
+    int dtype{};          // NOA_HIP_* or NOA_HIP_DTYPE_NONE (Empty)
+    bool is_value{};
+    double value[2]{};
+    bool present() const { return data != nullptr || is_value; }
+};
+
+struct Xform {
+    const void* data{};
+    int kind{};           // NOA_HIP_XFORM_NONE = Empty
+    bool batched{};
+    double single[9]{};
+};
+
+// Everything the kernels of one call need; coordinates are converted to the rotation precision on launch.
+struct ProjectDesc {
+    Arr input, input_weight, output, output_weight;
+    i64 in_shape[4]{}, out_shape[4]{};   // logical BDHW
+    i64 target[3]{};                     // logical DHW of the target (resolved default)
+    bool src_centered{}, dst_centered{};
+    int interp{};                        // NOEA_HIP_INTERP_* with _FAST erased
+    Xform rot, scale;
+    bool general{};                      // scaling or EWS present
+    double ews_radius[2]{};
+    double cutoff{}, fftfreq_sinc{}, fftfreq_blackman{};
+    bool coord_f64{};
+};
+
+// Launchers: return 0 or a NOA_HIP_ERROR_* code, with a message in `err`.
+int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err);
+int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err);
+int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
+
+} // namespace noa_hip

@@ -1,0 +1,952 @@
+/*
+ * TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
+ *
+ * CPU restatement of noa's central-slice operators (thomasfrosio/noa v0.10.0). Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library: as the parity checker of the
+ * HIP path and as the timed CPU baseline. Nothing in noa_amd/ links or calls it.
+ *
+ * It follows, expression by expression and in the same evaluation order, the reference's CPU path:
+ *   - FourierInsertRasterize   src/noa/xform/FourierProject.hpp:161-390
+ *   - FourierInsertInterpolate src/noa/xform/FourierProject.hpp:392-539
+ *   - FourierExtract (3D, 4D)  src/noa/xform/FourierProject.hpp:541-725
+ *   - windowed sinc helpers    src/noa/xform/FourierProject.hpp:97-159
+ *   - slice<->grid transforms  src/noa/xform/FourierProject.hpp:38-95
+ *   - interpolate_spectrum     src/noa/xform/core/Interpolation.hpp:416-526 (+ weights :64-146)
+ *   - index/frequency math     src/noa/fft/core/Frequency.hpp:80-145, :250-288
+ *   - CPU iwise execution      src/noa/runtime/cpu/Iwise.hpp:7-117 (+ atomics cpu/ComputeHandle.hpp:39-55)
+ *   - parameter defaults       src/noa/xform/FourierProject.hpp:1260-1487 (launchers)
+ *
+ * Compiled with -ffp-contract=off so that a*b+c is never fused, as in the reference's x86-64 build.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   - index/frequency/bounds: closed forms and the worked examples of Frequency.hpp:34-35, :57-58, :82-83,
+ *     :117-118, :253-256.
+ *   - rotations (euler2matrix, quaternions): rotate_z(90deg) maps x to y; scipy's independent ZYZ rotation;
+ *     quaternion <-> matrix round trips; orthonormality.
+ *   - windowed sinc: closed form, w(0)=1, window-size examples of FourierProject.hpp:118-122, :140-141.
+ *   - interpolate_spectrum: exact copies at integer frequencies, exact reproduction of a linear field
+ *     inside the bounds, Hermitian symmetry, zeros outside the bounds, and an independent numpy restatement.
+ *   - rasterize: identity-pose closed form incl. the x=0 Hermitian duplicate, mass conservation, the
+ *     reference's remap and weights invariants (TestXformFourierInsertRasterize.cpp:72-152), and an FP64
+ *     brute force written separately in numpy.
+ *   - insert (interpolate): identity pose closed form; equality with an independent per-slice scatter in
+ *     numpy; matrix == quaternion.
+ *   - extract: identity pose closed form; rasterize->extract round trip on integer grids; the reference's
+ *     remap invariant (TestXformFourierExtract.cpp:116-153); an independent numpy restatement.
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <algorithm>
+#include <omp.h>
+
+#include "../include/noa_hip.h"
+
+namespace oracle {
+
+using i64 = int64_t;
+
+// ---------------------------------------------------------------------------------------------------------
+// Value types. Complex arithmetic follows src/noa/base/Complex.hpp:157-216.
+// ---------------------------------------------------------------------------------------------------------
+template<class T> struct Cx { T re, im; };
+template<class T> struct is_cx { static constexpr bool value = false; };
+template<class T> struct is_cx<Cx<T>> { static constexpr bool value = true; };
+template<class T> struct real_of { using type = T; };
+template<class T> struct real_of<Cx<T>> { using type = T; };
+
+template<class T> inline Cx<T> operator+(Cx<T> a, Cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template<class T> inline Cx<T>& operator+=(Cx<T>& a, Cx<T> b) { a = a + b; return a; }
+// Complex * real = {s*re, s*im} (Complex.hpp:203-216).
+template<class T> inline Cx<T> operator*(Cx<T> a, T s) { return {s * a.re, s * a.im}; }
+
+template<class T> inline T zero() { return T{}; }
+
+// static_cast between value types of the same kind (real->real, complex->complex).
+template<class To, class From>
+inline To vcast(const From& v) {
+    if constexpr (is_cx<To>::value && is_cx<From>::value) {
+        using R = typename real_of<To>::type;
+        return To{static_cast<R>(v.re), static_cast<R>(v.im)};
+    } else if constexpr (!is_cx<To>::value && !is_cx<From>::value) {
+        return static_cast<To>(v);
+    } else {
+        static_assert(!is_cx<To>::value, "real to complex is not a spectrum type");
+        // complex -> real: cast_or_abs_squared (base/Math.hpp:220-225, Complex.hpp:398-402)
+        return static_cast<To>(v.re * v.re + v.im * v.im);
+    }
+}
+
+// cast_or_abs_squared<T>(value) (base/Math.hpp:220-225).
+template<class To, class From>
+inline To cast_or_abs_squared(const From& v) { return vcast<To>(v); }
+
+// ---------------------------------------------------------------------------------------------------------
+// Small vectors and matrices (row-major; Mat*Vec is a row-wise dot, base/Mat.hpp:419-424; dot is a left fold,
+// base/Vec.hpp:846-855).
+// ---------------------------------------------------------------------------------------------------------
+template<class T> struct V2 { T v[2]; T& operator[](int i) { return v[i]; } const T& operator[](int i) const { return v[i]; } };
+template<class T> struct V3 { T v[3]; T& operator[](int i) { return v[i]; } const T& operator[](int i) const { return v[i]; } };
+template<class T> struct M22 { T m[2][2]; };
+template<class T> struct M33 { T m[3][3]; };
+template<class T> struct Quat { T z, y, x, w; };
+
+template<class T> inline T dot3(const V3<T>& a, const V3<T>& b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+template<class T> inline T dot2(const V2<T>& a, const V2<T>& b) { return a[0] * b[0] + a[1] * b[1]; }
+
+template<class T> inline V3<T> matvec(const M33<T>& r, const V3<T>& x) {
+    V3<T> out;
+    for (int i = 0; i < 3; ++i)
+        out[i] = (r.m[i][0] * x[0] + r.m[i][1] * x[1]) + r.m[i][2] * x[2];
+    return out;
+}
+template<class T> inline V2<T> matvec(const M22<T>& s, const V2<T>& x) {
+    V2<T> out;
+    for (int i = 0; i < 2; ++i)
+        out[i] = s.m[i][0] * x[0] + s.m[i][1] * x[1];
+    return out;
+}
+// Quaternion::rotate (xform/core/Quaternion.hpp:167-181).
+template<class T> inline V3<T> quat_rotate(const Quat<T>& q, const V3<T>& zyx) {
+    const T a0 = q.w * zyx[2] + q.y * zyx[0] - q.z * zyx[1];
+    const T a1 = q.w * zyx[1] + q.z * zyx[2] - q.x * zyx[0];
+    const T a2 = q.w * zyx[0] + q.x * zyx[1] - q.y * zyx[2];
+    const T two = 2;
+    V3<T> out;
+    out[0] = zyx[0] + two * (q.x * a1 - q.y * a0);
+    out[1] = zyx[1] + two * (q.z * a0 - q.x * a2);
+    out[2] = zyx[2] + two * (q.y * a2 - q.z * a1);
+    return out;
+}
+
+// A batched rotation (Mat33 or Quaternion) or scaling (Mat22), as BatchedParameter (runtime/core/Utils.hpp:391-413).
+template<class T>
+struct Rotation {
+    const unsigned char* data{};
+    int kind{};
+    bool batched{};
+    V3<T> apply(i64 b, const V3<T>& x) const {
+        const i64 i = batched ? b : 0;
+        if (kind == NOA_HIP_MAT33_F32 || kind == NOA_HIP_MAT33_F64) {
+            M33<T> r;
+            const bool f32 = kind == NOA_HIP_MAT33_F32;
+            const size_t stride = f32 ? 48 : 80;   // alignas(16) Mat33 (base/Mat.hpp:177)
+            const unsigned char* p = data + i * stride;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    if (f32) { float e; std::memcpy(&e, p + (a * 3 + c) * 4, 4); r.m[a][c] = static_cast<T>(e); }
+                    else { double e; std::memcpy(&e, p + (a * 3 + c) * 8, 8); r.m[a][c] = static_cast<T>(e); }
+                }
+            return matvec(r, x);
+        } else {
+            Quat<T> q;
+            if (kind == NOA_HIP_QUAT_F32) {
+                float e[4]; std::memcpy(e, data + i * 16, 16);
+                q = {static_cast<T>(e[0]), static_cast<T>(e[1]), static_cast<T>(e[2]), static_cast<T>(e[3])};
+            } else {
+                double e[4]; std::memcpy(e, data + i * 32, 32);
+                q = {static_cast<T>(e[0]), static_cast<T>(e[1]), static_cast<T>(e[2]), static_cast<T>(e[3])};
+            }
+            return quat_rotate(q, x);
+        }
+    }
+};
+
+template<class T>
+struct Scaling {
+    const unsigned char* data{};
+    int kind{};      // NOA_HIP_XFORM_NONE = Empty (identity without arithmetic)
+    bool batched{};
+    V2<T> apply(i64 b, const V2<T>& x) const {
+        if (kind == NOA_HIP_XFORM_NONE)
+            return x;
+        const i64 i = batched ? b : 0;
+        M22<T> s;
+        const bool f32 = kind == NOA_HIP_MAT22_F32;
+        const unsigned char* p = data + i * (f32 ? 16 : 32);
+        for (int a = 0; a < 2; ++a)
+            for (int c = 0; c < 2; ++c) {
+                if (f32) { float e; std::memcpy(&e, p + (a * 2 + c) * 4, 4); s.m[a][c] = static_cast<T>(e); }
+                else { double e; std::memcpy(&e, p + (a * 2 + c) * 8, 8); s.m[a][c] = static_cast<T>(e); }
+            }
+        return matvec(s, x);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// Index <-> frequency (fft/core/Frequency.hpp).
+// ---------------------------------------------------------------------------------------------------------
+inline i64 index2frequency(bool centered, i64 index, i64 size) {               // Frequency.hpp:80-87
+    if (centered) return index - size / 2;
+    return index < (size + 1) / 2 ? index : index - size;
+}
+inline i64 frequency2index(bool centered, i64 frequency, i64 size) {           // Frequency.hpp:115-122
+    if (centered) return frequency + size / 2;
+    return frequency < 0 ? frequency + size : frequency;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Windowed sinc (FourierProject.hpp:97-159, noa::sinc base/Math.hpp:28-29).
+// ---------------------------------------------------------------------------------------------------------
+template<class T> inline T sinc(T x) { return x == 0 ? T(1) : std::sin(x) / x; }
+
+template<class T>
+inline T windowed_sinc(T fftfreq, T fftfreq_sinc, T fftfreq_blackman) {
+    const T PI = static_cast<T>(3.141592653589793238462643383279502884);
+    fftfreq *= PI;
+    const T s = sinc(fftfreq / fftfreq_sinc);
+    const T blackman_cutoff = fftfreq / fftfreq_blackman;
+    const T blackman = static_cast<T>(0.42) + static_cast<T>(0.5) * std::cos(blackman_cutoff) +
+                       static_cast<T>(0.08) * std::cos(2 * blackman_cutoff);
+    return s * blackman;
+}
+
+// noa::allclose(a, b) with the default epsilon 1e-6 (base/Math.hpp allclose: |a-b| <= eps).
+inline bool allclose_default(double a, double b) { return std::abs(a - b) <= 1e-6; }
+
+template<class T>
+inline i64 blackman_window_size(T fftfreq_blackman, T spectrum_size) {  // FourierProject.hpp:114-137
+    double samples = static_cast<double>(spectrum_size) * static_cast<double>(fftfreq_blackman);
+    if (allclose_default(samples, 1.))
+        samples = std::round(samples);
+    else
+        samples = std::ceil(samples);
+    const i64 n = std::max(i64{1}, static_cast<i64>(samples));
+    return 2 * n + 1 - 2;
+}
+
+template<class T>
+inline T w_index_to_fftfreq_offset(i64 w, i64 window_size, T spectrum_size) {  // :142-145
+    return static_cast<T>(w - window_size / 2) / spectrum_size;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Strided readers/writers over host memory, typed by template. A reader can also be a constant (AccessorValue)
+// or empty.
+// ---------------------------------------------------------------------------------------------------------
+template<class T>
+struct Reader3 {   // (batch, j, k) view, or (batch, i, j, k) for the 3D spectrum
+    const T* p{};
+    i64 s0{}, s1{}, s2{}, s3{};
+    bool is_value{};
+    T value{};
+    T operator()(i64 a, i64 b, i64 c) const { return is_value ? value : p[a * s0 + b * s1 + c * s2]; }
+    T operator()(i64 a, i64 b, i64 c, i64 d) const { return is_value ? value : p[a * s0 + b * s1 + c * s2 + d * s3]; }
+};
+
+template<class T>
+struct Writer3 {
+    T* p{};
+    i64 s0{}, s1{}, s2{};
+    T& operator()(i64 a, i64 b, i64 c) const { return p[a * s0 + b * s1 + c * s2]; }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// interpolate_spectrum<REMAP, INTERP, N> for N = 2, 3 (xform/core/Interpolation.hpp:435-526).
+// Supports NEAREST and LINEAR (window size 2: early flip, :459-463, :522-523).
+// `freq` is the centred frequency in samples; `shape` the logical shape; rfft along the last dimension.
+// ---------------------------------------------------------------------------------------------------------
+template<class V, class Coord>
+V interpolate_spectrum_2d(const Reader3<V>& in, i64 batch, Coord fy, Coord fx, i64 ny, i64 nx,
+                          bool centered, int interp) {
+    using R = typename real_of<V>::type;
+    R conjugate = 1;
+    if (fx < 0) { fy *= -1; fx *= -1; conjugate = -1; }    // flip_frequency (:417-426), Vec *= -1
+    const i64 lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;   // frequency_bounds<RFFT> (Frequency.hpp:258-276)
+    auto fetch = [&](i64 y, i64 x, R w, V& out) {
+        if (y < lo_y || y > hi_y || x < 0 || x > hi_x) return;
+        out += vcast<V>(in(batch, frequency2index(centered, y, ny), x)) * w;
+    };
+    V value = zero<V>();
+    const int base = interp >= 100 ? interp - 100 : interp;
+    if (base == NOA_HIP_INTERP_NEAREST) {
+        // The NEAREST branch returns before the conjugation of the early flip (:479-484): after a flip the
+        // imaginary part is NOT negated. Reproduced as is.
+        fetch(static_cast<i64>(std::round(fy)), static_cast<i64>(std::round(fx)), R(1), value);
+        return value;
+    }
+    const Coord fly = std::floor(fy), flx = std::floor(fx);
+    const i64 iy = static_cast<i64>(fly), ix = static_cast<i64>(flx);
+    const R ty = static_cast<R>(fy - fly), tx = static_cast<R>(fx - flx);
+    const R wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
+    for (int y = 0; y < 2; ++y) {
+        V vy = zero<V>();
+        for (int x = 0; x < 2; ++x)
+            fetch(iy + y, ix + x, wx[x], vy);
+        value += vy * wy[y];
+    }
+    if constexpr (is_cx<V>::value) value.im *= conjugate;
+    return value;
+}
+
+template<class V, class Coord>
+V interpolate_spectrum_3d(const Reader3<V>& in, Coord fz, Coord fy, Coord fx, i64 nz, i64 ny, i64 nx,
+                          bool centered, int interp) {
+    using R = typename real_of<V>::type;
+    R conjugate = 1;
+    if (fx < 0) { fz *= -1; fy *= -1; fx *= -1; conjugate = -1; }
+    const i64 lo_z = -nz / 2, hi_z = (nz - 1) / 2, lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
+    auto fetch = [&](i64 z, i64 y, i64 x, R w, V& out) {
+        if (z < lo_z || z > hi_z || y < lo_y || y > hi_y || x < 0 || x > hi_x) return;
+        out += vcast<V>(in(0, frequency2index(centered, z, nz), frequency2index(centered, y, ny), x)) * w;
+    };
+    V value = zero<V>();
+    const int base = interp >= 100 ? interp - 100 : interp;
+    if (base == NOA_HIP_INTERP_NEAREST) {
+        fetch(static_cast<i64>(std::round(fz)), static_cast<i64>(std::round(fy)),
+              static_cast<i64>(std::round(fx)), R(1), value);
+        return value;   // no conjugation after the flip, as in the reference (:479-484)
+    }
+    const Coord flz = std::floor(fz), fly = std::floor(fy), flx = std::floor(fx);
+    const i64 iz = static_cast<i64>(flz), iy = static_cast<i64>(fly), ix = static_cast<i64>(flx);
+    const R tz = static_cast<R>(fz - flz), ty = static_cast<R>(fy - fly), tx = static_cast<R>(fx - flx);
+    const R wz[2] = {1 - tz, tz}, wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
+    for (int z = 0; z < 2; ++z) {
+        V vz = zero<V>();
+        for (int y = 0; y < 2; ++y) {
+            V vy = zero<V>();
+            for (int x = 0; x < 2; ++x)
+                fetch(iz + z, iy + y, ix + x, wx[x], vy);
+            vz += vy * wy[y];
+        }
+        value += vz * wz[z];
+    }
+    if constexpr (is_cx<V>::value) value.im *= conjugate;
+    return value;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Execution: the CPU iwise (runtime/cpu/Iwise.hpp:97-117). Parallel only above 2^20 indices, with
+// threads = min(n_threads, n / 2^20). The rasterisation's complex atomic add is two `omp atomic` adds
+// (cpu/ComputeHandle.hpp:39-55); serial mode uses a plain +=.
+// ---------------------------------------------------------------------------------------------------------
+inline int iwise_threads(i64 n_elements, int n_threads) {
+    const i64 per = i64{1} << 20;
+    int t = n_elements <= per ? 1 : n_threads;
+    if (t > 1) t = static_cast<int>(std::min<i64>(n_threads, n_elements / per));
+    return std::max(t, 1);
+}
+
+template<class Op>
+void iwise3(i64 n0, i64 n1, i64 n2, const Op& op, int n_threads, bool force_threads) {
+    const int t = force_threads ? std::max(1, n_threads) : iwise_threads(n0 * n1 * n2, n_threads);
+    if (t > 1) {
+        #pragma omp parallel num_threads(t)
+        {
+            Op local = op;   // firstprivate(op)
+            #pragma omp for collapse(3)
+            for (i64 i = 0; i < n0; ++i)
+                for (i64 j = 0; j < n1; ++j)
+                    for (i64 k = 0; k < n2; ++k)
+                        local(i, j, k, true);
+        }
+    } else {
+        for (i64 i = 0; i < n0; ++i)
+            for (i64 j = 0; j < n1; ++j)
+                for (i64 k = 0; k < n2; ++k)
+                    op(i, j, k, false);
+    }
+}
+
+template<class Op>
+void iwise4(i64 n0, i64 n1, i64 n2, i64 n3, const Op& op, int n_threads, bool force_threads) {
+    const int t = force_threads ? std::max(1, n_threads) : iwise_threads(n0 * n1 * n2 * n3, n_threads);
+    if (t > 1) {
+        #pragma omp parallel num_threads(t)
+        {
+            Op local = op;
+            #pragma omp for collapse(4)
+            for (i64 i = 0; i < n0; ++i)
+                for (i64 j = 0; j < n1; ++j)
+                    for (i64 k = 0; k < n2; ++k)
+                        for (i64 l = 0; l < n3; ++l)
+                            local(i, j, k, l, true);
+        }
+    } else {
+        for (i64 i = 0; i < n0; ++i)
+            for (i64 j = 0; j < n1; ++j)
+                for (i64 k = 0; k < n2; ++k)
+                    for (i64 l = 0; l < n3; ++l)
+                        op(i, j, k, l, false);
+    }
+}
+
+template<class T>
+inline void atomic_add(T& dst, const T& v, bool parallel) {
+    if constexpr (is_cx<T>::value) {
+        if (parallel) {
+            #pragma omp atomic
+            dst.re += v.re;
+            #pragma omp atomic
+            dst.im += v.im;
+        } else {
+            dst = dst + v;
+        }
+    } else {
+        if (parallel) {
+            #pragma omp atomic
+            dst += v;
+        } else {
+            dst += v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Common per-call parameters.
+// ---------------------------------------------------------------------------------------------------------
+template<class Coord>
+struct Geometry {
+    Rotation<Coord> rot;
+    Scaling<Coord> scale;
+    bool general{};           // scale or EWS present (FourierProject.hpp:1312-1316)
+    Coord ews_inv[2]{};       // 1 / (2 * ews_radius) per component, or {0, 0}
+
+    // fourier_slice2grid (FourierProject.hpp:41-68).
+    V3<Coord> slice2grid(V2<Coord> f, i64 b) const {
+        V3<Coord> f3;
+        if (general) {
+            f = scale.apply(b, f);
+            f3[0] = (ews_inv[0] * f[0]) * f[0] + (ews_inv[1] * f[1]) * f[1];
+        } else {
+            f3[0] = 0;
+        }
+        f3[1] = f[0];
+        f3[2] = f[1];
+        return rot.apply(b, f3);
+    }
+    // fourier_grid2slice (FourierProject.hpp:71-95).
+    void grid2slice(const V3<Coord>& f, i64 b, Coord& fz, V2<Coord>& f2) const {
+        const V3<Coord> r = rot.apply(b, f);
+        f2[0] = r[1];
+        f2[1] = r[2];
+        fz = r[0];
+        if (general) {
+            fz -= (ews_inv[0] * f2[0]) * f2[0] + (ews_inv[1] * f2[1]) * f2[1];
+            f2 = scale.apply(b, f2);
+        }
+    }
+};
+
+template<class Coord>
+Geometry<Coord> make_geometry(const noa_hip_project_args& a) {
+    Geometry<Coord> g;
+    g.rot.data = static_cast<const unsigned char*>(a.rotation.data);
+    g.rot.kind = a.rotation.kind;
+    g.rot.batched = a.rotation.batched != 0;
+    g.scale.data = static_cast<const unsigned char*>(a.scaling.data);
+    g.scale.kind = a.scaling.kind;
+    g.scale.batched = a.scaling.batched != 0;
+    const bool has_ews = a.ews_radius[0] != 0 || a.ews_radius[1] != 0;
+    // fourier_project_has_scale: a single Mat22 counts only if it differs from the identity (:1234-1244).
+    bool has_scale = a.scaling.kind != NOA_HIP_XFORM_NONE;
+    if (has_scale && !a.scaling.batched) {
+        double e[4];
+        if (a.scaling.kind == NOA_HIP_MAT22_F32) { float f[4]; std::memcpy(f, a.scaling.data, 16); for (int i = 0; i < 4; ++i) e[i] = f[i]; }
+        else std::memcpy(e, a.scaling.data, 32);
+        has_scale = !(e[0] == 1 && e[1] == 0 && e[2] == 0 && e[3] == 1);
+    }
+    g.general = has_ews || has_scale;
+    if (!has_scale) g.scale.kind = NOA_HIP_XFORM_NONE;
+    if (g.general) {
+        const Coord r0 = static_cast<Coord>(a.ews_radius[0]), r1 = static_cast<Coord>(a.ews_radius[1]);
+        if (r0 != 0 || r1 != 0) {   // `ews_radius != 0 ? 1 / (2 * ews_radius) : {}` (:231)
+            g.ews_inv[0] = 1 / (2 * r0);
+            g.ews_inv[1] = 1 / (2 * r1);
+        }
+    }
+    return g;
+}
+
+inline bool remap_src_centered(int remap) { return (remap & 4) != 0; }   // is_xc2xx
+inline bool remap_dst_centered(int remap) { return (remap & 8) != 0; }   // is_xx2xc
+
+template<class T>
+Reader3<T> reader_slices(const noa_hip_view& v) {   // filter {0, 2, 3}
+    Reader3<T> r;
+    r.p = static_cast<const T*>(v.data);
+    r.s0 = v.strides[0]; r.s1 = v.strides[2]; r.s2 = v.strides[3];
+    r.is_value = v.is_value != 0;
+    if constexpr (is_cx<T>::value) r.value = T{static_cast<typename real_of<T>::type>(v.value[0]), static_cast<typename real_of<T>::type>(v.value[1])};
+    else r.value = static_cast<T>(v.value[0]);
+    return r;
+}
+
+template<class T>
+Reader3<T> reader_interp2(const noa_hip_view& v) {   // to_interpolator_input<2>: batch stride 0 if B == 1
+    Reader3<T> r = reader_slices<T>(v);
+    if (v.shape[0] == 1) r.s0 = 0;
+    return r;
+}
+
+template<class T>
+Reader3<T> reader_volume(const noa_hip_view& v) {    // 3D interpolator input (batch, z, y, x)
+    Reader3<T> r;
+    r.p = static_cast<const T*>(v.data);
+    r.s0 = 0; r.s1 = v.strides[1]; r.s2 = v.strides[2]; r.s3 = v.strides[3];
+    r.is_value = v.is_value != 0;
+    return r;
+}
+
+template<class T>
+Writer3<T> writer_volume(const noa_hip_view& v) {    // filter {1, 2, 3}
+    Writer3<T> w;
+    w.p = static_cast<T*>(v.data);
+    w.s0 = v.strides[1]; w.s1 = v.strides[2]; w.s2 = v.strides[3];
+    return w;
+}
+
+template<class T>
+Writer3<T> writer_slices(const noa_hip_view& v) {    // filter {0, 2, 3}
+    Writer3<T> w;
+    w.p = static_cast<T*>(v.data);
+    w.s0 = v.strides[0]; w.s1 = v.strides[2]; w.s2 = v.strides[3];
+    return w;
+}
+
+inline void target3(const noa_hip_project_args& a, const int64_t* grid_logical, i64 out[3]) {
+    const bool any0 = a.target_shape[0] == 0 || a.target_shape[1] == 0 || a.target_shape[2] == 0 || a.target_shape[3] == 0;
+    for (int i = 0; i < 3; ++i)
+        out[i] = any0 ? grid_logical[i + 1] : a.target_shape[i + 1];
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// FourierInsertRasterize (FourierProject.hpp:161-390).
+// ---------------------------------------------------------------------------------------------------------
+template<class Coord, class In, class Out, class W>
+struct RasterizeOp {
+    using RIn = typename real_of<In>::type;
+    using ROut = typename real_of<Out>::type;
+    Reader3<In> slices;
+    Reader3<W> wslices; bool has_wslices{};
+    Writer3<Out> volume;
+    Writer3<W> wvolume; bool has_wvolume{};
+    Geometry<Coord> geo;
+    i64 grid[3]{};           // logical (D, H, W)
+    i64 slice_h{};
+    Coord f_slice[2]{};      // (H, W)
+    Coord f_target[3]{};
+    Coord cutoff_sqd{};
+    bool slices_centered{}, volume_centered{};
+
+    void operator()(i64 b, i64 y, i64 u, bool parallel) const {
+        const i64 v = index2frequency(slices_centered, y, slice_h);
+        V2<Coord> f2;
+        f2[0] = static_cast<Coord>(v) / f_slice[0];
+        f2[1] = static_cast<Coord>(u) / f_slice[1];
+        V3<Coord> f3 = geo.slice2grid(f2, b);
+        if (dot3(f3, f3) > cutoff_sqd)
+            return;
+        RIn conjugate = 1;
+        if (f3[2] < 0) {
+            f3[0] = -f3[0]; f3[1] = -f3[1]; f3[2] = -f3[2];
+            if constexpr (is_cx<In>::value) conjugate = -1;
+        }
+        V3<Coord> freq;
+        for (int i = 0; i < 3; ++i) freq[i] = f3[i] * f_target[i];
+
+        // get_input_value_ (:270-277): complex -> complex multiplies by `conjugate` (a negation when -1).
+        Out value;
+        const In raw = slices(b, y, u);
+        if constexpr (is_cx<In>::value && is_cx<Out>::value) value = vcast<Out>(raw * conjugate);
+        else value = cast_or_abs_squared<Out>(raw);
+        // get_input_weight_ (:279-286).
+        W weight{};
+        if (has_wvolume) weight = has_wslices ? static_cast<W>(wslices(b, y, u)) : W(1);
+
+        rasterize(value, weight, freq, parallel);
+    }
+
+    void rasterize(Out value, W weight, const V3<Coord>& freq, bool parallel) const {
+        i64 base0[3];
+        for (int i = 0; i < 3; ++i) base0[i] = static_cast<i64>(std::floor(freq[i]));
+        // set_rasterization_weights_ (:290-304)
+        Coord fr[2][3];
+        for (int i = 0; i < 3; ++i) {
+            fr[1][i] = freq[i] - static_cast<Coord>(base0[i]);
+            fr[0][i] = 1 - fr[1][i];
+        }
+        Coord k[2][2][2];
+        for (int w = 0; w < 2; ++w)
+            for (int v = 0; v < 2; ++v)
+                for (int u = 0; u < 2; ++u)
+                    k[w][v][u] = fr[w][0] * fr[v][1] * fr[u][2];
+
+        const i64 width = grid[2] / 2 + 1;   // stored rfft width (see quirk note below)
+        for (int w = 0; w < 2; ++w) {
+            for (int v = 0; v < 2; ++v) {
+                for (int u = 0; u < 2; ++u) {
+                    const i64 iw = frequency2index(volume_centered, base0[0] + w, grid[0]);
+                    const i64 iv = frequency2index(volume_centered, base0[1] + v, grid[1]);
+                    const i64 iu = base0[2] + u;
+                    // The reference bounds iu by the logical W (:332); a tap with iu >= W/2+1 only occurs at
+                    // freq_x == W/2 with a zero weight (cutoff <= 0.5), so skipping it is identical.
+                    if (iw >= 0 && iw < grid[0] && iv >= 0 && iv < grid[1] && iu >= 0 && iu < width) {
+                        const Coord fraction = k[w][v][u];
+                        atomic_add(volume(iw, iv, iu), value * static_cast<ROut>(fraction), parallel);
+                        if (has_wvolume)
+                            atomic_add(wvolume(iw, iv, iu), weight * static_cast<W>(fraction), parallel);
+                    }
+                }
+            }
+        }
+        // Hermitian x=0 plane (:349-372).
+        if (base0[2] == 0) {
+            if constexpr (is_cx<Out>::value) value.im = -value.im;
+            for (int w = 0; w < 2; ++w) {
+                for (int v = 0; v < 2; ++v) {
+                    const i64 iw = frequency2index(volume_centered, -(base0[0] + w), grid[0]);
+                    const i64 iv = frequency2index(volume_centered, -(base0[1] + v), grid[1]);
+                    if (iw >= 0 && iw < grid[0] && iv >= 0 && iv < grid[1]) {
+                        const Coord fraction = k[w][v][0];
+                        atomic_add(volume(iw, iv, 0), value * static_cast<ROut>(fraction), parallel);
+                        if (has_wvolume)
+                            atomic_add(wvolume(iw, iv, 0), weight * static_cast<W>(fraction), parallel);
+                    }
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// FourierInsertInterpolate (FourierProject.hpp:392-539).
+// ---------------------------------------------------------------------------------------------------------
+template<class Coord, class In, class Out, class W>
+struct InsertOp {
+    Reader3<In> slices;
+    Reader3<W> wslices; bool has_wslices{};
+    Writer3<Out> volume;
+    Writer3<W> wvolume; bool has_wvolume{};
+    Geometry<Coord> geo;
+    i64 grid[3]{};
+    i64 n_slices{};
+    i64 slice_shape[2]{};
+    Coord f_slice[2]{};
+    Coord f_target[3]{};
+    Coord cutoff_sqd{}, fftfreq_sinc{}, fftfreq_blackman{};
+    bool slices_centered{}, volume_centered{};
+    int interp{};
+
+    void operator()(i64 oz, i64 oy, i64 ox, bool) const {
+        V3<Coord> f;
+        f[0] = static_cast<Coord>(index2frequency(volume_centered, oz, grid[0])) / f_target[0];
+        f[1] = static_cast<Coord>(index2frequency(volume_centered, oy, grid[1])) / f_target[1];
+        f[2] = static_cast<Coord>(ox) / f_target[2];
+        if (dot3(f, f) > cutoff_sqd)
+            return;
+        In value = zero<In>();
+        W weights{};
+        for (i64 i = 0; i < n_slices; ++i) {
+            Coord fz;
+            V2<Coord> f2;
+            geo.grid2slice(f, i, fz, f2);
+            In i_value = zero<In>();
+            W i_weights{};
+            if (std::abs(fz) <= fftfreq_blackman) {
+                const Coord window = windowed_sinc(fz, fftfreq_sinc, fftfreq_blackman);
+                const Coord py = f2[0] * f_slice[0], px = f2[1] * f_slice[1];
+                using RIn = typename real_of<In>::type;
+                i_value = mul(interpolate_spectrum_2d<In, Coord>(slices, i, py, px, slice_shape[0], slice_shape[1],
+                                                                 slices_centered, interp),
+                              static_cast<RIn>(window));
+                if (has_wvolume) {
+                    if (has_wslices)
+                        i_weights = interpolate_spectrum_2d<W, Coord>(wslices, i, py, px, slice_shape[0], slice_shape[1],
+                                                                      slices_centered, interp) * static_cast<W>(window);
+                    else
+                        i_weights = static_cast<W>(window);
+                }
+            }
+            value = add(value, i_value);
+            if (has_wvolume) weights += i_weights;
+        }
+        Out& out = volume(oz, oy, ox);
+        out = add(out, cast_or_abs_squared<Out>(value));
+        if (has_wvolume) wvolume(oz, oy, ox) += weights;
+    }
+
+    template<class T, class R> static T mul(T a, R s) { if constexpr (is_cx<T>::value) return a * s; else return a * s; }
+    template<class T> static T add(T a, T b) { if constexpr (is_cx<T>::value) return a + b; else return a + b; }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// FourierExtract (FourierProject.hpp:541-725).
+// ---------------------------------------------------------------------------------------------------------
+template<class Coord, class In, class Out, class W>
+struct ExtractOp {
+    using ROut = typename real_of<Out>::type;
+    Reader3<In> volume;
+    Reader3<W> wvolume; bool has_wvolume{};
+    Writer3<Out> slices;
+    Writer3<W> wslices; bool has_wslices{};
+    Geometry<Coord> geo;
+    i64 grid[3]{};
+    i64 slice_h{};
+    Coord f_slice[2]{};
+    Coord f_target[3]{};
+    Coord cutoff_sqd{}, fftfreq_sinc{}, fftfreq_blackman{};
+    i64 blackman_size{};
+    bool volume_centered{}, slices_centered{};
+    int interp{};
+
+    V3<Coord> fftfreq_in_volume(i64 b, i64 y, i64 x) const {   // compute_fftfreq_in_volume_ (:699-705)
+        V2<Coord> f2;
+        f2[0] = static_cast<Coord>(index2frequency(slices_centered, y, slice_h)) / f_slice[0];
+        f2[1] = static_cast<Coord>(x) / f_slice[1];
+        return geo.slice2grid(f2, b);
+    }
+
+    In sample(const V3<Coord>& f) const {
+        return interpolate_spectrum_3d<In, Coord>(volume, f[0] * f_target[0], f[1] * f_target[1], f[2] * f_target[2],
+                                                  grid[0], grid[1], grid[2], volume_centered, interp);
+    }
+    W sample_weight(const V3<Coord>& f) const {
+        return interpolate_spectrum_3d<W, Coord>(wvolume, f[0] * f_target[0], f[1] * f_target[1], f[2] * f_target[2],
+                                                 grid[0], grid[1], grid[2], volume_centered, interp);
+    }
+
+    // 3D operator (:632-657).
+    void operator()(i64 b, i64 y, i64 x, bool) const {
+        const V3<Coord> f = fftfreq_in_volume(b, y, x);
+        Out value = zero<Out>();
+        W weight{};
+        if (dot3(f, f) <= cutoff_sqd) {
+            value = cast_or_abs_squared<Out>(sample(f));
+            if (has_wslices)
+                weight = has_wvolume ? static_cast<W>(sample_weight(f)) : W(1);
+        }
+        slices(b, y, x) = value;
+        if (has_wslices) wslices(b, y, x) = weight;
+    }
+
+    // 4D operator (:661-694).
+    void operator()(i64 b, i64 w, i64 y, i64 x, bool parallel) const {
+        V3<Coord> f = fftfreq_in_volume(b, y, x);
+        const Coord offset = w_index_to_fftfreq_offset<Coord>(w, blackman_size, f_target[0]);
+        f[0] += offset;
+        if (dot3(f, f) > cutoff_sqd)
+            return;
+        const Coord conv = windowed_sinc(offset, fftfreq_sinc, fftfreq_blackman);
+        const Out value = cast_or_abs_squared<Out>(sample(f));
+        atomic_add(slices(b, y, x), value * static_cast<ROut>(conv), parallel);
+        if (has_wslices) {
+            const W weight = has_wvolume ? static_cast<W>(sample_weight(f)) : W(1);
+            atomic_add(wslices(b, y, x), weight * static_cast<W>(conv), parallel);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// Entry points.
+// ---------------------------------------------------------------------------------------------------------
+thread_local std::string g_error;
+
+template<class Coord, class In, class Out, class W>
+int run_rasterize(const noa_hip_project_args& a, int n_threads, bool force) {
+    RasterizeOp<Coord, In, Out, W> op;
+    op.slices = reader_slices<In>(a.input);
+    op.has_wslices = a.input_weight.data != nullptr || a.input_weight.is_value;
+    if (op.has_wslices) op.wslices = reader_slices<W>(a.input_weight);
+    op.volume = writer_volume<Out>(a.output);
+    op.has_wvolume = a.output_weight.data != nullptr;
+    if (op.has_wvolume) op.wvolume = writer_volume<W>(a.output_weight);
+    op.geo = make_geometry<Coord>(a);
+    for (int i = 0; i < 3; ++i) op.grid[i] = a.output_shape[i + 1];
+    op.slice_h = a.input_shape[2];
+    op.f_slice[0] = static_cast<Coord>(a.input_shape[2]);
+    op.f_slice[1] = static_cast<Coord>(a.input_shape[3]);
+    i64 t[3]; target3(a, a.output_shape, t);
+    for (int i = 0; i < 3; ++i) op.f_target[i] = static_cast<Coord>(t[i]);
+    op.cutoff_sqd = std::max(static_cast<Coord>(a.fftfreq_cutoff), Coord(0));
+    op.cutoff_sqd *= op.cutoff_sqd;
+    op.slices_centered = remap_src_centered(a.remap);
+    op.volume_centered = remap_dst_centered(a.remap);
+    iwise3(a.input_shape[0], a.input_shape[2], a.input_shape[3] / 2 + 1, op, n_threads, force);
+    return 0;
+}
+
+template<class Coord, class In, class Out, class W>
+int run_insert(const noa_hip_project_args& a, int n_threads, bool force) {
+    InsertOp<Coord, In, Out, W> op;
+    op.slices = reader_interp2<In>(a.input);
+    op.has_wslices = a.input_weight.data != nullptr || a.input_weight.is_value;
+    if (op.has_wslices) op.wslices = reader_interp2<W>(a.input_weight);
+    op.volume = writer_volume<Out>(a.output);
+    op.has_wvolume = a.output_weight.data != nullptr;
+    if (op.has_wvolume) op.wvolume = writer_volume<W>(a.output_weight);
+    op.geo = make_geometry<Coord>(a);
+    for (int i = 0; i < 3; ++i) op.grid[i] = a.output_shape[i + 1];
+    op.n_slices = a.input_shape[0];
+    op.slice_shape[0] = a.input_shape[2];
+    op.slice_shape[1] = a.input_shape[3];
+    op.f_slice[0] = static_cast<Coord>(a.input_shape[2]);
+    op.f_slice[1] = static_cast<Coord>(a.input_shape[3]);
+    i64 t[3]; target3(a, a.output_shape, t);
+    for (int i = 0; i < 3; ++i) op.f_target[i] = static_cast<Coord>(t[i]);
+    op.cutoff_sqd = std::max(static_cast<Coord>(a.fftfreq_cutoff), Coord(0));
+    op.cutoff_sqd *= op.cutoff_sqd;
+    const Coord max_output_size = static_cast<Coord>(std::min({t[0], t[1], t[2]}));
+    op.fftfreq_sinc = std::max(static_cast<Coord>(a.fftfreq_sinc), 1 / max_output_size);
+    op.fftfreq_blackman = std::max(static_cast<Coord>(a.fftfreq_blackman), 1 / max_output_size);
+    op.slices_centered = remap_src_centered(a.remap);
+    op.volume_centered = remap_dst_centered(a.remap);
+    op.interp = a.interp;
+    iwise3(a.output_shape[1], a.output_shape[2], a.output_shape[3] / 2 + 1, op, n_threads, force);
+    return 0;
+}
+
+template<class Coord, class In, class Out, class W>
+int run_extract(const noa_hip_project_args& a, int n_threads, bool force) {
+    ExtractOp<Coord, In, Out, W> op;
+    op.volume = reader_volume<In>(a.input);
+    op.has_wvolume = a.input_weight.data != nullptr;
+    if (op.has_wvolume) op.wvolume = reader_volume<W>(a.input_weight);
+    op.slices = writer_slices<Out>(a.output);
+    op.has_wslices = a.output_weight.data != nullptr;
+    if (op.has_wslices) op.wslices = writer_slices<W>(a.output_weight);
+    op.geo = make_geometry<Coord>(a);
+    for (int i = 0; i < 3; ++i) op.grid[i] = a.input_shape[i + 1];
+    op.slice_h = a.output_shape[2];
+    op.f_slice[0] = static_cast<Coord>(a.output_shape[2]);
+    op.f_slice[1] = static_cast<Coord>(a.output_shape[3]);
+    i64 t[3]; target3(a, a.input_shape, t);
+    for (int i = 0; i < 3; ++i) op.f_target[i] = static_cast<Coord>(t[i]);
+    op.cutoff_sqd = std::max(static_cast<Coord>(a.fftfreq_cutoff), Coord(0));
+    op.cutoff_sqd *= op.cutoff_sqd;
+    op.fftfreq_sinc = std::max(static_cast<Coord>(a.fftfreq_sinc), 1 / op.f_target[0]);
+    op.fftfreq_blackman = std::max(static_cast<Coord>(a.fftfreq_blackman), 1 / op.f_target[0]);
+    op.blackman_size = blackman_window_size<Coord>(op.fftfreq_blackman, op.f_target[0]);
+    op.volume_centered = remap_src_centered(a.remap);
+    op.slices_centered = remap_dst_centered(a.remap);
+    op.interp = a.interp;
+    const i64 B = a.output_shape[0], H = a.output_shape[2], Wr = a.output_shape[3] / 2 + 1;
+    if (op.blackman_size > 1) {
+        // ewise Zero of the outputs first (:1442-1446).
+        for (i64 b = 0; b < B; ++b)
+            for (i64 y = 0; y < H; ++y)
+                for (i64 x = 0; x < Wr; ++x) {
+                    op.slices(b, y, x) = zero<Out>();
+                    if (op.has_wslices) op.wslices(b, y, x) = W{};
+                }
+        iwise4(B, op.blackman_size, H, Wr, op, n_threads, force);
+    } else {
+        iwise3(B, H, Wr, op, n_threads, force);
+    }
+    return 0;
+}
+
+enum Op { RASTERIZE, INSERT, EXTRACT };
+
+template<class Coord>
+int dispatch_values(Op op, const noa_hip_project_args& a, int n_threads, bool force) {
+    const int in = a.input.dtype, out = a.output.dtype;
+    auto go = [&](auto in_tag, auto out_tag, auto w_tag) -> int {
+        using In = decltype(in_tag);
+        using Out = decltype(out_tag);
+        using W = decltype(w_tag);
+        switch (op) {
+            case RASTERIZE: return run_rasterize<Coord, In, Out, W>(a, n_threads, force);
+            case INSERT: return run_insert<Coord, In, Out, W>(a, n_threads, force);
+            case EXTRACT: return run_extract<Coord, In, Out, W>(a, n_threads, force);
+        }
+        return 1;
+    };
+    if (in == NOA_HIP_C32 && out == NOA_HIP_C32) return go(Cx<float>{}, Cx<float>{}, float{});
+    if (in == NOA_HIP_C32 && out == NOA_HIP_F32) return go(Cx<float>{}, float{}, float{});
+    if (in == NOA_HIP_F32 && out == NOA_HIP_F32) return go(float{}, float{}, float{});
+    if (in == NOA_HIP_C64 && out == NOA_HIP_C64) return go(Cx<double>{}, Cx<double>{}, double{});
+    if (in == NOA_HIP_C64 && out == NOA_HIP_F64) return go(Cx<double>{}, double{}, double{});
+    if (in == NOA_HIP_F64 && out == NOA_HIP_F64) return go(double{}, double{}, double{});
+    g_error = "oracle: unsupported value type combination";
+    return NOA_HIP_ERROR_UNSUPPORTED;
+}
+
+inline bool weight_ok(const noa_hip_view& v, int real_dtype) {
+    return (v.data == nullptr && !v.is_value) || v.dtype == real_dtype;
+}
+
+template<class Coord>
+int dispatch_checked(Op op, const noa_hip_project_args& a, int n_threads, bool force) {
+    const bool f64v = a.output.dtype == NOA_HIP_F64 || a.output.dtype == NOA_HIP_C64;
+    const int wt = f64v ? NOA_HIP_F64 : NOA_HIP_F32;
+    if (!weight_ok(a.input_weight, wt) || !weight_ok(a.output_weight, wt)) {
+        g_error = "oracle: weights must be real with the precision of the values";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    }
+    return dispatch_values<Coord>(op, a, n_threads, force);
+    return NOA_HIP_ERROR_UNSUPPORTED;
+}
+
+int dispatch(Op op, const noa_hip_project_args* a, int n_threads, bool force) {
+    g_error.clear();
+    if (!a) { g_error = "oracle: null args"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    const int base = a->interp >= 100 ? a->interp - 100 : a->interp;
+    if (op != RASTERIZE && base != NOA_HIP_INTERP_LINEAR && base != NOA_HIP_INTERP_NEAREST) {
+        g_error = "oracle: only NEAREST and LINEAR are restated";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    }
+    const int k = a->rotation.kind;
+    const bool f64 = k == NOA_HIP_MAT33_F64 || k == NOA_HIP_QUAT_F64;
+    if (k < NOA_HIP_MAT33_F32 || k > NOA_HIP_QUAT_F64) { g_error = "oracle: invalid rotation kind"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    return f64 ? dispatch_checked<double>(op, *a, n_threads, force) : dispatch_checked<float>(op, *a, n_threads, force);
+}
+
+} // namespace oracle
+
+extern "C" {
+
+const char* oracle_last_error(void) { return oracle::g_error.c_str(); }
+
+/* n_threads: thread limit (noa's Session thread limit); force != 0 uses exactly n_threads (ignores the 2^20
+ * per-thread heuristic of cpu/Iwise.hpp:104-116). */
+int oracle_rasterize_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
+    return oracle::dispatch(oracle::RASTERIZE, a, n_threads, force != 0);
+}
+int oracle_insert_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
+    return oracle::dispatch(oracle::INSERT, a, n_threads, force != 0);
+}
+int oracle_extract_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
+    return oracle::dispatch(oracle::EXTRACT, a, n_threads, force != 0);
+}
+
+/* Pinned helpers, exposed for the oracle's own tests. */
+long long oracle_index2frequency(int centered, long long index, long long size) {
+    return oracle::index2frequency(centered != 0, index, size);
+}
+long long oracle_frequency2index(int centered, long long frequency, long long size) {
+    return oracle::frequency2index(centered != 0, frequency, size);
+}
+double oracle_windowed_sinc_f64(double f, double fs, double fb) { return oracle::windowed_sinc<double>(f, fs, fb); }
+float oracle_windowed_sinc_f32(float f, float fs, float fb) { return oracle::windowed_sinc<float>(f, fs, fb); }
+long long oracle_blackman_window_size(double fb, double n) { return oracle::blackman_window_size<double>(fb, n); }
+int oracle_iwise_threads(long long n, int n_threads) { return oracle::iwise_threads(n, n_threads); }
+
+/* interpolate_spectrum on host arrays (c32 2D/3D, LINEAR or NEAREST), for the interpolation pins. */
+void oracle_interpolate_spectrum_3d_c32(const float* data, const long long strides[4], long long nz, long long ny,
+                                        long long nx, int centered, int interp, const float* freqs, long long n,
+                                        float* out) {
+    oracle::Reader3<oracle::Cx<float>> r;
+    r.p = reinterpret_cast<const oracle::Cx<float>*>(data);
+    r.s0 = 0; r.s1 = strides[1]; r.s2 = strides[2]; r.s3 = strides[3];
+    for (long long i = 0; i < n; ++i) {
+        auto v = oracle::interpolate_spectrum_3d<oracle::Cx<float>, float>(
+            r, freqs[3 * i], freqs[3 * i + 1], freqs[3 * i + 2], nz, ny, nx, centered != 0, interp);
+        out[2 * i] = v.re; out[2 * i + 1] = v.im;
+    }
+}
+void oracle_interpolate_spectrum_2d_c32(const float* data, const long long strides[3], long long batch,
+                                        long long ny, long long nx, int centered, int interp, const float* freqs,
+                                        long long n, float* out) {
+    oracle::Reader3<oracle::Cx<float>> r;
+    r.p = reinterpret_cast<const oracle::Cx<float>*>(data);
+    r.s0 = strides[0]; r.s1 = strides[1]; r.s2 = strides[2];
+    for (long long i = 0; i < n; ++i) {
+        auto v = oracle::interpolate_spectrum_2d<oracle::Cx<float>, float>(
+            r, batch, freqs[2 * i], freqs[2 * i + 1], ny, nx, centered != 0, interp);
+        out[2 * i] = v.re; out[2 * i + 1] = v.im;
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,256 @@
+"""TEST INFRASTRUCTURE -- Python front-end of the CPU oracle (oracle/_build/liboracle.so).
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg. The functions take numpy
+arrays (host memory) with the same argument order as noa_amd.xform, build the same C descriptor
+(include/noa_hip.h) and run noa's CPU algorithm restated in oracle/noa_oracle.cpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+LIB_PATH = ROOT / "_build" / "liboracle.so"
+
+# ctypes layout of include/noa_hip.h, restated here so that the oracle does not import the product.
+from ctypes import c_double, c_int32, c_int64, c_void_p  # noqa: E402
+
+
+class View(ctypes.Structure):
+    _fields_ = [("data", c_void_p), ("shape", c_int64 * 4), ("strides", c_int64 * 4), ("dtype", c_int32),
+                ("is_value", c_int32), ("value", c_double * 2)]
+
+
+class Transform(ctypes.Structure):
+    _fields_ = [("data", c_void_p), ("kind", c_int32), ("batched", c_int32), ("count", c_int64)]
+
+
+class ProjectArgs(ctypes.Structure):
+    _fields_ = [("input", View), ("input_weight", View), ("output", View), ("output_weight", View),
+                ("input_shape", c_int64 * 4), ("output_shape", c_int64 * 4), ("target_shape", c_int64 * 4),
+                ("remap", c_int32), ("interp", c_int32), ("rotation", Transform), ("scaling", Transform),
+                ("fftfreq_cutoff", c_double), ("fftfreq_sinc", c_double), ("fftfreq_blackman", c_double),
+                ("ews_radius", c_double * 2)]
+
+
+F32, F64, C32, C64 = 1, 2, 3, 4
+MAT33_F32, MAT33_F64, QUAT_F32, QUAT_F64, MAT22_F32, MAT22_F64 = 1, 2, 3, 4, 5, 6
+LAYOUTS = {"h2h": 0, "h": 0, "hc2h": 4, "h2hc": 8, "hc2hc": 12, "hc": 12}
+
+_NP_DTYPES = {np.dtype(np.float32): F32, np.dtype(np.float64): F64, np.dtype(np.complex64): C32,
+              np.dtype(np.complex128): C64}
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(ROOT)], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.POINTER(ProjectArgs)
+        for n in ("oracle_rasterize_central_slices_3d", "oracle_insert_central_slices_3d",
+                  "oracle_extract_central_slices_3d"):
+            getattr(L, n).argtypes = [P, ctypes.c_int, ctypes.c_int]
+            getattr(L, n).restype = ctypes.c_int
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_index2frequency.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong]
+        L.oracle_index2frequency.restype = ctypes.c_longlong
+        L.oracle_frequency2index.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong]
+        L.oracle_frequency2index.restype = ctypes.c_longlong
+        L.oracle_windowed_sinc_f64.argtypes = [c_double] * 3
+        L.oracle_windowed_sinc_f64.restype = c_double
+        L.oracle_windowed_sinc_f32.argtypes = [ctypes.c_float] * 3
+        L.oracle_windowed_sinc_f32.restype = ctypes.c_float
+        L.oracle_blackman_window_size.argtypes = [c_double, c_double]
+        L.oracle_blackman_window_size.restype = ctypes.c_longlong
+        L.oracle_iwise_threads.argtypes = [ctypes.c_longlong, ctypes.c_int]
+        L.oracle_iwise_threads.restype = ctypes.c_int
+        L.oracle_interpolate_spectrum_3d_c32.argtypes = [c_void_p, c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                                         ctypes.c_longlong, ctypes.c_int, ctypes.c_int, c_void_p,
+                                                         ctypes.c_longlong, c_void_p]
+        L.oracle_interpolate_spectrum_2d_c32.argtypes = [c_void_p, c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                                         ctypes.c_longlong, ctypes.c_int, ctypes.c_int, c_void_p,
+                                                         ctypes.c_longlong, c_void_p]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _pad4(a: np.ndarray):
+    shape = list(a.shape)
+    strides = [s // a.itemsize for s in a.strides]
+    while len(shape) < 4:
+        strides.insert(0, shape[0] * strides[0] if shape else 1)
+        shape.insert(0, 1)
+    return shape, strides
+
+
+def _view(obj, keep, real_t, cx_t):
+    v = View()
+    if obj is None:
+        return v
+    if isinstance(obj, (int, float, complex, np.number)):
+        c = complex(obj)
+        v.is_value = 1
+        v.dtype = cx_t if isinstance(obj, (complex, np.complexfloating)) else real_t
+        v.value[0], v.value[1] = c.real, c.imag
+        for i in range(4):
+            v.shape[i] = 1
+        return v
+    a = np.asarray(obj)
+    shape, strides = _pad4(a)
+    v.data = a.ctypes.data
+    v.dtype = _NP_DTYPES[a.dtype]
+    for i in range(4):
+        v.shape[i] = shape[i]
+        v.strides[i] = strides[i]
+    keep.append(a)
+    return v
+
+
+def _transform(obj, keep, is_rotation, quaternion=False):
+    t = Transform()
+    if obj is None:
+        return t
+    a = np.asarray(obj)
+    f64 = a.dtype == np.float64
+    if quaternion:
+        elem, kind = (4,), (QUAT_F64 if f64 else QUAT_F32)
+    elif is_rotation:
+        elem, kind = (3, 3), (MAT33_F64 if f64 else MAT33_F32)
+    else:
+        elem, kind = (2, 2), (MAT22_F64 if f64 else MAT22_F32)
+    if a.shape == elem:
+        packed = np.ascontiguousarray(a.reshape(-1))
+        t.batched = 0
+        t.count = 1
+    else:
+        n = a.shape[0]
+        if kind in (MAT33_F32, MAT33_F64):
+            packed = np.zeros((n, 12 if kind == MAT33_F32 else 10), dtype=a.dtype)
+            packed[:, :9] = a.reshape(n, 9)
+        else:
+            packed = np.ascontiguousarray(a.reshape(n, -1))
+        t.batched = 1
+        t.count = n
+    keep.append(packed)
+    t.data = packed.ctypes.data
+    t.kind = kind
+    return t
+
+
+def _args(remap, inp, inp_w, in_shape, out, out_w, out_shape, rotation, scaling, cutoff, target_shape, ews,
+          sinc, blackman, interp, keep, quaternion):
+    a = ProjectArgs()
+    real_t, cx_t = (F64, C64) if out.dtype in (np.float64, np.complex128) else (F32, C32)
+    a.input = _view(inp, keep, real_t, cx_t)
+    a.input_weight = _view(inp_w, keep, real_t, real_t)
+    a.output = _view(out, keep, real_t, cx_t)
+    a.output_weight = _view(out_w, keep, real_t, real_t)
+    for i in range(4):
+        a.input_shape[i] = in_shape[i]
+        a.output_shape[i] = out_shape[i]
+        a.target_shape[i] = target_shape[i]
+    a.remap = LAYOUTS[remap.lower()] if isinstance(remap, str) else remap
+    a.interp = int(interp)
+    a.rotation = _transform(rotation, keep, True, quaternion)
+    a.scaling = _transform(scaling, keep, False)
+    a.fftfreq_cutoff = cutoff
+    a.fftfreq_sinc = sinc
+    a.fftfreq_blackman = blackman
+    a.ews_radius[0], a.ews_radius[1] = ews
+    return a
+
+
+def _run(name, a, n_threads, force):
+    status = getattr(lib(), name)(ctypes.byref(a), n_threads, 1 if force else 0)
+    if status != 0:
+        raise OracleError(lib().oracle_last_error().decode())
+
+
+def rasterize_central_slices_3d(remap, slice, slice_weight, slice_shape, volume, volume_weight, volume_shape,
+                                inv_scaling=None, fwd_rotation=None, *, fftfreq_cutoff=0.5,
+                                target_shape=(0, 0, 0, 0), ews_radius=(0.0, 0.0), quaternion=False,
+                                n_threads=1, force_threads=False):
+    keep = []
+    a = _args(remap, slice, slice_weight, slice_shape, volume, volume_weight, volume_shape, fwd_rotation,
+              inv_scaling, fftfreq_cutoff, target_shape, ews_radius, -1.0, -1.0, 1, keep, quaternion)
+    _run("oracle_rasterize_central_slices_3d", a, n_threads, force_threads)
+
+
+def insert_central_slices_3d(remap, slice, slice_weight, slice_shape, volume, volume_weight, volume_shape,
+                             fwd_scaling=None, inv_rotation=None, *, interp=1, fftfreq_sinc=-1.0,
+                             fftfreq_blackman=-1.0, fftfreq_cutoff=0.5, target_shape=(0, 0, 0, 0),
+                             ews_radius=(0.0, 0.0), quaternion=False, n_threads=1, force_threads=False):
+    keep = []
+    a = _args(remap, slice, slice_weight, slice_shape, volume, volume_weight, volume_shape, inv_rotation,
+              fwd_scaling, fftfreq_cutoff, target_shape, ews_radius, fftfreq_sinc, fftfreq_blackman, interp, keep,
+              quaternion)
+    _run("oracle_insert_central_slices_3d", a, n_threads, force_threads)
+
+
+def extract_central_slices_3d(remap, volume, volume_weight, volume_shape, slice, slice_weight, slice_shape,
+                              inv_scaling=None, fwd_rotation=None, *, interp=1, fftfreq_sinc=-1.0,
+                              fftfreq_blackman=-1.0, fftfreq_cutoff=0.5, target_shape=(0, 0, 0, 0),
+                              ews_radius=(0.0, 0.0), quaternion=False, n_threads=1, force_threads=False):
+    keep = []
+    a = _args(remap, volume, volume_weight, volume_shape, slice, slice_weight, slice_shape, fwd_rotation,
+              inv_scaling, fftfreq_cutoff, target_shape, ews_radius, fftfreq_sinc, fftfreq_blackman, interp, keep,
+              quaternion)
+    _run("oracle_extract_central_slices_3d", a, n_threads, force_threads)
+
+
+def index2frequency(centered, index, size):
+    return lib().oracle_index2frequency(int(centered), index, size)
+
+
+def frequency2index(centered, frequency, size):
+    return lib().oracle_frequency2index(int(centered), frequency, size)
+
+
+def windowed_sinc(f, fs, fb, f32=False):
+    return (lib().oracle_windowed_sinc_f32 if f32 else lib().oracle_windowed_sinc_f64)(f, fs, fb)
+
+
+def blackman_window_size(fb, n):
+    return lib().oracle_blackman_window_size(fb, n)
+
+
+def iwise_threads(n, n_threads):
+    return lib().oracle_iwise_threads(n, n_threads)
+
+
+def interpolate_spectrum_3d(volume: np.ndarray, freqs: np.ndarray, shape, centered: bool, interp=1) -> np.ndarray:
+    """volume: (D, H, W//2+1) complex64; freqs: (N, 3) float32 centred frequencies in samples."""
+    v = np.ascontiguousarray(volume.reshape((1,) + volume.shape[-3:]), dtype=np.complex64)
+    st = np.array([s // 8 for s in v.strides], dtype=np.int64)
+    f = np.ascontiguousarray(freqs, dtype=np.float32)
+    out = np.zeros(len(f), dtype=np.complex64)
+    lib().oracle_interpolate_spectrum_3d_c32(v.ctypes.data, st.ctypes.data, shape[0], shape[1], shape[2],
+                                             int(centered), interp, f.ctypes.data, len(f), out.ctypes.data)
+    return out
+
+
+def interpolate_spectrum_2d(slices: np.ndarray, batch: int, freqs: np.ndarray, shape, centered: bool,
+                            interp=1) -> np.ndarray:
+    """slices: (B, H, W//2+1) complex64; freqs: (N, 2) float32 centred frequencies in samples."""
+    s = np.ascontiguousarray(slices, dtype=np.complex64)
+    st = np.array([x // 8 for x in s.strides], dtype=np.int64)
+    f = np.ascontiguousarray(freqs, dtype=np.float32)
+    out = np.zeros(len(f), dtype=np.complex64)
+    lib().oracle_interpolate_spectrum_2d_c32(s.ctypes.data, st.ctypes.data, batch, shape[0], shape[1],
+                                             int(centered), interp, f.ctypes.data, len(f), out.ctypes.data)
+    return out

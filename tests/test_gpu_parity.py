@@ -1,0 +1,216 @@
+"""Parity of the HIP path (noa_amd.xform -> C ABI -> gfx950 kernels) with the CPU oracle on the same inputs.
+
+Bars: extraction (LINEAR/NEAREST, no w-window) is bit-exact with the oracle (no transcendental functions,
+no fused multiply-add on either side). Interpolated insertion sums in the reference's slice order and differs
+only through sin/cos ulps: rel-L2 <= 1e-6. Rasterisation accumulates with float atomics in an unspecified
+order: rel-L2 <= 1e-6 against the oracle's serial order (bounded and stated in DESIGN.md).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle as O
+from helpers import h2hc, mat33_blocks, random_rotations, random_spectrum, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def nx():
+    import noa_amd.xform as m
+    return m
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def cpu(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+# ------------------------------------------------------------------------------------------------ extraction
+EXTRACT_CASES = [
+    # remap, in dtype, out dtype, interp
+    ("h2h", np.complex64, np.complex64, 1),
+    ("hc2hc", np.complex64, np.complex64, 1),
+    ("hc2h", np.complex64, np.complex64, 1),
+    ("h2hc", np.complex64, np.complex64, 1),
+    ("hc2hc", np.complex64, np.float32, 1),
+    ("h2h", np.float32, np.float32, 1),
+    ("hc2hc", np.complex64, np.complex64, 0),
+    ("h2h", np.complex128, np.complex128, 1),
+]
+
+
+@pytest.mark.parametrize("remap,tin,tout,interp", EXTRACT_CASES)
+def test_extract_bit_exact(remap, tin, tout, interp):
+    print(f"case extract {remap} {np.dtype(tin)}->{np.dtype(tout)} interp={interp}", flush=True)
+    rng = np.random.default_rng(100)
+    n, b = 48, 12
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng, tin)
+    rdt = np.float64 if tin == np.complex128 else np.float32
+    rot = random_rotations(b, rng, dtype=rdt)
+    want = np.zeros((b, 1, n, n // 2 + 1), dtype=tout)
+    O.extract_central_slices_3d(remap, vol, None, (1, n, n, n), want, None, (b, 1, n, n), None, rot, interp=interp)
+    out = torch.zeros((b, 1, n, n // 2 + 1), dtype=getattr(torch, np.dtype(tout).name), device=DEV)
+    nx().extract_central_slices_3d(remap, to_dev(vol), None, (1, n, n, n), out, None, (b, 1, n, n), None, to_dev(rot),
+                                   nx().ExtractCentralSlicesOptions(interp=nx().Interp(interp)))
+    np.testing.assert_array_equal(cpu(out), want)
+
+
+def test_extract_weights_scaling_ews_target_quaternions():
+    print("case extract weights/scaling/ews/target/quaternion", flush=True)
+    rng = np.random.default_rng(101)
+    n, b = 40, 9
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)
+    wvol = rng.uniform(0, 2, (1, n, n, n // 2 + 1)).astype(np.float32)
+    rot = random_rotations(b, rng)
+    scale = np.stack([np.diag(rng.uniform(0.9, 1.1, 2)).astype(np.float32) for _ in range(b)])
+    target = (1, 44, 44, 44)
+    q = nx().quaternion_from_matrix(rot.astype(np.float64)).astype(np.float32)
+    for rotation, oracle_rot, quat in ((to_dev(rot), rot, False), (nx().Quaternions(to_dev(q)), q, True)):
+        want = np.zeros((b, 1, n, n // 2 + 1), np.complex64)
+        wwant = np.zeros((b, 1, n, n // 2 + 1), np.float32)
+        O.extract_central_slices_3d("hc2h", vol, wvol, (1, n, n, n), want, wwant, (b, 1, n, n), scale, oracle_rot,
+                                    fftfreq_cutoff=0.45, target_shape=target, ews_radius=(60.0, 70.0), quaternion=quat)
+        out = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+        wout = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+        opts = nx().ExtractCentralSlicesOptions(fftfreq_cutoff=0.45, target_shape=target, ews_radius=(60.0, 70.0))
+        nx().extract_central_slices_3d("hc2h", to_dev(vol), to_dev(wvol), (1, n, n, n), out, wout, (b, 1, n, n),
+                                       to_dev(scale), rotation, opts)
+        np.testing.assert_array_equal(cpu(out), want)
+        np.testing.assert_array_equal(cpu(wout), wwant)
+
+
+def test_extract_w_windowed_sinc():
+    print("case extract w-windowed sinc", flush=True)
+    rng = np.random.default_rng(102)
+    n, b = 32, 6
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)
+    rot = random_rotations(b, rng)
+    want = np.zeros((b, 1, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((b, 1, n, n // 2 + 1), np.float32)
+    O.extract_central_slices_3d("hc2hc", vol, None, (1, n, n, n), want, wwant, (b, 1, n, n), None, rot,
+                                fftfreq_sinc=0.02, fftfreq_blackman=0.1)
+    out = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    wout = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+    opts = nx().ExtractCentralSlicesOptions(w_windowed_sinc=nx().WindowedSinc(0.02, 0.1))
+    nx().extract_central_slices_3d("hc2hc", to_dev(vol), None, (1, n, n, n), out, wout, (b, 1, n, n), None,
+                                   to_dev(rot), opts)
+    assert rel_l2(cpu(out), want) < 1e-6
+    assert rel_l2(cpu(wout), wwant) < 1e-6
+
+
+def test_extract_odd_sizes_strided_output_single_rotation():
+    print("case extract odd/strided/single", flush=True)
+    rng = np.random.default_rng(103)
+    n, h = 33, 35
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)
+    rot = random_rotations(1, rng)[0]
+    want = np.zeros((4, 1, h, h // 2 + 1), np.complex64)
+    O.extract_central_slices_3d("hc2hc", vol, None, (1, n, n, n), want, None, (4, 1, h, h), None, rot,
+                                fftfreq_cutoff=0.7)
+    big = torch.zeros((4, 1, h, h // 2 + 1 + 5), dtype=torch.complex64, device=DEV)
+    out = big[..., : h // 2 + 1]
+    nx().extract_central_slices_3d("hc2hc", to_dev(vol), None, (1, n, n, n), out, None, (4, 1, h, h), None, rot,
+                                   nx().ExtractCentralSlicesOptions(fftfreq_cutoff=0.7))
+    np.testing.assert_array_equal(cpu(out), want)
+
+
+# ------------------------------------------------------------------------------------------------ rasterisation
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h", "h2hc"])
+def test_rasterize_parity(remap):
+    print(f"case rasterize {remap}", flush=True)
+    rng = np.random.default_rng(110)
+    n, b = 48, 24
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    rot = random_rotations(b, rng)
+    vol = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wvol = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.rasterize_central_slices_3d(remap, s, w, (b, 1, n, n), vol, wvol, (1, n, n, n), None, rot)
+    gv = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    gw = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+    nx().rasterize_central_slices_3d(remap, to_dev(s), to_dev(w), (b, 1, n, n), gv, gw, (1, n, n, n), None,
+                                     to_dev(rot))
+    assert rel_l2(cpu(gv), vol) < 1e-6
+    assert rel_l2(cpu(gw), wvol) < 1e-6
+
+
+def test_rasterize_real_value_input_and_f64():
+    print("case rasterize value/f64", flush=True)
+    rng = np.random.default_rng(111)
+    n, b = 32, 10
+    rot = random_rotations(b, rng, dtype=np.float64)
+    vol = np.zeros((1, n, n, n // 2 + 1), np.float64)
+    O.rasterize_central_slices_3d("hc2hc", 2.5, None, (b, 1, n, n), vol, None, (1, n, n, n), None, rot)
+    gv = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float64, device=DEV)
+    nx().rasterize_central_slices_3d("hc2hc", 2.5, None, (b, 1, n, n), gv, None, (1, n, n, n), None, to_dev(rot))
+    assert rel_l2(cpu(gv), vol) < 1e-12
+
+
+# ------------------------------------------------------------------------------------------------ insertion
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h", "h2hc"])
+def test_insert_parity(remap):
+    print(f"case insert {remap}", flush=True)
+    rng = np.random.default_rng(120)
+    n, b = 40, 30
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)     # insertion accumulates: start from non-zero
+    wvol = rng.uniform(0, 1, (1, n, n, n // 2 + 1)).astype(np.float32)
+    gv, gw = to_dev(vol), to_dev(wvol)
+    O.insert_central_slices_3d(remap, s, w, (b, 1, n, n), vol, wvol, (1, n, n, n), None, inv,
+                               fftfreq_sinc=0.03, fftfreq_blackman=0.06)
+    opts = nx().InsertCentralSlicesOptions(windowed_sinc=nx().WindowedSinc(0.03, 0.06))
+    nx().insert_central_slices_3d(remap, to_dev(s), to_dev(w), (b, 1, n, n), gv, gw, (1, n, n, n), None, to_dev(inv),
+                                  opts)
+    assert rel_l2(cpu(gv), vol) < 1e-6
+    assert rel_l2(cpu(gw), wvol) < 1e-6
+
+
+def test_insert_defaults_value_input_single_slice_broadcast():
+    print("case insert value/broadcast", flush=True)
+    rng = np.random.default_rng(121)
+    n = 32
+    inv = np.ascontiguousarray(np.transpose(random_rotations(7, rng), (0, 2, 1)))
+    s = random_spectrum((1, 1, n, n // 2 + 1), rng)   # one slice, broadcast over 7 rotations
+    vol = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wvol = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("hc2hc", s, 1.0, (7, 1, n, n), vol, wvol, (1, n, n, n), None, inv)
+    # noa broadcasts a B=1 input over the batch only through the interpolator (shape must still match).
+    s7 = np.broadcast_to(s, (7, 1, n, n // 2 + 1))
+    gv = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    gw = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+    nx().insert_central_slices_3d("hc2hc", to_dev(s7), 1.0, (7, 1, n, n), gv, gw, (1, n, n, n), None, to_dev(inv))
+    vol7 = np.zeros_like(vol)
+    wvol7 = np.zeros_like(wvol)
+    O.insert_central_slices_3d("hc2hc", np.ascontiguousarray(s7), 1.0, (7, 1, n, n), vol7, wvol7, (1, n, n, n), None, inv)
+    assert rel_l2(cpu(gv), vol7) < 1e-6
+    assert rel_l2(cpu(gw), wvol7) < 1e-6
+    assert rel_l2(vol7, vol) < 1e-6
+
+
+# ------------------------------------------------------------------------------------------------ errors
+def test_invalid_arguments_raise_before_launch():
+    n = 16
+    vol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    out = torch.zeros((2, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    rot = torch.eye(3, device=DEV).repeat(2, 1, 1)
+    with pytest.raises(nx().NoaError, match="shape"):
+        nx().extract_central_slices_3d("h2h", vol, None, (1, n, n, n), out, None, (3, 1, n, n), None, rot)
+    with pytest.raises(nx().NoaError, match="n_slices"):
+        nx().extract_central_slices_3d("h2h", vol, None, (1, n, n, n), out, None, (2, 1, n, n), None,
+                                       torch.eye(3, device=DEV).repeat(3, 1, 1))
+    with pytest.raises(nx().NoaError, match="HX2HX"):
+        nx().extract_central_slices_3d("f2f", vol, None, (1, n, n, n), out, None, (2, 1, n, n), None, rot)
+    with pytest.raises(nx().NoaError, match="overlap"):
+        nx().rasterize_central_slices_3d("h2h", vol[:, :1, :, :].reshape(1, 1, n, n // 2 + 1).expand(1, 1, n, n // 2 + 1),
+                                         None, (1, 1, n, n), vol, None, (1, n, n, n), None, rot[:1])
+    with pytest.raises(nx().NoaError, match="same device"):
+        nx().extract_central_slices_3d("h2h", vol.cpu(), None, (1, n, n, n), out, None, (2, 1, n, n), None, rot)

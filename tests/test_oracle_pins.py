@@ -1,0 +1,352 @@
+"""Pins of the CPU oracle (oracle/noa_oracle.cpp) against what the mathematics and independent libraries fix.
+
+Each test is chosen so that a plausible mistake (dropped term, wrong sign or index, transposed rotation,
+wrong centring) fails it: closed forms, numpy/scipy routines, adjoint identities, the reference's own
+self-consistency invariants (tests/noa/xform/TestXformFourier*.cpp), and exactness of linear fields.
+"""
+import numpy as np
+import pytest
+import scipy.signal
+
+import oracle as O
+from helpers import allclose_abs_safe, freq_grid, h2hc, hc2h, random_rotations, random_spectrum, rel_l2
+from noa_amd.xform import euler2matrix, quaternion_from_matrix, rotate_y, rotate_z
+
+
+# ------------------------------------------------------------------------------------------------ index math
+@pytest.mark.parametrize("n", [5, 6, 9, 10, 64, 65])
+def test_index_frequency_vs_numpy_fftfreq(n):
+    for centered in (False, True):
+        f = freq_grid(n, centered)
+        for i in range(n):
+            assert O.index2frequency(centered, i, n) == f[i]
+            assert O.frequency2index(centered, int(f[i]), n) == i
+
+
+def test_index_frequency_reference_examples():
+    # src/noa/fft/core/Frequency.hpp:82-83 and :117-118
+    assert [O.index2frequency(True, i, 5) for i in range(5)] == [-2, -1, 0, 1, 2]
+    assert [O.index2frequency(False, i, 5) for i in range(5)] == [0, 1, 2, -2, -1]
+    assert [O.index2frequency(True, i, 6) for i in range(6)] == [-3, -2, -1, 0, 1, 2]
+    assert [O.index2frequency(False, i, 6) for i in range(6)] == [0, 1, 2, -3, -2, -1]
+
+
+def test_iwise_thread_heuristic():
+    # runtime/cpu/Iwise.hpp:104-116: serial up to 2^20 elements, then min(threads, n / 2^20).
+    assert O.iwise_threads(1 << 20, 8) == 1
+    assert O.iwise_threads((1 << 20) + 1, 8) == 1
+    assert O.iwise_threads(3 << 20, 8) == 3
+    assert O.iwise_threads(64 << 20, 8) == 8
+    assert O.iwise_threads(4096 * 256 * 129, 6) == 6
+
+
+# ------------------------------------------------------------------------------------------------ windowed sinc
+@pytest.mark.parametrize("mult", [1, 2, 5])
+def test_windowed_sinc_is_sinc_times_scipy_blackman(mult):
+    fs = 1 / 64
+    fb = mult * fs
+    m = 101
+    x = np.linspace(-1, 1, m)
+    want = np.sinc(x * fb / fs) * scipy.signal.windows.blackman(m, sym=True)
+    got = np.array([O.windowed_sinc(v * fb, fs, fb) for v in x])
+    np.testing.assert_allclose(got, want, atol=1e-12)
+    assert abs(O.windowed_sinc(0.0, fs, fb) - 1.0) <= 2e-16   # 0.42 + 0.5 + 0.08 in f64
+    w0 = (np.float32(0.42) + np.float32(0.5)) + np.float32(0.08)   # 0.99999994 in f32
+    assert O.windowed_sinc(0.0, fs, fb, f32=True) == w0
+    np.testing.assert_allclose(O.windowed_sinc(fb, fs, fb, f32=True), 0.0, atol=1e-6)
+
+
+def test_blackman_window_size_reference_examples():
+    # FourierProject.hpp:118-122 lists rfftfreq samples 0..0.5 by 0.05, i.e. a spectrum of size 20 (the comment
+    # says 10): 20 * 0.23 = 4.6 -> 5 -> 11 -> 9 after removing the two zero edges. Size 10 gives 2.3 -> 3 -> 5.
+    assert O.blackman_window_size(0.23, 20) == 9
+    assert O.blackman_window_size(0.23, 10) == 5
+    assert O.blackman_window_size(1 / 256, 256) == 1      # default 1/N -> one sample
+    assert O.blackman_window_size(np.float32(1 / 100), 100) == 1   # allclose(., 1) rounds instead of ceil
+    assert O.blackman_window_size(0.05, 100) == 9
+
+
+# ------------------------------------------------------------------------------------------------ interpolation
+def linear_field(d, h, w, centered, c, a, dtype=np.complex64):
+    """Hermitian field V(f) = c + i a.f on the stored half (x >= 0): V(-f) = conj(V(f)), linear in f."""
+    fz = freq_grid(d, centered)[:, None, None].astype(np.float64)
+    fy = freq_grid(h, centered)[None, :, None].astype(np.float64)
+    fx = np.arange(w // 2 + 1)[None, None, :].astype(np.float64)
+    return (c + 1j * (a[0] * fz + a[1] * fy + a[2] * fx)).astype(dtype)
+
+
+@pytest.mark.parametrize("centered", [False, True])
+def test_interp3_reproduces_linear_hermitian_field(centered):
+    rng = np.random.default_rng(1)
+    n = 24
+    c, a = 0.7, np.array([0.3, -1.1, 0.45])
+    vol = linear_field(n, n, n, centered, c, a)
+    f = rng.uniform(-(n / 2 - 2), n / 2 - 2, size=(500, 3)).astype(np.float32)
+    got = O.interpolate_spectrum_3d(vol, f, (n, n, n), centered)
+    want = c + 1j * (f.astype(np.float64) @ a)
+    assert rel_l2(got, want) < 2e-6
+
+
+@pytest.mark.parametrize("centered", [False, True])
+def test_interp3_integer_frequencies_are_copies(centered):
+    rng = np.random.default_rng(2)
+    n = 16
+    vol = random_spectrum((n, n, n // 2 + 1), rng)
+    fz = rng.integers(-n // 2, (n - 1) // 2 + 1, 200)
+    fy = rng.integers(-n // 2, (n - 1) // 2 + 1, 200)
+    fx = rng.integers(0, n // 2 + 1, 200)
+    got = O.interpolate_spectrum_3d(vol, np.stack([fz, fy, fx], -1).astype(np.float32), (n, n, n), centered)
+    iz = np.array([O.frequency2index(centered, int(v), n) for v in fz])
+    iy = np.array([O.frequency2index(centered, int(v), n) for v in fy])
+    np.testing.assert_array_equal(got, vol[iz, iy, fx])
+
+
+def test_interp3_hermitian_flip_and_out_of_bounds():
+    rng = np.random.default_rng(3)
+    n = 16
+    vol = random_spectrum((n, n, n // 2 + 1), rng)
+    f = rng.uniform(-5, 5, size=(100, 3)).astype(np.float32)
+    f[:, 2] = np.where(np.abs(f[:, 2]) < 0.1, 0.5, f[:, 2])
+    a = O.interpolate_spectrum_3d(vol, f, (n, n, n), False)
+    b = O.interpolate_spectrum_3d(vol, -f, (n, n, n), False)
+    np.testing.assert_array_equal(a, np.conj(b))
+    far = np.array([[n, 0, 0], [0, 0, n], [0, -n, 1]], dtype=np.float32)
+    np.testing.assert_array_equal(O.interpolate_spectrum_3d(vol, far, (n, n, n), False), 0)
+
+
+def test_interp_nearest_does_not_conjugate_after_flip():
+    # Interpolation.hpp:479-484: the NEAREST branch returns before the early-flip conjugation.
+    rng = np.random.default_rng(4)
+    n = 16
+    vol = random_spectrum((n, n, n // 2 + 1), rng)
+    f = np.array([[1, 2, -3]], dtype=np.float32)
+    got = O.interpolate_spectrum_3d(vol, f, (n, n, n), False, interp=0)
+    assert got[0] == vol[n - 1, n - 2, 3]
+
+
+# ------------------------------------------------------------------------------------------------ extraction
+def _extract(remap, vol, n, rot, out_dtype=np.complex64, cutoff=0.5, n_slices=None, **kw):
+    b = rot.shape[0] if rot.ndim == 3 else 1
+    out = np.zeros((b, 1, n, n // 2 + 1), dtype=out_dtype)
+    O.extract_central_slices_3d(remap, vol.reshape((1,) + vol.shape[-3:]), None, (1, n, n, n), out, None,
+                                (b, 1, n, n), None, rot, fftfreq_cutoff=cutoff, **kw)
+    return out
+
+
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc"])
+def test_extract_identity_pose_is_the_central_plane(remap):
+    rng = np.random.default_rng(5)
+    n = 32
+    centered = remap == "hc2hc"
+    vol = random_spectrum((n, n, n // 2 + 1), rng)
+    out = _extract(remap, vol, n, np.eye(3, dtype=np.float32)[None], cutoff=0.5)[0, 0]
+    z0 = O.frequency2index(centered, 0, n)
+    v = freq_grid(n, centered)[:, None]
+    u = np.arange(n // 2 + 1)[None, :]
+    inside = (v / n) ** 2 + (u / n) ** 2 <= 0.25
+    want = np.where(inside, vol[z0], 0)
+    np.testing.assert_array_equal(out, want)
+
+
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h", "h2hc"])
+def test_extract_linear_field_at_random_poses(remap):
+    rng = np.random.default_rng(6)
+    n = 32
+    src_c, dst_c = remap in ("hc2h", "hc2hc"), remap in ("h2hc", "hc2hc")
+    c, a = -0.4, np.array([0.8, 0.25, -0.6])
+    vol = linear_field(n, n, n, src_c, c, a)
+    rot = random_rotations(8, rng)
+    out = _extract(remap, vol, n, rot, cutoff=0.5)[:, 0]
+    v = freq_grid(n, dst_c).astype(np.float64)
+    u = np.arange(n // 2 + 1, dtype=np.float64)
+    p = np.stack([np.zeros((n, n // 2 + 1)), np.broadcast_to(v[:, None] / n, (n, n // 2 + 1)),
+                  np.broadcast_to(u[None, :] / n, (n, n // 2 + 1))], -1)
+    for b in range(rot.shape[0]):
+        f = p @ rot[b].astype(np.float64).T * n     # R . (0, v/H, u/W), in samples
+        want = c + 1j * (f @ a)
+        ok = (np.abs(f) <= n / 2 - 2).all(-1) & ((f / n) ** 2).sum(-1).__le__(0.25)
+        assert ok.sum() > 100
+        assert rel_l2(out[b][ok], want[ok]) < 5e-6
+        assert np.all(out[b][((f / n) ** 2).sum(-1) > 0.2501] == 0)
+
+
+def test_extract_remap_invariant_reference():
+    # TestXformFourierExtract.cpp:116-153: hc2hc == remap(h2hc, hc2h), exactly.
+    rng = np.random.default_rng(7)
+    n = 32
+    vol = random_spectrum((n, n, n // 2 + 1), rng)
+    rot = np.stack([rotate_y(np.deg2rad(2.0 * i), np.float32) for i in range(6)])
+    a = _extract("hc2hc", vol, n, rot, cutoff=0.45)
+    b = _extract("hc2h", vol, n, rot, cutoff=0.45)
+    np.testing.assert_array_equal(a, h2hc(b, 2))
+
+
+def test_extract_volume_centering_invariant():
+    rng = np.random.default_rng(8)
+    n = 32
+    vol_c = random_spectrum((n, n, n // 2 + 1), rng)
+    rot = random_rotations(5, rng)
+    a = _extract("hc2h", vol_c, n, rot)
+    b = _extract("h2h", np.ascontiguousarray(hc2h(vol_c, 3)), n, rot)
+    np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------------------------------------------------ rasterisation
+def _rasterize(remap, slices, n, rot, vol_dtype=np.complex64, weights=None, wout=False, cutoff=0.5, **kw):
+    vol = np.zeros((1, n, n, n // 2 + 1), dtype=vol_dtype)
+    wvol = np.zeros((1, n, n, n // 2 + 1), dtype=np.float64 if vol_dtype in (np.float64, np.complex128) else np.float32) if wout else None
+    b = slices.shape[0]
+    O.rasterize_central_slices_3d(remap, slices, weights, (b, 1, n, n), vol, wvol, (1, n, n, n), None, rot,
+                                  fftfreq_cutoff=cutoff, **kw)
+    return (vol, wvol) if wout else vol
+
+
+def test_rasterize_identity_pose_closed_form():
+    rng = np.random.default_rng(9)
+    n = 16
+    s = random_spectrum((1, 1, n, n // 2 + 1), rng)
+    vol = _rasterize("h2h", s, n, np.eye(3, dtype=np.float32)[None], cutoff=0.5)[0]
+    v = freq_grid(n, False)[:, None]
+    u = np.arange(n // 2 + 1)[None, :]
+    inside = (v / n) ** 2 + (u / n) ** 2 <= 0.25
+    plane = np.where(inside, s[0, 0], 0)
+    # Hermitian duplicate of the x=0 column: vol[0, -v, 0] += conj(s[v, 0]).
+    col = plane[:, 0].copy()
+    neg = np.array([O.frequency2index(False, -int(f), n) for f in freq_grid(n, False)])
+    want = plane.copy()
+    want[:, 0] = col + np.conj(col[neg])
+    np.testing.assert_allclose(vol[0], want, rtol=0, atol=1e-6)
+    assert np.all(vol[1:] == 0)
+
+
+def test_rasterize_is_the_adjoint_of_extraction_for_real_data():
+    # <rasterize(s), V> == <s, extract(V)> for real data when V vanishes on the x=0 plane (where rasterisation
+    # adds its Hermitian duplicates). Pins the rotation, index, weight and flip conventions of both operators.
+    rng = np.random.default_rng(10)
+    n = 24
+    rot = random_rotations(6, rng, dtype=np.float64)
+    s = rng.standard_normal((6, 1, n, n // 2 + 1))
+    V = rng.standard_normal((n, n, n // 2 + 1))
+    V[:, :, 0] = 0
+    for remap in ("h2h", "hc2hc"):
+        vol = _rasterize(remap, s, n, rot, vol_dtype=np.float64, cutoff=0.45)
+        ext = _extract(remap, V, n, rot, out_dtype=np.float64, cutoff=0.45)
+        lhs = float((vol[0] * V).sum())
+        rhs = float((s * ext).sum())
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1.0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rasterize_weights_equal_values_reference(dtype):
+    # TestXformFourierInsertRasterize.cpp:122-152
+    rng = np.random.default_rng(11)
+    n = 32
+    rot = np.stack([euler2matrix(np.deg2rad([0, 2.0 * i, 0]), "zyx", dtype=np.float64) for i in range(10)])
+    s = rng.uniform(-10, 10, (10, 1, n, n // 2 + 1)).astype(dtype)
+    vol, wvol = _rasterize("hc2hc", s, n, rot, vol_dtype=dtype, weights=s.copy(), wout=True, cutoff=0.45)
+    assert allclose_abs_safe(vol, wvol, 5e-5)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_rasterize_remap_invariants_reference(dtype):
+    # TestXformFourierInsertRasterize.cpp:72-120
+    n = 32
+    rot = np.stack([euler2matrix(np.deg2rad([0, 2.0 * i, 0]), "zyx", dtype=np.float32) for i in range(10)])
+    s = np.linspace(1, 10, 10 * (n // 2) * (n // 2 + 1)).reshape(10, 1, n // 2, n // 2 + 1)
+    s = (s + (1j * s[..., ::-1] if dtype == np.complex64 else 0)).astype(dtype)
+    sh = (10, 1, n // 2, n // 2)
+
+    def run(remap):
+        vol = np.zeros((1, n, n, n // 2 + 1), dtype=dtype)
+        O.rasterize_central_slices_3d(remap, s, None, sh, vol, None, (1, n, n, n), None, rot, fftfreq_cutoff=0.45)
+        return vol
+
+    assert allclose_abs_safe(run("hc2hc"), h2hc(run("hc2h"), 3), 5e-5)
+    assert allclose_abs_safe(run("h2hc"), h2hc(run("h2h"), 3), 5e-5)
+
+
+def test_rasterize_negates_flipped_complex_values():
+    # FourierProject.hpp:251-256 with :270-274: `value * conjugate` with conjugate=-1 negates (not conjugates).
+    n = 16
+    s = np.zeros((1, 1, n, n // 2 + 1), dtype=np.complex64)
+    s[0, 0, 3, 2] = 1 + 2j                   # v=3, u=2
+    r_pi = rotate_z(np.pi, np.float64).astype(np.float32)[None]   # (0, v, u) -> (0, -v, -u): flipped back
+    vol = _rasterize("h2h", s, n, r_pi)
+    got = vol[0, 0, 3, 2]
+    np.testing.assert_allclose(got, -(1 + 2j), atol=1e-5)
+    assert np.abs(vol).sum() - np.abs(got) < 1e-5
+
+
+# ------------------------------------------------------------------------------------------------ insertion
+def _insert(remap, slices, n, inv_rot, weights=None, wout=False, vol_dtype=np.complex64, **kw):
+    vol = np.zeros((1, n, n, n // 2 + 1), dtype=vol_dtype)
+    wvol = np.zeros((1, n, n, n // 2 + 1), dtype=np.float32) if wout else None
+    b = slices.shape[0]
+    O.insert_central_slices_3d(remap, slices, weights, (b, 1, n, n), vol, wvol, (1, n, n, n), None, inv_rot, **kw)
+    return (vol, wvol) if wout else vol
+
+
+def test_insert_identity_pose_closed_form():
+    rng = np.random.default_rng(12)
+    n = 16
+    s = random_spectrum((1, 1, n, n // 2 + 1), rng)
+    vol, wvol = _insert("h2h", s, n, np.eye(3, dtype=np.float32)[None], wout=True)
+    v = freq_grid(n, False)[:, None]
+    u = np.arange(n // 2 + 1)[None, :]
+    inside = (v / n) ** 2 + (u / n) ** 2 <= 0.25
+    w0 = (np.float32(0.42) + np.float32(0.5)) + np.float32(0.08)   # windowed_sinc(0) in f32
+    want = np.where(inside, s[0, 0], 0)
+    np.testing.assert_array_equal(vol[0, 0], (want.real * w0).astype(np.float32) + 1j * (want.imag * w0).astype(np.float32))
+    np.testing.assert_array_equal(wvol[0, 0], np.where(inside, w0, 0).astype(np.float32))
+    # |fz| = 1/n is the edge of the default window: sinc(pi) * blackman(1) ~ 0.
+    assert np.abs(vol[0, 1]).max() < 1e-6 * np.abs(s).max()
+    assert np.abs(vol[0, -1]).max() < 1e-6 * np.abs(s).max()
+    assert np.all(vol[0, 2:-1] == 0)
+
+
+def test_insert_linear_field_at_random_poses():
+    rng = np.random.default_rng(13)
+    n = 32
+    c, a = 0.5, np.array([0.9, -0.3])
+    v = freq_grid(n, False)[:, None].astype(np.float64)
+    u = np.arange(n // 2 + 1)[None, :].astype(np.float64)
+    s = (c + 1j * (a[0] * v + a[1] * u)).astype(np.complex64)[None, None]
+    fs = fb = 3.0 / n
+    for k in range(3):
+        fwd = random_rotations(1, rng)[0]
+        inv = np.ascontiguousarray(fwd.T)[None]
+        vol = _insert("h2h", s, n, inv, fftfreq_sinc=fs, fftfreq_blackman=fb)[0]
+        fz = freq_grid(n, False)[:, None, None] / n
+        fy = freq_grid(n, False)[None, :, None] / n
+        fx = np.arange(n // 2 + 1)[None, None, :] / n
+        f = np.stack(np.broadcast_arrays(fz, fy, fx), -1)
+        g = f @ inv[0].astype(np.float64).T                 # (fz', fy', fx') = R_inv . f
+        x = g[..., 0]
+        win = np.sinc(x / fs) * (0.42 + 0.5 * np.cos(np.pi * x / fb) + 0.08 * np.cos(2 * np.pi * x / fb))
+        want = win * (c + 1j * (a[0] * g[..., 1] * n + a[1] * g[..., 2] * n))
+        ok = (np.abs(x) <= fb * 0.999) & (np.abs(g[..., 1:] * n) <= n / 2 - 2).all(-1) & ((f ** 2).sum(-1) <= 0.25)
+        out = (np.abs(x) > fb * 1.001)
+        assert ok.sum() > 200
+        assert rel_l2(vol[ok], want[ok]) < 2e-5
+        assert np.all(vol[out] == 0)
+
+
+def test_insert_is_linear_in_the_slices():
+    rng = np.random.default_rng(14)
+    n = 24
+    s = random_spectrum((6, 1, n, n // 2 + 1), rng)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(6, rng), (0, 2, 1)))
+    full = _insert("hc2hc", s, n, inv)
+    part = _insert("hc2hc", s[:3], n, inv[:3]) + _insert("hc2hc", s[3:], n, inv[3:])
+    assert rel_l2(full, part) < 1e-6
+
+
+def test_insert_matrix_equals_quaternion():
+    rng = np.random.default_rng(15)
+    n = 24
+    s = random_spectrum((5, 1, n, n // 2 + 1), rng)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(5, rng, dtype=np.float64), (0, 2, 1)))
+    a = _insert("h2h", s, n, inv.astype(np.float32))
+    q = quaternion_from_matrix(inv).astype(np.float32)
+    b = _insert("h2h", s, n, q, quaternion=True)
+    assert rel_l2(a, b) < 1e-5
