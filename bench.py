@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Throughput of the central-slice hot path on one MI355X: 256^3 extraction + insertion, in slices/s.
+
+One step = extract 4096 slices (256x129 complex64, "hc") from a 256^3 volume, then insert 10,000 CTF-weighted
+slices (with CTF^2 weights) into a 256^3 volume + weights (FP32, LINEAR, windowed sinc of 1 voxel). Inputs
+are synthetic (seeded) and already resident in HBM when the timed region starts.
+
+Prints ONE JSON line (stdout): the metric, per-kernel achieved bandwidth against the MI355X HBM peak, and the
+CPU baseline (the restated noa CPU path, timed on a bounded sample on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_extract(b, n):
+    """Output slices + volume + one Mat33 payload per slice (each read or written once)."""
+    return b * n * (n // 2 + 1) * 8 + n * n * (n // 2 + 1) * 8 + b * 36
+
+
+def algorithmic_bytes_insert(b, n):
+    """c32 slices + f32 weights read once, c32 volume and f32 weights read and written once, Mat33 per slice."""
+    px = n * (n // 2 + 1)
+    vox = n * n * (n // 2 + 1)
+    return b * px * (8 + 4) + 2 * vox * 8 + 2 * vox * 4 + b * 36
+
+
+def cpu_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))   # at most the CPU share of one GPU box
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ext, ins, n_ext_sample=64, n_ins_sample=48):
+    """The oracle (noa's CPU algorithm, OpenMP) on a bounded sample of the same workload.
+
+    Threads follow noa's heuristic min(cores, n_elements / 2^20) evaluated on the FULL workload
+    (cpu/Iwise.hpp:104-116): all cores for the 4096-slice extraction, 8 for the 256^3 insertion."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as O
+
+    cores = cpu_cores()
+    n = ext.volume.shape[1]
+    vol = ext.volume.cpu().numpy()
+    rot = ext.poses[:n_ext_sample]
+    out = np.zeros((n_ext_sample, 1, n, n // 2 + 1), np.complex64)
+    t_ext = min(cores, (4096 * n * (n // 2 + 1)) >> 20)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.extract_central_slices_3d("hc2hc", vol, None, (1, n, n, n), out, None, (n_ext_sample, 1, n, n), None, rot,
+                                    n_threads=t_ext, force_threads=True)
+        times.append(time.perf_counter() - t0)
+    ext_per_slice = statistics.median(times) / n_ext_sample
+
+    s = ins.slices[:n_ins_sample].cpu().numpy()
+    w = ins.weights[:n_ins_sample].cpu().numpy()
+    inv = np.ascontiguousarray(np.transpose(ins.poses[:n_ins_sample], (0, 2, 1)))
+    v = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wv = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    t_ins = min(cores, (n * n * (n // 2 + 1)) >> 20)
+    t0 = time.perf_counter()
+    O.insert_central_slices_3d("h2h", s, w, (n_ins_sample, 1, n, n), v, wv, (1, n, n, n), None, inv,
+                               n_threads=t_ins, force_threads=True)
+    ins_per_slice = (time.perf_counter() - t0) / n_ins_sample
+
+    n_e, n_i = ext.slice_shape[0], ins.slice_shape[0]
+    step_s = ext_per_slice * n_e + ins_per_slice * n_i
+    return {
+        "value": (n_e + n_i) / step_s,
+        "unit": "slices/s",
+        "cores": max(t_ext, t_ins),
+        "kind": "port",
+        "sample": (f"oracle (noa CPU algorithm restated, OpenMP): extraction of {n_ext_sample} of the {n_e} poses "
+                   f"with {t_ext} threads ({ext_per_slice * 1e3:.3f} ms/slice), insertion of {n_ins_sample} of the "
+                   f"{n_i} slices into 256^3 with {t_ins} threads ({ins_per_slice * 1e3:.1f} ms/slice), "
+                   f"extrapolated linearly to one step; CPU: {cpu_model()}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--n-insert", type=int, default=10_000)
+    ap.add_argument("--n-extract", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import noa_amd.xform as nx
+    from noa_amd import workloads
+
+    dev = torch.device("cuda:0")
+    log("generating inputs ...")
+    ext = workloads.extract_256(dev, b=args.n_extract)
+    ins = workloads.insert_256(dev, b=args.n_insert)
+    n = 256
+    out = torch.empty((args.n_extract, 1, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    vol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    wvol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in ("extract", "insert")}
+    durations = {"extract": [], "insert": []}
+
+    def step(record: bool):
+        if record:
+            ev["extract"][0].record()
+        nx.extract_central_slices_3d("hc2hc", ext.volume, None, ext.volume_shape, out, None, ext.slice_shape, None,
+                                     ext.rotations)
+        if record:
+            ev["extract"][1].record()
+            ev["insert"][0].record()
+        nx.insert_central_slices_3d("h2h", ins.slices, ins.weights, ins.slice_shape, vol, wvol, ins.volume_shape,
+                                    None, ins.inv_rotations)
+        if record:
+            ev["insert"][1].record()
+
+    log("warmup ...")
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+
+    # Per-kernel durations (HIP events on the stream the kernels are launched on).
+    for _ in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()
+        for k in durations:
+            durations[k].append(ev[k][0].elapsed_time(ev[k][1]) * 1e-3)
+
+    # Whole-step throughput, K steps bracketed by synchronisations.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    slices_per_step = args.n_extract + args.n_insert
+    value = slices_per_step / (wall / args.steps)
+
+    kernels = {}
+    for name, nbytes, nsl in (("extract", algorithmic_bytes_extract(args.n_extract, n), args.n_extract),
+                              ("insert", algorithmic_bytes_insert(args.n_insert, n), args.n_insert)):
+        avg = statistics.mean(durations[name])
+        gbs = nbytes / avg / 1e9
+        kernels[name] = {"avg_ms": avg * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": gbs,
+                         "frac_hbm_peak": gbs / HBM_PEAK_GBS, "slices_per_s": nsl / avg}
+        log(f"{name}: {avg * 1e3:.3f} ms, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak), "
+            f"{nsl / avg / 1e6:.2f} M slices/s")
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    total_bytes = sum(k["algorithmic_bytes"] for k in kernels.values())
+    total_time = sum(k["avg_ms"] for k in kernels.values()) * 1e-3
+
+    result = {
+        "metric": "slices/s and % HBM peak, 256^3 insert+extract",
+        "value": value,
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"256^3 extract {args.n_extract} slices (hc, LINEAR) + insert {args.n_insert} CTF-weighted "
+                        "slices with CTF^2 weights (h2h, LINEAR, windowed sinc 1 voxel)",
+            "volume": [256, 256, 256],
+            "slices_per_step": slices_per_step,
+            "combined_frac_hbm_peak": total_bytes / total_time / 1e9 / HBM_PEAK_GBS,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": kernels[dom]["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": kernels[dom]["frac_hbm_peak"],
+            "traffic": None,
+        },
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(ext, ins)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

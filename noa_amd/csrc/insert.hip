@@ -3,26 +3,45 @@
 // Replaces FourierInsertInterpolate (src/noa/xform/FourierProject.hpp:392-539) launched by
 // launch_insert_central_slices_3d (:1319-1394).
 //
-// Each voxel of the stored half-volume is owned by one lane, which sums the contributions of the slices in
-// increasing slice order and then adds the sum to the volume (read-modify-write, no atomics): the summation
-// order is the reference's, so the result is deterministic and bit-comparable with the CPU path up to the
-// sin/cos of the windowed sinc.
+// noa computes, for every voxel of the stored half-volume, the sum over ALL slices (in slice order) of
+// windowed_sinc(fz) * interp2(slice, f2), then adds it to the volume. Only slices whose slab |fz| <= f_b passes
+// through the voxel contribute (about 0.6 % of the voxel-slice pairs at 256^3 with the default 1-voxel window);
+// the others add exactly zero. This file keeps noa's per-voxel expressions and skips the zero terms.
 //
-// A slice only contributes to voxels within its windowed-sinc slab |fz| <= fftfreq_blackman, i.e. about 0.6 %
-// of the (voxel, slice) pairs at 256^3. The volume is cut into bricks of 4x8x8 voxels (one 256-lane workgroup
-// per brick); a first kernel tests every (brick, slice) pair against a conservative slab bound (the brick's
-// bounding sphere) and writes, per brick, the ordered list of slices that may touch it. The gather kernel
-// then walks only that list. Skipped slices would have added exactly zero, so the result is unchanged.
+// Staged path (no scaling/EWS, array inputs, single-precision values):
+//   - Work item = (16^3 brick of the volume, range of slices). A workgroup of 4 waves owns the brick's 4096
+//     voxels in an LDS accumulator: no atomics.
+//   - Each wave scans the item's slices 64 at a time: lane l tests whether the slab of slice s0+l intersects
+//     the brick box; a ballot yields the passing slices, processed in increasing slice order.
+//   - For a passing slice, the brick's footprint on the slice (the projection of the brick box, plus its
+//     Hermitian mirror when it crosses x=0) is copied to LDS with coalesced row loads, zero outside the
+//     slice's frequency bounds.
+//   - The brick voxels inside the slab are enumerated along the volume axis closest to the slab normal: each
+//     lane owns one column of the 16x16 face and tests only the few candidates (exact noa expressions); hits
+//     interpolate from LDS and accumulate in LDS (one owner per voxel per slice; barriers order the slices).
+//   - Bricks near the origin meet most slices; their slice range is cut into segments so that work items are
+//     balanced. A brick with one segment adds its sum to the volume directly; a split brick writes one partial
+//     sum per segment, and a second kernel adds the partials of each brick in segment order. The result is
+//     deterministic; it differs from noa's strict slice order only by this regrouping of the sum.
+// General path (scaling or EWS, constant-value inputs, double precision): noa's per-voxel gather over every
+// slice, one lane per voxel.
 #include "common.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <map>
 #include <mutex>
-#include <unordered_map>
+#include <tuple>
+#include <vector>
 
 namespace noa_hip {
 
-constexpr i32 BZ = 4, BY = 8, BX = 8;   // brick = 256 voxels = one workgroup
+constexpr i32 BE = 16;                 // brick edge (voxels)
+constexpr i32 NT = 512;                // threads per workgroup (two per column of the brick face)
+constexpr i32 AP = 17;                 // LDS accumulator pitch along x (bank-conflict padding)
+constexpr i32 ACC = BE * BE * AP;      // accumulator slots
+constexpr i32 RS = 32;                 // staged region capacity along y and x (pixels), LDS row pitch
 
 template<class C, class In, class W>
 struct InsertParams {
@@ -30,111 +49,462 @@ struct InsertParams {
     const W* win; i64 wis_b; i32 wis_y, wis_x; bool win_present, win_is_value; W win_value;
     void* vol; i32 vs0, vs1, vs2;
     W* wvol; i32 wvs0, wvs1, wvs2; bool has_wvol;
-    i32 grid[3];          // logical D, H, W
+    i32 grid[3];          // logical D, H, W of the volume
     i32 wr;               // W/2+1
-    i32 bricks[3];        // bricks along z, y, x
+    i32 nb[3];            // bricks along z, y, x
+    const int4* items;    // {brick, first slice, end slice, partial slot or -1}
+    const int4* reduce;   // {brick, first partial slot, number of partials, 0}
+    float* partials;      // per slot: ACC complex accumulators then ACC weights (as float pairs / floats)
     C target[3];
     C slice_f[2];
     i32 slice_shape[2];   // logical H, W of the slices
-    i32 n_slices;
+    i32 n_slices, slice_begin, slice_end;
     C cutoff_sqd, fftfreq_sinc, fftfreq_blackman;
     C ews[2];
     XformArg<C> rot, scale;
     bool general, slices_centered, vol_centered;
-    // Brick culling (list mode): per brick, `counts[brick]` slice indices at lists[brick * n_slices ...].
-    const i32* lists; const i32* counts;
+    C inv_target[3];      // 1 / target, used only where target is a power of two (then x / t == x * (1 / t))
+    bool pow2[3];
+    float wc[32];         // Chebyshev coefficients of the windowed sinc (single-precision staged path)
+    i32 wn;               // their degree
+    float inv_fb;         // 1 / fftfreq_blackman
 };
 
-// Voxel coordinates of lane `t` of brick `brick`.
-__device__ __forceinline__ void brick_voxel(i32 brick, i32 t, const i32 nb[3], i32& z, i32& y, i32& x) {
-    const i32 bx = brick % nb[2];
-    const i32 r = brick / nb[2];
-    const i32 by = r % nb[1];
-    const i32 bz = r / nb[1];
-    z = bz * BZ + t / (BY * BX);
-    y = by * BY + (t / BX) % BY;
-    x = bx * BX + t % BX;
+// Frequency of index `i` along volume axis `a` (0: z, 1: y, 2: x = rfft half, index == frequency).
+__device__ __forceinline__ i32 axis_freq(i32 a, i32 i, bool centered, const i32 grid[3]) {
+    return a == 2 ? i : index2frequency(centered, i, grid[a]);
 }
 
-// Conservative test of every (brick, slice) pair: does the slab |fz| <= fftfreq_blackman of the slice
-// intersect the bounding sphere of the brick (in normalised frequencies)? One lane per brick; the slices are
-// walked in order so that each brick's list is sorted.
+// The 3x3 matrix of a rotation (for quaternions: Quaternion::to_matrix, Quaternion.hpp:69-94); only used to
+// bound and enumerate, never for the exact per-voxel expressions.
 template<class C>
-__global__ __launch_bounds__(256) void insert_cull_kernel(const InsertParams<C, float, float> p, i32* lists,
-                                                          i32* counts) {
-    const i32 n_bricks = p.bricks[0] * p.bricks[1] * p.bricks[2];
-    const i32 brick = static_cast<i32>(blockIdx.x) * 256 + static_cast<i32>(threadIdx.x);
-    if (brick >= n_bricks) return;
-    // Brick centre and radius in normalised frequencies (double: the bound must not be tighter than exact).
-    i32 z0, y0, x0;
-    brick_voxel(brick, 0, p.bricks, z0, y0, x0);
-    const double hz = 0.5 * (BZ - 1), hy = 0.5 * (BY - 1), hx = 0.5 * (BX - 1);
-    // Frequencies of the brick's voxels along z and y may wrap (non-centred layouts): use the min/max of the
-    // frequencies actually covered, which are contiguous inside a brick except across the wrap point.
-    double lo[3], hi[3];
-    const i32 starts[3] = {z0, y0, x0};
-    const i32 sizes[3] = {BZ, BY, BX};
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = 1e30; hi[a] = -1e30;
-        for (int k = 0; k < sizes[a]; ++k) {
-            const i32 idx = starts[a] + k;
-            const i32 n = p.grid[a];
-            if (a < 2 && idx >= n) continue;
-            if (a == 2 && idx >= p.wr) continue;
-            const i32 fr = a < 2 ? index2frequency(p.vol_centered, idx, n) : idx;
-            const double f = static_cast<double>(fr) / static_cast<double>(p.target[a]);
-            lo[a] = fmin(lo[a], f); hi[a] = fmax(hi[a], f);
-        }
+__device__ __forceinline__ void rot_matrix(const Rot<C>& r, C m[9]) {
+    if (!r.quat) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) m[i] = r.m[i];
+        return;
     }
-    (void)hz; (void)hy; (void)hx;
-    if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) { counts[brick] = 0; return; }
-    double c[3], rad2 = 0, closest2 = 0;
-    for (int a = 0; a < 3; ++a) {
-        c[a] = 0.5 * (lo[a] + hi[a]);
-        const double h = 0.5 * (hi[a] - lo[a]);
-        rad2 += h * h;
-        const double d = lo[a] > 0 ? lo[a] : (hi[a] < 0 ? -hi[a] : 0.0);
-        closest2 += d * d;
-    }
-    i32 n = 0;
-    if (closest2 <= static_cast<double>(p.cutoff_sqd) * (1.0 + 1e-6) + 1e-12) {
-        const double rad = sqrt(rad2);
-        // Bound on |fz| error between float arithmetic and this double test: generous relative slack.
-        const double slab = static_cast<double>(p.fftfreq_blackman) * (1.0 + 1e-4) + rad + 1e-6;
-        for (i32 i = 0; i < p.n_slices; ++i) {
-            const Rot<C> r = load_rotation(p.rot, i);
-            double fz;
-            if (!r.quat) {
-                fz = (static_cast<double>(r.m[0]) * c[0] + static_cast<double>(r.m[1]) * c[1]) + static_cast<double>(r.m[2]) * c[2];
-            } else {
-                vec3<double> v{c[0], c[1], c[2]};
-                fz = quat_rotate<double>(r.m[0], r.m[1], r.m[2], r.m[3], v).z;
-            }
-            // The row norm of a rotation is 1; allow for non-normalised input with the actual norm.
-            double norm;
-            if (!r.quat) {
-                norm = sqrt(static_cast<double>(r.m[0]) * r.m[0] + static_cast<double>(r.m[1]) * r.m[1] +
-                            static_cast<double>(r.m[2]) * r.m[2]);
-            } else {
-                norm = static_cast<double>(r.m[0]) * r.m[0] + static_cast<double>(r.m[1]) * r.m[1] +
-                       static_cast<double>(r.m[2]) * r.m[2] + static_cast<double>(r.m[3]) * r.m[3];
-            }
-            if (fabs(fz) <= slab * fmax(norm, 1.0))
-                lists[static_cast<i64>(brick) * p.n_slices + n++] = i;
-        }
-    }
-    counts[brick] = n;
+    const C z = r.m[0], y = r.m[1], x = r.m[2], w = r.m[3];
+    const C tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const C twx = tx * w, twy = ty * w, twz = tz * w;
+    const C txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    m[0] = 1 - (txx + tyy); m[1] = tyz + twx; m[2] = txz - twy;
+    m[3] = tyz - twx; m[4] = 1 - (txx + tzz); m[5] = txy + twz;
+    m[6] = txz + twy; m[7] = txy - twz; m[8] = 1 - (tyy + tzz);
 }
 
-template<class C, class In, class Out, class W, int INTERP, bool GENERAL>
-__global__ __launch_bounds__(256) void insert_kernel(const InsertParams<C, In, W> p) {
-    using RIn = typename vt<In>::real;
-    const i32 brick = static_cast<i32>(xcd_swizzle(blockIdx.x, gridDim.x));
-    i32 z, y, x;
-    brick_voxel(brick, static_cast<i32>(threadIdx.x), p.bricks, z, y, x);
-    if (z >= p.grid[0] || y >= p.grid[1] || x >= p.wr) return;
+struct Region { i32 y0, x0, h, w; };
 
-    // operator() (:478-519)
+__device__ __forceinline__ i32 acc_index(i32 lz, i32 ly, i32 lx) { return (lz * BE + ly) * AP + lx; }
+
+// Windowed sinc of the staged kernel in single precision: a Chebyshev expansion in t = 2 (fz/f_b)^2 - 1 fitted on
+// the host per (f_s, f_b) to an absolute error below 2e-9 on [-f_b, f_b] (noa's FP32 expression itself carries
+// ~1e-7 of rounding). Evaluated with Clenshaw's recurrence.
+__device__ __forceinline__ float window_poly(float fz, float inv_fb, const float* c, i32 n) {
+    const float q = fz * inv_fb;
+    const float t = 2.f * (q * q) - 1.f;
+    const float t2 = 2.f * t;
+    float b1 = 0.f, b2 = 0.f;
+    for (i32 k = n; k >= 1; --k) {
+        const float b0 = fmaf(t2, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fmaf(t, b1, c[0] - b2);
+}
+
+template<class C>
+__device__ __forceinline__ C window_of(const C fz, const C fs, const C fb, const float* wc, i32 wn, float inv_fb) {
+    if constexpr (sizeof(C) == 4) {
+        if (wn < 0) return windowed_sinc<C>(fz, fs, fb);
+        return window_poly(fz, inv_fb, wc, wn);
+    } else {
+        return windowed_sinc<C>(fz, fs, fb);
+    }
+}
+
+// Footprint regions of a brick box on one slice (A: px >= 0, B: px < 0 mirrored), see the kernel below.
+struct Footprint {
+    Region reg[2];
+    bool use_a, use_b, staged;
+};
+
+template<class C>
+__device__ __forceinline__ Footprint footprint(const C m[9], const C cen[3], const C half[3], C hs, C ws) {
+    const C pyc = (m[3] * cen[0] + m[4] * cen[1] + m[5] * cen[2]) * hs;
+    const C pxc = (m[6] * cen[0] + m[7] * cen[1] + m[8] * cen[2]) * ws;
+    const C hy = (fabs(m[3]) * half[0] + fabs(m[4]) * half[1] + fabs(m[5]) * half[2]) * hs;
+    const C hx = (fabs(m[6]) * half[0] + fabs(m[7]) * half[1] + fabs(m[8]) * half[2]) * ws;
+    // One extra pixel on each side absorbs the rounding difference with the exact per-voxel positions.
+    Footprint fp;
+    fp.reg[0].y0 = static_cast<i32>(floor(pyc - hy)) - 1;
+    fp.reg[0].h = static_cast<i32>(floor(pyc + hy)) + 3 - fp.reg[0].y0;
+    fp.reg[0].x0 = max(0, static_cast<i32>(floor(pxc - hx)) - 1);
+    fp.reg[0].w = static_cast<i32>(floor(pxc + hx)) + 3 - fp.reg[0].x0;
+    fp.reg[1].y0 = static_cast<i32>(floor(-(pyc + hy))) - 1;
+    fp.reg[1].h = static_cast<i32>(floor(-(pyc - hy))) + 3 - fp.reg[1].y0;
+    fp.reg[1].x0 = 0;
+    fp.reg[1].w = max(1, static_cast<i32>(floor(-(pxc - hx))) + 3);
+    fp.use_a = pxc + hx >= C(-0.5);
+    fp.use_b = pxc - hx < C(0.5);
+    fp.staged = (!fp.use_a || (fp.reg[0].h <= RS && fp.reg[0].w <= RS)) &&
+                (!fp.use_b || (fp.reg[1].h <= RS && fp.reg[1].w <= RS));
+    return fp;
+}
+
+template<class C> __device__ __forceinline__ C bcast(C v, i32 lane) {
+    if constexpr (sizeof(C) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const i32 lo = __builtin_amdgcn_readlane(static_cast<i32>(b & 0xffffffffLL), lane);
+        const i32 hi = __builtin_amdgcn_readlane(static_cast<i32>(b >> 32), lane);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    }
+}
+
+template<class C, class In, class Out, class W, int INTERP>
+__global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C, In, W> p) {
+    using RIn = typename vt<In>::real;
+    constexpr i32 SLOTS = RS * RS / NT;            // staged elements per thread and region
+    constexpr i32 NCOL = BE * BE;                  // columns of the brick face
+    __shared__ In acc_v[ACC];
+    __shared__ W acc_w[ACC];
+    __shared__ In st_v[2][RS * RS];
+    __shared__ W st_w[2][RS * RS];
+
+    const i32 tid = static_cast<i32>(threadIdx.x);
+    const i32 lane = tid & 63;
+    const int4 item = p.items[blockIdx.x];
+    const i32 brick = item.x;
+    const i32 bxi = brick % p.nb[2];
+    const i32 rem = brick / p.nb[2];
+    const i32 byi = rem % p.nb[1];
+    const i32 bzi = rem / p.nb[1];
+    const i32 org[3] = {bzi * BE, byi * BE, bxi * BE};
+    const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
+
+    for (i32 v = tid; v < ACC; v += NT) {
+        acc_v[v] = vzero<In>();
+        acc_w[v] = W(0);
+    }
+
+    // Brick box in normalised frequencies, and whether each axis is free of the non-centred wrap (then the
+    // frequencies of the brick along it are org-frequency + k).
+    C cen[3], half[3];
+    i32 f0[3], n_in[3];
+    bool contiguous[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        i32 lo = 0x7fffffff, hi = -0x7fffffff, n = 0;
+        bool contig = true;
+        const i32 first = axis_freq(a, org[a], p.vol_centered, p.grid);
+        for (i32 k = 0; k < BE; ++k) {
+            const i32 idx = org[a] + k;
+            if (idx >= ext[a]) break;
+            const i32 f = axis_freq(a, idx, p.vol_centered, p.grid);
+            contig = contig && (f == first + k);
+            lo = min(lo, f);
+            hi = max(hi, f);
+            ++n;
+        }
+        f0[a] = first;
+        n_in[a] = n;
+        contiguous[a] = contig;
+        cen[a] = (static_cast<C>(lo) + static_cast<C>(hi)) * C(0.5) / p.target[a];
+        half[a] = (static_cast<C>(hi) - static_cast<C>(lo)) * C(0.5) / p.target[a];
+    }
+    const C slack = C(1e-5);
+    const i32 Hs = p.slice_shape[0], Ws = p.slice_shape[1];
+    const i32 lo_y = -Hs / 2, hi_y = (Hs - 1) / 2, hi_x = Ws / 2;
+
+    // Iterator over the passing slices of the item, in increasing order. Every wave scans the same 64 slices:
+    // lane l reads the rotation of slice scan0+l once, tests its slab against the brick box, and the rotation of
+    // a passing slice is later broadcast from that lane (no scalar memory traffic per slice).
+    i32 scan0 = item.y - 64;
+    unsigned long long mask = 0;
+    Rot<C> scan_rot{};
+    scan_rot.quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
+    auto next_slice = [&](Rot<C>& out) -> i32 {
+        while (mask == 0) {
+            scan0 += 64;
+            if (scan0 >= item.z) return -1;
+            const i32 s = scan0 + lane;
+            bool pass = false;
+            if (s < item.z) {
+                scan_rot = load_rotation(p.rot, s);
+                C m[9];
+                rot_matrix(scan_rot, m);
+                const C d = fabs(m[0] * cen[0] + m[1] * cen[1] + m[2] * cen[2]);
+                const C bound = p.fftfreq_blackman + fabs(m[0]) * half[0] + fabs(m[1]) * half[1] + fabs(m[2]) * half[2];
+                pass = d <= bound + slack;
+            }
+            mask = __ballot(pass);
+        }
+        const i32 k = __ffsll(mask) - 1;
+        mask &= mask - 1;
+        out.quat = scan_rot.quat;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out.m[q] = bcast(scan_rot.m[q], k);
+        return scan0 + k;
+    };
+
+    // Loads the staged elements of slice i into registers (issued ahead of their use). Thread t owns column
+    // t % RS of rows t / RS + (NT / RS) j of each region.
+    In rv[2][SLOTS];
+    W rw[2][SLOTS];
+    const i32 my_col = tid % RS, my_row = tid / RS;
+    auto load_footprint = [&](i32 i, const Footprint& f) {
+        const In* __restrict__ sl = p.in + static_cast<i64>(i) * p.is_b;
+        const W* __restrict__ wsl = p.win_present ? p.win + static_cast<i64>(i) * p.wis_b : nullptr;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+#pragma unroll
+            for (int j = 0; j < SLOTS; ++j) {
+                rv[r][j] = vzero<In>();
+                rw[r][j] = W(0);
+            }
+            if (!f.staged || !(r == 0 ? f.use_a : f.use_b)) continue;   // uniform
+            const Region g = f.reg[r];
+            const i32 xf = g.x0 + my_col;
+            if (my_col >= g.w || xf > hi_x) continue;
+#pragma unroll
+            for (int j = 0; j < SLOTS; ++j) {
+                const i32 row = my_row + j * (NT / RS);
+                const i32 yf = g.y0 + row;
+                if (row < g.h && yf >= lo_y && yf <= hi_y) {
+                    const i32 off = frequency2index(p.slices_centered, yf, Hs);
+                    rv[r][j] = sl[off * p.is_y + xf * p.is_x];
+                    if (wsl) rw[r][j] = wsl[off * p.wis_y + xf * p.wis_x];
+                }
+            }
+        }
+    };
+
+    Rot<C> rot{};
+    i32 cur = next_slice(rot);
+    Footprint fp{};
+    C m[9];
+    if (cur >= 0) {
+        rot_matrix(rot, m);
+        fp = footprint(m, cen, half, p.slice_f[0], p.slice_f[1]);
+        load_footprint(cur, fp);
+    }
+    // Column of the brick face owned by this thread, and which candidates (even or odd) it takes.
+    const i32 col_id = tid % NCOL, parity = tid / NCOL;
+    while (cur >= 0) {
+        // Publish the staged footprint of the current slice.
+        if (fp.staged) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                if (!(r == 0 ? fp.use_a : fp.use_b)) continue;
+#pragma unroll
+                for (int j = 0; j < SLOTS; ++j) {
+                    st_v[r][tid + j * NT] = rv[r][j];
+                    st_w[r][tid + j * NT] = rw[r][j];
+                }
+            }
+        }
+        __syncthreads();
+
+        // Prefetch the next passing slice while the current one is processed.
+        Rot<C> rot_n{};
+        const i32 nxt = next_slice(rot_n);
+        Footprint fp_n{};
+        C m_n[9];
+        if (nxt >= 0) {
+            rot_matrix(rot_n, m_n);
+            fp_n = footprint(m_n, cen, half, p.slice_f[0], p.slice_f[1]);
+            load_footprint(nxt, fp_n);
+        }
+
+        // Enumerate the brick voxels near the slab along the axis closest to the normal.
+        const C al[3] = {m[0] / p.target[0], m[1] / p.target[1], m[2] / p.target[2]};
+        i32 ax = 0;
+        if (fabs(al[1]) > fabs(al[ax])) ax = 1;
+        if (fabs(al[2]) > fabs(al[ax])) ax = 2;
+        const i32 bx_ = ax == 0 ? 1 : 0;   // face axes, the faster one (c) last
+        const i32 cx_ = ax == 2 ? 1 : 2;
+        const i32 gb = org[bx_] + col_id / BE;
+        const i32 gc = org[cx_] + col_id % BE;
+        if (gb < ext[bx_] && gc < ext[cx_]) {
+            const i32 fbq = axis_freq(bx_, gb, p.vol_centered, p.grid);
+            const i32 fcq = axis_freq(cx_, gc, p.vol_centered, p.grid);
+            const C beta = al[bx_] * static_cast<C>(fbq) + al[cx_] * static_cast<C>(fcq);
+            // Candidates along the axis: |al[ax] * f + beta| <= f_b in exact arithmetic, widened by eps samples to
+            // cover the rounding of the exact per-voxel test below (a few ulps of |f| <= 1, divided by
+            // |al[ax]| >= 1/(sqrt(3) T)).
+            i32 k_lo = 0, k_hi = n_in[ax] - 1;
+            if (contiguous[ax] && fabs(al[ax]) > C(1e-20)) {
+                const C eps = C(1e-3) + C(8e-6) * p.target[ax];
+                C lo = (-p.fftfreq_blackman - beta) / al[ax];
+                C hi = (p.fftfreq_blackman - beta) / al[ax];
+                if (lo > hi) { const C t = lo; lo = hi; hi = t; }
+                const C klo = ceil(lo - eps) - static_cast<C>(f0[ax]);
+                const C khi = floor(hi + eps) - static_cast<C>(f0[ax]);
+                k_lo = static_cast<i32>(fmin(fmax(klo, C(0)), static_cast<C>(n_in[ax])));
+                k_hi = static_cast<i32>(fmax(fmin(khi, static_cast<C>(n_in[ax] - 1)), C(-1)));
+            }
+            // The normalised frequency as noa computes it: index2frequency / target, with the division replaced by
+            // the (exact) multiplication by 1/target when target is a power of two.
+            auto nfreq = [&](i32 a, i32 fr) -> C {
+                return p.pow2[a] ? static_cast<C>(fr) * p.inv_target[a] : static_cast<C>(fr) / p.target[a];
+            };
+            const C fbv = nfreq(bx_, fbq);
+            const C fcv = nfreq(cx_, fcq);
+            const In* __restrict__ sl = p.in + static_cast<i64>(cur) * p.is_b;
+            const W* __restrict__ wsl = p.win_present ? p.win + static_cast<i64>(cur) * p.wis_b : nullptr;
+            for (i32 kk = k_lo + parity; kk <= k_hi; kk += 2) {
+                C fv[3];
+                fv[ax] = nfreq(ax, axis_freq(ax, org[ax] + kk, p.vol_centered, p.grid));
+                fv[bx_] = fbv;
+                fv[cx_] = fcv;
+                vec3<C> f;
+                f.z = fv[0]; f.y = fv[1]; f.x = fv[2];
+                // The exact tests of noa (:479-494).
+                if (dot3(f) > p.cutoff_sqd) continue;
+                Scale<C> sc;
+                sc.identity = true;
+                vec2<C> f2;
+                const C fz = grid2slice<false>(f, sc, rot, C(0), C(0), f2);
+                if (!(fabs(fz) <= p.fftfreq_blackman)) continue;
+                const C window = window_of<C>(fz, p.fftfreq_sinc, p.fftfreq_blackman, p.wc, p.wn, p.inv_fb);
+                C py = f2.y * p.slice_f[0], px = f2.x * p.slice_f[1];
+                In v;
+                W wv = W(0);
+                if (fp.staged) {
+                    RIn conj = 1;
+                    i32 r = 0;
+                    if (px < 0) { py *= -1; px *= -1; conj = -1; r = 1; }
+                    const Region g = fp.reg[r];
+                    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+                        const i32 yy = static_cast<i32>(round(py)) - g.y0, xx = static_cast<i32>(round(px)) - g.x0;
+                        const i32 o = min(max(yy * RS + xx, 0), RS * RS - 1);
+                        v = vadd(vzero<In>(), vmul(st_v[r][o], RIn(1)));
+                        wv = W(0) + st_w[r][o] * W(1);
+                        (void)conj;   // NEAREST does not conjugate after the flip (Interpolation.hpp:479-484)
+                    } else {
+                        const C fly = floor(py), flx = floor(px);
+                        const i32 y0 = static_cast<i32>(fly) - g.y0, x0 = static_cast<i32>(flx) - g.x0;
+                        const RIn ty = static_cast<RIn>(py - fly), tx = static_cast<RIn>(px - flx);
+                        const RIn wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
+                        // The margins of the footprint keep every tap inside the region; the clamp only guards the
+                        // LDS index.
+                        const i32 o0 = min(max(y0 * RS + x0, 0), RS * RS - RS - 2);
+                        v = vzero<In>();
+#pragma unroll
+                        for (int yy = 0; yy < 2; ++yy) {
+                            In vy = vzero<In>();
+                            W wy_ = W(0);
+#pragma unroll
+                            for (int xx = 0; xx < 2; ++xx) {
+                                const i32 o = o0 + yy * RS + xx;
+                                vy = vadd(vy, vmul(st_v[r][o], wx[xx]));
+                                wy_ = wy_ + st_w[r][o] * static_cast<W>(wx[xx]);
+                            }
+                            v = vadd(v, vmul(vy, wy[yy]));
+                            wv = wv + wy_ * static_cast<W>(wy[yy]);
+                        }
+                        vimag_mul(v, conj);
+                    }
+                } else {
+                    v = interp2<INTERP, In>(sl, p.is_y, p.is_x, Hs, Ws, p.slices_centered, py, px);
+                    if (wsl) wv = interp2<INTERP, W>(wsl, p.wis_y, p.wis_x, Hs, Ws, p.slices_centered, py, px);
+                }
+                i32 li[3];
+                li[ax] = kk; li[bx_] = gb - org[bx_]; li[cx_] = gc - org[cx_];
+                const i32 lv = acc_index(li[0], li[1], li[2]);
+                acc_v[lv] = vadd(acc_v[lv], vmul(v, static_cast<RIn>(window)));
+                if (p.has_wvol)
+                    acc_w[lv] = acc_w[lv] + (p.win_present ? wv * static_cast<W>(window) : static_cast<W>(window));
+            }
+        }
+        __syncthreads();
+        cur = nxt;
+        if (cur >= 0) {
+            fp = fp_n;
+            rot = rot_n;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) m[q] = m_n[q];
+        }
+    }
+
+    if (item.w >= 0) {
+        // Split brick: store this segment's sums (every slot; the reduction applies the cutoff).
+        In* pv = reinterpret_cast<In*>(p.partials + static_cast<i64>(item.w) * ACC * 3);
+        W* pw = reinterpret_cast<W*>(p.partials + static_cast<i64>(item.w) * ACC * 3 + ACC * 2);
+        for (i32 v = tid; v < ACC; v += NT) {
+            pv[v] = acc_v[v];
+            pw[v] = acc_w[v];
+        }
+        return;
+    }
+    // value and weights of every in-cutoff voxel are added to the volume (:516-518).
+    for (i32 v = tid; v < BE * BE * BE; v += NT) {
+        const i32 lz = v / (BE * BE), ly = (v / BE) % BE, lx = v % BE;
+        const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
+        if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
+        vec3<C> f;
+        f.z = static_cast<C>(index2frequency(p.vol_centered, z, p.grid[0])) / p.target[0];
+        f.y = static_cast<C>(index2frequency(p.vol_centered, y, p.grid[1])) / p.target[1];
+        f.x = static_cast<C>(x) / p.target[2];
+        if (dot3(f) > p.cutoff_sqd) continue;
+        const i32 a = acc_index(lz, ly, lx);
+        Out* o = static_cast<Out*>(p.vol) + (z * p.vs0 + y * p.vs1 + x * p.vs2);
+        *o = vadd(*o, vcast<Out>(acc_v[a]));
+        if (p.has_wvol) {
+            W* w = p.wvol + (z * p.wvs0 + y * p.wvs1 + x * p.wvs2);
+            *w = *w + acc_w[a];
+        }
+    }
+}
+
+// Adds the partial sums of each split brick to the volume, in segment order.
+template<class C, class In, class Out, class W>
+__global__ __launch_bounds__(NT) void insert_reduce_kernel(const InsertParams<C, In, W> p) {
+    const int4 red = p.reduce[blockIdx.x];
+    const i32 brick = red.x;
+    const i32 bxi = brick % p.nb[2];
+    const i32 rem = brick / p.nb[2];
+    const i32 org[3] = {(rem / p.nb[1]) * BE, (rem % p.nb[1]) * BE, bxi * BE};
+    const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
+    for (i32 v = static_cast<i32>(threadIdx.x); v < BE * BE * BE; v += NT) {
+        const i32 lz = v / (BE * BE), ly = (v / BE) % BE, lx = v % BE;
+        const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
+        if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
+        vec3<C> f;
+        f.z = static_cast<C>(index2frequency(p.vol_centered, z, p.grid[0])) / p.target[0];
+        f.y = static_cast<C>(index2frequency(p.vol_centered, y, p.grid[1])) / p.target[1];
+        f.x = static_cast<C>(x) / p.target[2];
+        if (dot3(f) > p.cutoff_sqd) continue;
+        const i32 a = acc_index(lz, ly, lx);
+        In sv = vzero<In>();
+        W sw = W(0);
+        for (i32 s = 0; s < red.z; ++s) {
+            const float* base = p.partials + static_cast<i64>(red.y + s) * ACC * 3;
+            sv = vadd(sv, reinterpret_cast<const In*>(base)[a]);
+            sw = sw + reinterpret_cast<const W*>(base + ACC * 2)[a];
+        }
+        Out* o = static_cast<Out*>(p.vol) + (z * p.vs0 + y * p.vs1 + x * p.vs2);
+        *o = vadd(*o, vcast<Out>(sv));
+        if (p.has_wvol) {
+            W* w = p.wvol + (z * p.wvs0 + y * p.wvs1 + x * p.wvs2);
+            *w = *w + sw;
+        }
+    }
+}
+
+// General path: one lane per voxel, every slice in order (noa's operator as is).
+template<class C, class In, class Out, class W, int INTERP, bool GENERAL>
+__global__ __launch_bounds__(256) void insert_general_kernel(const InsertParams<C, In, W> p) {
+    using RIn = typename vt<In>::real;
+    const i32 nxy = p.grid[1] * p.wr;
+    const i64 t = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= static_cast<i64>(p.grid[0]) * nxy) return;
+    const i32 z = static_cast<i32>(t / nxy);
+    const i32 r = static_cast<i32>(t - static_cast<i64>(z) * nxy);
+    const i32 y = r / p.wr, x = r - (r / p.wr) * p.wr;
+
     vec3<C> f;
     f.z = static_cast<C>(index2frequency(p.vol_centered, z, p.grid[0])) / p.target[0];
     f.y = static_cast<C>(index2frequency(p.vol_centered, y, p.grid[1])) / p.target[1];
@@ -143,10 +513,7 @@ __global__ __launch_bounds__(256) void insert_kernel(const InsertParams<C, In, W
 
     In value = vzero<In>();
     W weights{};
-    const i32 count = p.counts[brick];
-    const i32* list = p.lists + static_cast<i64>(brick) * p.n_slices;
-    for (i32 k = 0; k < count; ++k) {
-        const i32 i = list[k];
+    for (i32 i = p.slice_begin; i < p.slice_end; ++i) {
         const Rot<C> rot = load_rotation(p.rot, i);
         Scale<C> sc;
         sc.identity = true;
@@ -201,16 +568,161 @@ template<class T> T value_of(const Arr& a) {
     }
 }
 
-// Per-device scratch for the brick lists, grown on demand and kept for the process lifetime. Calls that
-// overlap on one device from different host threads are serialised by the mutex only up to the launch; the
-// kernels of two overlapping calls on different streams would share the buffer, so such calls take a private
-// allocation instead.
-struct Scratch {
-    void* ptr{};
-    size_t bytes{};
+i32 host_freq(bool centered, i32 i, i32 n) { return centered ? i - n / 2 : (i < (n + 1) / 2 ? i : i - n); }
+
+// Chebyshev fit of windowed_sinc(f, fs, fb) (FourierProject.hpp:99-110) on f in [-fb, fb], as a series in
+// t = 2 (f/fb)^2 - 1 (the window is even). The degree grows until the absolute error, checked on a dense grid in
+// double precision, is below 2e-9. Kept per (fs, fb) for the process lifetime.
+struct WindowFit {
+    float c[32];
+    i32 n;
 };
-std::mutex g_scratch_mutex;
-std::unordered_map<int, Scratch> g_scratch;
+std::mutex g_window_mutex;
+std::map<std::pair<float, float>, WindowFit> g_windows;
+
+WindowFit fit_window(float fs, float fb) {
+    std::lock_guard<std::mutex> lock(g_window_mutex);
+    const auto key = std::make_pair(fs, fb);
+    auto it = g_windows.find(key);
+    if (it != g_windows.end()) return it->second;
+    const double PI = 3.141592653589793238462643383279502884;
+    auto w = [&](double f) {
+        const double x = PI * f / fs;
+        const double s = x == 0 ? 1.0 : std::sin(x) / x;
+        const double b = PI * f / fb;
+        return s * (0.42 + 0.5 * std::cos(b) + 0.08 * std::cos(2 * b));
+    };
+    WindowFit fit{};
+    for (i32 n = 6; n <= 31; ++n) {
+        const i32 m = n + 1;
+        double c[32] = {};
+        for (i32 k = 0; k <= n; ++k) {
+            double sum = 0;
+            for (i32 j = 0; j < m; ++j) {
+                const double th = PI * (j + 0.5) / m;
+                const double t = std::cos(th);
+                const double u = 0.5 * (t + 1);
+                sum += w(static_cast<double>(fb) * std::sqrt(u)) * std::cos(k * th);
+            }
+            c[k] = 2.0 * sum / m;
+        }
+        c[0] *= 0.5;
+        double err = 0;
+        for (i32 j = 0; j <= 2000; ++j) {
+            const double u = j / 2000.0, t = 2 * u - 1;
+            double b1 = 0, b2 = 0;
+            for (i32 k = n; k >= 1; --k) { const double b0 = 2 * t * b1 - b2 + c[k]; b2 = b1; b1 = b0; }
+            const double approx = t * b1 - b2 + c[0];
+            err = std::max(err, std::abs(approx - w(static_cast<double>(fb) * std::sqrt(u))));
+        }
+        for (i32 k = 0; k <= n; ++k) fit.c[k] = static_cast<float>(c[k]);
+        fit.n = n;
+        if (err < 2e-9) break;
+        if (n == 31 && err > 1e-7) fit.n = -1;   // too oscillatory: evaluate noa's expression instead
+    }
+    g_windows[key] = fit;
+    return fit;
+}
+
+// Work plan of the staged kernel for one geometry: the active bricks (box within the cutoff sphere), each cut
+// into slice segments so that the expected number of passing slices per item is at most kListTarget. The
+// expected count assumes orientations uniform on the sphere: a brick at distance r meets a fraction
+// min(1, t / r) of the slabs, t = f_b + mean projected half-extent. Items are sorted by expected work, largest
+// first. Kept per (device, geometry, slice count, window) for the process lifetime.
+constexpr double kListTarget = 384.0;
+
+struct Plan {
+    int4* items{};
+    int4* reduce{};
+    i32 n_items{}, n_reduce{}, n_partials{};
+};
+using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double>;
+std::mutex g_plan_mutex;
+std::map<PlanKey, Plan> g_plans;
+
+int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff, bool centered, i32 n_slices,
+              double fftfreq_blackman, Plan& plan, std::string& err) {
+    const PlanKey key{device, grid[0], grid[1], grid[2], target[0], target[1], target[2], cutoff, centered, n_slices,
+                      fftfreq_blackman};
+    std::lock_guard<std::mutex> lock(g_plan_mutex);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+        plan = it->second;
+        return NOA_HIP_OK;
+    }
+    const i32 ext[3] = {grid[0], grid[1], grid[2] / 2 + 1};
+    const i32 nb[3] = {(ext[0] + BE - 1) / BE, (ext[1] + BE - 1) / BE, (ext[2] + BE - 1) / BE};
+    const double c2 = std::max(cutoff, 0.0) * std::max(cutoff, 0.0);
+    struct Entry { double work; int4 item; };
+    std::vector<Entry> entries;
+    std::vector<int4> reduce;
+    i32 n_partials = 0;
+    for (i32 bz = 0; bz < nb[0]; ++bz)
+        for (i32 by = 0; by < nb[1]; ++by)
+            for (i32 bx = 0; bx < nb[2]; ++bx) {
+                const i32 org[3] = {bz * BE, by * BE, bx * BE};
+                double lo[3], hi[3];
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = 1e300; hi[a] = -1e300;
+                    for (i32 k = 0; k < BE && org[a] + k < ext[a]; ++k) {
+                        const i32 f = a == 2 ? org[a] + k : host_freq(centered, org[a] + k, grid[a]);
+                        const double fn = static_cast<double>(f) / static_cast<double>(target[a]);
+                        lo[a] = std::min(lo[a], fn);
+                        hi[a] = std::max(hi[a], fn);
+                    }
+                }
+                double closest = 0, r2 = 0, t = fftfreq_blackman;
+                for (int a = 0; a < 3; ++a) {
+                    const double d = lo[a] > 0 ? lo[a] : (hi[a] < 0 ? -hi[a] : 0.0);
+                    closest += d * d;
+                    const double c = 0.5 * (lo[a] + hi[a]);
+                    r2 += c * c;
+                    t += 0.25 * (hi[a] - lo[a]);   // E|n_a| = 1/2 times the half-extent
+                }
+                if (closest > c2 * (1 + 1e-6) + 1e-12) continue;
+                const i32 brick = (bz * nb[1] + by) * nb[2] + bx;
+                const double density = std::min(1.0, t / std::max(std::sqrt(r2), 1e-12));
+                const double expected = density * n_slices;
+                const i32 n_seg = std::max<i32>(1, static_cast<i32>(std::ceil(expected / kListTarget)));
+                const i32 len = (n_slices + n_seg - 1) / n_seg;
+                const i32 first_slot = n_seg > 1 ? n_partials : -1;
+                i32 made = 0;
+                for (i32 sgm = 0; sgm < n_seg; ++sgm) {
+                    const i32 b = sgm * len, e = std::min(n_slices, b + len);
+                    if (b >= e) break;
+                    entries.push_back({expected / n_seg, make_int4(brick, b, e, n_seg > 1 ? n_partials + made : -1)});
+                    ++made;
+                }
+                if (n_seg > 1) {
+                    reduce.push_back(make_int4(brick, first_slot, made, 0));
+                    n_partials += made;
+                }
+            }
+    std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
+    std::vector<int4> items(entries.size());
+    for (size_t k = 0; k < entries.size(); ++k) items[k] = entries[k].item;
+    Plan pl;
+    pl.n_items = static_cast<i32>(items.size());
+    pl.n_reduce = static_cast<i32>(reduce.size());
+    pl.n_partials = n_partials;
+    if (!items.empty()) {
+        if (hipMalloc(&pl.items, items.size() * sizeof(int4)) != hipSuccess ||
+            hipMemcpy(pl.items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "insert: could not upload the work plan";
+            return NOA_HIP_ERROR_RUNTIME;
+        }
+    }
+    if (!reduce.empty()) {
+        if (hipMalloc(&pl.reduce, reduce.size() * sizeof(int4)) != hipSuccess ||
+            hipMemcpy(pl.reduce, reduce.data(), reduce.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "insert: could not upload the work plan";
+            return NOA_HIP_ERROR_RUNTIME;
+        }
+    }
+    g_plans[key] = pl;
+    plan = pl;
+    return NOA_HIP_OK;
+}
 
 template<class C, class In, class Out, class W>
 int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
@@ -230,11 +742,14 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     for (int i = 0; i < 3; ++i) {
         p.grid[i] = static_cast<i32>(d.out_shape[i + 1]);
         p.target[i] = static_cast<C>(d.target[i]);
+        const i64 t = d.target[i];
+        p.pow2[i] = t > 0 && (t & (t - 1)) == 0;
+        p.inv_target[i] = C(1) / p.target[i];
     }
     p.wr = p.grid[2] / 2 + 1;
-    p.bricks[0] = (p.grid[0] + BZ - 1) / BZ;
-    p.bricks[1] = (p.grid[1] + BY - 1) / BY;
-    p.bricks[2] = (p.wr + BX - 1) / BX;
+    p.nb[0] = (p.grid[0] + BE - 1) / BE;
+    p.nb[1] = (p.grid[1] + BE - 1) / BE;
+    p.nb[2] = (p.wr + BE - 1) / BE;
     p.slice_f[0] = static_cast<C>(d.in_shape[2]);
     p.slice_f[1] = static_cast<C>(d.in_shape[3]);
     p.slice_shape[0] = static_cast<i32>(d.in_shape[2]);
@@ -255,59 +770,62 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     p.scale = to_arg<C>(d.scale);
     p.slices_centered = d.src_centered;
     p.vol_centered = d.dst_centered;
-
-    const i64 n_bricks = static_cast<i64>(p.bricks[0]) * p.bricks[1] * p.bricks[2];
-    if (n_bricks == 0 || p.n_slices == 0) return NOA_HIP_OK;
-
-    // Brick lists: n_bricks counts + n_bricks * n_slices indices.
-    const size_t need = static_cast<size_t>(n_bricks) * (static_cast<size_t>(p.n_slices) + 1) * sizeof(i32);
-    int device = 0;
-    (void)hipGetDevice(&device);
-    void* scratch = nullptr;
     {
-        std::lock_guard<std::mutex> lock(g_scratch_mutex);
-        Scratch& s = g_scratch[device];
-        if (s.bytes < need) {
-            if (s.ptr) {
-                // The previous buffer may still be used by queued kernels of an earlier call.
-                (void)hipDeviceSynchronize();
-                (void)hipFree(s.ptr);
-                s.ptr = nullptr; s.bytes = 0;
-            }
-            if (hipMalloc(&s.ptr, need) != hipSuccess) {
-                err = "insert: could not allocate the brick lists";
-                return NOA_HIP_ERROR_RUNTIME;
-            }
-            s.bytes = need;
-        }
-        scratch = s.ptr;
+        const WindowFit fit = fit_window(static_cast<float>(p.fftfreq_sinc), static_cast<float>(p.fftfreq_blackman));
+        for (int k = 0; k < 32; ++k) p.wc[k] = fit.c[k];
+        p.wn = fit.n;
+        p.inv_fb = 1.f / static_cast<float>(p.fftfreq_blackman);
     }
-    i32* counts = static_cast<i32*>(scratch);
-    i32* lists = counts + n_bricks;
-    p.counts = counts;
-    p.lists = lists;
-
-    InsertParams<C, float, float> q{};
-    q.grid[0] = p.grid[0]; q.grid[1] = p.grid[1]; q.grid[2] = p.grid[2]; q.wr = p.wr;
-    for (int i = 0; i < 3; ++i) { q.bricks[i] = p.bricks[i]; q.target[i] = p.target[i]; }
-    q.n_slices = p.n_slices; q.cutoff_sqd = p.cutoff_sqd; q.fftfreq_blackman = p.fftfreq_blackman;
-    q.rot = p.rot; q.vol_centered = p.vol_centered;
-    const unsigned cull_blocks = static_cast<unsigned>((n_bricks + 255) / 256);
-    if (p.general) {
-        // Scaling/EWS bend or stretch the slab: cull nothing (every brick gets the full list).
-        // The list then holds 0..n_slices-1; it is written by the same kernel with an infinite slab.
-        q.fftfreq_blackman = static_cast<C>(1e30);
-    }
-    hipLaunchKernelGGL((insert_cull_kernel<C>), dim3(cull_blocks), dim3(256), 0, stream, q, lists, counts);
-
-    const dim3 grid(static_cast<unsigned>(n_bricks)), block(256);
+    p.slice_begin = 0;
+    p.slice_end = p.n_slices;
+    if (p.n_slices == 0) return NOA_HIP_OK;
     const bool nearest = d.interp == NOA_HIP_INTERP_NEAREST;
+
+    constexpr bool single = sizeof(typename vt<In>::real) == 4 && sizeof(W) == 4;
+    if constexpr (single) {
+        if (!p.general && !p.in_is_value && !p.win_is_value) {
+            int device = 0;
+            (void)hipGetDevice(&device);
+            Plan plan;
+            const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
+                                     p.n_slices, static_cast<double>(p.fftfreq_blackman), plan, err);
+            if (st != NOA_HIP_OK) return st;
+            if (plan.n_items == 0) return NOA_HIP_OK;
+            p.items = plan.items;
+            p.reduce = plan.reduce;
+            void* partials = nullptr;
+            if (plan.n_partials > 0) {
+                const size_t bytes = static_cast<size_t>(plan.n_partials) * ACC * 3 * sizeof(float);
+                if (hipMallocAsync(&partials, bytes, stream) != hipSuccess) {
+                    err = "insert: could not allocate the partial sums";
+                    return NOA_HIP_ERROR_RUNTIME;
+                }
+            }
+            p.partials = static_cast<float*>(partials);
+            if (nearest)
+                hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST>), dim3(plan.n_items), dim3(NT), 0, stream, p);
+            else
+                hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR>), dim3(plan.n_items), dim3(NT), 0, stream, p);
+            if (plan.n_reduce > 0)
+                hipLaunchKernelGGL((insert_reduce_kernel<C, In, Out, W>), dim3(plan.n_reduce), dim3(NT), 0, stream, p);
+            hipError_t e = hipGetLastError();
+            if (partials) {
+                const hipError_t fe = hipFreeAsync(partials, stream);
+                if (e == hipSuccess) e = fe;
+            }
+            if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+            return NOA_HIP_OK;
+        }
+    }
+
+    const i64 n_vox = static_cast<i64>(p.grid[0]) * p.grid[1] * p.wr;
+    const dim3 grid(static_cast<unsigned>((n_vox + 255) / 256)), block(256);
     if (p.general) {
-        if (nearest) hipLaunchKernelGGL((insert_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((insert_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, true>), grid, block, 0, stream, p);
+        if (nearest) hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, true>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, true>), grid, block, 0, stream, p);
     } else {
-        if (nearest) hipLaunchKernelGGL((insert_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, false>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((insert_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, false>), grid, block, 0, stream, p);
+        if (nearest) hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, false>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, false>), grid, block, 0, stream, p);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }

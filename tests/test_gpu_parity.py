@@ -214,3 +214,98 @@ def test_invalid_arguments_raise_before_launch():
                                          None, (1, 1, n, n), vol, None, (1, n, n, n), None, rot[:1])
     with pytest.raises(nx().NoaError, match="same device"):
         nx().extract_central_slices_3d("h2h", vol.cpu(), None, (1, n, n, n), out, None, (2, 1, n, n), None, rot)
+
+
+# ------------------------------------------------------------------------------------------------ insertion paths
+def _insert_dev(remap, s, w, shape, n, inv, scale=None, opts=None, out_dtype=torch.complex64, quat=False):
+    vol = torch.zeros((1, n, n, n // 2 + 1), dtype=out_dtype, device=DEV)
+    wvol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+    rot = nx().Quaternions(to_dev(inv)) if quat else to_dev(inv)
+    nx().insert_central_slices_3d(remap, to_dev(s), None if w is None else to_dev(w), shape, vol, wvol,
+                                  (1, n, n, n), scale, rot, opts)
+    return cpu(vol), cpu(wvol)
+
+
+@pytest.mark.parametrize("remap,interp,fb", [("h2h", 1, -1.0), ("hc2hc", 1, 0.05), ("hc2h", 0, -1.0),
+                                             ("h2hc", 1, 0.02)])
+def test_insert_staged_equals_general_path(remap, interp, fb):
+    # A batched identity scaling selects the general (one lane per voxel, every slice) kernel; no scaling selects
+    # the staged brick kernel. Both sum the same terms in the same order.
+    print(f"case insert staged-vs-general {remap} interp={interp} fb={fb}", flush=True)
+    rng = np.random.default_rng(130)
+    n, b = 48, 40
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    opts = nx().InsertCentralSlicesOptions(interp=nx().Interp(interp), windowed_sinc=nx().WindowedSinc(-1, fb))
+    eye = torch.eye(2, device=DEV).repeat(b, 1, 1)
+    v1, w1 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, None, opts)
+    v2, w2 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, eye, opts)
+    assert rel_l2(v1, v2) < 1e-6 and rel_l2(w1, w2) < 1e-6
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d(remap, s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv, interp=interp,
+                               fftfreq_blackman=fb)
+    assert rel_l2(v1, want) < 1e-6 and rel_l2(w1, wwant) < 1e-6
+
+
+def test_insert_staged_quaternion_real_and_abs2_outputs():
+    print("case insert staged quaternion / f32 / c32->f32", flush=True)
+    rng = np.random.default_rng(131)
+    n, b = 40, 25
+    fwd = random_rotations(b, rng, dtype=np.float64)
+    inv = np.ascontiguousarray(np.transpose(fwd, (0, 2, 1)))
+    q = nx().quaternion_from_matrix(inv).astype(np.float32)
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    O.insert_central_slices_3d("hc2hc", s, None, (b, 1, n, n), want, None, (1, n, n, n), None, q, quaternion=True)
+    got, _ = _insert_dev("hc2hc", s, None, (b, 1, n, n), n, q, quat=True)
+    assert rel_l2(got, want) < 1e-6
+    # complex slices into a real volume: |z|^2 of the complex sum (cast_or_abs_squared)
+    want_r = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("h2h", s, None, (b, 1, n, n), want_r, None, (1, n, n, n), None, inv.astype(np.float32))
+    got_r, _ = _insert_dev("h2h", s, None, (b, 1, n, n), n, inv.astype(np.float32), out_dtype=torch.float32)
+    assert rel_l2(got_r, want_r) < 1e-6
+    sr = rng.standard_normal((b, 1, n, n // 2 + 1)).astype(np.float32)
+    want_f = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("h2h", sr, None, (b, 1, n, n), want_f, None, (1, n, n, n), None, inv.astype(np.float32))
+    got_f, _ = _insert_dev("h2h", sr, None, (b, 1, n, n), n, inv.astype(np.float32), out_dtype=torch.float32)
+    assert rel_l2(got_f, want_f) < 1e-6
+
+
+def test_insert_many_slices_split_bricks():
+    # 1500 slices into 32^3: the bricks near the origin get several slice segments whose partial sums are
+    # reduced in segment order; compared with the oracle's strict slice order and with itself (determinism).
+    print("case insert split bricks", flush=True)
+    rng = np.random.default_rng(133)
+    n, b = 32, 1500
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("hc2hc", s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv)
+    v1, w1 = _insert_dev("hc2hc", s, w, (b, 1, n, n), n, inv)
+    v2, w2 = _insert_dev("hc2hc", s, w, (b, 1, n, n), n, inv)
+    assert rel_l2(v1, want) < 1e-6 and rel_l2(w1, wwant) < 1e-6
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(w1, w2)
+
+
+def test_insert_oversampled_and_odd_sizes():
+    # Slices larger than the volume (footprints exceed the LDS staging: global-read fallback), odd volume size,
+    # explicit target shape and a cutoff above 0.5.
+    print("case insert oversampled/odd", flush=True)
+    rng = np.random.default_rng(132)
+    n, h, b = 33, 64, 12
+    s = random_spectrum((b, 1, h, h // 2 + 1), rng)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    for target, cutoff in (((0, 0, 0, 0), 0.5), ((1, 40, 40, 40), 0.6)):
+        want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+        O.insert_central_slices_3d("hc2hc", s, None, (b, 1, h, h), want, None, (1, n, n, n), None, inv,
+                                   fftfreq_cutoff=cutoff, target_shape=target)
+        vol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+        opts = nx().InsertCentralSlicesOptions(fftfreq_cutoff=cutoff, target_shape=target)
+        nx().insert_central_slices_3d("hc2hc", to_dev(s), None, (b, 1, h, h), vol, None, (1, n, n, n), None,
+                                      to_dev(inv), opts)
+        assert rel_l2(cpu(vol), want) < 1e-6
