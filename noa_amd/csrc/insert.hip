@@ -9,16 +9,18 @@
 // the others add exactly zero. This file keeps noa's per-voxel expressions and skips the zero terms.
 //
 // Staged path (no scaling/EWS, array inputs, single-precision values):
-//   - Work item = (16^3 brick of the volume, range of slices). A workgroup of 4 waves owns the brick's 4096
-//     voxels in an LDS accumulator: no atomics.
-//   - Each wave scans the item's slices 64 at a time: lane l tests whether the slab of slice s0+l intersects
-//     the brick box; a ballot yields the passing slices, processed in increasing slice order.
-//   - For a passing slice, the brick's footprint on the slice (the projection of the brick box, plus its
-//     Hermitian mirror when it crosses x=0) is copied to LDS with coalesced row loads, zero outside the
-//     slice's frequency bounds.
-//   - The brick voxels inside the slab are enumerated along the volume axis closest to the slab normal: each
-//     lane owns one column of the 16x16 face and tests only the few candidates (exact noa expressions); hits
-//     interpolate from LDS and accumulate in LDS (one owner per voxel per slice; barriers order the slices).
+//   - Work item = (16^3 brick of the volume, range of slices). A workgroup owns the brick's 4096 voxels in an LDS
+//     accumulator: no atomics.
+//   - Each wave scans the item's slices 64 at a time: lane l tests whether the slab of slice s0+l intersects the
+//     brick box; a ballot yields the passing slices, processed in increasing slice order. The rotations of the
+//     next 64 slices are loaded while the current ones are consumed.
+//   - For a passing slice, the brick's footprint on the slice (the projection of the brick box, plus its Hermitian
+//     mirror when it crosses x=0) is copied to LDS with coalesced row loads, zero outside the slice's frequency
+//     bounds. The footprint of the next passing slice is loaded into registers while the current one is used.
+//   - The brick voxels near the slab are enumerated along the volume axis closest to the slab normal: each lane of
+//     a wave counts the candidates of one column of the 16x16 face, a wave-wide prefix sum packs the candidates of
+//     the wave's columns onto its 64 lanes, and each candidate runs noa's exact per-voxel tests; a hit interpolates
+//     the slice from LDS and accumulates in LDS (one owner per voxel per slice; barriers order the slices).
 //   - Bricks near the origin meet most slices; their slice range is cut into segments so that work items are
 //     balanced. A brick with one segment adds its sum to the volume directly; a split brick writes one partial
 //     sum per segment, and a second kernel adds the partials of each brick in segment order. The result is
@@ -30,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -38,10 +41,12 @@
 namespace noa_hip {
 
 constexpr i32 BE = 16;                 // brick edge (voxels)
-constexpr i32 NT = 512;                // threads per workgroup (two per column of the brick face)
 constexpr i32 AP = 17;                 // LDS accumulator pitch along x (bank-conflict padding)
 constexpr i32 ACC = BE * BE * AP;      // accumulator slots
 constexpr i32 RS = 32;                 // staged region capacity along y and x (pixels), LDS row pitch
+constexpr i32 NCOL = BE * BE;          // columns of a brick face
+constexpr i32 NWIN = 12;               // degree of the window polynomial
+constexpr i32 NT_REDUCE = 256;         // threads per workgroup of the partial-sum reduction
 
 template<class C, class In, class W>
 struct InsertParams {
@@ -63,16 +68,23 @@ struct InsertParams {
     C ews[2];
     XformArg<C> rot, scale;
     bool general, slices_centered, vol_centered;
-    C inv_target[3];      // 1 / target, used only where target is a power of two (then x / t == x * (1 / t))
-    bool pow2[3];
-    float wc[32];         // Chebyshev coefficients of the windowed sinc (single-precision staged path)
-    i32 wn;               // their degree
+    C inv_target[3];      // 1 / target
+    bool pow2;            // every target is a power of two (then x / t == x * (1 / t) exactly)
+    float wa[NWIN + 1];   // windowed sinc as a polynomial in t = 2 (fz/f_b)^2 - 1 (single-precision staged path)
+    bool wpoly;           // use wa; otherwise evaluate noa's expression
     float inv_fb;         // 1 / fftfreq_blackman
 };
 
-// Frequency of index `i` along volume axis `a` (0: z, 1: y, 2: x = rfft half, index == frequency).
-__device__ __forceinline__ i32 axis_freq(i32 a, i32 i, bool centered, const i32 grid[3]) {
-    return a == 2 ? i : index2frequency(centered, i, grid[a]);
+// Keeps a value in vector registers: the compiler would otherwise re-read kernel arguments inside the loops,
+// and each scalar load there makes the wave wait for all of its outstanding LDS operations.
+template<class T> __device__ __forceinline__ T pin(T v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Frequency of index `i` along a volume axis; `is_x`: the rfft half axis (index == frequency).
+__device__ __forceinline__ i32 axis_freq(bool is_x, i32 i, bool centered, i32 n) {
+    return is_x ? i : index2frequency(centered, i, n);
 }
 
 // The 3x3 matrix of a rotation (for quaternions: Quaternion::to_matrix, Quaternion.hpp:69-94); only used to
@@ -93,39 +105,25 @@ __device__ __forceinline__ void rot_matrix(const Rot<C>& r, C m[9]) {
     m[6] = txz + twy; m[7] = txy - twz; m[8] = 1 - (tyy + tzz);
 }
 
-struct Region { i32 y0, x0, h, w; };
-
 __device__ __forceinline__ i32 acc_index(i32 lz, i32 ly, i32 lx) { return (lz * BE + ly) * AP + lx; }
 
-// Windowed sinc of the staged kernel in single precision: a Chebyshev expansion in t = 2 (fz/f_b)^2 - 1 fitted on
-// the host per (f_s, f_b) to an absolute error below 2e-9 on [-f_b, f_b] (noa's FP32 expression itself carries
-// ~1e-7 of rounding). Evaluated with Clenshaw's recurrence.
-__device__ __forceinline__ float window_poly(float fz, float inv_fb, const float* c, i32 n) {
+// Windowed sinc of the staged kernel in single precision: a polynomial of degree NWIN in t = 2 (fz/f_b)^2 - 1
+// (the window is even in fz), fitted on the host per (f_s, f_b) and accepted only when its error, evaluated
+// in single precision, stays below 4e-7 on [-f_b, f_b] (noa's FP32 expression itself carries ~2e-7 of rounding).
+__device__ __forceinline__ float window_poly(float fz, float inv_fb, const float (&a)[NWIN + 1]) {
     const float q = fz * inv_fb;
-    const float t = 2.f * (q * q) - 1.f;
-    const float t2 = 2.f * t;
-    float b1 = 0.f, b2 = 0.f;
-    for (i32 k = n; k >= 1; --k) {
-        const float b0 = fmaf(t2, b1, c[k] - b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    return fmaf(t, b1, c[0] - b2);
+    const float t = fmaf(2.f * q, q, -1.f);
+    float r = a[NWIN];
+#pragma unroll
+    for (i32 k = NWIN - 1; k >= 0; --k) r = fmaf(r, t, a[k]);
+    return r;
 }
 
-template<class C>
-__device__ __forceinline__ C window_of(const C fz, const C fs, const C fb, const float* wc, i32 wn, float inv_fb) {
-    if constexpr (sizeof(C) == 4) {
-        if (wn < 0) return windowed_sinc<C>(fz, fs, fb);
-        return window_poly(fz, inv_fb, wc, wn);
-    } else {
-        return windowed_sinc<C>(fz, fs, fb);
-    }
-}
-
-// Footprint regions of a brick box on one slice (A: px >= 0, B: px < 0 mirrored), see the kernel below.
+// Footprint of a brick box on one slice. Region A covers px >= 0; region B the mirrored part px < 0 (its x0 is
+// 0). Each region is at most RS x RS pixels when `staged`.
 struct Footprint {
-    Region reg[2];
+    i32 ay0, ax0, ah, aw;
+    i32 by0, bh, bw;
     bool use_a, use_b, staged;
 };
 
@@ -137,19 +135,58 @@ __device__ __forceinline__ Footprint footprint(const C m[9], const C cen[3], con
     const C hx = (fabs(m[6]) * half[0] + fabs(m[7]) * half[1] + fabs(m[8]) * half[2]) * ws;
     // One extra pixel on each side absorbs the rounding difference with the exact per-voxel positions.
     Footprint fp;
-    fp.reg[0].y0 = static_cast<i32>(floor(pyc - hy)) - 1;
-    fp.reg[0].h = static_cast<i32>(floor(pyc + hy)) + 3 - fp.reg[0].y0;
-    fp.reg[0].x0 = max(0, static_cast<i32>(floor(pxc - hx)) - 1);
-    fp.reg[0].w = static_cast<i32>(floor(pxc + hx)) + 3 - fp.reg[0].x0;
-    fp.reg[1].y0 = static_cast<i32>(floor(-(pyc + hy))) - 1;
-    fp.reg[1].h = static_cast<i32>(floor(-(pyc - hy))) + 3 - fp.reg[1].y0;
-    fp.reg[1].x0 = 0;
-    fp.reg[1].w = max(1, static_cast<i32>(floor(-(pxc - hx))) + 3);
+    fp.ay0 = static_cast<i32>(floor(pyc - hy)) - 1;
+    fp.ah = static_cast<i32>(floor(pyc + hy)) + 3 - fp.ay0;
+    fp.ax0 = max(0, static_cast<i32>(floor(pxc - hx)) - 1);
+    fp.aw = static_cast<i32>(floor(pxc + hx)) + 3 - fp.ax0;
+    fp.by0 = static_cast<i32>(floor(-(pyc + hy))) - 1;
+    fp.bh = static_cast<i32>(floor(-(pyc - hy))) + 3 - fp.by0;
+    fp.bw = max(1, static_cast<i32>(floor(-(pxc - hx))) + 3);
     fp.use_a = pxc + hx >= C(-0.5);
     fp.use_b = pxc - hx < C(0.5);
-    fp.staged = (!fp.use_a || (fp.reg[0].h <= RS && fp.reg[0].w <= RS)) &&
-                (!fp.use_b || (fp.reg[1].h <= RS && fp.reg[1].w <= RS));
+    fp.staged = (!fp.use_a || (fp.ah <= RS && fp.aw <= RS)) && (!fp.use_b || (fp.bh <= RS && fp.bw <= RS));
     return fp;
+}
+
+// What a (brick, slice) pair needs besides the rotation: the footprint, and the enumeration along the volume axis A
+// closest to the slab normal (al = normal in index units; B and Cc: the face axes, B < Cc).
+template<class C>
+struct PairInfo {
+    Footprint fp;
+    i32 flags;            // 1: use_a, 2: use_b, 4: staged, bits 3-4: axis A, 32: candidate range from the slab
+    C inv_alA, alB, alC;
+};
+
+template<class C>
+__device__ __forceinline__ PairInfo<C> pair_info(const C m[9], const C cen[3], const C half[3], C hs, C ws, C it0,
+                                                 C it1, C it2, const bool contiguous[3]) {
+    PairInfo<C> r;
+    r.fp = footprint(m, cen, half, hs, ws);
+    const C al0 = m[0] * it0, al1 = m[1] * it1, al2 = m[2] * it2;
+    i32 ax = 0;
+    if (fabs(al1) > fabs(al0)) ax = 1;
+    if (fabs(al2) > fabs(ax == 0 ? al0 : al1)) ax = 2;
+    const C alA = ax == 0 ? al0 : (ax == 1 ? al1 : al2);
+    r.alB = ax == 0 ? al1 : al0;
+    r.alC = ax == 2 ? al1 : al2;
+    const bool contig = ax == 0 ? contiguous[0] : (ax == 1 ? contiguous[1] : contiguous[2]);
+    const bool ranged = contig && fabs(alA) > C(1e-20);
+    r.inv_alA = ranged ? C(1) / alA : C(0);
+    r.flags = (r.fp.use_a ? 1 : 0) | (r.fp.use_b ? 2 : 0) | (r.fp.staged ? 4 : 0) | (ax << 3) | (ranged ? 32 : 0);
+    return r;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: shifts by 1, 2, 4, 8 inside each row of 16
+// lanes (lanes whose source falls outside the row add 0), then the last lane of rows 0 and 2 is added to rows 1
+// and 3 (row_bcast:15), and lane 31 to rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ i32 wave_inclusive_sum(i32 x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return x;
 }
 
 template<class C> __device__ __forceinline__ C bcast(C v, i32 lane) {
@@ -163,11 +200,13 @@ template<class C> __device__ __forceinline__ C bcast(C v, i32 lane) {
     }
 }
 
-template<class C, class In, class Out, class W, int INTERP>
+template<class C, class In, class Out, class W, int INTERP, int NT>
 __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C, In, W> p) {
     using RIn = typename vt<In>::real;
     constexpr i32 SLOTS = RS * RS / NT;            // staged elements per thread and region
-    constexpr i32 NCOL = BE * BE;                  // columns of the brick face
+    constexpr i32 NWAVE = NT / 64;
+    constexpr i32 CPW = NCOL / NWAVE;              // face columns per wave
+    static_assert(CPW <= 64 && NCOL % NWAVE == 0 && (RS * RS) % NT == 0, "launch shape");
     __shared__ In acc_v[ACC];
     __shared__ W acc_w[ACC];
     __shared__ In st_v[2][RS * RS];
@@ -175,6 +214,7 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
 
     const i32 tid = static_cast<i32>(threadIdx.x);
     const i32 lane = tid & 63;
+    const i32 wave = tid >> 6;
     const int4 item = p.items[blockIdx.x];
     const i32 brick = item.x;
     const i32 bxi = brick % p.nb[2];
@@ -182,7 +222,9 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     const i32 byi = rem % p.nb[1];
     const i32 bzi = rem / p.nb[1];
     const i32 org[3] = {bzi * BE, byi * BE, bxi * BE};
+    const i32 grid[3] = {p.grid[0], p.grid[1], p.grid[2]};
     const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
+    const bool vc = p.vol_centered;
 
     for (i32 v = tid; v < ACC; v += NT) {
         acc_v[v] = vzero<In>();
@@ -190,7 +232,7 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     }
 
     // Brick box in normalised frequencies, and whether each axis is free of the non-centred wrap (then the
-    // frequencies of the brick along it are org-frequency + k).
+    // frequencies of the brick along it are first-frequency + k).
     C cen[3], half[3];
     i32 f0[3], n_in[3];
     bool contiguous[3];
@@ -198,11 +240,11 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     for (int a = 0; a < 3; ++a) {
         i32 lo = 0x7fffffff, hi = -0x7fffffff, n = 0;
         bool contig = true;
-        const i32 first = axis_freq(a, org[a], p.vol_centered, p.grid);
+        const i32 first = axis_freq(a == 2, org[a], vc, grid[a]);
         for (i32 k = 0; k < BE; ++k) {
             const i32 idx = org[a] + k;
             if (idx >= ext[a]) break;
-            const i32 f = axis_freq(a, idx, p.vol_centered, p.grid);
+            const i32 f = axis_freq(a == 2, idx, vc, grid[a]);
             contig = contig && (f == first + k);
             lo = min(lo, f);
             hi = max(hi, f);
@@ -218,35 +260,82 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     const i32 Hs = p.slice_shape[0], Ws = p.slice_shape[1];
     const i32 lo_y = -Hs / 2, hi_y = (Hs - 1) / 2, hi_x = Ws / 2;
 
-    // Iterator over the passing slices of the item, in increasing order. Every wave scans the same 64 slices:
-    // lane l reads the rotation of slice scan0+l once, tests its slab against the brick box, and the rotation of
-    // a passing slice is later broadcast from that lane (no scalar memory traffic per slice).
-    i32 scan0 = item.y - 64;
-    unsigned long long mask = 0;
-    Rot<C> scan_rot{};
-    scan_rot.quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
-    auto next_slice = [&](Rot<C>& out) -> i32 {
-        while (mask == 0) {
-            scan0 += 64;
-            if (scan0 >= item.z) return -1;
-            const i32 s = scan0 + lane;
-            bool pass = false;
-            if (s < item.z) {
-                scan_rot = load_rotation(p.rot, s);
-                C m[9];
-                rot_matrix(scan_rot, m);
-                const C d = fabs(m[0] * cen[0] + m[1] * cen[1] + m[2] * cen[2]);
-                const C bound = p.fftfreq_blackman + fabs(m[0]) * half[0] + fabs(m[1]) * half[1] + fabs(m[2]) * half[2];
-                pass = d <= bound + slack;
-            }
-            mask = __ballot(pass);
-        }
-        const i32 k = __ffsll(mask) - 1;
-        mask &= mask - 1;
-        out.quat = scan_rot.quat;
+    // Values of the hot loop, held in vector registers.
+    const C fb = pin(p.fftfreq_blackman);
+    const C fs = pin(p.fftfreq_sinc);
+    const C cutoff_sqd = pin(p.cutoff_sqd);
+    const C hs_f = pin(p.slice_f[0]), ws_f = pin(p.slice_f[1]);
+    const C t0 = pin(p.target[0]), t1 = pin(p.target[1]), t2 = pin(p.target[2]);
+    const C it0 = pin(p.inv_target[0]), it1 = pin(p.inv_target[1]), it2 = pin(p.inv_target[2]);
+    const bool pow2 = p.pow2;
+    const bool wpoly = p.wpoly;
+    const bool has_wvol = p.has_wvol;
+    const bool win_present = p.win_present;
+    float wa[NWIN + 1];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) out.m[q] = bcast(scan_rot.m[q], k);
-        return scan0 + k;
+    for (int k = 0; k <= NWIN; ++k) wa[k] = pin(p.wa[k]);
+    const float inv_fb = pin(p.inv_fb);
+
+    // Iterator over the passing slices of the item, in increasing order. Every wave scans the same 64 slices:
+    // lane l holds slice chunk0+l, tests its slab against the brick box and prepares what the pair (brick, slice)
+    // needs (footprint, enumeration axis); a passing slice's values are then broadcast from that lane. The next
+    // chunk's rotations are loaded one chunk ahead.
+    const bool quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
+    auto scan_load = [&](i32 s0, Rot<C>& r) {
+        const i32 s = s0 + lane;
+        if (s < item.z) r = load_rotation(p.rot, s);
+    };
+    i32 chunk0 = item.y;
+    Rot<C> chunk{}, ahead{};
+    chunk.quat = quat;
+    ahead.quat = quat;
+    PairInfo<C> mine{};       // this lane's slice of the current chunk
+    unsigned long long mask = 0;
+    auto scan_prepare = [&]() {
+        bool pass = false;
+        if (chunk0 + lane < item.z) {
+            C m[9];
+            rot_matrix(chunk, m);
+            const C d = fabs(m[0] * cen[0] + m[1] * cen[1] + m[2] * cen[2]);
+            const C bound = fb + fabs(m[0]) * half[0] + fabs(m[1]) * half[1] + fabs(m[2]) * half[2];
+            pass = d <= bound + slack;
+            mine = pair_info(m, cen, half, hs_f, ws_f, it0, it1, it2, contiguous);
+        }
+        mask = __ballot(pass);
+    };
+    scan_load(chunk0, chunk);
+    scan_load(chunk0 + 64, ahead);
+    scan_prepare();
+    auto next_slice = [&](Rot<C>& rot_out, PairInfo<C>& out) -> i32 {
+        while (mask == 0) {
+            chunk0 += 64;
+            if (chunk0 >= item.z) return -1;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) chunk.m[q] = ahead.m[q];
+            scan_load(chunk0 + 64, ahead);
+            scan_prepare();
+        }
+        const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
+        mask &= mask - 1;
+        rot_out.quat = quat;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) rot_out.m[q] = bcast(chunk.m[q], k);
+        out.fp.ay0 = __builtin_amdgcn_readlane(mine.fp.ay0, k);
+        out.fp.ax0 = __builtin_amdgcn_readlane(mine.fp.ax0, k);
+        out.fp.ah = __builtin_amdgcn_readlane(mine.fp.ah, k);
+        out.fp.aw = __builtin_amdgcn_readlane(mine.fp.aw, k);
+        out.fp.by0 = __builtin_amdgcn_readlane(mine.fp.by0, k);
+        out.fp.bh = __builtin_amdgcn_readlane(mine.fp.bh, k);
+        out.fp.bw = __builtin_amdgcn_readlane(mine.fp.bw, k);
+        const i32 flags = __builtin_amdgcn_readlane(mine.flags, k);
+        out.flags = flags;
+        out.fp.use_a = flags & 1;
+        out.fp.use_b = flags & 2;
+        out.fp.staged = flags & 4;
+        out.inv_alA = bcast(mine.inv_alA, k);
+        out.alB = bcast(mine.alB, k);
+        out.alC = bcast(mine.alC, k);
+        return chunk0 + k;
     };
 
     // Loads the staged elements of slice i into registers (issued ahead of their use). Thread t owns column
@@ -254,54 +343,51 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     In rv[2][SLOTS];
     W rw[2][SLOTS];
     const i32 my_col = tid % RS, my_row = tid / RS;
-    auto load_footprint = [&](i32 i, const Footprint& f) {
-        const In* __restrict__ sl = p.in + static_cast<i64>(i) * p.is_b;
-        const W* __restrict__ wsl = p.win_present ? p.win + static_cast<i64>(i) * p.wis_b : nullptr;
+    auto load_region = [&](const In* __restrict__ sl, const W* __restrict__ wsl, i32 y0, i32 x0, i32 h, i32 w,
+                           In (&v)[SLOTS], W (&wv)[SLOTS]) {
+        const i32 xf = x0 + my_col;
+        const bool col_ok = my_col < w && xf <= hi_x;
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-#pragma unroll
-            for (int j = 0; j < SLOTS; ++j) {
-                rv[r][j] = vzero<In>();
-                rw[r][j] = W(0);
-            }
-            if (!f.staged || !(r == 0 ? f.use_a : f.use_b)) continue;   // uniform
-            const Region g = f.reg[r];
-            const i32 xf = g.x0 + my_col;
-            if (my_col >= g.w || xf > hi_x) continue;
-#pragma unroll
-            for (int j = 0; j < SLOTS; ++j) {
-                const i32 row = my_row + j * (NT / RS);
-                const i32 yf = g.y0 + row;
-                if (row < g.h && yf >= lo_y && yf <= hi_y) {
-                    const i32 off = frequency2index(p.slices_centered, yf, Hs);
-                    rv[r][j] = sl[off * p.is_y + xf * p.is_x];
-                    if (wsl) rw[r][j] = wsl[off * p.wis_y + xf * p.wis_x];
-                }
+        for (int j = 0; j < SLOTS; ++j) {
+            const i32 row = my_row + j * (NT / RS);
+            const i32 yf = y0 + row;
+            v[j] = vzero<In>();
+            wv[j] = W(0);
+            if (col_ok && row < h && yf >= lo_y && yf <= hi_y) {
+                const i32 off = frequency2index(p.slices_centered, yf, Hs);
+                v[j] = sl[off * p.is_y + xf * p.is_x];
+                if (wsl) wv[j] = wsl[off * p.wis_y + xf * p.wis_x];
             }
         }
     };
+    auto load_footprint = [&](i32 i, const Footprint& f) {
+        if (!f.staged) return;
+        const In* __restrict__ sl = p.in + static_cast<i64>(i) * p.is_b;
+        const W* __restrict__ wsl = win_present ? p.win + static_cast<i64>(i) * p.wis_b : nullptr;
+        if (f.use_a) load_region(sl, wsl, f.ay0, f.ax0, f.ah, f.aw, rv[0], rw[0]);
+        if (f.use_b) load_region(sl, wsl, f.by0, 0, f.bh, f.bw, rv[1], rw[1]);
+    };
+
+    __shared__ i32 cand[NWAVE][64];    // packed candidates of one pass of each wave: column-in-wave << 4 | k
+    i32* const my_cand = cand[wave];
 
     Rot<C> rot{};
-    i32 cur = next_slice(rot);
-    Footprint fp{};
-    C m[9];
-    if (cur >= 0) {
-        rot_matrix(rot, m);
-        fp = footprint(m, cen, half, p.slice_f[0], p.slice_f[1]);
-        load_footprint(cur, fp);
-    }
-    // Column of the brick face owned by this thread, and which candidates (even or odd) it takes.
-    const i32 col_id = tid % NCOL, parity = tid / NCOL;
+    PairInfo<C> pi{};
+    i32 cur = next_slice(rot, pi);
+    if (cur >= 0) load_footprint(cur, pi.fp);
     while (cur >= 0) {
+        const Footprint fp = pi.fp;
         // Publish the staged footprint of the current slice.
         if (fp.staged) {
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                if (!(r == 0 ? fp.use_a : fp.use_b)) continue;
-#pragma unroll
-                for (int j = 0; j < SLOTS; ++j) {
-                    st_v[r][tid + j * NT] = rv[r][j];
-                    st_w[r][tid + j * NT] = rw[r][j];
+            for (int j = 0; j < SLOTS; ++j) {
+                if (fp.use_a) {
+                    st_v[0][tid + j * NT] = rv[0][j];
+                    st_w[0][tid + j * NT] = rw[0][j];
+                }
+                if (fp.use_b) {
+                    st_v[1][tid + j * NT] = rv[1][j];
+                    st_w[1][tid + j * NT] = rw[1][j];
                 }
             }
         }
@@ -309,123 +395,150 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
 
         // Prefetch the next passing slice while the current one is processed.
         Rot<C> rot_n{};
-        const i32 nxt = next_slice(rot_n);
-        Footprint fp_n{};
-        C m_n[9];
-        if (nxt >= 0) {
-            rot_matrix(rot_n, m_n);
-            fp_n = footprint(m_n, cen, half, p.slice_f[0], p.slice_f[1]);
-            load_footprint(nxt, fp_n);
-        }
+        PairInfo<C> pi_n{};
+        const i32 nxt = next_slice(rot_n, pi_n);
+        if (nxt >= 0) load_footprint(nxt, pi_n.fp);
 
-        // Enumerate the brick voxels near the slab along the axis closest to the normal.
-        const C al[3] = {m[0] / p.target[0], m[1] / p.target[1], m[2] / p.target[2]};
-        i32 ax = 0;
-        if (fabs(al[1]) > fabs(al[ax])) ax = 1;
-        if (fabs(al[2]) > fabs(al[ax])) ax = 2;
-        const i32 bx_ = ax == 0 ? 1 : 0;   // face axes, the faster one (c) last
-        const i32 cx_ = ax == 2 ? 1 : 2;
-        const i32 gb = org[bx_] + col_id / BE;
-        const i32 gc = org[cx_] + col_id % BE;
-        if (gb < ext[bx_] && gc < ext[cx_]) {
-            const i32 fbq = axis_freq(bx_, gb, p.vol_centered, p.grid);
-            const i32 fcq = axis_freq(cx_, gc, p.vol_centered, p.grid);
-            const C beta = al[bx_] * static_cast<C>(fbq) + al[cx_] * static_cast<C>(fcq);
-            // Candidates along the axis: |al[ax] * f + beta| <= f_b in exact arithmetic, widened by eps samples to
-            // cover the rounding of the exact per-voxel test below (a few ulps of |f| <= 1, divided by
-            // |al[ax]| >= 1/(sqrt(3) T)).
-            i32 k_lo = 0, k_hi = n_in[ax] - 1;
-            if (contiguous[ax] && fabs(al[ax]) > C(1e-20)) {
-                const C eps = C(1e-3) + C(8e-6) * p.target[ax];
-                C lo = (-p.fftfreq_blackman - beta) / al[ax];
-                C hi = (p.fftfreq_blackman - beta) / al[ax];
-                if (lo > hi) { const C t = lo; lo = hi; hi = t; }
-                const C klo = ceil(lo - eps) - static_cast<C>(f0[ax]);
-                const C khi = floor(hi + eps) - static_cast<C>(f0[ax]);
-                k_lo = static_cast<i32>(fmin(fmax(klo, C(0)), static_cast<C>(n_in[ax])));
-                k_hi = static_cast<i32>(fmax(fmin(khi, static_cast<C>(n_in[ax] - 1)), C(-1)));
-            }
-            // The normalised frequency as noa computes it: index2frequency / target, with the division replaced by
-            // the (exact) multiplication by 1/target when target is a power of two.
-            auto nfreq = [&](i32 a, i32 fr) -> C {
-                return p.pow2[a] ? static_cast<C>(fr) * p.inv_target[a] : static_cast<C>(fr) / p.target[a];
-            };
-            const C fbv = nfreq(bx_, fbq);
-            const C fcv = nfreq(cx_, fcq);
-            const In* __restrict__ sl = p.in + static_cast<i64>(cur) * p.is_b;
-            const W* __restrict__ wsl = p.win_present ? p.win + static_cast<i64>(cur) * p.wis_b : nullptr;
-            for (i32 kk = k_lo + parity; kk <= k_hi; kk += 2) {
-                C fv[3];
-                fv[ax] = nfreq(ax, axis_freq(ax, org[ax] + kk, p.vol_centered, p.grid));
-                fv[bx_] = fbv;
-                fv[cx_] = fcv;
-                vec3<C> f;
-                f.z = fv[0]; f.y = fv[1]; f.x = fv[2];
-                // The exact tests of noa (:479-494).
-                if (dot3(f) > p.cutoff_sqd) continue;
-                Scale<C> sc;
-                sc.identity = true;
-                vec2<C> f2;
-                const C fz = grid2slice<false>(f, sc, rot, C(0), C(0), f2);
-                if (!(fabs(fz) <= p.fftfreq_blackman)) continue;
-                const C window = window_of<C>(fz, p.fftfreq_sinc, p.fftfreq_blackman, p.wc, p.wn, p.inv_fb);
-                C py = f2.y * p.slice_f[0], px = f2.x * p.slice_f[1];
-                In v;
-                W wv = W(0);
-                if (fp.staged) {
-                    RIn conj = 1;
-                    i32 r = 0;
-                    if (px < 0) { py *= -1; px *= -1; conj = -1; r = 1; }
-                    const Region g = fp.reg[r];
-                    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
-                        const i32 yy = static_cast<i32>(round(py)) - g.y0, xx = static_cast<i32>(round(px)) - g.x0;
-                        const i32 o = min(max(yy * RS + xx, 0), RS * RS - 1);
-                        v = vadd(vzero<In>(), vmul(st_v[r][o], RIn(1)));
-                        wv = W(0) + st_w[r][o] * W(1);
-                        (void)conj;   // NEAREST does not conjugate after the flip (Interpolation.hpp:479-484)
-                    } else {
-                        const C fly = floor(py), flx = floor(px);
-                        const i32 y0 = static_cast<i32>(fly) - g.y0, x0 = static_cast<i32>(flx) - g.x0;
-                        const RIn ty = static_cast<RIn>(py - fly), tx = static_cast<RIn>(px - flx);
-                        const RIn wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
-                        // The margins of the footprint keep every tap inside the region; the clamp only guards the
-                        // LDS index.
-                        const i32 o0 = min(max(y0 * RS + x0, 0), RS * RS - RS - 2);
-                        v = vzero<In>();
-#pragma unroll
-                        for (int yy = 0; yy < 2; ++yy) {
-                            In vy = vzero<In>();
-                            W wy_ = W(0);
-#pragma unroll
-                            for (int xx = 0; xx < 2; ++xx) {
-                                const i32 o = o0 + yy * RS + xx;
-                                vy = vadd(vy, vmul(st_v[r][o], wx[xx]));
-                                wy_ = wy_ + st_w[r][o] * static_cast<W>(wx[xx]);
-                            }
-                            v = vadd(v, vmul(vy, wy[yy]));
-                            wv = wv + wy_ * static_cast<W>(wy[yy]);
-                        }
-                        vimag_mul(v, conj);
-                    }
-                } else {
-                    v = interp2<INTERP, In>(sl, p.is_y, p.is_x, Hs, Ws, p.slices_centered, py, px);
-                    if (wsl) wv = interp2<INTERP, W>(wsl, p.wis_y, p.wis_x, Hs, Ws, p.slices_centered, py, px);
+        // Enumeration axis A: the volume axis closest to the slab normal (in index units). B and Cc span the face.
+        const i32 ax = (pi.flags >> 3) & 3;
+        const i32 orgB = ax == 0 ? org[1] : org[0];
+        const i32 orgC = ax == 2 ? org[1] : org[2];
+        const i32 gridB = ax == 0 ? grid[1] : grid[0];
+        const i32 gridC = grid[1];                                  // unused when Cc is x
+        const bool xC = ax != 2;
+        const i32 extB = ax == 0 ? ext[1] : ext[0];
+        const i32 extC = ax == 2 ? ext[1] : ext[2];
+        const i32 f0A = ax == 0 ? f0[0] : (ax == 1 ? f0[1] : f0[2]);
+        const i32 nA = ax == 0 ? n_in[0] : (ax == 1 ? n_in[1] : n_in[2]);
+        const C tA = ax == 0 ? t0 : (ax == 1 ? t1 : t2);
+        const bool ranged = pi.flags & 32;
+
+        // Candidates of this lane's column: |alA * f + beta| <= f_b in exact arithmetic, widened by eps samples to
+        // cover the rounding of the exact per-voxel test (a few ulps of |f| <= 1, divided by |alA| >= 1/(sqrt(3) T)).
+        i32 cnt = 0, k_lo = 0;
+        if (lane < CPW) {
+            const i32 col = wave * CPW + lane;
+            const i32 gb = orgB + col / BE, gc = orgC + col % BE;
+            if (gb < extB && gc < extC) {
+                i32 k_hi = nA - 1;
+                if (ranged) {
+                    const i32 fbq = index2frequency(vc, gb, gridB);
+                    const i32 fcq = axis_freq(xC, gc, vc, gridC);
+                    const C beta = pi.alB * static_cast<C>(fbq) + pi.alC * static_cast<C>(fcq);
+                    const C eps = C(1e-3) + C(8e-6) * tA;
+                    C lo = (-fb - beta) * pi.inv_alA;
+                    C hi = (fb - beta) * pi.inv_alA;
+                    if (lo > hi) { const C t = lo; lo = hi; hi = t; }
+                    const C klo = ceil(lo - eps) - static_cast<C>(f0A);
+                    const C khi = floor(hi + eps) - static_cast<C>(f0A);
+                    k_lo = static_cast<i32>(fmin(fmax(klo, C(0)), static_cast<C>(nA)));
+                    k_hi = static_cast<i32>(fmax(fmin(khi, static_cast<C>(nA - 1)), C(-1)));
                 }
-                i32 li[3];
-                li[ax] = kk; li[bx_] = gb - org[bx_]; li[cx_] = gc - org[cx_];
-                const i32 lv = acc_index(li[0], li[1], li[2]);
-                acc_v[lv] = vadd(acc_v[lv], vmul(v, static_cast<RIn>(window)));
-                if (p.has_wvol)
-                    acc_w[lv] = acc_w[lv] + (p.win_present ? wv * static_cast<W>(window) : static_cast<W>(window));
+                cnt = max(0, k_hi - k_lo + 1);
             }
+        }
+        const i32 incl = wave_inclusive_sum(cnt);
+        const i32 total = __builtin_amdgcn_readlane(incl, 63);
+        const i32 excl = incl - cnt;
+
+        const In* __restrict__ sl = p.in + static_cast<i64>(cur) * p.is_b;
+        const W* __restrict__ wsl = win_present ? p.win + static_cast<i64>(cur) * p.wis_b : nullptr;
+        for (i32 base = 0; base < total; base += 64) {
+            // Pack the candidates [base, base + 64) of the wave into the list, one per lane.
+            __builtin_amdgcn_wave_barrier();
+            {
+                const i32 i0 = max(0, base - excl), i1 = min(cnt, base + 64 - excl);
+                for (i32 i = i0; i < i1; ++i) my_cand[excl + i - base] = (lane << 4) | (k_lo + i);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const i32 e = my_cand[lane];
+            if (base + lane >= total) continue;
+            const i32 k = e & 15;
+            const i32 col = wave * CPW + ((e >> 4) & 63);
+            const i32 lb = col / BE, lc = col % BE;
+            const i32 lz = ax == 0 ? k : lb;
+            const i32 ly = ax == 0 ? lb : (ax == 1 ? k : lc);
+            const i32 lx = ax == 2 ? k : lc;
+            const i32 frz = index2frequency(vc, org[0] + lz, grid[0]);
+            const i32 fry = index2frequency(vc, org[1] + ly, grid[1]);
+            const i32 frx = org[2] + lx;
+            // The normalised frequency as noa computes it: index2frequency / target (:479-484).
+            vec3<C> f;
+            if (pow2) {
+                f.z = static_cast<C>(frz) * it0; f.y = static_cast<C>(fry) * it1; f.x = static_cast<C>(frx) * it2;
+            } else {
+                f.z = static_cast<C>(frz) / t0; f.y = static_cast<C>(fry) / t1; f.x = static_cast<C>(frx) / t2;
+            }
+            // The exact tests of noa (:479-494).
+            if (dot3(f) > cutoff_sqd) continue;
+            Scale<C> sc;
+            sc.identity = true;
+            vec2<C> f2;
+            const C fz = grid2slice<false>(f, sc, rot, C(0), C(0), f2);
+            if (!(fabs(fz) <= fb)) continue;
+            C window;
+            if constexpr (sizeof(C) == 4) {
+                window = wpoly ? window_poly(fz, inv_fb, wa) : windowed_sinc<C>(fz, fs, fb);
+            } else {
+                window = windowed_sinc<C>(fz, fs, fb);
+            }
+            C py = f2.y * hs_f, px = f2.x * ws_f;
+            In v;
+            W wv = W(0);
+            if (fp.staged) {
+                RIn conj = 1;
+                const bool mirror = px < 0;
+                if (mirror) { py *= -1; px *= -1; conj = -1; }
+                const i32 gy0 = mirror ? fp.by0 : fp.ay0;
+                const i32 gx0 = mirror ? 0 : fp.ax0;
+                const In* __restrict__ sv = mirror ? st_v[1] : st_v[0];
+                const W* __restrict__ sw = mirror ? st_w[1] : st_w[0];
+                if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+                    const i32 yy = static_cast<i32>(round(py)) - gy0, xx = static_cast<i32>(round(px)) - gx0;
+                    const i32 o = min(max(yy * RS + xx, 0), RS * RS - 1);
+                    v = vadd(vzero<In>(), vmul(sv[o], RIn(1)));
+                    wv = W(0) + sw[o] * W(1);
+                    (void)conj;   // NEAREST does not conjugate after the flip (Interpolation.hpp:479-484)
+                } else {
+                    const C fly = floor(py), flx = floor(px);
+                    const i32 y0 = static_cast<i32>(fly) - gy0, x0 = static_cast<i32>(flx) - gx0;
+                    const RIn ty = static_cast<RIn>(py - fly), tx = static_cast<RIn>(px - flx);
+                    const RIn wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
+                    // The margins of the footprint keep every tap inside the region; the clamp only guards the
+                    // LDS index.
+                    const i32 o0 = min(max(y0 * RS + x0, 0), RS * RS - RS - 2);
+                    v = vzero<In>();
+#pragma unroll
+                    for (int yy = 0; yy < 2; ++yy) {
+                        In vy = vzero<In>();
+                        W wy_ = W(0);
+#pragma unroll
+                        for (int xx = 0; xx < 2; ++xx) {
+                            const i32 o = o0 + yy * RS + xx;
+                            vy = vadd(vy, vmul(sv[o], wx[xx]));
+                            wy_ = wy_ + sw[o] * static_cast<W>(wx[xx]);
+                        }
+                        v = vadd(v, vmul(vy, wy[yy]));
+                        wv = wv + wy_ * static_cast<W>(wy[yy]);
+                    }
+                    vimag_mul(v, conj);
+                }
+            } else {
+                v = interp2<INTERP, In>(sl, p.is_y, p.is_x, Hs, Ws, p.slices_centered, py, px);
+                if (wsl) wv = interp2<INTERP, W>(wsl, p.wis_y, p.wis_x, Hs, Ws, p.slices_centered, py, px);
+            }
+            const i32 lv = acc_index(lz, ly, lx);
+            acc_v[lv] = vadd(acc_v[lv], vmul(v, static_cast<RIn>(window)));
+            if (has_wvol)
+                acc_w[lv] = acc_w[lv] + (win_present ? wv * static_cast<W>(window) : static_cast<W>(window));
         }
         __syncthreads();
         cur = nxt;
         if (cur >= 0) {
-            fp = fp_n;
+            pi = pi_n;
             rot = rot_n;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) m[q] = m_n[q];
         }
     }
 
@@ -445,8 +558,8 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
         const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
         if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
         vec3<C> f;
-        f.z = static_cast<C>(index2frequency(p.vol_centered, z, p.grid[0])) / p.target[0];
-        f.y = static_cast<C>(index2frequency(p.vol_centered, y, p.grid[1])) / p.target[1];
+        f.z = static_cast<C>(index2frequency(vc, z, grid[0])) / p.target[0];
+        f.y = static_cast<C>(index2frequency(vc, y, grid[1])) / p.target[1];
         f.x = static_cast<C>(x) / p.target[2];
         if (dot3(f) > p.cutoff_sqd) continue;
         const i32 a = acc_index(lz, ly, lx);
@@ -461,14 +574,14 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
 
 // Adds the partial sums of each split brick to the volume, in segment order.
 template<class C, class In, class Out, class W>
-__global__ __launch_bounds__(NT) void insert_reduce_kernel(const InsertParams<C, In, W> p) {
+__global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertParams<C, In, W> p) {
     const int4 red = p.reduce[blockIdx.x];
     const i32 brick = red.x;
     const i32 bxi = brick % p.nb[2];
     const i32 rem = brick / p.nb[2];
     const i32 org[3] = {(rem / p.nb[1]) * BE, (rem % p.nb[1]) * BE, bxi * BE};
     const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
-    for (i32 v = static_cast<i32>(threadIdx.x); v < BE * BE * BE; v += NT) {
+    for (i32 v = static_cast<i32>(threadIdx.x); v < BE * BE * BE; v += NT_REDUCE) {
         const i32 lz = v / (BE * BE), ly = (v / BE) % BE, lx = v % BE;
         const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
         if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
@@ -570,12 +683,14 @@ template<class T> T value_of(const Arr& a) {
 
 i32 host_freq(bool centered, i32 i, i32 n) { return centered ? i - n / 2 : (i < (n + 1) / 2 ? i : i - n); }
 
-// Chebyshev fit of windowed_sinc(f, fs, fb) (FourierProject.hpp:99-110) on f in [-fb, fb], as a series in
-// t = 2 (f/fb)^2 - 1 (the window is even). The degree grows until the absolute error, checked on a dense grid in
-// double precision, is below 2e-9. Kept per (fs, fb) for the process lifetime.
+// Polynomial form of windowed_sinc(f, fs, fb) (FourierProject.hpp:99-110) on f in [-fb, fb], in t = 2 (f/fb)^2 - 1
+// (the window is even): Chebyshev interpolation of degree NWIN at the Chebyshev nodes, converted to monomials in
+// t and rounded to single precision. The fit is accepted when the kernel's own single-precision evaluation
+// (window_poly, reproduced here with the same roundings) stays within 4e-7 of the double-precision window on a
+// dense grid; otherwise the kernel evaluates noa's expression. Kept per (fs, fb) for the process lifetime.
 struct WindowFit {
-    float c[32];
-    i32 n;
+    float a[NWIN + 1];
+    bool ok;
 };
 std::mutex g_window_mutex;
 std::map<std::pair<float, float>, WindowFit> g_windows;
@@ -592,34 +707,41 @@ WindowFit fit_window(float fs, float fb) {
         const double b = PI * f / fb;
         return s * (0.42 + 0.5 * std::cos(b) + 0.08 * std::cos(2 * b));
     };
-    WindowFit fit{};
-    for (i32 n = 6; n <= 31; ++n) {
-        const i32 m = n + 1;
-        double c[32] = {};
-        for (i32 k = 0; k <= n; ++k) {
-            double sum = 0;
-            for (i32 j = 0; j < m; ++j) {
-                const double th = PI * (j + 0.5) / m;
-                const double t = std::cos(th);
-                const double u = 0.5 * (t + 1);
-                sum += w(static_cast<double>(fb) * std::sqrt(u)) * std::cos(k * th);
-            }
-            c[k] = 2.0 * sum / m;
+    constexpr i32 n = NWIN, m = NWIN + 1;
+    double c[NWIN + 1] = {};
+    for (i32 k = 0; k <= n; ++k) {
+        double sum = 0;
+        for (i32 j = 0; j < m; ++j) {
+            const double th = PI * (j + 0.5) / m;
+            const double u = 0.5 * (std::cos(th) + 1);
+            sum += w(static_cast<double>(fb) * std::sqrt(u)) * std::cos(k * th);
         }
-        c[0] *= 0.5;
-        double err = 0;
-        for (i32 j = 0; j <= 2000; ++j) {
-            const double u = j / 2000.0, t = 2 * u - 1;
-            double b1 = 0, b2 = 0;
-            for (i32 k = n; k >= 1; --k) { const double b0 = 2 * t * b1 - b2 + c[k]; b2 = b1; b1 = b0; }
-            const double approx = t * b1 - b2 + c[0];
-            err = std::max(err, std::abs(approx - w(static_cast<double>(fb) * std::sqrt(u))));
-        }
-        for (i32 k = 0; k <= n; ++k) fit.c[k] = static_cast<float>(c[k]);
-        fit.n = n;
-        if (err < 2e-9) break;
-        if (n == 31 && err > 1e-7) fit.n = -1;   // too oscillatory: evaluate noa's expression instead
+        c[k] = 2.0 * sum / m;
     }
+    c[0] *= 0.5;
+    // T_k(t) in monomials: T_{k+1} = 2 t T_k - T_{k-1}.
+    double tk[NWIN + 1][NWIN + 1] = {};
+    tk[0][0] = 1;
+    tk[1][1] = 1;
+    for (i32 k = 2; k <= n; ++k)
+        for (i32 j = 0; j <= k; ++j) tk[k][j] = (j > 0 ? 2 * tk[k - 1][j - 1] : 0.0) - tk[k - 2][j];
+    WindowFit fit{};
+    for (i32 j = 0; j <= n; ++j) {
+        double a = 0;
+        for (i32 k = j; k <= n; ++k) a += c[k] * tk[k][j];
+        fit.a[j] = static_cast<float>(a);
+    }
+    const float inv_fb = 1.f / fb;
+    double err = 0;
+    for (i32 j = -4000; j <= 4000; ++j) {
+        const float f = fb * (static_cast<float>(j) / 4000.f);
+        const float q = f * inv_fb;
+        const float t = std::fmaf(2.f * q, q, -1.f);
+        float r = fit.a[n];
+        for (i32 k = n - 1; k >= 0; --k) r = std::fmaf(r, t, fit.a[k]);
+        err = std::max(err, std::abs(static_cast<double>(r) - w(static_cast<double>(f))));
+    }
+    fit.ok = err <= 4e-7;
     g_windows[key] = fit;
     return fit;
 }
@@ -630,6 +752,15 @@ WindowFit fit_window(float fs, float fb) {
 // min(1, t / r) of the slabs, t = f_b + mean projected half-extent. Items are sorted by expected work, largest
 // first. Kept per (device, geometry, slice count, window) for the process lifetime.
 constexpr double kListTarget = 384.0;
+
+// Threads per workgroup of the staged kernel: 256 (default) or 512, from NOA_HIP_INSERT_THREADS.
+int staged_threads() {
+    static const int n = [] {
+        const char* e = std::getenv("NOA_HIP_INSERT_THREADS");
+        return e && std::atoi(e) == 512 ? 512 : 256;
+    }();
+    return n;
+}
 
 struct Plan {
     int4* items{};
@@ -739,11 +870,12 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     p.has_wvol = d.output_weight.data != nullptr;
     p.wvol = static_cast<W*>(d.output_weight.data);
     p.wvs0 = (i32)d.output_weight.s0; p.wvs1 = (i32)d.output_weight.s1; p.wvs2 = (i32)d.output_weight.s2;
+    p.pow2 = true;
     for (int i = 0; i < 3; ++i) {
         p.grid[i] = static_cast<i32>(d.out_shape[i + 1]);
         p.target[i] = static_cast<C>(d.target[i]);
         const i64 t = d.target[i];
-        p.pow2[i] = t > 0 && (t & (t - 1)) == 0;
+        p.pow2 = p.pow2 && t > 0 && (t & (t - 1)) == 0;
         p.inv_target[i] = C(1) / p.target[i];
     }
     p.wr = p.grid[2] / 2 + 1;
@@ -772,8 +904,8 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     p.vol_centered = d.dst_centered;
     {
         const WindowFit fit = fit_window(static_cast<float>(p.fftfreq_sinc), static_cast<float>(p.fftfreq_blackman));
-        for (int k = 0; k < 32; ++k) p.wc[k] = fit.c[k];
-        p.wn = fit.n;
+        for (int k = 0; k <= NWIN; ++k) p.wa[k] = fit.a[k];
+        p.wpoly = fit.ok;
         p.inv_fb = 1.f / static_cast<float>(p.fftfreq_blackman);
     }
     p.slice_begin = 0;
@@ -802,12 +934,16 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                 }
             }
             p.partials = static_cast<float*>(partials);
-            if (nearest)
-                hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST>), dim3(plan.n_items), dim3(NT), 0, stream, p);
-            else
-                hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR>), dim3(plan.n_items), dim3(NT), 0, stream, p);
+            const dim3 grid_items(static_cast<unsigned>(plan.n_items));
+            if (staged_threads() == 512) {
+                if (nearest) hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, 512>), grid_items, dim3(512), 0, stream, p);
+                else hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, 512>), grid_items, dim3(512), 0, stream, p);
+            } else {
+                if (nearest) hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, 256>), grid_items, dim3(256), 0, stream, p);
+                else hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, 256>), grid_items, dim3(256), 0, stream, p);
+            }
             if (plan.n_reduce > 0)
-                hipLaunchKernelGGL((insert_reduce_kernel<C, In, Out, W>), dim3(plan.n_reduce), dim3(NT), 0, stream, p);
+                hipLaunchKernelGGL((insert_reduce_kernel<C, In, Out, W>), dim3(plan.n_reduce), dim3(NT_REDUCE), 0, stream, p);
             hipError_t e = hipGetLastError();
             if (partials) {
                 const hipError_t fe = hipFreeAsync(partials, stream);
