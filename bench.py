@@ -22,6 +22,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "slices/s and % HBM peak, 256\u00b3 insert+extract; p50/p99 request latency"
 
 
 def log(*a):
@@ -114,6 +115,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-insert", type=int, default=10_000)
     ap.add_argument("--n-extract", type=int, default=4096)
+    ap.add_argument("--probe", action="store_true", help="one launch of each operator, no timing (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.",
@@ -134,41 +136,47 @@ def main():
     wvol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
 
-    ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in ("extract", "insert")}
-    durations = {"extract": [], "insert": []}
-
-    def step(record: bool):
-        if record:
-            ev["extract"][0].record()
+    def extract():
         nx.extract_central_slices_3d("hc2hc", ext.volume, None, ext.volume_shape, out, None, ext.slice_shape, None,
                                      ext.rotations)
-        if record:
-            ev["extract"][1].record()
-            ev["insert"][0].record()
+
+    def insert():
         nx.insert_central_slices_3d("h2h", ins.slices, ins.weights, ins.slice_shape, vol, wvol, ins.volume_shape,
                                     None, ins.inv_rotations)
-        if record:
-            ev["insert"][1].record()
+
+    if args.probe:
+        # One launch of each operator and a device copy of known size (1 GiB read + 1 GiB written), for hardware
+        # counter runs: the copy calibrates FETCH_SIZE / WRITE_SIZE against known bytes.
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).normal_()
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        extract()
+        insert()
+        torch.cuda.synchronize()
+        log("probe done: copy 1 GiB, extract, insert")
+        return
 
     log("warmup ...")
     for _ in range(args.warmup):
-        step(False)
+        extract()
+        insert()
     torch.cuda.synchronize()
 
-    # Per-kernel durations (HIP events on the stream the kernels are launched on).
-    for _ in range(args.steps):
-        step(True)
-        torch.cuda.synchronize()
-        for k in durations:
-            durations[k].append(ev[k][0].elapsed_time(ev[k][1]) * 1e-3)
-
-    # Whole-step throughput, K steps bracketed by synchronisations.
+    # K steps bracketed by synchronisations; HIP events on the launch stream time each operator inside them.
+    marks = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(False)
+    for k in range(args.steps):
+        marks[k][0].record()
+        extract()
+        marks[k][1].record()
+        insert()
+        marks[k][2].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    durations = {"extract": [m[0].elapsed_time(m[1]) * 1e-3 for m in marks],
+                 "insert": [m[1].elapsed_time(m[2]) * 1e-3 for m in marks]}
     ms_per_step = wall / args.steps * 1e3
     slices_per_step = args.n_extract + args.n_insert
     value = slices_per_step / (wall / args.steps)
@@ -187,7 +195,7 @@ def main():
     total_time = sum(k["avg_ms"] for k in kernels.values()) * 1e-3
 
     result = {
-        "metric": "slices/s and % HBM peak, 256^3 insert+extract",
+        "metric": METRIC,
         "value": value,
         "unit": "slices/s",
         "n_gpus": 1,

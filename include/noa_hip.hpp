@@ -1,0 +1,322 @@
+/*
+ * noa_hip.hpp -- C++20 facade of the MI355X central-slice library with noa's spelling.
+ *
+ * Header-only, over the C ABI of noa_hip.h. It mirrors the public functions of noa::xform
+ * (src/noa/xform/FourierProject.hpp:1601-1914) for C++ code that does not link noa itself:
+ *
+ *   noa_hip::xform::rasterize_central_slices_3d<"h2h">(slice, slice_weight, slice_shape,
+ *                                                      volume, volume_weight, volume_shape,
+ *                                                      inv_scaling, fwd_rotation, options, stream);
+ *   noa_hip::xform::insert_central_slices_3d<"h2h">(..., fwd_scaling, inv_rotation, options, stream);
+ *   noa_hip::xform::extract_central_slices_3d<"hc2hc">(volume, volume_weight, volume_shape,
+ *                                                      slice, slice_weight, slice_shape,
+ *                                                      inv_scaling, fwd_rotation, options, stream);
+ *
+ * Arguments follow noa: BDHW logical shapes (Shape4), rfft-shaped strided arrays (View, element strides),
+ * an Empty for absent weights or transforms, a plain value for a constant input (rasterisation, insertion),
+ * one transform (Mat33, Quaternion, Mat22) or one per slice (a View of transforms in device memory).
+ * The layout template argument is a string literal as in noa ("h2h", "h", "hc2hc", "hc", "hc2h", "h2hc").
+ * Errors throw noa_hip::Exception with the library's message. Calls are asynchronous on `stream`
+ * (a hipStream_t, nullptr = the null stream), and borrow the buffers.
+ */
+#ifndef NOA_HIP_HPP
+#define NOA_HIP_HPP
+
+#include "noa_hip.h"
+
+#include <array>
+#include <complex>
+#include <concepts>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <type_traits>
+
+namespace noa_hip {
+
+using i64 = std::int64_t;
+using f32 = float;
+using f64 = double;
+using c32 = std::complex<float>;    // same layout as noa::Complex<f32>: {real, imag}
+using c64 = std::complex<double>;
+
+/* noa::Exception stand-in. */
+class Exception : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+/* noa::Empty: an absent argument. */
+struct Empty {};
+
+/* BDHW shape (noa::Shape4 = Shape<isize, 4>). */
+struct Shape4 {
+    std::array<i64, 4> v{};
+    constexpr Shape4() = default;
+    constexpr Shape4(i64 b, i64 d, i64 h, i64 w) : v{b, d, h, w} {}
+    constexpr i64 operator[](int i) const { return v[static_cast<std::size_t>(i)]; }
+    constexpr i64& operator[](int i) { return v[static_cast<std::size_t>(i)]; }
+    constexpr Shape4 rfft() const { return {v[0], v[1], v[2], v[3] / 2 + 1}; }
+    constexpr bool is_empty() const { return v[0] == 0 && v[1] == 0 && v[2] == 0 && v[3] == 0; }
+    constexpr i64 elements() const { return v[0] * v[1] * v[2] * v[3]; }
+    constexpr bool operator==(const Shape4&) const = default;
+};
+
+/* Strided 4D array of device memory (noa::View): rfft shape, strides in elements. */
+template<class T>
+struct View {
+    T* ptr{};
+    Shape4 shape{};
+    Shape4 strides{};
+    View() = default;
+    View(T* p, Shape4 s, Shape4 st) : ptr(p), shape(s), strides(st) {}
+    /* C-contiguous view. */
+    View(T* p, Shape4 s)
+        : ptr(p), shape(s), strides(s[1] * s[2] * s[3], s[2] * s[3], s[3], 1) {}
+    T* get() const { return ptr; }
+    i64 ssize() const { return shape.elements(); }
+};
+
+/* noa::Mat33<f32> (48-byte stride: alignas(16)), Mat33<f64> (80-byte stride), row-major. */
+template<class T>
+struct alignas(16) Mat33 {
+    T m[3][3];
+};
+/* noa::Quaternion: {z, y, x, w}. */
+template<class T>
+struct alignas(sizeof(T) * 4) Quaternion {
+    T z, y, x, w;
+};
+/* noa::Mat22, row-major. */
+template<class T>
+struct alignas(sizeof(T) * 4) Mat22 {
+    T m[2][2];
+};
+static_assert(sizeof(Mat33<f32>) == 48 && sizeof(Mat33<f64>) == 80);
+static_assert(sizeof(Quaternion<f32>) == 16 && sizeof(Quaternion<f64>) == 32);
+static_assert(sizeof(Mat22<f32>) == 16 && sizeof(Mat22<f64>) == 32);
+
+namespace xform {
+
+/* noa::fft::Layout restricted to the half-to-half remaps, constructible from a string literal. */
+struct Layout {
+    int32_t bits{};
+    static constexpr int32_t from_string(std::string_view name) {
+        if (name == "h2h" || name == "H2H" || name == "h" || name == "H") return NOA_HIP_H2H;
+        if (name == "h2hc" || name == "H2HC") return NOA_HIP_H2HC;
+        if (name == "hc2h" || name == "HC2H") return NOA_HIP_HC2H;
+        if (name == "hc2hc" || name == "HC2HC" || name == "hc" || name == "HC") return NOA_HIP_HC2HC;
+        throw Exception("invalid layout: only the half-to-half remaps h2h, h2hc, hc2h and hc2hc are supported");
+    }
+    template<std::size_t N>
+    constexpr Layout(const char (&name)[N]) : bits(from_string(std::string_view(name, N - 1))) {}
+};
+
+/* noa::xform::Interp (Interp.hpp:17-41). */
+enum class Interp : int32_t {
+    NEAREST = NOA_HIP_INTERP_NEAREST,
+    LINEAR = NOA_HIP_INTERP_LINEAR,
+    CUBIC = NOA_HIP_INTERP_CUBIC,
+    CUBIC_BSPLINE = NOA_HIP_INTERP_CUBIC_BSPLINE,
+    LANCZOS4 = NOA_HIP_INTERP_LANCZOS4,
+    LANCZOS6 = NOA_HIP_INTERP_LANCZOS6,
+    LANCZOS8 = NOA_HIP_INTERP_LANCZOS8,
+    NEAREST_FAST = NOA_HIP_INTERP_NEAREST_FAST,
+    LINEAR_FAST = NOA_HIP_INTERP_LINEAR_FAST,
+    CUBIC_FAST = NOA_HIP_INTERP_CUBIC_FAST,
+    CUBIC_BSPLINE_FAST = NOA_HIP_INTERP_CUBIC_BSPLINE_FAST,
+    LANCZOS4_FAST = NOA_HIP_INTERP_LANCZOS4_FAST,
+    LANCZOS6_FAST = NOA_HIP_INTERP_LANCZOS6_FAST,
+    LANCZOS8_FAST = NOA_HIP_INTERP_LANCZOS8_FAST,
+};
+
+/* Options (FourierProject.hpp:1602-1627, :1703-1739, :1817-1838), same fields and defaults. */
+struct RasterizeCentralSlicesOptions {
+    f64 fftfreq_cutoff{0.5};
+    Shape4 target_shape{};
+    std::array<f64, 2> ews_radius{};
+};
+struct WindowedSinc {
+    f64 fftfreq_sinc{-1};
+    f64 fftfreq_blackman{-1};
+};
+struct InsertCentralSlicesOptions {
+    Interp interp{Interp::LINEAR};
+    WindowedSinc windowed_sinc{};
+    f64 fftfreq_cutoff{0.5};
+    Shape4 target_shape{};
+    std::array<f64, 2> ews_radius{};
+};
+struct ExtractCentralSlicesOptions {
+    Interp interp{Interp::LINEAR};
+    WindowedSinc w_windowed_sinc{};
+    f64 fftfreq_cutoff{0.5};
+    Shape4 target_shape{};
+    std::array<f64, 2> ews_radius{};
+};
+
+namespace details {
+    template<class T> struct is_view : std::false_type {};
+    template<class T> struct is_view<View<T>> : std::true_type {};
+    template<class T> constexpr bool is_view_v = is_view<std::remove_cvref_t<T>>::value;
+
+    template<class T>
+    constexpr int32_t dtype() {
+        using U = std::remove_cv_t<T>;
+        if constexpr (std::is_same_v<U, f32>) return NOA_HIP_F32;
+        else if constexpr (std::is_same_v<U, f64>) return NOA_HIP_F64;
+        else if constexpr (std::is_same_v<U, c32>) return NOA_HIP_C32;
+        else if constexpr (std::is_same_v<U, c64>) return NOA_HIP_C64;
+        else static_assert(sizeof(U) == 0, "unsupported value type");
+    }
+
+    template<class T>
+    constexpr int32_t xform_kind() {
+        using U = std::remove_cv_t<T>;
+        if constexpr (std::is_same_v<U, Mat33<f32>>) return NOA_HIP_MAT33_F32;
+        else if constexpr (std::is_same_v<U, Mat33<f64>>) return NOA_HIP_MAT33_F64;
+        else if constexpr (std::is_same_v<U, Quaternion<f32>>) return NOA_HIP_QUAT_F32;
+        else if constexpr (std::is_same_v<U, Quaternion<f64>>) return NOA_HIP_QUAT_F64;
+        else if constexpr (std::is_same_v<U, Mat22<f32>>) return NOA_HIP_MAT22_F32;
+        else if constexpr (std::is_same_v<U, Mat22<f64>>) return NOA_HIP_MAT22_F64;
+        else static_assert(sizeof(U) == 0, "unsupported transform type");
+    }
+
+    /* A View -> strided view; a value -> constant (noa's AccessorValue); Empty -> absent. */
+    template<class T>
+    noa_hip_view view(const T& a) {
+        noa_hip_view v{};
+        using U = std::remove_cvref_t<T>;
+        if constexpr (std::is_same_v<U, Empty>) {
+            return v;
+        } else if constexpr (is_view_v<U>) {
+            v.data = const_cast<void*>(static_cast<const void*>(a.ptr));
+            for (int i = 0; i < 4; ++i) {
+                v.shape[i] = a.shape[i];
+                v.strides[i] = a.strides[i];
+            }
+            v.dtype = dtype<std::remove_pointer_t<decltype(a.ptr)>>();
+        } else {
+            v.is_value = 1;
+            v.dtype = dtype<U>();
+            if constexpr (std::is_same_v<U, c32> || std::is_same_v<U, c64>) {
+                v.value[0] = static_cast<f64>(a.real());
+                v.value[1] = static_cast<f64>(a.imag());
+            } else {
+                v.value[0] = static_cast<f64>(a);
+            }
+        }
+        return v;
+    }
+
+    /* Empty -> none; one transform -> broadcast (host value, copied at the call); a View -> one per slice. */
+    template<class T>
+    noa_hip_transform transform(const T& t) {
+        noa_hip_transform x{};
+        using U = std::remove_cvref_t<T>;
+        if constexpr (std::is_same_v<U, Empty>) {
+            x.kind = NOA_HIP_XFORM_NONE;
+        } else if constexpr (is_view_v<U>) {
+            x.data = t.ptr;
+            x.kind = xform_kind<std::remove_pointer_t<decltype(t.ptr)>>();
+            x.batched = 1;
+            x.count = t.ssize();
+        } else {
+            x.data = &t;
+            x.kind = xform_kind<U>();
+            x.batched = 0;
+            x.count = 1;
+        }
+        return x;
+    }
+
+    template<class Options>
+    void options(noa_hip_project_args& a, const Options& o) {
+        a.fftfreq_cutoff = o.fftfreq_cutoff;
+        for (int i = 0; i < 4; ++i) a.target_shape[i] = o.target_shape[i];
+        a.ews_radius[0] = o.ews_radius[0];
+        a.ews_radius[1] = o.ews_radius[1];
+        if constexpr (requires { o.interp; }) a.interp = static_cast<int32_t>(o.interp);
+        if constexpr (requires { o.windowed_sinc; }) {
+            a.fftfreq_sinc = o.windowed_sinc.fftfreq_sinc;
+            a.fftfreq_blackman = o.windowed_sinc.fftfreq_blackman;
+        }
+        if constexpr (requires { o.w_windowed_sinc; }) {
+            a.fftfreq_sinc = o.w_windowed_sinc.fftfreq_sinc;
+            a.fftfreq_blackman = o.w_windowed_sinc.fftfreq_blackman;
+        }
+    }
+
+    /* The descriptor of one call (exposed so that a caller can inspect or reuse it). */
+    template<class Input, class InputWeight, class Output, class OutputWeight, class Scale, class Rotate,
+             class Options>
+    noa_hip_project_args make_args(int32_t remap, const Input& input, const InputWeight& input_weight,
+                                   const Shape4& input_shape, const Output& output,
+                                   const OutputWeight& output_weight, const Shape4& output_shape,
+                                   const Scale& scaling, const Rotate& rotation, const Options& opt) {
+        noa_hip_project_args a;
+        noa_hip_project_args_init(&a);
+        a.input = view(input);
+        a.input_weight = view(input_weight);
+        a.output = view(output);
+        a.output_weight = view(output_weight);
+        for (int i = 0; i < 4; ++i) {
+            a.input_shape[i] = input_shape[i];
+            a.output_shape[i] = output_shape[i];
+        }
+        a.remap = remap;
+        a.rotation = transform(rotation);
+        a.scaling = transform(scaling);
+        options(a, opt);
+        return a;
+    }
+
+    inline void check(int status) {
+        if (status != NOA_HIP_OK) throw Exception(noa_hip_last_error());
+    }
+} // namespace details
+
+/* noa::xform::rasterize_central_slices_3d (FourierProject.hpp:1653-1692). Accumulates into volume. */
+template<Layout REMAP, class Input, class InputWeight = Empty, class Output, class OutputWeight = Empty,
+         class Scale = Empty, class Rotate>
+void rasterize_central_slices_3d(const Input& slice, const InputWeight& slice_weight, const Shape4& slice_shape,
+                                 const Output& volume, const OutputWeight& volume_weight,
+                                 const Shape4& volume_shape, const Scale& inv_scaling, const Rotate& fwd_rotation,
+                                 const RasterizeCentralSlicesOptions& options = {}, void* stream = nullptr) {
+    const noa_hip_project_args a = details::make_args(REMAP.bits, slice, slice_weight, slice_shape, volume,
+                                                      volume_weight, volume_shape, inv_scaling, fwd_rotation,
+                                                      options);
+    details::check(noa_hip_rasterize_central_slices_3d(&a, stream));
+}
+
+/* noa::xform::insert_central_slices_3d (FourierProject.hpp:1770-1815). Accumulates into volume. */
+template<Layout REMAP, class Input, class InputWeight = Empty, class Output, class OutputWeight = Empty,
+         class Scale = Empty, class Rotate>
+void insert_central_slices_3d(const Input& slice, const InputWeight& slice_weight, const Shape4& slice_shape,
+                              const Output& volume, const OutputWeight& volume_weight, const Shape4& volume_shape,
+                              const Scale& fwd_scaling, const Rotate& inv_rotation,
+                              const InsertCentralSlicesOptions& options = {}, void* stream = nullptr) {
+    const noa_hip_project_args a = details::make_args(REMAP.bits, slice, slice_weight, slice_shape, volume,
+                                                      volume_weight, volume_shape, fwd_scaling, inv_rotation,
+                                                      options);
+    details::check(noa_hip_insert_central_slices_3d(&a, stream));
+}
+
+/* noa::xform::extract_central_slices_3d (FourierProject.hpp:1869-1914). Overwrites slice. */
+template<Layout REMAP, class Input, class InputWeight = Empty, class Output, class OutputWeight = Empty,
+         class Scale = Empty, class Rotate>
+void extract_central_slices_3d(const Input& volume, const InputWeight& volume_weight, const Shape4& volume_shape,
+                               const Output& slice, const OutputWeight& slice_weight, const Shape4& slice_shape,
+                               const Scale& inv_scaling, const Rotate& fwd_rotation,
+                               const ExtractCentralSlicesOptions& options = {}, void* stream = nullptr) {
+    const noa_hip_project_args a = details::make_args(REMAP.bits, volume, volume_weight, volume_shape, slice,
+                                                      slice_weight, slice_shape, inv_scaling, fwd_rotation,
+                                                      options);
+    details::check(noa_hip_extract_central_slices_3d(&a, stream));
+}
+
+} // namespace xform
+} // namespace noa_hip
+
+#endif // NOA_HIP_HPP

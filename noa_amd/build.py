@@ -79,5 +79,26 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     return LIB
 
 
+TESTS = PKG.parent / "tests"
+FACADE_TEST = TESTS / "_build" / "test_facade"
+
+
+def build_tests(verbose: bool = False) -> Path:
+    """The C++ facade test program (host code only: g++ against include/noa_hip.hpp and the built library)."""
+    FACADE_TEST.parent.mkdir(parents=True, exist_ok=True)
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{rocm / 'include'}",
+           "-D__HIP_PLATFORM_AMD__", str(TESTS / "cpp" / "test_facade.cpp"), "-o", str(FACADE_TEST),
+           f"-L{BUILD}", "-lnoa_hip", f"-L{rocm / 'lib'}", "-lamdhip64",
+           f"-Wl,-rpath,$ORIGIN/../../noa_amd/_build:{rocm / 'lib'}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed on the facade test:\n{res.stdout}\n{res.stderr}")
+    return FACADE_TEST
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_tests(verbose=True))

@@ -57,6 +57,8 @@ struct InsertParams {
     i32 grid[3];          // logical D, H, W of the volume
     i32 wr;               // W/2+1
     i32 nb[3];            // bricks along z, y, x
+    i32 bd[3];            // brick dims (voxels along z, y, x) of the launch
+    i32 apitch;           // x pitch of a brick accumulator (and of its partial sums)
     const int4* items;    // {brick, first slice, end slice, partial slot or -1}
     const int4* reduce;   // {brick, first partial slot, number of partials, 0}
     float* partials;      // per slot: ACC complex accumulators then ACC weights (as float pairs / floats)
@@ -572,6 +574,408 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// One-wave path: a work item is (8x8x16 sub-brick, range of slices) processed by one wave with no barrier
+// between slices. The accumulator (12 KiB of LDS) is private to the wave; the slice taps are read from global
+// memory (L2-resident footprints). Each voxel is owned by one lane per slice and slices are processed in order,
+// so the sums are deterministic.
+// ---------------------------------------------------------------------------------------------------------
+constexpr i32 WZ = 8, WY = 8, WX = 16;     // sub-brick voxels along z, y, x (WZ == WY: the face axis B has 8)
+constexpr i32 WVOX = WZ * WY * WX;
+
+// Enumeration data of a (sub-brick, slice) pair: the volume axis A closest to the slab normal (in index units)
+// and the normal's components along A (inverted) and the face axes B < Cc.
+template<class C>
+struct AxisInfo {
+    i32 flags;            // bits 0-1: axis A, 4: candidate range from the slab
+    C inv_alA, alB, alC;
+};
+
+template<class C>
+__device__ __forceinline__ AxisInfo<C> axis_info(const C m[9], C it0, C it1, C it2, const bool contiguous[3]) {
+    AxisInfo<C> r;
+    const C al0 = m[0] * it0, al1 = m[1] * it1, al2 = m[2] * it2;
+    i32 ax = 0;
+    if (fabs(al1) > fabs(al0)) ax = 1;
+    if (fabs(al2) > fabs(ax == 0 ? al0 : al1)) ax = 2;
+    const C alA = ax == 0 ? al0 : (ax == 1 ? al1 : al2);
+    r.alB = ax == 0 ? al1 : al0;
+    r.alC = ax == 2 ? al1 : al2;
+    const bool contig = ax == 0 ? contiguous[0] : (ax == 1 ? contiguous[1] : contiguous[2]);
+    const bool ranged = contig && fabs(alA) > C(1e-20);
+    r.inv_alA = ranged ? C(1) / alA : C(0);
+    r.flags = ax | (ranged ? 16 : 0);
+    return r;
+}
+
+constexpr i32 WG = 2;                      // candidates per lane and pass
+
+// The taps of one 2D interpolation of a slice and of its weights (Interpolation.hpp:435-526): noa's flip of the
+// frequency when x < 0, the fractions, and the four (LINEAR) or one (NEAREST) tap values, each loaded from a
+// clamped address and zeroed when outside the frequency bounds. All loads are issued before any is used.
+template<class In, class W, class C>
+struct Taps {
+    In v[2][2];
+    W w[2][2];
+    typename vt<In>::real ty, tx, conj;
+};
+
+template<int INTERP, class In, class W, class C>
+__device__ __forceinline__ Taps<In, W, C> fetch_taps(const In* __restrict__ vb, i32 vsy, i32 vsx,
+                                                     const W* __restrict__ wb, i32 wsy, i32 wsx, bool has_w, i32 ny,
+                                                     i32 nx, bool centered, C fy, C fx, bool valid) {
+    using R = typename vt<In>::real;
+    Taps<In, W, C> t;
+    t.conj = 1;
+    if (fx < 0) { fy *= -1; fx *= -1; t.conj = -1; }
+    const i32 lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
+    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+        const i32 y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
+        const bool ok = valid && y >= lo_y && y <= hi_y && x >= 0 && x <= hi_x;
+        const i32 iy = ok ? frequency2index(centered, y, ny) : 0, ix = ok ? x : 0;
+        const In a = vb[iy * vsy + ix * vsx];
+        const W b = has_w ? wb[iy * wsy + ix * wsx] : W(0);
+        t.v[0][0] = ok ? a : vzero<In>();
+        t.w[0][0] = ok ? b : W(0);
+        t.ty = 0;
+        t.tx = 0;
+    } else {
+        const C fly = floor(fy), flx = floor(fx);
+        const i32 y0 = static_cast<i32>(fly), x0 = static_cast<i32>(flx);
+        t.ty = static_cast<R>(fy - fly);
+        t.tx = static_cast<R>(fx - flx);
+        i32 ov[2][2], ow[2][2];
+        bool ok[2][2];
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const bool oky = valid && y0 + y >= lo_y && y0 + y <= hi_y;
+            const i32 iy = oky ? frequency2index(centered, y0 + y, ny) : 0;
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                ok[y][x] = oky && x0 + x >= 0 && x0 + x <= hi_x;
+                const i32 ix = ok[y][x] ? x0 + x : 0;
+                ov[y][x] = ok[y][x] ? iy * vsy + ix * vsx : 0;
+                ow[y][x] = ok[y][x] ? iy * wsy + ix * wsx : 0;
+            }
+        }
+        In a[2][2];
+        W b[2][2];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                a[y][x] = vb[ov[y][x]];
+                b[y][x] = has_w ? wb[ow[y][x]] : W(0);
+            }
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                t.v[y][x] = ok[y][x] ? a[y][x] : vzero<In>();
+                t.w[y][x] = ok[y][x] ? b[y][x] : W(0);
+            }
+    }
+    return t;
+}
+
+// Sums the taps in noa's order: x innermost, then y; then the imaginary part is negated if the frequency was
+// flipped (LINEAR only; NEAREST does not conjugate, Interpolation.hpp:479-484).
+template<int INTERP, class In, class W, class C>
+__device__ __forceinline__ void combine_taps(const Taps<In, W, C>& t, bool has_w, In& value, W& weight) {
+    using R = typename vt<In>::real;
+    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+        value = vadd(vzero<In>(), vmul(t.v[0][0], R(1)));
+        weight = W(0) + t.w[0][0] * W(1);
+    } else {
+        const R wy[2] = {1 - t.ty, t.ty}, wx[2] = {1 - t.tx, t.tx};
+        value = vzero<In>();
+        weight = W(0);
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            In vy = vzero<In>();
+            W wyv = W(0);
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                vy = vadd(vy, vmul(t.v[y][x], wx[x]));
+                wyv = wyv + t.w[y][x] * static_cast<W>(wx[x]);
+            }
+            value = vadd(value, vmul(vy, wy[y]));
+            weight = weight + wyv * static_cast<W>(wy[y]);
+        }
+        vimag_mul(value, t.conj);
+    }
+    (void)has_w;
+}
+
+template<class C, class In, class Out, class W, int INTERP>
+__global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, In, W> p) {
+    using RIn = typename vt<In>::real;
+    __shared__ In acc_v[WVOX];
+    __shared__ W acc_w[WVOX];
+    __shared__ i32 cand[64 * WG];          // packed candidates of one pass: column << 4 | k
+
+    const i32 lane = static_cast<i32>(threadIdx.x);
+    const int4 item = p.items[blockIdx.x];
+    const i32 brick = item.x;
+    const i32 bxi = brick % p.nb[2];
+    const i32 rem = brick / p.nb[2];
+    const i32 byi = rem % p.nb[1];
+    const i32 bzi = rem / p.nb[1];
+    constexpr i32 dims[3] = {WZ, WY, WX};
+    const i32 org[3] = {bzi * WZ, byi * WY, bxi * WX};
+    const i32 grid[3] = {p.grid[0], p.grid[1], p.grid[2]};
+    const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
+    const bool vc = p.vol_centered;
+
+    for (i32 v = lane; v < WVOX; v += 64) {
+        acc_v[v] = vzero<In>();
+        acc_w[v] = W(0);
+    }
+
+    // Sub-brick box in normalised frequencies, and whether each axis is free of the non-centred wrap (then the
+    // frequencies along it are first-frequency + k).
+    C cen[3], half[3];
+    i32 f0[3], n_in[3];
+    bool contiguous[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        i32 lo = 0x7fffffff, hi = -0x7fffffff, n = 0;
+        bool contig = true;
+        const i32 first = axis_freq(a == 2, org[a], vc, grid[a]);
+        for (i32 k = 0; k < dims[a]; ++k) {
+            const i32 idx = org[a] + k;
+            if (idx >= ext[a]) break;
+            const i32 f = axis_freq(a == 2, idx, vc, grid[a]);
+            contig = contig && (f == first + k);
+            lo = min(lo, f);
+            hi = max(hi, f);
+            ++n;
+        }
+        f0[a] = first;
+        n_in[a] = n;
+        contiguous[a] = contig;
+        cen[a] = (static_cast<C>(lo) + static_cast<C>(hi)) * C(0.5) / p.target[a];
+        half[a] = (static_cast<C>(hi) - static_cast<C>(lo)) * C(0.5) / p.target[a];
+    }
+    const C slack = C(1e-5);
+    const i32 Hs = p.slice_shape[0], Ws = p.slice_shape[1];
+
+    // Values of the hot loop, held in vector registers.
+    const C fb = pin(p.fftfreq_blackman);
+    const C fs = pin(p.fftfreq_sinc);
+    const C cutoff_sqd = pin(p.cutoff_sqd);
+    const C hs_f = pin(p.slice_f[0]), ws_f = pin(p.slice_f[1]);
+    const C t0 = pin(p.target[0]), t1 = pin(p.target[1]), t2 = pin(p.target[2]);
+    const C it0 = pin(p.inv_target[0]), it1 = pin(p.inv_target[1]), it2 = pin(p.inv_target[2]);
+    const bool pow2 = p.pow2;
+    const bool wpoly = p.wpoly;
+    const bool has_wvol = p.has_wvol;
+    const bool win_present = p.win_present;
+    float wa[NWIN + 1];
+#pragma unroll
+    for (int k = 0; k <= NWIN; ++k) wa[k] = pin(p.wa[k]);
+    const float inv_fb = pin(p.inv_fb);
+
+    // Iterator over the passing slices of the item, in increasing order: lane l holds slice chunk0+l, tests its
+    // slab against the sub-brick box and prepares the enumeration; a passing slice's values are broadcast from
+    // that lane. The next chunk's rotations are loaded one chunk ahead.
+    const bool quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
+    auto scan_load = [&](i32 s0, Rot<C>& r) {
+        const i32 s = s0 + lane;
+        if (s < item.z) r = load_rotation(p.rot, s);
+    };
+    i32 chunk0 = item.y;
+    Rot<C> chunk{}, ahead{};
+    chunk.quat = quat;
+    ahead.quat = quat;
+    AxisInfo<C> mine{};
+    unsigned long long mask = 0;
+    auto scan_prepare = [&]() {
+        bool pass = false;
+        if (chunk0 + lane < item.z) {
+            C m[9];
+            rot_matrix(chunk, m);
+            const C d = fabs(m[0] * cen[0] + m[1] * cen[1] + m[2] * cen[2]);
+            const C bound = fb + fabs(m[0]) * half[0] + fabs(m[1]) * half[1] + fabs(m[2]) * half[2];
+            pass = d <= bound + slack;
+            mine = axis_info(m, it0, it1, it2, contiguous);
+        }
+        mask = __ballot(pass);
+    };
+    scan_load(chunk0, chunk);
+    scan_load(chunk0 + 64, ahead);
+    scan_prepare();
+    auto next_slice = [&](Rot<C>& rot_out, AxisInfo<C>& out) -> i32 {
+        while (mask == 0) {
+            chunk0 += 64;
+            if (chunk0 >= item.z) return -1;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) chunk.m[q] = ahead.m[q];
+            scan_load(chunk0 + 64, ahead);
+            scan_prepare();
+        }
+        const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
+        mask &= mask - 1;
+        rot_out.quat = quat;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) rot_out.m[q] = bcast(chunk.m[q], k);
+        out.flags = __builtin_amdgcn_readlane(mine.flags, k);
+        out.inv_alA = bcast(mine.inv_alA, k);
+        out.alB = bcast(mine.alB, k);
+        out.alC = bcast(mine.alC, k);
+        return chunk0 + k;
+    };
+
+    Rot<C> rot{};
+    AxisInfo<C> ai{};
+    for (i32 cur = next_slice(rot, ai); cur >= 0; cur = next_slice(rot, ai)) {
+        // Face of axis A: B (z, or y when A is z; 8 voxels) and Cc (x, or y when A is x). Columns: 8 x FC.
+        const i32 ax = ai.flags & 3;
+        const bool ranged = ai.flags & 16;
+        const i32 lc_bits = ax == 2 ? 3 : 4;
+        const i32 fc = 1 << lc_bits;
+        const i32 n_col = WZ << lc_bits;
+        const i32 orgB = ax == 0 ? org[1] : org[0];
+        const i32 orgC = ax == 2 ? org[1] : org[2];
+        const i32 gridB = ax == 0 ? grid[1] : grid[0];
+        const i32 gridC = grid[1];                                  // unused when Cc is x
+        const bool xC = ax != 2;
+        const i32 extB = ax == 0 ? ext[1] : ext[0];
+        const i32 extC = ax == 2 ? ext[1] : ext[2];
+        const i32 f0A = ax == 0 ? f0[0] : (ax == 1 ? f0[1] : f0[2]);
+        const i32 nA = ax == 0 ? n_in[0] : (ax == 1 ? n_in[1] : n_in[2]);
+        const C tA = ax == 0 ? t0 : (ax == 1 ? t1 : t2);
+
+        // Candidates of the lane's columns (lane and lane + 64): |alA * f + beta| <= f_b in exact arithmetic,
+        // widened by eps samples to cover the rounding of the exact per-voxel test (a few ulps of |f| <= 1,
+        // divided by |alA| >= 1/(sqrt(3) T)).
+        i32 cnt[2] = {0, 0}, klo[2] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const i32 col = lane + 64 * j;
+            if (col >= n_col) continue;
+            const i32 gb = orgB + (col >> lc_bits), gc = orgC + (col & (fc - 1));
+            if (gb >= extB || gc >= extC) continue;
+            i32 k_lo = 0, k_hi = nA - 1;
+            if (ranged) {
+                const i32 fbq = index2frequency(vc, gb, gridB);
+                const i32 fcq = axis_freq(xC, gc, vc, gridC);
+                const C beta = ai.alB * static_cast<C>(fbq) + ai.alC * static_cast<C>(fcq);
+                const C eps = C(1e-3) + C(8e-6) * tA;
+                C lo = (-fb - beta) * ai.inv_alA;
+                C hi = (fb - beta) * ai.inv_alA;
+                if (lo > hi) { const C t = lo; lo = hi; hi = t; }
+                const C kl = ceil(lo - eps) - static_cast<C>(f0A);
+                const C kh = floor(hi + eps) - static_cast<C>(f0A);
+                k_lo = static_cast<i32>(fmin(fmax(kl, C(0)), static_cast<C>(nA)));
+                k_hi = static_cast<i32>(fmax(fmin(kh, static_cast<C>(nA - 1)), C(-1)));
+            }
+            cnt[j] = max(0, k_hi - k_lo + 1);
+            klo[j] = k_lo;
+        }
+        const i32 mine_cnt = cnt[0] + cnt[1];
+        const i32 incl = wave_inclusive_sum(mine_cnt);
+        const i32 total = __builtin_amdgcn_readlane(incl, 63);
+        const i32 excl = incl - mine_cnt;
+
+        const In* __restrict__ sl = p.in + static_cast<i64>(cur) * p.is_b;
+        const W* __restrict__ wsl = win_present ? p.win + static_cast<i64>(cur) * p.wis_b : p.win;
+        for (i32 base = 0; base < total; base += 64 * WG) {
+            // Pack the candidates [base, base + 64 WG) of the wave into the list, WG per lane.
+            __syncthreads();
+            {
+                const i32 i0 = max(0, base - excl), i1 = min(mine_cnt, base + 64 * WG - excl);
+                for (i32 i = i0; i < i1; ++i) {
+                    const bool second = i >= cnt[0];
+                    const i32 col = second ? lane + 64 : lane;
+                    const i32 k = second ? klo[1] + (i - cnt[0]) : klo[0] + i;
+                    cand[excl + i - base] = (col << 4) | k;
+                }
+            }
+            __syncthreads();
+            // Phase 1: noa's exact per-voxel tests (:479-494), the window, and every tap load of the WG candidates
+            // issued together.
+            Taps<In, W, C> tp[WG];
+            C window[WG];
+            i32 lv[WG];
+            bool valid[WG];
+#pragma unroll
+            for (int g = 0; g < WG; ++g) {
+                const i32 e = cand[g * 64 + lane];
+                const i32 k = e & (ax == 2 ? 15 : 7);
+                const i32 col = (e >> 4) & (n_col - 1);
+                const i32 lb = col >> lc_bits, lc = col & (fc - 1);
+                const i32 lz = ax == 0 ? k : lb;
+                const i32 ly = ax == 1 ? k : (ax == 0 ? lb : lc);
+                const i32 lx = ax == 2 ? k : lc;
+                const i32 frz = index2frequency(vc, org[0] + lz, grid[0]);
+                const i32 fry = index2frequency(vc, org[1] + ly, grid[1]);
+                const i32 frx = org[2] + lx;
+                // The normalised frequency as noa computes it: index2frequency / target (:479-484).
+                vec3<C> f;
+                if (pow2) {
+                    f.z = static_cast<C>(frz) * it0; f.y = static_cast<C>(fry) * it1; f.x = static_cast<C>(frx) * it2;
+                } else {
+                    f.z = static_cast<C>(frz) / t0; f.y = static_cast<C>(fry) / t1; f.x = static_cast<C>(frx) / t2;
+                }
+                Scale<C> sc;
+                sc.identity = true;
+                vec2<C> f2;
+                const C fz = grid2slice<false>(f, sc, rot, C(0), C(0), f2);
+                valid[g] = base + g * 64 + lane < total && !(dot3(f) > cutoff_sqd) && fabs(fz) <= fb;
+                if constexpr (sizeof(C) == 4) {
+                    window[g] = wpoly ? window_poly(fz, inv_fb, wa) : windowed_sinc<C>(fz, fs, fb);
+                } else {
+                    window[g] = windowed_sinc<C>(fz, fs, fb);
+                }
+                lv[g] = (lz * WY + ly) * WX + lx;
+                tp[g] = fetch_taps<INTERP>(sl, p.is_y, p.is_x, wsl, p.wis_y, p.wis_x, win_present, Hs, Ws,
+                                           p.slices_centered, f2.y * hs_f, f2.x * ws_f, valid[g]);
+            }
+            // Phase 2: interpolate (Interpolation.hpp:435-526) and accumulate.
+#pragma unroll
+            for (int g = 0; g < WG; ++g) {
+                if (!valid[g]) continue;
+                In v;
+                W wv;
+                combine_taps<INTERP>(tp[g], win_present, v, wv);
+                acc_v[lv[g]] = vadd(acc_v[lv[g]], vmul(v, static_cast<RIn>(window[g])));
+                if (has_wvol)
+                    acc_w[lv[g]] = acc_w[lv[g]] + (win_present ? wv * static_cast<W>(window[g]) : static_cast<W>(window[g]));
+            }
+        }
+    }
+    __syncthreads();
+
+    if (item.w >= 0) {
+        // Split sub-brick: store this segment's sums (every slot; the reduction applies the cutoff).
+        In* pv = reinterpret_cast<In*>(p.partials + static_cast<i64>(item.w) * WVOX * 3);
+        W* pw = reinterpret_cast<W*>(p.partials + static_cast<i64>(item.w) * WVOX * 3 + WVOX * 2);
+        for (i32 v = lane; v < WVOX; v += 64) {
+            pv[v] = acc_v[v];
+            pw[v] = acc_w[v];
+        }
+        return;
+    }
+    // value and weights of every in-cutoff voxel are added to the volume (:516-518).
+    for (i32 v = lane; v < WVOX; v += 64) {
+        const i32 lz = v / (WY * WX), ly = (v / WX) % WY, lx = v % WX;
+        const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
+        if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
+        vec3<C> f;
+        f.z = static_cast<C>(index2frequency(vc, z, grid[0])) / p.target[0];
+        f.y = static_cast<C>(index2frequency(vc, y, grid[1])) / p.target[1];
+        f.x = static_cast<C>(x) / p.target[2];
+        if (dot3(f) > p.cutoff_sqd) continue;
+        Out* o = static_cast<Out*>(p.vol) + (z * p.vs0 + y * p.vs1 + x * p.vs2);
+        *o = vadd(*o, vcast<Out>(acc_v[v]));
+        if (p.has_wvol) {
+            W* w = p.wvol + (z * p.wvs0 + y * p.wvs1 + x * p.wvs2);
+            *w = *w + acc_w[v];
+        }
+    }
+}
+
 // Adds the partial sums of each split brick to the volume, in segment order.
 template<class C, class In, class Out, class W>
 __global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertParams<C, In, W> p) {
@@ -579,10 +983,12 @@ __global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertPa
     const i32 brick = red.x;
     const i32 bxi = brick % p.nb[2];
     const i32 rem = brick / p.nb[2];
-    const i32 org[3] = {(rem / p.nb[1]) * BE, (rem % p.nb[1]) * BE, bxi * BE};
+    const i32 bd0 = p.bd[0], bd1 = p.bd[1], bd2 = p.bd[2];
+    const i32 org[3] = {(rem / p.nb[1]) * bd0, (rem % p.nb[1]) * bd1, bxi * bd2};
     const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
-    for (i32 v = static_cast<i32>(threadIdx.x); v < BE * BE * BE; v += NT_REDUCE) {
-        const i32 lz = v / (BE * BE), ly = (v / BE) % BE, lx = v % BE;
+    const i64 slot = static_cast<i64>(bd0) * bd1 * p.apitch;
+    for (i32 v = static_cast<i32>(threadIdx.x); v < bd0 * bd1 * bd2; v += NT_REDUCE) {
+        const i32 lz = v / (bd1 * bd2), ly = (v / bd2) % bd1, lx = v % bd2;
         const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
         if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
         vec3<C> f;
@@ -590,13 +996,13 @@ __global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertPa
         f.y = static_cast<C>(index2frequency(p.vol_centered, y, p.grid[1])) / p.target[1];
         f.x = static_cast<C>(x) / p.target[2];
         if (dot3(f) > p.cutoff_sqd) continue;
-        const i32 a = acc_index(lz, ly, lx);
+        const i32 a = (lz * bd1 + ly) * p.apitch + lx;
         In sv = vzero<In>();
         W sw = W(0);
         for (i32 s = 0; s < red.z; ++s) {
-            const float* base = p.partials + static_cast<i64>(red.y + s) * ACC * 3;
+            const float* base = p.partials + static_cast<i64>(red.y + s) * slot * 3;
             sv = vadd(sv, reinterpret_cast<const In*>(base)[a]);
-            sw = sw + reinterpret_cast<const W*>(base + ACC * 2)[a];
+            sw = sw + reinterpret_cast<const W*>(base + slot * 2)[a];
         }
         Out* o = static_cast<Out*>(p.vol) + (z * p.vs0 + y * p.vs1 + x * p.vs2);
         *o = vadd(*o, vcast<Out>(sv));
@@ -746,35 +1152,24 @@ WindowFit fit_window(float fs, float fb) {
     return fit;
 }
 
-// Work plan of the staged kernel for one geometry: the active bricks (box within the cutoff sphere), each cut
-// into slice segments so that the expected number of passing slices per item is at most kListTarget. The
-// expected count assumes orientations uniform on the sphere: a brick at distance r meets a fraction
-// min(1, t / r) of the slabs, t = f_b + mean projected half-extent. Items are sorted by expected work, largest
-// first. Kept per (device, geometry, slice count, window) for the process lifetime.
-constexpr double kListTarget = 384.0;
-
-// Threads per workgroup of the staged kernel: 256 (default) or 512, from NOA_HIP_INSERT_THREADS.
-int staged_threads() {
-    static const int n = [] {
-        const char* e = std::getenv("NOA_HIP_INSERT_THREADS");
-        return e && std::atoi(e) == 512 ? 512 : 256;
-    }();
-    return n;
-}
-
+// Work plan of the culled kernels for one geometry: the active bricks (box within the cutoff sphere), each cut into
+// slice segments so that the expected number of passing slices per item is at most `list_target`. The expected
+// count assumes orientations uniform on the sphere: a brick at distance r meets a fraction min(1, t / r) of the
+// slabs, t = f_b + mean projected half-extent. Items are sorted by expected work, largest first. Kept per (device,
+// geometry, brick shape, slice count, window) for the process lifetime.
 struct Plan {
     int4* items{};
     int4* reduce{};
     i32 n_items{}, n_reduce{}, n_partials{};
 };
-using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double>;
+using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double, i32, i32, i32, double>;
 std::mutex g_plan_mutex;
 std::map<PlanKey, Plan> g_plans;
 
 int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff, bool centered, i32 n_slices,
-              double fftfreq_blackman, Plan& plan, std::string& err) {
+              double fftfreq_blackman, const i32 bd[3], double list_target, Plan& plan, std::string& err) {
     const PlanKey key{device, grid[0], grid[1], grid[2], target[0], target[1], target[2], cutoff, centered, n_slices,
-                      fftfreq_blackman};
+                      fftfreq_blackman, bd[0], bd[1], bd[2], list_target};
     std::lock_guard<std::mutex> lock(g_plan_mutex);
     auto it = g_plans.find(key);
     if (it != g_plans.end()) {
@@ -782,7 +1177,7 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
         return NOA_HIP_OK;
     }
     const i32 ext[3] = {grid[0], grid[1], grid[2] / 2 + 1};
-    const i32 nb[3] = {(ext[0] + BE - 1) / BE, (ext[1] + BE - 1) / BE, (ext[2] + BE - 1) / BE};
+    const i32 nb[3] = {(ext[0] + bd[0] - 1) / bd[0], (ext[1] + bd[1] - 1) / bd[1], (ext[2] + bd[2] - 1) / bd[2]};
     const double c2 = std::max(cutoff, 0.0) * std::max(cutoff, 0.0);
     struct Entry { double work; int4 item; };
     std::vector<Entry> entries;
@@ -791,11 +1186,11 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
     for (i32 bz = 0; bz < nb[0]; ++bz)
         for (i32 by = 0; by < nb[1]; ++by)
             for (i32 bx = 0; bx < nb[2]; ++bx) {
-                const i32 org[3] = {bz * BE, by * BE, bx * BE};
+                const i32 org[3] = {bz * bd[0], by * bd[1], bx * bd[2]};
                 double lo[3], hi[3];
                 for (int a = 0; a < 3; ++a) {
                     lo[a] = 1e300; hi[a] = -1e300;
-                    for (i32 k = 0; k < BE && org[a] + k < ext[a]; ++k) {
+                    for (i32 k = 0; k < bd[a] && org[a] + k < ext[a]; ++k) {
                         const i32 f = a == 2 ? org[a] + k : host_freq(centered, org[a] + k, grid[a]);
                         const double fn = static_cast<double>(f) / static_cast<double>(target[a]);
                         lo[a] = std::min(lo[a], fn);
@@ -814,7 +1209,7 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
                 const i32 brick = (bz * nb[1] + by) * nb[2] + bx;
                 const double density = std::min(1.0, t / std::max(std::sqrt(r2), 1e-12));
                 const double expected = density * n_slices;
-                const i32 n_seg = std::max<i32>(1, static_cast<i32>(std::ceil(expected / kListTarget)));
+                const i32 n_seg = std::max<i32>(1, static_cast<i32>(std::ceil(expected / list_target)));
                 const i32 len = (n_slices + n_seg - 1) / n_seg;
                 const i32 first_slot = n_seg > 1 ? n_partials : -1;
                 i32 made = 0;
@@ -855,6 +1250,27 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
     return NOA_HIP_OK;
 }
 
+// Culled kernel of the insertion: the one-wave sub-brick kernel (default) or the staged workgroup kernel
+// (NOA_HIP_INSERT_KERNEL=staged, with NOA_HIP_INSERT_THREADS=256|512), and the slice count per item.
+struct KernelChoice {
+    bool staged;
+    int threads;
+    double list_target;
+};
+KernelChoice kernel_choice() {
+    static const KernelChoice c = [] {
+        KernelChoice k{};
+        const char* e = std::getenv("NOA_HIP_INSERT_KERNEL");
+        k.staged = e && std::string(e) == "staged";
+        const char* t = std::getenv("NOA_HIP_INSERT_THREADS");
+        k.threads = t && std::atoi(t) == 512 ? 512 : 256;
+        const char* l = std::getenv("NOA_HIP_INSERT_LIST");
+        k.list_target = l ? std::max(16.0, std::atof(l)) : (k.staged ? 384.0 : 256.0);
+        return k;
+    }();
+    return c;
+}
+
 template<class C, class In, class Out, class W>
 int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     InsertParams<C, In, W> p{};
@@ -879,9 +1295,14 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
         p.inv_target[i] = C(1) / p.target[i];
     }
     p.wr = p.grid[2] / 2 + 1;
-    p.nb[0] = (p.grid[0] + BE - 1) / BE;
-    p.nb[1] = (p.grid[1] + BE - 1) / BE;
-    p.nb[2] = (p.wr + BE - 1) / BE;
+    const KernelChoice kc = kernel_choice();
+    p.bd[0] = kc.staged ? BE : WZ;
+    p.bd[1] = kc.staged ? BE : WY;
+    p.bd[2] = kc.staged ? BE : WX;
+    p.apitch = kc.staged ? AP : WX;
+    p.nb[0] = (p.grid[0] + p.bd[0] - 1) / p.bd[0];
+    p.nb[1] = (p.grid[1] + p.bd[1] - 1) / p.bd[1];
+    p.nb[2] = (p.wr + p.bd[2] - 1) / p.bd[2];
     p.slice_f[0] = static_cast<C>(d.in_shape[2]);
     p.slice_f[1] = static_cast<C>(d.in_shape[3]);
     p.slice_shape[0] = static_cast<i32>(d.in_shape[2]);
@@ -920,14 +1341,16 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             (void)hipGetDevice(&device);
             Plan plan;
             const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
-                                     p.n_slices, static_cast<double>(p.fftfreq_blackman), plan, err);
+                                     p.n_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target, plan,
+                                     err);
             if (st != NOA_HIP_OK) return st;
             if (plan.n_items == 0) return NOA_HIP_OK;
             p.items = plan.items;
             p.reduce = plan.reduce;
             void* partials = nullptr;
             if (plan.n_partials > 0) {
-                const size_t bytes = static_cast<size_t>(plan.n_partials) * ACC * 3 * sizeof(float);
+                const size_t bytes = static_cast<size_t>(plan.n_partials) * p.bd[0] * p.bd[1] * p.apitch * 3 *
+                                     sizeof(float);
                 if (hipMallocAsync(&partials, bytes, stream) != hipSuccess) {
                     err = "insert: could not allocate the partial sums";
                     return NOA_HIP_ERROR_RUNTIME;
@@ -935,7 +1358,10 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             }
             p.partials = static_cast<float*>(partials);
             const dim3 grid_items(static_cast<unsigned>(plan.n_items));
-            if (staged_threads() == 512) {
+            if (!kc.staged) {
+                if (nearest) hipLaunchKernelGGL((insert_wave_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST>), grid_items, dim3(64), 0, stream, p);
+                else hipLaunchKernelGGL((insert_wave_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR>), grid_items, dim3(64), 0, stream, p);
+            } else if (kc.threads == 512) {
                 if (nearest) hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, 512>), grid_items, dim3(512), 0, stream, p);
                 else hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, 512>), grid_items, dim3(512), 0, stream, p);
             } else {

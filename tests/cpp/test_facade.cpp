@@ -1,0 +1,184 @@
+// Test program of the C++ facade (include/noa_hip.hpp).
+//
+//   test_facade          descriptors built from noa-style arguments, and validation errors (no GPU needed)
+//   test_facade --gpu    identity-pose extraction and insertion through the facade on device 0 (closed forms)
+//
+// Prints one line per check and exits non-zero on the first failure.
+#include "noa_hip.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace nx = noa_hip::xform;
+using noa_hip::c32;
+using noa_hip::Empty;
+using noa_hip::Mat33;
+using noa_hip::Quaternion;
+using noa_hip::Shape4;
+using noa_hip::View;
+
+#define EXPECT(cond)                                                                 \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);              \
+            std::exit(1);                                                            \
+        }                                                                            \
+    } while (0)
+
+#define HIP_OK(x) EXPECT((x) == hipSuccess)
+
+static void check_layouts() {
+    EXPECT(nx::Layout("h2h").bits == NOA_HIP_H2H);
+    EXPECT(nx::Layout("h").bits == NOA_HIP_H2H);
+    EXPECT(nx::Layout("hc2h").bits == NOA_HIP_HC2H);
+    EXPECT(nx::Layout("h2hc").bits == NOA_HIP_H2HC);
+    EXPECT(nx::Layout("hc2hc").bits == NOA_HIP_HC2HC);
+    EXPECT(nx::Layout("hc").bits == NOA_HIP_HC2HC);
+    std::printf("ok layouts\n");
+}
+
+static void check_descriptor() {
+    // An insertion of 5 slices 64x64 (rfft 64x33, padded rows of 40) into a 64^3 volume, per-slice inverse
+    // rotations on the device (here a fake device address: nothing is dereferenced), constant slice weight.
+    auto* slices = reinterpret_cast<c32*>(0x1000);
+    auto* volume = reinterpret_cast<c32*>(0x2000);
+    auto* rot = reinterpret_cast<Mat33<float>*>(0x3000);
+    const View<c32> s(slices, Shape4(5, 1, 64, 33), Shape4(64 * 40, 64 * 40, 40, 1));
+    const View<c32> v(volume, Shape4(1, 64, 64, 33));
+    const View<Mat33<float>> r(rot, Shape4(1, 1, 1, 5));
+    nx::InsertCentralSlicesOptions opt;
+    opt.interp = nx::Interp::NEAREST;
+    opt.windowed_sinc = {0.02, 0.05};
+    opt.fftfreq_cutoff = 0.4;
+    opt.target_shape = Shape4(1, 128, 128, 128);
+    const auto a = nx::details::make_args(nx::Layout("hc2h").bits, s, 2.5f, Shape4(5, 1, 64, 64), v, Empty{},
+                                          Shape4(1, 64, 64, 64), Empty{}, r, opt);
+    EXPECT(a.remap == NOA_HIP_HC2H);
+    EXPECT(a.interp == NOA_HIP_INTERP_NEAREST);
+    EXPECT(a.input.data == slices && a.input.dtype == NOA_HIP_C32);
+    EXPECT(a.input.shape[0] == 5 && a.input.shape[3] == 33 && a.input.strides[2] == 40 && a.input.strides[0] == 2560);
+    EXPECT(a.input_weight.is_value == 1 && a.input_weight.dtype == NOA_HIP_F32 && a.input_weight.value[0] == 2.5);
+    EXPECT(a.output.data == volume && a.output.strides[1] == 64 * 33 && a.output.strides[3] == 1);
+    EXPECT(a.output_weight.data == nullptr && a.output_weight.is_value == 0);
+    EXPECT(a.input_shape[3] == 64 && a.output_shape[1] == 64);
+    EXPECT(a.target_shape[1] == 128);
+    EXPECT(a.rotation.kind == NOA_HIP_MAT33_F32 && a.rotation.batched == 1 && a.rotation.count == 5 &&
+           a.rotation.data == rot);
+    EXPECT(a.scaling.kind == NOA_HIP_XFORM_NONE);
+    EXPECT(a.fftfreq_cutoff == 0.4 && a.fftfreq_sinc == 0.02 && a.fftfreq_blackman == 0.05);
+
+    // One broadcast quaternion: a host value.
+    const Quaternion<double> q{0, 0, 0, 1};
+    const auto b = nx::details::make_args(NOA_HIP_HC2HC, v, Empty{}, Shape4(1, 64, 64, 64), s, Empty{},
+                                          Shape4(5, 1, 64, 64), Empty{}, q, nx::ExtractCentralSlicesOptions{});
+    EXPECT(b.rotation.kind == NOA_HIP_QUAT_F64 && b.rotation.batched == 0 && b.rotation.data == &q);
+    EXPECT(b.interp == NOA_HIP_INTERP_LINEAR && b.fftfreq_cutoff == 0.5 && b.fftfreq_sinc == -1);
+    std::printf("ok descriptors\n");
+}
+
+static void check_errors() {
+    // Wrong volume shape for the array: rejected before anything touches the (fake) memory.
+    const View<c32> s(reinterpret_cast<c32*>(0x10000000), Shape4(2, 1, 32, 17));
+    const View<c32> v(reinterpret_cast<c32*>(0x20000000), Shape4(1, 32, 32, 17));
+    const Mat33<float> eye{{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    bool thrown = false;
+    try {
+        nx::extract_central_slices_3d<"hc2hc">(v, Empty{}, Shape4(1, 16, 32, 32), s, Empty{}, Shape4(2, 1, 32, 32),
+                                             Empty{}, eye);
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+        EXPECT(std::strstr(e.what(), "shape") != nullptr);
+    }
+    EXPECT(thrown);
+    // CUBIC is a valid noa call that this library does not implement.
+    thrown = false;
+    try {
+        nx::ExtractCentralSlicesOptions opt;
+        opt.interp = nx::Interp::CUBIC;
+        nx::extract_central_slices_3d<"hc2hc">(v, Empty{}, Shape4(1, 32, 32, 32), s, Empty{}, Shape4(2, 1, 32, 32),
+                                             Empty{}, eye, opt);
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+        EXPECT(std::strstr(e.what(), "nterp") != nullptr);
+    }
+    EXPECT(thrown);
+}
+
+// Identity poses on device 0: the extracted slice is the central plane z = 0 of the volume (exactly), and the
+// inserted slice lands on the plane z = 0 multiplied by the window at 0 (noa's windowed_sinc(0)).
+static void check_gpu() {
+    const int n = 32, wr = n / 2 + 1;
+    const size_t nvox = static_cast<size_t>(n) * n * wr, npix = static_cast<size_t>(n) * wr;
+    std::vector<c32> hv(nvox);
+    for (size_t i = 0; i < nvox; ++i) hv[i] = c32(static_cast<float>(i % 97) - 40.f, static_cast<float>(i % 31));
+    c32 *dv = nullptr, *ds = nullptr, *dins = nullptr;
+    float* dw = nullptr;
+    HIP_OK(hipMalloc(&dv, nvox * sizeof(c32)));
+    HIP_OK(hipMalloc(&ds, npix * sizeof(c32)));
+    HIP_OK(hipMalloc(&dins, nvox * sizeof(c32)));
+    HIP_OK(hipMalloc(&dw, nvox * sizeof(float)));
+    HIP_OK(hipMemcpy(dv, hv.data(), nvox * sizeof(c32), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(dins, 0, nvox * sizeof(c32)));
+    HIP_OK(hipMemset(dw, 0, nvox * sizeof(float)));
+
+    const Mat33<float> eye{{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const View<c32> vol(dv, Shape4(1, n, n, wr));
+    const View<c32> sl(ds, Shape4(1, 1, n, wr));
+    nx::extract_central_slices_3d<"h2h">(vol, Empty{}, Shape4(1, n, n, n), sl, Empty{}, Shape4(1, 1, n, n), Empty{},
+                                         eye);
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<c32> hs(npix);
+    HIP_OK(hipMemcpy(hs.data(), ds, npix * sizeof(c32), hipMemcpyDeviceToHost));
+    // Non-centred volume: plane z = 0 is index 0. Pixels beyond the 0.5 cutoff are 0.
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < wr; ++x) {
+            const int fy = y < (n + 1) / 2 ? y : y - n;
+            const double f2 = (static_cast<double>(fy) / n) * (static_cast<double>(fy) / n) +
+                              (static_cast<double>(x) / n) * (static_cast<double>(x) / n);
+            const c32 want = f2 > 0.25 ? c32(0, 0) : hv[static_cast<size_t>(y) * wr + x];
+            EXPECT(hs[static_cast<size_t>(y) * wr + x] == want);
+        }
+    std::printf("ok gpu extraction (identity pose, exact)\n");
+
+    const View<c32> ins(dins, Shape4(1, n, n, wr));
+    const View<float> w(dw, Shape4(1, n, n, wr));
+    nx::insert_central_slices_3d<"h2h">(sl, Empty{}, Shape4(1, 1, n, n), ins, w, Shape4(1, n, n, n), Empty{}, eye);
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<c32> hi(nvox);
+    std::vector<float> hw(nvox);
+    HIP_OK(hipMemcpy(hi.data(), dins, nvox * sizeof(c32), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(hw.data(), dw, nvox * sizeof(float), hipMemcpyDeviceToHost));
+    const float w0 = 0.99999994f;   // noa's windowed_sinc(0) in FP32: sinc(0) * (0.42 + 0.5 + 0.08)
+    double max_err = 0;
+    for (int z = 0; z < n; ++z)
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < wr; ++x) {
+                const size_t i = (static_cast<size_t>(z) * n + y) * wr + x;
+                const c32 want = z == 0 ? hs[static_cast<size_t>(y) * wr + x] * w0 : c32(0, 0);
+                max_err = std::fmax(max_err, std::abs(hi[i] - want));
+                if (z != 0) EXPECT(std::fabs(hw[i]) < 1e-6f);
+            }
+    std::printf("gpu insertion max abs error %.3g\n", max_err);
+    EXPECT(max_err < 1e-4);
+    std::printf("ok gpu insertion (identity pose, plane z = 0)\n");
+    HIP_OK(hipFree(dv));
+    HIP_OK(hipFree(ds));
+    HIP_OK(hipFree(dins));
+    HIP_OK(hipFree(dw));
+}
+
+int main(int argc, char** argv) {
+    check_layouts();
+    check_descriptor();
+    check_errors();
+    if (argc > 1 && std::strcmp(argv[1], "--gpu") == 0) check_gpu();
+    std::printf("all facade checks passed\n");
+    return 0;
+}

@@ -1,0 +1,169 @@
+"""The C ABI without a GPU: exported symbols, struct layouts, enum values, defaults and validation errors.
+
+Every check runs on the host; calls that reach the device checks fail there with a runtime error, after the
+argument rules (which run first) have passed.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = ROOT / "include" / "noa_hip.h"
+LIB = ROOT / "noa_amd" / "_build" / "libnoa_hip.so"
+FACADE_TEST = ROOT / "tests" / "_build" / "test_facade"
+
+
+def header_functions() -> list[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(noa_hip_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not LIB.exists():
+        pytest.fail(f"{LIB} is not built (run __graft_entry__.build())")
+    from noa_amd import _lib
+    return _lib.lib()
+
+
+def test_library_exports_every_header_function():
+    names = header_functions()
+    assert {"noa_hip_rasterize_central_slices_3d", "noa_hip_insert_central_slices_3d",
+            "noa_hip_extract_central_slices_3d", "noa_hip_last_error"} <= set(names)
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, f"declared in noa_hip.h but not exported: {missing}"
+
+
+def test_struct_layout_matches_the_c_compiler(tmp_path):
+    """offsetof/sizeof of every ABI struct from gcc equals the ctypes mirror."""
+    from noa_amd import abi
+    fields = {
+        "noa_hip_view": ("View", ["data", "shape", "strides", "dtype", "is_value", "value"]),
+        "noa_hip_transform": ("Transform", ["data", "kind", "batched", "count"]),
+        "noa_hip_project_args": ("ProjectArgs", [f[0] for f in abi.ProjectArgs._fields_]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "noa_hip.h"', "int main(void) {"]
+    for c_name, (_, names) in fields.items():
+        lines.append(f'printf("{c_name} size %zu\\n", sizeof({c_name}));')
+        for n in names:
+            lines.append(f'printf("{c_name} {n} %zu\\n", offsetof({c_name}, {n}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        struct, field, value = line.split()
+        got[(struct, field)] = int(value)
+    for c_name, (py_name, names) in fields.items():
+        cls = getattr(abi, py_name)
+        assert got[(c_name, "size")] == ctypes.sizeof(cls), c_name
+        for n in names:
+            assert got[(c_name, n)] == getattr(cls, n).offset, (c_name, n)
+
+
+def test_enum_values_match_noa(lib):
+    from noa_amd import abi
+    text = HEADER.read_text()
+    for name, value in [("NOA_HIP_H2H", abi.H2H), ("NOA_HIP_HC2H", abi.HC2H), ("NOA_HIP_H2HC", abi.H2HC),
+                        ("NOA_HIP_HC2HC", abi.HC2HC), ("NOA_HIP_C32", abi.C32), ("NOA_HIP_MAT33_F32", abi.MAT33_F32),
+                        ("NOA_HIP_QUAT_F64", abi.QUAT_F64)]:
+        m = re.search(rf"\b{name}\s*=\s*(\d+)", text)
+        assert m and int(m.group(1)) == value, name
+    # noa's Layout bits (fft/core/Layout.hpp:39-46): SRC_FULL=1, DST_FULL=2, SRC_CENTERED=4, DST_CENTERED=8.
+    assert (abi.H2H, abi.HC2H, abi.H2HC, abi.HC2HC) == (0, 4, 8, 12)
+    assert lib.noa_hip_abi_version() == abi.ABI_VERSION
+
+
+def test_args_init_sets_noa_defaults(lib):
+    from noa_amd import abi
+    a = abi.ProjectArgs()
+    lib.noa_hip_project_args_init(ctypes.byref(a))
+    assert a.fftfreq_cutoff == 0.5 and a.fftfreq_sinc == -1 and a.fftfreq_blackman == -1
+    assert a.interp == 1 and a.remap == abi.H2H and a.scaling.kind == abi.XFORM_NONE
+    assert list(a.ews_radius) == [0, 0] and list(a.target_shape) == [0, 0, 0, 0]
+
+
+def _args(lib, n=16, b=2, remap=0):
+    """An extraction of b slices from an n^3 volume, with host buffers (never dereferenced by the checks)."""
+    from noa_amd import abi
+    vol = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    out = np.zeros((b, 1, n, n // 2 + 1), np.complex64)
+    rot = np.zeros((b, 12), np.float32)
+    a = abi.ProjectArgs()
+    lib.noa_hip_project_args_init(ctypes.byref(a))
+
+    def view(v, arr, dtype):
+        v.data = arr.ctypes.data
+        for i in range(4):
+            v.shape[i] = arr.shape[i]
+            v.strides[i] = arr.strides[i] // arr.itemsize
+        v.dtype = dtype
+
+    view(a.input, vol, abi.C32)
+    view(a.output, out, abi.C32)
+    for i, s in enumerate((1, n, n, n)):
+        a.input_shape[i] = s
+    for i, s in enumerate((b, 1, n, n)):
+        a.output_shape[i] = s
+    a.remap = remap
+    a.rotation.data = rot.ctypes.data
+    a.rotation.kind = abi.MAT33_F32
+    a.rotation.batched = 1
+    a.rotation.count = b
+    return a, (vol, out, rot)
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda a: a.input_shape.__setitem__(1, 8), "shape"),
+    (lambda a: setattr(a, "remap", 1), "emap"),
+    (lambda a: setattr(a.rotation, "count", 3), "rotation"),
+    (lambda a: setattr(a, "interp", 3), "nterp"),
+])
+def test_validation_errors_before_any_device_access(lib, mutate, message):
+    from noa_amd import abi
+    a, keep = _args(lib)
+    mutate(a)
+    status = lib.noa_hip_extract_central_slices_3d(ctypes.byref(a), None)
+    err = lib.noa_hip_last_error().decode()
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
+    assert message in err, err
+    del keep
+
+
+def test_valid_arguments_reach_the_device_check(lib):
+    """Host buffers pass every argument rule and are rejected only by the device-memory check."""
+    from noa_amd import abi
+    a, keep = _args(lib)
+    status = lib.noa_hip_extract_central_slices_3d(ctypes.byref(a), None)
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_RUNTIME)
+    assert "device" in lib.noa_hip_last_error().decode().lower()
+    del keep
+
+
+def test_cpp_facade_descriptors_and_errors():
+    """include/noa_hip.hpp: noa-style arguments -> descriptor fields, and errors as noa_hip::Exception."""
+    if not FACADE_TEST.exists():
+        pytest.fail(f"{FACADE_TEST} is not built (run __graft_entry__.build())")
+    res = subprocess.run([str(FACADE_TEST)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "all facade checks passed" in res.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_facade_on_the_gpu():
+    """The facade end to end on device 0: identity-pose extraction (exact) and insertion (closed form)."""
+    res = subprocess.run([str(FACADE_TEST), "--gpu"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "ok gpu extraction" in res.stdout and "ok gpu insertion" in res.stdout

@@ -350,3 +350,121 @@ def test_insert_matrix_equals_quaternion():
     q = quaternion_from_matrix(inv).astype(np.float32)
     b = _insert("h2h", s, n, q, quaternion=True)
     assert rel_l2(a, b) < 1e-5
+
+
+# ------------------------------------------------------------------------------------------------ EWS, scaling,
+# w-window and target shape: the oracle branches beyond plain central slices.
+def _slice_freqs(n, centered):
+    """(H, W/2+1, 2) normalised (fy, fx) of the slice pixels."""
+    v = freq_grid(n, centered).astype(np.float64) / n
+    u = np.arange(n // 2 + 1, dtype=np.float64) / n
+    return np.stack(np.broadcast_arrays(v[:, None], u[None, :]), -1)
+
+
+@pytest.mark.parametrize("radius", [(2.0, 2.0), (2.0, -3.0)])
+def test_extract_ews_follows_the_ewald_sphere(radius):
+    """With an Ewald radius r (per axis), a slice pixel at (fy, fx) lies at fz = r - sqrt(r^2 - f^2) on the sphere
+    through the origin; noa's paraboloid fz = fy^2/(2 r_y) + fx^2/(2 r_x) agrees to f^4/(8 r^3). A linear field
+    V = c + i a.f (index units) read at identity pose must match the sphere within that bound; a wrong sign, a
+    missing factor 2 or swapped axes miss it by ~f^2/(2r) (~1 sample here)."""
+    n = 32
+    c, a = 0.3, np.array([1.7, 0.2, -0.4])
+    vol = linear_field(n, n, n, False, c, a)
+    out = _extract("h2h", vol, n, np.eye(3, dtype=np.float32)[None], cutoff=0.45, ews_radius=radius)[0, 0]
+    f = _slice_freqs(n, False)
+    ry, rx = radius
+    fz_sphere = (ry - np.sign(ry) * np.sqrt(ry * ry - f[..., 0] ** 2)) + (rx - np.sign(rx) * np.sqrt(rx * rx - f[..., 1] ** 2))
+    want = c + 1j * (a[0] * fz_sphere * n + a[1] * f[..., 0] * n + a[2] * f[..., 1] * n)
+    rho2 = (f ** 2).sum(-1) + fz_sphere ** 2
+    ok = rho2 <= 0.45 ** 2 * 0.98
+    bound = abs(a[0]) * n * ((f[..., 0] ** 4) / (8 * abs(ry) ** 3) + (f[..., 1] ** 4) / (8 * abs(rx) ** 3)) + 1e-4
+    assert ok.sum() > 300
+    assert np.all(np.abs(out[ok] - want[ok]) <= bound[ok] * 1.05 + 2e-5 * np.abs(want[ok]))
+    # The paraboloid term is large enough to matter: without it the error would be ~ |a_z| n f^2 / (2 r).
+    assert np.abs(a[0] * n * (f[..., 0] ** 2 / (2 * ry))[ok]).max() > 20 * bound[ok].max()
+
+
+def test_extract_scaling_is_an_in_plane_transform_before_the_rotation():
+    """Inverse scaling S acts on the slice's (fy, fx) before the rotation: with S an in-plane rotation Q, extracting
+    with (S=Q, R) equals extracting with (no S, R . diag(1, Q)). A transposed S or S applied after R fails."""
+    rng = np.random.default_rng(21)
+    n = 24
+    vol = random_spectrum((n, n, n // 2 + 1), rng)
+    t = 0.37
+    q = np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+    big = np.eye(3)
+    big[1:, 1:] = q
+    # Cutoff 0.45: no pixel lies within rounding distance of the cutoff circle (|f|^2 n^2 is an integer), where
+    # the two equivalent forms could round to different sides.
+    for k in range(3):
+        r = random_rotations(1, rng, dtype=np.float64)[0]
+        a = _extract("hc2hc", vol, n, r.astype(np.float32)[None], cutoff=0.45)
+        out = np.zeros_like(a)
+        O.extract_central_slices_3d("hc2hc", vol[None], None, (1, n, n, n), out, None, (1, 1, n, n),
+                                    q.astype(np.float32), r.astype(np.float32)[None], fftfreq_cutoff=0.45)
+        b = _extract("hc2hc", vol, n, (r @ big).astype(np.float32)[None], cutoff=0.45)
+        assert rel_l2(out, b) < 1e-5
+        assert rel_l2(out, a) > 1e-2
+
+
+def test_insert_scaling_is_an_in_plane_transform_after_the_rotation():
+    """Forward scaling S acts on (fy, fx) after the inverse rotation: (S=Q, R_inv) == (no S, diag(1, Q) . R_inv)."""
+    rng = np.random.default_rng(22)
+    n = 16
+    s = random_spectrum((3, 1, n, n // 2 + 1), rng)
+    t = -0.52
+    q = np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+    big = np.eye(3)
+    big[1:, 1:] = q
+    inv = np.ascontiguousarray(np.transpose(random_rotations(3, rng, dtype=np.float64), (0, 2, 1)))
+    vol_a = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    O.insert_central_slices_3d("h2h", s, None, (3, 1, n, n), vol_a, None, (1, n, n, n), q.astype(np.float32),
+                               inv.astype(np.float32))
+    vol_b = _insert("h2h", s, n, (big @ inv).astype(np.float32))
+    vol_c = _insert("h2h", s, n, (inv @ big).astype(np.float32))
+    assert rel_l2(vol_a, vol_b) < 1e-5
+    assert rel_l2(vol_a, vol_c) > 1e-2
+
+
+def test_extract_w_window_is_the_window_sum_for_a_linear_field():
+    """The w-windowed extraction (window > 1) sums noa's windowed sinc at offsets (w - size/2)/D along the volume's
+    z. For a linear field the odd part cancels (symmetric offsets), so the result is (sum of the window) x the
+    plain extraction. The window is recomputed here from numpy's sinc and scipy's Blackman form."""
+    rng = np.random.default_rng(23)
+    n = 32
+    c, a = 0.4, np.array([0.6, -0.8, 0.3])
+    vol = linear_field(n, n, n, True, c, a)
+    fs, fb = 2.0 / n, 3.0 / n       # f_s = 1/n would put every non-zero offset on a zero of the sinc
+    size = O.blackman_window_size(fb, n)
+    assert size == 5
+    offsets = (np.arange(size) - size // 2) / n
+    wsum = np.sum(np.sinc(offsets / fs) * (0.42 + 0.5 * np.cos(np.pi * offsets / fb) + 0.08 * np.cos(2 * np.pi * offsets / fb)))
+    rot = random_rotations(4, rng)
+    plain = _extract("hc2hc", vol, n, rot, cutoff=0.3)
+    win = _extract("hc2hc", vol, n, rot, cutoff=0.3, fftfreq_sinc=fs, fftfreq_blackman=fb)
+    f = _slice_freqs(n, True)
+    inner = (f ** 2).sum(-1) <= (0.3 - offsets.max()) ** 2 * 0.99
+    for b in range(4):
+        assert rel_l2(win[b, 0][inner], wsum * plain[b, 0][inner]) < 1e-5
+    assert abs(wsum - 1.0) > 0.05          # the sum differs from 1: a dropped weight would be visible
+
+
+def test_extract_target_shape_scales_frequencies():
+    """With a target shape T, slice frequencies land on f * T samples of the volume (not f * D): a linear field in
+    index units gives c + i a.(R f) T."""
+    rng = np.random.default_rng(24)
+    n, t = 32, 48
+    c, a = 0.2, np.array([0.5, 0.9, -0.7])
+    vol = linear_field(n, n, n, False, c, a)
+    rot = random_rotations(3, rng)
+    out = np.zeros((3, 1, n, n // 2 + 1), np.complex64)
+    O.extract_central_slices_3d("h2h", vol[None], None, (1, n, n, n), out, None, (3, 1, n, n), None, rot,
+                                fftfreq_cutoff=0.5, target_shape=(1, t, t, t))
+    f = _slice_freqs(n, False)
+    p = np.concatenate([np.zeros(f.shape[:-1] + (1,)), f], -1)
+    for b in range(3):
+        k = p @ rot[b].astype(np.float64).T * t
+        want = c + 1j * (k @ a)
+        ok = (np.abs(k) <= n / 2 - 2).all(-1)
+        assert ok.sum() > 100
+        assert rel_l2(out[b, 0][ok], want[ok]) < 5e-6
