@@ -30,8 +30,9 @@ def log(*a):
 
 
 def algorithmic_bytes_extract(b, n):
-    """Output slices + volume + one Mat33 payload per slice (each read or written once)."""
-    return b * n * (n // 2 + 1) * 8 + n * n * (n // 2 + 1) * 8 + b * 36
+    """Output slices + volume + one Mat33 payload and one (y, x) f32 shift per slice (each read or written once);
+    the phase shift is fused, so the slices are written once."""
+    return b * n * (n // 2 + 1) * 8 + n * n * (n // 2 + 1) * 8 + b * 36 + b * 8
 
 
 def algorithmic_bytes_insert(b, n):
@@ -79,6 +80,7 @@ def cpu_baseline(ext, ins, n_ext_sample=64, n_ins_sample=48):
         t0 = time.perf_counter()
         O.extract_central_slices_3d("hc2hc", vol, None, (1, n, n, n), out, None, (n_ext_sample, 1, n, n), None, rot,
                                     n_threads=t_ext, force_threads=True)
+        O.phase_shift_2d("hc2hc", out, out, (n_ext_sample, 1, n, n), ext.shifts[:n_ext_sample])
         times.append(time.perf_counter() - t0)
     ext_per_slice = statistics.median(times) / n_ext_sample
 
@@ -100,11 +102,101 @@ def cpu_baseline(ext, ins, n_ext_sample=64, n_ins_sample=48):
         "unit": "slices/s",
         "cores": max(t_ext, t_ins),
         "kind": "port",
-        "sample": (f"oracle (noa CPU algorithm restated, OpenMP): extraction of {n_ext_sample} of the {n_e} poses "
+        "sample": (f"oracle (noa CPU algorithm restated, OpenMP): extraction + phase shift of {n_ext_sample} of the {n_e} poses "
                    f"with {t_ext} threads ({ext_per_slice * 1e3:.3f} ms/slice), insertion of {n_ins_sample} of the "
                    f"{n_i} slices into 256^3 with {t_ins} threads ({ins_per_slice * 1e3:.1f} ms/slice), "
                    f"extrapolated linearly to one step; CPU: {cpu_model()}"),
     }
+
+
+def serving_report(torch, dev, n_requests, qps, budget_gb=16.0):
+    """Poisson stream of mixed-box extraction/insertion requests through the continuous-batching loop
+    (noa_amd.serve): p50/p99 latency and served slices/s. Request inputs are on the device before the stream
+    starts; latency is completion (event passed) minus arrival."""
+    from noa_amd import serve, workloads
+
+    vols = workloads.serving_volumes(dev)
+    reqs = workloads.serving_requests(dev, n_requests=n_requests, qps=qps)
+    server = serve.Server(vols, budget_bytes=int(budget_gb * (1 << 30)), n_streams=3)
+    # Warm-up: the insertion work plans of every box and power-of-two batch (built once per geometry).
+    for box in workloads.SERVING_BOXES:
+        for k in (64, 128, 256, 512):
+            r = serve.Request(rid=-1, kind="insert", box=box, rotations=torch.eye(3, device=dev).expand(k, 3, 3).contiguous(),
+                              slices=torch.zeros((k, 1, box, box // 2 + 1), dtype=torch.complex64, device=dev),
+                              weights=torch.zeros((k, 1, box, box // 2 + 1), dtype=torch.float32, device=dev))
+            serve.run_alone(server, r)
+    server.stats = serve.ServeStats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = serve.run_open_loop(server, reqs, timeout_s=300)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kinds = [r.kind for r in reqs]
+    return {
+        "p50_ms": stats.percentile(50),
+        "p99_ms": stats.percentile(99),
+        "qps": qps,
+        "requests": len(reqs),
+        "extract_requests": kinds.count("extract"),
+        "insert_requests": kinds.count("insert"),
+        "slices_per_s": stats.slices / wall,
+        "launches": stats.launches,
+        "merged_requests": stats.merged,
+        "wall_s": wall,
+        "mix": "box {64,128,192,256,384} p {.3,.3,.2,.15,.05}; extract K~U{1..512} from 4 resident volumes per box "
+               "or insert K CTF-weighted slices into a per-request accumulator; Poisson arrivals",
+    }
+
+
+def precision_report(ext, nx, torch, n_ref=128):
+    """Extraction from the 256^3 volume stored as FP32, FP16 pairs and BF16 pairs (FP32 math): rel-L2 and max error
+    of each against the FP64 oracle (double volume and rotations) on the first n_ref poses, and the time of each for
+    all poses."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as O
+
+    n = ext.volume.shape[1]
+    b = ext.slice_shape[0]
+    vol64 = ext.volume.cpu().numpy().astype(np.complex128)
+    rot64 = ext.poses[:n_ref].astype(np.float64)
+    ref = np.zeros((n_ref, 1, n, n // 2 + 1), np.complex128)
+    O.extract_central_slices_3d("hc2hc", vol64, None, (1, n, n, n), ref, None, (n_ref, 1, n, n), None, rot64,
+                                n_threads=cpu_cores(), force_threads=True)
+    storages = {
+        "fp32": ext.volume,
+        "fp16": ext.volume.to(torch.complex32),
+        "bf16": nx.ComplexBF16.from_complex(ext.volume),
+    }
+    out = torch.empty((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=ext.volume.device)
+    report = {"report": "extraction volume-storage precision", "config": "256^3 hc volume CN(0,1)*exp(-20|f|^2), "
+              f"{b} uniform poses (error on the first {n_ref}), LINEAR, cutoff 0.5, FP32 math",
+              "oracle": "FP64 restatement of noa's CPU extraction (double volume and rotations)"}
+    for name, v in storages.items():
+        def run():
+            nx.extract_central_slices_3d("hc2hc", v, None, ext.volume_shape, out, None, ext.slice_shape, None,
+                                         ext.rotations)
+        run()
+        torch.cuda.synchronize()
+        got = out[:n_ref].cpu().numpy().astype(np.complex128)
+        err = got - ref
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(5):
+            run()
+        t1.record()
+        torch.cuda.synchronize()
+        report[name] = {
+            "rel_l2": float(np.linalg.norm(err) / np.linalg.norm(ref)),
+            "max_abs": float(np.abs(err).max()),
+            "max_abs_over_max_ref": float(np.abs(err).max() / np.abs(ref).max()),
+            "volume_bytes": int(v.pairs.numel() * 2 if name == "bf16" else v.numel() * v.element_size()),
+            "ms": t0.elapsed_time(t1) / 5,
+        }
+        log(f"{name}: rel-L2 {report[name]['rel_l2']:.3g}, max abs {report[name]['max_abs']:.3g}, "
+            f"{report[name]['ms']:.3f} ms")
+    return report
 
 
 def main():
@@ -116,6 +208,11 @@ def main():
     ap.add_argument("--n-insert", type=int, default=10_000)
     ap.add_argument("--n-extract", type=int, default=4096)
     ap.add_argument("--probe", action="store_true", help="one launch of each operator, no timing (counter runs)")
+    ap.add_argument("--serve-qps", type=float, default=200.0, help="request rate of the serving run")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving run")
+    ap.add_argument("--precision", action="store_true",
+                    help="FP32/FP16/BF16 volume storage for the 256^3 extraction: error against the FP64 oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.",
@@ -127,8 +224,14 @@ def main():
     from noa_amd import workloads
 
     dev = torch.device("cuda:0")
+    if args.serve_only:
+        print(json.dumps(serving_report(torch, dev, args.serve_requests, args.serve_qps)), flush=True)
+        return
     log("generating inputs ...")
     ext = workloads.extract_256(dev, b=args.n_extract)
+    if args.precision:
+        print(json.dumps(precision_report(ext, nx, torch)), flush=True)
+        return
     ins = workloads.insert_256(dev, b=args.n_insert)
     n = 256
     out = torch.empty((args.n_extract, 1, n, n // 2 + 1), dtype=torch.complex64, device=dev)
@@ -136,9 +239,12 @@ def main():
     wvol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
 
+    shifts = torch.from_numpy(ext.shifts).to(dev)
+
     def extract():
-        nx.extract_central_slices_3d("hc2hc", ext.volume, None, ext.volume_shape, out, None, ext.slice_shape, None,
-                                     ext.rotations)
+        # extract_central_slices_3d then phase_shift_2d of the slices by their in-plane shifts, fused.
+        nx.extract_central_slices_3d_shifted("hc2hc", ext.volume, None, ext.volume_shape, out, None, ext.slice_shape,
+                                             None, ext.rotations, shifts=shifts)
 
     def insert():
         nx.insert_central_slices_3d("h2h", ins.slices, ins.weights, ins.slice_shape, vol, wvol, ins.volume_shape,
@@ -208,8 +314,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"256^3 extract {args.n_extract} slices (hc, LINEAR) + insert {args.n_insert} CTF-weighted "
-                        "slices with CTF^2 weights (h2h, LINEAR, windowed sinc 1 voxel)",
+            "workload": f"256^3 extract {args.n_extract} slices (hc, LINEAR) with their in-plane shifts applied by a "
+                        f"fused phase_shift_2d + insert {args.n_insert} CTF-weighted slices with CTF^2 weights (h2h, "
+                        "LINEAR, windowed sinc 1 voxel)",
             "volume": [256, 256, 256],
             "slices_per_step": slices_per_step,
             "combined_frac_hbm_peak": total_bytes / total_time / 1e9 / HBM_PEAK_GBS,
@@ -228,6 +335,13 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(ext, ins)
+    if args.serve_requests > 0:
+        log("serving ...")
+        del ext, ins, out
+        torch.cuda.empty_cache()
+        result["serving"] = serving_report(torch, dev, args.serve_requests, args.serve_qps)
+        log(f"serving: p50 {result['serving']['p50_ms']:.2f} ms, p99 {result['serving']['p99_ms']:.2f} ms, "
+            f"{result['serving']['slices_per_s']:.0f} slices/s")
     print(json.dumps(result), flush=True)
 
 

@@ -140,6 +140,18 @@ typedef struct noa_hip_project_args {
     double ews_radius[2];          /* HW Ewald-sphere radius in 1/pixels; {0,0} = projections */
 } noa_hip_project_args;
 
+/* Per-batch 2D shifts of noa::signal::phase_shift_2d (src/noa/signal/PhaseShift.hpp:215-289): (y, x) in pixels,
+ * Vec<f32, 2> or Vec<f64, 2> (the coordinate precision). One shift broadcast over the batch (batched == 0; `data`
+ * is a HOST pointer to one pair, copied at the call) or one per batch (batched == 1; `data` is a DEVICE pointer to
+ * `count` contiguous pairs). Frequencies with |f| > fftfreq_cutoff are not shifted (noa's default cutoff: 1). */
+typedef struct noa_hip_shift2 {
+    const void* data;
+    int32_t dtype;            /* NOA_HIP_F32 or NOA_HIP_F64 */
+    int32_t batched;
+    int64_t count;
+    double fftfreq_cutoff;
+} noa_hip_shift2;
+
 /* Fill `args` with noa's defaults (cutoff 0.5, sinc/blackman -1, LINEAR, H2H, no scaling). */
 void noa_hip_project_args_init(noa_hip_project_args* args);
 
@@ -157,6 +169,22 @@ int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* str
 
 /* Replaces the GPU branch of noa::xform::extract_central_slices_3d (FourierProject.hpp:1869-1914). */
 int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream);
+
+/* Replaces the GPU branch of noa::signal::phase_shift_2d<REMAP> (src/noa/signal/PhaseShift.hpp:215-289, operator
+ * :78-139) for the rfft layouts: output[b, remap(y, x)] = input[b, y, x] * exp(-2 pi i shift_b . f(y, x)), f the
+ * normalised frequency. `input` may be an Empty view (data == NULL): the phase shifts themselves are written.
+ * `shape` is the logical BDHW shape with D == 1; input and output are c32 or c64 (B, 1, H, W/2+1) arrays, and may be
+ * the same array when the layout does not change. A single zero shift copies (or remaps), a single shift of half
+ * the shape uses the exact +-1 pattern (:250-267), as in noa. */
+int noa_hip_phase_shift_2d(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                           int32_t remap, const noa_hip_shift2* shifts, void* stream);
+
+/* Extraction followed by noa::signal::phase_shift_2d of the extracted slices in their own layout (hc2hc when the
+ * slices are centred, h2h otherwise), fused: the phase shift is applied before the slices are stored, so the
+ * slices are written once. Same arguments and rules as noa_hip_extract_central_slices_3d, plus the shifts (one
+ * per slice or one broadcast); extension of this library, no single noa function does both. */
+int noa_hip_extract_central_slices_3d_shifted(const noa_hip_project_args* args, const noa_hip_shift2* shifts,
+                                              void* stream);
 
 #ifdef __cplusplus
 }

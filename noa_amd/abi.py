@@ -91,6 +91,16 @@ def default_args() -> ProjectArgs:
     return a
 
 
+class Shift2(ctypes.Structure):
+    _fields_ = [
+        ("data", c_void_p),
+        ("dtype", c_int32),
+        ("batched", c_int32),
+        ("count", c_int64),
+        ("fftfreq_cutoff", c_double),
+    ]
+
+
 def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     """Set the argument/return types of the exported functions."""
     P = ctypes.POINTER(ProjectArgs)
@@ -105,4 +115,10 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_abi_version.restype = ctypes.c_int
     lib.noa_hip_project_args_init.argtypes = [P]
     lib.noa_hip_project_args_init.restype = None
+    S = ctypes.POINTER(Shift2)
+    lib.noa_hip_extract_central_slices_3d_shifted.argtypes = [P, S, c_void_p]
+    lib.noa_hip_extract_central_slices_3d_shifted.restype = ctypes.c_int
+    lib.noa_hip_phase_shift_2d.argtypes = [ctypes.POINTER(View), ctypes.POINTER(View), ctypes.POINTER(c_int64),
+                                           c_int32, S, c_void_p]
+    lib.noa_hip_phase_shift_2d.restype = ctypes.c_int
     return lib

@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include "../../include/noa_hip.h"
@@ -334,7 +336,113 @@ ProjectDesc validate(Kind kind, const noa_hip_project_args& a, int& device) {
     return d;
 }
 
-int call(Kind kind, const noa_hip_project_args* args, void* stream) {
+// Shifts of phase_shift_2d (PhaseShift.hpp:190-208 checks, :250-267 single-shift special cases).
+bool allclose_f64(double x, double y) {   // noa::allclose with noa's defaults (base/Math.hpp:296-305)
+    const double diff = std::abs(x - y);
+    if (!std::isfinite(diff)) return false;
+    return diff <= 1e-6 || diff <= std::max(std::abs(x), std::abs(y)) * 2.220446049250313e-16 * 2;
+}
+bool allclose_f32(float x, float y) {
+    const float diff = std::abs(x - y);
+    if (!std::isfinite(diff)) return false;
+    return diff <= 1e-6f || diff <= std::max(std::abs(x), std::abs(y)) * 1.1920929e-07f * 2;
+}
+
+ShiftDesc resolve_shift(const noa_hip_shift2* sh, int64_t batch, int64_t h, int64_t w) {
+    ShiftDesc d;
+    NOA_CHECK(sh != nullptr, INVALID, "shifts: null");
+    NOA_CHECK(sh->dtype == NOA_HIP_F32 || sh->dtype == NOA_HIP_F64, INVALID,
+              "shifts should be Vec<f32,2> or Vec<f64,2>, got dtype=%d", sh->dtype);
+    NOA_CHECK(sh->data != nullptr, INVALID, "shifts: Empty array detected");
+    d.present = true;
+    d.f64 = sh->dtype == NOA_HIP_F64;
+    d.batched = sh->batched != 0;
+    d.cutoff = sh->fftfreq_cutoff;
+    d.data = sh->data;
+    if (d.batched) {
+        NOA_CHECK(sh->count == batch, INVALID,
+                  "The input shift(s) should be entered as a 1d contiguous vector, with one shift per output batch, "
+                  "but got %lld shifts and %lld batches", (long long)sh->count, (long long)batch);
+        NOA_CHECK(reinterpret_cast<uintptr_t>(sh->data) % (d.f64 ? 16 : 8) == 0, INVALID,
+                  "shifts should be aligned to their pair size");
+        return d;
+    }
+    const bool zero = d.f64 ? (allclose_f64(static_cast<const double*>(sh->data)[0], 0) &&
+                               allclose_f64(static_cast<const double*>(sh->data)[1], 0))
+                            : (allclose_f32(static_cast<const float*>(sh->data)[0], 0) &&
+                               allclose_f32(static_cast<const float*>(sh->data)[1], 0));
+    for (int i = 0; i < 2; ++i)
+        d.single[i] = d.f64 ? static_cast<const double*>(sh->data)[i] : static_cast<const float*>(sh->data)[i];
+    const bool half = d.f64 ? (allclose_f64(std::abs(d.single[0]), static_cast<double>(h) / 2) &&
+                               allclose_f64(std::abs(d.single[1]), static_cast<double>(w) / 2))
+                            : (allclose_f32(std::abs(static_cast<float>(d.single[0])), static_cast<float>(h) / 2) &&
+                               allclose_f32(std::abs(static_cast<float>(d.single[1])), static_cast<float>(w) / 2));
+    d.mode = zero ? 1 : (half && sh->fftfreq_cutoff >= std::sqrt(0.5) ? 2 : 0);
+    d.data = nullptr;
+    return d;
+}
+
+PhaseShiftDesc validate_phase_shift(const noa_hip_view* in, const noa_hip_view* out, const int64_t* shape,
+                                    int32_t remap, const noa_hip_shift2* shifts, int& device) {
+    NOA_CHECK(out != nullptr && shape != nullptr, INVALID, "null arguments");
+    NOA_CHECK(out->data != nullptr && !out->is_value, INVALID, "Empty array detected");
+    NOA_CHECK(out->dtype == NOA_HIP_C32 || out->dtype == NOA_HIP_C64, INVALID,
+              "phase_shift_2d: the output should be c32 or c64, got dtype=%d", out->dtype);
+    NOA_CHECK(remap == NOA_HIP_H2H || remap == NOA_HIP_HC2H || remap == NOA_HIP_H2HC || remap == NOA_HIP_HC2HC, INVALID,
+              "phase_shift_2d: the remap should be H2H, HC2H, H2HC or HC2HC (rfft layouts), but got %d", remap);
+    for (int i = 0; i < 4; ++i) NOA_CHECK(shape[i] > 0, INVALID, "Empty array detected");
+    NOA_CHECK(shape[1] == 1, UNSUPPORTED, "phase_shift_2d: only 2d arrays (depth 1) are supported, got shape=%s",
+              shape_str(shape).c_str());
+    int64_t expected[4];
+    rfft_shape(shape, expected);
+    NOA_CHECK(same_shape(out->shape, expected), INVALID,
+              "Given the logical shape %s and FFT layout %d, the expected physical shape should be %s, but got %s",
+              shape_str(shape).c_str(), remap, shape_str(expected).c_str(), shape_str(out->shape).c_str());
+    const bool has_input = in != nullptr && in->data != nullptr;
+    const bool layout_change = ((remap & 4) != 0) != ((remap & 8) != 0);
+    if (has_input) {
+        NOA_CHECK(!in->is_value, INVALID, "phase_shift_2d: the input should be an array");
+        NOA_CHECK(in->dtype == out->dtype, UNSUPPORTED, "phase_shift_2d: input and output should have the same type");
+        NOA_CHECK(same_shape(in->shape, expected), INVALID, "phase_shift_2d: the input should have the shape %s, got %s",
+                  shape_str(expected).c_str(), shape_str(in->shape).c_str());
+        NOA_CHECK(!layout_change || in->data != out->data, INVALID, "In-place remapping is not allowed");
+        NOA_CHECK(in->data == out->data || !overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
+    }
+    auto i32_ok = [](const noa_hip_view& v) { return fits_i32(v, 2); };
+    NOA_CHECK(i32_ok(*out) && (!has_input || i32_ok(*in)), UNSUPPORTED, "offsets within a slice exceed 32-bit indexing");
+    PhaseShiftDesc d;
+    d.shift = resolve_shift(shifts, shape[0], shape[2], shape[3]);
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    if (has_input)
+        NOA_CHECK(device_of(in->data, nullptr) == device, INVALID,
+                  "The input and output arrays must be on the same device");
+    if (d.shift.batched)
+        NOA_CHECK(device_of(d.shift.data, nullptr) == device, INVALID,
+                  "The shift and output arrays must be on the same device");
+    if (has_input) d.input = to_arr(*in, false);
+    d.output = to_arr(*out, false);
+    d.b = shape[0];
+    d.h = shape[2];
+    d.w = shape[3];
+    d.src_centered = (remap & 4) != 0;
+    d.dst_centered = (remap & 8) != 0;
+    d.c64 = out->dtype == NOA_HIP_C64;
+    return d;
+}
+
+int guarded(int device, const std::function<int(std::string&)>& launch) {
+    int current = 0;
+    (void)hipGetDevice(&current);
+    if (current != device) (void)hipSetDevice(device);
+    std::string err;
+    const int status = launch(err);
+    if (current != device) (void)hipSetDevice(current);
+    if (status != NOA_HIP_OK) g_last_error = err;
+    return status;
+}
+
+int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr) {
     g_last_error.clear();
     if (!args) {
         g_last_error = "null arguments";
@@ -343,7 +451,20 @@ int call(Kind kind, const noa_hip_project_args* args, void* stream) {
     int device = -1;
     ProjectDesc d;
     try {
-        d = validate(kind, *args, device);
+        if (shifts) {
+            // Host-only checks of the shifts first, then the extraction's own checks (device checks last).
+            NOA_CHECK(args->output.dtype == NOA_HIP_C32 || args->output.dtype == NOA_HIP_C64, INVALID,
+                      "phase shift: the extracted slices should be complex");
+            const ShiftDesc sd = resolve_shift(shifts, args->output_shape[0], args->output_shape[2],
+                                               args->output_shape[3]);
+            d = validate(kind, *args, device);
+            if (sd.batched)
+                NOA_CHECK(device_of(sd.data, nullptr) == device, INVALID,
+                          "The shift and output arrays must be on the same device");
+            d.shift = sd;
+        } else {
+            d = validate(kind, *args, device);
+        }
     } catch (const Fail& f) {
         g_last_error = f.msg;
         return f.code;
@@ -391,6 +512,28 @@ int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* str
 }
 int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream) {
     return noa_hip::call(noa_hip::Kind::EXTRACT, args, stream);
+}
+int noa_hip_extract_central_slices_3d_shifted(const noa_hip_project_args* args, const noa_hip_shift2* shifts,
+                                              void* stream) {
+    if (!shifts) {
+        noa_hip::g_last_error = "shifts: null";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    return noa_hip::call(noa_hip::Kind::EXTRACT, args, stream, shifts);
+}
+int noa_hip_phase_shift_2d(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                           int32_t remap, const noa_hip_shift2* shifts, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    PhaseShiftDesc d;
+    try {
+        d = validate_phase_shift(input, output, shape, remap, shifts, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) { return launch_phase_shift(d, static_cast<hipStream_t>(stream), err); });
 }
 
 } // extern "C"

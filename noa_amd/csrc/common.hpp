@@ -308,6 +308,93 @@ __device__ __forceinline__ Math interp3(const Storage* __restrict__ vol, i32 sz,
     }
 }
 
+// The same 3D interpolation split in two phases, so that a lane can have the loads of several samples in flight:
+// fetch3 resolves the flip, the fractions and the tap offsets and issues every load (an out-of-bounds tap reads
+// element 0 and is zeroed), combine3 sums the taps in noa's order (x innermost, then y, then z) and conjugates.
+// `valid == false` fetches nothing useful (all taps zero). The result equals interp3's bit for bit.
+template<class Math, class C>
+struct Taps3 {
+    Math v[2][2][2];      // raw loads; zeroed in combine3 where `ok` is clear (the loads stay in flight until then)
+    u32 ok;               // bit (z*4 + y*2 + x): tap inside the bounds
+    typename vt<Math>::real tz, ty, tx, conj;
+};
+
+template<int INTERP, class Math, class Storage, class C>
+__device__ __forceinline__ Taps3<Math, C> fetch3(const Storage* __restrict__ vol, i32 sz, i32 sy, i32 sx,
+                                                 const Bounds3& bd, bool centered, C fz, C fy, C fx, bool valid) {
+    using R = typename vt<Math>::real;
+    Taps3<Math, C> t;
+    t.conj = 1;
+    if (fx < 0) { fz *= -1; fy *= -1; fx *= -1; t.conj = -1; }
+    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+        const i32 z = static_cast<i32>(round(fz)), y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
+        const bool ok = valid && z >= bd.lo_z && z <= bd.hi_z && y >= bd.lo_y && y <= bd.hi_y && x >= 0 && x <= bd.hi_x;
+        const u32 off = ok ? static_cast<u32>(frequency2index(centered, z, bd.nz) * sz +
+                                              frequency2index(centered, y, bd.ny) * sy + x * sx) : 0u;
+        t.v[0][0][0] = load_as<Math>(vol + off);
+        t.ok = ok ? 1u : 0u;
+        t.tz = t.ty = t.tx = 0;
+        t.conj = 1;   // no conjugation after the flip for NEAREST (Interpolation.hpp:479-484)
+    } else {
+        const C flz = floor(fz), fly = floor(fy), flx = floor(fx);
+        const i32 z0 = static_cast<i32>(flz), y0 = static_cast<i32>(fly), x0 = static_cast<i32>(flx);
+        t.tz = static_cast<R>(fz - flz);
+        t.ty = static_cast<R>(fy - fly);
+        t.tx = static_cast<R>(fx - flx);
+        bool okz[2], oky[2], okx[2];
+        i32 oz[2], oy[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            okz[i] = valid && z0 + i >= bd.lo_z && z0 + i <= bd.hi_z;
+            oky[i] = y0 + i >= bd.lo_y && y0 + i <= bd.hi_y;
+            okx[i] = x0 + i >= 0 && x0 + i <= bd.hi_x;
+            oz[i] = frequency2index(centered, z0 + i, bd.nz) * sz;
+            oy[i] = frequency2index(centered, y0 + i, bd.ny) * sy;
+        }
+        t.ok = 0;
+#pragma unroll
+        for (int z = 0; z < 2; ++z)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    const bool ok = okz[z] && oky[y] && okx[x];
+                    const u32 off = ok ? static_cast<u32>(oz[z] + oy[y] + (x0 + x) * sx) : 0u;
+                    t.v[z][y][x] = load_as<Math>(vol + off);
+                    t.ok |= ok ? (1u << (z * 4 + y * 2 + x)) : 0u;
+                }
+    }
+    return t;
+}
+
+template<int INTERP, class Math, class C>
+__device__ __forceinline__ Math combine3(const Taps3<Math, C>& t) {
+    using R = typename vt<Math>::real;
+    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+        return vadd(vzero<Math>(), vmul(t.ok ? t.v[0][0][0] : vzero<Math>(), R(1)));
+    } else {
+        const R wz[2] = {1 - t.tz, t.tz}, wy[2] = {1 - t.ty, t.ty}, wx[2] = {1 - t.tx, t.tx};
+        Math value = vzero<Math>();
+#pragma unroll
+        for (int z = 0; z < 2; ++z) {
+            Math vz = vzero<Math>();
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                Math vy = vzero<Math>();
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    const Math v = (t.ok >> (z * 4 + y * 2 + x)) & 1u ? t.v[z][y][x] : vzero<Math>();
+                    vy = vadd(vy, vmul(v, wx[x]));
+                }
+                vz = vadd(vz, vmul(vy, wy[y]));
+            }
+            value = vadd(value, vmul(vz, wz[z]));
+        }
+        vimag_mul(value, t.conj);
+        return value;
+    }
+}
+
 // 2D: slice `b` of a stack with strides (sb, sy, sx); returns the interpolated value.
 // A constant input (noa's AccessorValue) is read through the same code with `is_value` set.
 template<int INTERP, class Math, class C>
@@ -343,6 +430,43 @@ __device__ __forceinline__ Math interp2(const Math* __restrict__ base, i32 sy, i
         vimag_mul(value, conjugate);
         return value;
     }
+}
+
+// Phase shift of one rfft element (signal/PhaseShift.hpp:113-131 with nf::phase_shift, fft/core/Frequency.hpp:
+// 228-248): fftfreq = frequency * (1 / shape) in the shift precision R, no shift beyond the cutoff, otherwise
+// exp(-2 pi i (shift . fftfreq)) evaluated in the value precision T. mode 2: a shift of half the shape, the exact
+// +-1 pattern of PhaseShiftHalf (:18-69); mode 1: no shift.
+template<class V, class R>
+__device__ __forceinline__ V phase_shift_at(int mode, R sy, R sx, i32 fy, i32 fx, R norm_y, R norm_x, R cutoff_sqd) {
+    using T = typename vt<V>::real;
+    V phase;
+    phase.x = T(1);
+    phase.y = T(0);
+    if (mode == 2) {
+        const i32 ay = fy % 2, ax = fx % 2;
+        phase.x = static_cast<T>((1 - 2 * (ay < 0 ? -ay : ay)) * (1 - 2 * (ax < 0 ? -ax : ax)));
+    } else if (mode == 0) {
+        const R fz = R(0) * R(1);
+        const R fyn = static_cast<R>(fy) * norm_y, fxn = static_cast<R>(fx) * norm_x;
+        if ((fz * fz + fyn * fyn) + fxn * fxn <= cutoff_sqd) {
+            const R two_pi = R(2) * static_cast<R>(3.141592653589793238462643383279502884);
+            const R factor = -((two_pi * sy) * fyn + (two_pi * sx) * fxn);
+            T sn, cs;
+            if constexpr (sizeof(T) == 4) sincosf(static_cast<T>(factor), &sn, &cs);
+            else ::sincos(static_cast<T>(factor), &sn, &cs);
+            phase.x = cs;
+            phase.y = sn;
+        }
+    }
+    return phase;
+}
+
+// noa's Complex multiplication order (base/Complex.hpp:195-201).
+template<class V> __device__ __forceinline__ V cmul(V a, V b) {
+    V r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
 }
 
 // Device-wide float atomics. Outputs are coarse-grained hipMalloc memory: the hardware float atomic is exact.

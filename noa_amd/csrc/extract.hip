@@ -37,7 +37,35 @@ struct ExtractParams {
     i32 window;            // blackman window size along w (1 = plain 3D extraction)
     XformArg<C> rot, scale;
     bool general, vol_centered, slice_centered, has_wvol, has_wout;
+    // Fused phase shift of the output slices (phase_shift_2d in the slices' own layout); shift_mode < 0: none.
+    int shift_mode;
+    bool shift_f64, shift_batched;
+    const void* shifts;
+    float shift_f[2], norm_f[2], cutoff_sqd_f;
+    double shift_d[2], norm_d[2], cutoff_sqd_d;
 };
+
+// The phase shift of slice b at pixel (fy, u) applied to an extracted complex value (PhaseShift.hpp:113-131).
+template<class C, class Out>
+__device__ __forceinline__ Out shift_value(const ExtractParams<C>& p, i32 b, i32 fy, i32 u, Out value) {
+    if constexpr (vt<Out>::cx) {
+        if (p.shift_mode < 0 || p.shift_mode == 1) return value;
+        Out phase;
+        if (p.shift_f64) {
+            const double* s = static_cast<const double*>(p.shifts);
+            const double sy = p.shift_batched ? s[2 * b] : p.shift_d[0], sx = p.shift_batched ? s[2 * b + 1] : p.shift_d[1];
+            phase = phase_shift_at<Out, double>(p.shift_mode, sy, sx, fy, u, p.norm_d[0], p.norm_d[1], p.cutoff_sqd_d);
+        } else {
+            const float* s = static_cast<const float*>(p.shifts);
+            const float sy = p.shift_batched ? s[2 * b] : p.shift_f[0], sx = p.shift_batched ? s[2 * b + 1] : p.shift_f[1];
+            phase = phase_shift_at<Out, float>(p.shift_mode, sy, sx, fy, u, p.norm_f[0], p.norm_f[1], p.cutoff_sqd_f);
+        }
+        return cmul(value, phase);
+    } else {
+        (void)p; (void)b; (void)fy; (void)u;
+        return value;
+    }
+}
 
 template<class C, class S, class M, class Out, class W, int INTERP>
 __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) {
@@ -57,6 +85,49 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
     W* __restrict__ wout = p.has_wout ? static_cast<W*>(p.wout) + static_cast<i64>(b) * p.ws_b : nullptr;
 
     const i32 first = tile * p.ppt * 256 + static_cast<i32>(threadIdx.x);
+    if (p.window == 1) {
+        // 3D operator (:632-657): overwrite, 0 beyond the cutoff. The tap loads of the lane's pixels (up to
+        // MAXP) are all issued before the first is used.
+        constexpr i32 MAXP = 4;
+        Taps3<M, C> tv[MAXP];
+        Taps3<W, C> tw[MAXP];
+        bool inside[MAXP];
+        i32 ys[MAXP], us[MAXP];
+#pragma unroll
+        for (i32 k = 0; k < MAXP; ++k) {
+            const i32 pix = first + k * 256;
+            const bool live = k < p.ppt && pix < p.pixels;
+            const i32 y = live ? pix / p.slice_wr : 0;
+            const i32 u = live ? pix - y * p.slice_wr : 0;
+            ys[k] = live ? y : -1;
+            us[k] = u;
+            // compute_fftfreq_in_volume_ (:699-705)
+            vec2<C> f2;
+            f2.y = static_cast<C>(index2frequency(p.slice_centered, y, p.slice_h)) / p.slice_f[0];
+            f2.x = static_cast<C>(u) / p.slice_f[1];
+            const vec3<C> f = p.general ? slice2grid<true>(f2, sc, rot, p.ews[0], p.ews[1])
+                                        : slice2grid<false>(f2, sc, rot, p.ews[0], p.ews[1]);
+            inside[k] = live && dot3(f) <= p.cutoff_sqd;
+            const C fz = f.z * p.target[0], fy = f.y * p.target[1], fx = f.x * p.target[2];
+            tv[k] = fetch3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx, inside[k]);
+            if (p.has_wout && p.has_wvol)
+                tw[k] = fetch3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx, inside[k]);
+        }
+#pragma unroll
+        for (i32 k = 0; k < MAXP; ++k) {
+            if (ys[k] < 0) continue;
+            Out value = inside[k] ? vcast<Out>(combine3<INTERP>(tv[k])) : vzero<Out>();
+            if (p.shift_mode >= 0)
+                value = shift_value(p, b, index2frequency(p.slice_centered, ys[k], p.slice_h), us[k], value);
+            out[ys[k] * p.os_y + us[k] * p.os_x] = value;
+            if (p.has_wout) {
+                W weight{};
+                if (inside[k]) weight = p.has_wvol ? combine3<INTERP>(tw[k]) : W(1);
+                wout[ys[k] * p.ws_y + us[k] * p.ws_x] = weight;
+            }
+        }
+        return;
+    }
     for (i32 k = 0; k < p.ppt; ++k) {
         const i32 pix = first + k * 256;
         if (pix >= p.pixels) break;
@@ -70,36 +141,26 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
         const vec3<C> f = p.general ? slice2grid<true>(f2, sc, rot, p.ews[0], p.ews[1])
                                     : slice2grid<false>(f2, sc, rot, p.ews[0], p.ews[1]);
 
+        // 4D operator (:661-694), summed over w in increasing order.
+        using RO = typename vt<Out>::real;
         Out value = vzero<Out>();
         W weight{};
-        if (p.window == 1) {
-            // 3D operator (:632-657): overwrite, 0 beyond the cutoff.
-            if (dot3(f) <= p.cutoff_sqd) {
-                const C fz = f.z * p.target[0], fy = f.y * p.target[1], fx = f.x * p.target[2];
-                value = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx));
-                if (p.has_wout)
-                    weight = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx)
+        for (i32 w = 0; w < p.window; ++w) {
+            const C offset = static_cast<C>(w - p.window / 2) / p.target[0];   // w_index_to_fftfreq_offset
+            vec3<C> fw = f;
+            fw.z += offset;
+            if (dot3(fw) > p.cutoff_sqd) continue;
+            const C fz = fw.z * p.target[0], fy = fw.y * p.target[1], fx = fw.x * p.target[2];
+            const C conv = windowed_sinc<C>(offset, p.fftfreq_sinc, p.fftfreq_blackman);
+            const Out v = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx));
+            value = vadd(value, vmul(v, static_cast<RO>(conv)));
+            if (p.has_wout) {
+                const W wv = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx)
                                         : W(1);
-            }
-        } else {
-            // 4D operator (:661-694), summed over w in increasing order.
-            using RO = typename vt<Out>::real;
-            for (i32 w = 0; w < p.window; ++w) {
-                const C offset = static_cast<C>(w - p.window / 2) / p.target[0];   // w_index_to_fftfreq_offset
-                vec3<C> fw = f;
-                fw.z += offset;
-                if (dot3(fw) > p.cutoff_sqd) continue;
-                const C fz = fw.z * p.target[0], fy = fw.y * p.target[1], fx = fw.x * p.target[2];
-                const C conv = windowed_sinc<C>(offset, p.fftfreq_sinc, p.fftfreq_blackman);
-                const Out v = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx));
-                value = vadd(value, vmul(v, static_cast<RO>(conv)));
-                if (p.has_wout) {
-                    const W wv = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx)
-                                            : W(1);
-                    weight = weight + wv * static_cast<W>(conv);
-                }
+                weight = weight + wv * static_cast<W>(conv);
             }
         }
+        if (p.shift_mode >= 0) value = shift_value(p, b, index2frequency(p.slice_centered, y, p.slice_h), u, value);
         out[y * p.os_y + u * p.os_x] = value;
         if (p.has_wout) wout[y * p.ws_y + u * p.ws_x] = weight;
     }
@@ -144,6 +205,21 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     p.slice_h = static_cast<i32>(d.out_shape[2]);
     p.slice_wr = static_cast<i32>(d.out_shape[3] / 2 + 1);
     p.pixels = p.slice_h * p.slice_wr;
+    p.shift_mode = d.shift.present ? d.shift.mode : -1;
+    p.shift_f64 = d.shift.f64;
+    p.shift_batched = d.shift.batched;
+    p.shifts = d.shift.data;
+    for (int i = 0; i < 2; ++i) {
+        p.shift_f[i] = static_cast<float>(d.shift.single[i]);
+        p.shift_d[i] = d.shift.single[i];
+    }
+    // m_norm = 1 / shape in the shift precision (PhaseShift.hpp:108).
+    p.norm_f[0] = 1.f / static_cast<float>(d.out_shape[2]);
+    p.norm_f[1] = 1.f / static_cast<float>(d.out_shape[3]);
+    p.norm_d[0] = 1.0 / static_cast<double>(d.out_shape[2]);
+    p.norm_d[1] = 1.0 / static_cast<double>(d.out_shape[3]);
+    p.cutoff_sqd_f = static_cast<float>(d.shift.cutoff) * static_cast<float>(d.shift.cutoff);
+    p.cutoff_sqd_d = d.shift.cutoff * d.shift.cutoff;
     C cutoff = std::max(static_cast<C>(d.cutoff), C(0));
     p.cutoff_sqd = cutoff * cutoff;
     p.general = d.general;

@@ -217,7 +217,11 @@ __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C,
     const i32 tid = static_cast<i32>(threadIdx.x);
     const i32 lane = tid & 63;
     const i32 wave = tid >> 6;
-    const int4 item = p.items[blockIdx.x];
+    // The plan is built for a power-of-two slice count: clamp the item's slice range to the actual count (an
+    // empty range still flushes, so a split brick's partial slot is always written).
+    int4 item = p.items[blockIdx.x];
+    item.z = min(item.z, p.n_slices);
+    item.y = min(item.y, item.z);
     const i32 brick = item.x;
     const i32 bxi = brick % p.nb[2];
     const i32 rem = brick / p.nb[2];
@@ -615,8 +619,9 @@ constexpr i32 WG = 2;                      // candidates per lane and pass
 // clamped address and zeroed when outside the frequency bounds. All loads are issued before any is used.
 template<class In, class W, class C>
 struct Taps {
-    In v[2][2];
+    In v[2][2];           // raw loads; zeroed in combine_taps where `ok` is clear (loads stay in flight until then)
     W w[2][2];
+    u32 ok;               // bit (y*2 + x): tap inside the bounds
     typename vt<In>::real ty, tx, conj;
 };
 
@@ -633,10 +638,9 @@ __device__ __forceinline__ Taps<In, W, C> fetch_taps(const In* __restrict__ vb, 
         const i32 y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
         const bool ok = valid && y >= lo_y && y <= hi_y && x >= 0 && x <= hi_x;
         const i32 iy = ok ? frequency2index(centered, y, ny) : 0, ix = ok ? x : 0;
-        const In a = vb[iy * vsy + ix * vsx];
-        const W b = has_w ? wb[iy * wsy + ix * wsx] : W(0);
-        t.v[0][0] = ok ? a : vzero<In>();
-        t.w[0][0] = ok ? b : W(0);
+        t.v[0][0] = vb[iy * vsy + ix * vsx];
+        t.w[0][0] = has_w ? wb[iy * wsy + ix * wsx] : W(0);
+        t.ok = ok ? 1u : 0u;
         t.ty = 0;
         t.tx = 0;
     } else {
@@ -658,21 +662,14 @@ __device__ __forceinline__ Taps<In, W, C> fetch_taps(const In* __restrict__ vb, 
                 ow[y][x] = ok[y][x] ? iy * wsy + ix * wsx : 0;
             }
         }
-        In a[2][2];
-        W b[2][2];
+        t.ok = 0;
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
-                a[y][x] = vb[ov[y][x]];
-                b[y][x] = has_w ? wb[ow[y][x]] : W(0);
-            }
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                t.v[y][x] = ok[y][x] ? a[y][x] : vzero<In>();
-                t.w[y][x] = ok[y][x] ? b[y][x] : W(0);
+                t.v[y][x] = vb[ov[y][x]];
+                t.w[y][x] = has_w ? wb[ow[y][x]] : W(0);
+                t.ok |= ok[y][x] ? (1u << (y * 2 + x)) : 0u;
             }
     }
     return t;
@@ -684,8 +681,8 @@ template<int INTERP, class In, class W, class C>
 __device__ __forceinline__ void combine_taps(const Taps<In, W, C>& t, bool has_w, In& value, W& weight) {
     using R = typename vt<In>::real;
     if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
-        value = vadd(vzero<In>(), vmul(t.v[0][0], R(1)));
-        weight = W(0) + t.w[0][0] * W(1);
+        value = vadd(vzero<In>(), vmul(t.ok ? t.v[0][0] : vzero<In>(), R(1)));
+        weight = W(0) + (t.ok ? t.w[0][0] : W(0)) * W(1);
     } else {
         const R wy[2] = {1 - t.ty, t.ty}, wx[2] = {1 - t.tx, t.tx};
         value = vzero<In>();
@@ -696,8 +693,9 @@ __device__ __forceinline__ void combine_taps(const Taps<In, W, C>& t, bool has_w
             W wyv = W(0);
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
-                vy = vadd(vy, vmul(t.v[y][x], wx[x]));
-                wyv = wyv + t.w[y][x] * static_cast<W>(wx[x]);
+                const bool ok = (t.ok >> (y * 2 + x)) & 1u;
+                vy = vadd(vy, vmul(ok ? t.v[y][x] : vzero<In>(), wx[x]));
+                wyv = wyv + (ok ? t.w[y][x] : W(0)) * static_cast<W>(wx[x]);
             }
             value = vadd(value, vmul(vy, wy[y]));
             weight = weight + wyv * static_cast<W>(wy[y]);
@@ -715,7 +713,11 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
     __shared__ i32 cand[64 * WG];          // packed candidates of one pass: column << 4 | k
 
     const i32 lane = static_cast<i32>(threadIdx.x);
-    const int4 item = p.items[blockIdx.x];
+    // The plan is built for a power-of-two slice count: clamp the item's slice range to the actual count (an
+    // empty range still flushes, so a split brick's partial slot is always written).
+    int4 item = p.items[blockIdx.x];
+    item.z = min(item.z, p.n_slices);
+    item.y = min(item.y, item.z);
     const i32 brick = item.x;
     const i32 bxi = brick % p.nb[2];
     const i32 rem = brick / p.nb[2];
@@ -1340,8 +1342,12 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             int device = 0;
             (void)hipGetDevice(&device);
             Plan plan;
+            // Plans are kept per power-of-two slice count (items are clamped to the actual count in the kernel), so
+            // that callers with varying batch sizes reuse a handful of plans.
+            i32 plan_slices = 64;
+            while (plan_slices < p.n_slices) plan_slices *= 2;
             const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
-                                     p.n_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target, plan,
+                                     plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target, plan,
                                      err);
             if (st != NOA_HIP_OK) return st;
             if (plan.n_items == 0) return NOA_HIP_OK;

@@ -30,6 +30,17 @@ struct Xform {
     double single[9]{};
 };
 
+// Resolved per-batch 2D shifts of a phase shift (single shifts are copied from the host).
+struct ShiftDesc {
+    bool present{};
+    const void* data{};
+    bool f64{};
+    bool batched{};
+    double single[2]{};
+    double cutoff{1};
+    int mode{};           // 0: general formula, 1: no shift (copy/remap), 2: half-shape shift (+-1 pattern)
+};
+
 // Everything the kernels of one call need; coordinates are converted to the rotation precision on launch.
 struct ProjectDesc {
     Arr input, input_weight, output, output_weight;
@@ -42,9 +53,20 @@ struct ProjectDesc {
     double ews_radius[2]{};
     double cutoff{}, fftfreq_sinc{}, fftfreq_blackman{};
     bool coord_f64{};
+    ShiftDesc shift;                     // extraction only: phase shift of the output slices (fused)
+};
+
+// A phase shift over (B, 1, H, W/2+1) rfft arrays.
+struct PhaseShiftDesc {
+    Arr input, output;    // (batch, y, x) strides; input.data == nullptr: write the phase shifts
+    i64 b{}, h{}, w{};    // logical batch, height, width
+    bool src_centered{}, dst_centered{};
+    bool c64{};
+    ShiftDesc shift;
 };
 
 // Launchers: return 0 or a NOA_HIP_ERROR_* code, with a message in `err`.
+int launch_phase_shift(const PhaseShiftDesc& d, hipStream_t stream, std::string& err);
 int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);

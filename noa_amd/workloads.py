@@ -136,3 +136,57 @@ def correctness_64(device="cuda", n: int = 64, b: int = 64, seed: int = SEED_BAS
     rng = np.random.default_rng(seed)
     poses = uniform_rotations(b, rng)
     return slices, poses
+
+
+SERVING_BOXES = (64, 128, 192, 256, 384)
+SERVING_P = (0.3, 0.3, 0.2, 0.15, 0.05)
+
+
+def serving_volumes(device="cuda", per_box: int = 4, seed: int = SEED_BASE + 5) -> dict:
+    """4 resident complex64 "hc" volumes per box size, CN(0,1) x exp(-20 |f|^2) as the extraction workload."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    vols = {}
+    for n in SERVING_BOXES:
+        fz = fftfreq_grid(n, True, device)[:, None, None]
+        fy = fftfreq_grid(n, True, device)[None, :, None]
+        fx = (torch.arange(n // 2 + 1, device=device, dtype=torch.float32) / n)[None, None, :]
+        env = torch.exp(-20 * (fz * fz + fy * fy + fx * fx))
+        vols[n] = []
+        for _ in range(per_box):
+            shape = (1, n, n, n // 2 + 1)
+            re = torch.randn(shape, generator=g, device=device)
+            im = torch.randn(shape, generator=g, device=device)
+            vols[n].append(torch.complex(re * env, im * env).contiguous())
+    return vols
+
+
+def serving_requests(device="cuda", n_requests: int = 300, qps: float = 200.0, seed: int = SEED_BASE + 5,
+                     max_k: int = 512):
+    """A Poisson request stream at `qps`: box from {64, 128, 192, 256, 384} with p {.3, .3, .2, .15, .05}; half
+    extract K ~ U{1..max_k} slices at uniform poses from one of the 4 resident volumes of that box, half insert K
+    CTF-weighted slices (as the insertion workload, at that box) into a per-request accumulator. Inputs are
+    generated on the device before the stream starts."""
+    from .serve import Request
+    g = torch.Generator(device=device).manual_seed(seed + 1)
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+    reqs = []
+    for i in range(n_requests):
+        box = int(rng.choice(SERVING_BOXES, p=SERVING_P))
+        k = int(rng.integers(1, max_k + 1))
+        poses = uniform_rotations(k, rng)
+        if rng.uniform() < 0.5:
+            reqs.append(Request(rid=i, kind="extract", box=box, volume=int(rng.integers(0, 4)),
+                                rotations=torch.from_numpy(poses).to(device), arrival=float(arrivals[i])))
+        else:
+            img = torch.randn((k, box, box), generator=g, device=device)
+            spec = torch.fft.rfft2(img, norm="forward")
+            ctf = ctf_anisotropic((box, box), rng.uniform(0.5, 3.0, k), rng.uniform(0.0, 0.1, k),
+                                  rng.uniform(0.0, np.pi, k), device=device)
+            inv = np.ascontiguousarray(np.transpose(poses, (0, 2, 1)))
+            reqs.append(Request(rid=i, kind="insert", box=box, rotations=torch.from_numpy(inv).to(device),
+                                slices=(spec * ctf).reshape(k, 1, box, box // 2 + 1).contiguous(),
+                                weights=(ctf * ctf).reshape(k, 1, box, box // 2 + 1).contiguous(),
+                                arrival=float(arrivals[i])))
+            del img, spec, ctf
+    return reqs

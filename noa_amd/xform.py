@@ -509,6 +509,82 @@ def extract_central_slices_3d(remap, volume, volume_weight, volume_shape, slice,
     _call("noa_hip_extract_central_slices_3d", args, device)
 
 
+def _shift2(shifts, count: int, device: torch.device, keep: list, cutoff: float) -> abi.Shift2:
+    """One (y, x) shift (host value) or (B, 2) shifts (device array), float32 or float64."""
+    sh = abi.Shift2()
+    sh.fftfreq_cutoff = float(cutoff)
+    if isinstance(shifts, torch.Tensor) and shifts.ndim == 2:
+        if shifts.device != device:
+            raise NoaError("The shift and output arrays must be on the same device")
+        if shifts.dtype not in (torch.float32, torch.float64):
+            raise NoaError("shifts should be float32 or float64")
+        t = shifts.contiguous()
+        keep.append(t)
+        sh.data = t.data_ptr()
+        sh.dtype = abi.F64 if t.dtype == torch.float64 else abi.F32
+        sh.batched = 1
+        sh.count = t.shape[0]
+        return sh
+    a = np.asarray(shifts.cpu() if isinstance(shifts, torch.Tensor) else shifts)
+    if a.dtype not in (np.float32, np.float64):
+        a = a.astype(np.float64)
+    if a.ndim == 2:
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        return _shift2(t, count, device, keep, cutoff)
+    if a.shape != (2,):
+        raise NoaError(f"shifts: expected one (y, x) shift or (B, 2) shifts, got {a.shape}")
+    buf = np.ascontiguousarray(a)
+    keep.append(buf)
+    sh.data = buf.ctypes.data
+    sh.dtype = abi.F64 if buf.dtype == np.float64 else abi.F32
+    sh.batched = 0
+    sh.count = 1
+    return sh
+
+
+def extract_central_slices_3d_shifted(remap, volume, volume_weight, volume_shape, slice, slice_weight, slice_shape,
+                                      inv_scaling=None, fwd_rotation=None, shifts=None, shift_fftfreq_cutoff=1.0,
+                                      options: ExtractCentralSlicesOptions | None = None) -> None:
+    """extract_central_slices_3d followed by noa::signal::phase_shift_2d of the slices in their own layout, fused
+    into the extraction (the slices are written once). `shifts`: (y, x) in pixels, one or one per slice."""
+    opts = options or ExtractCentralSlicesOptions()
+    slice_shape = _shape4(slice_shape, "slice_shape")
+    volume_shape = _shape4(volume_shape, "volume_shape")
+    keep: list = []
+    args = abi.default_args()
+    interp = _interp_of(volume, volume_weight, opts.interp)
+    device = _fill(args, remap=remap, inp=volume, inp_w=volume_weight, out=slice, out_w=slice_weight,
+                   in_shape=volume_shape, out_shape=slice_shape, rotation=fwd_rotation, scaling=inv_scaling,
+                   count=slice_shape[0], opts=opts, keep=keep, allow_value=False, interp=interp)
+    args.fftfreq_sinc = float(opts.w_windowed_sinc.fftfreq_sinc)
+    args.fftfreq_blackman = float(opts.w_windowed_sinc.fftfreq_blackman)
+    sh = _shift2(shifts, slice_shape[0], device, keep, shift_fftfreq_cutoff)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    status = lib().noa_hip_extract_central_slices_3d_shifted(ctypes.byref(args), ctypes.byref(sh),
+                                                             ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
+
+
+def phase_shift_2d(remap, input, output, shape, shifts, fftfreq_cutoff: float = 1.0) -> None:
+    """noa::signal::phase_shift_2d<REMAP> (signal/PhaseShift.hpp:215-289) for the rfft layouts h2h, h2hc, hc2h,
+    hc2hc: output = input * exp(-2 pi i shift . f). `input` may be None (the phase shifts are written) or `output`
+    itself when the layout does not change; shifts: one (y, x) pair or one per batch."""
+    shape = _shape4(shape, "shape")
+    if not isinstance(output, torch.Tensor) or output.device.type != "cuda":
+        raise NoaError("phase_shift_2d: the output should be a device tensor")
+    keep: list = []
+    out_v = _view(output, keep, abi.F32, abi.C32, False, "output")
+    in_v = _view(input, keep, abi.F32, abi.C32, False, "input") if input is not None else abi.View()
+    sh = _shift2(shifts, shape[0], output.device, keep, fftfreq_cutoff)
+    c_shape = (ctypes.c_int64 * 4)(*shape)
+    stream = torch.cuda.current_stream(output.device).cuda_stream
+    status = lib().noa_hip_phase_shift_2d(ctypes.byref(in_v), ctypes.byref(out_v), c_shape, layout(remap),
+                                          ctypes.byref(sh), ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
+
+
 def rfft_shape(shape: Sequence[int]) -> tuple[int, int, int, int]:
     """Shape4::rfft (src/noa/runtime/core/Shape.hpp:380): (B, D, H, W//2+1)."""
     b, d, h, w = _shape4(shape, "shape")

@@ -39,6 +39,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <limits>
 #include <omp.h>
 
 #include "../include/noa_hip.h"
@@ -893,6 +894,70 @@ int dispatch(Op op, const noa_hip_project_args* a, int n_threads, bool force) {
     return f64 ? dispatch_checked<double>(op, *a, n_threads, force) : dispatch_checked<float>(op, *a, n_threads, force);
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// noa::signal::phase_shift_2d for rfft layouts (src/noa/signal/PhaseShift.hpp:78-139, :215-289), and
+// nf::phase_shift (src/noa/fft/core/Frequency.hpp:228-248). Arrays: contiguous (B, H, W/2+1).
+// ------------------------------------------------------------------------------------------------------------
+template<class T>
+constexpr T pi_v() { return static_cast<T>(3.141592653589793238462643383279502884); }
+
+template<class V, class R>
+int phase_shift_2d(const V* input, V* output, i64 b, i64 h, i64 w, int remap, const R* shifts, bool batched,
+                   double cutoff) {
+    using real_t = typename real_of<V>::type;
+    const bool src_c = (remap & 4) != 0, dst_c = (remap & 8) != 0;
+    const i64 wr = w / 2 + 1;
+    // Single shift special cases (:250-267): zero -> copy/remap (or ones); |shift| == shape/2 and a cutoff that
+    // covers the whole spectrum -> the +-1 pattern of PhaseShiftHalf (:18-69).
+    int mode = 0;   // 0: general, 1: none, 2: half
+    if (!batched) {
+        auto allclose = [](R x, R y) {
+            const R diff = std::abs(x - y);
+            if (!std::isfinite(diff)) return false;
+            return diff <= static_cast<R>(1e-6) || diff <= std::max(std::abs(x), std::abs(y)) * std::numeric_limits<R>::epsilon() * 2;
+        };
+        if (allclose(shifts[0], R(0)) && allclose(shifts[1], R(0))) mode = 1;
+        else if (allclose(std::abs(shifts[0]), static_cast<R>(h) / 2) && allclose(std::abs(shifts[1]), static_cast<R>(w) / 2) &&
+                 cutoff >= std::sqrt(0.5)) mode = 2;
+    }
+    const R norm_y = R(1) / static_cast<R>(h), norm_x = R(1) / static_cast<R>(w);   // m_norm = 1 / shape
+    const R cutoff_sqd = static_cast<R>(cutoff) * static_cast<R>(cutoff);
+    for (i64 bi = 0; bi < b; ++bi) {
+        const R sy = shifts[batched ? 2 * bi : 0], sx = shifts[batched ? 2 * bi + 1 : 1];
+        for (i64 y = 0; y < h; ++y) {
+            const i64 fy = index2frequency(src_c, y, h);
+            const i64 oy = frequency2index(dst_c, fy, h);
+            for (i64 x = 0; x < wr; ++x) {
+                V phase{};
+                if constexpr (is_cx<V>::value) { phase.re = 1; phase.im = 0; }
+                if (mode == 2) {
+                    const real_t s = static_cast<real_t>((1 - 2 * std::abs(fy % 2)) * (1 - 2 * std::abs(x % 2)));
+                    phase.re = s;
+                    phase.im = 0;
+                } else if (mode == 0) {
+                    // fftfreq = frequency * (1 / shape), the z component being 0 (D == 1).
+                    const R fz = R(0) * R(1), fyn = static_cast<R>(fy) * norm_y, fxn = static_cast<R>(x) * norm_x;
+                    if ((fz * fz + fyn * fyn) + fxn * fxn <= cutoff_sqd) {
+                        const R two_pi = 2 * pi_v<R>();
+                        const R factor = -((two_pi * sy) * fyn + (two_pi * sx) * fxn);
+                        const real_t f = static_cast<real_t>(factor);
+                        phase.re = std::cos(f);
+                        phase.im = std::sin(f);
+                    }
+                }
+                V& o = output[(bi * h + oy) * wr + x];
+                if (input) {
+                    const V a = input[(bi * h + y) * wr + x];
+                    o = V{a.re * phase.re - a.im * phase.im, a.re * phase.im + a.im * phase.re};
+                } else {
+                    o = phase;
+                }
+            }
+        }
+    }
+    return NOA_HIP_OK;
+}
+
 } // namespace oracle
 
 extern "C" {
@@ -909,6 +974,19 @@ int oracle_insert_central_slices_3d(const noa_hip_project_args* a, int n_threads
 }
 int oracle_extract_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
     return oracle::dispatch(oracle::EXTRACT, a, n_threads, force != 0);
+}
+
+/* noa::signal::phase_shift_2d<REMAP> on contiguous (b, h, w/2+1) c32 or c64 arrays; `input` may be NULL (the
+ * phase shifts are written); shifts: (y, x) pairs, f32 or f64, one (batched == 0) or b of them. */
+int oracle_phase_shift_2d(const void* input, void* output, int c64, long long b, long long h, long long w, int remap,
+                          const void* shifts, int shifts_f64, int batched, double cutoff) {
+    using namespace oracle;
+    if (c64) {
+        if (shifts_f64) return phase_shift_2d(static_cast<const Cx<double>*>(input), static_cast<Cx<double>*>(output), b, h, w, remap, static_cast<const double*>(shifts), batched != 0, cutoff);
+        return phase_shift_2d(static_cast<const Cx<double>*>(input), static_cast<Cx<double>*>(output), b, h, w, remap, static_cast<const float*>(shifts), batched != 0, cutoff);
+    }
+    if (shifts_f64) return phase_shift_2d(static_cast<const Cx<float>*>(input), static_cast<Cx<float>*>(output), b, h, w, remap, static_cast<const double*>(shifts), batched != 0, cutoff);
+    return phase_shift_2d(static_cast<const Cx<float>*>(input), static_cast<Cx<float>*>(output), b, h, w, remap, static_cast<const float*>(shifts), batched != 0, cutoff);
 }
 
 /* Pinned helpers, exposed for the oracle's own tests. */

@@ -254,3 +254,37 @@ def interpolate_spectrum_2d(slices: np.ndarray, batch: int, freqs: np.ndarray, s
     lib().oracle_interpolate_spectrum_2d_c32(s.ctypes.data, st.ctypes.data, batch, shape[0], shape[1],
                                              int(centered), interp, f.ctypes.data, len(f), out.ctypes.data)
     return out
+
+
+def phase_shift_2d(remap, input, output, shape, shifts, fftfreq_cutoff=1.0):
+    """noa::signal::phase_shift_2d<REMAP> (signal/PhaseShift.hpp:215-289) on contiguous (B, 1, H, W/2+1) arrays.
+
+    input: complex64/complex128 array or None (the phase shifts are written); shifts: one (y, x) pair or one per
+    batch (B, 2), float32 or float64 (the coordinate precision)."""
+    out = np.asarray(output)
+    if out.dtype not in (np.complex64, np.complex128) or not out.flags.c_contiguous:
+        raise OracleError("phase_shift_2d: the output should be a contiguous complex array")
+    b, d, h, wr = out.shape
+    w = shape[3]
+    if d != 1 or shape[0] != b or shape[1] != 1 or shape[2] != h or w // 2 + 1 != wr:
+        raise OracleError("phase_shift_2d: shape mismatch")
+    inp = None
+    if input is not None:
+        inp = np.ascontiguousarray(input, dtype=out.dtype)
+        if inp.shape != out.shape:
+            raise OracleError("phase_shift_2d: the input should have the output's shape")
+    sh = np.asarray(shifts)
+    if sh.dtype not in (np.float32, np.float64):
+        sh = sh.astype(np.float64)
+    batched = sh.ndim == 2
+    if batched and sh.shape != (b, 2) or not batched and sh.shape != (2,):
+        raise OracleError("phase_shift_2d: expected one (y, x) shift or one per batch")
+    sh = np.ascontiguousarray(sh)
+    fn = lib().oracle_phase_shift_2d
+    fn.restype = ctypes.c_int
+    status = fn(ctypes.c_void_p(inp.ctypes.data if inp is not None else 0), ctypes.c_void_p(out.ctypes.data),
+                int(out.dtype == np.complex128), ctypes.c_longlong(b), ctypes.c_longlong(h), ctypes.c_longlong(w),
+                LAYOUTS[remap.lower()] if isinstance(remap, str) else int(remap), ctypes.c_void_p(sh.ctypes.data),
+                int(sh.dtype == np.float64), int(batched), ctypes.c_double(fftfreq_cutoff))
+    if status != 0:
+        raise OracleError(lib().oracle_last_error().decode())

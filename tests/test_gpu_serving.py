@@ -1,0 +1,79 @@
+"""The continuous-batching request loop: every result is the same alone and in the mix, and merging happens."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(device, n=24, boxes=(64, 128, 192), max_k=40, seed=5):
+    from noa_amd import workloads
+    from noa_amd.serve import Request
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=device).manual_seed(seed)
+    reqs = []
+    for i in range(n):
+        box = int(rng.choice(boxes))
+        k = int(rng.integers(1, max_k + 1))
+        poses = workloads.uniform_rotations(k, rng)
+        arrival = 0.0005 * i
+        if i % 2 == 0:
+            reqs.append(Request(rid=i, kind="extract", box=box, volume=int(rng.integers(0, 2)),
+                                rotations=torch.from_numpy(poses).to(device), arrival=arrival))
+        else:
+            spec = torch.fft.rfft2(torch.randn((k, box, box), generator=g, device=device), norm="forward")
+            w = torch.rand((k, box, box // 2 + 1), generator=g, device=device)
+            inv = np.ascontiguousarray(np.transpose(poses, (0, 2, 1)))
+            reqs.append(Request(rid=i, kind="insert", box=box, rotations=torch.from_numpy(inv).to(device),
+                                slices=spec.reshape(k, 1, box, box // 2 + 1).contiguous(),
+                                weights=w.reshape(k, 1, box, box // 2 + 1).contiguous(), arrival=arrival))
+    return reqs
+
+
+def _clone(r):
+    from noa_amd.serve import Request
+    return Request(rid=r.rid, kind=r.kind, box=r.box, rotations=r.rotations, volume=r.volume, slices=r.slices,
+                   weights=r.weights, arrival=0.0)
+
+
+def test_results_are_the_same_alone_and_in_the_mix():
+    from noa_amd import workloads
+    from noa_amd.serve import Server, run_alone, run_open_loop
+    dev = torch.device("cuda")
+    vols = {b: v[:2] for b, v in workloads.serving_volumes(dev, per_box=2).items() if b in (64, 128, 192)}
+    reqs = _mix(dev)
+    torch.cuda.synchronize()
+    server = Server(vols, budget_bytes=256 << 20, n_streams=3)
+    stats = run_open_loop(server, reqs, timeout_s=60)
+    torch.cuda.synchronize()
+    assert len(stats.latencies_ms) == len(reqs)
+    assert all(r.done is not None and r.done >= r.arrival for r in reqs)
+    assert stats.merged > 0, "no extraction requests were merged into a shared launch"
+    for r in reqs:
+        alone = run_alone(Server(vols, budget_bytes=256 << 20, n_streams=1), _clone(r))
+        assert torch.equal(r.result, alone.result), f"request {r.rid} ({r.kind}) differs alone and in the mix"
+        if r.kind == "insert":
+            assert torch.equal(r.result_weight, alone.result_weight)
+
+
+def test_budget_is_respected_and_large_requests_still_run():
+    from noa_amd import workloads
+    from noa_amd.serve import Server, run_open_loop
+    dev = torch.device("cuda")
+    vols = {b: v[:2] for b, v in workloads.serving_volumes(dev, per_box=2).items() if b in (64, 128, 192)}
+    reqs = _mix(dev, n=10, seed=6)
+    torch.cuda.synchronize()
+    budget = 4 << 20          # smaller than one 192^3 accumulator: such requests run alone
+    server = Server(vols, budget_bytes=budget, n_streams=2)
+    peak = 0
+    orig = server._launch
+
+    def launch_and_watch():
+        nonlocal peak
+        orig()
+        peak = max(peak, server.in_use)
+    server._launch = launch_and_watch
+    stats = run_open_loop(server, reqs, timeout_s=60)
+    assert len(stats.latencies_ms) == len(reqs)
+    largest = max(r.output_bytes() for r in reqs)
+    assert peak <= max(budget, largest)

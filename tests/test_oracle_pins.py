@@ -468,3 +468,51 @@ def test_extract_target_shape_scales_frequencies():
         ok = (np.abs(k) <= n / 2 - 2).all(-1)
         assert ok.sum() > 100
         assert rel_l2(out[b, 0][ok], want[ok]) < 5e-6
+
+
+# ------------------------------------------------------------------------------------------------ phase shift
+@pytest.mark.parametrize("h, w", [(32, 32), (30, 24), (17, 21)])
+def test_phase_shift_is_the_fft_shift_theorem(h, w):
+    """rfft2(roll(image, s)) == phase_shift_2d(rfft2(image), s) for integer shifts (numpy's FFT and roll are
+    independent of the oracle): pins the sign, the 2 pi factor and the (y, x) order of the shift."""
+    rng = np.random.default_rng(41)
+    img = rng.standard_normal((3, h, w))
+    shifts = np.array([[3, -5], [0, 7], [-h // 2, 1]], dtype=np.float64)
+    spec = np.fft.rfft2(img).astype(np.complex128)[:, None]
+    out = np.zeros_like(spec)
+    O.phase_shift_2d("h2h", spec, out, (3, 1, h, w), shifts)
+    want = np.stack([np.fft.rfft2(np.roll(img[i], (int(shifts[i, 0]), int(shifts[i, 1])), axis=(0, 1)))
+                     for i in range(3)])[:, None]
+    assert rel_l2(out, want) < 1e-12
+    # Single-precision shifts and values.
+    out32 = np.zeros(spec.shape, np.complex64)
+    O.phase_shift_2d("h2h", spec.astype(np.complex64), out32, (3, 1, h, w), shifts.astype(np.float32))
+    assert rel_l2(out32, want) < 1e-5
+
+
+def test_phase_shift_cutoff_remap_and_special_cases():
+    rng = np.random.default_rng(42)
+    h = w = 32
+    img = rng.standard_normal((1, h, w))
+    spec = np.fft.rfft2(img).astype(np.complex128)[:, None]
+    # Half shifts take the +-1 pattern: identical to the general formula up to rounding, and exactly real.
+    half = np.zeros_like(spec)
+    O.phase_shift_2d("h2h", None, half, (1, 1, h, w), np.array([h / 2, -w / 2]))
+    assert np.all(half.imag == 0) and set(np.unique(half.real)) == {-1.0, 1.0}
+    want = np.fft.rfft2(np.roll(img[0], (h // 2, -w // 2), axis=(0, 1)))
+    got = np.zeros_like(spec)
+    O.phase_shift_2d("h2h", spec, got, (1, 1, h, w), np.array([h / 2, -w / 2]))
+    assert rel_l2(got[0, 0], want) < 1e-12
+    # A zero shift copies; h2hc also recentres.
+    got = np.zeros_like(spec)
+    O.phase_shift_2d("h2hc", spec, got, (1, 1, h, w), np.array([0.0, 0.0]))
+    np.testing.assert_array_equal(got[0, 0], np.fft.fftshift(spec[0, 0], axes=0))
+    # Frequencies beyond the cutoff are not shifted (batched shifts skip the special cases).
+    got = np.zeros_like(spec)
+    O.phase_shift_2d("h2h", spec, got, (1, 1, h, w), np.array([[2.5, 1.5]]), fftfreq_cutoff=0.25)
+    fy = np.fft.fftfreq(h)[:, None]
+    fx = np.arange(w // 2 + 1)[None, :] / w
+    outside = fy ** 2 + fx ** 2 > 0.25 ** 2
+    np.testing.assert_array_equal(got[0, 0][outside], spec[0, 0][outside])
+    ph = np.exp(-2j * np.pi * (2.5 * fy + 1.5 * fx))
+    assert rel_l2(got[0, 0][~outside], (spec[0, 0] * ph)[~outside]) < 1e-12
