@@ -52,7 +52,8 @@ def test_half_storage_equals_fp32_math_on_the_rounded_volume(storage):
 
 def test_storage_error_against_the_fp64_oracle():
     """rel-L2 of each storage against the FP64 oracle (double volume, double rotations): FP32 ~1e-7, FP16 ~1e-3,
-    BF16 ~1e-2 (8-bit significand)."""
+    BF16 ~1e-2 (8-bit significand). Pixels within 1e-3 of the 0.5 cutoff circle are left out: there FP32 and FP64
+    coordinates can fall on different sides of the cutoff, and the pixel is the value or 0."""
     import noa_amd.xform as nx
     rng = np.random.default_rng(32)
     n = 48
@@ -63,9 +64,12 @@ def test_storage_error_against_the_fp64_oracle():
                                 None, rot)
     dev = torch.from_numpy(vol32).cuda()
     errs = {}
+    fy = (np.arange(n) - n // 2)[:, None] / n
+    fx = np.arange(n // 2 + 1)[None, :] / n
+    keep = np.broadcast_to(np.abs(np.sqrt(fy ** 2 + fx ** 2) - 0.5) > 1e-3, (16, 1, n, n // 2 + 1))
     for name, v in (("fp32", dev), ("fp16", dev.to(torch.complex32)), ("bf16", nx.ComplexBF16.from_complex(dev))):
         got = _extract_gpu(v, n, rot.astype(np.float32))
-        errs[name] = rel_l2(got.astype(np.complex128), ref)
+        errs[name] = rel_l2(got.astype(np.complex128)[keep], ref[keep])
     assert errs["fp32"] < 1e-5
     assert errs["fp16"] < 2e-3
     assert errs["bf16"] < 2e-2
