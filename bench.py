@@ -22,6 +22,15 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM-side bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
+# `bench.py --probe`, scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); recorded in
+# profiles/insert_wave_kernel_256cube_10k_slices.md and profiles/extract_kernel_256cube_4096_slices.md for the
+# default workload sizes.
+MEASURED_TRAFFIC = {
+    ("insert", 10_000): (9_107_753 * 2.0 + 387_096) * 1024,
+    ("extract", 4096): (3_885_198 * 2.0 + 1_056_768) * 1024,
+}
 METRIC = "slices/s and % HBM peak, 256\u00b3 insert+extract; p50/p99 request latency"
 
 
@@ -328,12 +337,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": {"insert": "insert_wave_kernel (+ insert_reduce_kernel, 2% of the operator)",
+                       "extract": "extract_kernel"}[dom],
             "achieved": kernels[dom]["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": kernels[dom]["frac_hbm_peak"],
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((dom, args.n_insert if dom == "insert" else args.n_extract)),
         },
         "kernels": kernels,
     }
