@@ -316,7 +316,63 @@ void extract_central_slices_3d(const Input& volume, const InputWeight& volume_we
     details::check(noa_hip_extract_central_slices_3d(&a, stream));
 }
 
+namespace details {
+    /* One (y, x) shift on the host, or a View of (y, x) pairs (Vec<f32,2> / Vec<f64,2>) on the device. */
+    template<class T> struct Vec2 { T y, x; };
+    template<class Shift>
+    noa_hip_shift2 shifts(const Shift& s, f64 cutoff) {
+        noa_hip_shift2 r{};
+        r.fftfreq_cutoff = cutoff;
+        using U = std::remove_cvref_t<Shift>;
+        if constexpr (is_view_v<U>) {
+            using E = std::remove_cv_t<std::remove_pointer_t<decltype(s.ptr)>>;
+            r.data = s.ptr;
+            r.dtype = std::is_same_v<E, Vec2<f64>> ? NOA_HIP_F64 : NOA_HIP_F32;
+            r.batched = 1;
+            r.count = s.ssize();
+        } else {
+            r.data = &s;
+            r.dtype = std::is_same_v<U, Vec2<f64>> ? NOA_HIP_F64 : NOA_HIP_F32;
+            r.batched = 0;
+            r.count = 1;
+        }
+        return r;
+    }
+} // namespace details
+
+template<class T> using Vec2 = details::Vec2<T>;
+
+/* extract_central_slices_3d followed by noa::signal::phase_shift_2d of the slices in their own layout, fused
+ * (the slices are written once). `shifts`: one Vec2 (host) or a View of one Vec2 per slice (device). */
+template<Layout REMAP, class Input, class InputWeight = Empty, class Output, class OutputWeight = Empty,
+         class Scale = Empty, class Rotate, class Shift>
+void extract_central_slices_3d_shifted(const Input& volume, const InputWeight& volume_weight,
+                                       const Shape4& volume_shape, const Output& slice,
+                                       const OutputWeight& slice_weight, const Shape4& slice_shape,
+                                       const Scale& inv_scaling, const Rotate& fwd_rotation, const Shift& shifts,
+                                       const ExtractCentralSlicesOptions& options = {}, f64 shift_cutoff = 1,
+                                       void* stream = nullptr) {
+    const noa_hip_project_args a = details::make_args(REMAP.bits, volume, volume_weight, volume_shape, slice,
+                                                      slice_weight, slice_shape, inv_scaling, fwd_rotation,
+                                                      options);
+    const noa_hip_shift2 sh = details::shifts(shifts, shift_cutoff);
+    details::check(noa_hip_extract_central_slices_3d_shifted(&a, &sh, stream));
+}
+
 } // namespace xform
+
+namespace signal {
+/* noa::signal::phase_shift_2d<REMAP> (src/noa/signal/PhaseShift.hpp:215-289) for the rfft layouts. `input` may be
+ * Empty (the phase shifts are written). */
+template<xform::Layout REMAP, class Input, class Output, class Shift>
+void phase_shift_2d(const Input& input, const Output& output, const Shape4& shape, const Shift& shifts,
+                    f64 fftfreq_cutoff = 1, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input);
+    const noa_hip_view out = xform::details::view(output);
+    const noa_hip_shift2 sh = xform::details::shifts(shifts, fftfreq_cutoff);
+    xform::details::check(noa_hip_phase_shift_2d(&in, &out, shape.v.data(), REMAP.bits, &sh, stream));
+}
+} // namespace signal
 } // namespace noa_hip
 
 #endif // NOA_HIP_HPP

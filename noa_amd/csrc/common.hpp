@@ -23,6 +23,7 @@ namespace noa_hip {
 using i32 = int32_t;
 using i64 = int64_t;
 using u32 = uint32_t;
+using u64 = uint64_t;
 
 // ---------------------------------------------------------------------------------------------------------
 // Value types: real (float, double) and complex (float2, double2 as {re, im}).
@@ -308,6 +309,37 @@ __device__ __forceinline__ Math interp3(const Storage* __restrict__ vol, i32 sz,
     }
 }
 
+// Element loaders of the two-phase interpolation. GlobalLoad: a flat pointer (64-bit address per tap).
+// BufferLoad: a buffer resource over at most 4 GiB (base and size in scalar registers, a 32-bit byte offset per tap,
+// out-of-range offsets read 0), which saves the 64-bit address arithmetic of every tap.
+template<class S>
+struct GlobalLoad {
+    const S* __restrict__ base;
+    template<class Math> __device__ __forceinline__ Math load(u32 elem) const { return load_as<Math>(base + elem); }
+};
+
+template<class S>
+struct BufferLoad {
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ __forceinline__ BufferLoad(const S* base, u32 bytes)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<S*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
+    template<class Math> __device__ __forceinline__ Math load(u32 elem) const {
+        const u32 off = elem * static_cast<u32>(sizeof(S));
+        S v;
+        if constexpr (sizeof(S) == 4) {
+            const u32 r = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+            __builtin_memcpy(&v, &r, 4);
+        } else if constexpr (sizeof(S) == 8) {
+            const auto r = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
+            __builtin_memcpy(&v, &r, 8);
+        } else {
+            const auto r = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
+            __builtin_memcpy(&v, &r, 16);
+        }
+        return load_as<Math>(&v);
+    }
+};
+
 // The same 3D interpolation split in two phases, so that a lane can have the loads of several samples in flight:
 // fetch3 resolves the flip, the fractions and the tap offsets and issues every load (an out-of-bounds tap reads
 // element 0 and is zeroed), combine3 sums the taps in noa's order (x innermost, then y, then z) and conjugates.
@@ -319,8 +351,8 @@ struct Taps3 {
     typename vt<Math>::real tz, ty, tx, conj;
 };
 
-template<int INTERP, class Math, class Storage, class C>
-__device__ __forceinline__ Taps3<Math, C> fetch3(const Storage* __restrict__ vol, i32 sz, i32 sy, i32 sx,
+template<int INTERP, class Math, class Loader, class C>
+__device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, i32 sz, i32 sy, i32 sx,
                                                  const Bounds3& bd, bool centered, C fz, C fy, C fx, bool valid) {
     using R = typename vt<Math>::real;
     Taps3<Math, C> t;
@@ -331,7 +363,7 @@ __device__ __forceinline__ Taps3<Math, C> fetch3(const Storage* __restrict__ vol
         const bool ok = valid && z >= bd.lo_z && z <= bd.hi_z && y >= bd.lo_y && y <= bd.hi_y && x >= 0 && x <= bd.hi_x;
         const u32 off = ok ? static_cast<u32>(frequency2index(centered, z, bd.nz) * sz +
                                               frequency2index(centered, y, bd.ny) * sy + x * sx) : 0u;
-        t.v[0][0][0] = load_as<Math>(vol + off);
+        t.v[0][0][0] = vol.template load<Math>(off);
         t.ok = ok ? 1u : 0u;
         t.tz = t.ty = t.tx = 0;
         t.conj = 1;   // no conjugation after the flip for NEAREST (Interpolation.hpp:479-484)
@@ -360,7 +392,7 @@ __device__ __forceinline__ Taps3<Math, C> fetch3(const Storage* __restrict__ vol
                 for (int x = 0; x < 2; ++x) {
                     const bool ok = okz[z] && oky[y] && okx[x];
                     const u32 off = ok ? static_cast<u32>(oz[z] + oy[y] + (x0 + x) * sx) : 0u;
-                    t.v[z][y][x] = load_as<Math>(vol + off);
+                    t.v[z][y][x] = vol.template load<Math>(off);
                     t.ok |= ok ? (1u << (z * 4 + y * 2 + x)) : 0u;
                 }
     }

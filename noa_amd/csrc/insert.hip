@@ -75,6 +75,7 @@ struct InsertParams {
     float wa[NWIN + 1];   // windowed sinc as a polynomial in t = 2 (fz/f_b)^2 - 1 (single-precision staged path)
     bool wpoly;           // use wa; otherwise evaluate noa's expression
     float inv_fb;         // 1 / fftfreq_blackman
+    u32 slice_bytes, wslice_bytes;   // byte extent of one slice and of its weights (buffer loads)
 };
 
 // Keeps a value in vector registers: the compiler would otherwise re-read kernel arguments inside the loops,
@@ -625,11 +626,47 @@ struct Taps {
     typename vt<In>::real ty, tx, conj;
 };
 
+// A slice (or its weights) read through a buffer resource: base and byte size in scalar registers, a 32-bit byte
+// offset per load, and out-of-range offsets read 0.
+struct SliceBuf {
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ __forceinline__ SliceBuf(const void* base, u32 bytes)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
+    template<class T> __device__ __forceinline__ T load(u32 elem) const {
+        T v;
+        const u32 off = elem * static_cast<u32>(sizeof(T));
+        if constexpr (sizeof(T) == 4) {
+            const u32 r = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+            __builtin_memcpy(&v, &r, 4);
+        } else if constexpr (sizeof(T) == 8) {
+            const auto r = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
+            __builtin_memcpy(&v, &r, 8);
+        } else {
+            const auto r = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
+            __builtin_memcpy(&v, &r, 16);
+        }
+        return v;
+    }
+    // Two consecutive elements (the x and x+1 taps of a row with unit x stride) in one load.
+    template<class T> __device__ __forceinline__ void load_pair(u32 elem, T& a, T& b) const {
+        const u32 off = elem * static_cast<u32>(sizeof(T));
+        if constexpr (sizeof(T) == 4) {
+            const auto r = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
+            __builtin_memcpy(&a, &r, 4);
+            __builtin_memcpy(&b, reinterpret_cast<const char*>(&r) + 4, 4);
+        } else {
+            const auto r = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
+            __builtin_memcpy(&a, &r, 8);
+            __builtin_memcpy(&b, reinterpret_cast<const char*>(&r) + 8, 8);
+        }
+    }
+};
+
+// `pairs`: both arrays have a unit x stride, so each row's two taps come from one load.
 template<int INTERP, class In, class W, class C>
-__device__ __forceinline__ Taps<In, W, C> fetch_taps(const In* __restrict__ vb, i32 vsy, i32 vsx,
-                                                     const W* __restrict__ wb, i32 wsy, i32 wsx, bool has_w, i32 ny,
-                                                     i32 nx, bool centered, C fy, C fx, bool valid) {
-    using R = typename vt<In>::real;
+__device__ __forceinline__ Taps<In, W, C> fetch_taps(const SliceBuf& vb, i32 vsy, i32 vsx, const SliceBuf& wb,
+                                                     i32 wsy, i32 wsx, bool has_w, bool pairs, i32 ny, i32 nx,
+                                                     bool centered, C fy, C fx, bool valid) {
     Taps<In, W, C> t;
     t.conj = 1;
     if (fx < 0) { fy *= -1; fx *= -1; t.conj = -1; }
@@ -638,39 +675,37 @@ __device__ __forceinline__ Taps<In, W, C> fetch_taps(const In* __restrict__ vb, 
         const i32 y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
         const bool ok = valid && y >= lo_y && y <= hi_y && x >= 0 && x <= hi_x;
         const i32 iy = ok ? frequency2index(centered, y, ny) : 0, ix = ok ? x : 0;
-        t.v[0][0] = vb[iy * vsy + ix * vsx];
-        t.w[0][0] = has_w ? wb[iy * wsy + ix * wsx] : W(0);
+        t.v[0][0] = vb.template load<In>(static_cast<u32>(iy * vsy + ix * vsx));
+        t.w[0][0] = has_w ? wb.template load<W>(static_cast<u32>(iy * wsy + ix * wsx)) : W(0);
         t.ok = ok ? 1u : 0u;
         t.ty = 0;
         t.tx = 0;
     } else {
         const C fly = floor(fy), flx = floor(fx);
         const i32 y0 = static_cast<i32>(fly), x0 = static_cast<i32>(flx);
-        t.ty = static_cast<R>(fy - fly);
-        t.tx = static_cast<R>(fx - flx);
-        i32 ov[2][2], ow[2][2];
-        bool ok[2][2];
+        t.ty = static_cast<typename vt<In>::real>(fy - fly);
+        t.tx = static_cast<typename vt<In>::real>(fx - flx);
+        const bool okx0 = x0 >= 0 && x0 <= hi_x, okx1 = x0 + 1 >= 0 && x0 + 1 <= hi_x;
+        t.ok = 0;
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
             const bool oky = valid && y0 + y >= lo_y && y0 + y <= hi_y;
             const i32 iy = oky ? frequency2index(centered, y0 + y, ny) : 0;
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                ok[y][x] = oky && x0 + x >= 0 && x0 + x <= hi_x;
-                const i32 ix = ok[y][x] ? x0 + x : 0;
-                ov[y][x] = ok[y][x] ? iy * vsy + ix * vsx : 0;
-                ow[y][x] = ok[y][x] ? iy * wsy + ix * wsx : 0;
+            const i32 x = oky && okx0 ? x0 : 0;
+            t.ok |= (oky && okx0 ? 1u : 0u) << (y * 2);
+            t.ok |= (oky && okx1 ? 1u : 0u) << (y * 2 + 1);
+            if (pairs) {
+                vb.load_pair(static_cast<u32>(iy * vsy + x), t.v[y][0], t.v[y][1]);
+                if (has_w) wb.load_pair(static_cast<u32>(iy * wsy + x), t.w[y][0], t.w[y][1]);
+                else { t.w[y][0] = W(0); t.w[y][1] = W(0); }
+            } else {
+                const i32 x1 = oky && okx1 ? x0 + 1 : 0;
+                t.v[y][0] = vb.template load<In>(static_cast<u32>(iy * vsy + x * vsx));
+                t.v[y][1] = vb.template load<In>(static_cast<u32>(iy * vsy + x1 * vsx));
+                t.w[y][0] = has_w ? wb.template load<W>(static_cast<u32>(iy * wsy + x * wsx)) : W(0);
+                t.w[y][1] = has_w ? wb.template load<W>(static_cast<u32>(iy * wsy + x1 * wsx)) : W(0);
             }
         }
-        t.ok = 0;
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                t.v[y][x] = vb[ov[y][x]];
-                t.w[y][x] = has_w ? wb[ow[y][x]] : W(0);
-                t.ok |= ok[y][x] ? (1u << (y * 2 + x)) : 0u;
-            }
     }
     return t;
 }
@@ -761,6 +796,8 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
     }
     const C slack = C(1e-5);
     const i32 Hs = p.slice_shape[0], Ws = p.slice_shape[1];
+    const bool all_contig = contiguous[0] && contiguous[1] && contiguous[2];
+    const bool pairs = p.is_x == 1 && (!p.win_present || p.wis_x == 1);
 
     // Values of the hot loop, held in vector registers.
     const C fb = pin(p.fftfreq_blackman);
@@ -851,15 +888,32 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
         // Candidates of the lane's columns (lane and lane + 64): |alA * f + beta| <= f_b in exact arithmetic,
         // widened by eps samples to cover the rounding of the exact per-voxel test (a few ulps of |f| <= 1,
         // divided by |alA| >= 1/(sqrt(3) T)).
+        // With contiguous face axes the range centre is linear in the column: centre = u0 + ub lb + uc lc (in
+        // samples along A, relative to the sub-brick's first sample), computed with two FMAs.
+        const bool contigB = ax == 0 ? contiguous[1] : contiguous[0];
+        const bool contigC = ax == 2 ? contiguous[1] : contiguous[2];
+        const bool linear = ranged && contigB && contigC;
+        const i32 f0B = ax == 0 ? f0[1] : f0[0];
+        const i32 f0C = ax == 2 ? f0[1] : f0[2];
+        const C eps_l = C(1e-3) + C(8e-6) * tA;
+        const C ub = -ai.alB * ai.inv_alA, uc = -ai.alC * ai.inv_alA;
+        const C u0 = -(ai.alB * static_cast<C>(f0B) + ai.alC * static_cast<C>(f0C)) * ai.inv_alA - static_cast<C>(f0A);
+        const C halfw = fabs(fb * ai.inv_alA) + eps_l;
         i32 cnt[2] = {0, 0}, klo[2] = {0, 0};
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const i32 col = lane + 64 * j;
             if (col >= n_col) continue;
-            const i32 gb = orgB + (col >> lc_bits), gc = orgC + (col & (fc - 1));
+            const i32 lb = col >> lc_bits, lc = col & (fc - 1);
+            const i32 gb = orgB + lb, gc = orgC + lc;
             if (gb >= extB || gc >= extC) continue;
             i32 k_lo = 0, k_hi = nA - 1;
-            if (ranged) {
+            if (linear) {
+                const C centre = fma(ub, static_cast<C>(lb), fma(uc, static_cast<C>(lc), u0));
+                const C kl = ceil(centre - halfw), kh = floor(centre + halfw);
+                k_lo = static_cast<i32>(fmin(fmax(kl, C(0)), static_cast<C>(nA)));
+                k_hi = static_cast<i32>(fmax(fmin(kh, static_cast<C>(nA - 1)), C(-1)));
+            } else if (ranged) {
                 const i32 fbq = index2frequency(vc, gb, gridB);
                 const i32 fcq = axis_freq(xC, gc, vc, gridC);
                 const C beta = ai.alB * static_cast<C>(fbq) + ai.alC * static_cast<C>(fcq);
@@ -880,8 +934,9 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
         const i32 total = __builtin_amdgcn_readlane(incl, 63);
         const i32 excl = incl - mine_cnt;
 
-        const In* __restrict__ sl = p.in + static_cast<i64>(cur) * p.is_b;
-        const W* __restrict__ wsl = win_present ? p.win + static_cast<i64>(cur) * p.wis_b : p.win;
+        const SliceBuf sl(p.in + static_cast<i64>(cur) * p.is_b, p.slice_bytes);
+        const SliceBuf wsl(win_present ? static_cast<const void*>(p.win + static_cast<i64>(cur) * p.wis_b)
+                                       : static_cast<const void*>(p.in), win_present ? p.wslice_bytes : 0u);
         for (i32 base = 0; base < total; base += 64 * WG) {
             // Pack the candidates [base, base + 64 WG) of the wave into the list, WG per lane.
             __syncthreads();
@@ -903,6 +958,8 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
             bool valid[WG];
 #pragma unroll
             for (int g = 0; g < WG; ++g) {
+                valid[g] = false;
+                if (base + g * 64 >= total) continue;   // uniform: an empty group of 64 is skipped
                 const i32 e = cand[g * 64 + lane];
                 const i32 k = e & (ax == 2 ? 15 : 7);
                 const i32 col = (e >> 4) & (n_col - 1);
@@ -910,8 +967,8 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
                 const i32 lz = ax == 0 ? k : lb;
                 const i32 ly = ax == 1 ? k : (ax == 0 ? lb : lc);
                 const i32 lx = ax == 2 ? k : lc;
-                const i32 frz = index2frequency(vc, org[0] + lz, grid[0]);
-                const i32 fry = index2frequency(vc, org[1] + ly, grid[1]);
+                const i32 frz = all_contig ? f0[0] + lz : index2frequency(vc, org[0] + lz, grid[0]);
+                const i32 fry = all_contig ? f0[1] + ly : index2frequency(vc, org[1] + ly, grid[1]);
                 const i32 frx = org[2] + lx;
                 // The normalised frequency as noa computes it: index2frequency / target (:479-484).
                 vec3<C> f;
@@ -931,13 +988,13 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
                     window[g] = windowed_sinc<C>(fz, fs, fb);
                 }
                 lv[g] = (lz * WY + ly) * WX + lx;
-                tp[g] = fetch_taps<INTERP>(sl, p.is_y, p.is_x, wsl, p.wis_y, p.wis_x, win_present, Hs, Ws,
+                tp[g] = fetch_taps<INTERP, In, W>(sl, p.is_y, p.is_x, wsl, p.wis_y, p.wis_x, win_present, pairs, Hs, Ws,
                                            p.slices_centered, f2.y * hs_f, f2.x * ws_f, valid[g]);
             }
             // Phase 2: interpolate (Interpolation.hpp:435-526) and accumulate.
 #pragma unroll
             for (int g = 0; g < WG; ++g) {
-                if (!valid[g]) continue;
+                if (base + g * 64 >= total || !valid[g]) continue;
                 In v;
                 W wv;
                 combine_taps<INTERP>(tp[g], win_present, v, wv);
@@ -1336,9 +1393,20 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     if (p.n_slices == 0) return NOA_HIP_OK;
     const bool nearest = d.interp == NOA_HIP_INTERP_NEAREST;
 
+    // Byte extents of one slice and of its weights, for the buffer loads of the one-wave kernel.
+    auto extent = [&](i64 sy, i64 sx, size_t elem) -> u64 {
+        if (sy < 0 || sx < 0) return ~u64(0);
+        return (static_cast<u64>(p.slice_shape[0] - 1) * sy + static_cast<u64>(p.slice_shape[1] / 2) * sx + 1) * elem;
+    };
+    const u64 sb = extent(p.is_y, p.is_x, sizeof(In));
+    const u64 wb = p.win_present ? extent(p.wis_y, p.wis_x, sizeof(W)) : 0;
+    const bool buffers_ok = sb < 0xffffffffULL && wb < 0xffffffffULL;
+    p.slice_bytes = buffers_ok ? static_cast<u32>(sb) : 0;
+    p.wslice_bytes = buffers_ok ? static_cast<u32>(wb) : 0;
+
     constexpr bool single = sizeof(typename vt<In>::real) == 4 && sizeof(W) == 4;
     if constexpr (single) {
-        if (!p.general && !p.in_is_value && !p.win_is_value) {
+        if (!p.general && !p.in_is_value && !p.win_is_value && (buffers_ok || kc.staged)) {
             int device = 0;
             (void)hipGetDevice(&device);
             Plan plan;

@@ -174,10 +174,24 @@ static void check_gpu() {
     HIP_OK(hipFree(dw));
 }
 
+static void check_phase_shift_errors() {
+    // A depth-2 shape is not a 2d phase shift: rejected before any memory is touched.
+    const View<c32> s(reinterpret_cast<c32*>(0x10000000), Shape4(2, 2, 32, 17));
+    bool thrown = false;
+    try {
+        noa_hip::signal::phase_shift_2d<"h2h">(Empty{}, s, Shape4(2, 2, 32, 32), nx::Vec2<float>{1.f, 2.f});
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+    }
+    EXPECT(thrown);
+}
+
 int main(int argc, char** argv) {
     check_layouts();
     check_descriptor();
     check_errors();
+    check_phase_shift_errors();
     if (argc > 1 && std::strcmp(argv[1], "--gpu") == 0) check_gpu();
     std::printf("all facade checks passed\n");
     return 0;
