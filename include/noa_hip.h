@@ -186,6 +186,21 @@ int noa_hip_phase_shift_2d(const noa_hip_view* input, const noa_hip_view* output
 int noa_hip_extract_central_slices_3d_shifted(const noa_hip_project_args* args, const noa_hip_shift2* shifts,
                                               void* stream);
 
+/* Replaces the GPU branch of noa::fft::remap (src/noa/fft/Remap.hpp:205-340) for the half-to-half layouts (h2h,
+ * h2hc, hc2h, hc2hc) of (B, D, H, W/2+1) rfft arrays: output[b, remap(z, y), x] = input[b, z, y, x], the D and H
+ * indices fftshifted or ifftshifted. Input and output have the same element type; they must not overlap when the
+ * layout changes (a layout-preserving remap is a copy and may be in place). */
+int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                  void* stream);
+
+/* Replaces the GPU branch of noa::xform::fourier_interpolation_correction (src/noa/xform/FourierProject.hpp:
+ * 2062-2087, operator :1008-1061): real-space correction of linear interpolation, output = input / sinc^2(pi r)
+ * (post_correction != 0) or input * sinc^2(pi r), r the distance to the box centre in fractions of the box. Real
+ * f32/f64 (B, D, H, W) arrays; the input may have a batch of 1 (broadcast) and may be the output. interp must be
+ * LINEAR or LINEAR_FAST, as in noa. */
+int noa_hip_fourier_interpolation_correction(const noa_hip_view* input, const noa_hip_view* output, int32_t interp,
+                                             int32_t post_correction, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

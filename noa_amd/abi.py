@@ -121,4 +121,9 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_phase_shift_2d.argtypes = [ctypes.POINTER(View), ctypes.POINTER(View), ctypes.POINTER(c_int64),
                                            c_int32, S, c_void_p]
     lib.noa_hip_phase_shift_2d.restype = ctypes.c_int
+    V = ctypes.POINTER(View)
+    lib.noa_hip_remap.argtypes = [c_int32, V, V, ctypes.POINTER(c_int64), c_void_p]
+    lib.noa_hip_remap.restype = ctypes.c_int
+    lib.noa_hip_fourier_interpolation_correction.argtypes = [V, V, c_int32, c_int32, c_void_p]
+    lib.noa_hip_fourier_interpolation_correction.restype = ctypes.c_int
     return lib

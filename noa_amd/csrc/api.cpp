@@ -442,6 +442,75 @@ int guarded(int device, const std::function<int(std::string&)>& launch) {
     return status;
 }
 
+Arr4 to_arr4(const noa_hip_view& v) {
+    Arr4 a;
+    a.data = v.data;
+    a.dtype = v.dtype;
+    for (int i = 0; i < 4; ++i) a.s[i] = v.strides[i];
+    return a;
+}
+
+RemapDesc validate_remap(int32_t remap, const noa_hip_view* in, const noa_hip_view* out, const int64_t* shape,
+                         int& device) {
+    NOA_CHECK(in != nullptr && out != nullptr && shape != nullptr, INVALID, "null arguments");
+    NOA_CHECK(in->data != nullptr && out->data != nullptr && !in->is_value && !out->is_value, INVALID,
+              "Empty array detected");
+    NOA_CHECK(remap == NOA_HIP_H2H || remap == NOA_HIP_HC2H || remap == NOA_HIP_H2HC || remap == NOA_HIP_HC2HC,
+              UNSUPPORTED, "remap: only the half-to-half layouts are implemented by this library, got %d", remap);
+    NOA_CHECK(dtype_size(in->dtype) != 0 && in->dtype == out->dtype, UNSUPPORTED,
+              "remap: input and output should have the same element type");
+    for (int i = 0; i < 4; ++i) NOA_CHECK(shape[i] > 0, INVALID, "Empty array detected");
+    int64_t expected[4];
+    rfft_shape(shape, expected);
+    NOA_CHECK(same_shape(in->shape, expected), INVALID,
+              "Given the remap, the input fft is expected to have a physical shape of %s, but got %s",
+              shape_str(expected).c_str(), shape_str(in->shape).c_str());
+    NOA_CHECK(same_shape(out->shape, expected), INVALID,
+              "Given the remap, the output fft is expected to have a physical shape of %s, but got %s",
+              shape_str(expected).c_str(), shape_str(out->shape).c_str());
+    const bool change = ((remap & 4) != 0) != ((remap & 8) != 0);
+    NOA_CHECK(!change || !overlap(*in, *out), UNSUPPORTED,
+              "remap: in-place remapping with a layout change is not supported");
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    NOA_CHECK(device_of(in->data, nullptr) == device, INVALID, "The arrays should be on the same device");
+    RemapDesc d;
+    d.input = to_arr4(*in);
+    d.output = to_arr4(*out);
+    for (int i = 0; i < 4; ++i) d.shape[i] = shape[i];
+    d.src_centered = (remap & 4) != 0;
+    d.dst_centered = (remap & 8) != 0;
+    return d;
+}
+
+GriddingDesc validate_gridding(const noa_hip_view* in, const noa_hip_view* out, int32_t interp, int32_t post,
+                               int& device) {
+    NOA_CHECK(in != nullptr && out != nullptr, INVALID, "null arguments");
+    NOA_CHECK(in->data != nullptr && out->data != nullptr && !in->is_value && !out->is_value, INVALID,
+              "Empty array detected");
+    NOA_CHECK(interp == NOA_HIP_INTERP_LINEAR || interp == NOA_HIP_INTERP_LINEAR_FAST, INVALID,
+              "Interpolation method %d is currently not supported", interp);
+    NOA_CHECK(is_real(in->dtype) && is_real(out->dtype), INVALID,
+              "fourier_interpolation_correction: real f32 or f64 arrays are expected");
+    for (int i = 0; i < 4; ++i) NOA_CHECK(out->shape[i] > 0, INVALID, "Empty array detected");
+    const bool broadcast = in->shape[0] == 1 && out->shape[0] > 1;
+    NOA_CHECK((in->shape[0] == out->shape[0] || broadcast) && in->shape[1] == out->shape[1] &&
+              in->shape[2] == out->shape[2] && in->shape[3] == out->shape[3], INVALID,
+              "Cannot broadcast an array of shape %s into an array of shape %s", shape_str(in->shape).c_str(),
+              shape_str(out->shape).c_str());
+    NOA_CHECK(in->data == out->data || !overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    NOA_CHECK(device_of(in->data, nullptr) == device, INVALID, "The arrays should be on the same device");
+    GriddingDesc d;
+    d.input = to_arr4(*in);
+    if (broadcast) d.input.s[0] = 0;
+    d.output = to_arr4(*out);
+    for (int i = 0; i < 4; ++i) d.shape[i] = out->shape[i];
+    d.post = post != 0;
+    return d;
+}
+
 int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr) {
     g_last_error.clear();
     if (!args) {
@@ -520,6 +589,36 @@ int noa_hip_extract_central_slices_3d_shifted(const noa_hip_project_args* args, 
         return NOA_HIP_ERROR_INVALID_ARGUMENT;
     }
     return noa_hip::call(noa_hip::Kind::EXTRACT, args, stream, shifts);
+}
+int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                  void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    RemapDesc d;
+    try {
+        d = validate_remap(remap, input, output, shape, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) { return launch_remap(d, static_cast<hipStream_t>(stream), err); });
+}
+int noa_hip_fourier_interpolation_correction(const noa_hip_view* input, const noa_hip_view* output, int32_t interp,
+                                             int32_t post_correction, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    GriddingDesc d;
+    try {
+        d = validate_gridding(input, output, interp, post_correction, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) {
+        return launch_gridding_correction(d, static_cast<hipStream_t>(stream), err);
+    });
 }
 int noa_hip_phase_shift_2d(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
                            int32_t remap, const noa_hip_shift2* shifts, void* stream) {

@@ -65,7 +65,26 @@ struct PhaseShiftDesc {
     ShiftDesc shift;
 };
 
+// A strided 4D array (b, z, y, x strides) and the elementwise operators on spectra and images.
+struct Arr4 {
+    void* data{};
+    i64 s[4]{};
+    int dtype{};
+};
+struct RemapDesc {
+    Arr4 input, output;
+    i64 shape[4]{};          // logical BDHW
+    bool src_centered{}, dst_centered{};
+};
+struct GriddingDesc {
+    Arr4 input, output;      // input batch stride 0 when broadcast
+    i64 shape[4]{};          // BDHW of the output
+    bool post{};
+};
+
 // Launchers: return 0 or a NOA_HIP_ERROR_* code, with a message in `err`.
+int launch_remap(const RemapDesc& d, hipStream_t stream, std::string& err);
+int launch_gridding_correction(const GriddingDesc& d, hipStream_t stream, std::string& err);
 int launch_phase_shift(const PhaseShiftDesc& d, hipStream_t stream, std::string& err);
 int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err);

@@ -585,6 +585,39 @@ def phase_shift_2d(remap, input, output, shape, shifts, fftfreq_cutoff: float = 
         raise NoaError(last_error())
 
 
+def remap(remap, input, output, shape) -> None:
+    """noa::fft::remap (fft/Remap.hpp:205-340) between the half-to-half layouts h2h, h2hc, hc2h, hc2hc of rfft
+    arrays (B, D, H, W//2+1): the D and H axes are fftshifted or ifftshifted."""
+    shape = _shape4(shape, "shape")
+    if not isinstance(output, torch.Tensor) or output.device.type != "cuda":
+        raise NoaError("remap: the output should be a device tensor")
+    keep: list = []
+    in_v = _view(input, keep, abi.F32, abi.C32, False, "input")
+    out_v = _view(output, keep, abi.F32, abi.C32, False, "output")
+    c_shape = (ctypes.c_int64 * 4)(*shape)
+    stream = torch.cuda.current_stream(output.device).cuda_stream
+    status = lib().noa_hip_remap(layout(remap), ctypes.byref(in_v), ctypes.byref(out_v), c_shape,
+                                 ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
+
+
+def fourier_interpolation_correction(input, output, interp: Interp = Interp.LINEAR,
+                                     post_correction: bool = True) -> None:
+    """noa::xform::fourier_interpolation_correction (FourierProject.hpp:2062-2087): divides (post) or multiplies
+    (pre) a real-space image or volume by linear interpolation's transfer function sinc^2(pi r)."""
+    if not isinstance(output, torch.Tensor) or output.device.type != "cuda":
+        raise NoaError("fourier_interpolation_correction: the output should be a device tensor")
+    keep: list = []
+    in_v = _view(input, keep, abi.F32, abi.C32, False, "input")
+    out_v = _view(output, keep, abi.F32, abi.C32, False, "output")
+    stream = torch.cuda.current_stream(output.device).cuda_stream
+    status = lib().noa_hip_fourier_interpolation_correction(ctypes.byref(in_v), ctypes.byref(out_v), int(interp),
+                                                            int(bool(post_correction)), ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
+
+
 def rfft_shape(shape: Sequence[int]) -> tuple[int, int, int, int]:
     """Shape4::rfft (src/noa/runtime/core/Shape.hpp:380): (B, D, H, W//2+1)."""
     b, d, h, w = _shape4(shape, "shape")

@@ -958,6 +958,52 @@ int phase_shift_2d(const V* input, V* output, i64 b, i64 h, i64 w, int remap, co
     return NOA_HIP_OK;
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// noa::xform::fourier_interpolation_correction (src/noa/xform/FourierProject.hpp:1008-1061, :2062-2087): real-space
+// correction of linear interpolation, x sinc^2(pi |r|) (pre) or / sinc^2 (post), r = (index - shape/2) / shape with
+// half = 0 along dimensions of size 1. Contiguous (B, D, H, W) arrays, the input broadcast over the batch when
+// its batch is 1.
+// ------------------------------------------------------------------------------------------------------------
+template<class In, class Out, class Coord>
+void gridding_correction(const In* input, Out* output, i64 in_batch, const i64 shape[4], bool post) {
+    const Coord fs[3] = {static_cast<Coord>(shape[1]), static_cast<Coord>(shape[2]), static_cast<Coord>(shape[3])};
+    Coord half[3];
+    for (int i = 0; i < 3; ++i) half[i] = fs[i] / 2 * static_cast<Coord>(shape[i + 1] != 1);
+    const i64 n3 = shape[1] * shape[2] * shape[3];
+    for (i64 b = 0; b < shape[0]; ++b)
+        for (i64 j = 0; j < shape[1]; ++j)
+            for (i64 k = 0; k < shape[2]; ++k)
+                for (i64 l = 0; l < shape[3]; ++l) {
+                    Coord d[3] = {static_cast<Coord>(j), static_cast<Coord>(k), static_cast<Coord>(l)};
+                    for (int i = 0; i < 3; ++i) { d[i] -= half[i]; d[i] /= fs[i]; }
+                    const Coord PI = static_cast<Coord>(3.141592653589793238462643383279502884);
+                    const Coord radius = std::sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+                    const Coord x = PI * radius;
+                    const Coord sinc = x == 0 ? Coord(1) : std::sin(x) / x;
+                    const In sinc2 = static_cast<In>(sinc * sinc);
+                    const i64 e = (j * shape[2] + k) * shape[3] + l;
+                    const In value = input[(in_batch == 1 ? 0 : b) * n3 + e];
+                    output[b * n3 + e] = static_cast<Out>(post ? value / sinc2 : value * sinc2);
+                }
+}
+
+// nf::remap for the half-to-half rfft layouts (fft/Remap.hpp, fft/core/Frequency.hpp:198-225): out[b, remap(z, y),
+// x] = in[b, z, y, x], the z and y indices fftshifted (non-centred -> centred) or ifftshifted. Contiguous arrays.
+template<class T>
+void remap_hx2hx(const T* input, T* output, const i64 shape[4], int remap) {
+    const bool src_c = (remap & 4) != 0, dst_c = (remap & 8) != 0;
+    const i64 wr = shape[3] / 2 + 1;
+    for (i64 b = 0; b < shape[0]; ++b)
+        for (i64 z = 0; z < shape[1]; ++z) {
+            const i64 oz = frequency2index(dst_c, index2frequency(src_c, z, shape[1]), shape[1]);
+            for (i64 y = 0; y < shape[2]; ++y) {
+                const i64 oy = frequency2index(dst_c, index2frequency(src_c, y, shape[2]), shape[2]);
+                for (i64 x = 0; x < wr; ++x)
+                    output[((b * shape[1] + oz) * shape[2] + oy) * wr + x] = input[((b * shape[1] + z) * shape[2] + y) * wr + x];
+            }
+        }
+}
+
 } // namespace oracle
 
 extern "C" {
@@ -987,6 +1033,30 @@ int oracle_phase_shift_2d(const void* input, void* output, int c64, long long b,
     }
     if (shifts_f64) return phase_shift_2d(static_cast<const Cx<float>*>(input), static_cast<Cx<float>*>(output), b, h, w, remap, static_cast<const double*>(shifts), batched != 0, cutoff);
     return phase_shift_2d(static_cast<const Cx<float>*>(input), static_cast<Cx<float>*>(output), b, h, w, remap, static_cast<const float*>(shifts), batched != 0, cutoff);
+}
+
+/* fourier_interpolation_correction on contiguous (B, D, H, W) real arrays; f32 or f64 in and out (FP64 math if
+ * either is f64). */
+int oracle_fourier_interpolation_correction(const void* input, int in_f64, long long in_batch, void* output,
+                                            int out_f64, const long long shape[4], int post) {
+    using namespace oracle;
+    const i64 sh[4] = {shape[0], shape[1], shape[2], shape[3]};
+    if (in_f64 && out_f64) gridding_correction<double, double, double>(static_cast<const double*>(input), static_cast<double*>(output), in_batch, sh, post != 0);
+    else if (in_f64) gridding_correction<double, float, double>(static_cast<const double*>(input), static_cast<float*>(output), in_batch, sh, post != 0);
+    else if (out_f64) gridding_correction<float, double, double>(static_cast<const float*>(input), static_cast<double*>(output), in_batch, sh, post != 0);
+    else gridding_correction<float, float, float>(static_cast<const float*>(input), static_cast<float*>(output), in_batch, sh, post != 0);
+    return 0;
+}
+
+/* nf::remap between half-to-half layouts on contiguous rfft arrays of `elem_bytes` 4, 8 or 16. */
+int oracle_remap_hx2hx(const void* input, void* output, int elem_bytes, const long long shape[4], int remap) {
+    using namespace oracle;
+    const i64 sh[4] = {shape[0], shape[1], shape[2], shape[3]};
+    if (elem_bytes == 4) remap_hx2hx(static_cast<const float*>(input), static_cast<float*>(output), sh, remap);
+    else if (elem_bytes == 8) remap_hx2hx(static_cast<const double*>(input), static_cast<double*>(output), sh, remap);
+    else if (elem_bytes == 16) remap_hx2hx(static_cast<const Cx<double>*>(input), static_cast<Cx<double>*>(output), sh, remap);
+    else return NOA_HIP_ERROR_UNSUPPORTED;
+    return 0;
 }
 
 /* Pinned helpers, exposed for the oracle's own tests. */

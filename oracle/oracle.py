@@ -288,3 +288,27 @@ def phase_shift_2d(remap, input, output, shape, shifts, fftfreq_cutoff=1.0):
                 int(sh.dtype == np.float64), int(batched), ctypes.c_double(fftfreq_cutoff))
     if status != 0:
         raise OracleError(lib().oracle_last_error().decode())
+
+
+def fourier_interpolation_correction(input, output, post_correction=True):
+    """noa::xform::fourier_interpolation_correction (FourierProject.hpp:2062-2087), LINEAR, on contiguous real
+    (B, D, H, W) arrays; the input may have a batch of 1 (broadcast)."""
+    out = np.asarray(output)
+    inp = np.ascontiguousarray(input)
+    if out.dtype not in (np.float32, np.float64) or inp.dtype not in (np.float32, np.float64) or out.ndim != 4:
+        raise OracleError("fourier_interpolation_correction: real (B, D, H, W) arrays expected")
+    shape = (ctypes.c_longlong * 4)(*out.shape)
+    lib().oracle_fourier_interpolation_correction(
+        ctypes.c_void_p(inp.ctypes.data), int(inp.dtype == np.float64), ctypes.c_longlong(inp.shape[0]),
+        ctypes.c_void_p(out.ctypes.data), int(out.dtype == np.float64), shape, int(bool(post_correction)))
+
+
+def remap_hx2hx(remap, input, output, shape):
+    """nf::remap between h2h, h2hc, hc2h, hc2hc on contiguous rfft arrays (B, D, H, W//2+1)."""
+    out = np.asarray(output)
+    inp = np.ascontiguousarray(input)
+    sh = (ctypes.c_longlong * 4)(*shape)
+    status = lib().oracle_remap_hx2hx(ctypes.c_void_p(inp.ctypes.data), ctypes.c_void_p(out.ctypes.data),
+                                      inp.itemsize, sh, LAYOUTS[remap.lower()] if isinstance(remap, str) else remap)
+    if status != 0:
+        raise OracleError("remap_hx2hx: unsupported element size")

@@ -516,3 +516,41 @@ def test_phase_shift_cutoff_remap_and_special_cases():
     np.testing.assert_array_equal(got[0, 0][outside], spec[0, 0][outside])
     ph = np.exp(-2j * np.pi * (2.5 * fy + 1.5 * fx))
     assert rel_l2(got[0, 0][~outside], (spec[0, 0] * ph)[~outside]) < 1e-12
+
+
+# ------------------------------------------------------------------------------------------------ gridding / remap
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 1, 20, 15), (1, 9, 8, 7)])
+@pytest.mark.parametrize("post", [True, False])
+def test_gridding_correction_is_numpy_sinc_squared(shape, post):
+    """Linear interpolation's real-space transfer function sinc^2(pi r) (numpy's normalised sinc), r the distance
+    to the centre in fractions of the box (centre at n/2, 0 along a dimension of size 1)."""
+    rng = np.random.default_rng(61)
+    x = rng.standard_normal(shape)
+    out = np.zeros_like(x)
+    O.fourier_interpolation_correction(x, out, post_correction=post)
+    axes = []
+    for n in shape[1:]:
+        half = n / 2 if n != 1 else 0.0
+        axes.append((np.arange(n) - half) / n)
+    r = np.sqrt(sum(np.meshgrid(*[a ** 2 for a in axes], indexing="ij")))
+    s2 = np.sinc(r) ** 2
+    want = x / s2 if post else x * s2
+    assert rel_l2(out, want) < 1e-13
+    # f32 in, f32 math.
+    out32 = np.zeros(shape, np.float32)
+    O.fourier_interpolation_correction(x.astype(np.float32), out32, post_correction=post)
+    assert rel_l2(out32, want) < 1e-6
+
+
+@pytest.mark.parametrize("remap, fn", [("h2hc", np.fft.fftshift), ("hc2h", np.fft.ifftshift)])
+@pytest.mark.parametrize("shape", [(2, 8, 10, 10), (1, 9, 7, 12), (3, 1, 5, 6)])
+def test_remap_is_numpy_fftshift_on_the_full_axes(remap, fn, shape):
+    rng = np.random.default_rng(62)
+    rshape = (shape[0], shape[1], shape[2], shape[3] // 2 + 1)
+    x = (rng.standard_normal(rshape) + 1j * rng.standard_normal(rshape)).astype(np.complex64)
+    out = np.zeros_like(x)
+    O.remap_hx2hx(remap, x, out, shape)
+    np.testing.assert_array_equal(out, fn(x, axes=(1, 2)))
+    back = np.zeros_like(x)
+    O.remap_hx2hx("hc2h" if remap == "h2hc" else "h2hc", out, back, shape)
+    np.testing.assert_array_equal(back, x)
