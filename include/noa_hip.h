@@ -152,8 +152,37 @@ typedef struct noa_hip_shift2 {
     double fftfreq_cutoff;
 } noa_hip_shift2;
 
+/* Arguments of noa::xform::insert_and_extract_central_slices_3d (src/noa/xform/FourierProject.hpp:1916-2049,
+ * options :1916-1946): the input slices are inserted into a virtual volume, which is sampled at the frequencies
+ * of the output slices without ever being stored. */
+typedef struct noa_hip_insert_extract_args {
+    noa_hip_view input;                 /* (B_in, 1, H, W/2+1) slices to insert, or a constant value */
+    noa_hip_view input_weight;          /* optional real weights of the input slices (array or value) */
+    noa_hip_view output;                /* (B_out, 1, H', W'/2+1) extracted slices */
+    noa_hip_view output_weight;         /* optional real weights of the extracted slices */
+    int64_t input_shape[4];             /* logical BDHW of the input slices */
+    int64_t output_shape[4];            /* logical BDHW of the output slices */
+    int32_t remap;                      /* input layout -> output layout: H2H, HC2H, H2HC or HC2HC */
+    int32_t interp;                     /* NOA_HIP_INTERP_*: interpolation of the input slices */
+    noa_hip_transform input_scaling;    /* fwd Mat22 scaling of the inserted slices; optional */
+    noa_hip_transform input_rotation;   /* inv rotations of the inserted slices (B_in); required */
+    noa_hip_transform output_scaling;   /* inv Mat22 scaling of the extracted slices; optional */
+    noa_hip_transform output_rotation;  /* fwd rotations of the extracted slices (B_out); required */
+    double input_fftfreq_sinc;          /* input_windowed_sinc, along the normal of each input slice; -1 = default */
+    double input_fftfreq_blackman;
+    double w_fftfreq_sinc;              /* w_windowed_sinc, along the depth of the virtual volume; -1 = default */
+    double w_fftfreq_blackman;
+    double fftfreq_cutoff;              /* cutoff of the virtual volume; default 0.5 */
+    double ews_radius[2];               /* HW Ewald-sphere radius in 1/pixels; {0,0} = projections */
+    int32_t add_to_output;              /* add to the output slices and weights instead of overwriting them */
+    int32_t correct_weights;            /* divide the values by max(1, |weight|) (3D only, not with add_to_output) */
+} noa_hip_insert_extract_args;
+
 /* Fill `args` with noa's defaults (cutoff 0.5, sinc/blackman -1, LINEAR, H2H, no scaling). */
 void noa_hip_project_args_init(noa_hip_project_args* args);
+
+/* Fill `args` with the defaults of InsertAndExtractCentralSlicesOptions (FourierProject.hpp:1916-1946). */
+void noa_hip_insert_extract_args_init(noa_hip_insert_extract_args* args);
 
 /* Library version and ABI version. */
 int noa_hip_abi_version(void);
@@ -169,6 +198,12 @@ int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* str
 
 /* Replaces the GPU branch of noa::xform::extract_central_slices_3d (FourierProject.hpp:1869-1914). */
 int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream);
+
+/* Replaces the GPU branch of noa::xform::insert_and_extract_central_slices_3d (FourierProject.hpp:1916-2049,
+ * launcher :1489-1598, operator FourierInsertExtract :727-1006). Without a w window (w_fftfreq_blackman giving a
+ * window of one sample) the output slices are OVERWRITTEN (0 beyond the cutoff) or, with add_to_output, added to;
+ * with a w window they are zeroed first unless add_to_output, then the windowed sum over w is added. */
+int noa_hip_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_args* args, void* stream);
 
 /* Replaces the GPU branch of noa::signal::phase_shift_2d<REMAP> (src/noa/signal/PhaseShift.hpp:215-289, operator
  * :78-139) for the rfft layouts: output[b, remap(y, x)] = input[b, y, x] * exp(-2 pi i shift_b . f(y, x)), f the

@@ -91,6 +91,44 @@ def default_args() -> ProjectArgs:
     return a
 
 
+class InsertExtractArgs(ctypes.Structure):
+    _fields_ = [
+        ("input", View),
+        ("input_weight", View),
+        ("output", View),
+        ("output_weight", View),
+        ("input_shape", c_int64 * 4),
+        ("output_shape", c_int64 * 4),
+        ("remap", c_int32),
+        ("interp", c_int32),
+        ("input_scaling", Transform),
+        ("input_rotation", Transform),
+        ("output_scaling", Transform),
+        ("output_rotation", Transform),
+        ("input_fftfreq_sinc", c_double),
+        ("input_fftfreq_blackman", c_double),
+        ("w_fftfreq_sinc", c_double),
+        ("w_fftfreq_blackman", c_double),
+        ("fftfreq_cutoff", c_double),
+        ("ews_radius", c_double * 2),
+        ("add_to_output", c_int32),
+        ("correct_weights", c_int32),
+    ]
+
+
+def default_insert_extract_args() -> InsertExtractArgs:
+    """InsertAndExtractCentralSlicesOptions' defaults, as noa_hip_insert_extract_args_init."""
+    a = InsertExtractArgs()
+    a.remap = H2H
+    a.interp = 1  # LINEAR
+    a.input_fftfreq_sinc = -1.0
+    a.input_fftfreq_blackman = -1.0
+    a.w_fftfreq_sinc = -1.0
+    a.w_fftfreq_blackman = -1.0
+    a.fftfreq_cutoff = 0.5
+    return a
+
+
 class Shift2(ctypes.Structure):
     _fields_ = [
         ("data", c_void_p),
@@ -122,6 +160,11 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
                                            c_int32, S, c_void_p]
     lib.noa_hip_phase_shift_2d.restype = ctypes.c_int
     V = ctypes.POINTER(View)
+    IE = ctypes.POINTER(InsertExtractArgs)
+    lib.noa_hip_insert_and_extract_central_slices_3d.argtypes = [IE, c_void_p]
+    lib.noa_hip_insert_and_extract_central_slices_3d.restype = ctypes.c_int
+    lib.noa_hip_insert_extract_args_init.argtypes = [IE]
+    lib.noa_hip_insert_extract_args_init.restype = None
     lib.noa_hip_remap.argtypes = [c_int32, V, V, ctypes.POINTER(c_int64), c_void_p]
     lib.noa_hip_remap.restype = ctypes.c_int
     lib.noa_hip_fourier_interpolation_correction.argtypes = [V, V, c_int32, c_int32, c_void_p]

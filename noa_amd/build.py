@@ -34,7 +34,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "phase_shift.hip", "spectrum.hip"]
+SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "insert_extract.hip", "phase_shift.hip", "spectrum.hip"]
 
 
 def hipcc() -> str:

@@ -511,6 +511,191 @@ GriddingDesc validate_gridding(const noa_hip_view* in, const noa_hip_view* out, 
     return d;
 }
 
+// insert_and_extract_central_slices_3d: fourier_projection_check_parameters<INSERT_EXTRACT> (:1095-1185) and the
+// public function's own check of correct_weights (:2011-2018).
+InsertExtractDesc validate_insert_extract(const noa_hip_insert_extract_args& a, int& device) {
+    // Output slices (:1109-1113).
+    NOA_CHECK(a.output.data != nullptr && !a.output.is_value, INVALID, "Empty array detected");
+    check_view_common(a.output, "output");
+    int64_t expected[4];
+    rfft_shape(a.output_shape, expected);
+    NOA_CHECK(same_shape(a.output.shape, expected), INVALID,
+              "The shape of the rfft output does not match the expected shape. Got output:shape=%s and expected:shape=%s",
+              shape_str(a.output.shape).c_str(), shape_str(expected).c_str());
+    // Inputs (:1115-1136): arrays, textures (as their buffer) or values.
+    auto check_input = [&](const noa_hip_view& v, const char* name, bool is_weight) {
+        if (is_empty_view(v)) return;
+        if (v.is_value) {
+            if (is_weight) NOA_CHECK(is_real(v.dtype), INVALID, "%s: weights should be real", name);
+            return;
+        }
+        check_view_common(v, name);
+        NOA_CHECK(!overlap(v, a.output), INVALID, "Input and output arrays should not overlap");
+        if (!is_empty_view(a.output_weight) && a.output_weight.data)
+            NOA_CHECK(!overlap(v, a.output_weight), INVALID, "Input and output arrays should not overlap");
+        int64_t exp[4];
+        rfft_shape(a.input_shape, exp);
+        NOA_CHECK(same_shape(v.shape, exp), INVALID,
+                  "The shape of the rfft does not match the expected shape. Got %s:shape=%s and shape=%s", name,
+                  shape_str(v.shape).c_str(), shape_str(exp).c_str());
+        if (is_weight) NOA_CHECK(is_real(v.dtype), INVALID, "%s: weights should be real", name);
+    };
+    NOA_CHECK(!is_empty_view(a.input), INVALID, "input: Empty array detected");
+    check_input(a.input, "input", false);
+    check_input(a.input_weight, "input_weight", true);
+    // Output weights (:1138-1147).
+    if (!is_empty_view(a.output_weight)) {
+        NOA_CHECK(!a.output_weight.is_value && a.output_weight.data, INVALID, "Empty array detected");
+        check_view_common(a.output_weight, "output_weight");
+        NOA_CHECK(!overlap(a.output_weight, a.output), INVALID, "Output arrays should not overlap");
+        NOA_CHECK(same_shape(a.output_weight.shape, expected), INVALID,
+                  "The shape of the rfft does not match the expected shape. Got output_weight:shape=%s and shape=%s",
+                  shape_str(a.output_weight.shape).c_str(), shape_str(expected).c_str());
+        NOA_CHECK(is_real(a.output_weight.dtype), INVALID, "output_weight: weights should be real");
+    }
+    // 2D slices on both sides (:1160-1164).
+    for (int i = 0; i < 4; ++i)
+        NOA_CHECK(a.input_shape[i] > 0 && a.output_shape[i] > 0, INVALID, "Empty array detected");
+    NOA_CHECK(a.input_shape[1] == 1 && a.output_shape[1] == 1, INVALID,
+              "2d slices are expected but got shape input:shape=%s and output:shape=%s",
+              shape_str(a.input_shape).c_str(), shape_str(a.output_shape).c_str());
+    NOA_CHECK(a.remap == NOA_HIP_H2H || a.remap == NOA_HIP_HC2H || a.remap == NOA_HIP_H2HC || a.remap == NOA_HIP_HC2HC,
+              INVALID, "The remap should be H2H, HC2H, H2HC or HC2HC, but got %d", a.remap);
+    const int interp = a.interp >= 100 ? a.interp - 100 : a.interp;
+    NOA_CHECK(interp >= 0 && interp <= 6, INVALID, "Invalid interpolation method %d", a.interp);
+    NOA_CHECK(interp == NOA_HIP_INTERP_LINEAR || interp == NOA_HIP_INTERP_NEAREST, UNSUPPORTED,
+              "Interpolation method %d is not implemented by this library (NEAREST and LINEAR are)", a.interp);
+    const int in = a.input.dtype, out = a.output.dtype;
+    NOA_CHECK((is_complex(in) && is_complex(out)) || (is_real(in) && is_real(out)) || (is_complex(in) && is_real(out)),
+              INVALID, "Invalid input/output types: %d -> %d", in, out);
+    NOA_CHECK(in != NOA_HIP_C16 && in != NOA_HIP_CBF16, UNSUPPORTED,
+              "FP16/BF16 storage is only supported for the extracted volume");
+
+    // Transforms (:1166-1183): the input ones have one element per input slice, the output ones per output slice.
+    auto check_transform = [&](const noa_hip_transform& t, const char* name, bool rotation, int64_t required) {
+        if (rotation) {
+            NOA_CHECK(t.kind >= NOA_HIP_MAT33_F32 && t.kind <= NOA_HIP_QUAT_F64, INVALID,
+                      "%s should be Mat33 or Quaternion (f32 or f64), got kind=%d", name, t.kind);
+        } else if (t.kind == NOA_HIP_XFORM_NONE) {
+            return;
+        } else {
+            NOA_CHECK(t.kind == NOA_HIP_MAT22_F32 || t.kind == NOA_HIP_MAT22_F64, INVALID,
+                      "%s should be Mat22 (f32 or f64), got kind=%d", name, t.kind);
+        }
+        NOA_CHECK(t.data != nullptr, INVALID, "%s should not be empty", name);
+        if (t.batched) {
+            NOA_CHECK(t.count == required, INVALID,
+                      "%s should be a contiguous vector with n_slices=%lld elements, but got %lld elements", name,
+                      (long long)required, (long long)t.count);
+            NOA_CHECK(reinterpret_cast<uintptr_t>(t.data) % 16 == 0 || t.kind == NOA_HIP_MAT33_F64, INVALID,
+                      "%s should be 16-byte aligned", name);
+        }
+    };
+    check_transform(a.input_scaling, "input_scaling", false, a.input_shape[0]);
+    check_transform(a.input_rotation, "input_rotation", true, a.input_shape[0]);
+    check_transform(a.output_scaling, "output_scaling", false, a.output_shape[0]);
+    check_transform(a.output_rotation, "output_rotation", true, a.output_shape[0]);
+    auto rot64 = [](const noa_hip_transform& t) { return t.kind == NOA_HIP_MAT33_F64 || t.kind == NOA_HIP_QUAT_F64; };
+    const bool coord64 = rot64(a.input_rotation);
+    NOA_CHECK(rot64(a.output_rotation) == coord64, UNSUPPORTED,
+              "input_rotation and output_rotation should have the same precision");
+    for (const noa_hip_transform* t : {&a.input_scaling, &a.output_scaling})
+        if (t->kind != NOA_HIP_XFORM_NONE)
+            NOA_CHECK((t->kind == NOA_HIP_MAT22_F64) == coord64, INVALID,
+                      "The scaling and rotation should have the same precision");
+
+    // correct_weights (:2011-2018): the w window size in the output rotation's precision, from the option as given.
+    const double volume_z = static_cast<double>(std::min(a.output_shape[2], a.output_shape[3]));
+    const double fb_raw = coord64 ? a.w_fftfreq_blackman : static_cast<double>(static_cast<float>(a.w_fftfreq_blackman));
+    const double z_c = coord64 ? volume_z : static_cast<double>(static_cast<float>(volume_z));
+    const int64_t w_raw = blackman_window_size(fb_raw, z_c);
+    NOA_CHECK(!a.correct_weights || (!a.add_to_output && w_raw == 1), INVALID,
+              "options.correct_weights=true is not compatible with options.add_to_output=true and "
+              "options.w_windowed_sinc.fftfreq_blackman=%g (=%lld pixels)", a.w_fftfreq_blackman, (long long)w_raw);
+
+    // 32-bit offsets within a slice, weights of the values' precision.
+    auto check_i32 = [&](const noa_hip_view& v, const char* name) {
+        if (is_empty_view(v) || v.is_value) return;
+        NOA_CHECK(fits_i32(v, 2), UNSUPPORTED, "%s: offsets within a slice exceed 32-bit indexing", name);
+    };
+    check_i32(a.input, "input");
+    check_i32(a.input_weight, "input_weight");
+    check_i32(a.output, "output");
+    check_i32(a.output_weight, "output_weight");
+    const bool out64 = out == NOA_HIP_F64 || out == NOA_HIP_C64;
+    const bool in64 = in == NOA_HIP_F64 || in == NOA_HIP_C64;
+    NOA_CHECK(in64 == out64, UNSUPPORTED, "input and output should have the same precision");
+    const int wtype = out64 ? NOA_HIP_F64 : NOA_HIP_F32;
+    if (!is_empty_view(a.input_weight)) NOA_CHECK(a.input_weight.dtype == wtype, UNSUPPORTED, "input_weight: expected %s weights", out64 ? "f64" : "f32");
+    if (!is_empty_view(a.output_weight)) NOA_CHECK(a.output_weight.dtype == wtype, UNSUPPORTED, "output_weight: expected %s weights", out64 ? "f64" : "f32");
+
+    // Device checks last.
+    device = device_of(a.output.data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    auto check_device = [&](const noa_hip_view& v, const char* name) {
+        if (is_empty_view(v) || v.is_value) return;
+        const int dev = device_of(v.data, nullptr);
+        NOA_CHECK(dev == device, INVALID, "The arrays should be on the same device, but got %s:device=%d and output:device=%d",
+                  name, dev, device);
+    };
+    check_device(a.input, "input");
+    check_device(a.input_weight, "input_weight");
+    check_device(a.output_weight, "output_weight");
+    for (const auto& [t, name] : {std::pair{&a.input_scaling, "input_scaling"}, std::pair{&a.input_rotation, "input_rotation"},
+                                  std::pair{&a.output_scaling, "output_scaling"}, std::pair{&a.output_rotation, "output_rotation"}})
+        if (t->kind != NOA_HIP_XFORM_NONE && t->batched)
+            NOA_CHECK(device_of(t->data, nullptr) == device, INVALID, "%s should be on the compute device", name);
+
+    // Flatten.
+    InsertExtractDesc d;
+    d.input = to_arr(a.input, false);
+    d.input_weight = to_arr(a.input_weight, false);
+    d.output = to_arr(a.output, false);
+    d.output_weight = to_arr(a.output_weight, false);
+    if (is_empty_view(a.input_weight)) { d.input_weight.data = nullptr; d.input_weight.is_value = false; }
+    if (is_empty_view(a.output_weight)) d.output_weight.data = nullptr;
+    for (int i = 0; i < 4; ++i) { d.in_shape[i] = a.input_shape[i]; d.out_shape[i] = a.output_shape[i]; }
+    d.src_centered = (a.remap & 4) != 0;
+    d.dst_centered = (a.remap & 8) != 0;
+    d.interp = interp;
+    d.rot_in = to_xform(a.input_rotation);
+    d.scale_in = to_xform(a.input_scaling);
+    d.rot_out = to_xform(a.output_rotation);
+    d.scale_out = to_xform(a.output_scaling);
+    auto scaled = [](Xform& x) {   // a single identity Mat22 is no scaling (:1234-1244)
+        bool has = x.kind != NOA_HIP_XFORM_NONE;
+        if (has && !x.batched) {
+            const double* s = x.single;
+            has = !(s[0] == 1 && s[1] == 0 && s[2] == 0 && s[3] == 1);
+        }
+        if (!has) x.kind = NOA_HIP_XFORM_NONE;
+        return has;
+    };
+    const bool has_scale_in = scaled(d.scale_in), has_scale_out = scaled(d.scale_out);
+    d.ews_radius[0] = a.ews_radius[0];
+    d.ews_radius[1] = a.ews_radius[1];
+    d.general = a.ews_radius[0] != 0 || a.ews_radius[1] != 0 || has_scale_in || has_scale_out;
+    d.cutoff = a.fftfreq_cutoff;
+    d.in_sinc = a.input_fftfreq_sinc;
+    d.in_blackman = a.input_fftfreq_blackman;
+    d.w_sinc = a.w_fftfreq_sinc;
+    d.w_blackman = a.w_fftfreq_blackman;
+    // The operator's window (:842-844): the option clamped to 1 / volume_z, in the coordinate precision.
+    if (coord64) {
+        const double fb = std::max(a.w_fftfreq_blackman, 1 / volume_z);
+        d.w_size = blackman_window_size(fb, volume_z);
+    } else {
+        const float zf = static_cast<float>(volume_z);
+        const float fb = std::max(static_cast<float>(a.w_fftfreq_blackman), 1 / zf);
+        d.w_size = blackman_window_size(static_cast<double>(fb), static_cast<double>(zf));
+    }
+    d.add = a.add_to_output != 0;
+    d.correct = a.correct_weights != 0;
+    NOA_CHECK(d.w_size == 1 || !d.correct, INVALID, "correct_weights is not compatible with a w window");
+    d.coord_f64 = coord64;
+    return d;
+}
+
 int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr) {
     g_last_error.clear();
     if (!args) {
@@ -569,6 +754,18 @@ void noa_hip_project_args_init(noa_hip_project_args* args) {
     args->fftfreq_blackman = -1;
 }
 
+void noa_hip_insert_extract_args_init(noa_hip_insert_extract_args* args) {
+    if (!args) return;
+    std::memset(args, 0, sizeof(*args));
+    args->remap = NOA_HIP_H2H;
+    args->interp = NOA_HIP_INTERP_LINEAR;
+    args->input_fftfreq_sinc = -1;
+    args->input_fftfreq_blackman = -1;
+    args->w_fftfreq_sinc = -1;
+    args->w_fftfreq_blackman = -1;
+    args->fftfreq_cutoff = 0.5;
+}
+
 int noa_hip_abi_version(void) { return NOA_HIP_ABI_VERSION; }
 
 const char* noa_hip_last_error(void) { return noa_hip::g_last_error.c_str(); }
@@ -589,6 +786,25 @@ int noa_hip_extract_central_slices_3d_shifted(const noa_hip_project_args* args, 
         return NOA_HIP_ERROR_INVALID_ARGUMENT;
     }
     return noa_hip::call(noa_hip::Kind::EXTRACT, args, stream, shifts);
+}
+int noa_hip_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_args* args, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!args) {
+        g_last_error = "null arguments";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    int device = -1;
+    InsertExtractDesc d;
+    try {
+        d = validate_insert_extract(*args, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) {
+        return launch_insert_extract(d, static_cast<hipStream_t>(stream), err);
+    });
 }
 int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
                   void* stream) {

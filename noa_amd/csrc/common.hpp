@@ -77,6 +77,18 @@ template<class Math, class Storage> __device__ __forceinline__ Math load_as(cons
     }
 }
 
+// The value `v` of lane `lane`, for every lane of the wave (float or double).
+template<class C> __device__ __forceinline__ C bcast(C v, i32 lane) {
+    if constexpr (sizeof(C) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const i32 lo = __builtin_amdgcn_readlane(static_cast<i32>(b & 0xffffffffLL), lane);
+        const i32 hi = __builtin_amdgcn_readlane(static_cast<i32>(b >> 32), lane);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Index <-> frequency (fft/core/Frequency.hpp:80-87, :115-122).
 // ---------------------------------------------------------------------------------------------------------

@@ -189,15 +189,6 @@ XformArg<C> to_arg(const Xform& x) {
     return a;
 }
 
-// noa's blackman_window_size (FourierProject.hpp:114-137).
-i64 blackman_window_size(double fftfreq_blackman, double spectrum_size) {
-    double samples = spectrum_size * fftfreq_blackman;
-    if (std::abs(samples - 1.) <= 1e-6) samples = std::round(samples);
-    else samples = std::ceil(samples);
-    const i64 n = std::max<i64>(1, static_cast<i64>(samples));
-    return 2 * n + 1 - 2;
-}
-
 template<class C, class S, class M, class Out, class W>
 int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     ExtractParams<C> p{};

@@ -192,16 +192,6 @@ __device__ __forceinline__ i32 wave_inclusive_sum(i32 x) {
     return x;
 }
 
-template<class C> __device__ __forceinline__ C bcast(C v, i32 lane) {
-    if constexpr (sizeof(C) == 4) {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-    } else {
-        const long long b = __double_as_longlong(v);
-        const i32 lo = __builtin_amdgcn_readlane(static_cast<i32>(b & 0xffffffffLL), lane);
-        const i32 hi = __builtin_amdgcn_readlane(static_cast<i32>(b >> 32), lane);
-        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-    }
-}
 
 template<class C, class In, class Out, class W, int INTERP, int NT>
 __global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C, In, W> p) {

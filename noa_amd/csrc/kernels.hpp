@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
+#include <cmath>
 #include <string>
 
 #include "../../include/noa_hip.h"
@@ -56,6 +58,21 @@ struct ProjectDesc {
     ShiftDesc shift;                     // extraction only: phase shift of the output slices (fused)
 };
 
+// Extraction from a virtual volume of inserted central slices (insert_and_extract_central_slices_3d).
+struct InsertExtractDesc {
+    Arr input, input_weight, output, output_weight;   // slices: (batch, y, x) strides
+    i64 in_shape[4]{}, out_shape[4]{};                 // logical BDHW
+    bool src_centered{}, dst_centered{};
+    int interp{};
+    Xform rot_in, scale_in, rot_out, scale_out;        // inv rotation + fwd scaling (insert), fwd + inv (extract)
+    bool general{};                                    // any scaling or EWS present
+    double ews_radius[2]{};
+    double cutoff{}, in_sinc{}, in_blackman{}, w_sinc{}, w_blackman{};
+    i64 w_size{};                                      // samples of the w window (1: the 3D operator)
+    bool add{}, correct{};
+    bool coord_f64{};
+};
+
 // A phase shift over (B, 1, H, W/2+1) rfft arrays.
 struct PhaseShiftDesc {
     Arr input, output;    // (batch, y, x) strides; input.data == nullptr: write the phase shifts
@@ -82,6 +99,15 @@ struct GriddingDesc {
     bool post{};
 };
 
+// noa's blackman_window_size (FourierProject.hpp:114-137): samples of the w window, always odd.
+inline i64 blackman_window_size(double fftfreq_blackman, double spectrum_size) {
+    double samples = spectrum_size * fftfreq_blackman;
+    if (std::abs(samples - 1.) <= 1e-6) samples = std::round(samples);
+    else samples = std::ceil(samples);
+    const i64 n = std::max<i64>(1, static_cast<i64>(samples));
+    return 2 * n + 1 - 2;
+}
+
 // Launchers: return 0 or a NOA_HIP_ERROR_* code, with a message in `err`.
 int launch_remap(const RemapDesc& d, hipStream_t stream, std::string& err);
 int launch_gridding_correction(const GriddingDesc& d, hipStream_t stream, std::string& err);
@@ -89,5 +115,6 @@ int launch_phase_shift(const PhaseShiftDesc& d, hipStream_t stream, std::string&
 int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
+int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
 
 } // namespace noa_hip

@@ -102,6 +102,18 @@ class ExtractCentralSlicesOptions:
     ews_radius: Sequence[float] = (0.0, 0.0)
 
 
+@dataclass
+class InsertAndExtractCentralSlicesOptions:
+    """InsertAndExtractCentralSlicesOptions (FourierProject.hpp:1916-1946)."""
+    interp: Interp = Interp.LINEAR
+    input_windowed_sinc: WindowedSinc = field(default_factory=WindowedSinc)
+    w_windowed_sinc: WindowedSinc = field(default_factory=WindowedSinc)
+    add_to_output: bool = False
+    correct_weights: bool = False
+    fftfreq_cutoff: float = 0.5
+    ews_radius: Sequence[float] = (0.0, 0.0)
+
+
 # ---------------------------------------------------------------------------------------------------------------
 # Input forms.
 # ---------------------------------------------------------------------------------------------------------------
@@ -507,6 +519,56 @@ def extract_central_slices_3d(remap, volume, volume_weight, volume_shape, slice,
     args.fftfreq_sinc = float(opts.w_windowed_sinc.fftfreq_sinc)
     args.fftfreq_blackman = float(opts.w_windowed_sinc.fftfreq_blackman)
     _call("noa_hip_extract_central_slices_3d", args, device)
+
+
+def insert_and_extract_central_slices_3d(remap, input_slice, input_weight, input_slice_shape, output_slice,
+                                         output_weight, output_slice_shape, input_fwd_scaling=None,
+                                         input_inv_rotation=None, output_inv_scaling=None, output_fwd_rotation=None,
+                                         options: InsertAndExtractCentralSlicesOptions | None = None) -> None:
+    """Extracts 2D rfft central slices from a virtual volume made of other 2D rfft central slices.
+
+    noa::xform::insert_and_extract_central_slices_3d (FourierProject.hpp:1916-2049): the input slices (INVERSE
+    rotations, forward scaling) are inserted, with a windowed sinc along their normal, into a volume that is never
+    stored, and sampled at the output slices (FORWARD rotations, inverse scaling). The outputs are overwritten, or
+    added to with `add_to_output`; a w window wider than one sample sums the virtual volume along w (:890-924).
+    `input_slice` and `input_weight` may be constant values or Textures.
+    """
+    opts = options or InsertAndExtractCentralSlicesOptions()
+    input_slice_shape = _shape4(input_slice_shape, "input_slice_shape")
+    output_slice_shape = _shape4(output_slice_shape, "output_slice_shape")
+    keep: list = []
+    args = abi.default_insert_extract_args()
+    device = _device_of(output_slice)
+    real_t, cx_t = _value_dtypes(output_slice)
+    args.input = _view(input_slice, keep, real_t, cx_t, True, "input")
+    args.input_weight = _view(input_weight, keep, real_t, real_t, True, "input_weight")
+    args.output = _view(output_slice, keep, real_t, cx_t, False, "output")
+    args.output_weight = _view(output_weight, keep, real_t, real_t, False, "output_weight")
+    for i in range(4):
+        args.input_shape[i] = input_slice_shape[i]
+        args.output_shape[i] = output_slice_shape[i]
+    args.remap = layout(remap)
+    args.interp = int(_interp_of(input_slice, input_weight, opts.interp))
+    args.input_scaling = _transform(input_fwd_scaling, keep, input_slice_shape[0], device, False, "input_scaling")
+    args.input_rotation = _transform(input_inv_rotation, keep, input_slice_shape[0], device, True, "input_rotation")
+    args.output_scaling = _transform(output_inv_scaling, keep, output_slice_shape[0], device, False,
+                                     "output_scaling")
+    args.output_rotation = _transform(output_fwd_rotation, keep, output_slice_shape[0], device, True,
+                                      "output_rotation")
+    args.input_fftfreq_sinc = float(opts.input_windowed_sinc.fftfreq_sinc)
+    args.input_fftfreq_blackman = float(opts.input_windowed_sinc.fftfreq_blackman)
+    args.w_fftfreq_sinc = float(opts.w_windowed_sinc.fftfreq_sinc)
+    args.w_fftfreq_blackman = float(opts.w_windowed_sinc.fftfreq_blackman)
+    args.fftfreq_cutoff = float(opts.fftfreq_cutoff)
+    ews = tuple(opts.ews_radius) if isinstance(opts.ews_radius, (tuple, list)) else (opts.ews_radius,) * 2
+    args.ews_radius[0] = float(ews[0])
+    args.ews_radius[1] = float(ews[1])
+    args.add_to_output = int(bool(opts.add_to_output))
+    args.correct_weights = int(bool(opts.correct_weights))
+    stream = torch.cuda.current_stream(device).cuda_stream
+    status = lib().noa_hip_insert_and_extract_central_slices_3d(ctypes.byref(args), ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
 
 
 def _shift2(shifts, count: int, device: torch.device, keep: list, cutoff: float) -> abi.Shift2:

@@ -9,6 +9,7 @@
  *   - FourierInsertRasterize   src/noa/xform/FourierProject.hpp:161-390
  *   - FourierInsertInterpolate src/noa/xform/FourierProject.hpp:392-539
  *   - FourierExtract (3D, 4D)  src/noa/xform/FourierProject.hpp:541-725
+ *   - FourierInsertExtract     src/noa/xform/FourierProject.hpp:727-1006 (launcher :1489-1598)
  *   - windowed sinc helpers    src/noa/xform/FourierProject.hpp:97-159
  *   - slice<->grid transforms  src/noa/xform/FourierProject.hpp:38-95
  *   - interpolate_spectrum     src/noa/xform/core/Interpolation.hpp:416-526 (+ weights :64-146)
@@ -430,34 +431,45 @@ struct Geometry {
     }
 };
 
-template<class Coord>
-Geometry<Coord> make_geometry(const noa_hip_project_args& a) {
-    Geometry<Coord> g;
-    g.rot.data = static_cast<const unsigned char*>(a.rotation.data);
-    g.rot.kind = a.rotation.kind;
-    g.rot.batched = a.rotation.batched != 0;
-    g.scale.data = static_cast<const unsigned char*>(a.scaling.data);
-    g.scale.kind = a.scaling.kind;
-    g.scale.batched = a.scaling.batched != 0;
-    const bool has_ews = a.ews_radius[0] != 0 || a.ews_radius[1] != 0;
-    // fourier_project_has_scale: a single Mat22 counts only if it differs from the identity (:1234-1244).
-    bool has_scale = a.scaling.kind != NOA_HIP_XFORM_NONE;
-    if (has_scale && !a.scaling.batched) {
+// fourier_project_has_scale: a single Mat22 counts only if it differs from the identity (:1234-1244).
+inline bool has_scale(const noa_hip_transform& scaling) {
+    bool has = scaling.kind != NOA_HIP_XFORM_NONE;
+    if (has && !scaling.batched) {
         double e[4];
-        if (a.scaling.kind == NOA_HIP_MAT22_F32) { float f[4]; std::memcpy(f, a.scaling.data, 16); for (int i = 0; i < 4; ++i) e[i] = f[i]; }
-        else std::memcpy(e, a.scaling.data, 32);
-        has_scale = !(e[0] == 1 && e[1] == 0 && e[2] == 0 && e[3] == 1);
+        if (scaling.kind == NOA_HIP_MAT22_F32) { float f[4]; std::memcpy(f, scaling.data, 16); for (int i = 0; i < 4; ++i) e[i] = f[i]; }
+        else std::memcpy(e, scaling.data, 32);
+        has = !(e[0] == 1 && e[1] == 0 && e[2] == 0 && e[3] == 1);
     }
-    g.general = has_ews || has_scale;
-    if (!has_scale) g.scale.kind = NOA_HIP_XFORM_NONE;
+    return has;
+}
+
+// `general` (WrapNoEwaldAndScale<false>) is decided by the caller: EWS or any scaling of the call.
+template<class Coord>
+Geometry<Coord> make_geometry(const noa_hip_transform& rotation, const noa_hip_transform& scaling,
+                              const double ews_radius[2], bool general) {
+    Geometry<Coord> g;
+    g.rot.data = static_cast<const unsigned char*>(rotation.data);
+    g.rot.kind = rotation.kind;
+    g.rot.batched = rotation.batched != 0;
+    g.scale.data = static_cast<const unsigned char*>(scaling.data);
+    g.scale.kind = has_scale(scaling) ? scaling.kind : NOA_HIP_XFORM_NONE;
+    g.scale.batched = scaling.batched != 0;
+    g.general = general;
     if (g.general) {
-        const Coord r0 = static_cast<Coord>(a.ews_radius[0]), r1 = static_cast<Coord>(a.ews_radius[1]);
+        const Coord r0 = static_cast<Coord>(ews_radius[0]), r1 = static_cast<Coord>(ews_radius[1]);
         if (r0 != 0 || r1 != 0) {   // `ews_radius != 0 ? 1 / (2 * ews_radius) : {}` (:231)
             g.ews_inv[0] = 1 / (2 * r0);
             g.ews_inv[1] = 1 / (2 * r1);
         }
     }
     return g;
+}
+
+inline bool has_ews(const double ews_radius[2]) { return ews_radius[0] != 0 || ews_radius[1] != 0; }
+
+template<class Coord>
+Geometry<Coord> make_geometry(const noa_hip_project_args& a) {
+    return make_geometry<Coord>(a.rotation, a.scaling, a.ews_radius, has_ews(a.ews_radius) || has_scale(a.scaling));
 }
 
 inline bool remap_src_centered(int remap) { return (remap & 4) != 0; }   // is_xc2xx
@@ -740,6 +752,98 @@ struct ExtractOp {
 };
 
 // ---------------------------------------------------------------------------------------------------------
+// FourierInsertExtract (FourierProject.hpp:727-1006): extraction from a virtual volume made of central slices.
+// ---------------------------------------------------------------------------------------------------------
+template<class Coord, class In, class Out, class W>
+struct InsertExtractOp {
+    using RIn = typename real_of<In>::type;
+    using ROut = typename real_of<Out>::type;
+    Reader3<In> slices;
+    Reader3<W> wslices; bool has_wslices{};
+    Writer3<Out> out;
+    Writer3<W> wout; bool has_wout{};
+    Geometry<Coord> in_geo, out_geo;     // insert: fwd scaling + inv rotation; extract: inv scaling + fwd rotation
+    i64 n_in{}, in_shape[2]{}, out_h{};
+    Coord f_in[2]{}, f_out[2]{};
+    Coord volume_z{}, cutoff_sqd{}, in_sinc{}, in_blackman{}, w_sinc{}, w_blackman{};
+    i64 w_size{};
+    bool in_centered{}, out_centered{}, add{}, correct{};
+    int interp{};
+
+    V3<Coord> fftfreq_in_volume(i64 b, i64 y, i64 x) const {   // compute_fftfreq_in_volume_ (:927-933)
+        V2<Coord> f2;
+        f2[0] = static_cast<Coord>(index2frequency(out_centered, y, out_h)) / f_out[0];
+        f2[1] = static_cast<Coord>(x) / f_out[1];
+        return out_geo.slice2grid(f2, b);
+    }
+
+    // sample_virtual_volume_ (:935-977): every input slice within the window along its normal contributes its
+    // interpolated value (and weight, or 1) times the windowed sinc, in slice order.
+    void sample(const V3<Coord>& f, bool correct_weights, In& value, W& weight) const {
+        value = zero<In>();
+        weight = W{};
+        for (i64 i = 0; i < n_in; ++i) {
+            Coord fz;
+            V2<Coord> f2;
+            in_geo.grid2slice(f, i, fz, f2);
+            if (std::abs(fz) < in_blackman) {
+                const Coord window = windowed_sinc(fz, in_sinc, in_blackman);
+                const Coord py = f2[0] * f_in[0], px = f2[1] * f_in[1];
+                value += interpolate_spectrum_2d<In, Coord>(slices, i, py, px, in_shape[0], in_shape[1], in_centered,
+                                                            interp) * static_cast<RIn>(window);
+                if (has_wslices)
+                    weight += interpolate_spectrum_2d<W, Coord>(wslices, i, py, px, in_shape[0], in_shape[1],
+                                                                in_centered, interp) * static_cast<W>(window);
+                else
+                    weight += static_cast<W>(window);
+            }
+        }
+        if (correct_weights) {
+            const W final_weight = std::max(W(1), std::abs(weight));
+            if constexpr (is_cx<In>::value) value = In{value.re / static_cast<RIn>(final_weight), value.im / static_cast<RIn>(final_weight)};
+            else value = value / static_cast<RIn>(final_weight);
+        }
+    }
+
+    // 3D operator (:858-886).
+    void operator()(i64 b, i64 y, i64 x, bool) const {
+        const V3<Coord> f = fftfreq_in_volume(b, y, x);
+        if (dot3(f, f) > cutoff_sqd) {
+            if (!add) {
+                out(b, y, x) = zero<Out>();
+                if (has_wout) wout(b, y, x) = W{};
+            }
+            return;
+        }
+        In value;
+        W weight;
+        sample(f, correct, value, weight);
+        Out& o = out(b, y, x);
+        o = add ? o + cast_or_abs_squared<Out>(value) : cast_or_abs_squared<Out>(value);
+        if (has_wout) {
+            W& w = wout(b, y, x);
+            w = add ? w + weight : weight;
+        }
+    }
+
+    // 4D operator (:890-924): one w offset of the windowed sum along the depth of the virtual volume.
+    void operator()(i64 b, i64 w, i64 y, i64 x, bool parallel) const {
+        V3<Coord> f = fftfreq_in_volume(b, y, x);
+        const Coord offset = w_index_to_fftfreq_offset<Coord>(w, w_size, volume_z);
+        f[0] += offset;
+        if (dot3(f, f) > cutoff_sqd)
+            return;
+        In value;
+        W weight;
+        sample(f, false, value, weight);
+        const Coord conv = windowed_sinc(offset, w_sinc, w_blackman);
+        atomic_add(out(b, y, x), cast_or_abs_squared<Out>(value) * static_cast<ROut>(conv), parallel);
+        if (has_wout)
+            atomic_add(wout(b, y, x), weight * static_cast<W>(conv), parallel);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------
 // Entry points.
 // ---------------------------------------------------------------------------------------------------------
 thread_local std::string g_error;
@@ -836,6 +940,71 @@ int run_extract(const noa_hip_project_args& a, int n_threads, bool force) {
         iwise3(B, H, Wr, op, n_threads, force);
     }
     return 0;
+}
+
+// launch_insert_and_extract_central_slices_3d (:1489-1598) with the operator's constructor (:791-845).
+template<class Coord, class In, class Out, class W>
+int run_insert_extract(const noa_hip_insert_extract_args& a, int n_threads, bool force) {
+    InsertExtractOp<Coord, In, Out, W> op;
+    op.slices = reader_interp2<In>(a.input);
+    op.has_wslices = a.input_weight.data != nullptr || a.input_weight.is_value;
+    if (op.has_wslices) op.wslices = reader_interp2<W>(a.input_weight);
+    op.out = writer_slices<Out>(a.output);
+    op.has_wout = a.output_weight.data != nullptr;
+    if (op.has_wout) op.wout = writer_slices<W>(a.output_weight);
+    const bool general = has_ews(a.ews_radius) || has_scale(a.input_scaling) || has_scale(a.output_scaling);
+    op.in_geo = make_geometry<Coord>(a.input_rotation, a.input_scaling, a.ews_radius, general);
+    op.out_geo = make_geometry<Coord>(a.output_rotation, a.output_scaling, a.ews_radius, general);
+    op.n_in = a.input_shape[0];
+    op.in_shape[0] = a.input_shape[2];
+    op.in_shape[1] = a.input_shape[3];
+    op.f_in[0] = static_cast<Coord>(a.input_shape[2]);
+    op.f_in[1] = static_cast<Coord>(a.input_shape[3]);
+    op.out_h = a.output_shape[2];
+    op.f_out[0] = static_cast<Coord>(a.output_shape[2]);
+    op.f_out[1] = static_cast<Coord>(a.output_shape[3]);
+    op.cutoff_sqd = std::max(static_cast<Coord>(a.fftfreq_cutoff), Coord(0));
+    op.cutoff_sqd *= op.cutoff_sqd;
+    op.volume_z = static_cast<Coord>(std::min(a.output_shape[2], a.output_shape[3]));   // no z: smallest axis
+    op.in_sinc = std::max(static_cast<Coord>(a.input_fftfreq_sinc), 1 / op.volume_z);
+    op.in_blackman = std::max(static_cast<Coord>(a.input_fftfreq_blackman), 1 / op.volume_z);
+    op.w_sinc = std::max(static_cast<Coord>(a.w_fftfreq_sinc), 1 / op.volume_z);
+    op.w_blackman = std::max(static_cast<Coord>(a.w_fftfreq_blackman), 1 / op.volume_z);
+    op.w_size = blackman_window_size<Coord>(op.w_blackman, op.volume_z);
+    op.in_centered = remap_src_centered(a.remap);
+    op.out_centered = remap_dst_centered(a.remap);
+    op.add = a.add_to_output != 0;
+    op.correct = a.correct_weights != 0;
+    op.interp = a.interp;
+    const i64 B = a.output_shape[0], H = a.output_shape[2], Wr = a.output_shape[3] / 2 + 1;
+    if (op.w_size > 1) {
+        if (op.correct) { g_error = "oracle: correct_weights with a w window"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+        if (!op.add) {   // ewise Zero of the outputs first (:1543-1550)
+            for (i64 b = 0; b < B; ++b)
+                for (i64 y = 0; y < H; ++y)
+                    for (i64 x = 0; x < Wr; ++x) {
+                        op.out(b, y, x) = zero<Out>();
+                        if (op.has_wout) op.wout(b, y, x) = W{};
+                    }
+        }
+        iwise4(B, op.w_size, H, Wr, op, n_threads, force);
+    } else {
+        iwise3(B, H, Wr, op, n_threads, force);
+    }
+    return 0;
+}
+
+template<class Coord>
+int dispatch_insert_extract(const noa_hip_insert_extract_args& a, int n_threads, bool force) {
+    const int in = a.input.dtype, out = a.output.dtype;
+    if (in == NOA_HIP_C32 && out == NOA_HIP_C32) return run_insert_extract<Coord, Cx<float>, Cx<float>, float>(a, n_threads, force);
+    if (in == NOA_HIP_C32 && out == NOA_HIP_F32) return run_insert_extract<Coord, Cx<float>, float, float>(a, n_threads, force);
+    if (in == NOA_HIP_F32 && out == NOA_HIP_F32) return run_insert_extract<Coord, float, float, float>(a, n_threads, force);
+    if (in == NOA_HIP_C64 && out == NOA_HIP_C64) return run_insert_extract<Coord, Cx<double>, Cx<double>, double>(a, n_threads, force);
+    if (in == NOA_HIP_C64 && out == NOA_HIP_F64) return run_insert_extract<Coord, Cx<double>, double, double>(a, n_threads, force);
+    if (in == NOA_HIP_F64 && out == NOA_HIP_F64) return run_insert_extract<Coord, double, double, double>(a, n_threads, force);
+    g_error = "oracle: unsupported value type combination";
+    return NOA_HIP_ERROR_UNSUPPORTED;
 }
 
 enum Op { RASTERIZE, INSERT, EXTRACT };
@@ -1020,6 +1189,23 @@ int oracle_insert_central_slices_3d(const noa_hip_project_args* a, int n_threads
 }
 int oracle_extract_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
     return oracle::dispatch(oracle::EXTRACT, a, n_threads, force != 0);
+}
+
+/* insert_and_extract_central_slices_3d; the coordinate precision is the input rotation's (:1514). */
+int oracle_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_args* a, int n_threads, int force) {
+    using namespace oracle;
+    g_error.clear();
+    if (!a) { g_error = "oracle: null args"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    const int base = a->interp >= 100 ? a->interp - 100 : a->interp;
+    if (base != NOA_HIP_INTERP_LINEAR && base != NOA_HIP_INTERP_NEAREST) {
+        g_error = "oracle: only NEAREST and LINEAR are restated";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    }
+    const int k = a->input_rotation.kind;
+    if (k < NOA_HIP_MAT33_F32 || k > NOA_HIP_QUAT_F64) { g_error = "oracle: invalid rotation kind"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    const bool f64 = k == NOA_HIP_MAT33_F64 || k == NOA_HIP_QUAT_F64;
+    return f64 ? dispatch_insert_extract<double>(*a, n_threads, force != 0)
+               : dispatch_insert_extract<float>(*a, n_threads, force != 0);
 }
 
 /* noa::signal::phase_shift_2d<REMAP> on contiguous (b, h, w/2+1) c32 or c64 arrays; `input` may be NULL (the

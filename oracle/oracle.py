@@ -36,6 +36,16 @@ class ProjectArgs(ctypes.Structure):
                 ("ews_radius", c_double * 2)]
 
 
+class InsertExtractArgs(ctypes.Structure):
+    _fields_ = [("input", View), ("input_weight", View), ("output", View), ("output_weight", View),
+                ("input_shape", c_int64 * 4), ("output_shape", c_int64 * 4), ("remap", c_int32),
+                ("interp", c_int32), ("input_scaling", Transform), ("input_rotation", Transform),
+                ("output_scaling", Transform), ("output_rotation", Transform),
+                ("input_fftfreq_sinc", c_double), ("input_fftfreq_blackman", c_double),
+                ("w_fftfreq_sinc", c_double), ("w_fftfreq_blackman", c_double), ("fftfreq_cutoff", c_double),
+                ("ews_radius", c_double * 2), ("add_to_output", c_int32), ("correct_weights", c_int32)]
+
+
 F32, F64, C32, C64 = 1, 2, 3, 4
 MAT33_F32, MAT33_F64, QUAT_F32, QUAT_F64, MAT22_F32, MAT22_F64 = 1, 2, 3, 4, 5, 6
 LAYOUTS = {"h2h": 0, "h": 0, "hc2h": 4, "h2hc": 8, "hc2hc": 12, "hc": 12}
@@ -62,6 +72,9 @@ def lib() -> ctypes.CDLL:
                   "oracle_extract_central_slices_3d"):
             getattr(L, n).argtypes = [P, ctypes.c_int, ctypes.c_int]
             getattr(L, n).restype = ctypes.c_int
+        L.oracle_insert_and_extract_central_slices_3d.argtypes = [ctypes.POINTER(InsertExtractArgs), ctypes.c_int,
+                                                                  ctypes.c_int]
+        L.oracle_insert_and_extract_central_slices_3d.restype = ctypes.c_int
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_index2frequency.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong]
         L.oracle_index2frequency.restype = ctypes.c_longlong
@@ -211,6 +224,43 @@ def extract_central_slices_3d(remap, volume, volume_weight, volume_shape, slice,
               inv_scaling, fftfreq_cutoff, target_shape, ews_radius, fftfreq_sinc, fftfreq_blackman, interp, keep,
               quaternion)
     _run("oracle_extract_central_slices_3d", a, n_threads, force_threads)
+
+
+def insert_and_extract_central_slices_3d(remap, input_slice, input_weight, input_slice_shape, output_slice,
+                                         output_weight, output_slice_shape, input_fwd_scaling=None,
+                                         input_inv_rotation=None, output_inv_scaling=None,
+                                         output_fwd_rotation=None, *, interp=1, input_fftfreq_sinc=-1.0,
+                                         input_fftfreq_blackman=-1.0, w_fftfreq_sinc=-1.0, w_fftfreq_blackman=-1.0,
+                                         add_to_output=False, correct_weights=False, fftfreq_cutoff=0.5,
+                                         ews_radius=(0.0, 0.0), input_quaternion=False, output_quaternion=False,
+                                         n_threads=1, force_threads=False):
+    """FourierInsertExtract (FourierProject.hpp:727-1006) on host arrays; the output slices are updated in place."""
+    keep = []
+    a = InsertExtractArgs()
+    out = np.asarray(output_slice)
+    real_t, cx_t = (F64, C64) if out.dtype in (np.float64, np.complex128) else (F32, C32)
+    a.input = _view(input_slice, keep, real_t, cx_t)
+    a.input_weight = _view(input_weight, keep, real_t, real_t)
+    a.output = _view(output_slice, keep, real_t, cx_t)
+    a.output_weight = _view(output_weight, keep, real_t, real_t)
+    for i in range(4):
+        a.input_shape[i] = input_slice_shape[i]
+        a.output_shape[i] = output_slice_shape[i]
+    a.remap = LAYOUTS[remap.lower()] if isinstance(remap, str) else remap
+    a.interp = int(interp)
+    a.input_scaling = _transform(input_fwd_scaling, keep, False)
+    a.input_rotation = _transform(input_inv_rotation, keep, True, input_quaternion)
+    a.output_scaling = _transform(output_inv_scaling, keep, False)
+    a.output_rotation = _transform(output_fwd_rotation, keep, True, output_quaternion)
+    a.input_fftfreq_sinc = input_fftfreq_sinc
+    a.input_fftfreq_blackman = input_fftfreq_blackman
+    a.w_fftfreq_sinc = w_fftfreq_sinc
+    a.w_fftfreq_blackman = w_fftfreq_blackman
+    a.fftfreq_cutoff = fftfreq_cutoff
+    a.ews_radius[0], a.ews_radius[1] = ews_radius
+    a.add_to_output = int(bool(add_to_output))
+    a.correct_weights = int(bool(correct_weights))
+    _run("oracle_insert_and_extract_central_slices_3d", a, n_threads, force_threads)
 
 
 def index2frequency(centered, index, size):

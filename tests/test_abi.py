@@ -51,6 +51,8 @@ def test_struct_layout_matches_the_c_compiler(tmp_path):
         "noa_hip_view": ("View", ["data", "shape", "strides", "dtype", "is_value", "value"]),
         "noa_hip_transform": ("Transform", ["data", "kind", "batched", "count"]),
         "noa_hip_project_args": ("ProjectArgs", [f[0] for f in abi.ProjectArgs._fields_]),
+        "noa_hip_insert_extract_args": ("InsertExtractArgs", [f[0] for f in abi.InsertExtractArgs._fields_]),
+        "noa_hip_shift2": ("Shift2", [f[0] for f in abi.Shift2._fields_]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "noa_hip.h"', "int main(void) {"]
     for c_name, (_, names) in fields.items():
@@ -93,6 +95,72 @@ def test_args_init_sets_noa_defaults(lib):
     assert a.fftfreq_cutoff == 0.5 and a.fftfreq_sinc == -1 and a.fftfreq_blackman == -1
     assert a.interp == 1 and a.remap == abi.H2H and a.scaling.kind == abi.XFORM_NONE
     assert list(a.ews_radius) == [0, 0] and list(a.target_shape) == [0, 0, 0, 0]
+
+
+def test_insert_extract_args_init_sets_noa_defaults(lib):
+    """InsertAndExtractCentralSlicesOptions (FourierProject.hpp:1916-1946)."""
+    from noa_amd import abi
+    a = abi.InsertExtractArgs()
+    lib.noa_hip_insert_extract_args_init(ctypes.byref(a))
+    assert a.fftfreq_cutoff == 0.5 and a.interp == 1 and a.remap == abi.H2H
+    assert (a.input_fftfreq_sinc, a.input_fftfreq_blackman, a.w_fftfreq_sinc, a.w_fftfreq_blackman) == (-1, -1, -1, -1)
+    assert a.add_to_output == 0 and a.correct_weights == 0
+    assert a.input_scaling.kind == abi.XFORM_NONE and a.output_scaling.kind == abi.XFORM_NONE
+
+
+def _ie_args(lib, n=16, b_in=3, b_out=2):
+    """A virtual-volume extraction with host buffers (never dereferenced by the checks)."""
+    from noa_amd import abi
+    inp = np.zeros((b_in, 1, n, n // 2 + 1), np.complex64)
+    out = np.zeros((b_out, 1, n, n // 2 + 1), np.complex64)
+    rin = np.zeros((b_in, 12), np.float32)
+    rout = np.zeros((b_out, 12), np.float32)
+    a = abi.InsertExtractArgs()
+    lib.noa_hip_insert_extract_args_init(ctypes.byref(a))
+    for v, arr in ((a.input, inp), (a.output, out)):
+        v.data = arr.ctypes.data
+        for i in range(4):
+            v.shape[i] = arr.shape[i]
+            v.strides[i] = arr.strides[i] // arr.itemsize
+        v.dtype = abi.C32
+    for i in range(4):
+        a.input_shape[i] = (b_in, 1, n, n)[i]
+        a.output_shape[i] = (b_out, 1, n, n)[i]
+    for t, arr, cnt in ((a.input_rotation, rin, b_in), (a.output_rotation, rout, b_out)):
+        t.data = arr.ctypes.data
+        t.kind = abi.MAT33_F32
+        t.batched = 1
+        t.count = cnt
+    return a, (inp, out, rin, rout)
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda a: a.output_shape.__setitem__(1, 4), "shape"),
+    (lambda a: setattr(a.input_rotation, "count", 2), "input_rotation"),
+    (lambda a: setattr(a.output_rotation, "count", 3), "output_rotation"),
+    (lambda a: setattr(a, "correct_weights", 1) or setattr(a, "add_to_output", 1), "correct_weights"),
+    (lambda a: setattr(a, "correct_weights", 1) or setattr(a, "w_fftfreq_blackman", 0.3), "correct_weights"),
+    (lambda a: setattr(a, "interp", 2), "nterp"),
+    (lambda a: setattr(a.output_rotation, "kind", 2), "same precision"),
+])
+def test_insert_extract_validation_errors(lib, mutate, message):
+    from noa_amd import abi
+    a, keep = _ie_args(lib)
+    mutate(a)
+    status = lib.noa_hip_insert_and_extract_central_slices_3d(ctypes.byref(a), None)
+    err = lib.noa_hip_last_error().decode()
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
+    assert message in err, err
+    del keep
+
+
+def test_insert_extract_valid_arguments_reach_the_device_check(lib):
+    from noa_amd import abi
+    a, keep = _ie_args(lib)
+    status = lib.noa_hip_insert_and_extract_central_slices_3d(ctypes.byref(a), None)
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_RUNTIME)
+    assert "device" in lib.noa_hip_last_error().decode().lower()
+    del keep
 
 
 def _args(lib, n=16, b=2, remap=0):

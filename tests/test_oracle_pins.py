@@ -554,3 +554,234 @@ def test_remap_is_numpy_fftshift_on_the_full_axes(remap, fn, shape):
     back = np.zeros_like(x)
     O.remap_hx2hx("hc2h" if remap == "h2hc" else "h2hc", out, back, shape)
     np.testing.assert_array_equal(back, x)
+
+
+# ------------------------------------------------------------------------------------------------ virtual volume
+# insert_and_extract_central_slices_3d (FourierProject.hpp:727-1006, :1916-2049).
+def _insext(remap, slices, n_out_shape, rin, rout, weights=None, wout=True, out_dtype=None, start=None, **kw):
+    b_in, _, h, wr = slices.shape if hasattr(slices, "shape") else kw.pop("in_shape4")
+    b_out, h_out, w_out = n_out_shape
+    dt = out_dtype or (slices.dtype if hasattr(slices, "dtype") else np.complex64)
+    out = np.zeros((b_out, 1, h_out, w_out // 2 + 1), dt) if start is None else start.copy()
+    wdt = np.float64 if np.dtype(dt) in (np.float64, np.complex128) else np.float32
+    wo = np.zeros(out.shape, wdt) if wout else None
+    w_in = kw.pop("w_in", None)
+    O.insert_and_extract_central_slices_3d(remap, slices, weights if w_in is None else w_in,
+                                           (b_in, 1, h, kw.pop("w_in_logical", h)), out, wo,
+                                           (b_out, 1, h_out, w_out), None, rin, None, rout, **kw)
+    return out, wo
+
+
+def test_insert_extract_identity_closed_form():
+    """Identity poses everywhere: every input slice lies in the output plane (fz = 0, window 1), so the output is
+    the sum of the inputs inside the cutoff, the weight their count, and 0 beyond the cutoff."""
+    rng = np.random.default_rng(40)
+    n, b = 16, 3
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    eye_in = np.stack([np.eye(3, dtype=np.float32)] * b)
+    eye_out = np.eye(3, dtype=np.float32)[None]
+    w0 = np.float32((np.float32(0.42) + np.float32(0.5)) + np.float32(0.08))
+    v = freq_grid(n, True)[:, None] / n
+    u = np.arange(n // 2 + 1)[None, :] / n
+    inside = v ** 2 + u ** 2 <= 0.25
+    total = s[0, 0] * w0
+    for i in range(1, b):
+        total = total + s[i, 0] * w0
+    out, wo = _insext("hc2hc", s, (1, n, n), eye_in, eye_out)
+    np.testing.assert_allclose(out[0, 0], np.where(inside, total, 0), rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(wo[0, 0], np.where(inside, b * w0, 0), rtol=1e-6)
+    # correct_weights divides by max(1, |weight|): the mean of the inputs.
+    out, _ = _insext("hc2hc", s, (1, n, n), eye_in, eye_out, correct_weights=True)
+    np.testing.assert_allclose(out[0, 0], np.where(inside, total / (b * w0), 0), rtol=1e-6, atol=1e-6)
+    # complex -> real output is the power spectrum; add_to_output adds onto the outputs, beyond the cutoff too.
+    out, _ = _insext("hc2hc", s, (1, n, n), eye_in, eye_out, out_dtype=np.float32)
+    np.testing.assert_allclose(out[0, 0], np.where(inside, np.abs(total) ** 2, 0), rtol=1e-5)
+    start = random_spectrum((1, 1, n, n // 2 + 1), rng)
+    out, _ = _insext("hc2hc", s, (1, n, n), eye_in, eye_out, start=start, add_to_output=True)
+    np.testing.assert_allclose(out[0, 0], start[0, 0] + np.where(inside, total, 0), rtol=1e-6, atol=1e-6)
+
+
+def test_insert_extract_linear_field_at_random_poses():
+    """Input slices holding the same Hermitian linear field c + i (a . k) (k the pixel frequency): interpolation is
+    exact, so each output pixel is sum_i window(fz_i) (c + i a . k_i), with (fz_i, k_i / n) = R_in_i R_out (0, f),
+    evaluated here in float64 from the matrices. A transposed rotation or a wrong window fails it."""
+    rng = np.random.default_rng(41)
+    n, b_in, b_out = 32, 40, 3
+    c, a = 0.5, np.array([0.9, -0.3])
+    v = freq_grid(n, True)[:, None].astype(np.float64)
+    u = np.arange(n // 2 + 1)[None, :].astype(np.float64)
+    field = (c + 1j * (a[0] * v + a[1] * u)).astype(np.complex64)
+    s = np.broadcast_to(field, (b_in, 1, n, n // 2 + 1)).copy()
+    rin = random_rotations(b_in, rng)
+    rout = random_rotations(b_out, rng)
+    fs, fb = 0.04, 0.07
+    out, wo = _insext("hc2hc", s, (b_out, n, n), rin, rout, input_fftfreq_sinc=fs, input_fftfreq_blackman=fb)
+    f2 = np.stack(np.broadcast_arrays(v / n, u / n), -1)                        # (H, W/2+1, 2)
+    f3 = np.concatenate([np.zeros(f2.shape[:-1] + (1,)), f2], -1)              # (0, fy, fx)
+    for j in range(b_out):
+        g3 = f3 @ rout[j].astype(np.float64).T                                 # R_out (0, f)
+        want = np.zeros(f2.shape[:-1], np.complex128)
+        want_w = np.zeros(f2.shape[:-1])
+        ok = np.ones(f2.shape[:-1], bool)
+        for i in range(b_in):
+            g = g3 @ rin[i].astype(np.float64).T                               # R_in R_out (0, f)
+            x = g[..., 0]
+            win = np.sinc(x / fs) * (0.42 + 0.5 * np.cos(np.pi * x / fb) + 0.08 * np.cos(2 * np.pi * x / fb))
+            hit = np.abs(x) < fb
+            k = g[..., 1:] * n
+            want += np.where(hit, win * (c + 1j * (a[0] * k[..., 0] + a[1] * k[..., 1])), 0)
+            want_w += np.where(hit, win, 0)
+            ok &= ~hit | (np.abs(k) <= n / 2 - 2).all(-1)                       # taps inside the slice
+            ok &= np.abs(np.abs(x) - fb) > 1e-4                                 # away from the window's edge
+        ok &= (f2 ** 2).sum(-1) <= 0.25
+        assert ok.sum() > 200 and np.count_nonzero(want_w[ok]) > 100
+        assert rel_l2(out[j, 0][ok], want[ok]) < 2e-5
+        assert rel_l2(wo[j, 0][ok], want_w[ok]) < 2e-5
+
+
+def _signed_permutations(n, rng):
+    """Random rotations that map the integer grid onto itself (signed permutation matrices with det +1)."""
+    out = []
+    import itertools
+    perms = list(itertools.permutations(range(3)))
+    while len(out) < n:
+        m = np.zeros((3, 3), np.float32)
+        p = perms[rng.integers(len(perms))]
+        for r in range(3):
+            m[r, p[r]] = rng.choice([-1.0, 1.0])
+        if np.linalg.det(m) > 0:
+            out.append(m)
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("remap", ["hc2hc", "h2h"])
+def test_insert_extract_equals_insert_then_extract_on_grid_poses(remap):
+    """With poses that keep every frequency on the grid, inserting into a real volume and extracting from it is
+    exact, so the virtual volume must give the same slices: this ties the operator's two transforms and layouts to
+    the (pinned) insertion and extraction. The slices are spectra of real images (Hermitian along x = 0, where the
+    two paths read opposite halves) of odd size (no Nyquist row)."""
+    rng = np.random.default_rng(42)
+    n, b_in, b_out = 15, 12, 6
+    s = np.fft.rfft2(rng.standard_normal((b_in, 1, n, n))).astype(np.complex64)
+    if remap == "hc2hc":
+        s = h2hc(s, 2)
+    rin = _signed_permutations(b_in, rng)
+    rout = _signed_permutations(b_out, rng)
+    vol = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wvol = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    vol_layout = {"hc2hc": "hc2hc", "h2h": "h2h"}[remap]
+    O.insert_central_slices_3d(vol_layout, s, None, (b_in, 1, n, n), vol, wvol, (1, n, n, n), None, rin)
+    ext = np.zeros((b_out, 1, n, n // 2 + 1), np.complex64)
+    wext = np.zeros((b_out, 1, n, n // 2 + 1), np.float32)
+    O.extract_central_slices_3d(vol_layout, vol, wvol, (1, n, n, n), ext, wext, (b_out, 1, n, n), None, rout)
+    out, wo = _insext(remap, s, (b_out, n, n), rin, rout)
+    assert np.count_nonzero(wo) > 100
+    assert rel_l2(out, ext) < 1e-6
+    np.testing.assert_allclose(wo, wext, rtol=1e-6, atol=1e-6)
+
+
+def test_insert_extract_w_window_closed_form():
+    """Identity poses with a w window: the output is the input sum times sum_w window_in(o_w) window_w(o_w) over
+    the offsets o_w = (w - size/2) / n inside the cutoff, zeros first."""
+    rng = np.random.default_rng(43)
+    n, b = 40, 2
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    eye_in = np.stack([np.eye(3, dtype=np.float32)] * b)
+    eye_out = np.eye(3, dtype=np.float32)[None]
+    fs_in, fb_in, fs_w, fb_w, cutoff = 0.05, 0.08, 0.03, 0.06, 0.45   # above the 1 / n clamp (:838-841)
+    size = O.blackman_window_size(fb_w, n)
+    assert size == 5                                                             # 2 * ceil(40 * 0.06) + 1 - 2
+    start = random_spectrum((1, 1, n, n // 2 + 1), rng)
+    out, wo = _insext("hc2hc", s, (1, n, n), eye_in, eye_out, start=start, input_fftfreq_sinc=fs_in,
+                      input_fftfreq_blackman=fb_in, w_fftfreq_sinc=fs_w, w_fftfreq_blackman=fb_w,
+                      fftfreq_cutoff=cutoff)
+
+    def win(x, fsinc, fblack):
+        return np.sinc(x / fsinc) * (0.42 + 0.5 * np.cos(np.pi * x / fblack) + 0.08 * np.cos(2 * np.pi * x / fblack))
+
+    v = freq_grid(n, True)[:, None] / n
+    u = np.arange(n // 2 + 1)[None, :] / n
+    factor = np.zeros(v.shape[:1] + u.shape[1:])
+    for w in range(size):
+        o = (w - size // 2) / n
+        inside = o ** 2 + v ** 2 + u ** 2 <= cutoff ** 2
+        factor += np.where(inside & (abs(o) < fb_in), win(o, fs_in, fb_in) * win(o, fs_w, fb_w), 0)
+    total = s.astype(np.complex128).sum(0)[0]
+    assert np.count_nonzero(factor) > 100
+    np.testing.assert_allclose(out[0, 0], factor * total, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(wo[0, 0], factor * b, rtol=1e-5, atol=1e-6)
+    # correct_weights is refused with a w window (FourierProject.hpp:1544).
+    with pytest.raises(O.OracleError):
+        _insext("hc2hc", s, (1, n, n), eye_in, eye_out, w_fftfreq_blackman=fb_w, correct_weights=True)
+
+
+def test_insert_extract_add_to_output_splits_the_inputs():
+    rng = np.random.default_rng(44)
+    n, b_in, b_out = 24, 10, 3
+    s = random_spectrum((b_in, 1, n, n // 2 + 1), rng)
+    rin = random_rotations(b_in, rng)
+    rout = random_rotations(b_out, rng)
+    kw = dict(input_fftfreq_sinc=0.05, input_fftfreq_blackman=0.1)
+    full, wfull = _insext("h2h", s, (b_out, n, n), rin, rout, **kw)
+    part, wpart = _insext("h2h", s[:4], (b_out, n, n), rin[:4], rout, **kw)
+    out = part.copy()
+    wo = wpart.copy()
+    O.insert_and_extract_central_slices_3d("h2h", s[4:], None, (b_in - 4, 1, n, n), out, wo, (b_out, 1, n, n),
+                                           None, rin[4:], None, rout, add_to_output=True, **kw)
+    assert rel_l2(out, full) < 1e-6
+    assert rel_l2(wo, wfull) < 1e-6
+
+
+def _tilts(n, step, offset, sign):
+    return np.stack([euler2matrix(np.deg2rad([0.0, sign * (i * step + offset), 0.0]), axes="zyx", dtype=np.float32)
+                     for i in range(n)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_insert_extract_remap_invariant_reference(dtype):
+    """TestXformFourierInsertInterpolateExtract.cpp:89-139 at 64^2: hc2hc == h2hc-remap of hc2h."""
+    n, b_in, b_out = 64, 20, 5
+    lin = np.linspace(-50, 50, b_in * n * (n // 2 + 1)).reshape(b_in, 1, n, n // 2 + 1)
+    s = (lin + 1j * lin).astype(dtype) if np.dtype(dtype).kind == "c" else lin.astype(dtype)
+    rin, rout = _tilts(b_in, 2, 0, -1), _tilts(b_out, 2, 1, 1)
+    kw = dict(input_fftfreq_sinc=0.01, fftfreq_cutoff=0.8)
+    a, _ = _insext("hc2hc", s, (b_out, n, n), rin, rout, wout=False, **kw)
+    b, _ = _insext("hc2h", s, (b_out, n, n), rin, rout, wout=False, **kw)
+    assert np.count_nonzero(a) > 1000
+    assert allclose_abs_safe(a, h2hc(b, 2), 5e-5)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_insert_extract_weights_invariant_reference(dtype):
+    """TestXformFourierInsertInterpolateExtract.cpp:141-182 at 64^2: real inputs used as their own weights give
+    output weights equal to the output values."""
+    rng = np.random.default_rng(45)
+    n, b_in, b_out = 64, 20, 5
+    s = rng.uniform(-10, 10, (b_in, 1, n, n // 2 + 1)).astype(dtype)
+    rin, rout = _tilts(b_in, 2, 0, -1), _tilts(b_out, 2, 1, 1)
+    out, wo = _insext("hc2hc", s, (b_out, n, n), rin.astype(dtype), rout.astype(dtype), w_in=s.copy(),
+                      input_fftfreq_sinc=0.01, fftfreq_cutoff=0.8)
+    assert np.count_nonzero(out) > 1000
+    assert allclose_abs_safe(out, wo, 5e-5)
+
+
+def test_insert_extract_matrix_equals_quaternion_reference():
+    """TestXformFourierInsertInterpolateExtract.cpp:231-283 at 64^2 (compared below 0.49 cycle/pixel). The
+    reference compares per element with allclose_abs_safe(5e-5); the matrix and the quaternion round the rotated
+    coordinates differently (~1e-7 of a pixel), which moves components near a zero crossing by more than 5e-5 of
+    their own size on this data, so the bound here is 5e-5 of the spectrum's largest magnitude, plus rel-L2."""
+    rng = np.random.default_rng(46)
+    n = 64
+    m = euler2matrix(np.deg2rad([45.0, 0.0, 0.0]), axes="zyx", dtype=np.float32)[None]
+    q = quaternion_from_matrix(m).astype(np.float32)
+    q /= np.linalg.norm(q, axis=-1, keepdims=True)
+    spec = np.fft.fftshift(np.fft.rfft2(rng.uniform(-50, 50, (n, n))), axes=0).astype(np.complex64)[None, None]
+    rout = np.eye(3, dtype=np.float32)[None]
+    kw = dict(input_fftfreq_sinc=0.01, fftfreq_cutoff=0.8, wout=False)
+    a, _ = _insext("hc2hc", spec, (1, n, n), m, rout, **kw)
+    b, _ = _insext("hc2hc", spec, (1, n, n), q, rout, input_quaternion=True, **kw)
+    keep = np.sqrt((freq_grid(n, True)[:, None] / n) ** 2 + (np.arange(n // 2 + 1)[None, :] / n) ** 2) <= 0.49
+    assert np.count_nonzero(a[0, 0][keep]) > 1000
+    a0, b0 = np.where(keep, a[0, 0], 0), np.where(keep, b[0, 0], 0)
+    assert np.abs(a0 - b0).max() <= 5e-5 * np.abs(a0).max()
+    assert rel_l2(b0, a0) < 1e-5
