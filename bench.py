@@ -213,6 +213,60 @@ def precision_report(ext, nx, torch, n_ref=128):
     return report
 
 
+def neighbours_report(torch, dev, nx, ins, n, reps=5):
+    """The steps on either side of the hot path, timed alone (not part of the step): the CTF prologue of the
+    insertion (slice * CTF and CTF^2 in one pass over the insertion workload's raw spectra) and an extraction from a
+    virtual volume (a 61-view +-60 degree tilt series of 256^2 slices reprojected at 61 tilts offset by 1 degree)."""
+    import numpy as np
+    import noa_amd.signal as ns
+    rep = {}
+    b = ins.raw.shape[0]
+    vals = torch.empty_like(ins.raw)
+    wts = torch.empty((b, 1, n, n // 2 + 1), dtype=torch.float32, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) * 1e-3 / reps
+
+    t = timed(lambda: ns.ctf_anisotropic_weighted("h2h", ins.raw, vals, wts, ins.slice_shape, ins.ctfs))
+    nbytes = b * n * (n // 2 + 1) * (8 + 8 + 4) + b * 56
+    dv = (vals - ins.slices).abs().max().item() / ins.slices.abs().max().item()
+    rep["ctf_prologue"] = {
+        "workload": f"{b} slices 256^2 (h2h): value * CTF and CTF^2, CTFAnisotropic<f32> per slice",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS, "slices_per_s": b / t,
+        "max_rel_diff_vs_workload_inputs": dv}
+    log(f"ctf prologue: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del vals, wts
+
+    tilts_in = np.deg2rad(np.arange(-60, 61, 2, dtype=np.float64))
+    tilts_out = tilts_in + np.deg2rad(1.0)
+    rin = np.stack([nx.euler2matrix([0.0, -a, 0.0], axes="zyx", dtype=np.float32) for a in tilts_in])
+    rout = np.stack([nx.euler2matrix([0.0, a, 0.0], axes="zyx", dtype=np.float32) for a in tilts_out])
+    b_in, b_out = len(tilts_in), len(tilts_out)
+    src = ins.raw[:b_in]
+    out = torch.empty((b_out, 1, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    wout = torch.empty((b_out, 1, n, n // 2 + 1), dtype=torch.float32, device=dev)
+    rin_d, rout_d = torch.from_numpy(rin).to(dev), torch.from_numpy(rout).to(dev)
+    t = timed(lambda: nx.insert_and_extract_central_slices_3d(
+        "h2h", src, None, (b_in, 1, n, n), out, wout, (b_out, 1, n, n), None, rin_d, None, rout_d))
+    pairs = b_out * n * (n // 2 + 1) * b_in
+    rep["virtual_volume"] = {
+        "workload": f"insert_and_extract_central_slices_3d: {b_in} tilts (+-60, 2 deg) of 256^2 -> {b_out} tilts "
+                    "offset by 1 deg, h2h, LINEAR, default windows",
+        "avg_ms": t * 1e3, "output_slices_per_s": b_out / t, "pixel_slice_pairs_per_s": pairs / t,
+        "nonzero_weight_fraction": (wout != 0).float().mean().item()}
+    log(f"virtual volume: {t * 1e3:.3f} ms, {pairs / t / 1e9:.1f} G pixel-slice pairs/s")
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +279,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=200.0, help="request rate of the serving run")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving run")
+    ap.add_argument("--no-neighbours", action="store_true",
+                    help="skip the CTF prologue and virtual-volume timings (not part of the step)")
     ap.add_argument("--precision", action="store_true",
                     help="FP32/FP16/BF16 volume storage for the 256^3 extraction: error against the FP64 oracle")
     args = ap.parse_args()
@@ -246,7 +302,7 @@ def main():
     if args.precision:
         print(json.dumps(precision_report(ext, nx, torch)), flush=True)
         return
-    ins = workloads.insert_256(dev, b=args.n_insert)
+    ins = workloads.insert_256(dev, b=args.n_insert, keep_raw=not args.probe)
     n = 256
     out = torch.empty((args.n_extract, 1, n, n // 2 + 1), dtype=torch.complex64, device=dev)
     vol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
@@ -347,6 +403,9 @@ def main():
         },
         "kernels": kernels,
     }
+    if not args.no_neighbours:
+        log("prologue / virtual volume ...")
+        result["neighbours"] = neighbours_report(torch, dev, nx, ins, n)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(ext, ins)

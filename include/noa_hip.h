@@ -178,6 +178,39 @@ typedef struct noa_hip_insert_extract_args {
     int32_t correct_weights;            /* divide the values by max(1, |weight|) (3D only, not with add_to_output) */
 } noa_hip_insert_extract_args;
 
+/* noa::signal::CTFAnisotropic<f32> and <f64> as stored by noa (src/noa/signal/core/CTF.hpp:564-575, member order;
+ * Vec<f32,2> is 8-byte aligned, Vec<f64,2> 16-byte aligned): 56 and 112 bytes. The members are noa's internal
+ * values (defocus in angstroms with noa's sign, cs in angstroms, voltage in volts, bfactor / 4 and the derived k1,
+ * k2, k3); a binding passes noa's objects as they are. */
+typedef struct noa_hip_ctf_anisotropic_f32 {
+    float pixel_size[2];
+    float defocus_value, defocus_astigmatism, defocus_angle;
+    float phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+} noa_hip_ctf_anisotropic_f32;
+typedef struct noa_hip_ctf_anisotropic_f64 {
+    double pixel_size[2];
+    double defocus_value, defocus_astigmatism, defocus_angle;
+    double phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+} noa_hip_ctf_anisotropic_f64;
+
+/* CTFs of a ctf_anisotropic call: one CTF broadcast over the batch (batched == 0; `data` is a HOST pointer to one
+ * object, copied at the call) or one per batch (batched == 1; `data` is a DEVICE pointer to `count` objects). */
+typedef struct noa_hip_ctf {
+    const void* data;
+    int32_t dtype;          /* NOA_HIP_F32 (noa_hip_ctf_anisotropic_f32) or NOA_HIP_F64 */
+    int32_t batched;
+    int64_t count;
+} noa_hip_ctf;
+
+/* CTFOptions (src/noa/signal/CTF.hpp:196-209): fftfreq_range {start, stop, endpoint}, ctf_abs, ctf_squared. */
+typedef struct noa_hip_ctf_options {
+    double fftfreq_start;   /* must be 0 for 2D spectra */
+    double fftfreq_stop;    /* <= 0: the highest frequency of each dimension (default -1) */
+    int32_t endpoint;       /* default 1 */
+    int32_t ctf_abs;
+    int32_t ctf_squared;
+} noa_hip_ctf_options;
+
 /* Fill `args` with noa's defaults (cutoff 0.5, sinc/blackman -1, LINEAR, H2H, no scaling). */
 void noa_hip_project_args_init(noa_hip_project_args* args);
 
@@ -204,6 +237,23 @@ int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* st
  * window of one sample) the output slices are OVERWRITTEN (0 beyond the cutoff) or, with add_to_output, added to;
  * with a w window they are zeroed first unless add_to_output, then the windowed sum over w is added. */
 int noa_hip_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_args* args, void* stream);
+
+/* Replaces the GPU branch of noa::signal::ctf_anisotropic<REMAP> (src/noa/signal/CTF.hpp:430-500, operator :14-125)
+ * for (B, 1, H, W/2+1) rfft arrays: output[b, y, x] = input[b, remap(y), x] * ctf_b(f) (|.|^2 for a complex input
+ * and a real output), or the CTF itself when `input` is Empty (data == NULL). f is the frequency times the step of
+ * the fftfreq range; the CTF is squared (ctf_squared) or its absolute value taken (ctf_abs). Strides must be
+ * row-major along H, W. The output may be the input when the layout does not change. */
+int noa_hip_ctf_anisotropic_2d(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                               int32_t remap, const noa_hip_ctf* ctf, const noa_hip_ctf_options* options,
+                               void* stream);
+
+/* Both CTF steps before a CTF-weighted insertion in one pass: output = input * ctf and output_weight = ctf^2, the
+ * results of noa's ctf_anisotropic<REMAP>(input, output, ...) followed by ctf_anisotropic<REMAP>({}, output_weight,
+ * ..., {.ctf_squared = true}) with the same CTFs and range (ctf_abs and ctf_squared of `options` are ignored).
+ * Extension of this library: the input is read once and the CTF evaluated once per element. */
+int noa_hip_ctf_anisotropic_2d_weighted(const noa_hip_view* input, const noa_hip_view* output,
+                                        const noa_hip_view* output_weight, const int64_t shape[4], int32_t remap,
+                                        const noa_hip_ctf* ctf, const noa_hip_ctf_options* options, void* stream);
 
 /* Replaces the GPU branch of noa::signal::phase_shift_2d<REMAP> (src/noa/signal/PhaseShift.hpp:215-289, operator
  * :78-139) for the rfft layouts: output[b, remap(y, x)] = input[b, y, x] * exp(-2 pi i shift_b . f(y, x)), f the

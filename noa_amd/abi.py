@@ -129,6 +129,30 @@ def default_insert_extract_args() -> InsertExtractArgs:
     return a
 
 
+class CtfAnisotropicF32(ctypes.Structure):
+    """noa::signal::CTFAnisotropic<f32> members (signal/core/CTF.hpp:564-575): 56 bytes, 8-byte aligned."""
+    _fields_ = [("pixel_size", ctypes.c_float * 2), ("defocus_value", ctypes.c_float),
+                ("defocus_astigmatism", ctypes.c_float), ("defocus_angle", ctypes.c_float),
+                ("phase_shift", ctypes.c_float), ("cs", ctypes.c_float), ("voltage", ctypes.c_float),
+                ("amplitude", ctypes.c_float), ("bfactor_forth", ctypes.c_float), ("scale", ctypes.c_float),
+                ("k1", ctypes.c_float), ("k2", ctypes.c_float), ("k3", ctypes.c_float)]
+
+
+class CtfAnisotropicF64(ctypes.Structure):
+    """noa::signal::CTFAnisotropic<f64>: 112 bytes, 16-byte aligned."""
+    _fields_ = [(name, ctypes.c_double * 2 if name == "pixel_size" else ctypes.c_double)
+                for name, _ in CtfAnisotropicF32._fields_]
+
+
+class Ctf(ctypes.Structure):
+    _fields_ = [("data", c_void_p), ("dtype", c_int32), ("batched", c_int32), ("count", c_int64)]
+
+
+class CtfOptions(ctypes.Structure):
+    _fields_ = [("fftfreq_start", c_double), ("fftfreq_stop", c_double), ("endpoint", c_int32),
+                ("ctf_abs", c_int32), ("ctf_squared", c_int32)]
+
+
 class Shift2(ctypes.Structure):
     _fields_ = [
         ("data", c_void_p),
@@ -165,6 +189,12 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_insert_and_extract_central_slices_3d.restype = ctypes.c_int
     lib.noa_hip_insert_extract_args_init.argtypes = [IE]
     lib.noa_hip_insert_extract_args_init.restype = None
+    C = ctypes.POINTER(Ctf)
+    CO = ctypes.POINTER(CtfOptions)
+    lib.noa_hip_ctf_anisotropic_2d.argtypes = [V, V, ctypes.POINTER(c_int64), c_int32, C, CO, c_void_p]
+    lib.noa_hip_ctf_anisotropic_2d.restype = ctypes.c_int
+    lib.noa_hip_ctf_anisotropic_2d_weighted.argtypes = [V, V, V, ctypes.POINTER(c_int64), c_int32, C, CO, c_void_p]
+    lib.noa_hip_ctf_anisotropic_2d_weighted.restype = ctypes.c_int
     lib.noa_hip_remap.argtypes = [c_int32, V, V, ctypes.POINTER(c_int64), c_void_p]
     lib.noa_hip_remap.restype = ctypes.c_int
     lib.noa_hip_fourier_interpolation_correction.argtypes = [V, V, c_int32, c_int32, c_void_p]

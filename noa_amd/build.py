@@ -34,7 +34,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "insert_extract.hip", "phase_shift.hip", "spectrum.hip"]
+SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "insert_extract.hip", "phase_shift.hip",
+           "spectrum.hip", "ctf.hip"]
 
 
 def hipcc() -> str:

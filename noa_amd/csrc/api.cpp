@@ -696,6 +696,107 @@ InsertExtractDesc validate_insert_extract(const noa_hip_insert_extract_args& a, 
     return d;
 }
 
+// ctf_anisotropic (signal/CTF.hpp:143-189 checks): 2D rfft arrays, one CTF or one per batch.
+CtfDesc validate_ctf(const noa_hip_view* in, const noa_hip_view* out, const noa_hip_view* wout, const int64_t* shape,
+                     int32_t remap, const noa_hip_ctf* ctf, const noa_hip_ctf_options* opt, bool fused, int& device) {
+    NOA_CHECK(out != nullptr && shape != nullptr && ctf != nullptr, INVALID, "null arguments");
+    NOA_CHECK(out->data != nullptr && !out->is_value, INVALID, "Empty array detected");
+    NOA_CHECK(remap == NOA_HIP_H2H || remap == NOA_HIP_HC2H || remap == NOA_HIP_H2HC || remap == NOA_HIP_HC2HC,
+              UNSUPPORTED, "ctf_anisotropic: only the rfft layouts (h2h, hc2h, h2hc, hc2hc) are implemented, got %d",
+              remap);
+    for (int i = 0; i < 4; ++i) NOA_CHECK(shape[i] > 0, INVALID, "Empty array detected");
+    NOA_CHECK(shape[1] == 1 && shape[2] > 1 && shape[3] > 1, INVALID,
+              "Only (batched) 2d arrays are supported with anisotropic CTFs, but got ctf:shape=%s",
+              shape_str(shape).c_str());
+    int64_t expected[4];
+    rfft_shape(shape, expected);
+    NOA_CHECK(same_shape(out->shape, expected), INVALID,
+              "The output shape doesn't match the expected shape. Got output:shape=%s and expected:shape=%s",
+              shape_str(out->shape).c_str(), shape_str(expected).c_str());
+    NOA_CHECK(out->strides[3] <= out->strides[2] || out->shape[2] == 1, UNSUPPORTED,
+              "ctf_anisotropic: the output should be row-major along (H, W)");
+    NOA_CHECK(is_real(out->dtype) || out->dtype == NOA_HIP_C32 || out->dtype == NOA_HIP_C64, INVALID,
+              "ctf_anisotropic: the output should be f32, f64, c32 or c64, got dtype=%d", out->dtype);
+    const bool has_input = in != nullptr && in->data != nullptr;
+    NOA_CHECK(!fused || has_input, INVALID, "ctf_anisotropic (weighted): the input should not be empty");
+    const bool layout_change = ((remap & 4) != 0) != ((remap & 8) != 0);
+    if (has_input) {
+        NOA_CHECK(!in->is_value, INVALID, "ctf_anisotropic: the input should be an array");
+        NOA_CHECK((in->shape[0] == shape[0] || in->shape[0] == 1) && in->shape[1] == 1 &&
+                  in->shape[2] == expected[2] && in->shape[3] == expected[3], INVALID,
+                  "Cannot broadcast an array of shape %s into an array of shape %s", shape_str(in->shape).c_str(),
+                  shape_str(expected).c_str());
+        NOA_CHECK(!layout_change || !overlap(*in, *out), INVALID,
+                  "This function cannot execute an in-place multiplication and a remapping");
+        NOA_CHECK(in->data == out->data || !overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
+        const int i = in->dtype, o = out->dtype;
+        NOA_CHECK((is_complex(i) && is_complex(o)) || (is_real(i) && is_real(o)) || (is_complex(i) && is_real(o)),
+                  INVALID, "Invalid input/output types: %d -> %d", i, o);
+        NOA_CHECK(i != NOA_HIP_C16 && i != NOA_HIP_CBF16, UNSUPPORTED, "ctf_anisotropic: f16/bf16 inputs are not supported");
+        const bool i64p = i == NOA_HIP_F64 || i == NOA_HIP_C64, o64p = o == NOA_HIP_F64 || o == NOA_HIP_C64;
+        NOA_CHECK(i64p == o64p, UNSUPPORTED, "input and output should have the same precision");
+    }
+    if (fused) {
+        NOA_CHECK(wout != nullptr && wout->data != nullptr && !wout->is_value, INVALID, "output_weight: Empty array detected");
+        NOA_CHECK(same_shape(wout->shape, expected), INVALID,
+                  "The output_weight shape doesn't match the expected shape. Got %s and expected %s",
+                  shape_str(wout->shape).c_str(), shape_str(expected).c_str());
+        const bool o64p = out->dtype == NOA_HIP_F64 || out->dtype == NOA_HIP_C64;
+        NOA_CHECK(wout->dtype == (o64p ? NOA_HIP_F64 : NOA_HIP_F32), UNSUPPORTED,
+                  "output_weight: expected %s weights", o64p ? "f64" : "f32");
+        NOA_CHECK(!overlap(*wout, *out) && !overlap(*wout, *in), INVALID, "Output arrays should not overlap");
+    }
+    // CTFs.
+    NOA_CHECK(ctf->dtype == NOA_HIP_F32 || ctf->dtype == NOA_HIP_F64, INVALID,
+              "ctf: expected CTFAnisotropic<f32> or CTFAnisotropic<f64>, got dtype=%d", ctf->dtype);
+    NOA_CHECK(ctf->data != nullptr, INVALID, "ctf: Empty array detected");
+    if (ctf->batched) {
+        NOA_CHECK(ctf->count == shape[0], INVALID,
+                  "The CTFs should be specified as a contiguous vector with %lld elements, but got %lld elements",
+                  (long long)shape[0], (long long)ctf->count);
+        NOA_CHECK(reinterpret_cast<uintptr_t>(ctf->data) % (ctf->dtype == NOA_HIP_F64 ? 16 : 8) == 0, INVALID,
+                  "ctf: the array should be aligned as CTFAnisotropic");
+    }
+    // Options (:196-209; the start must be 0 for multidimensional spectra).
+    noa_hip_ctf_options o{0, -1, 1, 0, 0};
+    if (opt) o = *opt;
+    NOA_CHECK(allclose_f64(o.fftfreq_start, 0.), INVALID,
+              "For multidimensional cases, the starting fftfreq should be 0, but got %g", o.fftfreq_start);
+    auto i32_ok = [](const noa_hip_view& v) { return fits_i32(v, 2); };
+    NOA_CHECK(i32_ok(*out) && (!has_input || i32_ok(*in)) && (!fused || i32_ok(*wout)), UNSUPPORTED,
+              "offsets within a slice exceed 32-bit indexing");
+
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    if (has_input)
+        NOA_CHECK(device_of(in->data, nullptr) == device, INVALID,
+                  "The input and output arrays must be on the same device");
+    if (fused) NOA_CHECK(device_of(wout->data, nullptr) == device, INVALID, "The arrays should be on the same device");
+    if (ctf->batched)
+        NOA_CHECK(device_of(ctf->data, nullptr) == device, INVALID, "The ctf and output arrays must be on the same device");
+
+    CtfDesc d;
+    if (has_input) {
+        d.input = to_arr(*in, false);
+        if (in->shape[0] == 1) d.input.s0 = 0;   // broadcast over the batch
+    }
+    d.output = to_arr(*out, false);
+    if (fused) d.output_weight = to_arr(*wout, false);
+    for (int i = 0; i < 4; ++i) d.shape[i] = shape[i];
+    d.src_centered = (remap & 4) != 0;
+    d.dst_centered = (remap & 8) != 0;
+    d.ctf_abs = !fused && o.ctf_abs != 0;
+    d.ctf_squared = !fused && o.ctf_squared != 0;
+    d.range_start = o.fftfreq_start;
+    d.range_stop = o.fftfreq_stop;
+    d.endpoint = o.endpoint != 0;
+    d.ctf_f64 = ctf->dtype == NOA_HIP_F64;
+    d.ctf_batched = ctf->batched != 0;
+    if (d.ctf_batched) d.ctf_data = ctf->data;
+    else std::memcpy(d.ctf_single, ctf->data, d.ctf_f64 ? 112 : 56);
+    return d;
+}
+
 int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr) {
     g_last_error.clear();
     if (!args) {
@@ -805,6 +906,36 @@ int noa_hip_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_ar
     return guarded(device, [&](std::string& err) {
         return launch_insert_extract(d, static_cast<hipStream_t>(stream), err);
     });
+}
+int noa_hip_ctf_anisotropic_2d(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                               int32_t remap, const noa_hip_ctf* ctf, const noa_hip_ctf_options* options,
+                               void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    CtfDesc d;
+    try {
+        d = validate_ctf(input, output, nullptr, shape, remap, ctf, options, false, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) { return launch_ctf(d, static_cast<hipStream_t>(stream), err); });
+}
+int noa_hip_ctf_anisotropic_2d_weighted(const noa_hip_view* input, const noa_hip_view* output,
+                                        const noa_hip_view* output_weight, const int64_t shape[4], int32_t remap,
+                                        const noa_hip_ctf* ctf, const noa_hip_ctf_options* options, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    CtfDesc d;
+    try {
+        d = validate_ctf(input, output, output_weight, shape, remap, ctf, options, true, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) { return launch_ctf(d, static_cast<hipStream_t>(stream), err); });
 }
 int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
                   void* stream) {

@@ -1,0 +1,175 @@
+// noa::signal::ctf_anisotropic for 2D rfft layouts (src/noa/signal/CTF.hpp:14-125 operator, :430-500 public
+// function) and the CTFAnisotropic<T> model it evaluates (src/noa/signal/core/CTF.hpp:291-575), plus a fused
+// form that writes input * CTF and CTF^2 in one pass: the prologue of the CTF-weighted insertion, where noa calls
+// the operator twice (once on the slices, once to write the squared CTF as the weights).
+//
+// Elementwise and HBM-bound: one lane per rfft element. The CTF is evaluated once per element, in the CTF's
+// precision, with noa's expressions in noa's order.
+#include <cstring>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace noa_hip {
+
+// The members of CTFAnisotropic<T> in declaration order (core/CTF.hpp:564-575): the layout of noa's objects, so a
+// device array of noa CTFs is read as is.
+template<class T>
+struct alignas(2 * sizeof(T)) CtfAniso {
+    T pixel_size[2];
+    T defocus_value, defocus_astigmatism, defocus_angle;   // angstroms (underfocus negative), radians
+    T phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+};
+static_assert(sizeof(CtfAniso<float>) == 56 && sizeof(CtfAniso<double>) == 112, "noa's CTFAnisotropic layout");
+
+// CTFAnisotropic::value_at (core/CTF.hpp:473-488) with phi2defocus_ (:540-543), cartesian2phi<false> and
+// cartesian2rho (xform/core/Polar.hpp:100-135).
+template<class T>
+__device__ __forceinline__ T ctf_value_at(const CtfAniso<T>& c, T fy, T fx) {
+    const T sy = fy / c.pixel_size[0], sx = fx / c.pixel_size[1];
+    const T phi = atan2(sy, sx);
+    const T rho = sqrt(sy * sy + sx * sx);
+    const T r2 = rho * rho;
+    const T r4 = r2 * r2;
+    const T defocus = c.defocus_value + c.defocus_astigmatism * cos(2 * (phi - c.defocus_angle));
+    const T phase = c.k1 * r2 * defocus + c.k2 * r4 - c.phase_shift - c.k3;
+    T ctf = -sin(phase);
+    if (c.bfactor_forth != 0) ctf *= exp(c.bfactor_forth * r2);
+    return ctf * c.scale;
+}
+
+template<class T, class In, class Out, class W>
+struct CtfParams {
+    const In* in; i64 is_b; i32 is_y, is_x;     // null: the CTF itself is written
+    Out* out; i64 os_b; i32 os_y, os_x;
+    W* wout; i64 ws_b; i32 ws_y, ws_x;          // fused form: CTF^2 (null otherwise)
+    const CtfAniso<T>* ctfs;                    // batched (device), or null: `single`
+    CtfAniso<T> single;
+    u32 h, wr, n;                               // logical height, rfft width, elements (batch * h * wr)
+    T step[2];                                  // fftfreq per frequency along y and x
+    bool src_centered, dst_centered, ctf_abs, ctf_squared;
+};
+
+// One lane per element, x fastest: loads and stores are contiguous along the rows.
+template<class T, class In, class Out, class W, bool HAS_INPUT>
+__global__ __launch_bounds__(256) void ctf_kernel(const CtfParams<T, In, Out, W> p) {
+    using RIn = typename vt<In>::real;
+    const u32 t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= p.n) return;
+    const u32 row = t / p.wr, x = t - row * p.wr;
+    const u32 b = row / p.h, y = row - b * p.h;
+    const CtfAniso<T> c = p.ctfs ? p.ctfs[b] : p.single;
+    const i32 fy = index2frequency(p.dst_centered, static_cast<i32>(y), static_cast<i32>(p.h));
+    // fftfreq = frequency * step (signal/CTF.hpp:84-85), then value_at in the CTF's precision.
+    T ctf = ctf_value_at(c, static_cast<T>(fy) * p.step[0], static_cast<T>(x) * p.step[1]);
+    if constexpr (HAS_INPUT) {
+        if (p.wout) {
+            // Fused form: the value is multiplied by the CTF, the weight is the squared CTF.
+            const i32 iy = frequency2index(p.src_centered, fy, static_cast<i32>(p.h));   // remap_indices<REMAP, true>
+            const In v = p.in[static_cast<i64>(b) * p.is_b + static_cast<i64>(iy) * p.is_y + static_cast<i64>(x) * p.is_x];
+            p.out[static_cast<i64>(b) * p.os_b + static_cast<i64>(y) * p.os_y + static_cast<i64>(x) * p.os_x] =
+                vcast<Out>(vmul(v, static_cast<RIn>(ctf)));
+            p.wout[static_cast<i64>(b) * p.ws_b + static_cast<i64>(y) * p.ws_y + static_cast<i64>(x) * p.ws_x] =
+                static_cast<W>(ctf * ctf);
+            return;
+        }
+    }
+    if (p.ctf_squared) ctf *= ctf;
+    else if (p.ctf_abs) ctf = fabs(ctf);
+    Out* const o = p.out + (static_cast<i64>(b) * p.os_b + static_cast<i64>(y) * p.os_y + static_cast<i64>(x) * p.os_x);
+    if constexpr (HAS_INPUT) {
+        const i32 iy = frequency2index(p.src_centered, fy, static_cast<i32>(p.h));
+        const In v = p.in[static_cast<i64>(b) * p.is_b + static_cast<i64>(iy) * p.is_y + static_cast<i64>(x) * p.is_x];
+        *o = vcast<Out>(vmul(v, static_cast<RIn>(ctf)));
+    } else {
+        Out value = vzero<Out>();
+        if constexpr (vt<Out>::cx) value.x = static_cast<typename vt<Out>::real>(ctf);
+        else value = static_cast<Out>(ctf);
+        *o = value;
+    }
+}
+
+namespace {
+
+// Linspace{start, stop <= 0 ? highest_fftfreq(n) : stop, endpoint}.for_size(n / 2 + 1).step (signal/CTF.hpp:65-77,
+// runtime/core/Iwise.hpp:67-78, fft/core/Frequency.hpp:9-20), in the coordinate precision.
+template<class T>
+T fftfreq_step(i64 n, double start, double stop, bool endpoint) {
+    const i64 size = n / 2 + 1;
+    const T t_start = static_cast<T>(start), t_stop_opt = static_cast<T>(stop);
+    const T t_stop = t_stop_opt <= 0 ? static_cast<T>(n / 2) / static_cast<T>(n) : t_stop_opt;
+    const i64 count = std::max<i64>(1, size - (endpoint ? 1 : 0));
+    return (t_stop - t_start) / static_cast<T>(count);
+}
+
+template<class T, class In, class Out, class W>
+int run(const CtfDesc& d, hipStream_t stream, std::string& err) {
+    CtfParams<T, In, Out, W> p{};
+    p.in = static_cast<const In*>(d.input.data);
+    p.is_b = d.input.s0; p.is_y = static_cast<i32>(d.input.s1); p.is_x = static_cast<i32>(d.input.s2);
+    p.out = static_cast<Out*>(d.output.data);
+    p.os_b = d.output.s0; p.os_y = static_cast<i32>(d.output.s1); p.os_x = static_cast<i32>(d.output.s2);
+    p.wout = static_cast<W*>(d.output_weight.data);
+    p.ws_b = d.output_weight.s0; p.ws_y = static_cast<i32>(d.output_weight.s1); p.ws_x = static_cast<i32>(d.output_weight.s2);
+    p.ctfs = d.ctf_batched ? static_cast<const CtfAniso<T>*>(d.ctf_data) : nullptr;
+    if (!d.ctf_batched) std::memcpy(&p.single, d.ctf_single, sizeof(CtfAniso<T>));
+    p.h = static_cast<u32>(d.shape[2]);
+    p.wr = static_cast<u32>(d.shape[3] / 2 + 1);
+    const i64 n = d.shape[0] * d.shape[2] * (d.shape[3] / 2 + 1);
+    if (n > 0x7fffffffLL) { err = "ctf: more than 2^31 elements in one call"; return NOA_HIP_ERROR_UNSUPPORTED; }
+    p.n = static_cast<u32>(n);
+    p.step[0] = fftfreq_step<T>(d.shape[2], d.range_start, d.range_stop, d.endpoint);
+    p.step[1] = fftfreq_step<T>(d.shape[3], d.range_start, d.range_stop, d.endpoint);
+    p.src_centered = d.src_centered;
+    p.dst_centered = d.dst_centered;
+    p.ctf_abs = d.ctf_abs;
+    p.ctf_squared = d.ctf_squared;
+    const i64 blocks = (n + 255) / 256;
+    if (blocks == 0) return NOA_HIP_OK;
+    const dim3 grid(static_cast<u32>(blocks)), block(256);
+    if (p.in) {
+        if constexpr (vt<In>::cx || !vt<Out>::cx) hipLaunchKernelGGL((ctf_kernel<T, In, Out, W, true>), grid, block, 0, stream, p);
+    } else {
+        hipLaunchKernelGGL((ctf_kernel<T, In, Out, W, false>), grid, block, 0, stream, p);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+    return NOA_HIP_OK;
+}
+
+template<class T>
+int dispatch(const CtfDesc& d, hipStream_t stream, std::string& err) {
+    const int in = d.input.data ? d.input.dtype : NOA_HIP_DTYPE_NONE, out = d.output.dtype;
+    switch (in) {
+        case NOA_HIP_C32:
+            if (out == NOA_HIP_C32) return run<T, float2, float2, float>(d, stream, err);
+            if (out == NOA_HIP_F32) return run<T, float2, float, float>(d, stream, err);
+            break;
+        case NOA_HIP_F32:
+            if (out == NOA_HIP_F32) return run<T, float, float, float>(d, stream, err);
+            break;
+        case NOA_HIP_C64:
+            if (out == NOA_HIP_C64) return run<T, double2, double2, double>(d, stream, err);
+            if (out == NOA_HIP_F64) return run<T, double2, double, double>(d, stream, err);
+            break;
+        case NOA_HIP_F64:
+            if (out == NOA_HIP_F64) return run<T, double, double, double>(d, stream, err);
+            break;
+        default:   // no input: the CTF is written as a real or complex value
+            if (out == NOA_HIP_F32) return run<T, float, float, float>(d, stream, err);
+            if (out == NOA_HIP_C32) return run<T, float, float2, float>(d, stream, err);
+            if (out == NOA_HIP_F64) return run<T, double, double, double>(d, stream, err);
+            if (out == NOA_HIP_C64) return run<T, double, double2, double>(d, stream, err);
+            break;
+    }
+    err = "ctf: unsupported input/output value types";
+    return NOA_HIP_ERROR_UNSUPPORTED;
+}
+
+} // namespace
+
+int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err) {
+    return d.ctf_f64 ? dispatch<double>(d, stream, err) : dispatch<float>(d, stream, err);
+}
+
+} // namespace noa_hip

@@ -73,6 +73,20 @@ struct InsertExtractDesc {
     bool coord_f64{};
 };
 
+// noa::signal::ctf_anisotropic over (B, 1, H, W/2+1) rfft arrays; with `output_weight`, the fused form
+// (output = input * ctf, output_weight = ctf^2).
+struct CtfDesc {
+    Arr input, output, output_weight;    // (batch, y, x) strides; input.data == nullptr: write the CTF
+    i64 shape[4]{};                      // logical BDHW
+    bool src_centered{}, dst_centered{};
+    bool ctf_abs{}, ctf_squared{};
+    double range_start{}, range_stop{-1};
+    bool endpoint{true};
+    bool ctf_f64{}, ctf_batched{};
+    const void* ctf_data{};              // device array of CTFAnisotropic<T> when batched
+    alignas(16) unsigned char ctf_single[112]{};   // one CTFAnisotropic<T>, copied from the host
+};
+
 // A phase shift over (B, 1, H, W/2+1) rfft arrays.
 struct PhaseShiftDesc {
     Arr input, output;    // (batch, y, x) strides; input.data == nullptr: write the phase shifts
@@ -116,5 +130,6 @@ int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
+int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err);
 
 } // namespace noa_hip

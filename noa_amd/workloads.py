@@ -88,6 +88,8 @@ class InsertWorkload:
     volume_shape: tuple
     inv_rotations: Mat33Array      # inverse rotations (insert_central_slices_3d takes R^T)
     poses: np.ndarray              # forward rotations, host copy
+    raw: torch.Tensor | None = None       # (B, 1, N, N//2+1) complex64 spectra before the CTF (keep_raw)
+    ctfs: torch.Tensor | None = None      # (B, 56) uint8: the slices' CTFAnisotropic<f32> objects (keep_raw)
 
 
 def extract_256(device="cuda", n: int = 256, b: int = 4096, seed: int = SEED_BASE + 2) -> ExtractWorkload:
@@ -108,7 +110,7 @@ def extract_256(device="cuda", n: int = 256, b: int = 4096, seed: int = SEED_BAS
 
 
 def insert_256(device="cuda", n: int = 256, b: int = 10_000, seed: int = SEED_BASE + 3,
-               chunk: int = 1000) -> InsertWorkload:
+               chunk: int = 1000, keep_raw: bool = False) -> InsertWorkload:
     g = torch.Generator(device=device).manual_seed(seed)
     rng = np.random.default_rng(seed)
     poses = uniform_rotations(b, rng)
@@ -117,6 +119,7 @@ def insert_256(device="cuda", n: int = 256, b: int = 10_000, seed: int = SEED_BA
     angle = rng.uniform(0.0, np.pi, b)
     slices = torch.empty((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=device)
     weights = torch.empty((b, 1, n, n // 2 + 1), dtype=torch.float32, device=device)
+    raw = torch.empty_like(slices) if keep_raw else None
     for s in range(0, b, chunk):
         e = min(b, s + chunk)
         img = torch.randn((e - s, n, n), generator=g, device=device)
@@ -124,9 +127,16 @@ def insert_256(device="cuda", n: int = 256, b: int = 10_000, seed: int = SEED_BA
         ctf = ctf_anisotropic((n, n), defocus[s:e], astig[s:e], angle[s:e], device=device)
         slices[s:e, 0] = spec * ctf
         weights[s:e, 0] = ctf * ctf
+        if keep_raw:
+            raw[s:e, 0] = spec
         del img, spec, ctf
     inv = np.ascontiguousarray(np.transpose(poses, (0, 2, 1)))
-    return InsertWorkload(slices, weights, (b, 1, n, n), (1, n, n, n), Mat33Array(inv, device), poses)
+    ctfs = None
+    if keep_raw:
+        from .signal import CTFAnisotropic, ctf_array
+        ctfs = ctf_array([CTFAnisotropic((1.0, 1.0), (defocus[i], astig[i], angle[i]), 300.0, 0.07, 2.7)
+                          for i in range(b)], device).reshape(b, 56)
+    return InsertWorkload(slices, weights, (b, 1, n, n), (1, n, n, n), Mat33Array(inv, device), poses, raw, ctfs)
 
 
 def correctness_64(device="cuda", n: int = 64, b: int = 64, seed: int = SEED_BASE + 1):

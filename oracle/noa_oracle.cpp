@@ -10,6 +10,7 @@
  *   - FourierInsertInterpolate src/noa/xform/FourierProject.hpp:392-539
  *   - FourierExtract (3D, 4D)  src/noa/xform/FourierProject.hpp:541-725
  *   - FourierInsertExtract     src/noa/xform/FourierProject.hpp:727-1006 (launcher :1489-1598)
+ *   - CTFAnisotropic, CTF op   src/noa/signal/core/CTF.hpp:12-25, :291-575; src/noa/signal/CTF.hpp:14-125
  *   - windowed sinc helpers    src/noa/xform/FourierProject.hpp:97-159
  *   - slice<->grid transforms  src/noa/xform/FourierProject.hpp:38-95
  *   - interpolate_spectrum     src/noa/xform/core/Interpolation.hpp:416-526 (+ weights :64-146)
@@ -1173,6 +1174,107 @@ void remap_hx2hx(const T* input, T* output, const i64 shape[4], int remap) {
         }
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// noa::signal::CTFAnisotropic<T> (src/noa/signal/core/CTF.hpp:291-575) and the CTF operator for 2D rfft layouts
+// (src/noa/signal/CTF.hpp:14-125, public function :430-500).
+// ------------------------------------------------------------------------------------------------------------
+template<class T>
+struct alignas(2 * sizeof(T)) CtfAniso {   // members in declaration order (:564-575)
+    T pixel_size[2];
+    T defocus_value, defocus_astigmatism, defocus_angle;
+    T phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+};
+
+// relativistic_electron_wavelength (core/CTF.hpp:14-25), constants of base/Math.hpp:13-17.
+inline double relativistic_electron_wavelength(double energy) {
+    const double h = 6.62607015e-34, c = 299792458.0, m0 = 9.1093837015e-31, e = 1.602176634e-19;
+    const double eV = e * energy;
+    const double numerator = h * c;
+    const double denominator = std::sqrt(eV * (2 * m0 * c * c + eV));
+    return numerator / denominator;
+}
+
+// The constructor from physical parameters (:318-345): pixel size (A/px), defocus (um, positive underfocus),
+// astigmatism (um), its angle (rad), voltage (kV), amplitude fraction, cs (mm), phase shift (rad), bfactor (A^2),
+// scale.
+template<class T>
+CtfAniso<T> make_ctf(const T pixel_size[2], T defocus, T astigmatism, T angle, T voltage, T amplitude, T cs,
+                     T phase_shift, T bfactor, T scale) {
+    CtfAniso<T> c{};
+    c.pixel_size[0] = pixel_size[0];
+    c.pixel_size[1] = pixel_size[1];
+    c.phase_shift = phase_shift;
+    c.cs = cs * static_cast<T>(1e7);
+    c.voltage = voltage * static_cast<T>(1e3);
+    c.amplitude = amplitude;
+    c.scale = scale;
+    c.defocus_value = -defocus * static_cast<T>(1e4);         // set_defocus (:405-411)
+    c.defocus_astigmatism = -astigmatism * static_cast<T>(1e4);
+    c.defocus_angle = angle;
+    c.bfactor_forth = bfactor / 4;                             // set_bfactor (:401-404)
+    const double lambda = relativistic_electron_wavelength(static_cast<double>(c.voltage)) * 1e10;   // (:545-551)
+    const double PI = 3.1415926535897932384626433832795;
+    c.k1 = static_cast<T>(PI * lambda);
+    c.k2 = static_cast<T>(PI * 0.5 * static_cast<double>(c.cs) * lambda * lambda * lambda);
+    c.k3 = std::atan(c.amplitude / std::sqrt(1 - c.amplitude * c.amplitude));   // (:553-555)
+    return c;
+}
+
+// value_at (:473-488) with phi2defocus_ (:540-543), cartesian2phi<false> / cartesian2rho (xform/core/Polar.hpp).
+template<class T>
+T ctf_value_at(const CtfAniso<T>& c, T fy, T fx) {
+    const T sy = fy / c.pixel_size[0], sx = fx / c.pixel_size[1];
+    const T phi = std::atan2(sy, sx);
+    const T rho = std::sqrt(sy * sy + sx * sx);
+    const T r1 = rho;
+    const T r2 = r1 * r1;
+    const T r4 = r2 * r2;
+    const T defocus = c.defocus_value + c.defocus_astigmatism * std::cos(2 * (phi - c.defocus_angle));
+    const T phase = c.k1 * r2 * defocus + c.k2 * r4 - c.phase_shift - c.k3;
+    T ctf = -std::sin(phase);
+    if (c.bfactor_forth != 0) ctf *= std::exp(c.bfactor_forth * r2);
+    return ctf * c.scale;
+}
+
+// The operator on contiguous (b, h, w/2+1) arrays; `input` may be null (the CTF is written), and has a batch of
+// `in_batch` (1: broadcast). Linspace::for_size (runtime/core/Iwise.hpp:67-78) gives the fftfreq step per axis.
+template<class T, class In, class Out, bool HAS_INPUT>
+void ctf_anisotropic_2d(const In* input, i64 in_batch, Out* output, i64 b, i64 h, i64 w, int remap,
+                        const CtfAniso<T>* ctfs, bool batched, double start, double stop, bool endpoint,
+                        bool ctf_abs, bool ctf_squared) {
+    const bool src_c = (remap & 4) != 0, dst_c = (remap & 8) != 0;
+    const i64 wr = w / 2 + 1;
+    T step[2];
+    const i64 dims[2] = {h, w};
+    for (int i = 0; i < 2; ++i) {
+        const i64 size = dims[i] / 2 + 1;
+        const T t_start = static_cast<T>(start), t_stop_opt = static_cast<T>(stop);
+        const T t_stop = t_stop_opt <= 0 ? static_cast<T>(dims[i] / 2) / static_cast<T>(dims[i]) : t_stop_opt;
+        const i64 count = std::max<i64>(1, size - (endpoint ? 1 : 0));
+        step[i] = (t_stop - t_start) / static_cast<T>(count);
+    }
+    using RIn = typename real_of<In>::type;
+    for (i64 bi = 0; bi < b; ++bi) {
+        const CtfAniso<T>& c = ctfs[batched ? bi : 0];
+        for (i64 y = 0; y < h; ++y) {
+            const i64 fy = index2frequency(dst_c, y, h);
+            const i64 iy = frequency2index(src_c, fy, h);
+            for (i64 x = 0; x < wr; ++x) {
+                T ctf = ctf_value_at(c, static_cast<T>(fy) * step[0], static_cast<T>(x) * step[1]);
+                if (ctf_squared) ctf *= ctf;
+                else if (ctf_abs) ctf = std::abs(ctf);
+                Out& o = output[(bi * h + y) * wr + x];
+                if constexpr (HAS_INPUT) {
+                    o = cast_or_abs_squared<Out>(input[((in_batch == 1 ? 0 : bi) * h + iy) * wr + x] * static_cast<RIn>(ctf));
+                } else {
+                    if constexpr (is_cx<Out>::value) o = Out{static_cast<typename real_of<Out>::type>(ctf), 0};
+                    else o = static_cast<Out>(ctf);
+                }
+            }
+        }
+    }
+}
+
 } // namespace oracle
 
 extern "C" {
@@ -1244,6 +1346,65 @@ int oracle_remap_hx2hx(const void* input, void* output, int elem_bytes, const lo
     else return NOA_HIP_ERROR_UNSUPPORTED;
     return 0;
 }
+
+/* noa::signal::ctf_anisotropic<REMAP> on contiguous (b, h, w/2+1) arrays. dtypes: 1 f32, 2 f64, 3 c32, 4 c64
+ * (input 0: none); ctf_f64 selects CTFAnisotropic<f64>; `ctfs` holds one (batched == 0) or b objects. */
+int oracle_ctf_anisotropic_2d(const void* input, int in_dtype, long long in_batch, void* output, int out_dtype,
+                              long long b, long long h, long long w, int remap, const void* ctfs, int ctf_f64,
+                              int batched, double start, double stop, int endpoint, int ctf_abs, int ctf_squared) {
+    using namespace oracle;
+    auto go = [&](auto t_tag) -> int {
+        using T = decltype(t_tag);
+        const auto* c = static_cast<const CtfAniso<T>*>(ctfs);
+        auto run = [&](auto in_tag, auto out_tag) {
+            using In = decltype(in_tag);
+            using Out = decltype(out_tag);
+            if (in_dtype == 0)
+                ctf_anisotropic_2d<T, In, Out, false>(nullptr, in_batch, static_cast<Out*>(output), b, h, w, remap, c,
+                                                      batched != 0, start, stop, endpoint != 0, ctf_abs != 0,
+                                                      ctf_squared != 0);
+            else if constexpr (is_cx<In>::value || !is_cx<Out>::value)
+                ctf_anisotropic_2d<T, In, Out, true>(static_cast<const In*>(input), in_batch, static_cast<Out*>(output), b, h, w,
+                                                     remap, c, batched != 0, start, stop, endpoint != 0, ctf_abs != 0,
+                                                     ctf_squared != 0);
+            return 0;
+        };
+        if (in_dtype == 3 && out_dtype == 3) return run(Cx<float>{}, Cx<float>{});
+        if (in_dtype == 3 && out_dtype == 1) return run(Cx<float>{}, float{});
+        if (in_dtype == 1 && out_dtype == 1) return run(float{}, float{});
+        if (in_dtype == 4 && out_dtype == 4) return run(Cx<double>{}, Cx<double>{});
+        if (in_dtype == 4 && out_dtype == 2) return run(Cx<double>{}, double{});
+        if (in_dtype == 2 && out_dtype == 2) return run(double{}, double{});
+        if (in_dtype == 0 && out_dtype == 1) return run(float{}, float{});
+        if (in_dtype == 0 && out_dtype == 2) return run(double{}, double{});
+        if (in_dtype == 0 && out_dtype == 3) return run(float{}, Cx<float>{});
+        if (in_dtype == 0 && out_dtype == 4) return run(double{}, Cx<double>{});
+        g_error = "oracle: unsupported CTF value types";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    };
+    return ctf_f64 ? go(double{}) : go(float{});
+}
+
+/* CTFAnisotropic<T> from physical parameters, written to `out` (56 or 112 bytes). */
+void oracle_ctf_anisotropic_make(int f64, const double pixel_size[2], double defocus, double astigmatism,
+                                 double angle, double voltage, double amplitude, double cs, double phase_shift,
+                                 double bfactor, double scale, void* out) {
+    using namespace oracle;
+    if (f64) {
+        const double ps[2] = {pixel_size[0], pixel_size[1]};
+        const auto c = make_ctf<double>(ps, defocus, astigmatism, angle, voltage, amplitude, cs, phase_shift, bfactor, scale);
+        std::memcpy(out, &c, sizeof(c));
+    } else {
+        const float ps[2] = {static_cast<float>(pixel_size[0]), static_cast<float>(pixel_size[1])};
+        const auto c = make_ctf<float>(ps, static_cast<float>(defocus), static_cast<float>(astigmatism),
+                                       static_cast<float>(angle), static_cast<float>(voltage),
+                                       static_cast<float>(amplitude), static_cast<float>(cs),
+                                       static_cast<float>(phase_shift), static_cast<float>(bfactor),
+                                       static_cast<float>(scale));
+        std::memcpy(out, &c, sizeof(c));
+    }
+}
+double oracle_relativistic_electron_wavelength(double volts) { return oracle::relativistic_electron_wavelength(volts); }
 
 /* Pinned helpers, exposed for the oracle's own tests. */
 long long oracle_index2frequency(int centered, long long index, long long size) {

@@ -362,3 +362,54 @@ def remap_hx2hx(remap, input, output, shape):
                                       inp.itemsize, sh, LAYOUTS[remap.lower()] if isinstance(remap, str) else remap)
     if status != 0:
         raise OracleError("remap_hx2hx: unsupported element size")
+
+
+_CTF_DT = {np.dtype(np.float32): 1, np.dtype(np.float64): 2, np.dtype(np.complex64): 3, np.dtype(np.complex128): 4}
+
+
+def make_ctf(pixel_size, defocus, voltage, amplitude, cs, phase_shift=0.0, bfactor=0.0, scale=1.0, f64=False):
+    """noa::signal::CTFAnisotropic<T> from physical parameters (signal/core/CTF.hpp:318-345), as raw bytes.
+    defocus = (value um, astigmatism um, angle rad)."""
+    L = lib()
+    L.oracle_ctf_anisotropic_make.argtypes = [ctypes.c_int, c_void_p] + [c_double] * 9 + [c_void_p]
+    L.oracle_ctf_anisotropic_make.restype = None
+    ps = (c_double * 2)(*pixel_size)
+    buf = ctypes.create_string_buffer(112 if f64 else 56)
+    L.oracle_ctf_anisotropic_make(int(f64), ctypes.addressof(ps), defocus[0], defocus[1], defocus[2], voltage,
+                                  amplitude, cs, phase_shift, bfactor, scale, ctypes.addressof(buf))
+    return bytes(buf.raw)
+
+
+def relativistic_electron_wavelength(volts):
+    L = lib()
+    L.oracle_relativistic_electron_wavelength.argtypes = [c_double]
+    L.oracle_relativistic_electron_wavelength.restype = c_double
+    return L.oracle_relativistic_electron_wavelength(volts)
+
+
+def ctf_anisotropic_2d(remap, input, output, shape, ctfs, *, fftfreq_range=(0.0, -1.0, True), ctf_abs=False,
+                       ctf_squared=False):
+    """noa::signal::ctf_anisotropic<REMAP> (signal/CTF.hpp:14-125) on contiguous (B, 1, H, W/2+1) arrays.
+    ctfs: one CTF (bytes from make_ctf) or a list of B of them; input None writes the CTF."""
+    out = np.asarray(output)
+    if not out.flags.c_contiguous:
+        raise OracleError("ctf_anisotropic_2d: contiguous output expected")
+    b, h, w = shape[0], shape[2], shape[3]
+    items = ctfs if isinstance(ctfs, (list, tuple)) else [ctfs]
+    f64 = len(items[0]) == 112
+    raw = ctypes.create_string_buffer(b"".join(items), len(items) * len(items[0]))
+    inp = None if input is None else np.ascontiguousarray(input)
+    L = lib()
+    L.oracle_ctf_anisotropic_2d.argtypes = [c_void_p, ctypes.c_int, ctypes.c_longlong, c_void_p, ctypes.c_int,
+                                            ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
+                                            c_void_p, ctypes.c_int, ctypes.c_int, c_double, c_double, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int]
+    L.oracle_ctf_anisotropic_2d.restype = ctypes.c_int
+    status = L.oracle_ctf_anisotropic_2d(
+        None if inp is None else inp.ctypes.data, 0 if inp is None else _CTF_DT[inp.dtype],
+        1 if inp is None else inp.shape[0], out.ctypes.data, _CTF_DT[out.dtype], b, h, w,
+        LAYOUTS[remap.lower()] if isinstance(remap, str) else remap, ctypes.addressof(raw), int(f64),
+        int(len(items) > 1), float(fftfreq_range[0]), float(fftfreq_range[1]),
+        int(bool(fftfreq_range[2]) if len(fftfreq_range) > 2 else 1), int(bool(ctf_abs)), int(bool(ctf_squared)))
+    if status != 0:
+        raise OracleError(lib().oracle_last_error().decode())

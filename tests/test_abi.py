@@ -53,6 +53,10 @@ def test_struct_layout_matches_the_c_compiler(tmp_path):
         "noa_hip_project_args": ("ProjectArgs", [f[0] for f in abi.ProjectArgs._fields_]),
         "noa_hip_insert_extract_args": ("InsertExtractArgs", [f[0] for f in abi.InsertExtractArgs._fields_]),
         "noa_hip_shift2": ("Shift2", [f[0] for f in abi.Shift2._fields_]),
+        "noa_hip_ctf_anisotropic_f32": ("CtfAnisotropicF32", [f[0] for f in abi.CtfAnisotropicF32._fields_]),
+        "noa_hip_ctf_anisotropic_f64": ("CtfAnisotropicF64", [f[0] for f in abi.CtfAnisotropicF64._fields_]),
+        "noa_hip_ctf": ("Ctf", [f[0] for f in abi.Ctf._fields_]),
+        "noa_hip_ctf_options": ("CtfOptions", [f[0] for f in abi.CtfOptions._fields_]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "noa_hip.h"', "int main(void) {"]
     for c_name, (_, names) in fields.items():
@@ -235,3 +239,38 @@ def test_cpp_facade_on_the_gpu():
     res = subprocess.run([str(FACADE_TEST), "--gpu"], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "ok gpu extraction" in res.stdout and "ok gpu insertion" in res.stdout
+
+
+def test_ctf_struct_sizes_are_noas():
+    """CTFAnisotropic<f32> is 56 bytes and CTFAnisotropic<f64> 112 bytes (Vec<f64,2> is 16-byte aligned)."""
+    from noa_amd import abi
+    assert ctypes.sizeof(abi.CtfAnisotropicF32) == 56
+    assert ctypes.sizeof(abi.CtfAnisotropicF64) == 112
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda k: k["shape"].__setitem__(1, 2), "2d arrays"),
+    (lambda k: k.__setitem__("remap", 1), "rfft layouts"),
+    (lambda k: setattr(k["ctf"], "count", 3), "contiguous vector"),
+    (lambda k: setattr(k["opt"], "fftfreq_start", 0.1), "starting fftfreq"),
+    (lambda k: setattr(k["ctf"], "dtype", 3), "CTFAnisotropic"),
+])
+def test_ctf_validation_errors(lib, mutate, message):
+    from noa_amd import abi
+    b, h, w = 2, 16, 16
+    out = np.zeros((b, 1, h, w // 2 + 1), np.float32)
+    ctfs = np.zeros((b, 14), np.float32)
+    v = abi.View()
+    v.data = out.ctypes.data
+    for i in range(4):
+        v.shape[i] = out.shape[i]
+        v.strides[i] = out.strides[i] // 4
+    v.dtype = abi.F32
+    c = abi.Ctf(ctfs.ctypes.data, abi.F32, 1, b)
+    k = {"shape": (ctypes.c_int64 * 4)(b, 1, h, w), "remap": abi.H2H, "ctf": c, "opt": abi.CtfOptions(0, -1, 1, 0, 0)}
+    mutate(k)
+    status = lib.noa_hip_ctf_anisotropic_2d(None, ctypes.byref(v), k["shape"], k["remap"], ctypes.byref(k["ctf"]),
+                                            ctypes.byref(k["opt"]), None)
+    err = lib.noa_hip_last_error().decode()
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
+    assert message in err, err

@@ -785,3 +785,77 @@ def test_insert_extract_matrix_equals_quaternion_reference():
     a0, b0 = np.where(keep, a[0, 0], 0), np.where(keep, b[0, 0], 0)
     assert np.abs(a0 - b0).max() <= 5e-5 * np.abs(a0).max()
     assert rel_l2(b0, a0) < 1e-5
+
+
+# ------------------------------------------------------------------------------------------------ CTF
+# noa::signal::CTFAnisotropic and ctf_anisotropic (signal/core/CTF.hpp:291-575, signal/CTF.hpp:14-125).
+@pytest.mark.parametrize("kv, angstrom", [(100, 0.0370144), (200, 0.0250793), (300, 0.0196875)])
+def test_electron_wavelength_textbook_values(kv, angstrom):
+    assert O.relativistic_electron_wavelength(kv * 1e3) * 1e10 == pytest.approx(angstrom, rel=2e-6)
+
+
+def _ctf_physics(n_y, n_x, centered, pixel_size, defocus, voltage, amplitude, cs, phase_shift, bfactor, scale):
+    """The CTF from the physics, float64: sqrt(1 - A^2) sin(g) + A cos(g), g = pi lambda dz k^2 - pi/2 Cs lambda^3 k^4
+    + phase shift, with the astigmatic defocus dz(phi) = dz + da cos(2 (phi - angle)), phi = atan2(ky, kx), k in
+    1/A, times exp(B k^2 / 4) and the scale."""
+    lam = O.relativistic_electron_wavelength(voltage * 1e3) * 1e10
+    fy = freq_grid(n_y, centered)[:, None] / n_y
+    fx = np.arange(n_x // 2 + 1)[None, :] / n_x
+    ky, kx = fy / pixel_size[0], fx / pixel_size[1]
+    k2 = ky ** 2 + kx ** 2
+    phi = np.arctan2(ky, kx)
+    dz = (defocus[0] + defocus[1] * np.cos(2 * (phi - defocus[2]))) * 1e4
+    g = np.pi * lam * dz * k2 - np.pi / 2 * (cs * 1e7) * lam ** 3 * k2 ** 2 + phase_shift
+    return (np.sqrt(1 - amplitude ** 2) * np.sin(g) + amplitude * np.cos(g)) * np.exp(bfactor / 4 * k2) * scale
+
+
+@pytest.mark.parametrize("shape, centered", [((64, 64), False), ((48, 40), True), ((33, 35), True)])
+def test_ctf_matches_the_physics(shape, centered):
+    """Anisotropic pixel size, astigmatism, phase plate, B-factor and scale against the textbook formula (f64)."""
+    params = dict(pixel_size=(1.1, 0.9), defocus=(1.8, 0.08, 0.4), voltage=300, amplitude=0.07, cs=2.7,
+                  phase_shift=0.3, bfactor=-20.0, scale=1.5)
+    ctf = O.make_ctf(**params, f64=True)
+    h, w = shape
+    out = np.zeros((1, 1, h, w // 2 + 1), np.float64)
+    O.ctf_anisotropic_2d("hc2hc" if centered else "h2h", None, out, (1, 1, h, w), ctf)
+    want = _ctf_physics(h, w, centered, **params)
+    # The fftfreq step is (n/2)/n / (n/2) = 1/n for even n; for odd n it is ((n-1)/2/n) / ((n-1)/2), also 1/n.
+    np.testing.assert_allclose(out[0, 0], want, rtol=1e-9, atol=1e-9)
+
+
+def test_ctf_options_layouts_and_input():
+    rng = np.random.default_rng(50)
+    b, h, w = 3, 32, 30
+    ctfs = [O.make_ctf((1.0, 1.0), (0.5 + i, 0.05 * i, 0.2 * i), 300, 0.07, 2.7) for i in range(b)]
+    plain = np.zeros((b, 1, h, w // 2 + 1), np.float32)
+    O.ctf_anisotropic_2d("h2h", None, plain, (b, 1, h, w), ctfs)
+    for i in range(b):   # batched CTFs == one call per batch with the single CTF
+        one = np.zeros((1, 1, h, w // 2 + 1), np.float32)
+        O.ctf_anisotropic_2d("h2h", None, one, (1, 1, h, w), ctfs[i])
+        np.testing.assert_array_equal(one[0], plain[i])
+    sq = np.zeros_like(plain)
+    O.ctf_anisotropic_2d("h2h", None, sq, (b, 1, h, w), ctfs, ctf_squared=True)
+    np.testing.assert_array_equal(sq, plain * plain)
+    ab = np.zeros_like(plain)
+    O.ctf_anisotropic_2d("h2h", None, ab, (b, 1, h, w), ctfs, ctf_abs=True)
+    np.testing.assert_array_equal(ab, np.abs(plain))
+    # Output layout: "h2hc" writes the centred layout of the same values.
+    cen = np.zeros_like(plain)
+    O.ctf_anisotropic_2d("h2hc", None, cen, (b, 1, h, w), ctfs)
+    np.testing.assert_array_equal(cen, h2hc(plain, 2))
+    # Input: multiplied element by element, read through the remap; complex -> real is the power spectrum.
+    x = random_spectrum((b, 1, h, w // 2 + 1), rng)
+    y = np.zeros_like(x)
+    O.ctf_anisotropic_2d("hc2h", h2hc(x, 2), y, (b, 1, h, w), ctfs)
+    np.testing.assert_array_equal(y, x * plain)
+    p = np.zeros_like(plain)
+    O.ctf_anisotropic_2d("h2h", x, p, (b, 1, h, w), ctfs)
+    np.testing.assert_allclose(p, np.abs(x * plain) ** 2, rtol=1e-6)
+    # A frequency range (0, 0.25) on a 64 box samples fftfreq k / 128: the CTF of a 128 box at the same frequencies.
+    c64 = O.make_ctf((1.0, 1.0), (1.0, 0.05, 0.3), 300, 0.1, 2.7, f64=True)
+    part = np.zeros((1, 1, 64, 33), np.float64)
+    full = np.zeros((1, 1, 128, 65), np.float64)
+    O.ctf_anisotropic_2d("h2h", None, part, (1, 1, 64, 64), c64, fftfreq_range=(0.0, 0.25, True))
+    O.ctf_anisotropic_2d("h2h", None, full, (1, 1, 128, 128), c64)
+    fy = freq_grid(64, False)
+    np.testing.assert_array_equal(part[0, 0], full[0, 0][np.where(fy < 0, fy + 128, fy)][:, :33])
