@@ -25,6 +25,7 @@
 #include "noa_hip.h"
 
 #include <array>
+#include <cmath>
 #include <complex>
 #include <concepts>
 #include <cstdint>
@@ -153,6 +154,17 @@ struct ExtractCentralSlicesOptions {
     WindowedSinc w_windowed_sinc{};
     f64 fftfreq_cutoff{0.5};
     Shape4 target_shape{};
+    std::array<f64, 2> ews_radius{};
+};
+
+/* InsertAndExtractCentralSlicesOptions (FourierProject.hpp:1916-1946). */
+struct InsertAndExtractCentralSlicesOptions {
+    Interp interp{Interp::LINEAR};
+    WindowedSinc input_windowed_sinc{};
+    WindowedSinc w_windowed_sinc{};
+    bool add_to_output{false};
+    bool correct_weights{false};
+    f64 fftfreq_cutoff{0.5};
     std::array<f64, 2> ews_radius{};
 };
 
@@ -317,6 +329,70 @@ void extract_central_slices_3d(const Input& volume, const InputWeight& volume_we
 }
 
 namespace details {
+    template<class InputScale, class InputRotate, class OutputScale, class OutputRotate, class Input,
+             class InputWeight, class Output, class OutputWeight>
+    noa_hip_insert_extract_args make_insert_extract_args(
+        int32_t remap, const Input& input, const InputWeight& input_weight, const Shape4& input_shape,
+        const Output& output, const OutputWeight& output_weight, const Shape4& output_shape,
+        const InputScale& input_scaling, const InputRotate& input_rotation, const OutputScale& output_scaling,
+        const OutputRotate& output_rotation, const InsertAndExtractCentralSlicesOptions& o) {
+        noa_hip_insert_extract_args a;
+        noa_hip_insert_extract_args_init(&a);
+        a.input = view(input);
+        a.input_weight = view(input_weight);
+        a.output = view(output);
+        a.output_weight = view(output_weight);
+        for (int i = 0; i < 4; ++i) {
+            a.input_shape[i] = input_shape[i];
+            a.output_shape[i] = output_shape[i];
+        }
+        a.remap = remap;
+        a.interp = static_cast<int32_t>(o.interp);
+        a.input_scaling = transform(input_scaling);
+        a.input_rotation = transform(input_rotation);
+        a.output_scaling = transform(output_scaling);
+        a.output_rotation = transform(output_rotation);
+        a.input_fftfreq_sinc = o.input_windowed_sinc.fftfreq_sinc;
+        a.input_fftfreq_blackman = o.input_windowed_sinc.fftfreq_blackman;
+        a.w_fftfreq_sinc = o.w_windowed_sinc.fftfreq_sinc;
+        a.w_fftfreq_blackman = o.w_windowed_sinc.fftfreq_blackman;
+        a.fftfreq_cutoff = o.fftfreq_cutoff;
+        a.ews_radius[0] = o.ews_radius[0];
+        a.ews_radius[1] = o.ews_radius[1];
+        a.add_to_output = o.add_to_output ? 1 : 0;
+        a.correct_weights = o.correct_weights ? 1 : 0;
+        return a;
+    }
+} // namespace details
+
+/* noa::xform::insert_and_extract_central_slices_3d (FourierProject.hpp:1995-2049): extraction from a virtual volume
+ * made of the input slices. Overwrites the outputs (or adds to them with add_to_output). */
+template<Layout REMAP, class Input, class InputWeight = Empty, class Output, class OutputWeight = Empty,
+         class InputScale = Empty, class InputRotate, class OutputScale = Empty, class OutputRotate>
+void insert_and_extract_central_slices_3d(const Input& input_slice, const InputWeight& input_weight,
+                                          const Shape4& input_slice_shape, const Output& output_slice,
+                                          const OutputWeight& output_weight, const Shape4& output_slice_shape,
+                                          const InputScale& input_fwd_scaling, const InputRotate& input_inv_rotation,
+                                          const OutputScale& output_inv_scaling,
+                                          const OutputRotate& output_fwd_rotation,
+                                          const InsertAndExtractCentralSlicesOptions& options = {},
+                                          void* stream = nullptr) {
+    const noa_hip_insert_extract_args a = details::make_insert_extract_args(
+        REMAP.bits, input_slice, input_weight, input_slice_shape, output_slice, output_weight, output_slice_shape,
+        input_fwd_scaling, input_inv_rotation, output_inv_scaling, output_fwd_rotation, options);
+    details::check(noa_hip_insert_and_extract_central_slices_3d(&a, stream));
+}
+
+/* noa::xform::fourier_interpolation_correction (FourierProject.hpp:2062-2087) on real (B, D, H, W) arrays. */
+template<class Input, class Output>
+void fourier_interpolation_correction(const Input& input, const Output& output, Interp interp = Interp::LINEAR,
+                                      bool post_correction = true, void* stream = nullptr) {
+    const noa_hip_view in = details::view(input), out = details::view(output);
+    details::check(noa_hip_fourier_interpolation_correction(&in, &out, static_cast<int32_t>(interp),
+                                                            post_correction ? 1 : 0, stream));
+}
+
+namespace details {
     /* One (y, x) shift on the host, or a View of (y, x) pairs (Vec<f32,2> / Vec<f64,2>) on the device. */
     template<class T> struct Vec2 { T y, x; };
     template<class Shift>
@@ -372,7 +448,108 @@ void phase_shift_2d(const Input& input, const Output& output, const Shape4& shap
     const noa_hip_shift2 sh = xform::details::shifts(shifts, fftfreq_cutoff);
     xform::details::check(noa_hip_phase_shift_2d(&in, &out, shape.v.data(), REMAP.bits, &sh, stream));
 }
+
+/* noa::signal::CTFAnisotropic<T> (src/noa/signal/core/CTF.hpp:291-575): the same members in the same order (the
+ * memory layout of noa's objects), built by noa's constructor from physical parameters. */
+template<class T>
+struct alignas(2 * sizeof(T)) CTFAnisotropic {
+    static_assert(std::is_same_v<T, f32> || std::is_same_v<T, f64>);
+    struct Defocus { T value, astigmatism, angle; };   // um (positive underfocus), um, radians
+    T pixel_size[2]{};
+    T defocus_value{}, defocus_astigmatism{}, defocus_angle{};   // angstroms (noa's sign), radians
+    T phase_shift{}, cs{}, voltage{}, amplitude{}, bfactor_forth{}, scale{}, k1{}, k2{}, k3{};
+
+    CTFAnisotropic() = default;
+    CTFAnisotropic(const std::array<T, 2>& pixel_size_, Defocus defocus, T voltage_kv, T amplitude_, T cs_mm,
+                   T phase_shift_ = 0, T bfactor = 0, T scale_ = 1) {
+        pixel_size[0] = pixel_size_[0];
+        pixel_size[1] = pixel_size_[1];
+        phase_shift = phase_shift_;
+        cs = cs_mm * static_cast<T>(1e7);
+        voltage = voltage_kv * static_cast<T>(1e3);
+        amplitude = amplitude_;
+        scale = scale_;
+        defocus_value = -defocus.value * static_cast<T>(1e4);
+        defocus_astigmatism = -defocus.astigmatism * static_cast<T>(1e4);
+        defocus_angle = defocus.angle;
+        bfactor_forth = bfactor / 4;
+        const f64 h = 6.62607015e-34, c = 299792458.0, m0 = 9.1093837015e-31, e = 1.602176634e-19;
+        const f64 ev = e * static_cast<f64>(voltage);
+        const f64 lambda = (h * c) / std::sqrt(ev * (2 * m0 * c * c + ev)) * 1e10;
+        const f64 pi = 3.1415926535897932384626433832795;
+        k1 = static_cast<T>(pi * lambda);
+        k2 = static_cast<T>(pi * 0.5 * static_cast<f64>(cs) * lambda * lambda * lambda);
+        k3 = std::atan(amplitude / std::sqrt(1 - amplitude * amplitude));
+    }
+};
+static_assert(sizeof(CTFAnisotropic<f32>) == sizeof(noa_hip_ctf_anisotropic_f32));
+static_assert(sizeof(CTFAnisotropic<f64>) == sizeof(noa_hip_ctf_anisotropic_f64));
+
+/* CTFOptions (src/noa/signal/CTF.hpp:196-209). */
+struct CTFOptions {
+    struct Range { f64 start{0}, stop{-1}; bool endpoint{true}; } fftfreq_range{};
+    bool ctf_abs{false};
+    bool ctf_squared{false};
+};
+
+namespace details {
+    template<class CTF>
+    noa_hip_ctf ctf(const CTF& c) {
+        noa_hip_ctf r{};
+        using U = std::remove_cvref_t<CTF>;
+        if constexpr (xform::details::is_view_v<U>) {
+            using E = std::remove_cv_t<std::remove_pointer_t<decltype(c.ptr)>>;
+            r.data = c.ptr;
+            r.dtype = std::is_same_v<E, CTFAnisotropic<f64>> ? NOA_HIP_F64 : NOA_HIP_F32;
+            r.batched = 1;
+            r.count = c.ssize();
+        } else {
+            r.data = &c;
+            r.dtype = std::is_same_v<U, CTFAnisotropic<f64>> ? NOA_HIP_F64 : NOA_HIP_F32;
+            r.batched = 0;
+            r.count = 1;
+        }
+        return r;
+    }
+    inline noa_hip_ctf_options ctf_options(const CTFOptions& o) {
+        return {o.fftfreq_range.start, o.fftfreq_range.stop, o.fftfreq_range.endpoint ? 1 : 0, o.ctf_abs ? 1 : 0,
+                o.ctf_squared ? 1 : 0};
+    }
+} // namespace details
+
+/* noa::signal::ctf_anisotropic<REMAP> (src/noa/signal/CTF.hpp:430-500) for 2D rfft arrays. `input` may be Empty
+ * (the CTF is written). `ctf`: one CTFAnisotropic (host) or a View of one per batch (device). */
+template<xform::Layout REMAP, class Input, class Output, class CTF>
+void ctf_anisotropic(const Input& input, const Output& output, const Shape4& shape, const CTF& ctf,
+                     const CTFOptions& options = {}, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    const noa_hip_ctf c = details::ctf(ctf);
+    const noa_hip_ctf_options o = details::ctf_options(options);
+    xform::details::check(noa_hip_ctf_anisotropic_2d(&in, &out, shape.v.data(), REMAP.bits, &c, &o, stream));
+}
+
+/* output = input * CTF and output_weight = CTF^2 in one pass (the CTF-weighted insertion's prologue). */
+template<xform::Layout REMAP, class Input, class Output, class OutputWeight, class CTF>
+void ctf_anisotropic_weighted(const Input& input, const Output& output, const OutputWeight& output_weight,
+                              const Shape4& shape, const CTF& ctf, const CTFOptions& options = {},
+                              void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    const noa_hip_view w = xform::details::view(output_weight);
+    const noa_hip_ctf c = details::ctf(ctf);
+    const noa_hip_ctf_options o = details::ctf_options(options);
+    xform::details::check(noa_hip_ctf_anisotropic_2d_weighted(&in, &out, &w, shape.v.data(), REMAP.bits, &c, &o,
+                                                              stream));
+}
 } // namespace signal
+
+namespace fft {
+/* noa::fft::remap (src/noa/fft/Remap.hpp:205-340) between the half-to-half layouts. */
+template<xform::Layout REMAP, class Input, class Output>
+void remap(const Input& input, const Output& output, const Shape4& shape, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    xform::details::check(noa_hip_remap(REMAP.bits, &in, &out, shape.v.data(), stream));
+}
+} // namespace fft
 } // namespace noa_hip
 
 #endif // NOA_HIP_HPP

@@ -187,11 +187,73 @@ static void check_phase_shift_errors() {
     EXPECT(thrown);
 }
 
+static void check_insert_extract_descriptor() {
+    // 20 input slices (inverse rotations on the device) -> 5 output slices (one broadcast forward rotation).
+    auto* in = reinterpret_cast<c32*>(0x10000000);
+    auto* out = reinterpret_cast<c32*>(0x20000000);
+    auto* wout = reinterpret_cast<float*>(0x30000000);
+    auto* rot = reinterpret_cast<Mat33<float>*>(0x40000000);
+    const View<c32> i(in, Shape4(20, 1, 64, 33));
+    const View<c32> o(out, Shape4(5, 1, 64, 33));
+    const View<float> w(wout, Shape4(5, 1, 64, 33));
+    const View<Mat33<float>> r(rot, Shape4(1, 1, 1, 20));
+    const Mat33<float> eye{{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    nx::InsertAndExtractCentralSlicesOptions opt;
+    opt.input_windowed_sinc = {0.01, 0.02};
+    opt.w_windowed_sinc = {0.03, 0.04};
+    opt.add_to_output = true;
+    opt.fftfreq_cutoff = 0.8;
+    const auto a = nx::details::make_insert_extract_args(nx::Layout("hc2hc").bits, i, Empty{}, Shape4(20, 1, 64, 64),
+                                                         o, w, Shape4(5, 1, 64, 64), Empty{}, r, Empty{}, eye, opt);
+    EXPECT(a.remap == NOA_HIP_HC2HC && a.interp == NOA_HIP_INTERP_LINEAR);
+    EXPECT(a.input.data == in && a.output.data == out && a.output_weight.data == wout);
+    EXPECT(a.input_weight.data == nullptr && a.input_weight.is_value == 0);
+    EXPECT(a.input_rotation.batched == 1 && a.input_rotation.count == 20 && a.input_rotation.data == rot);
+    EXPECT(a.output_rotation.batched == 0 && a.output_rotation.data == &eye);
+    EXPECT(a.input_scaling.kind == NOA_HIP_XFORM_NONE && a.output_scaling.kind == NOA_HIP_XFORM_NONE);
+    EXPECT(a.input_fftfreq_sinc == 0.01 && a.input_fftfreq_blackman == 0.02);
+    EXPECT(a.w_fftfreq_sinc == 0.03 && a.w_fftfreq_blackman == 0.04);
+    EXPECT(a.add_to_output == 1 && a.correct_weights == 0 && a.fftfreq_cutoff == 0.8);
+    // correct_weights with add_to_output is refused before any memory is touched (FourierProject.hpp:2011-2018).
+    opt.correct_weights = true;
+    bool thrown = false;
+    try {
+        nx::insert_and_extract_central_slices_3d<"hc2hc">(i, Empty{}, Shape4(20, 1, 64, 64), o, w,
+                                                          Shape4(5, 1, 64, 64), Empty{}, r, Empty{}, eye, opt);
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+        EXPECT(std::strstr(e.what(), "correct_weights") != nullptr);
+    }
+    EXPECT(thrown);
+    std::printf("ok insert_extract descriptor\n");
+}
+
+static void check_ctf_object() {
+    // noa's constructor: 300 kV -> lambda = 0.0196875 A, k1 = pi * lambda, k3 = atan(A / sqrt(1 - A^2)), defocus in
+    // angstroms with the sign flipped, cs in angstroms, bfactor / 4.
+    const noa_hip::signal::CTFAnisotropic<double> c({1.0, 1.1}, {1.5, 0.05, 0.3}, 300.0, 0.07, 2.7, 0.2, -20.0, 2.0);
+    EXPECT(std::fabs(c.k1 / 3.141592653589793 - 0.0196875) < 1e-7);
+    EXPECT(std::fabs(c.k3 - std::asin(0.07)) < 1e-15);
+    EXPECT(c.defocus_value == -15000.0 && c.defocus_astigmatism == -500.0 && c.defocus_angle == 0.3);
+    EXPECT(c.cs == 2.7e7 && c.voltage == 3e5 && c.bfactor_forth == -5.0 && c.scale == 2.0);
+    EXPECT(c.pixel_size[1] == 1.1 && c.phase_shift == 0.2);
+    const double lam = c.k1 / 3.141592653589793;
+    EXPECT(std::fabs(c.k2 - 3.141592653589793 * 0.5 * 2.7e7 * lam * lam * lam) < 1e-9 * c.k2);
+    const noa_hip::signal::CTFAnisotropic<float> f({1.f, 1.f}, {1.f, 0.f, 0.f}, 300.f, 0.1f, 2.7f);
+    static_assert(sizeof(f) == 56 && alignof(noa_hip::signal::CTFAnisotropic<double>) == 16);
+    const auto d = noa_hip::signal::details::ctf(f);
+    EXPECT(d.dtype == NOA_HIP_F32 && d.batched == 0 && d.data == &f);
+    std::printf("ok ctf object\n");
+}
+
 int main(int argc, char** argv) {
     check_layouts();
     check_descriptor();
     check_errors();
     check_phase_shift_errors();
+    check_insert_extract_descriptor();
+    check_ctf_object();
     if (argc > 1 && std::strcmp(argv[1], "--gpu") == 0) check_gpu();
     std::printf("all facade checks passed\n");
     return 0;
