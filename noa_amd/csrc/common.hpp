@@ -321,35 +321,11 @@ __device__ __forceinline__ Math interp3(const Storage* __restrict__ vol, i32 sz,
     }
 }
 
-// Element loaders of the two-phase interpolation. GlobalLoad: a flat pointer (64-bit address per tap).
-// BufferLoad: a buffer resource over at most 4 GiB (base and size in scalar registers, a 32-bit byte offset per tap,
-// out-of-range offsets read 0), which saves the 64-bit address arithmetic of every tap.
+// Element loader of the two-phase interpolation: a flat pointer.
 template<class S>
 struct GlobalLoad {
     const S* __restrict__ base;
     template<class Math> __device__ __forceinline__ Math load(u32 elem) const { return load_as<Math>(base + elem); }
-};
-
-template<class S>
-struct BufferLoad {
-    __amdgpu_buffer_rsrc_t rsrc;
-    __device__ __forceinline__ BufferLoad(const S* base, u32 bytes)
-        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<S*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
-    template<class Math> __device__ __forceinline__ Math load(u32 elem) const {
-        const u32 off = elem * static_cast<u32>(sizeof(S));
-        S v;
-        if constexpr (sizeof(S) == 4) {
-            const u32 r = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
-            __builtin_memcpy(&v, &r, 4);
-        } else if constexpr (sizeof(S) == 8) {
-            const auto r = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
-            __builtin_memcpy(&v, &r, 8);
-        } else {
-            const auto r = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
-            __builtin_memcpy(&v, &r, 16);
-        }
-        return load_as<Math>(&v);
-    }
 };
 
 // The same 3D interpolation split in two phases, so that a lane can have the loads of several samples in flight:

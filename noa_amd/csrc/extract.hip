@@ -37,7 +37,6 @@ struct ExtractParams {
     i32 window;            // blackman window size along w (1 = plain 3D extraction)
     XformArg<C> rot, scale;
     bool general, vol_centered, slice_centered, has_wvol, has_wout;
-    u32 vol_bytes, wvol_bytes;   // byte extents of the volume and its weights (buffer loads; < 4 GiB)
     // Fused phase shift of the output slices (phase_shift_2d in the slices' own layout); shift_mode < 0: none.
     int shift_mode;
     bool shift_f64, shift_batched;
@@ -68,7 +67,7 @@ __device__ __forceinline__ Out shift_value(const ExtractParams<C>& p, i32 b, i32
     }
 }
 
-template<class C, class S, class M, class Out, class W, int INTERP, bool BUF>
+template<class C, class S, class M, class Out, class W, int INTERP>
 __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) {
     const u32 lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const i32 b = static_cast<i32>(lb / static_cast<u32>(p.tiles));
@@ -110,19 +109,11 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
                                         : slice2grid<false>(f2, sc, rot, p.ews[0], p.ews[1]);
             inside[k] = live && dot3(f) <= p.cutoff_sqd;
             const C fz = f.z * p.target[0], fy = f.y * p.target[1], fx = f.x * p.target[2];
-            if constexpr (BUF) {
-                tv[k] = fetch3<INTERP, M>(BufferLoad<S>(vol, p.vol_bytes), p.vs0, p.vs1, p.vs2, bd, p.vol_centered,
-                                          fz, fy, fx, inside[k]);
-                if (p.has_wout && p.has_wvol)
-                    tw[k] = fetch3<INTERP, W>(BufferLoad<W>(wvol, p.wvol_bytes), p.wvs0, p.wvs1, p.wvs2, bd,
-                                              p.vol_centered, fz, fy, fx, inside[k]);
-            } else {
-                tv[k] = fetch3<INTERP, M>(GlobalLoad<S>{vol}, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx,
+            tv[k] = fetch3<INTERP, M>(GlobalLoad<S>{vol}, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx,
+                                      inside[k]);
+            if (p.has_wout && p.has_wvol)
+                tw[k] = fetch3<INTERP, W>(GlobalLoad<W>{wvol}, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx,
                                           inside[k]);
-                if (p.has_wout && p.has_wvol)
-                    tw[k] = fetch3<INTERP, W>(GlobalLoad<W>{wvol}, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy,
-                                              fx, inside[k]);
-            }
         }
 #pragma unroll
         for (i32 k = 0; k < MAXP; ++k) {
@@ -250,27 +241,11 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     if (blocks == 0) return NOA_HIP_OK;
     if (blocks > 0x7fffffffLL) { err = "too many slices for one launch"; return NOA_HIP_ERROR_UNSUPPORTED; }
 
-    // Buffer loads when the volume (and weights) have non-negative strides and span less than 4 GiB.
-    auto span = [&](const Arr& a, size_t elem) -> u64 {
-        if (a.data == nullptr || a.s0 < 0 || a.s1 < 0 || a.s2 < 0) return ~u64(0);
-        return (static_cast<u64>(d.in_shape[1] - 1) * a.s0 + static_cast<u64>(d.in_shape[2] - 1) * a.s1 +
-                static_cast<u64>(d.in_shape[3] / 2) * a.s2 + 1) * elem;
-    };
-    const u64 vb = span(d.input, sizeof(S));
-    const u64 wb = p.has_wvol ? span(d.input_weight, sizeof(W)) : 0;
-    const bool buf = false && vb < 0xffffffffULL && wb < 0xffffffffULL;
-    p.vol_bytes = buf ? static_cast<u32>(vb) : 0;
-    p.wvol_bytes = buf ? static_cast<u32>(wb) : 0;
-
     const dim3 grid(static_cast<unsigned>(blocks)), block(256);
-    const bool nearest = d.interp == NOA_HIP_INTERP_NEAREST;
-    if (buf) {
-        if (nearest) hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_NEAREST, true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_LINEAR, true>), grid, block, 0, stream, p);
-    } else {
-        if (nearest) hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_NEAREST, false>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_LINEAR, false>), grid, block, 0, stream, p);
-    }
+    if (d.interp == NOA_HIP_INTERP_NEAREST)
+        hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_NEAREST>), grid, block, 0, stream, p);
+    else
+        hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_LINEAR>), grid, block, 0, stream, p);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
     return NOA_HIP_OK;
