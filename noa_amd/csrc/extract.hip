@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace noa_hip {
 
@@ -31,6 +32,7 @@ struct ExtractParams {
     C target[3];
     C slice_f[2];
     i32 slice_h, slice_wr, pixels, tiles, ppt;
+    float inv_wr;          // 1 / slice_wr (lean kernel's pixel -> row)
     C cutoff_sqd;
     C ews[2];
     C fftfreq_sinc, fftfreq_blackman;
@@ -168,6 +170,70 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
     }
 }
 
+// The common configuration on a lean path: complex FP32 volume and slices, no weights, no scaling or EWS, no w
+// window, LINEAR, unit x stride; the fused phase shift in FP32 (SHIFT 1: per-slice or broadcast shifts, SHIFT 2:
+// the half-shape +-1 pattern, SHIFT 0: none). Same expressions as extract_kernel (bit-identical results), minus the
+// work that the general kernel does for every pixel: the integer division of the pixel index (here a float
+// reciprocal, exact for slices below 2^21 pixels, checked on the host), the runtime selection between the plain and
+// the scaled/EWS transforms, the weight paths, and the runtime choice of the shift precision.
+template<int SHIFT>
+__global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<float> p) {
+    const u32 lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const i32 b = static_cast<i32>(lb / static_cast<u32>(p.tiles));
+    const i32 tile = static_cast<i32>(lb) - b * p.tiles;
+
+    const Rot<float> rot = load_rotation(p.rot, b);
+    Scale<float> sc;
+    sc.identity = true;
+    const Bounds3 bd = make_bounds3(p.grid[0], p.grid[1], p.grid[2]);
+    float sy = 0, sx = 0;
+    if constexpr (SHIFT == 1) {
+        const float* s = static_cast<const float*>(p.shifts);
+        sy = p.shift_batched ? s[2 * b] : p.shift_f[0];
+        sx = p.shift_batched ? s[2 * b + 1] : p.shift_f[1];
+    }
+    const float2* __restrict__ vol = static_cast<const float2*>(p.vol);
+    float2* __restrict__ out = static_cast<float2*>(p.out) + static_cast<i64>(b) * p.os_b;
+
+    constexpr i32 MAXP = 4;
+    Taps3<float2, float> tv[MAXP];
+    bool inside[MAXP];
+    i32 ys[MAXP], us[MAXP];
+    const i32 first = tile * p.ppt * 256 + static_cast<i32>(threadIdx.x);
+#pragma unroll
+    for (i32 k = 0; k < MAXP; ++k) {
+        const i32 pix = first + k * 256;
+        const bool live = k < p.ppt && pix < p.pixels;
+        // pix / wr from a float reciprocal: (pix + 0.5) / wr is at least 0.5 / wr away from an integer, far above
+        // the rounding error for pix < 2^21.
+        const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
+        const i32 y = live ? yq : 0;
+        const i32 u = live ? pix - yq * p.slice_wr : 0;
+        ys[k] = live ? y : -1;
+        us[k] = u;
+        vec2<float> f2;
+        const i32 fy = index2frequency(p.slice_centered, y, p.slice_h);
+        f2.y = static_cast<float>(fy) / p.slice_f[0];
+        f2.x = static_cast<float>(u) / p.slice_f[1];
+        const vec3<float> f = slice2grid<false>(f2, sc, rot, 0.f, 0.f);
+        inside[k] = live && dot3(f) <= p.cutoff_sqd;
+        const float fz = f.z * p.target[0], fyv = f.y * p.target[1], fx = f.x * p.target[2];
+        tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{vol}, p.vs0, p.vs1, 1, bd, p.vol_centered,
+                                                      fz, fyv, fx, inside[k]);
+    }
+#pragma unroll
+    for (i32 k = 0; k < MAXP; ++k) {
+        if (ys[k] < 0) continue;
+        float2 value = inside[k] ? combine3<NOA_HIP_INTERP_LINEAR>(tv[k]) : vzero<float2>();
+        if constexpr (SHIFT != 0) {
+            const i32 fy = index2frequency(p.slice_centered, ys[k], p.slice_h);
+            value = cmul(value, phase_shift_at<float2, float>(SHIFT == 1 ? 0 : 2, sy, sx, fy, us[k], p.norm_f[0],
+                                                               p.norm_f[1], p.cutoff_sqd_f));
+        }
+        out[ys[k] * p.os_y + us[k] * p.os_x] = value;
+    }
+}
+
 namespace {
 
 template<class C>
@@ -242,6 +308,20 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     if (blocks > 0x7fffffffLL) { err = "too many slices for one launch"; return NOA_HIP_ERROR_UNSUPPORTED; }
 
     const dim3 grid(static_cast<unsigned>(blocks)), block(256);
+    if constexpr (std::is_same_v<C, float> && std::is_same_v<S, float2> && std::is_same_v<Out, float2>) {
+        const bool lean = d.interp == NOA_HIP_INTERP_LINEAR && !p.general && p.window == 1 && !p.has_wvol &&
+                          !p.has_wout && p.vs2 == 1 && p.pixels < (1 << 21) &&
+                          (p.shift_mode < 0 || p.shift_mode == 1 || p.shift_mode == 2 || !p.shift_f64);
+        if (lean) {
+            p.inv_wr = 1.f / static_cast<float>(p.slice_wr);
+            if (p.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1>), grid, block, 0, stream, p);
+            else if (p.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2>), grid, block, 0, stream, p);
+            else hipLaunchKernelGGL((extract_c32_kernel<0>), grid, block, 0, stream, p);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+            return NOA_HIP_OK;
+        }
+    }
     if (d.interp == NOA_HIP_INTERP_NEAREST)
         hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_NEAREST>), grid, block, 0, stream, p);
     else

@@ -273,10 +273,12 @@ def test_insert_staged_quaternion_real_and_abs2_outputs():
     assert rel_l2(got_f, want_f) < 1e-6
 
 
-def test_insert_many_slices_split_bricks():
+@pytest.mark.parametrize("remap", ["hc2hc", "h2h"])
+def test_insert_many_slices_split_bricks(remap):
     # 1500 slices into 32^3: the bricks near the origin get several slice segments whose partial sums are
     # reduced in segment order; compared with the oracle's strict slice order and with itself (determinism).
-    print("case insert split bricks", flush=True)
+    # Power-of-two size, Mat33 rotations and weights in and out: the compile-time configurations of the kernel.
+    print(f"case insert split bricks {remap}", flush=True)
     rng = np.random.default_rng(133)
     n, b = 32, 1500
     s = random_spectrum((b, 1, n, n // 2 + 1), rng)
@@ -284,9 +286,9 @@ def test_insert_many_slices_split_bricks():
     inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
     want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
     wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
-    O.insert_central_slices_3d("hc2hc", s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv)
-    v1, w1 = _insert_dev("hc2hc", s, w, (b, 1, n, n), n, inv)
-    v2, w2 = _insert_dev("hc2hc", s, w, (b, 1, n, n), n, inv)
+    O.insert_central_slices_3d(remap, s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv)
+    v1, w1 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv)
+    v2, w2 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv)
     assert rel_l2(v1, want) < 1e-6 and rel_l2(w1, wwant) < 1e-6
     np.testing.assert_array_equal(v1, v2)
     np.testing.assert_array_equal(w1, w2)
