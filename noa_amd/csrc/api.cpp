@@ -220,9 +220,7 @@ ProjectDesc validate(Kind kind, const noa_hip_project_args& a, int& device) {
     // Interpolation.
     int interp = a.interp >= 100 ? a.interp - 100 : a.interp;
     NOA_CHECK(interp >= 0 && interp <= 6, INVALID, "Invalid interpolation method %d", a.interp);
-    if (kind != Kind::RASTERIZE)
-        NOA_CHECK(interp == NOA_HIP_INTERP_LINEAR || interp == NOA_HIP_INTERP_NEAREST, UNSUPPORTED,
-                  "Interpolation method %d is not implemented by this library (NEAREST and LINEAR are)", a.interp);
+
 
     // Value types: spectrum types (Traits.hpp:87) = c->c, r->r, c->r.
     const int in = a.input.dtype, out = a.output.dtype;
@@ -563,8 +561,7 @@ InsertExtractDesc validate_insert_extract(const noa_hip_insert_extract_args& a, 
               INVALID, "The remap should be H2H, HC2H, H2HC or HC2HC, but got %d", a.remap);
     const int interp = a.interp >= 100 ? a.interp - 100 : a.interp;
     NOA_CHECK(interp >= 0 && interp <= 6, INVALID, "Invalid interpolation method %d", a.interp);
-    NOA_CHECK(interp == NOA_HIP_INTERP_LINEAR || interp == NOA_HIP_INTERP_NEAREST, UNSUPPORTED,
-              "Interpolation method %d is not implemented by this library (NEAREST and LINEAR are)", a.interp);
+
     const int in = a.input.dtype, out = a.output.dtype;
     NOA_CHECK((is_complex(in) && is_complex(out)) || (is_real(in) && is_real(out)) || (is_complex(in) && is_real(out)),
               INVALID, "Invalid input/output types: %d -> %d", in, out);

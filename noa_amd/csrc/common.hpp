@@ -263,7 +263,8 @@ __device__ __forceinline__ C windowed_sinc(C fftfreq, C fftfreq_sinc, C fftfreq_
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// Spectrum interpolation (Interpolation.hpp:435-526) for NEAREST and LINEAR, rfft along x.
+// Spectrum interpolation (Interpolation.hpp:435-526), rfft along x: NEAREST and LINEAR (early flip), and the
+// windows of 4-8 taps (per-tap flip).
 // ---------------------------------------------------------------------------------------------------------
 struct Bounds3 { i32 lo_z, hi_z, lo_y, hi_y, hi_x; i32 nz, ny; };
 
@@ -271,10 +272,147 @@ __device__ __forceinline__ Bounds3 make_bounds3(i32 nz, i32 ny, i32 nx) {
     return {-nz / 2, (nz - 1) / 2, -ny / 2, (ny - 1) / 2, nx / 2, nz, ny};
 }
 
+// Template value of INTERP for the windows larger than 2 (CUBIC, CUBIC_BSPLINE, LANCZOS4/6/8): one instantiation,
+// the method itself is a run-time argument.
+constexpr int INTERP_WINDOW = NOA_HIP_INTERP_CUBIC;
+
+// Interp::window_size (Interp.hpp:55-77) of a method (the _FAST spellings erased).
+__host__ __device__ __forceinline__ int window_size(int interp) {
+    switch (interp) {
+        case NOA_HIP_INTERP_NEAREST: case NOA_HIP_INTERP_LINEAR: return 2;
+        case NOA_HIP_INTERP_CUBIC: case NOA_HIP_INTERP_CUBIC_BSPLINE: case NOA_HIP_INTERP_LANCZOS4: return 4;
+        case NOA_HIP_INTERP_LANCZOS6: return 6;
+        default: return 8;
+    }
+}
+
+// interpolation_weights (Interpolation.hpp:81-141) for one dimension of a window of 4, 6 or 8, in the value's real
+// precision R: CUBIC (Keys, A = -0.5), CUBIC_BSPLINE, LANCZOS (OpenCV's one-sincos form, normalised).
+template<class R>
+__device__ __forceinline__ void window_weights(int interp, R f, R (&w)[8]) {
+    if (interp == NOA_HIP_INTERP_CUBIC) {
+        const R A = static_cast<R>(-0.5);
+        w[0] = ((A * (f + 1) - 5 * A) * (f + 1) + 8 * A) * (f + 1) - 4 * A;
+        w[1] = ((A + 2) * f - (A + 3)) * f * f + 1;
+        w[2] = ((A + 2) * (1 - f) - (A + 3)) * (1 - f) * (1 - f) + 1;
+        w[3] = 1 - w[0] - w[1] - w[2];
+    } else if (interp == NOA_HIP_INTERP_CUBIC_BSPLINE) {
+        const R one_sixth = static_cast<R>(1) / static_cast<R>(6);
+        const R two_third = static_cast<R>(2) / static_cast<R>(3);
+        const R one_minus = 1 - f;
+        const R one_squared = one_minus * one_minus;
+        const R squared = f * f;
+        w[0] = one_sixth * one_squared * one_minus;
+        w[1] = two_third - static_cast<R>(0.5) * squared * (2 - f);
+        w[2] = two_third - static_cast<R>(0.5) * one_squared * (2 - one_minus);
+        w[3] = one_sixth * squared * f;
+    } else {
+        const int size = window_size(interp);
+        const int center = size / 2 - 1;
+        const R PI = static_cast<R>(3.1415926535897932384626433832795);
+        const R s45 = static_cast<R>(0.7071067811865475);
+        const R cs0[8] = {1, -s45, 0, s45, -1, s45, 0, -s45};
+        const R cs1[8] = {0, -s45, 1, -s45, 0, s45, -1, s45};
+        const R eps = sizeof(R) == 4 ? static_cast<R>(1.1920928955078125e-07) : static_cast<R>(2.220446049250313e-16);
+        for (int i = 0; i < 8; ++i) w[i] = 0;
+        R sum = 0;
+        if (f <= eps) {
+            w[center] = 1;
+            sum = 1;
+        } else if (f >= 1 - eps) {
+            w[center + 1] = 1;
+            sum = 1;
+        } else {
+            const R x0 = -(f + static_cast<R>(center));
+            const R y0 = x0 * PI * static_cast<R>(0.25);
+            R sn, cn;
+            if constexpr (sizeof(R) == 4) sincosf(y0, &sn, &cn);
+            else ::sincos(y0, &sn, &cn);
+            for (int i = 0; i < size; ++i) {
+                const R fij = -(f + static_cast<R>(center) - static_cast<R>(i));
+                const R yj = fij * PI * static_cast<R>(0.25);
+                w[i] = (cs0[i] * sn + cs1[i] * cn) / (yj * yj);
+                sum += w[i];
+            }
+        }
+        const R inv = 1 / sum;
+        for (int i = 0; i < size; ++i) w[i] *= inv;
+    }
+}
+
+// interpolate_spectrum for windows larger than 2 (Interpolation.hpp:435-526): the flip is decided per tap (a tap at
+// x < 0 reads the conjugate of the mirrored element), taps outside the bounds contribute 0; x innermost.
+template<class Math, class Storage, class C>
+__device__ Math interp3_window(int interp, const Storage* __restrict__ vol, i32 sz, i32 sy, i32 sx, const Bounds3& bd,
+                               bool centered, C fz, C fy, C fx) {
+    using R = typename vt<Math>::real;
+    const int size = window_size(interp);
+    const i32 start = -(size - 1) / 2;
+    const C flz = floor(fz), fly = floor(fy), flx = floor(fx);
+    const i32 iz = static_cast<i32>(flz) + start, iy = static_cast<i32>(fly) + start, ix = static_cast<i32>(flx) + start;
+    R wz[8], wy[8], wx[8];
+    window_weights<R>(interp, static_cast<R>(fz - flz), wz);
+    window_weights<R>(interp, static_cast<R>(fy - fly), wy);
+    window_weights<R>(interp, static_cast<R>(fx - flx), wx);
+    Math value = vzero<Math>();
+    for (int z = 0; z < size; ++z) {
+        Math vz = vzero<Math>();
+        for (int y = 0; y < size; ++y) {
+            Math vy = vzero<Math>();
+            for (int x = 0; x < size; ++x) {
+                i32 tz = iz + z, ty = iy + y, tx = ix + x;
+                R conj = 1;
+                if (tx < 0) { tz = -tz; ty = -ty; tx = -tx; conj = -1; }
+                if (tz < bd.lo_z || tz > bd.hi_z || ty < bd.lo_y || ty > bd.hi_y || tx > bd.hi_x) continue;
+                Math v = load_as<Math>(vol + (frequency2index(centered, tz, bd.nz) * sz +
+                                              frequency2index(centered, ty, bd.ny) * sy + tx * sx));
+                vimag_mul(v, conj);
+                vy = vadd(vy, vmul(v, wx[x]));
+            }
+            vz = vadd(vz, vmul(vy, wy[y]));
+        }
+        value = vadd(value, vmul(vz, wz[z]));
+    }
+    return value;
+}
+
+template<class Math, class C>
+__device__ Math interp2_window(int interp, const Math* __restrict__ base, i32 sy, i32 sx, i32 ny, i32 nx,
+                               bool centered, C fy, C fx, bool is_value, Math cval) {
+    using R = typename vt<Math>::real;
+    const int size = window_size(interp);
+    const i32 start = -(size - 1) / 2;
+    const i32 lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
+    const C fly = floor(fy), flx = floor(fx);
+    const i32 iy = static_cast<i32>(fly) + start, ix = static_cast<i32>(flx) + start;
+    R wy[8], wx[8];
+    window_weights<R>(interp, static_cast<R>(fy - fly), wy);
+    window_weights<R>(interp, static_cast<R>(fx - flx), wx);
+    Math value = vzero<Math>();
+    for (int y = 0; y < size; ++y) {
+        Math vy = vzero<Math>();
+        for (int x = 0; x < size; ++x) {
+            i32 ty = iy + y, tx = ix + x;
+            R conj = 1;
+            if (tx < 0) { ty = -ty; tx = -tx; conj = -1; }
+            if (ty < lo_y || ty > hi_y || tx > hi_x) continue;
+            Math v = is_value ? cval : base[frequency2index(centered, ty, ny) * sy + tx * sx];
+            vimag_mul(v, conj);
+            vy = vadd(vy, vmul(v, wx[x]));
+        }
+        value = vadd(value, vmul(vy, wy[y]));
+    }
+    return value;
+}
+
 // 3D: `vol` with strides (sz, sy, sx); Math = float2/float/double2/double, Storage = stored element type.
+// INTERP_WINDOW: the window method `mode` at run time.
 template<int INTERP, class Math, class Storage, class C>
 __device__ __forceinline__ Math interp3(const Storage* __restrict__ vol, i32 sz, i32 sy, i32 sx, const Bounds3& bd,
-                                       bool centered, C fz, C fy, C fx) {
+                                       bool centered, C fz, C fy, C fx, int mode = 0) {
+    if constexpr (INTERP == INTERP_WINDOW) {
+        return interp3_window<Math>(mode, vol, sz, sy, sx, bd, centered, fz, fy, fx);
+    }
     using R = typename vt<Math>::real;
     R conjugate = 1;
     if (fx < 0) { fz *= -1; fy *= -1; fx *= -1; conjugate = -1; }
@@ -419,7 +557,11 @@ __device__ __forceinline__ Math combine3(const Taps3<Math, C>& t) {
 // A constant input (noa's AccessorValue) is read through the same code with `is_value` set.
 template<int INTERP, class Math, class C>
 __device__ __forceinline__ Math interp2(const Math* __restrict__ base, i32 sy, i32 sx, i32 ny, i32 nx,
-                                       bool centered, C fy, C fx, bool is_value = false, Math cval = Math{}) {
+                                       bool centered, C fy, C fx, bool is_value = false, Math cval = Math{},
+                                       int mode = 0) {
+    if constexpr (INTERP == INTERP_WINDOW) {
+        return interp2_window<Math>(mode, base, sy, sx, ny, nx, centered, fy, fx, is_value, cval);
+    }
     using R = typename vt<Math>::real;
     R conjugate = 1;
     if (fx < 0) { fy *= -1; fx *= -1; conjugate = -1; }

@@ -37,6 +37,7 @@ struct ExtractParams {
     C ews[2];
     C fftfreq_sinc, fftfreq_blackman;
     i32 window;            // blackman window size along w (1 = plain 3D extraction)
+    int interp_mode;       // the method of the INTERP_WINDOW instantiation (CUBIC, CUBIC_BSPLINE, LANCZOS*)
     XformArg<C> rot, scale;
     bool general, vol_centered, slice_centered, has_wvol, has_wout;
     // Fused phase shift of the output slices (phase_shift_2d in the slices' own layout); shift_mode < 0: none.
@@ -95,6 +96,35 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
         Taps3<W, C> tw[MAXP];
         bool inside[MAXP];
         i32 ys[MAXP], us[MAXP];
+        if constexpr (INTERP == INTERP_WINDOW) {
+            // Windows of 4 to 8 taps per dimension: one pixel at a time.
+            for (i32 k = 0; k < p.ppt; ++k) {
+                const i32 pix = first + k * 256;
+                if (pix >= p.pixels) break;
+                const i32 y = pix / p.slice_wr, u = pix - y * p.slice_wr;
+                vec2<C> f2;
+                f2.y = static_cast<C>(index2frequency(p.slice_centered, y, p.slice_h)) / p.slice_f[0];
+                f2.x = static_cast<C>(u) / p.slice_f[1];
+                const vec3<C> f = p.general ? slice2grid<true>(f2, sc, rot, p.ews[0], p.ews[1])
+                                            : slice2grid<false>(f2, sc, rot, p.ews[0], p.ews[1]);
+                Out value = vzero<Out>();
+                W weight{};
+                if (dot3(f) <= p.cutoff_sqd) {
+                    const C fz = f.z * p.target[0], fy = f.y * p.target[1], fx = f.x * p.target[2];
+                    value = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx,
+                                                          p.interp_mode));
+                    if (p.has_wout)
+                        weight = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz,
+                                                                 fy, fx, p.interp_mode)
+                                            : W(1);
+                }
+                if (p.shift_mode >= 0)
+                    value = shift_value(p, b, index2frequency(p.slice_centered, y, p.slice_h), u, value);
+                out[y * p.os_y + u * p.os_x] = value;
+                if (p.has_wout) wout[y * p.ws_y + u * p.ws_x] = weight;
+            }
+            return;
+        }
 #pragma unroll
         for (i32 k = 0; k < MAXP; ++k) {
             const i32 pix = first + k * 256;
@@ -156,10 +186,12 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
             if (dot3(fw) > p.cutoff_sqd) continue;
             const C fz = fw.z * p.target[0], fy = fw.y * p.target[1], fx = fw.x * p.target[2];
             const C conv = windowed_sinc<C>(offset, p.fftfreq_sinc, p.fftfreq_blackman);
-            const Out v = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx));
+            const Out v = vcast<Out>(interp3<INTERP, M>(vol, p.vs0, p.vs1, p.vs2, bd, p.vol_centered, fz, fy, fx,
+                                                        p.interp_mode));
             value = vadd(value, vmul(v, static_cast<RO>(conv)));
             if (p.has_wout) {
-                const W wv = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx)
+                const W wv = p.has_wvol ? interp3<INTERP, W>(wvol, p.wvs0, p.wvs1, p.wvs2, bd, p.vol_centered, fz, fy, fx,
+                                                             p.interp_mode)
                                         : W(1);
                 weight = weight + wv * static_cast<W>(conv);
             }
@@ -322,10 +354,13 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             return NOA_HIP_OK;
         }
     }
+    p.interp_mode = d.interp;
     if (d.interp == NOA_HIP_INTERP_NEAREST)
         hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_NEAREST>), grid, block, 0, stream, p);
-    else
+    else if (d.interp == NOA_HIP_INTERP_LINEAR)
         hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, NOA_HIP_INTERP_LINEAR>), grid, block, 0, stream, p);
+    else
+        hipLaunchKernelGGL((extract_kernel<C, S, M, Out, W, INTERP_WINDOW>), grid, block, 0, stream, p);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
     return NOA_HIP_OK;

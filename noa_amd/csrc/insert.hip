@@ -78,6 +78,7 @@ struct InsertParams {
     bool wpoly;           // use wa; otherwise evaluate noa's expression
     float inv_fb;         // 1 / fftfreq_blackman
     u32 slice_bytes, wslice_bytes;   // byte extent of one slice and of its weights (buffer loads)
+    int interp_mode;      // the method of the INTERP_WINDOW instantiation (CUBIC, CUBIC_BSPLINE, LANCZOS*)
 };
 
 // Keeps a value in vector registers: the compiler would otherwise re-read kernel arguments inside the loops,
@@ -1078,13 +1079,14 @@ __global__ __launch_bounds__(256) void insert_general_kernel(const InsertParams<
             const C py = f2.y * p.slice_f[0], px = f2.x * p.slice_f[1];
             const In* base = p.in + static_cast<i64>(i) * p.is_b;
             const In v = interp2<INTERP, In>(base, p.is_y, p.is_x, p.slice_shape[0], p.slice_shape[1],
-                                             p.slices_centered, py, px, p.in_is_value, p.in_value);
+                                             p.slices_centered, py, px, p.in_is_value, p.in_value, p.interp_mode);
             value = vadd(value, vmul(v, static_cast<RIn>(window)));
             if (p.has_wvol) {
                 if (p.win_present) {
                     const W* wbase = p.win + static_cast<i64>(i) * p.wis_b;
                     const W wv = interp2<INTERP, W>(wbase, p.wis_y, p.wis_x, p.slice_shape[0], p.slice_shape[1],
-                                                    p.slices_centered, py, px, p.win_is_value, p.win_value);
+                                                    p.slices_centered, py, px, p.win_is_value, p.win_value,
+                                                    p.interp_mode);
                     weights = weights + wv * static_cast<W>(window);
                 } else {
                     weights = weights + static_cast<W>(window);
@@ -1381,7 +1383,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
 
     constexpr bool single = sizeof(typename vt<In>::real) == 4 && sizeof(W) == 4;
     if constexpr (single) {
-        if (!p.general && !p.in_is_value && !p.win_is_value && (buffers_ok || kc.staged)) {
+        if (!p.general && !p.in_is_value && !p.win_is_value && (buffers_ok || kc.staged) && window_size(d.interp) == 2) {
             int device = 0;
             (void)hipGetDevice(&device);
             Plan plan;
@@ -1443,13 +1445,15 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
 
     const i64 n_vox = static_cast<i64>(p.grid[0]) * p.grid[1] * p.wr;
     const dim3 grid(static_cast<unsigned>((n_vox + 255) / 256)), block(256);
-    if (p.general) {
-        if (nearest) hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, true>), grid, block, 0, stream, p);
-    } else {
-        if (nearest) hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, false>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, false>), grid, block, 0, stream, p);
-    }
+    p.interp_mode = d.interp;
+    auto launch = [&](auto interp_tag) {
+        constexpr int I = decltype(interp_tag)::value;
+        if (p.general) hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, I, true>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((insert_general_kernel<C, In, Out, W, I, false>), grid, block, 0, stream, p);
+    };
+    if (nearest) launch(std::integral_constant<int, NOA_HIP_INTERP_NEAREST>{});
+    else if (d.interp == NOA_HIP_INTERP_LINEAR) launch(std::integral_constant<int, NOA_HIP_INTERP_LINEAR>{});
+    else launch(std::integral_constant<int, INTERP_WINDOW>{});
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
     return NOA_HIP_OK;

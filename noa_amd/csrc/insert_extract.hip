@@ -15,6 +15,8 @@
 // The w-windowed (4D) form keeps the window sum in registers: the outputs start from 0 (or from their value when
 // adding) and the w terms are added in increasing w, the order of noa's serial CPU path; no atomics are needed
 // since one lane owns each output pixel.
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -44,6 +46,7 @@ struct InsExtParams {
     C ews[2];
     C cutoff_sqd, in_sinc, in_blackman, w_sinc, w_blackman, volume_z;
     i32 w_size;
+    int interp_mode;                       // the method of the INTERP_WINDOW instantiation
     bool add, correct, cull;
 };
 
@@ -153,12 +156,13 @@ __global__ __launch_bounds__(IE_TILE * IE_TILE * IE_WAVES) void insert_extract_k
                     const C window = windowed_sinc<C>(fz, p.in_sinc, p.in_blackman);
                     const C py = g.y * p.in_f[0], px = g.x * p.in_f[1];
                     const In v = interp2<INTERP, In>(p.in + static_cast<i64>(ii) * p.is_b, p.is_y, p.is_x, p.in_h,
-                                                     p.in_w, p.in_centered, py, px, p.in_is_value, p.in_value);
+                                                     p.in_w, p.in_centered, py, px, p.in_is_value, p.in_value,
+                                                     p.interp_mode);
                     value = vadd(value, vmul(v, static_cast<RIn>(window)));
                     if (p.win_present) {
                         const W wv = interp2<INTERP, W>(p.win + static_cast<i64>(ii) * p.wis_b, p.wis_y, p.wis_x,
                                                         p.in_h, p.in_w, p.in_centered, py, px, p.win_is_value,
-                                                        p.win_value);
+                                                        p.win_value, p.interp_mode);
                         weight = weight + wv * static_cast<W>(window);
                     } else {
                         weight = weight + static_cast<W>(window);
@@ -291,14 +295,15 @@ int run(const InsertExtractDesc& d, hipStream_t stream, std::string& err) {
     const i64 blocks = (tiles + IE_WAVES - 1) / IE_WAVES;
     if (blocks > 0x7fffffff) { err = "insert_and_extract: too many output slices"; return NOA_HIP_ERROR_UNSUPPORTED; }
     const dim3 grid(static_cast<u32>(blocks)), block(IE_TILE * IE_TILE * IE_WAVES);
-    const bool nearest = d.interp == NOA_HIP_INTERP_NEAREST;
-    if (d.general) {
-        if (nearest) hipLaunchKernelGGL((insert_extract_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((insert_extract_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, true>), grid, block, 0, stream, p);
-    } else {
-        if (nearest) hipLaunchKernelGGL((insert_extract_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, false>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((insert_extract_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, false>), grid, block, 0, stream, p);
-    }
+    p.interp_mode = d.interp;
+    auto launch = [&](auto interp_tag) {
+        constexpr int I = decltype(interp_tag)::value;
+        if (d.general) hipLaunchKernelGGL((insert_extract_kernel<C, In, Out, W, I, true>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((insert_extract_kernel<C, In, Out, W, I, false>), grid, block, 0, stream, p);
+    };
+    if (d.interp == NOA_HIP_INTERP_NEAREST) launch(std::integral_constant<int, NOA_HIP_INTERP_NEAREST>{});
+    else if (d.interp == NOA_HIP_INTERP_LINEAR) launch(std::integral_constant<int, NOA_HIP_INTERP_LINEAR>{});
+    else launch(std::integral_constant<int, INTERP_WINDOW>{});
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
     return NOA_HIP_OK;

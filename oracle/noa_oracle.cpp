@@ -13,7 +13,8 @@
  *   - CTFAnisotropic, CTF op   src/noa/signal/core/CTF.hpp:12-25, :291-575; src/noa/signal/CTF.hpp:14-125
  *   - windowed sinc helpers    src/noa/xform/FourierProject.hpp:97-159
  *   - slice<->grid transforms  src/noa/xform/FourierProject.hpp:38-95
- *   - interpolate_spectrum     src/noa/xform/core/Interpolation.hpp:416-526 (+ weights :64-146)
+ *   - interpolate_spectrum     src/noa/xform/core/Interpolation.hpp:416-526 (+ weights :64-146): NEAREST, LINEAR,
+ *                              CUBIC, CUBIC_BSPLINE, LANCZOS4/6/8
  *   - index/frequency math     src/noa/fft/core/Frequency.hpp:80-145, :250-288
  *   - CPU iwise execution      src/noa/runtime/cpu/Iwise.hpp:7-117 (+ atomics cpu/ComputeHandle.hpp:39-55)
  *   - parameter defaults       src/noa/xform/FourierProject.hpp:1260-1487 (launchers)
@@ -250,9 +251,138 @@ struct Writer3 {
 // Supports NEAREST and LINEAR (window size 2: early flip, :459-463, :522-523).
 // `freq` is the centred frequency in samples; `shape` the logical shape; rfft along the last dimension.
 // ---------------------------------------------------------------------------------------------------------
+// interpolation_weights (Interpolation.hpp:81-141) of the windows larger than 2, for one dimension, in the value's
+// real precision R: CUBIC (Keys, A = -0.5), CUBIC_BSPLINE, LANCZOS4/6/8 (OpenCV's one-sincos form, normalised).
+inline int window_size(int interp) {   // Interp::window_size (Interp.hpp:55-77)
+    const int base = interp >= 100 ? interp - 100 : interp;
+    switch (base) {
+        case NOA_HIP_INTERP_NEAREST: case NOA_HIP_INTERP_LINEAR: return 2;
+        case NOA_HIP_INTERP_CUBIC: case NOA_HIP_INTERP_CUBIC_BSPLINE: case NOA_HIP_INTERP_LANCZOS4: return 4;
+        case NOA_HIP_INTERP_LANCZOS6: return 6;
+        default: return 8;
+    }
+}
+
+template<class R>
+void window_weights(int interp, R f, R* w) {
+    const int base = interp >= 100 ? interp - 100 : interp;
+    if (base == NOA_HIP_INTERP_CUBIC) {
+        const R A = static_cast<R>(-0.5);
+        w[0] = ((A * (f + 1) - 5 * A) * (f + 1) + 8 * A) * (f + 1) - 4 * A;
+        w[1] = ((A + 2) * f - (A + 3)) * f * f + 1;
+        w[2] = ((A + 2) * (1 - f) - (A + 3)) * (1 - f) * (1 - f) + 1;
+        w[3] = 1 - w[0] - w[1] - w[2];
+    } else if (base == NOA_HIP_INTERP_CUBIC_BSPLINE) {
+        const R one_sixth = static_cast<R>(1) / static_cast<R>(6);
+        const R two_third = static_cast<R>(2) / static_cast<R>(3);
+        const R one_minus = 1 - f;
+        const R one_squared = one_minus * one_minus;
+        const R squared = f * f;
+        w[0] = one_sixth * one_squared * one_minus;
+        w[1] = two_third - static_cast<R>(0.5) * squared * (2 - f);
+        w[2] = two_third - static_cast<R>(0.5) * one_squared * (2 - one_minus);
+        w[3] = one_sixth * squared * f;
+    } else {   // LANCZOS4, 6, 8
+        const int size = window_size(interp);
+        const int center = size / 2 - 1;
+        const R PI = static_cast<R>(3.1415926535897932384626433832795);
+        const R s45 = static_cast<R>(0.7071067811865475);
+        const R cs[8][2] = {{1, 0}, {-s45, -s45}, {0, 1}, {s45, -s45}, {-1, 0}, {s45, s45}, {0, -1}, {-s45, s45}};
+        const R eps = std::numeric_limits<R>::epsilon();
+        for (int i = 0; i < size; ++i) w[i] = 0;
+        R sum = 0;
+        if (f <= eps) {
+            w[center] = 1;
+            sum = 1;
+        } else if (f >= 1 - eps) {
+            w[center + 1] = 1;
+            sum = 1;
+        } else {
+            const R x0 = -(f + static_cast<R>(center));
+            const R y0 = x0 * PI * static_cast<R>(0.25);
+            const R sn = std::sin(y0), cn = std::cos(y0);
+            for (int i = 0; i < size; ++i) {
+                const R fij = -(f + static_cast<R>(center) - static_cast<R>(i));
+                const R yj = fij * PI * static_cast<R>(0.25);
+                w[i] = (cs[i][0] * sn + cs[i][1] * cn) / (yj * yj);
+                sum += w[i];
+            }
+        }
+        const R inv = 1 / sum;
+        for (int i = 0; i < size; ++i) w[i] *= inv;
+    }
+}
+
+// interpolate_spectrum for windows larger than 2 (Interpolation.hpp:435-526): the flip is decided per tap (a tap
+// at x < 0 reads the conjugate of the mirrored element), taps outside the bounds contribute 0.
+template<class V, class Coord>
+V interpolate_spectrum_window_2d(const Reader3<V>& in, i64 batch, Coord fy, Coord fx, i64 ny, i64 nx, bool centered,
+                                 int interp) {
+    using R = typename real_of<V>::type;
+    const int size = window_size(interp);
+    const i64 start = -(size - 1) / 2;
+    const i64 lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
+    const Coord fly = std::floor(fy), flx = std::floor(fx);
+    const i64 iy = static_cast<i64>(fly), ix = static_cast<i64>(flx);
+    R wy[8], wx[8];
+    window_weights<R>(interp, static_cast<R>(fy - fly), wy);
+    window_weights<R>(interp, static_cast<R>(fx - flx), wx);
+    V value = zero<V>();
+    for (int y = 0; y < size; ++y) {
+        V vy = zero<V>();
+        for (int x = 0; x < size; ++x) {
+            i64 ty = iy + start + y, tx = ix + start + x;
+            R conj = 1;
+            if (tx < 0) { ty = -ty; tx = -tx; conj = -1; }
+            if (ty < lo_y || ty > hi_y || tx > hi_x) continue;
+            V v = vcast<V>(in(batch, frequency2index(centered, ty, ny), tx));
+            if constexpr (is_cx<V>::value) v.im *= conj;
+            vy += v * wx[x];
+        }
+        value += vy * wy[y];
+    }
+    return value;
+}
+
+template<class V, class Coord>
+V interpolate_spectrum_window_3d(const Reader3<V>& in, Coord fz, Coord fy, Coord fx, i64 nz, i64 ny, i64 nx,
+                                 bool centered, int interp) {
+    using R = typename real_of<V>::type;
+    const int size = window_size(interp);
+    const i64 start = -(size - 1) / 2;
+    const i64 lo_z = -nz / 2, hi_z = (nz - 1) / 2, lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
+    const Coord flz = std::floor(fz), fly = std::floor(fy), flx = std::floor(fx);
+    const i64 iz = static_cast<i64>(flz), iy = static_cast<i64>(fly), ix = static_cast<i64>(flx);
+    R wz[8], wy[8], wx[8];
+    window_weights<R>(interp, static_cast<R>(fz - flz), wz);
+    window_weights<R>(interp, static_cast<R>(fy - fly), wy);
+    window_weights<R>(interp, static_cast<R>(fx - flx), wx);
+    V value = zero<V>();
+    for (int z = 0; z < size; ++z) {
+        V vz = zero<V>();
+        for (int y = 0; y < size; ++y) {
+            V vy = zero<V>();
+            for (int x = 0; x < size; ++x) {
+                i64 tz = iz + start + z, ty = iy + start + y, tx = ix + start + x;
+                R conj = 1;
+                if (tx < 0) { tz = -tz; ty = -ty; tx = -tx; conj = -1; }
+                if (tz < lo_z || tz > hi_z || ty < lo_y || ty > hi_y || tx > hi_x) continue;
+                V v = vcast<V>(in(0, frequency2index(centered, tz, nz), frequency2index(centered, ty, ny), tx));
+                if constexpr (is_cx<V>::value) v.im *= conj;
+                vy += v * wx[x];
+            }
+            vz += vy * wy[y];
+        }
+        value += vz * wz[z];
+    }
+    return value;
+}
+
 template<class V, class Coord>
 V interpolate_spectrum_2d(const Reader3<V>& in, i64 batch, Coord fy, Coord fx, i64 ny, i64 nx,
                           bool centered, int interp) {
+    if (window_size(interp) > 2)
+        return interpolate_spectrum_window_2d<V, Coord>(in, batch, fy, fx, ny, nx, centered, interp);
     using R = typename real_of<V>::type;
     R conjugate = 1;
     if (fx < 0) { fy *= -1; fx *= -1; conjugate = -1; }    // flip_frequency (:417-426), Vec *= -1
@@ -286,6 +416,8 @@ V interpolate_spectrum_2d(const Reader3<V>& in, i64 batch, Coord fy, Coord fx, i
 template<class V, class Coord>
 V interpolate_spectrum_3d(const Reader3<V>& in, Coord fz, Coord fy, Coord fx, i64 nz, i64 ny, i64 nx,
                           bool centered, int interp) {
+    if (window_size(interp) > 2)
+        return interpolate_spectrum_window_3d<V, Coord>(in, fz, fy, fx, nz, ny, nx, centered, interp);
     using R = typename real_of<V>::type;
     R conjugate = 1;
     if (fx < 0) { fz *= -1; fy *= -1; fx *= -1; conjugate = -1; }
@@ -1054,9 +1186,9 @@ int dispatch(Op op, const noa_hip_project_args* a, int n_threads, bool force) {
     g_error.clear();
     if (!a) { g_error = "oracle: null args"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
     const int base = a->interp >= 100 ? a->interp - 100 : a->interp;
-    if (op != RASTERIZE && base != NOA_HIP_INTERP_LINEAR && base != NOA_HIP_INTERP_NEAREST) {
-        g_error = "oracle: only NEAREST and LINEAR are restated";
-        return NOA_HIP_ERROR_UNSUPPORTED;
+    if (op != RASTERIZE && (base < NOA_HIP_INTERP_NEAREST || base > NOA_HIP_INTERP_LANCZOS8)) {
+        g_error = "oracle: invalid interpolation method";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
     }
     const int k = a->rotation.kind;
     const bool f64 = k == NOA_HIP_MAT33_F64 || k == NOA_HIP_QUAT_F64;
@@ -1299,9 +1431,9 @@ int oracle_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_arg
     g_error.clear();
     if (!a) { g_error = "oracle: null args"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
     const int base = a->interp >= 100 ? a->interp - 100 : a->interp;
-    if (base != NOA_HIP_INTERP_LINEAR && base != NOA_HIP_INTERP_NEAREST) {
-        g_error = "oracle: only NEAREST and LINEAR are restated";
-        return NOA_HIP_ERROR_UNSUPPORTED;
+    if (base < NOA_HIP_INTERP_NEAREST || base > NOA_HIP_INTERP_LANCZOS8) {
+        g_error = "oracle: invalid interpolation method";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
     }
     const int k = a->input_rotation.kind;
     if (k < NOA_HIP_MAT33_F32 || k > NOA_HIP_QUAT_F64) { g_error = "oracle: invalid rotation kind"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
@@ -1405,6 +1537,23 @@ void oracle_ctf_anisotropic_make(int f64, const double pixel_size[2], double def
     }
 }
 double oracle_relativistic_electron_wavelength(double volts) { return oracle::relativistic_electron_wavelength(volts); }
+
+/* interpolation_weights of a window of 4 to 8 taps (CUBIC, CUBIC_BSPLINE, LANCZOS4/6/8) at fraction f; returns the
+ * window size. */
+int oracle_window_weights_f64(int interp, double f, double* out) {
+    double w[8];
+    oracle::window_weights<double>(interp, f, w);
+    const int n = oracle::window_size(interp);
+    for (int i = 0; i < n; ++i) out[i] = w[i];
+    return n;
+}
+int oracle_window_weights_f32(int interp, float f, float* out) {
+    float w[8];
+    oracle::window_weights<float>(interp, f, w);
+    const int n = oracle::window_size(interp);
+    for (int i = 0; i < n; ++i) out[i] = w[i];
+    return n;
+}
 
 /* Pinned helpers, exposed for the oracle's own tests. */
 long long oracle_index2frequency(int centered, long long index, long long size) {

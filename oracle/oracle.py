@@ -413,3 +413,17 @@ def ctf_anisotropic_2d(remap, input, output, shape, ctfs, *, fftfreq_range=(0.0,
         int(bool(fftfreq_range[2]) if len(fftfreq_range) > 2 else 1), int(bool(ctf_abs)), int(bool(ctf_squared)))
     if status != 0:
         raise OracleError(lib().oracle_last_error().decode())
+
+
+def window_weights(interp, f, f32=False):
+    """interpolation_weights of CUBIC, CUBIC_BSPLINE, LANCZOS4/6/8 at fraction f (Interpolation.hpp:81-141)."""
+    L = lib()
+    if f32:
+        out = (ctypes.c_float * 8)()
+        L.oracle_window_weights_f32.argtypes = [ctypes.c_int, ctypes.c_float, c_void_p]
+        n = L.oracle_window_weights_f32(int(interp), f, ctypes.addressof(out))
+        return np.array(out[:n], np.float32)
+    out = (c_double * 8)()
+    L.oracle_window_weights_f64.argtypes = [ctypes.c_int, c_double, c_void_p]
+    n = L.oracle_window_weights_f64(int(interp), f, ctypes.addressof(out))
+    return np.array(out[:n], np.float64)

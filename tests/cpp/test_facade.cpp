@@ -96,11 +96,11 @@ static void check_errors() {
         EXPECT(std::strstr(e.what(), "shape") != nullptr);
     }
     EXPECT(thrown);
-    // CUBIC is a valid noa call that this library does not implement.
+    // An interpolation method outside noa's Interp values.
     thrown = false;
     try {
         nx::ExtractCentralSlicesOptions opt;
-        opt.interp = nx::Interp::CUBIC;
+        opt.interp = static_cast<nx::Interp>(9);
         nx::extract_central_slices_3d<"hc2hc">(v, Empty{}, Shape4(1, 32, 32, 32), s, Empty{}, Shape4(2, 1, 32, 32),
                                              Empty{}, eye, opt);
     } catch (const noa_hip::Exception& e) {

@@ -144,7 +144,7 @@ def _ie_args(lib, n=16, b_in=3, b_out=2):
     (lambda a: setattr(a.output_rotation, "count", 3), "output_rotation"),
     (lambda a: setattr(a, "correct_weights", 1) or setattr(a, "add_to_output", 1), "correct_weights"),
     (lambda a: setattr(a, "correct_weights", 1) or setattr(a, "w_fftfreq_blackman", 0.3), "correct_weights"),
-    (lambda a: setattr(a, "interp", 2), "nterp"),
+    (lambda a: setattr(a, "interp", 7), "nterp"),
     (lambda a: setattr(a.output_rotation, "kind", 2), "same precision"),
 ])
 def test_insert_extract_validation_errors(lib, mutate, message):
@@ -201,7 +201,7 @@ def _args(lib, n=16, b=2, remap=0):
     (lambda a: a.input_shape.__setitem__(1, 8), "shape"),
     (lambda a: setattr(a, "remap", 1), "emap"),
     (lambda a: setattr(a.rotation, "count", 3), "rotation"),
-    (lambda a: setattr(a, "interp", 3), "nterp"),
+    (lambda a: setattr(a, "interp", 9), "nterp"),
 ])
 def test_validation_errors_before_any_device_access(lib, mutate, message):
     from noa_amd import abi

@@ -311,3 +311,91 @@ def test_insert_oversampled_and_odd_sizes():
         nx().insert_central_slices_3d("hc2hc", to_dev(s), None, (b, 1, h, h), vol, None, (1, n, n, n), None,
                                       to_dev(inv), opts)
         assert rel_l2(cpu(vol), want) < 1e-6
+
+
+# ------------------------------------------------------------------------------------------------ windows > 2
+# CUBIC (2), CUBIC_BSPLINE (3), LANCZOS4/6/8 (4, 5, 6): per-tap flip, 4-8 taps per dimension. CUBIC and the
+# B-spline are plain arithmetic (bit-identical to the oracle); Lanczos evaluates sin/cos (ulps apart).
+@pytest.mark.parametrize("interp", [2, 3, 4, 5, 6, 102])
+@pytest.mark.parametrize("remap", ["hc2hc", "h2h"])
+def test_extract_window_methods(interp, remap):
+    print(f"case extract window interp={interp} {remap}", flush=True)
+    rng = np.random.default_rng(140)
+    n, b = 24, 4
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)
+    rot = random_rotations(b, rng)
+    want = np.zeros((b, 1, n, n // 2 + 1), np.complex64)
+    O.extract_central_slices_3d(remap, vol, None, (1, n, n, n), want, None, (b, 1, n, n), None, rot, interp=interp)
+    out = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    opts = nx().ExtractCentralSlicesOptions(interp=nx().Interp(interp))
+    nx().extract_central_slices_3d(remap, to_dev(vol), None, (1, n, n, n), out, None, (b, 1, n, n), None,
+                                   to_dev(rot), opts)
+    got = cpu(out)
+    if interp % 100 in (2, 3):
+        np.testing.assert_array_equal(got, want)
+    else:
+        assert rel_l2(got, want) < 1e-6
+
+
+def test_extract_window_methods_weights_w_window_f64():
+    print("case extract window cubic + weights + w-window / f64", flush=True)
+    rng = np.random.default_rng(141)
+    n, b = 20, 3
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)
+    wvol = rng.uniform(0.5, 2, (1, n, n, n // 2 + 1)).astype(np.float32)
+    rot = random_rotations(b, rng)
+    want = np.zeros((b, 1, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((b, 1, n, n // 2 + 1), np.float32)
+    O.extract_central_slices_3d("hc2hc", vol, wvol, (1, n, n, n), want, wwant, (b, 1, n, n), None, rot, interp=2,
+                                fftfreq_sinc=0.1, fftfreq_blackman=0.15)
+    out = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    wout = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+    opts = nx().ExtractCentralSlicesOptions(interp=nx().Interp(2), w_windowed_sinc=nx().WindowedSinc(0.1, 0.15))
+    nx().extract_central_slices_3d("hc2hc", to_dev(vol), to_dev(wvol), (1, n, n, n), out, wout, (b, 1, n, n), None,
+                                   to_dev(rot), opts)
+    assert rel_l2(cpu(out), want) < 1e-6 and rel_l2(cpu(wout), wwant) < 1e-6
+    vol64 = vol.astype(np.complex128)
+    rot64 = rot.astype(np.float64)
+    want64 = np.zeros((b, 1, n, n // 2 + 1), np.complex128)
+    O.extract_central_slices_3d("h2h", vol64, None, (1, n, n, n), want64, None, (b, 1, n, n), None, rot64, interp=3)
+    out64 = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.complex128, device=DEV)
+    nx().extract_central_slices_3d("h2h", to_dev(vol64), None, (1, n, n, n), out64, None, (b, 1, n, n), None,
+                                   to_dev(rot64), nx().ExtractCentralSlicesOptions(interp=nx().Interp(3)))
+    np.testing.assert_array_equal(cpu(out64), want64)
+
+
+@pytest.mark.parametrize("interp", [2, 3, 5])
+def test_insert_window_methods(interp):
+    print(f"case insert window interp={interp}", flush=True)
+    rng = np.random.default_rng(142)
+    n, b = 24, 10
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("hc2h", s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv, interp=interp,
+                               fftfreq_sinc=0.05, fftfreq_blackman=0.08)
+    opts = nx().InsertCentralSlicesOptions(interp=nx().Interp(interp), windowed_sinc=nx().WindowedSinc(0.05, 0.08))
+    v, wv = _insert_dev("hc2h", s, w, (b, 1, n, n), n, inv, None, opts)
+    assert rel_l2(v, want) < 1e-6 and rel_l2(wv, wwant) < 1e-6
+
+
+@pytest.mark.parametrize("interp", [2, 3, 6])
+def test_insert_extract_window_methods(interp):
+    print(f"case insert_extract window interp={interp}", flush=True)
+    rng = np.random.default_rng(143)
+    n, b_in, b_out = 24, 12, 3
+    s = random_spectrum((b_in, 1, n, n // 2 + 1), rng)
+    rin = random_rotations(b_in, rng)
+    rout = random_rotations(b_out, rng)
+    want = np.zeros((b_out, 1, n, n // 2 + 1), np.complex64)
+    O.insert_and_extract_central_slices_3d("hc2hc", s, None, (b_in, 1, n, n), want, None, (b_out, 1, n, n), None,
+                                           rin, None, rout, interp=interp, input_fftfreq_sinc=0.05,
+                                           input_fftfreq_blackman=0.1)
+    out = torch.zeros((b_out, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+    opts = nx().InsertAndExtractCentralSlicesOptions(interp=nx().Interp(interp),
+                                                     input_windowed_sinc=nx().WindowedSinc(0.05, 0.1))
+    nx().insert_and_extract_central_slices_3d("hc2hc", to_dev(s), None, (b_in, 1, n, n), out, None, (b_out, 1, n, n),
+                                              None, to_dev(rin), None, to_dev(rout), opts)
+    assert rel_l2(cpu(out), want) < 1e-6

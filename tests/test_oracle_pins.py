@@ -859,3 +859,70 @@ def test_ctf_options_layouts_and_input():
     O.ctf_anisotropic_2d("h2h", None, full, (1, 1, 128, 128), c64)
     fy = freq_grid(64, False)
     np.testing.assert_array_equal(part[0, 0], full[0, 0][np.where(fy < 0, fy + 128, fy)][:, :33])
+
+
+# ------------------------------------------------------------------------------------------------ windows > 2
+# CUBIC, CUBIC_BSPLINE and LANCZOS spectrum interpolation (Interpolation.hpp:81-141, :435-526).
+CUBIC, BSPLINE, LANCZOS4, LANCZOS6, LANCZOS8 = 2, 3, 4, 5, 6
+
+
+def _keys(d, a=-0.5):
+    d = np.abs(d)
+    return np.where(d <= 1, (a + 2) * d ** 3 - (a + 3) * d ** 2 + 1,
+                    np.where(d < 2, a * d ** 3 - 5 * a * d ** 2 + 8 * a * d - 4 * a, 0.0))
+
+
+def _bspline(d):
+    d = np.abs(d)
+    return np.where(d <= 1, 2 / 3 - d ** 2 + d ** 3 / 2, np.where(d < 2, (2 - d) ** 3 / 6, 0.0))
+
+
+@pytest.mark.parametrize("f", [0.0, 0.1, 0.37, 0.5, 0.81, 0.999])
+def test_window_weights_are_the_textbook_kernels(f):
+    """Keys' cubic (a = -0.5) and the cubic B-spline at the tap distances 1+f, f, 1-f, 2-f; Lanczos as
+    sinc(d) sinc(d / 4) at d = i - center - f (noa uses the a = 4 kernel for every window), normalised."""
+    d4 = np.array([1 + f, f, 1 - f, 2 - f])
+    np.testing.assert_allclose(O.window_weights(CUBIC, f), _keys(d4), atol=1e-14)
+    np.testing.assert_allclose(O.window_weights(BSPLINE, f), _bspline(d4), atol=1e-14)
+    for mode, size in ((LANCZOS4, 4), (LANCZOS6, 6), (LANCZOS8, 8)):
+        center = size // 2 - 1
+        d = np.arange(size) - center - f
+        lz = np.sinc(d) * np.sinc(d / 4)
+        np.testing.assert_allclose(O.window_weights(mode, f), lz / lz.sum(), atol=1e-12)
+        assert O.window_weights(mode, f, f32=True).dtype == np.float32
+
+
+@pytest.mark.parametrize("mode", [CUBIC, BSPLINE, LANCZOS4, LANCZOS6, LANCZOS8])
+def test_window_interpolation_of_spectra(mode):
+    """Integer frequencies: CUBIC and Lanczos return the sample, the B-spline the [1/6, 2/3, 1/6] smoothing (taps
+    beyond x = 0 read the conjugate of the mirrored element). CUBIC and the B-spline reproduce a Hermitian linear
+    field c + i a.k exactly inside the bounds (linear precision); Lanczos (normalised) to 1e-2 of the field."""
+    rng = np.random.default_rng(47)
+    n = 24
+    s = random_spectrum((1, n, n // 2 + 1), rng)
+    freqs_int = np.array([[3, 5], [-4, 2], [0, 7], [6, 3]], np.float32)
+    got = O.interpolate_spectrum_2d(s, 0, freqs_int, (n, n), True, interp=mode)
+    fy = freq_grid(n, True)
+
+    def at(y, x):   # the spectrum at centred frequency (y, x), through Hermitian symmetry for x < 0
+        if x < 0:
+            return np.conj(s[0][np.where(fy == -y)[0][0], -x])
+        return s[0][np.where(fy == y)[0][0], x]
+
+    for (y, x), g in zip(freqs_int.astype(int), got):
+        if mode == BSPLINE:
+            k = np.array([1 / 6, 2 / 3, 1 / 6])
+            want = sum(k[a] * k[b] * at(y + a - 1, x + b - 1) for a in range(3) for b in range(3))
+        else:
+            want = at(y, x)
+        np.testing.assert_allclose(g, want, rtol=2e-6, atol=2e-6)
+    # A Hermitian linear field (imaginary part odd, real part even).
+    c, a = 0.5, np.array([0.9, -0.3])
+    v = fy[:, None].astype(np.float64)
+    u = np.arange(n // 2 + 1)[None, :].astype(np.float64)
+    field = (c + 1j * (a[0] * v + a[1] * u)).astype(np.complex64)[None]
+    pts = rng.uniform(-5, 5, (40, 2)).astype(np.float32)
+    got = O.interpolate_spectrum_2d(field, 0, pts, (n, n), True, interp=mode)
+    want = c + 1j * (pts[:, 0].astype(np.float64) * a[0] + pts[:, 1] * a[1])
+    tol = 1e-5 if mode in (CUBIC, BSPLINE) else 1e-2
+    np.testing.assert_allclose(got, want, atol=tol * 5)
