@@ -264,6 +264,19 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
         "avg_ms": t * 1e3, "output_slices_per_s": b_out / t, "pixel_slice_pairs_per_s": pairs / t,
         "nonzero_weight_fraction": (wout != 0).float().mean().item()}
     log(f"virtual volume: {t * 1e3:.3f} ms, {pairs / t / 1e9:.1f} G pixel-slice pairs/s")
+    del out, wout
+
+    # Cubic B-spline prefilter of a 256^3 f32 volume in place (before CUBIC_BSPLINE interpolation): three passes
+    # (x, y, z), each reading and writing the volume once.
+    vol = torch.randn((1, n, n, n), device=dev, dtype=torch.float32)
+    t = timed(lambda: nx.cubic_bspline_prefilter(vol, vol))
+    nbytes = 3 * 2 * vol.numel() * 4
+    rep["bspline_prefilter"] = {
+        "workload": f"cubic_bspline_prefilter of a {n}^3 f32 volume in place (x, y, z passes)",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
+    log(f"bspline prefilter: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del vol
     return rep
 
 

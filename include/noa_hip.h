@@ -286,6 +286,14 @@ int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* 
 int noa_hip_fourier_interpolation_correction(const noa_hip_view* input, const noa_hip_view* output, int32_t interp,
                                              int32_t post_correction, void* stream);
 
+/* Replaces the GPU branch of noa::xform::cubic_bspline_prefilter (src/noa/xform/CubicBSplinePrefilter.hpp:24-63,
+ * algorithm core/CubicBSplinePrefilter.hpp:22-107): the recursive cubic B-spline prefilter along x, then y, then z
+ * of the dimensions reordered to the output's rightmost order, so that CUBIC_BSPLINE interpolation of the output
+ * passes through the input samples. (B, D, H, W) views (shape and element strides) of f32, f64, c32 or c64; the
+ * input is broadcast into the output's shape (dimensions of size 1) and may be the output (in place, same data
+ * pointer); any other overlap is rejected. */
+int noa_hip_cubic_bspline_prefilter(const noa_hip_view* input, const noa_hip_view* output, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -392,6 +392,14 @@ void fourier_interpolation_correction(const Input& input, const Output& output, 
                                                             post_correction ? 1 : 0, stream));
 }
 
+/* noa::xform::cubic_bspline_prefilter (CubicBSplinePrefilter.hpp:24): f32, f64, c32 or c64 (B, D, H, W) arrays;
+ * the input may be broadcast into the output and may be the output. */
+template<class Input, class Output>
+void cubic_bspline_prefilter(const Input& input, const Output& output, void* stream = nullptr) {
+    const noa_hip_view in = details::view(input), out = details::view(output);
+    details::check(noa_hip_cubic_bspline_prefilter(&in, &out, stream));
+}
+
 namespace details {
     /* One (y, x) shift on the host, or a View of (y, x) pairs (Vec<f32,2> / Vec<f64,2>) on the device. */
     template<class T> struct Vec2 { T y, x; };

@@ -199,4 +199,6 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_remap.restype = ctypes.c_int
     lib.noa_hip_fourier_interpolation_correction.argtypes = [V, V, c_int32, c_int32, c_void_p]
     lib.noa_hip_fourier_interpolation_correction.restype = ctypes.c_int
+    lib.noa_hip_cubic_bspline_prefilter.argtypes = [V, V, c_void_p]
+    lib.noa_hip_cubic_bspline_prefilter.restype = ctypes.c_int
     return lib

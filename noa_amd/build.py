@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "insert_extract.hip", "phase_shift.hip",
-           "spectrum.hip", "ctf.hip"]
+           "spectrum.hip", "ctf.hip", "bspline.hip"]
 
 
 def hipcc() -> str:

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <string>
 
 #include "../../include/noa_hip.h"
@@ -509,6 +510,60 @@ GriddingDesc validate_gridding(const noa_hip_view* in, const noa_hip_view* out, 
     return d;
 }
 
+// cubic_bspline_prefilter (xform/CubicBSplinePrefilter.hpp:24-63): non-empty arrays of one of f32, f64, c32, c64 on
+// one device, the input broadcast into the output (runtime/Utils.hpp:106-117). The dimensions are then reordered to
+// the output's rightmost order (cpu/CubicBSplinePrefilter.hpp:136-141, Shape.hpp:871-886), and ndim() of the
+// reordered shape (Shape.hpp:315-335) picks the passes. In place when the data pointers are equal; other overlaps
+// are rejected, where noa would race.
+BSplineDesc validate_bspline(const noa_hip_view* in, const noa_hip_view* out, int& device) {
+    NOA_CHECK(in != nullptr && out != nullptr, INVALID, "null arguments");
+    NOA_CHECK(in->data != nullptr && out->data != nullptr && !in->is_value && !out->is_value, INVALID,
+              "Empty array detected");
+    for (int i = 0; i < 4; ++i)
+        NOA_CHECK(in->shape[i] > 0 && out->shape[i] > 0, INVALID, "Empty array detected");
+    NOA_CHECK(out->dtype == NOA_HIP_F32 || out->dtype == NOA_HIP_F64 || out->dtype == NOA_HIP_C32 ||
+              out->dtype == NOA_HIP_C64, INVALID, "cubic_bspline_prefilter: f32, f64, c32 or c64 arrays are expected");
+    NOA_CHECK(in->dtype == out->dtype, INVALID, "cubic_bspline_prefilter: the input and output types should match");
+    BSplineDesc d;
+    d.input = to_arr4(*in);
+    d.output = to_arr4(*out);
+    for (int i = 0; i < 4; ++i) {
+        d.shape[i] = out->shape[i];
+        if (in->shape[i] == 1 && out->shape[i] != 1) d.input.s[i] = 0;
+        else NOA_CHECK(in->shape[i] == out->shape[i], INVALID, "Cannot broadcast an array of shape %s into an array of shape %s",
+                       shape_str(in->shape).c_str(), shape_str(out->shape).c_str());
+    }
+    const bool inplace = in->data == out->data;
+    NOA_CHECK(inplace || !overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
+    if (inplace) d.input = d.output;   // filter_inplace reads the output (cpu/CubicBSplinePrefilter.hpp:70-90)
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    const int in_dev = device_of(in->data, nullptr);
+    NOA_CHECK(in_dev == device, INVALID,
+              "The input and output arrays must be on the same device, but got input:device=%d, output:device=%d",
+              in_dev, device);
+    // Rightmost order: a stable sort of D, H, W by decreasing output stride, size-1 dimensions first.
+    int order[3] = {0, 1, 2};
+    int64_t key[3];
+    for (int i = 0; i < 3; ++i) key[i] = d.shape[i + 1] <= 1 ? std::numeric_limits<int64_t>::max() : d.output.s[i + 1];
+    std::stable_sort(order, order + 3, [&](int x, int y) { return key[x] > key[y]; });
+    BSplineDesc r = d;
+    for (int i = 0; i < 3; ++i) {
+        r.shape[i + 1] = d.shape[order[i] + 1];
+        r.input.s[i + 1] = d.input.s[order[i] + 1];
+        r.output.s[i + 1] = d.output.s[order[i] + 1];
+    }
+    r.ndim = r.shape[1] > 1 ? 3 : (r.shape[2] > 1 && r.shape[3] > 1) ? 2 : 1;
+    // The kernels index lines and samples with 32-bit sizes.
+    NOA_CHECK(r.shape[1] <= 0x7fffffffLL && r.shape[2] <= 0x7fffffffLL && r.shape[3] <= 0x7fffffffLL, UNSUPPORTED,
+              "cubic_bspline_prefilter: dimensions above 2^31 are not supported");
+    const int64_t rows = r.shape[0] * r.shape[1] * r.shape[2], cols = r.shape[0] * r.shape[1] * r.shape[3],
+                  pages = r.shape[0] * r.shape[2] * r.shape[3];
+    NOA_CHECK(rows / 64 < 0xffffffffLL && cols / 256 < 0xffffffffLL && pages / 256 < 0xffffffffLL, UNSUPPORTED,
+              "cubic_bspline_prefilter: too many lines for one launch");
+    return r;
+}
+
 // insert_and_extract_central_slices_3d: fourier_projection_check_parameters<INSERT_EXTRACT> (:1095-1185) and the
 // public function's own check of correct_weights (:2011-2018).
 InsertExtractDesc validate_insert_extract(const noa_hip_insert_extract_args& a, int& device) {
@@ -933,6 +988,21 @@ int noa_hip_ctf_anisotropic_2d_weighted(const noa_hip_view* input, const noa_hip
         return f.code;
     }
     return guarded(device, [&](std::string& err) { return launch_ctf(d, static_cast<hipStream_t>(stream), err); });
+}
+int noa_hip_cubic_bspline_prefilter(const noa_hip_view* input, const noa_hip_view* output, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    BSplineDesc d;
+    try {
+        d = validate_bspline(input, output, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) {
+        return launch_bspline_prefilter(d, static_cast<hipStream_t>(stream), err);
+    });
 }
 int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
                   void* stream) {

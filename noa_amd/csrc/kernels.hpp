@@ -107,6 +107,14 @@ struct RemapDesc {
     i64 shape[4]{};          // logical BDHW
     bool src_centered{}, dst_centered{};
 };
+// Cubic B-spline prefilter: dimensions already reordered to the output's rightmost order, input strides broadcast
+// (0 along broadcast dimensions); ndim of the reordered shape picks the passes (x; x, y; x, y, z).
+struct BSplineDesc {
+    Arr4 input, output;
+    i64 shape[4]{};
+    int ndim{};
+};
+
 struct GriddingDesc {
     Arr4 input, output;      // input batch stride 0 when broadcast
     i64 shape[4]{};          // BDHW of the output
@@ -131,5 +139,6 @@ int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err)
 int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
 int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err);
+int launch_bspline_prefilter(const BSplineDesc& d, hipStream_t stream, std::string& err);
 
 } // namespace noa_hip

@@ -680,6 +680,22 @@ def fourier_interpolation_correction(input, output, interp: Interp = Interp.LINE
         raise NoaError(last_error())
 
 
+def cubic_bspline_prefilter(input, output) -> None:
+    """noa::xform::cubic_bspline_prefilter (src/noa/xform/CubicBSplinePrefilter.hpp:24): prefilters `input` into
+    `output` (f32, f64, c64 or c128 device tensors of up to 4 dimensions, BDHW) so that CUBIC_BSPLINE
+    interpolation of `output` passes through the samples of `input`. The input may be broadcast into the output's
+    shape and may be the output itself (in place)."""
+    if not isinstance(output, torch.Tensor) or output.device.type != "cuda":
+        raise NoaError("cubic_bspline_prefilter: the output should be a device tensor")
+    keep: list = []
+    in_v = _view(input, keep, abi.F32, abi.C32, False, "input")
+    out_v = _view(output, keep, abi.F32, abi.C32, False, "output")
+    stream = torch.cuda.current_stream(output.device).cuda_stream
+    status = lib().noa_hip_cubic_bspline_prefilter(ctypes.byref(in_v), ctypes.byref(out_v), ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
+
+
 def rfft_shape(shape: Sequence[int]) -> tuple[int, int, int, int]:
     """Shape4::rfft (src/noa/runtime/core/Shape.hpp:380): (B, D, H, W//2+1)."""
     b, d, h, w = _shape4(shape, "shape")

@@ -1407,9 +1407,137 @@ void ctf_anisotropic_2d(const In* input, i64 in_batch, Out* output, i64 b, i64 h
     }
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// Cubic B-spline prefilter (xform/CubicBSplinePrefilter.hpp:24, cpu/CubicBSplinePrefilter.hpp:9-158, 1D recursion
+// xform/core/CubicBSplinePrefilter.hpp:22-107): the causal then anticausal recursion of the cubic B-spline pole
+// z = sqrt(3) - 2 along each line, gain (1 - z)(1 - 1/z), the causal initialisation summing the first
+// min(12, n) samples (half-sample symmetric boundary), the anticausal one z / (z - 1) * c[n-1]. Lines along x,
+// then y, then z, after the dimensions are reordered to the output's rightmost order; ndim() of the reordered
+// shape picks 1, 2 or 3 passes. Element strides; input == output filters in place.
+// ------------------------------------------------------------------------------------------------------------
+template<class T>
+struct BSpline1d {
+    using R = typename real_of<T>::type;
+    static constexpr R POLE = static_cast<R>(-0.2679491924311228);
+    static constexpr R LAMBDA = (1 - POLE) * (1 - 1 / POLE);
+
+    static T mul(R s, T v) {
+        if constexpr (is_cx<T>::value) return T{s * v.re, s * v.im};
+        else return s * v;
+    }
+    static T sub(T a, T b) {
+        if constexpr (is_cx<T>::value) return T{a.re - b.re, a.im - b.im};
+        else return a - b;
+    }
+    // Causal initialisation (core/CubicBSplinePrefilter.hpp:87-100).
+    static T initial_causal(const T* c, i64 stride, i64 size) {
+        const i64 horizon = std::min<i64>(12, size);
+        R zn = POLE;
+        T sum = *c;
+        for (i64 n = 0; n < horizon; ++n) {
+            sum = sum + mul(zn, *c);
+            zn *= POLE;
+            c += stride;
+        }
+        return sum;
+    }
+    // filter() (:60-80); filter_inplace() is the same with input == output.
+    static void filter(const T* input, i64 is, T* output, i64 os, i64 size) {
+        T* c = output;
+        T prev;
+        *c = prev = mul(LAMBDA, initial_causal(input, is, size));
+        for (i64 n = 1; n < size; ++n) {
+            input += is;
+            c += os;
+            *c = prev = mul(LAMBDA, *input) + mul(POLE, prev);
+        }
+        *c = prev = mul(POLE / (POLE - 1), *c);
+        for (i64 n = size - 2; 0 <= n; --n) {
+            c -= os;
+            *c = prev = mul(POLE, sub(prev, *c));
+        }
+    }
+};
+
+// Shape::ndim() (runtime/core/Shape.hpp:315-335) of a BDHW shape.
+inline int ndim4(const i64 s[4]) {
+    if (s[1] > 1) return 3;
+    if (s[2] > 1 && s[3] > 1) return 2;
+    return 1;
+}
+
+template<class T>
+void cubic_bspline_prefilter(const T* input, const i64 is_[4], T* output, const i64 os_[4], const i64 shape_[4]) {
+    using F = BSpline1d<T>;
+    i64 is[4], os[4], sh[4];
+    for (int i = 0; i < 4; ++i) { is[i] = is_[i]; os[i] = os_[i]; sh[i] = shape_[i]; }
+    // Strides::rightmost_order (Shape.hpp:871-886): stable sort of D, H, W by decreasing output stride, a dimension
+    // of size 1 counting as the largest stride.
+    int order[3] = {0, 1, 2};
+    i64 key[3];
+    for (int i = 0; i < 3; ++i) key[i] = sh[i + 1] <= 1 ? std::numeric_limits<i64>::max() : os[i + 1];
+    std::stable_sort(order, order + 3, [&](int a, int b) { return key[a] > key[b]; });
+    {
+        i64 ti[4] = {is[0]}, to[4] = {os[0]}, ts[4] = {sh[0]};
+        for (int i = 0; i < 3; ++i) { ti[i + 1] = is[order[i] + 1]; to[i + 1] = os[order[i] + 1]; ts[i + 1] = sh[order[i] + 1]; }
+        for (int i = 0; i < 4; ++i) { is[i] = ti[i]; os[i] = to[i]; sh[i] = ts[i]; }
+    }
+    const bool inplace = static_cast<const void*>(input) == static_cast<const void*>(output);
+    // Rows (x), from the input (or in place).
+    auto rows = [&](i64 n_outer0, i64 n_outer1, i64 n_outer2, const i64 ib[3], const i64 ob[3], i64 isx, i64 osx,
+                    i64 n) {
+        for (i64 a = 0; a < n_outer0; ++a)
+            for (i64 b = 0; b < n_outer1; ++b)
+                for (i64 c = 0; c < n_outer2; ++c) {
+                    T* o = output + a * ob[0] + b * ob[1] + c * ob[2];
+                    if (inplace) F::filter(o, osx, o, osx, n);
+                    else F::filter(input + a * ib[0] + b * ib[1] + c * ib[2], isx, o, osx, n);
+                }
+    };
+    // In place along one axis of the output.
+    auto lines = [&](i64 n0, i64 n1, i64 n2, i64 s0, i64 s1, i64 s2, i64 stride, i64 n) {
+        for (i64 a = 0; a < n0; ++a)
+            for (i64 b = 0; b < n1; ++b)
+                for (i64 c = 0; c < n2; ++c) {
+                    T* o = output + a * s0 + b * s1 + c * s2;
+                    F::filter(o, stride, o, stride, n);
+                }
+    };
+    const int nd = ndim4(sh);
+    if (nd == 3) {
+        const i64 ib[3] = {is[0], is[1], is[2]}, ob[3] = {os[0], os[1], os[2]};
+        rows(sh[0], sh[1], sh[2], ib, ob, is[3], os[3], sh[3]);                 // every row
+        lines(sh[0], sh[1], sh[3], os[0], os[1], os[3], os[2], sh[2]);          // every column
+        lines(sh[0], sh[2], sh[3], os[0], os[2], os[3], os[1], sh[1]);          // every page
+    } else if (nd == 2) {
+        const i64 ib[3] = {is[0], is[2], 0}, ob[3] = {os[0], os[2], 0};
+        rows(sh[0], sh[2], 1, ib, ob, is[3], os[3], sh[3]);
+        lines(sh[0], sh[3], 1, os[0], os[3], 0, os[2], sh[2]);
+    } else {
+        const i64 ib[3] = {is[0], 0, 0}, ob[3] = {os[0], 0, 0};
+        rows(sh[0], 1, 1, ib, ob, is[3], os[3], sh[3]);
+    }
+}
+
 } // namespace oracle
 
 extern "C" {
+
+/* noa::xform::cubic_bspline_prefilter on strided (B, D, H, W) arrays; dtype 1 f32, 2 f64, 3 c32, 4 c64. Strides in
+ * elements; input == output filters in place. */
+int oracle_cubic_bspline_prefilter(const void* input, const long long in_strides[4], void* output,
+                                   const long long out_strides[4], const long long shape[4], int dtype) {
+    using namespace oracle;
+    i64 is[4], os[4], sh[4];
+    for (int i = 0; i < 4; ++i) { is[i] = in_strides[i]; os[i] = out_strides[i]; sh[i] = shape[i]; }
+    switch (dtype) {
+        case 1: cubic_bspline_prefilter(static_cast<const float*>(input), is, static_cast<float*>(output), os, sh); return 0;
+        case 2: cubic_bspline_prefilter(static_cast<const double*>(input), is, static_cast<double*>(output), os, sh); return 0;
+        case 3: cubic_bspline_prefilter(static_cast<const Cx<float>*>(input), is, static_cast<Cx<float>*>(output), os, sh); return 0;
+        case 4: cubic_bspline_prefilter(static_cast<const Cx<double>*>(input), is, static_cast<Cx<double>*>(output), os, sh); return 0;
+        default: return NOA_HIP_ERROR_UNSUPPORTED;
+    }
+}
 
 const char* oracle_last_error(void) { return oracle::g_error.c_str(); }
 

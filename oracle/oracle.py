@@ -364,6 +364,25 @@ def remap_hx2hx(remap, input, output, shape):
         raise OracleError("remap_hx2hx: unsupported element size")
 
 
+def cubic_bspline_prefilter(input, output):
+    """noa::xform::cubic_bspline_prefilter (xform/CubicBSplinePrefilter.hpp:24, cpu/CubicBSplinePrefilter.hpp:129-158)
+    on (B, D, H, W) numpy arrays of f32, f64, c64 or c128 with any strides (a view); `input is output` filters in
+    place. The input may be broadcast (stride 0)."""
+    out = output
+    if out.ndim != 4 or input.ndim != 4 or out.dtype != input.dtype or np.dtype(out.dtype) not in _CTF_DT:
+        raise OracleError("cubic_bspline_prefilter: (B, D, H, W) arrays of one of f32, f64, c64, c128 expected")
+    item = out.itemsize
+    ist = (ctypes.c_longlong * 4)(*[s // item for s in input.strides])
+    ost = (ctypes.c_longlong * 4)(*[s // item for s in out.strides])
+    sh = (ctypes.c_longlong * 4)(*out.shape)
+    L = lib()
+    L.oracle_cubic_bspline_prefilter.restype = ctypes.c_int
+    status = L.oracle_cubic_bspline_prefilter(ctypes.c_void_p(input.ctypes.data), ist, ctypes.c_void_p(out.ctypes.data),
+                                              ost, sh, _CTF_DT[np.dtype(out.dtype)])
+    if status != 0:
+        raise OracleError("cubic_bspline_prefilter: unsupported type")
+
+
 _CTF_DT = {np.dtype(np.float32): 1, np.dtype(np.float64): 2, np.dtype(np.complex64): 3, np.dtype(np.complex128): 4}
 
 

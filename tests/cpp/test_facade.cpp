@@ -168,6 +168,24 @@ static void check_gpu() {
     std::printf("gpu insertion max abs error %.3g\n", max_err);
     EXPECT(max_err < 1e-4);
     std::printf("ok gpu insertion (identity pose, plane z = 0)\n");
+    // Prefilter of a constant (1, 8, 16, 32) volume in place: constant coefficients, up to the truncated causal
+    // start (|z|^9 of the value along the 8-sample axis).
+    {
+        const size_t np = 8 * 16 * 32;
+        float* dp = nullptr;
+        HIP_OK(hipMalloc(&dp, np * sizeof(float)));
+        std::vector<float> hp(np, 3.f);
+        HIP_OK(hipMemcpy(dp, hp.data(), np * sizeof(float), hipMemcpyHostToDevice));
+        const View<float> pv(dp, Shape4(1, 8, 16, 32));
+        nx::cubic_bspline_prefilter(pv, pv);
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipMemcpy(hp.data(), dp, np * sizeof(float), hipMemcpyDeviceToHost));
+        double perr = 0;
+        for (float v : hp) perr = std::fmax(perr, std::fabs(v - 3.0));
+        EXPECT(perr < 1e-4);
+        std::printf("ok gpu prefilter (max deviation %.3g)\n", perr);
+        HIP_OK(hipFree(dp));
+    }
     HIP_OK(hipFree(dv));
     HIP_OK(hipFree(ds));
     HIP_OK(hipFree(dins));
@@ -229,6 +247,21 @@ static void check_insert_extract_descriptor() {
     std::printf("ok insert_extract descriptor\n");
 }
 
+static void check_bspline_prefilter_errors() {
+    // An input that does not broadcast into the output is refused before any memory is touched.
+    const View<float> in(reinterpret_cast<float*>(0x10000000), Shape4(1, 4, 3, 8));
+    const View<float> out(reinterpret_cast<float*>(0x20000000), Shape4(1, 4, 4, 8));
+    bool thrown = false;
+    try {
+        nx::cubic_bspline_prefilter(in, out);
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+        EXPECT(std::strstr(e.what(), "Cannot broadcast") != nullptr);
+    }
+    EXPECT(thrown);
+}
+
 static void check_ctf_object() {
     // noa's constructor: 300 kV -> lambda = 0.0196875 A, k1 = pi * lambda, k3 = atan(A / sqrt(1 - A^2)), defocus in
     // angstroms with the sign flipped, cs in angstroms, bfactor / 4.
@@ -254,6 +287,7 @@ int main(int argc, char** argv) {
     check_phase_shift_errors();
     check_insert_extract_descriptor();
     check_ctf_object();
+    check_bspline_prefilter_errors();
     if (argc > 1 && std::strcmp(argv[1], "--gpu") == 0) check_gpu();
     std::printf("all facade checks passed\n");
     return 0;

@@ -274,3 +274,47 @@ def test_ctf_validation_errors(lib, mutate, message):
     err = lib.noa_hip_last_error().decode()
     assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
     assert message in err, err
+
+
+def _host_view(arr, dtype):
+    from noa_amd import abi
+    v = abi.View()
+    v.data = arr.ctypes.data
+    a4 = arr.reshape((1,) * (4 - arr.ndim) + arr.shape)
+    for i in range(4):
+        v.shape[i] = a4.shape[i]
+        v.strides[i] = a4.strides[i] // arr.itemsize
+    v.dtype = dtype
+    return v
+
+
+@pytest.mark.parametrize("case, message", [
+    ("f16", "f32, f64, c32 or c64"),
+    ("types", "types should match"),
+    ("broadcast", "Cannot broadcast"),
+    ("overlap", "should not overlap"),
+    ("empty", "Empty array"),
+    ("host", "not device memory"),
+])
+def test_bspline_prefilter_validation_errors(lib, case, message):
+    """cubic_bspline_prefilter's argument checks (CubicBSplinePrefilter.hpp:24-35, runtime/Utils.hpp:106-117) run
+    before anything touches a device; host pointers stop at the device check."""
+    from noa_amd import abi
+    buf = np.zeros((2, 4, 8, 16), np.float32)
+    inp = _host_view(buf[0:1], abi.F32)
+    out = _host_view(buf[1:2], abi.F32)
+    if case == "f16":
+        inp.dtype = out.dtype = abi.C16
+    elif case == "types":
+        inp.dtype = abi.F64
+    elif case == "broadcast":
+        inp.shape[2] = 3
+    elif case == "overlap":
+        inp = _host_view(buf.reshape(-1)[1:].reshape(1, 1, 1, -1)[..., :512], abi.F32)
+        out = _host_view(buf.reshape(-1)[:512].reshape(1, 1, 1, 512), abi.F32)
+    elif case == "empty":
+        out.shape[3] = 0
+    status = lib.noa_hip_cubic_bspline_prefilter(ctypes.byref(inp), ctypes.byref(out), None)
+    err = lib.noa_hip_last_error().decode()
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
+    assert message in err, err

@@ -926,3 +926,86 @@ def test_window_interpolation_of_spectra(mode):
     want = c + 1j * (pts[:, 0].astype(np.float64) * a[0] + pts[:, 1] * a[1])
     tol = 1e-5 if mode in (CUBIC, BSPLINE) else 1e-2
     np.testing.assert_allclose(got, want, atol=tol * 5)
+
+
+# ---- cubic B-spline prefilter (xform/CubicBSplinePrefilter.hpp:24, core/CubicBSplinePrefilter.hpp:22-107) ----
+
+_Z = np.sqrt(3.0) - 2.0
+
+
+def _prefilter(x):
+    out = np.empty_like(x)
+    O.cubic_bspline_prefilter(x, out)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(3, 1, 1, 40), (2, 1, 30, 26), (2, 24, 20, 28)])
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_bspline_prefilter_is_scipys_reflect_spline_filter(shape, dtype):
+    """The recursion with noa's initialisations is the cubic B-spline prefilter with half-sample symmetric
+    boundaries: scipy.ndimage.spline_filter(order=3, mode='reflect') over the same axes, up to the truncation of
+    the causal initialisation to 12 terms (|z|^13 ~ 4e-8 of the signal). A wrong pole, gain, boundary formula or a
+    missing pass fails it by orders of magnitude."""
+    import scipy.ndimage as ndi
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(shape)
+    if dtype == np.complex128:
+        x = x + 1j * rng.standard_normal(shape)
+    got = _prefilter(x.astype(dtype))
+    parts = [x.real, x.imag] if dtype == np.complex128 else [x]
+    refs = []
+    for part in parts:
+        refs.append(np.stack([ndi.spline_filter(np.squeeze(part[b]), order=3, mode="reflect").reshape(shape[1:])
+                              for b in range(shape[0])]))
+    ref = refs[0] + 1j * refs[1] if dtype == np.complex128 else refs[0]
+    assert np.abs(got - ref).max() <= 1e-6 * np.abs(x).max()
+
+
+def test_bspline_prefilter_makes_the_spline_interpolate():
+    """B-spline coefficients c of samples s reproduce them at the integers: (c[k-1] + 4 c[k] + c[k+1]) / 6 = s[k],
+    the coefficients extended by half-sample symmetry at both ends (what the anticausal start z / (z - 1) c[n-1]
+    and the causal sum encode)."""
+    rng = np.random.default_rng(12)
+    s = rng.standard_normal((1, 1, 1, 50))
+    c = _prefilter(s).ravel()
+    pad = np.concatenate([[c[0]], c, [c[-1]]])
+    rec = (pad[:-2] + 4 * pad[1:-1] + pad[2:]) / 6
+    err = np.abs(rec - s.ravel())
+    assert err[1:].max() < 1e-14
+    assert err[0] < 2 * abs(_Z) ** 13 * np.abs(s).max()   # the 12-term causal initialisation
+
+
+def test_bspline_prefilter_short_lines_closed_forms():
+    """n = 1: c = (1 - z^2) s (causal sum (1 + z) s, gain (1 - z)(1 - 1/z), anticausal z / (z - 1)).
+    n = 2: the causal sum takes both samples, s0 + z s0 + z^2 s1 (horizon = n)."""
+    one = _prefilter(np.full((1, 1, 1, 1), 2.0))
+    assert np.isclose(one.item(), 2.0 * (1 - _Z * _Z), rtol=1e-15)
+    s = np.array([[[[1.5, -0.25]]]])
+    lam = (1 - _Z) * (1 - 1 / _Z)
+    c0 = lam * (s[..., 0] + _Z * s[..., 0] + _Z ** 2 * s[..., 1])
+    c1 = lam * s[..., 1] + _Z * c0
+    c1 = _Z / (_Z - 1) * c1
+    c0 = _Z * (c1 - c0)
+    got = _prefilter(s)
+    assert np.allclose(got[..., 0], c0, rtol=1e-14) and np.allclose(got[..., 1], c1, rtol=1e-14)
+
+
+def test_bspline_prefilter_dimensionality_and_order():
+    """Shape::ndim() of the rightmost-reordered shape picks the passes: a (1, 4, 1, 8) array is filtered as the
+    2D (4, 8) image (its size-1 axis is not a pass, which would scale by 1 - z^2), a (2, 5, 1, 1) array as lines of
+    5 along D. An F-ordered output is filtered along its contiguous axis first: the result equals filtering the
+    C-ordered transpose. A batch-broadcast input gives every batch the same result."""
+    rng = np.random.default_rng(13)
+    x = rng.standard_normal((1, 4, 1, 8))
+    assert np.array_equal(_prefilter(x).reshape(4, 8), _prefilter(x.reshape(1, 1, 4, 8)).reshape(4, 8))
+    y = rng.standard_normal((2, 5, 1, 1))
+    assert np.array_equal(_prefilter(y).reshape(2, 5), _prefilter(y.reshape(2, 1, 1, 5)).reshape(2, 5))
+    v = rng.standard_normal((1, 6, 7, 9))
+    f_out = np.empty((1, 9, 7, 6)).transpose(0, 3, 2, 1)          # shape (1, 6, 7, 9), x slowest
+    O.cubic_bspline_prefilter(v, f_out)
+    t = np.ascontiguousarray(v.transpose(0, 3, 2, 1))
+    assert np.array_equal(f_out, _prefilter(t).transpose(0, 3, 2, 1))
+    b = np.broadcast_to(v, (3, 6, 7, 9))
+    out = np.empty((3, 6, 7, 9))
+    O.cubic_bspline_prefilter(b, out)
+    assert all(np.array_equal(out[i], _prefilter(v)[0]) for i in range(3))
