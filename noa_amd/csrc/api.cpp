@@ -533,6 +533,9 @@ BSplineDesc validate_bspline(const noa_hip_view* in, const noa_hip_view* out, in
         else NOA_CHECK(in->shape[i] == out->shape[i], INVALID, "Cannot broadcast an array of shape %s into an array of shape %s",
                        shape_str(in->shape).c_str(), shape_str(out->shape).c_str());
     }
+    for (int i = 0; i < 4; ++i)
+        NOA_CHECK(out->shape[i] == 1 || out->strides[i] != 0, INVALID,
+                  "cubic_bspline_prefilter: the output should not be broadcast (stride 0 along dimension %d)", i);
     const bool inplace = in->data == out->data;
     NOA_CHECK(inplace || !overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
     if (inplace) d.input = d.output;   // filter_inplace reads the output (cpu/CubicBSplinePrefilter.hpp:70-90)

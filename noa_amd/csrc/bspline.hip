@@ -8,9 +8,11 @@
 // The recursion is sequential along a line, so one lane owns one line, with noa's expressions in noa's order
 // (bit-identical to the CPU path). What differs from a lane-per-line loop over memory is the access pattern:
 //  - lines along y and z: the 64 lanes of a wave own 64 neighbouring x, so every step of the recursion is one
-//    coalesced 64-element access; eight steps are loaded ahead of the dependent arithmetic;
+//    coalesced 64-element access; steps are loaded in batches of 16, the next batch in flight while the current
+//    one's dependent arithmetic runs;
 //  - lines along x (the contiguous axis): a wave owns 64 rows and walks them in 32-column chunks staged through
-//    LDS, loaded and stored as 128-byte row segments; the lanes then run their row's recursion out of LDS.
+//    LDS, loaded and stored as whole row segments; the lanes then run their row's recursion out of LDS while the
+//    next chunk's loads are in flight.
 // Each pass reads and writes the array once; the anticausal half re-reads the causal half's output, which the
 // column passes do straight from the L2/last-level cache and the row pass from LDS for the last chunk.
 #include "common.hpp"
@@ -32,8 +34,7 @@ template<class R> struct BSplineConst {
     static constexpr R ANTI = POLE / (POLE - 1);
 };
 
-constexpr int CHUNK = 32;          // columns per LDS chunk of the row pass
-constexpr int UNROLL = 8;          // steps loaded ahead in the column passes
+constexpr int UNROLL = 16;         // steps per batch in the column passes (two batches in flight)
 
 template<class T>
 struct LinesParams {
@@ -45,11 +46,15 @@ struct LinesParams {
     i32 n;                         // samples per line
 };
 
-// Rows: one wave per 64 lines, chunks of CHUNK columns through LDS. Block = one wave.
+// Rows: one wave per 64 lines, chunks of CHUNK columns through LDS. Block = one wave. The next chunk's global
+// loads are issued into registers before the current chunk's recursion runs, so their latency overlaps it.
 template<class T>
 __global__ __launch_bounds__(64) void bspline_rows_kernel(const LinesParams<T> p) {
     using R = typename vt<T>::real;
     using K = BSplineConst<R>;
+    // Columns per LDS chunk (64 columns of 4-byte values measured no faster at 256^3 and 512^3).
+    constexpr i32 CHUNK = 32;
+    constexpr i32 RPS = 64 / CHUNK;   // rows moved per load step
     __shared__ T tile[64][CHUNK + 1];
     __shared__ i64 ibase[64], obase[64];
     const i32 lane = static_cast<i32>(threadIdx.x);
@@ -69,22 +74,37 @@ __global__ __launch_bounds__(64) void bspline_rows_kernel(const LinesParams<T> p
     }
     __syncthreads();
 
-    // Cooperative chunk copies: step k moves rows 2k and 2k+1, lanes 0-31 and 32-63 along the columns.
-    const i32 col = lane & (CHUNK - 1), rsub = lane >> 5;
-    auto load_chunk = [&](const T* src, const i64* base, i64 step, i32 c0, i32 w) {
-        if (col < w) {
-            for (i32 k = 0; k < 32; ++k) {
-                const i32 r = 2 * k + rsub;
-                if (r < nrows) tile[r][col] = src[base[r] + static_cast<i64>(c0 + col) * step];
-            }
+    // Cooperative chunk copies: step k moves rows RPS*k .. RPS*k + RPS-1, CHUNK lanes along each row.
+    const i32 col = lane % CHUNK, rsub = lane / CHUNK;
+    // 16-byte values would need 128 registers for the prefetch: they are loaded straight into LDS instead.
+    constexpr bool PREFETCH = sizeof(T) <= 8;
+    T regs[PREFETCH ? CHUNK : 1];
+    auto load_regs = [&](const T* src, const i64* base, i64 step, i32 c0, i32 w) {
+#pragma unroll
+        for (i32 k = 0; k < CHUNK; ++k) {
+            const i32 r = RPS * k + rsub;
+            if (col < w && r < nrows) regs[k] = src[base[r] + static_cast<i64>(c0 + col) * step];
+        }
+    };
+    auto regs_to_tile = [&](i32 w) {
+#pragma unroll
+        for (i32 k = 0; k < CHUNK; ++k) {
+            const i32 r = RPS * k + rsub;
+            if (col < w && r < nrows) tile[r][col] = regs[k];
+        }
+    };
+    auto load_tile = [&](const T* src, const i64* base, i64 step, i32 c0, i32 w) {
+#pragma unroll
+        for (i32 k = 0; k < CHUNK; ++k) {
+            const i32 r = RPS * k + rsub;
+            if (col < w && r < nrows) tile[r][col] = src[base[r] + static_cast<i64>(c0 + col) * step];
         }
     };
     auto store_chunk = [&](i32 c0, i32 w) {
-        if (col < w) {
-            for (i32 k = 0; k < 32; ++k) {
-                const i32 r = 2 * k + rsub;
-                if (r < nrows) p.out[obase[r] + static_cast<i64>(c0 + col) * p.out_step] = tile[r][col];
-            }
+#pragma unroll
+        for (i32 k = 0; k < CHUNK; ++k) {
+            const i32 r = RPS * k + rsub;
+            if (col < w && r < nrows) p.out[obase[r] + static_cast<i64>(c0 + col) * p.out_step] = tile[r][col];
         }
     };
 
@@ -92,10 +112,14 @@ __global__ __launch_bounds__(64) void bspline_rows_kernel(const LinesParams<T> p
     const i32 last = ((n - 1) / CHUNK) * CHUNK;
     T prev = vzero<T>();
     // Causal pass (filter(), :60-73): the input's first chunk holds the min(12, n) samples of the initialisation.
+    if constexpr (PREFETCH) load_regs(p.in, ibase, p.in_step, 0, min(CHUNK, n));
     for (i32 c0 = 0; c0 < n; c0 += CHUNK) {
         const i32 w = min(CHUNK, n - c0);
-        load_chunk(p.in, ibase, p.in_step, c0, w);
+        if constexpr (PREFETCH) regs_to_tile(w);
+        else load_tile(p.in, ibase, p.in_step, c0, w);
         __syncthreads();
+        if constexpr (PREFETCH)
+            if (c0 != last) load_regs(p.in, ibase, p.in_step, c0 + CHUNK, min(CHUNK, n - c0 - CHUNK));
         if (lane < nrows) {
             T* row = tile[lane];
             i32 j = 0;
@@ -126,9 +150,12 @@ __global__ __launch_bounds__(64) void bspline_rows_kernel(const LinesParams<T> p
     for (i32 c0 = last; c0 >= 0; c0 -= CHUNK) {
         const i32 w = min(CHUNK, n - c0);
         if (c0 != last) {
-            load_chunk(p.out, obase, p.out_step, c0, w);
+            if constexpr (PREFETCH) regs_to_tile(w);
+            else load_tile(p.out, obase, p.out_step, c0, w);
             __syncthreads();
         }
+        if constexpr (PREFETCH)
+            if (c0 != 0) load_regs(p.out, obase, p.out_step, c0 - CHUNK, CHUNK);
         if (lane < nrows) {
             T* row = tile[lane];
             i32 j = w - 1;
@@ -148,7 +175,8 @@ __global__ __launch_bounds__(64) void bspline_rows_kernel(const LinesParams<T> p
     }
 }
 
-// Columns and pages, in place on the output: one lane per line, neighbouring lanes on neighbouring x.
+// Columns and pages, in place on the output: one lane per line, neighbouring lanes on neighbouring x. Steps are
+// loaded UNROLL at a time, the next batch issued before the current one's dependent arithmetic.
 template<class T>
 __global__ __launch_bounds__(256) void bspline_lines_kernel(const LinesParams<T> p) {
     using R = typename vt<T>::real;
@@ -171,34 +199,45 @@ __global__ __launch_bounds__(256) void bspline_lines_kernel(const LinesParams<T>
     }
     T prev = vmul(sum, K::LAMBDA);
     base[0] = prev;
+    T cur[UNROLL], nxt[UNROLL];
+#pragma unroll
+    for (i32 j = 0; j < UNROLL; ++j)
+        if (1 + j < n) cur[j] = base[(1 + j) * st];
     for (i32 i = 1; i < n; i += UNROLL) {
-        T v[UNROLL];
+        const i32 in = i + UNROLL;
 #pragma unroll
         for (i32 j = 0; j < UNROLL; ++j)
-            if (i + j < n) v[j] = base[(i + j) * st];
+            if (in + j < n) nxt[j] = base[(in + j) * st];
 #pragma unroll
         for (i32 j = 0; j < UNROLL; ++j) {
             if (i + j < n) {
-                prev = vadd(vmul(v[j], K::LAMBDA), vmul(prev, K::POLE));
+                prev = vadd(vmul(cur[j], K::LAMBDA), vmul(prev, K::POLE));
                 base[(i + j) * st] = prev;
             }
         }
+#pragma unroll
+        for (i32 j = 0; j < UNROLL; ++j) cur[j] = nxt[j];
     }
     // Anticausal initialisation (:102-106) from c[n-1], which `prev` holds, and recursion (:75-79).
     prev = vmul(prev, K::ANTI);
     base[(n - 1) * st] = prev;
+#pragma unroll
+    for (i32 j = 0; j < UNROLL; ++j)
+        if (n - 2 - j >= 0) cur[j] = base[(n - 2 - j) * st];
     for (i32 i = n - 2; i >= 0; i -= UNROLL) {
-        T v[UNROLL];
+        const i32 in = i - UNROLL;
 #pragma unroll
         for (i32 j = 0; j < UNROLL; ++j)
-            if (i - j >= 0) v[j] = base[(i - j) * st];
+            if (in - j >= 0) nxt[j] = base[(in - j) * st];
 #pragma unroll
         for (i32 j = 0; j < UNROLL; ++j) {
             if (i - j >= 0) {
-                prev = vmul(vsub(prev, v[j]), K::POLE);
+                prev = vmul(vsub(prev, cur[j]), K::POLE);
                 base[(i - j) * st] = prev;
             }
         }
+#pragma unroll
+        for (i32 j = 0; j < UNROLL; ++j) cur[j] = nxt[j];
     }
 }
 

@@ -294,6 +294,7 @@ def _host_view(arr, dtype):
     ("broadcast", "Cannot broadcast"),
     ("overlap", "should not overlap"),
     ("empty", "Empty array"),
+    ("outbcast", "should not be broadcast"),
     ("host", "not device memory"),
 ])
 def test_bspline_prefilter_validation_errors(lib, case, message):
@@ -314,6 +315,8 @@ def test_bspline_prefilter_validation_errors(lib, case, message):
         out = _host_view(buf.reshape(-1)[:512].reshape(1, 1, 1, 512), abi.F32)
     elif case == "empty":
         out.shape[3] = 0
+    elif case == "outbcast":
+        out.strides[1] = 0
     status = lib.noa_hip_cubic_bspline_prefilter(ctypes.byref(inp), ctypes.byref(out), None)
     err = lib.noa_hip_last_error().decode()
     assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
