@@ -277,6 +277,30 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
         "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
     log(f"bspline prefilter: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
     del vol
+
+    # FFTs on either side of the path (rocFFT plans cached as in noa): the c2r of the reconstructed 256^3 rfft
+    # volume, and the r2c of a stack of 1024 real 256^2 images into "h" slices.
+    import noa_amd.fft as nf
+    spec = torch.randn((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    real = torch.empty((1, n, n, n), dtype=torch.float32, device=dev)
+    t = timed(lambda: nf.c2r(spec, real))
+    nbytes = spec.numel() * 8 + real.numel() * 4
+    rep["fft_c2r_volume"] = {
+        "workload": f"fft::c2r of a {n}^3 volume (c32 rfft -> f32), norm FORWARD",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
+    log(f"fft c2r {n}^3: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del spec, real
+    imgs = torch.randn((1024, 1, n, n), dtype=torch.float32, device=dev)
+    out = torch.empty((1024, 1, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    t = timed(lambda: nf.r2c(imgs, out))
+    nbytes = imgs.numel() * 4 + out.numel() * 8
+    rep["fft_r2c_slices"] = {
+        "workload": f"fft::r2c of 1024 real {n}^2 images (f32 -> c32 rfft), norm FORWARD (1/N fused in the transform)",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS, "slices_per_s": 1024 / t}
+    log(f"fft r2c 1024 x {n}^2: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del imgs, out
     return rep
 
 

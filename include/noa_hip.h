@@ -89,6 +89,18 @@ enum {
     NOA_HIP_INTERP_LANCZOS8_FAST = 106
 };
 
+/* noa::fft::Norm and noa::fft::Sign (src/noa/fft/core/Transform.hpp:6-20). */
+enum {
+    NOA_HIP_NORM_FORWARD = 0,
+    NOA_HIP_NORM_ORTHO = 1,
+    NOA_HIP_NORM_BACKWARD = 2,
+    NOA_HIP_NORM_NONE = 3
+};
+enum {
+    NOA_HIP_SIGN_FORWARD = -1,
+    NOA_HIP_SIGN_BACKWARD = 1
+};
+
 /* Transform kinds (the Rotate and Scale template arguments, FourierProject.hpp:1086-1091). */
 enum {
     NOA_HIP_XFORM_NONE = 0,       /* Empty */
@@ -285,6 +297,43 @@ int noa_hip_remap(int32_t remap, const noa_hip_view* input, const noa_hip_view* 
  * LINEAR or LINEAR_FAST, as in noa. */
 int noa_hip_fourier_interpolation_correction(const noa_hip_view* input, const noa_hip_view* output, int32_t interp,
                                              int32_t post_correction, void* stream);
+
+/* noa::fft::FFTOptions (src/noa/fft/Transform.hpp:39-88); noa_hip_fft_options_init sets noa's defaults
+ * (norm FORWARD, cache_plan 1, plan_only 0, record_and_share_workspace 0). */
+typedef struct noa_hip_fft_options {
+    int32_t norm;
+    int32_t cache_plan;
+    int32_t plan_only;
+    int32_t record_and_share_workspace;
+} noa_hip_fft_options;
+
+void noa_hip_fft_options_init(noa_hip_fft_options* options);
+
+/* Replace the GPU branches of noa::fft::r2c, c2r and c2c (src/noa/fft/Transform.hpp:160-349; noa's CUDA path
+ * src/noa/fft/cuda/Transform.hpp, Plan.cpp) with rocFFT. (B, D, H, W) views with element strides (non-negative);
+ * r2c: real f32/f64 input of the logical shape into a complex output of shape (B, D, H, W/2+1) ("h" layout);
+ * c2r: the reverse (for 2D and 3D transforms the complex input is overwritten, as in noa); c2c: complex arrays of
+ * equal shapes, sign NOA_HIP_SIGN_FORWARD or _BACKWARD. Equal data pointers run in place (r2c/c2r then need noa's
+ * padded layout: the complex strides are half the real ones and the real rows hold 2 * (W/2+1) elements). The
+ * normalisation is noa's (1/N forward by default, N = D*H*W), applied by the transform itself. options may be NULL
+ * (noa's defaults). Plans are cached per host thread and device as in noa (4 plans by default). */
+int noa_hip_fft_r2c(const noa_hip_view* input, const noa_hip_view* output, const noa_hip_fft_options* options,
+                    void* stream);
+int noa_hip_fft_c2r(const noa_hip_view* input, const noa_hip_view* output, const noa_hip_fft_options* options,
+                    void* stream);
+int noa_hip_fft_c2c(const noa_hip_view* input, const noa_hip_view* output, int32_t sign,
+                    const noa_hip_fft_options* options, void* stream);
+
+/* The plan cache of the calling host thread on `device` (noa::fft::clear_cache, set_cache_limit, cache_limit,
+ * cache_size, workspace_left_to_allocate, set_workspace; src/noa/fft/Transform.hpp:90-142). clear_cache returns
+ * the number of plans released, set_cache_limit the number evicted, set_workspace the number of plans that took the
+ * caller's device buffer (which must outlive them), 0 when it is smaller than workspace_left_to_allocate. */
+int32_t noa_hip_fft_clear_cache(int32_t device);
+int32_t noa_hip_fft_set_cache_limit(int32_t count, int32_t device);
+int32_t noa_hip_fft_cache_limit(int32_t device);
+int32_t noa_hip_fft_cache_size(int32_t device);
+int64_t noa_hip_fft_workspace_left_to_allocate(int32_t device);
+int32_t noa_hip_fft_set_workspace(int32_t device, void* buffer, int64_t bytes);
 
 /* Replaces the GPU branch of noa::xform::cubic_bspline_prefilter (src/noa/xform/CubicBSplinePrefilter.hpp:24-63,
  * algorithm core/CubicBSplinePrefilter.hpp:22-107): the recursive cubic B-spline prefilter along x, then y, then z

@@ -557,6 +557,57 @@ void remap(const Input& input, const Output& output, const Shape4& shape, void* 
     const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
     xform::details::check(noa_hip_remap(REMAP.bits, &in, &out, shape.v.data(), stream));
 }
+
+/* noa::fft::Norm, Sign and FFTOptions (src/noa/fft/core/Transform.hpp:6-20, fft/Transform.hpp:39-88). */
+enum class Norm : int32_t { FORWARD = NOA_HIP_NORM_FORWARD, ORTHO = NOA_HIP_NORM_ORTHO,
+                            BACKWARD = NOA_HIP_NORM_BACKWARD, NONE = NOA_HIP_NORM_NONE };
+enum class Sign : int32_t { FORWARD = NOA_HIP_SIGN_FORWARD, BACKWARD = NOA_HIP_SIGN_BACKWARD };
+struct FFTOptions {
+    Norm norm = Norm::FORWARD;
+    bool cache_plan = true;
+    bool plan_only = false;
+    bool record_and_share_workspace = false;
+};
+
+namespace details {
+    inline noa_hip_fft_options options(const FFTOptions& o) {
+        return {static_cast<int32_t>(o.norm), o.cache_plan ? 1 : 0, o.plan_only ? 1 : 0,
+                o.record_and_share_workspace ? 1 : 0};
+    }
+}
+
+/* noa::fft::r2c, c2r and c2c (fft/Transform.hpp:160-349): real (f32/f64) <-> "h" rfft, and the full complex
+ * transform; in place when input and output share their data pointer (noa's padded layout for r2c/c2r). */
+template<class Input, class Output>
+void r2c(const Input& input, const Output& output, FFTOptions options = {}, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    const noa_hip_fft_options o = details::options(options);
+    xform::details::check(noa_hip_fft_r2c(&in, &out, &o, stream));
+}
+template<class Input, class Output>
+void c2r(const Input& input, const Output& output, FFTOptions options = {}, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    const noa_hip_fft_options o = details::options(options);
+    xform::details::check(noa_hip_fft_c2r(&in, &out, &o, stream));
+}
+template<class Input, class Output>
+void c2c(const Input& input, const Output& output, Sign sign, FFTOptions options = {}, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    const noa_hip_fft_options o = details::options(options);
+    xform::details::check(noa_hip_fft_c2c(&in, &out, static_cast<int32_t>(sign), &o, stream));
+}
+
+/* The plan cache of the calling thread (fft/Transform.hpp:90-142). */
+inline int32_t clear_cache(int32_t device) { return noa_hip_fft_clear_cache(device); }
+inline int32_t set_cache_limit(int32_t count, int32_t device) { return noa_hip_fft_set_cache_limit(count, device); }
+inline int32_t cache_limit(int32_t device) { return noa_hip_fft_cache_limit(device); }
+inline int32_t cache_size(int32_t device) { return noa_hip_fft_cache_size(device); }
+inline int64_t workspace_left_to_allocate(int32_t device) { return noa_hip_fft_workspace_left_to_allocate(device); }
+inline int32_t set_workspace(int32_t device, void* buffer, int64_t bytes) {
+    const int32_t n = noa_hip_fft_set_workspace(device, buffer, bytes);
+    if (n < 0) throw Exception(noa_hip_last_error());
+    return n;
+}
 } // namespace fft
 } // namespace noa_hip
 

@@ -153,6 +153,15 @@ class CtfOptions(ctypes.Structure):
                 ("ctf_abs", c_int32), ("ctf_squared", c_int32)]
 
 
+class FftOptions(ctypes.Structure):
+    _fields_ = [("norm", c_int32), ("cache_plan", c_int32), ("plan_only", c_int32),
+                ("record_and_share_workspace", c_int32)]
+
+
+NORM_FORWARD, NORM_ORTHO, NORM_BACKWARD, NORM_NONE = 0, 1, 2, 3
+SIGN_FORWARD, SIGN_BACKWARD = -1, 1
+
+
 class Shift2(ctypes.Structure):
     _fields_ = [
         ("data", c_void_p),
@@ -201,4 +210,21 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_fourier_interpolation_correction.restype = ctypes.c_int
     lib.noa_hip_cubic_bspline_prefilter.argtypes = [V, V, c_void_p]
     lib.noa_hip_cubic_bspline_prefilter.restype = ctypes.c_int
+    FO = ctypes.POINTER(FftOptions)
+    lib.noa_hip_fft_options_init.argtypes = [FO]
+    lib.noa_hip_fft_options_init.restype = None
+    for name in ("noa_hip_fft_r2c", "noa_hip_fft_c2r"):
+        getattr(lib, name).argtypes = [V, V, FO, c_void_p]
+        getattr(lib, name).restype = ctypes.c_int
+    lib.noa_hip_fft_c2c.argtypes = [V, V, c_int32, FO, c_void_p]
+    lib.noa_hip_fft_c2c.restype = ctypes.c_int
+    for name in ("noa_hip_fft_clear_cache", "noa_hip_fft_cache_limit", "noa_hip_fft_cache_size"):
+        getattr(lib, name).argtypes = [c_int32]
+        getattr(lib, name).restype = c_int32
+    lib.noa_hip_fft_set_cache_limit.argtypes = [c_int32, c_int32]
+    lib.noa_hip_fft_set_cache_limit.restype = c_int32
+    lib.noa_hip_fft_workspace_left_to_allocate.argtypes = [c_int32]
+    lib.noa_hip_fft_workspace_left_to_allocate.restype = c_int64
+    lib.noa_hip_fft_set_workspace.argtypes = [c_int32, c_void_p, c_int64]
+    lib.noa_hip_fft_set_workspace.restype = c_int32
     return lib

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "insert_extract.hip", "phase_shift.hip",
-           "spectrum.hip", "ctf.hip", "bspline.hip"]
+           "spectrum.hip", "ctf.hip", "bspline.hip", "fft.cpp", "fft_stage.hip"]
 
 
 def hipcc() -> str:
@@ -70,7 +70,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
         for f in futs:
             f.result()
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), f"-L{rocm / 'lib'}", "-lrocfft",
+           f"-Wl,-rpath,{rocm / 'lib'}", "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

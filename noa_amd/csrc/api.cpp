@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <limits>
 #include <string>
@@ -510,6 +511,102 @@ GriddingDesc validate_gridding(const noa_hip_view* in, const noa_hip_view* out, 
     return d;
 }
 
+// fft::r2c / c2r / c2c (fft/Transform.hpp:160-349): non-empty arrays, the type pairs noa's templates accept, the
+// shape rules of each transform, one device. Plus what rocFFT needs: non-negative strides, and for an in-place
+// real transform noa's padded layout (nf::empty, fft/Factory.hpp): the complex strides are half the real ones.
+FftDesc validate_fft(int type, const noa_hip_view* in, const noa_hip_view* out, int32_t sign,
+                     const noa_hip_fft_options* opt, int& device) {
+    NOA_CHECK(in != nullptr && out != nullptr, INVALID, "null arguments");
+    NOA_CHECK(in->data != nullptr && out->data != nullptr && !in->is_value && !out->is_value, INVALID,
+              "Empty array detected");
+    for (int i = 0; i < 4; ++i)
+        NOA_CHECK(in->shape[i] > 0 && out->shape[i] > 0, INVALID, "Empty array detected");
+    FftDesc d;
+    d.type = type;
+    int64_t expected[4];
+    if (type == FFT_R2C) {
+        NOA_CHECK((in->dtype == NOA_HIP_F32 && out->dtype == NOA_HIP_C32) ||
+                  (in->dtype == NOA_HIP_F64 && out->dtype == NOA_HIP_C64), INVALID,
+                  "r2c: the input should be f32 or f64 and the output the complex type of the same precision");
+        rfft_shape(in->shape, expected);
+        NOA_CHECK(same_shape(out->shape, expected), INVALID,
+                  "Given the real input with a shape of %s, the non-redundant shape of the complex output should be %s, "
+                  "but got %s", shape_str(in->shape).c_str(), shape_str(expected).c_str(), shape_str(out->shape).c_str());
+        for (int i = 0; i < 4; ++i) d.shape[i] = in->shape[i];
+        d.sign = NOA_HIP_SIGN_FORWARD;
+    } else if (type == FFT_C2R) {
+        NOA_CHECK((in->dtype == NOA_HIP_C32 && out->dtype == NOA_HIP_F32) ||
+                  (in->dtype == NOA_HIP_C64 && out->dtype == NOA_HIP_F64), INVALID,
+                  "c2r: the input should be c32 or c64 and the output the real type of the same precision");
+        rfft_shape(out->shape, expected);
+        NOA_CHECK(same_shape(in->shape, expected), INVALID,
+                  "Given the real output with a shape of %s, the non-redundant shape of the complex input should be %s, "
+                  "but got %s", shape_str(out->shape).c_str(), shape_str(expected).c_str(), shape_str(in->shape).c_str());
+        for (int i = 0; i < 4; ++i) d.shape[i] = out->shape[i];
+        d.sign = NOA_HIP_SIGN_BACKWARD;
+    } else {
+        NOA_CHECK((in->dtype == NOA_HIP_C32 || in->dtype == NOA_HIP_C64) && in->dtype == out->dtype, INVALID,
+                  "c2c: the input and output should both be c32 or both be c64");
+        NOA_CHECK(same_shape(in->shape, out->shape), INVALID,
+                  "The input and output shape should match (no broadcasting allowed), but got input %s and output %s",
+                  shape_str(in->shape).c_str(), shape_str(out->shape).c_str());
+        NOA_CHECK(sign == NOA_HIP_SIGN_FORWARD || sign == NOA_HIP_SIGN_BACKWARD, INVALID,
+                  "c2c: the sign should be -1 (forward) or +1 (backward), got %d", sign);
+        for (int i = 0; i < 4; ++i) d.shape[i] = in->shape[i];
+        d.sign = sign;
+    }
+    d.f64 = in->dtype == NOA_HIP_F64 || in->dtype == NOA_HIP_C64;
+    for (int i = 0; i < 4; ++i) {
+        NOA_CHECK(in->strides[i] >= 0 && out->strides[i] >= 0, UNSUPPORTED, "fft: negative strides are not supported");
+        d.in_s[i] = in->strides[i];
+        d.out_s[i] = out->strides[i];
+    }
+    // Rank (cuda/Plan.cpp:420-424): ndim of the DHW shape; a vector along D alone is refused, as in noa.
+    const int64_t D = d.shape[1], H = d.shape[2], W = d.shape[3];
+    d.rank = D > 1 ? 3 : (H > 1 && W > 1) ? 2 : 1;
+    NOA_CHECK(d.rank != 3 || H > 1 || W > 1, UNSUPPORTED,
+              "fft: a vector along the depth dimension is not supported (shape %s)", shape_str(d.shape).c_str());
+    // noa turns a column vector into a row vector for the plan, so its r2c/c2r would store W/2+1 values along a
+    // dimension of the rfft shape that holds W = 1: not supported here.
+    NOA_CHECK(type == FFT_C2C || !(d.rank == 1 && W == 1 && H > 1), UNSUPPORTED,
+              "fft: r2c and c2r of column vectors are not supported (shape %s)", shape_str(d.shape).c_str());
+    d.in = in->data;
+    d.out = out->data;
+    if (in->data == out->data) {
+        if (type == FFT_C2C) {
+            for (int i = 0; i < 4; ++i)
+                NOA_CHECK(in->strides[i] == out->strides[i], INVALID,
+                          "c2c: an in-place transform needs equal input and output strides");
+        } else {
+            const noa_hip_view& re = type == FFT_R2C ? *in : *out;
+            const noa_hip_view& cx = type == FFT_R2C ? *out : *in;
+            bool padded = re.strides[3] == 1 && cx.strides[3] == 1;
+            for (int i = 0; i < 3; ++i) padded = padded && re.strides[i] == 2 * cx.strides[i];
+            NOA_CHECK(padded, INVALID,
+                      "In-place r2c/c2r transforms need the padded layout: the real array's strides should be twice "
+                      "the complex array's, with unit x strides");
+        }
+    } else {
+        NOA_CHECK(!overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
+    }
+    const noa_hip_fft_options defaults{NOA_HIP_NORM_FORWARD, 1, 0, 0};
+    const noa_hip_fft_options& o = opt ? *opt : defaults;
+    NOA_CHECK(o.norm >= NOA_HIP_NORM_FORWARD && o.norm <= NOA_HIP_NORM_NONE, INVALID, "fft: invalid norm %d", o.norm);
+    d.norm = o.norm;
+    d.cache_plan = o.cache_plan != 0;
+    d.plan_only = o.plan_only != 0;
+    d.record_workspace = o.record_and_share_workspace != 0;
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    const int in_dev = device_of(in->data, nullptr);
+    NOA_CHECK(in_dev == device, INVALID,
+              "The input and output arrays must be on the same device, but got input:device=%d, output:device=%d",
+              in_dev, device);
+    NOA_CHECK(device < 64, UNSUPPORTED, "fft: the plan cache is limited to 64 visible devices");
+    d.device = device;
+    return d;
+}
+
 // cubic_bspline_prefilter (xform/CubicBSplinePrefilter.hpp:24-63): non-empty arrays of one of f32, f64, c32, c64 on
 // one device, the input broadcast into the output (runtime/Utils.hpp:106-117). The dimensions are then reordered to
 // the output's rightmost order (cpu/CubicBSplinePrefilter.hpp:136-141, Shape.hpp:871-886), and ndim() of the
@@ -991,6 +1088,73 @@ int noa_hip_ctf_anisotropic_2d_weighted(const noa_hip_view* input, const noa_hip
         return f.code;
     }
     return guarded(device, [&](std::string& err) { return launch_ctf(d, static_cast<hipStream_t>(stream), err); });
+}
+void noa_hip_fft_options_init(noa_hip_fft_options* options) {
+    if (!options) return;
+    options->norm = NOA_HIP_NORM_FORWARD;
+    options->cache_plan = 1;
+    options->plan_only = 0;
+    options->record_and_share_workspace = 0;
+}
+
+namespace {
+int fft_call(int type, const noa_hip_view* input, const noa_hip_view* output, int32_t sign,
+             const noa_hip_fft_options* options, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    FftDesc d;
+    try {
+        d = validate_fft(type, input, output, sign, options, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) {
+        try {
+            return launch_fft(d, static_cast<hipStream_t>(stream), err);
+        } catch (const std::exception& e) {
+            err = e.what();
+            return static_cast<int>(NOA_HIP_ERROR_RUNTIME);
+        }
+    });
+}
+} // namespace
+
+int noa_hip_fft_r2c(const noa_hip_view* input, const noa_hip_view* output, const noa_hip_fft_options* options,
+                    void* stream) {
+    return fft_call(noa_hip::FFT_R2C, input, output, NOA_HIP_SIGN_FORWARD, options, stream);
+}
+int noa_hip_fft_c2r(const noa_hip_view* input, const noa_hip_view* output, const noa_hip_fft_options* options,
+                    void* stream) {
+    return fft_call(noa_hip::FFT_C2R, input, output, NOA_HIP_SIGN_BACKWARD, options, stream);
+}
+int noa_hip_fft_c2c(const noa_hip_view* input, const noa_hip_view* output, int32_t sign,
+                    const noa_hip_fft_options* options, void* stream) {
+    return fft_call(noa_hip::FFT_C2C, input, output, sign, options, stream);
+}
+int32_t noa_hip_fft_clear_cache(int32_t device) { return noa_hip::fft_clear_cache(device); }
+int32_t noa_hip_fft_set_cache_limit(int32_t count, int32_t device) { return noa_hip::fft_set_cache_limit(count, device); }
+int32_t noa_hip_fft_cache_limit(int32_t device) { return noa_hip::fft_cache_limit(device); }
+int32_t noa_hip_fft_cache_size(int32_t device) { return noa_hip::fft_cache_size(device); }
+int64_t noa_hip_fft_workspace_left_to_allocate(int32_t device) {
+    return noa_hip::fft_workspace_left_to_allocate(device);
+}
+int32_t noa_hip_fft_set_workspace(int32_t device, void* buffer, int64_t bytes) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (buffer == nullptr || device_of(buffer, nullptr) != device) {
+        g_last_error = "The workspace should be device memory of the given device";
+        return -1;
+    }
+    int current = 0;
+    (void)hipGetDevice(&current);
+    if (current != device) (void)hipSetDevice(device);
+    std::string err;
+    const int n = fft_set_workspace(device, buffer, bytes, err);
+    if (current != device) (void)hipSetDevice(current);
+    if (n < 0) g_last_error = err;
+    return n;
 }
 int noa_hip_cubic_bspline_prefilter(const noa_hip_view* input, const noa_hip_view* output, void* stream) {
     using namespace noa_hip;

@@ -107,6 +107,22 @@ struct RemapDesc {
     i64 shape[4]{};          // logical BDHW
     bool src_centered{}, dst_centered{};
 };
+// FFTs (rocFFT): the logical BDHW shape (the real array's for r2c and c2r), element strides of each array in its own
+// element type, the rank picked by noa (1: a row or column vector, 2, 3).
+enum { FFT_R2C = 0, FFT_C2R = 1, FFT_C2C = 2 };
+struct FftDesc {
+    int type{};
+    int sign{-1};            // -1 forward, +1 backward (r2c: forward, c2r: backward)
+    bool f64{};
+    void* in{}; void* out{}; // equal: in place
+    i64 shape[4]{};
+    i64 in_s[4]{}, out_s[4]{};
+    int rank{};
+    int norm{};
+    bool cache_plan{true}, plan_only{}, record_workspace{};
+    int device{};
+};
+
 // Cubic B-spline prefilter: dimensions already reordered to the output's rightmost order, input strides broadcast
 // (0 along broadcast dimensions); ndim of the reordered shape picks the passes (x; x, y; x, y, z).
 struct BSplineDesc {
@@ -140,5 +156,14 @@ int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
 int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err);
 int launch_bspline_prefilter(const BSplineDesc& d, hipStream_t stream, std::string& err);
+int launch_fft(const FftDesc& d, hipStream_t stream, std::string& err);
+int launch_strided_copy(const void* src, const i64 ss[4], void* dst, const i64 ds[4], const i64 shape[4],
+                        bool f64, hipStream_t stream, std::string& err);
+int fft_clear_cache(int device);
+int fft_set_cache_limit(int count, int device);
+int fft_cache_limit(int device);
+int fft_cache_size(int device);
+i64 fft_workspace_left_to_allocate(int device);
+int fft_set_workspace(int device, void* buffer, i64 bytes, std::string& err);
 
 } // namespace noa_hip

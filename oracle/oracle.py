@@ -446,3 +446,64 @@ def window_weights(interp, f, f32=False):
     L.oracle_window_weights_f64.argtypes = [ctypes.c_int, c_double, c_void_p]
     n = L.oracle_window_weights_f64(int(interp), f, ctypes.addressof(out))
     return np.array(out[:n], np.float64)
+
+
+# ---- FFTs (noa::fft, src/noa/fft/Transform.hpp:145-349) -------------------------------------------------------
+# noa's CPU path calls FFTW3 (cmake/ext/fftw.cmake, tag `main`; not vendored in the reference). Its algorithm is the
+# discrete Fourier transform, X[k] = sum_n x[n] exp(sign 2 pi i k.n / N), which numpy.fft (pocketfft) evaluates
+# here in float64; noa's own part is the rank selection (fft/cuda/Plan.cpp:411-470: D, H, W of the logical shape,
+# 1D along W, or along H for a column vector) and the normalisation (Transform.hpp:146-158), restated below.
+
+FFT_NORMS = {"forward": 0, "ortho": 1, "backward": 2, "none": 3}
+
+
+def _fft_axes(shape):
+    b, d, h, w = shape
+    if d > 1:
+        return (1, 2, 3)
+    if h > 1 and w > 1:
+        return (2, 3)
+    return (2,) if (w == 1 and h > 1) else (3,)
+
+
+def _fft_scale(shape, sign, norm):
+    """Transform.hpp:146-158: the forward transform is scaled for FORWARD and ORTHO, the backward one for BACKWARD
+    and ORTHO; by 1/N or 1/sqrt(N), N = D * H * W of the logical shape."""
+    norm = FFT_NORMS[norm] if isinstance(norm, str) else int(norm)
+    n = float(shape[1] * shape[2] * shape[3])
+    scale = np.sqrt(n) if norm == 1 else n
+    if (sign < 0 and norm in (0, 1)) or (sign > 0 and norm in (2, 1)):
+        return 1.0 / scale
+    return 1.0
+
+
+def fft_r2c(x, norm="forward"):
+    """noa::fft::r2c of a real (B, D, H, W) array: the non-redundant half along W, non-centred."""
+    x = np.asarray(x, np.float64)
+    axes = _fft_axes(x.shape)
+    if axes == (2,):
+        raise OracleError("r2c of a column vector is not part of this oracle")
+    return np.fft.rfftn(x, axes=axes, norm="backward") * _fft_scale(x.shape, -1, norm)
+
+
+def fft_c2r(X, shape, norm="forward"):
+    """noa::fft::c2r into a real array of logical (B, D, H, W) shape."""
+    X = np.asarray(X, np.complex128)
+    axes = _fft_axes(shape)
+    if axes == (2,):
+        raise OracleError("c2r of a column vector is not part of this oracle")
+    s = [shape[a] for a in axes]
+    # numpy's irfftn includes 1/N; noa's unnormalised backward transform does not.
+    out = np.fft.irfftn(X, s=s, axes=axes, norm="forward")
+    return out * _fft_scale(shape, +1, norm)
+
+
+def fft_c2c(x, sign, norm="forward"):
+    """noa::fft::c2c with the exponent's sign (-1 forward, +1 backward)."""
+    x = np.asarray(x, np.complex128)
+    axes = _fft_axes(x.shape)
+    if sign < 0:
+        out = np.fft.fftn(x, axes=axes, norm="backward")
+    else:
+        out = np.fft.ifftn(x, axes=axes, norm="forward")
+    return out * _fft_scale(x.shape, sign, norm)

@@ -168,6 +168,36 @@ static void check_gpu() {
     std::printf("gpu insertion max abs error %.3g\n", max_err);
     EXPECT(max_err < 1e-4);
     std::printf("ok gpu insertion (identity pose, plane z = 0)\n");
+    // r2c of a 16x16 image (norm FORWARD: the DC term is the mean), then c2r back to the image.
+    {
+        const int m = 16, mr = m / 2 + 1;
+        float* dr = nullptr;
+        c32* dc = nullptr;
+        HIP_OK(hipMalloc(&dr, m * m * sizeof(float)));
+        HIP_OK(hipMalloc(&dc, m * mr * sizeof(c32)));
+        std::vector<float> hr(m * m);
+        double mean = 0;
+        for (int i = 0; i < m * m; ++i) { hr[i] = static_cast<float>((i * 37) % 11) - 5.f; mean += hr[i]; }
+        mean /= m * m;
+        HIP_OK(hipMemcpy(dr, hr.data(), hr.size() * sizeof(float), hipMemcpyHostToDevice));
+        const View<float> rv(dr, Shape4(1, 1, m, m));
+        const View<c32> cv(dc, Shape4(1, 1, m, mr));
+        noa_hip::fft::r2c(rv, cv);
+        HIP_OK(hipDeviceSynchronize());
+        c32 dc0;
+        HIP_OK(hipMemcpy(&dc0, dc, sizeof(c32), hipMemcpyDeviceToHost));
+        EXPECT(std::abs(dc0.real() - mean) < 1e-5 && std::abs(dc0.imag()) < 1e-5);
+        noa_hip::fft::c2r(cv, rv);
+        HIP_OK(hipDeviceSynchronize());
+        std::vector<float> back(m * m);
+        HIP_OK(hipMemcpy(back.data(), dr, back.size() * sizeof(float), hipMemcpyDeviceToHost));
+        double ferr = 0;
+        for (int i = 0; i < m * m; ++i) ferr = std::fmax(ferr, std::fabs(back[i] - hr[i]));
+        EXPECT(ferr < 1e-5);
+        std::printf("ok gpu fft round trip (max error %.3g)\n", ferr);
+        HIP_OK(hipFree(dr));
+        HIP_OK(hipFree(dc));
+    }
     // Prefilter of a constant (1, 8, 16, 32) volume in place: constant coefficients, up to the truncated causal
     // start (|z|^9 of the value along the 8-sample axis).
     {
@@ -262,6 +292,21 @@ static void check_bspline_prefilter_errors() {
     EXPECT(thrown);
 }
 
+static void check_fft_errors() {
+    // noa's r2c shape rule (fft/Transform.hpp:169-173) is checked before any memory is touched.
+    const View<float> real(reinterpret_cast<float*>(0x10000000), Shape4(1, 1, 16, 16));
+    const View<c32> bad(reinterpret_cast<c32*>(0x20000000), Shape4(1, 1, 16, 8));
+    bool thrown = false;
+    try {
+        noa_hip::fft::r2c(real, bad);
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+        EXPECT(std::strstr(e.what(), "non-redundant shape") != nullptr);
+    }
+    EXPECT(thrown);
+}
+
 static void check_ctf_object() {
     // noa's constructor: 300 kV -> lambda = 0.0196875 A, k1 = pi * lambda, k3 = atan(A / sqrt(1 - A^2)), defocus in
     // angstroms with the sign flipped, cs in angstroms, bfactor / 4.
@@ -288,6 +333,7 @@ int main(int argc, char** argv) {
     check_insert_extract_descriptor();
     check_ctf_object();
     check_bspline_prefilter_errors();
+    check_fft_errors();
     if (argc > 1 && std::strcmp(argv[1], "--gpu") == 0) check_gpu();
     std::printf("all facade checks passed\n");
     return 0;

@@ -321,3 +321,62 @@ def test_bspline_prefilter_validation_errors(lib, case, message):
     err = lib.noa_hip_last_error().decode()
     assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
     assert message in err, err
+
+
+def test_fft_options_init_sets_noas_defaults(lib):
+    """FFTOptions (fft/Transform.hpp:39-88): norm FORWARD, cache_plan true, plan_only and
+    record_and_share_workspace false."""
+    from noa_amd import abi
+    o = abi.FftOptions(9, 9, 9, 9)
+    lib.noa_hip_fft_options_init(ctypes.byref(o))
+    assert (o.norm, o.cache_plan, o.plan_only, o.record_and_share_workspace) == (abi.NORM_FORWARD, 1, 0, 0)
+
+
+@pytest.mark.parametrize("kind, case, message", [
+    ("r2c", "shape", "non-redundant shape of the complex output"),
+    ("r2c", "types", "same precision"),
+    ("c2r", "shape", "non-redundant shape of the complex input"),
+    ("c2c", "shape", "no broadcasting allowed"),
+    ("c2c", "sign", "the sign should be"),
+    ("r2c", "norm", "invalid norm"),
+    ("r2c", "column", "column vectors"),
+    ("r2c", "depth", "vector along the depth"),
+    ("r2c", "negative", "negative strides"),
+    ("r2c", "host", "not device memory"),
+])
+def test_fft_validation_errors(lib, kind, case, message):
+    """noa's checks of r2c/c2r/c2c (Transform.hpp:160-349) and this library's own (strides, vectors), all before any
+    device access."""
+    from noa_amd import abi
+    shape = (2, 1, 8, 8)
+    if case == "column":
+        shape = (2, 1, 8, 1)
+    elif case == "depth":
+        shape = (2, 8, 1, 1)
+    real = np.zeros(shape, np.float32)
+    cplx = np.zeros(shape[:3] + (shape[3] // 2 + 1,), np.complex64)
+    rv, cv = _host_view(real, abi.F32), _host_view(cplx, abi.C32)
+    o = abi.FftOptions()
+    lib.noa_hip_fft_options_init(ctypes.byref(o))
+    if case == "shape":
+        cv.shape[3] = 4
+    elif case == "types":
+        cv.dtype = abi.C64
+    elif case == "norm":
+        o.norm = 7
+    elif case == "negative":
+        rv.strides[2] = -8
+    if kind == "r2c":
+        status = lib.noa_hip_fft_r2c(ctypes.byref(rv), ctypes.byref(cv), ctypes.byref(o), None)
+    elif kind == "c2r":
+        status = lib.noa_hip_fft_c2r(ctypes.byref(cv), ctypes.byref(rv), ctypes.byref(o), None)
+    else:
+        c2 = np.zeros(shape, np.complex64)
+        a, b = _host_view(c2, abi.C32), _host_view(c2.copy(), abi.C32)
+        if case == "shape":
+            b.shape[3] = 4
+        status = lib.noa_hip_fft_c2c(ctypes.byref(a), ctypes.byref(b), 3 if case == "sign" else -1, ctypes.byref(o),
+                                     None)
+    err = lib.noa_hip_last_error().decode()
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
+    assert message in err, err

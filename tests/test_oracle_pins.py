@@ -1009,3 +1009,48 @@ def test_bspline_prefilter_dimensionality_and_order():
     out = np.empty((3, 6, 7, 9))
     O.cubic_bspline_prefilter(b, out)
     assert all(np.array_equal(out[i], _prefilter(v)[0]) for i in range(3))
+
+
+# ---- FFTs (noa::fft, fft/Transform.hpp:145-349; FFTW3's DFT) ------------------------------------------------------
+
+def _dft_matrix(n, sign):
+    k = np.arange(n)
+    return np.exp(sign * 2j * np.pi * np.outer(k, k) / n)
+
+
+@pytest.mark.parametrize("shape", [(2, 1, 1, 7), (1, 1, 5, 6), (1, 3, 4, 5), (1, 1, 6, 1)])
+def test_fft_oracle_is_the_dft(shape):
+    """Brute-force DFT, X[k] = sum_n x[n] exp(-2 pi i k.n / N) along each transformed axis (W; H and W; D, H and
+    W; H alone for a column vector), against the oracle's c2c (forward and backward sign) and r2c (the first W//2+1
+    columns). Norm NONE, so no scaling."""
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    axes = (1, 2, 3) if shape[1] > 1 else ((2, 3) if shape[2] > 1 and shape[3] > 1 else ((2,) if shape[3] == 1 else (3,)))
+    for sign in (-1, 1):
+        ref = x.copy()
+        for a in axes:
+            ref = np.moveaxis(np.tensordot(_dft_matrix(shape[a], sign), np.moveaxis(ref, a, 0), axes=(1, 0)), 0, a)
+        assert np.allclose(O.fft_c2c(x, sign, "none"), ref, rtol=0, atol=1e-10)
+    if axes != (2,):
+        xr = x.real
+        ref = xr.astype(np.complex128)
+        for a in axes:
+            ref = np.moveaxis(np.tensordot(_dft_matrix(shape[a], -1), np.moveaxis(ref, a, 0), axes=(1, 0)), 0, a)
+        assert np.allclose(O.fft_r2c(xr, "none"), ref[..., : shape[3] // 2 + 1], rtol=0, atol=1e-10)
+
+
+def test_fft_oracle_normalisation():
+    """noa's norms (Transform.hpp:146-158): FORWARD scales the forward transform by 1/N (the DC term is the mean),
+    BACKWARD the backward one, ORTHO both by 1/sqrt(N) (Parseval holds), NONE neither; a forward/backward pair with
+    the same norm (other than NONE) is the identity; NONE returns N times the input."""
+    rng = np.random.default_rng(22)
+    shape = (2, 1, 6, 10)
+    x = rng.standard_normal(shape)
+    n = 60
+    assert np.allclose(O.fft_r2c(x, "forward")[:, :, 0, 0], x.mean(axis=(1, 2, 3)).reshape(2, 1))
+    assert np.allclose(O.fft_r2c(x, "backward")[:, :, 0, 0], x.sum(axis=(1, 2, 3)).reshape(2, 1))
+    full = O.fft_c2c(x.astype(np.complex128), -1, "ortho")
+    assert np.allclose((np.abs(full) ** 2).sum(axis=(1, 2, 3)), (x ** 2).sum(axis=(1, 2, 3)))
+    for norm in ("forward", "backward", "ortho"):
+        assert np.allclose(O.fft_c2r(O.fft_r2c(x, norm), shape, norm), x, atol=1e-12)
+    assert np.allclose(O.fft_c2r(O.fft_r2c(x, "none"), shape, "none"), n * x, atol=1e-10)
