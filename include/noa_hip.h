@@ -335,6 +335,51 @@ int32_t noa_hip_fft_cache_size(int32_t device);
 int64_t noa_hip_fft_workspace_left_to_allocate(int32_t device);
 int32_t noa_hip_fft_set_workspace(int32_t device, void* buffer, int64_t bytes);
 
+/* noa::io::DataType values (src/noa/io/Encoding.hpp:19-26): the on-disk encodings of MRC files. */
+enum {
+    NOA_HIP_IO_UNKNOWN = 0,
+    NOA_HIP_IO_I8 = 1, NOA_HIP_IO_U8 = 2, NOA_HIP_IO_I16 = 3, NOA_HIP_IO_U16 = 4, NOA_HIP_IO_I32 = 5,
+    NOA_HIP_IO_U32 = 6, NOA_HIP_IO_I64 = 7, NOA_HIP_IO_U64 = 8,
+    NOA_HIP_IO_F16 = 9, NOA_HIP_IO_F32 = 10, NOA_HIP_IO_F64 = 11,
+    NOA_HIP_IO_C16 = 12, NOA_HIP_IO_C32 = 13, NOA_HIP_IO_C64 = 14,
+    NOA_HIP_IO_U4 = 15, NOA_HIP_IO_CI16 = 16
+};
+
+/* What noa's MRC encoder reads from and writes to the 1024-byte header (src/noa/io/ImageFile.cpp:14-166, :169-296):
+ * BDHW shape (a 2D image, a 3D volume (space group 1), a stack of 2D images (group 0) or of volumes (groups
+ * 401-630)), DHW spacing in Angstrom per pixel (0 = unset), the data type of the stored values (mode 0 -> I8, or U8
+ * with the IMOD flag; 1 I16, 2 F32, 3 CI16, 4 C32, 6 U16, 12 F16, 101 U4), the statistics when set, and where the
+ * data starts (1024 + the extended header). */
+typedef struct noa_hip_mrc_header {
+    int64_t shape[4];
+    double spacing[3];
+    int32_t dtype;            /* NOA_HIP_IO_* */
+    int32_t has_stats;
+    double stats[4];          /* min, max, mean, stddev */
+    int64_t data_offset;      /* bytes */
+    int32_t endian_swapped;   /* the file is big-endian (on this little-endian host) */
+    int32_t reserved;
+} noa_hip_mrc_header;
+
+/* Parses the header of an MRC file (noa::io::ImageFile(path, {.read = true}) with the MRC encoder). */
+int noa_hip_mrc_read_header(const char* path, noa_hip_mrc_header* header);
+
+/* noa::io::read_image of an MRC file into device memory (src/noa/io/IO.hpp:34-57): `output` is a f32, f64, c32 or
+ * c64 device view of the file's BDHW shape (any strides). The file is read in chunks into pinned host buffers and
+ * copied to the device while the next chunk is read; the values are decoded on the GPU with noa's rules (byte swap
+ * of big-endian files, then clamp_cast to the output type when clamp != 0, src/noa/io/Encoding.cpp:85-99,
+ * base/ClampCast.hpp). Complex files need a complex output; real files into a complex output give {value, 0}.
+ * Returns when the data is in `output` (the stream is synchronised). */
+int noa_hip_mrc_read(const char* path, const noa_hip_view* output, int32_t clamp, void* stream);
+
+/* noa::io::write_image into a new MRC file (src/noa/io/IO.hpp:119-163): `input` is a f32, f64, c32 or c64 device
+ * view; dtype is the requested NOA_HIP_IO_* encoding (UNKNOWN: F32 for real, C32 for complex input), replaced by
+ * the closest one MRC supports as noa does (ImageFile.hpp:606-628); spacing is DHW (NULL: unset); stats is NULL or
+ * {min, max, mean, stddev}. Values are encoded on the GPU (clamp_cast to the file type when clamp != 0) and written
+ * in chunks through pinned host buffers. Returns when the file is written and closed. */
+int noa_hip_mrc_write(const char* path, const noa_hip_view* input, const double* spacing, int32_t dtype,
+                      const double* stats, int32_t clamp, void* stream);
+
 /* Replaces the GPU branch of noa::xform::cubic_bspline_prefilter (src/noa/xform/CubicBSplinePrefilter.hpp:24-63,
  * algorithm core/CubicBSplinePrefilter.hpp:22-107): the recursive cubic B-spline prefilter along x, then y, then z
  * of the dimensions reordered to the output's rightmost order, so that CUBIC_BSPLINE interpolation of the output

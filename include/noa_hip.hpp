@@ -609,6 +609,46 @@ inline int32_t set_workspace(int32_t device, void* buffer, int64_t bytes) {
     return n;
 }
 } // namespace fft
+
+namespace io {
+/* noa::io::DataType (src/noa/io/Encoding.hpp:19-26). */
+enum class DataType : int32_t {
+    UNKNOWN = NOA_HIP_IO_UNKNOWN, I8 = NOA_HIP_IO_I8, U8 = NOA_HIP_IO_U8, I16 = NOA_HIP_IO_I16, U16 = NOA_HIP_IO_U16,
+    I32 = NOA_HIP_IO_I32, U32 = NOA_HIP_IO_U32, I64 = NOA_HIP_IO_I64, U64 = NOA_HIP_IO_U64, F16 = NOA_HIP_IO_F16,
+    F32 = NOA_HIP_IO_F32, F64 = NOA_HIP_IO_F64, C16 = NOA_HIP_IO_C16, C32 = NOA_HIP_IO_C32, C64 = NOA_HIP_IO_C64,
+    U4 = NOA_HIP_IO_U4, CI16 = NOA_HIP_IO_CI16
+};
+
+/* The MRC header as noa's ImageFile reports it. */
+inline noa_hip_mrc_header read_header(const char* path) {
+    noa_hip_mrc_header h{};
+    xform::details::check(noa_hip_mrc_read_header(path, &h));
+    return h;
+}
+
+/* noa::io::read_image into an existing device View of the file's shape (IO.hpp:34-57); synchronous. */
+template<class Output>
+void read_image(const char* path, const Output& output, bool clamp = true, void* stream = nullptr) {
+    const noa_hip_view out = xform::details::view(output);
+    xform::details::check(noa_hip_mrc_read(path, &out, clamp ? 1 : 0, stream));
+}
+
+/* noa::io::write_image of a device View into a new MRC file (IO.hpp:119-163); synchronous. spacing is DHW. */
+struct WriteOption {
+    std::array<f64, 3> spacing{};
+    DataType dtype{DataType::UNKNOWN};
+    bool has_stats{false};
+    std::array<f64, 4> stats{};   // min, max, mean, stddev
+    bool clamp{true};
+};
+template<class Input>
+void write_image(const Input& input, const char* path, const WriteOption& option = {}, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input);
+    xform::details::check(noa_hip_mrc_write(path, &in, option.spacing.data(), static_cast<int32_t>(option.dtype),
+                                            option.has_stats ? option.stats.data() : nullptr, option.clamp ? 1 : 0,
+                                            stream));
+}
+} // namespace io
 } // namespace noa_hip
 
 #endif // NOA_HIP_HPP

@@ -162,6 +162,16 @@ NORM_FORWARD, NORM_ORTHO, NORM_BACKWARD, NORM_NONE = 0, 1, 2, 3
 SIGN_FORWARD, SIGN_BACKWARD = -1, 1
 
 
+class MrcHeader(ctypes.Structure):
+    _fields_ = [("shape", c_int64 * 4), ("spacing", c_double * 3), ("dtype", c_int32), ("has_stats", c_int32),
+                ("stats", c_double * 4), ("data_offset", c_int64), ("endian_swapped", c_int32),
+                ("reserved", c_int32)]
+
+
+IO_DTYPES = dict(UNKNOWN=0, I8=1, U8=2, I16=3, U16=4, I32=5, U32=6, I64=7, U64=8, F16=9, F32=10, F64=11, C16=12,
+                 C32=13, C64=14, U4=15, CI16=16)
+
+
 class Shift2(ctypes.Structure):
     _fields_ = [
         ("data", c_void_p),
@@ -210,6 +220,12 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_fourier_interpolation_correction.restype = ctypes.c_int
     lib.noa_hip_cubic_bspline_prefilter.argtypes = [V, V, c_void_p]
     lib.noa_hip_cubic_bspline_prefilter.restype = ctypes.c_int
+    lib.noa_hip_mrc_read_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(MrcHeader)]
+    lib.noa_hip_mrc_read_header.restype = ctypes.c_int
+    lib.noa_hip_mrc_read.argtypes = [ctypes.c_char_p, V, c_int32, c_void_p]
+    lib.noa_hip_mrc_read.restype = ctypes.c_int
+    lib.noa_hip_mrc_write.argtypes = [ctypes.c_char_p, V, c_void_p, c_int32, c_void_p, c_int32, c_void_p]
+    lib.noa_hip_mrc_write.restype = ctypes.c_int
     FO = ctypes.POINTER(FftOptions)
     lib.noa_hip_fft_options_init.argtypes = [FO]
     lib.noa_hip_fft_options_init.restype = None

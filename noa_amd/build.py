@@ -35,7 +35,8 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["api.cpp", "extract.hip", "rasterize.hip", "insert.hip", "insert_extract.hip", "phase_shift.hip",
-           "spectrum.hip", "ctf.hip", "bspline.hip", "fft.cpp", "fft_stage.hip"]
+           "spectrum.hip", "ctf.hip", "bspline.hip", "fft.cpp", "fft_stage.hip", "mrc.cpp",
+           "mrc_codec.hip"]
 
 
 def hipcc() -> str:

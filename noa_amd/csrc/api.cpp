@@ -1089,6 +1089,39 @@ int noa_hip_ctf_anisotropic_2d_weighted(const noa_hip_view* input, const noa_hip
     }
     return guarded(device, [&](std::string& err) { return launch_ctf(d, static_cast<hipStream_t>(stream), err); });
 }
+int noa_hip_mrc_read_header(const char* path, noa_hip_mrc_header* header) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!header) { g_last_error = "null arguments"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    std::string err;
+    const int st = mrc_read_header(path, *header, err);
+    if (st != NOA_HIP_OK) g_last_error = err;
+    return st;
+}
+
+int noa_hip_mrc_read(const char* path, const noa_hip_view* output, int32_t clamp, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!output || !output->data) { g_last_error = "Empty array detected"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    const int device = device_of(output->data, nullptr);
+    if (device < 0) { g_last_error = "output: the pointer is not device memory"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    return guarded(device, [&](std::string& err) {
+        return mrc_read(path, *output, clamp != 0, static_cast<hipStream_t>(stream), err);
+    });
+}
+
+int noa_hip_mrc_write(const char* path, const noa_hip_view* input, const double* spacing, int32_t dtype,
+                      const double* stats, int32_t clamp, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!input || !input->data) { g_last_error = "Empty array detected"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    const int device = device_of(input->data, nullptr);
+    if (device < 0) { g_last_error = "input: the pointer is not device memory"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    return guarded(device, [&](std::string& err) {
+        return mrc_write(path, *input, spacing, dtype, stats, clamp != 0, static_cast<hipStream_t>(stream), err);
+    });
+}
+
 void noa_hip_fft_options_init(noa_hip_fft_options* options) {
     if (!options) return;
     options->norm = NOA_HIP_NORM_FORWARD;

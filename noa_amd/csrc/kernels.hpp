@@ -123,6 +123,18 @@ struct FftDesc {
     int device{};
 };
 
+// One chunk of MRC data: `count` elements from element `first` (BDHW order of the array's shape), between raw file
+// bytes in device memory and a strided f32/f64/c32/c64 device array.
+struct MrcChunk {
+    void* raw{};
+    int file_dtype{};        // NOA_HIP_IO_*
+    bool swap{}, clamp{};
+    i64 first{}, count{};
+    void* array{};
+    int array_dtype{};
+    i64 s[4]{}, n[4]{};
+};
+
 // Cubic B-spline prefilter: dimensions already reordered to the output's rightmost order, input strides broadcast
 // (0 along broadcast dimensions); ndim of the reordered shape picks the passes (x; x, y; x, y, z).
 struct BSplineDesc {
@@ -157,6 +169,12 @@ int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::s
 int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err);
 int launch_bspline_prefilter(const BSplineDesc& d, hipStream_t stream, std::string& err);
 int launch_fft(const FftDesc& d, hipStream_t stream, std::string& err);
+int launch_mrc_decode(const MrcChunk& c, hipStream_t stream, std::string& err);
+int launch_mrc_encode(const MrcChunk& c, hipStream_t stream, std::string& err);
+int mrc_read_header(const char* path, noa_hip_mrc_header& h, std::string& err);
+int mrc_read(const char* path, const noa_hip_view& out, bool clamp, hipStream_t stream, std::string& err);
+int mrc_write(const char* path, const noa_hip_view& in, const double* spacing, int dtype, const double* stats,
+              bool clamp, hipStream_t stream, std::string& err);
 int launch_strided_copy(const void* src, const i64 ss[4], void* dst, const i64 ds[4], const i64 shape[4],
                         bool f64, hipStream_t stream, std::string& err);
 int fft_clear_cache(int device);

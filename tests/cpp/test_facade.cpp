@@ -198,6 +198,30 @@ static void check_gpu() {
         HIP_OK(hipFree(dr));
         HIP_OK(hipFree(dc));
     }
+    // An MRC round trip through the device: f32 written as I16, read back into f32.
+    {
+        const int n = 8 * 6 * 10;
+        float* dm = nullptr;
+        HIP_OK(hipMalloc(&dm, n * sizeof(float)));
+        std::vector<float> hm(n);
+        for (int i = 0; i < n; ++i) hm[i] = static_cast<float>(i % 200) - 100.f;
+        HIP_OK(hipMemcpy(dm, hm.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        const View<float> mv(dm, Shape4(1, 8, 6, 10));
+        noa_hip::io::WriteOption wo;
+        wo.spacing = {1.5, 1.5, 1.5};
+        wo.dtype = noa_hip::io::DataType::I16;
+        noa_hip::io::write_image(mv, "test_facade_io.mrc", wo);
+        HIP_OK(hipMemset(dm, 0, n * sizeof(float)));
+        const noa_hip_mrc_header h = noa_hip::io::read_header("test_facade_io.mrc");
+        EXPECT(h.shape[1] == 8 && h.shape[2] == 6 && h.shape[3] == 10 && h.dtype == NOA_HIP_IO_I16);
+        noa_hip::io::read_image("test_facade_io.mrc", mv);
+        std::vector<float> back(n);
+        HIP_OK(hipMemcpy(back.data(), dm, n * sizeof(float), hipMemcpyDeviceToHost));
+        EXPECT(back == hm);
+        std::remove("test_facade_io.mrc");
+        std::printf("ok gpu mrc round trip\n");
+        HIP_OK(hipFree(dm));
+    }
     // Prefilter of a constant (1, 8, 16, 32) volume in place: constant coefficients, up to the truncated causal
     // start (|z|^9 of the value along the 8-sample axis).
     {
