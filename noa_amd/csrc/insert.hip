@@ -642,7 +642,11 @@ struct SliceBuf {
     }
 };
 
-template<int INTERP, class In, class W, class C>
+// Element index past the end of any slice or weight buffer (byte offsets 0xFFFFFFF8 for 8-byte values,
+// 0x7FFFFFFC for 4-byte weights): a buffer load there returns 0.
+constexpr u32 TAP_OUTSIDE = 0x1FFFFFFFu;
+
+template<int INTERP, bool ROWS, class In, class W, class C>
 __device__ __forceinline__ Taps<In, W, C> fetch_taps(const SliceBuf& vb, i32 vsy, i32 vsx, const SliceBuf& wb,
                                                      i32 wsy, i32 wsx, bool has_w, i32 ny, i32 nx,
                                                      bool centered, C fy, C fx, bool valid) {
@@ -650,7 +654,29 @@ __device__ __forceinline__ Taps<In, W, C> fetch_taps(const SliceBuf& vb, i32 vsy
     t.conj = 1;
     if (fx < 0) { fy *= -1; fx *= -1; t.conj = -1; }
     const i32 lo_y = -ny / 2, hi_y = (ny - 1) / 2, hi_x = nx / 2;
-    if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
+    if constexpr (ROWS && INTERP == NOA_HIP_INTERP_LINEAR) {
+        // Contiguous rows shared by the values and the weights (unit x strides, equal row pitches): one row offset
+        // per tap row, and a tap outside the bounds reads TAP_OUTSIDE, which the buffer load returns as 0 -- the
+        // zero noa's interpolation adds for it.
+        const C fly = floor(fy), flx = floor(fx);
+        const i32 y0 = static_cast<i32>(fly), x0 = static_cast<i32>(flx);
+        t.ty = static_cast<typename vt<In>::real>(fy - fly);
+        t.tx = static_cast<typename vt<In>::real>(fx - flx);
+        const bool okx0 = x0 >= 0 && x0 <= hi_x, okx1 = x0 + 1 >= 0 && x0 + 1 <= hi_x;
+        t.ok = 0xfu;
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const bool oky = valid && y0 + y >= lo_y && y0 + y <= hi_y;
+            const i32 row = frequency2index(centered, y0 + y, ny) * vsy;
+            const u32 e0 = oky && okx0 ? static_cast<u32>(row + x0) : TAP_OUTSIDE;
+            const u32 e1 = oky && okx1 ? static_cast<u32>(row + x0 + 1) : TAP_OUTSIDE;
+            t.v[y][0] = vb.template load<In>(e0);
+            t.v[y][1] = vb.template load<In>(e1);
+            t.w[y][0] = wb.template load<W>(e0);
+            t.w[y][1] = wb.template load<W>(e1);
+        }
+        (void)vsx; (void)wsy; (void)wsx; (void)has_w;
+    } else if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
         const i32 y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
         const bool ok = valid && y >= lo_y && y <= hi_y && x >= 0 && x <= hi_x;
         const i32 iy = ok ? frequency2index(centered, y, ny) : 0, ix = ok ? x : 0;
@@ -715,8 +741,9 @@ __device__ __forceinline__ void combine_taps(const Taps<In, W, C>& t, bool has_w
 }
 
 // HOT: 0 = every option read at run time; 1 and 2 = the common configuration with its options fixed at compile
-// time (power-of-two target, Mat33 rotations, input and output weights, polynomial window; 1: h2h, 2: hc2hc),
-// which frees scalar registers and removes the uniform branches of the hot loop (12 % faster, same results).
+// time (power-of-two target, Mat33 rotations, input and output weights with unit x strides and one row pitch,
+// polynomial window; 1: h2h, 2: hc2hc), which frees scalar registers, removes the uniform branches of the hot loop
+// and shares one row offset between the value and weight taps (same results).
 template<class C, class In, class Out, class W, int INTERP, int HOT = 0>
 __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, In, W> p) {
     using RIn = typename vt<In>::real;
@@ -964,8 +991,9 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
                     window[g] = windowed_sinc<C>(fz, fs, fb);
                 }
                 lv[g] = (lz * WY + ly) * WX + lx;
-                tp[g] = fetch_taps<INTERP, In, W>(sl, p.is_y, p.is_x, wsl, p.wis_y, p.wis_x, win_present, Hs, Ws,
-                                           HOT == 0 ? p.slices_centered : HOT == 2, f2.y * hs_f, f2.x * ws_f, valid[g]);
+                tp[g] = fetch_taps<INTERP, HOT != 0, In, W>(sl, p.is_y, p.is_x, wsl, p.wis_y, p.wis_x, win_present,
+                                                     Hs, Ws, HOT == 0 ? p.slices_centered : HOT == 2, f2.y * hs_f,
+                                                     f2.x * ws_f, valid[g]);
             }
             // Phase 2: interpolate (Interpolation.hpp:435-526) and accumulate.
 #pragma unroll
@@ -1412,7 +1440,8 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             // The compile-time configurations: both layouts centred or both not, and the options HOT fixes.
             const bool quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
             const bool hot_opts = p.pow2 && !quat && p.win_present && p.has_wvol && p.wpoly &&
-                                  p.vol_centered == p.slices_centered;
+                                  p.vol_centered == p.slices_centered && p.is_x == 1 && p.wis_x == 1 &&
+                                  p.is_y == p.wis_y && p.slice_bytes < 0x7ffffff0u && p.wslice_bytes < 0x7ffffff0u;
             const int hot = hot_opts ? (p.vol_centered ? 2 : 1) : 0;
             if (!kc.staged) {
                 if (nearest) {
