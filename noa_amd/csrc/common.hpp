@@ -477,9 +477,51 @@ struct Taps3 {
     typename vt<Math>::real tz, ty, tx, conj;
 };
 
+// Element offsets of a volume stored row-major with strides (sz, sy, sx).
+struct LinearLayout {
+    i32 sz, sy, sx;
+    __device__ __forceinline__ i32 z(i32 i) const { return i * sz; }
+    __device__ __forceinline__ i32 y(i32 i) const { return i * sy; }
+    __device__ __forceinline__ i32 x(i32 i) const { return i * sx; }
+};
+
+// Element offsets of a volume stored in blocks of 16 elements (4 x 2 x 2 along x, y, z), blocks row-major: one
+// 128-byte line of c32 values holds a 4x2x2 neighbourhood, so the eight taps of a trilinear gather fall in one to
+// four lines whatever the slice's orientation (row-major storage spends a whole line on the two x taps of a plane
+// seen edge-on along x).
+#ifndef NOA_HIP_BRICK_LOG2
+#define NOA_HIP_BRICK_LOG2 2, 1, 1
+#endif
+struct BrickLayout {
+    // log2 of the block extent along x, y, z (4 x 2 x 2: 16 c32 values, one 128-byte line).
+    static constexpr i32 LOG2[3] = {NOA_HIP_BRICK_LOG2};
+    static constexpr i32 LX = LOG2[0], LY = LOG2[1], LZ = LOG2[2];
+    i32 bz, by;   // element strides of one block step along z and along y
+    __device__ __forceinline__ i32 z(i32 i) const { return (i >> LZ) * bz + ((i & ((1 << LZ) - 1)) << (LX + LY)); }
+    __device__ __forceinline__ i32 y(i32 i) const { return (i >> LY) * by + ((i & ((1 << LY) - 1)) << LX); }
+    __device__ __forceinline__ i32 x(i32 i) const { return ((i >> LX) << (LX + LY + LZ)) + (i & ((1 << LX) - 1)); }
+    // Strides and element count of the block copy of a stored (nz, ny, wr) volume.
+    static __host__ __device__ void strides(i64 nz, i64 ny, i64 wr, i64& bz_, i64& by_, i64& count) {
+        const i64 bx = (wr + (1 << LX) - 1) >> LX;
+        by_ = bx << (LX + LY + LZ);
+        bz_ = ((ny + (1 << LY) - 1) >> LY) * by_;
+        count = ((nz + (1 << LZ) - 1) >> LZ) * bz_;
+    }
+};
+
+template<int INTERP, class Math, class Loader, class Layout, class C>
+__device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, const Layout& L, const Bounds3& bd, bool centered,
+                                                 C fz, C fy, C fx, bool valid);
+
 template<int INTERP, class Math, class Loader, class C>
 __device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, i32 sz, i32 sy, i32 sx,
                                                  const Bounds3& bd, bool centered, C fz, C fy, C fx, bool valid) {
+    return fetch3<INTERP, Math>(vol, LinearLayout{sz, sy, sx}, bd, centered, fz, fy, fx, valid);
+}
+
+template<int INTERP, class Math, class Loader, class Layout, class C>
+__device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, const Layout& L, const Bounds3& bd, bool centered,
+                                                 C fz, C fy, C fx, bool valid) {
     using R = typename vt<Math>::real;
     Taps3<Math, C> t;
     t.conj = 1;
@@ -487,8 +529,8 @@ __device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, i32 sz, i32 
     if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
         const i32 z = static_cast<i32>(round(fz)), y = static_cast<i32>(round(fy)), x = static_cast<i32>(round(fx));
         const bool ok = valid && z >= bd.lo_z && z <= bd.hi_z && y >= bd.lo_y && y <= bd.hi_y && x >= 0 && x <= bd.hi_x;
-        const u32 off = ok ? static_cast<u32>(frequency2index(centered, z, bd.nz) * sz +
-                                              frequency2index(centered, y, bd.ny) * sy + x * sx) : 0u;
+        const u32 off = ok ? static_cast<u32>(L.z(frequency2index(centered, z, bd.nz)) +
+                                              L.y(frequency2index(centered, y, bd.ny)) + L.x(x)) : 0u;
         t.v[0][0][0] = vol.template load<Math>(off);
         t.ok = ok ? 1u : 0u;
         t.tz = t.ty = t.tx = 0;
@@ -506,8 +548,8 @@ __device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, i32 sz, i32 
             okz[i] = valid && z0 + i >= bd.lo_z && z0 + i <= bd.hi_z;
             oky[i] = y0 + i >= bd.lo_y && y0 + i <= bd.hi_y;
             okx[i] = x0 + i >= 0 && x0 + i <= bd.hi_x;
-            oz[i] = frequency2index(centered, z0 + i, bd.nz) * sz;
-            oy[i] = frequency2index(centered, y0 + i, bd.ny) * sy;
+            oz[i] = L.z(frequency2index(centered, z0 + i, bd.nz));
+            oy[i] = L.y(frequency2index(centered, y0 + i, bd.ny));
         }
         t.ok = 0;
 #pragma unroll
@@ -517,7 +559,7 @@ __device__ __forceinline__ Taps3<Math, C> fetch3(const Loader& vol, i32 sz, i32 
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     const bool ok = okz[z] && oky[y] && okx[x];
-                    const u32 off = ok ? static_cast<u32>(oz[z] + oy[y] + (x0 + x) * sx) : 0u;
+                    const u32 off = ok ? static_cast<u32>(oz[z] + oy[y] + L.x(x0 + x)) : 0u;
                     t.v[z][y][x] = vol.template load<Math>(off);
                     t.ok |= ok ? (1u << (z * 4 + y * 2 + x)) : 0u;
                 }

@@ -13,6 +13,9 @@
 //   - The w-windowed-sinc variant (window size > 1, :661-694) loops over the window inside the lane that owns
 //     the pixel, in increasing w, so no zero fill and no atomics are needed; the sum order is that of noa's
 //     serial CPU loop.
+#include <cstdlib>
+#include <cstring>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -208,7 +211,20 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
 // work that the general kernel does for every pixel: the integer division of the pixel index (here a float
 // reciprocal, exact for slices below 2^21 pixels, checked on the host), the runtime selection between the plain and
 // the scaled/EWS transforms, the weight paths, and the runtime choice of the shift precision.
-template<int SHIFT>
+// Copy of a c32 volume (unit x stride) into the block layout of BrickLayout: one lane per element of the stored
+// (nz, ny, wr) volume; the block padding (sizes not multiples of the block) is never read.
+__global__ __launch_bounds__(256) void brick_volume_kernel(const float2* __restrict__ in, i32 vs0, i32 vs1,
+                                                           float2* __restrict__ out, i32 ny, i32 wr, i64 total,
+                                                           BrickLayout L) {
+    const i64 i = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= total) return;
+    const i32 x = static_cast<i32>(i % wr);
+    const i64 r = i / wr;
+    const i32 y = static_cast<i32>(r % ny), z = static_cast<i32>(r / ny);
+    out[L.z(z) + L.y(y) + L.x(x)] = in[static_cast<i64>(z) * vs0 + static_cast<i64>(y) * vs1 + x];
+}
+
+template<int SHIFT, bool BRICK>
 __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<float> p) {
     const u32 lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const i32 b = static_cast<i32>(lb / static_cast<u32>(p.tiles));
@@ -250,8 +266,12 @@ __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<fl
         const vec3<float> f = slice2grid<false>(f2, sc, rot, 0.f, 0.f);
         inside[k] = live && dot3(f) <= p.cutoff_sqd;
         const float fz = f.z * p.target[0], fyv = f.y * p.target[1], fx = f.x * p.target[2];
-        tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{vol}, p.vs0, p.vs1, 1, bd, p.vol_centered,
-                                                      fz, fyv, fx, inside[k]);
+        if constexpr (BRICK)
+            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{vol}, BrickLayout{p.vs0, p.vs1}, bd,
+                                                          p.vol_centered, fz, fyv, fx, inside[k]);
+        else
+            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{vol}, p.vs0, p.vs1, 1, bd, p.vol_centered,
+                                                          fz, fyv, fx, inside[k]);
     }
 #pragma unroll
     for (i32 k = 0; k < MAXP; ++k) {
@@ -346,9 +366,43 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                           (p.shift_mode < 0 || p.shift_mode == 1 || p.shift_mode == 2 || !p.shift_f64);
         if (lean) {
             p.inv_wr = 1.f / static_cast<float>(p.slice_wr);
-            if (p.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1>), grid, block, 0, stream, p);
-            else if (p.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2>), grid, block, 0, stream, p);
-            else hipLaunchKernelGGL((extract_c32_kernel<0>), grid, block, 0, stream, p);
+            // Many slices from one volume: gather from a 4x2x2-block copy of it (one read and one write of the
+            // volume, then every trilinear gather touches fewer cache lines). Worth it once the slices hold about
+            // twice the volume's elements; NOA_HIP_EXTRACT_LAYOUT=linear|bricks forces either.
+            const i32 nz = p.grid[0], ny = p.grid[1], wr = p.grid[2] / 2 + 1;
+            const i64 stored = static_cast<i64>(nz) * ny * wr;
+            i64 bz, by, brick_elems;
+            BrickLayout::strides(nz, ny, wr, bz, by, brick_elems);
+            const char* env = std::getenv("NOA_HIP_EXTRACT_LAYOUT");
+            bool bricks = n_slices * static_cast<i64>(p.pixels) >= 2 * stored;
+            if (env && std::strcmp(env, "linear") == 0) bricks = false;
+            if (env && std::strcmp(env, "bricks") == 0) bricks = true;
+            bricks = bricks && brick_elems < 0x7fffffffLL;
+            void* brick = nullptr;
+            if (bricks) {
+                if (hipMallocAsync(&brick, static_cast<size_t>(brick_elems) * sizeof(float2), stream) != hipSuccess) {
+                    (void)hipGetLastError();
+                    brick = nullptr;   // not enough memory for the copy: gather from the row-major volume
+                }
+            }
+            if (brick) {
+                const BrickLayout L{static_cast<i32>(bz), static_cast<i32>(by)};
+                hipLaunchKernelGGL(brick_volume_kernel, dim3(static_cast<u32>((stored + 255) / 256)), dim3(256), 0,
+                                   stream, static_cast<const float2*>(p.vol), p.vs0, p.vs1, static_cast<float2*>(brick),
+                                   ny, wr, stored, L);
+                ExtractParams<C> q = p;
+                q.vol = brick;
+                q.vs0 = L.bz;
+                q.vs1 = L.by;
+                if (q.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, true>), grid, block, 0, stream, q);
+                else if (q.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, true>), grid, block, 0, stream, q);
+                else hipLaunchKernelGGL((extract_c32_kernel<0, true>), grid, block, 0, stream, q);
+                (void)hipFreeAsync(brick, stream);
+            } else {
+                if (p.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, false>), grid, block, 0, stream, p);
+                else if (p.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, false>), grid, block, 0, stream, p);
+                else hipLaunchKernelGGL((extract_c32_kernel<0, false>), grid, block, 0, stream, p);
+            }
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
             return NOA_HIP_OK;

@@ -121,6 +121,43 @@ def test_extract_odd_sizes_strided_output_single_rotation():
     np.testing.assert_array_equal(cpu(out), want)
 
 
+@pytest.mark.parametrize("remap,n,h,shifts", [("hc2hc", 48, 48, False), ("h2h", 33, 35, True), ("hc2h", 31, 27, False),
+                                              ("h2hc", 24, 50, True)])
+def test_extract_volume_layouts(monkeypatch, remap, n, h, shifts):
+    """The c32 extraction gathers from a 4x2x2-block copy of the volume when many slices are extracted; forced both
+    ways here: the same bits in either layout, equal to the oracle (bit-exact without shifts, within the sin/cos
+    rounding of the fused phase shift with them), odd and even sizes, slices larger or smaller than the volume."""
+    print(f"case extract layouts {remap} n={n} h={h} shifts={shifts}", flush=True)
+    rng = np.random.default_rng(104)
+    b = 20
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng)
+    rot = random_rotations(b, rng)
+    sh = rng.uniform(-4, 4, (b, 2)).astype(np.float32)
+    want = np.zeros((b, 1, h, h // 2 + 1), np.complex64)
+    O.extract_central_slices_3d(remap, vol, None, (1, n, n, n), want, None, (b, 1, h, h), None, rot,
+                                fftfreq_cutoff=0.6)
+    if shifts:
+        out_layout = "hc2hc" if remap.endswith("hc") else "h2h"
+        O.phase_shift_2d(out_layout, want.copy(), want, (b, 1, h, h), sh)
+    outs = {}
+    for layout in ("bricks", "linear"):
+        monkeypatch.setenv("NOA_HIP_EXTRACT_LAYOUT", layout)
+        out = torch.zeros((b, 1, h, h // 2 + 1), dtype=torch.complex64, device=DEV)
+        opts = nx().ExtractCentralSlicesOptions(fftfreq_cutoff=0.6)
+        if shifts:
+            nx().extract_central_slices_3d_shifted(remap, to_dev(vol), None, (1, n, n, n), out, None, (b, 1, h, h),
+                                                   None, to_dev(rot), shifts=to_dev(sh), options=opts)
+        else:
+            nx().extract_central_slices_3d(remap, to_dev(vol), None, (1, n, n, n), out, None, (b, 1, h, h), None,
+                                           to_dev(rot), opts)
+        outs[layout] = cpu(out)
+    np.testing.assert_array_equal(outs["bricks"], outs["linear"])
+    if shifts:
+        assert rel_l2(outs["bricks"], want) < 3e-7
+    else:
+        np.testing.assert_array_equal(outs["bricks"], want)
+
+
 # ------------------------------------------------------------------------------------------------ rasterisation
 @pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h", "h2hc"])
 def test_rasterize_parity(remap):
