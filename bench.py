@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # default workload sizes.
 MEASURED_TRAFFIC = {
     ("insert", 10_000): (8_572_515 * 2.0 + 387_096) * 1024,
-    # the gather kernel plus the 4x2x2-block copy of the volume it reads
+    # the gather kernel plus the block copy of the volume it reads (counters measured with 4x2x2 blocks)
     ("extract", 4096): (1_868_633 * 2.0 + 1_056_768 + 33_053 * 2.0 + 68_354) * 1024,
 }
 METRIC = "slices/s and % HBM peak, 256\u00b3 insert+extract; p50/p99 request latency"

@@ -366,7 +366,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                           (p.shift_mode < 0 || p.shift_mode == 1 || p.shift_mode == 2 || !p.shift_f64);
         if (lean) {
             p.inv_wr = 1.f / static_cast<float>(p.slice_wr);
-            // Many slices from one volume: gather from a 4x2x2-block copy of it (one read and one write of the
+            // Many slices from one volume: gather from a block copy of it (BrickLayout) (one read and one write of the
             // volume, then every trilinear gather touches fewer cache lines). Worth it once the slices hold about
             // twice the volume's elements; NOA_HIP_EXTRACT_LAYOUT=linear|bricks forces either.
             const i32 nz = p.grid[0], ny = p.grid[1], wr = p.grid[2] / 2 + 1;
