@@ -384,6 +384,36 @@ def test_extract_ews_follows_the_ewald_sphere(radius):
     assert np.abs(a[0] * n * (f[..., 0] ** 2 / (2 * ry))[ok]).max() > 20 * bound[ok].max()
 
 
+@pytest.mark.parametrize("radius", [(2.0, -3.0), (-2.5, 1.5)])
+def test_insert_ews_weights_by_distance_to_the_paraboloid(radius):
+    """Insertion with an Ewald radius at identity pose: a voxel f = (fz, fy, fx) lies at signed distance
+    d = fz - (fy^2/(2 r_y) + fx^2/(2 r_x)) from the curved slice, and receives windowed_sinc(d) times the slice pixel
+    at (fy, fx) (an integer sample, so interpolation returns the pixel itself). The paraboloid moves the deposit
+    by up to two voxels here, so a wrong sign, a missing factor 2, swapped radii or the term dropped all fail."""
+    n = 32
+    rng = np.random.default_rng(23)
+    s = random_spectrum((1, 1, n, n // 2 + 1), rng)
+    fs = fb = 3.0 / n
+    vol, wvol = _insert("h2h", s, n, np.eye(3, dtype=np.float32)[None], wout=True,
+                        fftfreq_sinc=fs, fftfreq_blackman=fb, ews_radius=radius)
+    fz = freq_grid(n, False)[:, None, None] / n
+    fy = freq_grid(n, False)[None, :, None] / n
+    fx = np.arange(n // 2 + 1)[None, None, :] / n
+    fz, fy, fx = np.broadcast_arrays(fz, fy, fx)
+    d = fz - (fy ** 2 / (2 * radius[0]) + fx ** 2 / (2 * radius[1]))
+    win = np.sinc(d / fs) * (0.42 + 0.5 * np.cos(np.pi * d / fb) + 0.08 * np.cos(2 * np.pi * d / fb))
+    inside = (fz ** 2 + fy ** 2 + fx ** 2 <= 0.25) & (np.abs(d) <= fb)
+    want = np.where(inside, win * s[0, 0][None], 0)
+    wwant = np.where(inside, win, 0)
+    edge = np.abs(np.abs(d) - fb) < 1e-5           # f32 rounding may put these on either side of the window edge
+    assert inside.sum() > 2000
+    assert np.abs(vol[0] - want)[~edge].max() <= 2e-6 * np.abs(s).max()
+    assert np.abs(wvol[0] - wwant)[~edge].max() <= 2e-6
+    # The curvature matters at this size: the deposit peaks off the central plane for the outer pixels.
+    flat = np.where(inside, np.sinc(fz / fs) * s[0, 0][None], 0)
+    assert rel_l2(vol[0], flat) > 0.3
+
+
 def test_extract_scaling_is_an_in_plane_transform_before_the_rotation():
     """Inverse scaling S acts on the slice's (fy, fx) before the rotation: with S an in-plane rotation Q, extracting
     with (S=Q, R) equals extracting with (no S, R . diag(1, Q)). A transposed S or S applied after R fails."""
