@@ -63,12 +63,17 @@ enum {
     NOA_HIP_CBF16 = 6     /* {bf16 real, bf16 imag}: storage-only volume format for extraction (FP32 math) */
 };
 
-/* noa::fft::Layout values restricted to half-to-half layouts (src/noa/fft/core/Layout.hpp:39-68). */
+/* noa::fft::Layout values (src/noa/fft/core/Layout.hpp:39-68): the half-to-half layouts, and the full-to-full ones
+ * that the CTF functions also accept. */
 enum {
     NOA_HIP_H2H = 0,
     NOA_HIP_HC2H = 4,
     NOA_HIP_H2HC = 8,
-    NOA_HIP_HC2HC = 12
+    NOA_HIP_HC2HC = 12,
+    NOA_HIP_F2F = 3,
+    NOA_HIP_FC2F = 7,
+    NOA_HIP_F2FC = 11,
+    NOA_HIP_FC2FC = 15
 };
 
 /* noa::xform::Interp::Method values (src/noa/xform/core/Interp.hpp:17-41). */
@@ -205,11 +210,21 @@ typedef struct noa_hip_ctf_anisotropic_f64 {
     double phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
 } noa_hip_ctf_anisotropic_f64;
 
-/* CTFs of a ctf_anisotropic call: one CTF broadcast over the batch (batched == 0; `data` is a HOST pointer to one
- * object, copied at the call) or one per batch (batched == 1; `data` is a DEVICE pointer to `count` objects). */
+/* noa::signal::CTFIsotropic<f32> and <f64> as stored by noa (src/noa/signal/core/CTF.hpp:259-271, member order):
+ * 44 and 88 bytes, the same internal values as the anisotropic objects with one pixel size and one defocus. */
+typedef struct noa_hip_ctf_isotropic_f32 {
+    float pixel_size, defocus, phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+} noa_hip_ctf_isotropic_f32;
+typedef struct noa_hip_ctf_isotropic_f64 {
+    double pixel_size, defocus, phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+} noa_hip_ctf_isotropic_f64;
+
+/* CTFs of a ctf call: one CTF broadcast over the batch (batched == 0; `data` is a HOST pointer to one object, copied
+ * at the call) or one per batch (batched == 1; `data` is a DEVICE pointer to `count` objects). The objects are
+ * noa_hip_ctf_anisotropic_* for the ctf_anisotropic functions and noa_hip_ctf_isotropic_* for ctf_isotropic. */
 typedef struct noa_hip_ctf {
     const void* data;
-    int32_t dtype;          /* NOA_HIP_F32 (noa_hip_ctf_anisotropic_f32) or NOA_HIP_F64 */
+    int32_t dtype;          /* NOA_HIP_F32 (the f32 objects) or NOA_HIP_F64 */
     int32_t batched;
     int64_t count;
 } noa_hip_ctf;
@@ -251,13 +266,24 @@ int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* st
 int noa_hip_insert_and_extract_central_slices_3d(const noa_hip_insert_extract_args* args, void* stream);
 
 /* Replaces the GPU branch of noa::signal::ctf_anisotropic<REMAP> (src/noa/signal/CTF.hpp:430-500, operator :14-125)
- * for (B, 1, H, W/2+1) rfft arrays: output[b, y, x] = input[b, remap(y), x] * ctf_b(f) (|.|^2 for a complex input
- * and a real output), or the CTF itself when `input` is Empty (data == NULL). f is the frequency times the step of
- * the fftfreq range; the CTF is squared (ctf_squared) or its absolute value taken (ctf_abs). Strides must be
- * row-major along H, W. The output may be the input when the layout does not change. */
+ * for (B, 1, H, W) spectra, half (output (B, 1, H, W/2+1)) or full layouts: output[b, y, x] = input[b, remap(y, x)] *
+ * ctf_b(f) (|.|^2 for a complex input and a real output), or the CTF itself when `input` is Empty (data == NULL).
+ * f is the frequency times the step of the fftfreq range; the CTF is squared (ctf_squared) or its absolute value
+ * taken (ctf_abs). A half-layout output must be row-major along H, W; a full-layout output may have any strides (H
+ * and W are swapped first when W is not the innermost, as noa does). The input is broadcast to the output's shape.
+ * The output may be the input when the layout does not change. */
 int noa_hip_ctf_anisotropic_2d(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
                                int32_t remap, const noa_hip_ctf* ctf, const noa_hip_ctf_options* options,
                                void* stream);
+
+/* Replaces the GPU branch of noa::signal::ctf_isotropic<REMAP> (src/noa/signal/CTF.hpp:216-421, operator :14-125):
+ * 1D, 2D or 3D spectra (by the dimensionality of the logical BDHW `shape`), half (output shape with W/2+1) or full
+ * layouts, CTFIsotropic objects. output = input[remap] * ctf_b(|f|) (1D: ctf_b(f + start), the only case that
+ * allows a non-zero start), |.|^2 for a complex input and a real output, or the CTF itself when `input` is Empty.
+ * The dimensions are put in the output's rightmost order first, as noa does; a half-layout output whose W is then
+ * not innermost is rejected (noa's loop would leave the array). The input is broadcast to the output's shape. */
+int noa_hip_ctf_isotropic(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                          int32_t remap, const noa_hip_ctf* ctf, const noa_hip_ctf_options* options, void* stream);
 
 /* Both CTF steps before a CTF-weighted insertion in one pass: output = input * ctf and output_weight = ctf^2, the
  * results of noa's ctf_anisotropic<REMAP>(input, output, ...) followed by ctf_anisotropic<REMAP>({}, output_weight,

@@ -108,7 +108,13 @@ struct Layout {
         if (name == "h2hc" || name == "H2HC") return NOA_HIP_H2HC;
         if (name == "hc2h" || name == "HC2H") return NOA_HIP_HC2H;
         if (name == "hc2hc" || name == "HC2HC" || name == "hc" || name == "HC") return NOA_HIP_HC2HC;
-        throw Exception("invalid layout: only the half-to-half remaps h2h, h2hc, hc2h and hc2hc are supported");
+        // Full-to-full layouts: accepted by the CTF functions only (the other operators reject them).
+        if (name == "f2f" || name == "F2F" || name == "f" || name == "F") return NOA_HIP_F2F;
+        if (name == "f2fc" || name == "F2FC") return NOA_HIP_F2FC;
+        if (name == "fc2f" || name == "FC2F") return NOA_HIP_FC2F;
+        if (name == "fc2fc" || name == "FC2FC" || name == "fc" || name == "FC") return NOA_HIP_FC2FC;
+        throw Exception("invalid layout: the half-to-half (h2h, h2hc, hc2h, hc2hc) or full-to-full (f2f, f2fc, "
+                        "fc2f, fc2fc) remaps are supported");
     }
     template<std::size_t N>
     constexpr Layout(const char (&name)[N]) : bits(from_string(std::string_view(name, N - 1))) {}
@@ -493,6 +499,36 @@ struct alignas(2 * sizeof(T)) CTFAnisotropic {
 static_assert(sizeof(CTFAnisotropic<f32>) == sizeof(noa_hip_ctf_anisotropic_f32));
 static_assert(sizeof(CTFAnisotropic<f64>) == sizeof(noa_hip_ctf_anisotropic_f64));
 
+/* noa::signal::CTFIsotropic<T> (src/noa/signal/core/CTF.hpp:30-272): the same members in the same order, built by
+ * noa's constructor from physical parameters (:61-82). */
+template<class T>
+struct CTFIsotropic {
+    static_assert(std::is_same_v<T, f32> || std::is_same_v<T, f64>);
+    T pixel_size{}, defocus{}, phase_shift{}, cs{}, voltage{}, amplitude{}, bfactor_forth{}, scale{}, k1{}, k2{}, k3{};
+
+    CTFIsotropic() = default;
+    CTFIsotropic(T pixel_size_, T defocus_um, T voltage_kv, T amplitude_, T cs_mm, T phase_shift_ = 0, T bfactor = 0,
+                 T scale_ = 1) {
+        pixel_size = pixel_size_;
+        defocus = -defocus_um * static_cast<T>(1e4);
+        phase_shift = phase_shift_;
+        cs = cs_mm * static_cast<T>(1e7);
+        voltage = voltage_kv * static_cast<T>(1e3);
+        amplitude = amplitude_;
+        scale = scale_;
+        bfactor_forth = bfactor / 4;
+        const f64 h = 6.62607015e-34, c = 299792458.0, m0 = 9.1093837015e-31, e = 1.602176634e-19;
+        const f64 ev = e * static_cast<f64>(voltage);
+        const f64 lambda = (h * c) / std::sqrt(ev * (2 * m0 * c * c + ev)) * 1e10;
+        const f64 pi = 3.1415926535897932384626433832795;
+        k1 = static_cast<T>(pi * lambda);
+        k2 = static_cast<T>(pi * 0.5 * static_cast<f64>(cs) * lambda * lambda * lambda);
+        k3 = std::atan(amplitude / std::sqrt(1 - amplitude * amplitude));
+    }
+};
+static_assert(sizeof(CTFIsotropic<f32>) == sizeof(noa_hip_ctf_isotropic_f32));
+static_assert(sizeof(CTFIsotropic<f64>) == sizeof(noa_hip_ctf_isotropic_f64));
+
 /* CTFOptions (src/noa/signal/CTF.hpp:196-209). */
 struct CTFOptions {
     struct Range { f64 start{0}, stop{-1}; bool endpoint{true}; } fftfreq_range{};
@@ -508,12 +544,14 @@ namespace details {
         if constexpr (xform::details::is_view_v<U>) {
             using E = std::remove_cv_t<std::remove_pointer_t<decltype(c.ptr)>>;
             r.data = c.ptr;
-            r.dtype = std::is_same_v<E, CTFAnisotropic<f64>> ? NOA_HIP_F64 : NOA_HIP_F32;
+            r.dtype = std::is_same_v<E, CTFAnisotropic<f64>> || std::is_same_v<E, CTFIsotropic<f64>> ? NOA_HIP_F64
+                                                                                                     : NOA_HIP_F32;
             r.batched = 1;
             r.count = c.ssize();
         } else {
             r.data = &c;
-            r.dtype = std::is_same_v<U, CTFAnisotropic<f64>> ? NOA_HIP_F64 : NOA_HIP_F32;
+            r.dtype = std::is_same_v<U, CTFAnisotropic<f64>> || std::is_same_v<U, CTFIsotropic<f64>> ? NOA_HIP_F64
+                                                                                                     : NOA_HIP_F32;
             r.batched = 0;
             r.count = 1;
         }
@@ -525,8 +563,20 @@ namespace details {
     }
 } // namespace details
 
-/* noa::signal::ctf_anisotropic<REMAP> (src/noa/signal/CTF.hpp:430-500) for 2D rfft arrays. `input` may be Empty
- * (the CTF is written). `ctf`: one CTFAnisotropic (host) or a View of one per batch (device). */
+/* noa::signal::ctf_isotropic<REMAP> (src/noa/signal/CTF.hpp:216-421) for 1D, 2D or 3D spectra, half or full
+ * layouts. `input` may be Empty (the CTF is written). `ctf`: one CTFIsotropic (host) or a View of one per batch
+ * (device). */
+template<xform::Layout REMAP, class Input, class Output, class CTF>
+void ctf_isotropic(const Input& input, const Output& output, const Shape4& shape, const CTF& ctf,
+                   const CTFOptions& options = {}, void* stream = nullptr) {
+    const noa_hip_view in = xform::details::view(input), out = xform::details::view(output);
+    const noa_hip_ctf c = details::ctf(ctf);
+    const noa_hip_ctf_options o = details::ctf_options(options);
+    xform::details::check(noa_hip_ctf_isotropic(&in, &out, shape.v.data(), REMAP.bits, &c, &o, stream));
+}
+
+/* noa::signal::ctf_anisotropic<REMAP> (src/noa/signal/CTF.hpp:430-500) for 2D spectra, half or full layouts.
+ * `input` may be Empty (the CTF is written). `ctf`: one CTFAnisotropic (host) or a View of one per batch (device). */
 template<xform::Layout REMAP, class Input, class Output, class CTF>
 void ctf_anisotropic(const Input& input, const Output& output, const Shape4& shape, const CTF& ctf,
                      const CTFOptions& options = {}, void* stream = nullptr) {

@@ -29,6 +29,10 @@ H2H = 0
 HC2H = 4
 H2HC = 8
 HC2HC = 12
+F2F = 3
+FC2F = 7
+F2FC = 11
+FC2FC = 15
 
 # Transform kinds.
 XFORM_NONE = 0
@@ -144,6 +148,17 @@ class CtfAnisotropicF64(ctypes.Structure):
                 for name, _ in CtfAnisotropicF32._fields_]
 
 
+class CtfIsotropicF32(ctypes.Structure):
+    """noa::signal::CTFIsotropic<f32> members (signal/core/CTF.hpp:259-271): 44 bytes."""
+    _fields_ = [(name, ctypes.c_float) for name in ("pixel_size", "defocus", "phase_shift", "cs", "voltage",
+                                                     "amplitude", "bfactor_forth", "scale", "k1", "k2", "k3")]
+
+
+class CtfIsotropicF64(ctypes.Structure):
+    """noa::signal::CTFIsotropic<f64>: 88 bytes."""
+    _fields_ = [(name, ctypes.c_double) for name, _ in CtfIsotropicF32._fields_]
+
+
 class Ctf(ctypes.Structure):
     _fields_ = [("data", c_void_p), ("dtype", c_int32), ("batched", c_int32), ("count", c_int64)]
 
@@ -214,6 +229,8 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.noa_hip_ctf_anisotropic_2d.restype = ctypes.c_int
     lib.noa_hip_ctf_anisotropic_2d_weighted.argtypes = [V, V, V, ctypes.POINTER(c_int64), c_int32, C, CO, c_void_p]
     lib.noa_hip_ctf_anisotropic_2d_weighted.restype = ctypes.c_int
+    lib.noa_hip_ctf_isotropic.argtypes = [V, V, ctypes.POINTER(c_int64), c_int32, C, CO, c_void_p]
+    lib.noa_hip_ctf_isotropic.restype = ctypes.c_int
     lib.noa_hip_remap.argtypes = [c_int32, V, V, ctypes.POINTER(c_int64), c_void_p]
     lib.noa_hip_remap.restype = ctypes.c_int
     lib.noa_hip_fourier_interpolation_correction.argtypes = [V, V, c_int32, c_int32, c_void_p]

@@ -949,6 +949,142 @@ CtfDesc validate_ctf(const noa_hip_view* in, const noa_hip_view* out, const noa_
     return d;
 }
 
+// Shape::ndim() (runtime/core/Shape.hpp:315-335) of a BDHW shape.
+int ndim4(const int64_t* s) {
+    if (s[1] > 1) return 3;
+    if (s[2] > 1 && s[3] > 1) return 2;
+    return 1;
+}
+
+// ctf_isotropic (signal/CTF.hpp:216-421) and ctf_anisotropic (:430-500) through the general operator: the checks of
+// ctf_check_parameters (:143-189), the input broadcast (runtime/Utils.hpp:119-131), then noa's reordering of the
+// dimensions (1D: the first non-empty one; 2D: H and W swapped unless W is innermost; 3D: the rightmost order of
+// the output's strides, Shape.hpp:871-886).
+CtfNdDesc validate_ctf_nd(const noa_hip_view* in, const noa_hip_view* out, const int64_t* shape, int32_t remap,
+                          const noa_hip_ctf* ctf, const noa_hip_ctf_options* opt, bool isotropic, int& device) {
+    const char* fn = isotropic ? "ctf_isotropic" : "ctf_anisotropic";
+    NOA_CHECK(out != nullptr && shape != nullptr && ctf != nullptr, INVALID, "null arguments");
+    NOA_CHECK(out->data != nullptr && !out->is_value, INVALID, "Empty array detected");
+    const bool half = remap == NOA_HIP_H2H || remap == NOA_HIP_HC2H || remap == NOA_HIP_H2HC || remap == NOA_HIP_HC2HC;
+    const bool full = remap == NOA_HIP_F2F || remap == NOA_HIP_FC2F || remap == NOA_HIP_F2FC || remap == NOA_HIP_FC2FC;
+    NOA_CHECK(half || full, INVALID, "%s: the layout should be h2h, hc2h, h2hc, hc2hc, f2f, fc2f, f2fc or fc2fc, got %d",
+              fn, remap);
+    for (int i = 0; i < 4; ++i) NOA_CHECK(shape[i] > 0, INVALID, "Empty array detected");
+    int64_t expected[4];
+    if (half) rfft_shape(shape, expected);
+    else for (int i = 0; i < 4; ++i) expected[i] = shape[i];
+    NOA_CHECK(same_shape(out->shape, expected), INVALID,
+              "The output shape doesn't match the expected shape. Got output:shape=%s and expected:shape=%s",
+              shape_str(out->shape).c_str(), shape_str(expected).c_str());
+    NOA_CHECK(is_real(out->dtype) || out->dtype == NOA_HIP_C32 || out->dtype == NOA_HIP_C64, INVALID,
+              "%s: the output should be f32, f64, c32 or c64, got dtype=%d", fn, out->dtype);
+    const bool has_input = in != nullptr && in->data != nullptr;
+    const bool src_c = (remap & 4) != 0, dst_c = (remap & 8) != 0;
+    int64_t is[4]{};
+    if (has_input) {
+        NOA_CHECK(!in->is_value, INVALID, "%s: the input should be an array", fn);
+        for (int i = 0; i < 4; ++i) {
+            NOA_CHECK(in->shape[i] == expected[i] || in->shape[i] == 1, INVALID,
+                      "Cannot broadcast an array of shape %s into an array of shape %s", shape_str(in->shape).c_str(),
+                      shape_str(expected).c_str());
+            is[i] = in->shape[i] == expected[i] ? in->strides[i] : 0;
+        }
+        NOA_CHECK(src_c == dst_c || !overlap(*in, *out), INVALID,
+                  "This function cannot execute an in-place multiplication and a remapping");
+        NOA_CHECK(in->data == out->data || !overlap(*in, *out), INVALID, "Input and output arrays should not overlap");
+        const int i = in->dtype, o = out->dtype;
+        NOA_CHECK((is_complex(i) && is_complex(o)) || (is_real(i) && is_real(o)) || (is_complex(i) && is_real(o)),
+                  INVALID, "Invalid input/output types: %d -> %d", i, o);
+        NOA_CHECK(i != NOA_HIP_C16 && i != NOA_HIP_CBF16, UNSUPPORTED, "%s: f16/bf16 inputs are not supported", fn);
+        const bool i64p = i == NOA_HIP_F64 || i == NOA_HIP_C64, o64p = o == NOA_HIP_F64 || o == NOA_HIP_C64;
+        NOA_CHECK(i64p == o64p, UNSUPPORTED, "input and output should have the same precision");
+    }
+    NOA_CHECK(ctf->dtype == NOA_HIP_F32 || ctf->dtype == NOA_HIP_F64, INVALID,
+              "ctf: expected %s<f32> or %s<f64>, got dtype=%d", isotropic ? "CTFIsotropic" : "CTFAnisotropic",
+              isotropic ? "CTFIsotropic" : "CTFAnisotropic", ctf->dtype);
+    NOA_CHECK(ctf->data != nullptr, INVALID, "ctf: Empty array detected");
+    const bool f64 = ctf->dtype == NOA_HIP_F64;
+    const size_t ctf_bytes = isotropic ? (f64 ? 88 : 44) : (f64 ? 112 : 56);
+    const size_t ctf_align = isotropic ? (f64 ? 8 : 4) : (f64 ? 16 : 8);
+    if (ctf->batched) {
+        NOA_CHECK(ctf->count == shape[0], INVALID,
+                  "The CTFs should be specified as a contiguous vector with %lld elements, but got %lld elements",
+                  (long long)shape[0], (long long)ctf->count);
+        NOA_CHECK(reinterpret_cast<uintptr_t>(ctf->data) % ctf_align == 0, INVALID,
+                  "ctf: the array should be aligned as %s", isotropic ? "CTFIsotropic" : "CTFAnisotropic");
+    }
+    if (!isotropic)
+        NOA_CHECK(ndim4(shape) == 2, INVALID,
+                  "Only (batched) 2d arrays are supported with anisotropic CTFs, but got ctf:shape=%s",
+                  shape_str(shape).c_str());
+    noa_hip_ctf_options o{0, -1, 1, 0, 0};
+    if (opt) o = *opt;
+    NOA_CHECK(ndim4(expected) == 1 || allclose_f64(o.fftfreq_start, 0.), INVALID,
+              "For multidimensional cases, the starting fftfreq should be 0, but got %g", o.fftfreq_start);
+
+
+    // noa's reordering: sel[i] is the output's axis of the i-th iterated dimension (outermost first).
+    int64_t sh[4], os[4];
+    for (int i = 0; i < 4; ++i) { sh[i] = shape[i]; os[i] = out->strides[i]; }
+    const int nd = ndim4(shape);
+    int sel[3] = {0, 0, 0};
+    int n = nd;
+    if (nd == 1) {
+        int index = 3;   // non_empty_dhw_dimension (runtime/core/Utils.hpp:30-35)
+        for (int i = 1; i < 4; ++i) if (sh[i] > 1) { index = i; break; }
+        sel[0] = index;
+    } else if (nd == 2) {
+        const int64_t k2 = sh[2] <= 1 ? std::numeric_limits<int64_t>::max() : os[2];
+        const int64_t k3 = sh[3] <= 1 ? std::numeric_limits<int64_t>::max() : os[3];
+        sel[0] = 2; sel[1] = 3;
+        if (!(k2 >= k3)) { sel[0] = 3; sel[1] = 2; }
+    } else {
+        int order[3] = {0, 1, 2};
+        int64_t key[3];
+        for (int i = 0; i < 3; ++i) key[i] = sh[i + 1] <= 1 ? std::numeric_limits<int64_t>::max() : os[i + 1];
+        std::stable_sort(order, order + 3, [&](int a, int b) { return key[a] > key[b]; });
+        for (int i = 0; i < 3; ++i) sel[i] = order[i] + 1;
+    }
+    CtfNdDesc d;
+    d.n_dims = n;
+    d.batch = shape[0];
+    d.output = to_arr4(*out);
+    if (has_input) { d.input = to_arr4(*in); d.input.s[0] = is[0]; }
+    for (int i = 0; i < n; ++i) {
+        const int a = sel[i];
+        d.dims[i] = sh[a];
+        const int64_t iter = (half && i == n - 1) ? sh[a] / 2 + 1 : sh[a];
+        NOA_CHECK(iter <= expected[a], UNSUPPORTED,
+                  "%s: with a half layout, the output's width should be its innermost dimension (got strides %s)", fn,
+                  shape_str(out->strides).c_str());
+        d.output.s[i + 1] = os[a];
+        if (has_input) d.input.s[i + 1] = is[a];
+    }
+    for (int i = n; i < 3; ++i) { d.output.s[i + 1] = 0; d.input.s[i + 1] = 0; }
+    d.half = half;
+    d.src_centered = src_c;
+    d.dst_centered = dst_c;
+    d.isotropic = isotropic;
+    d.ctf_abs = o.ctf_abs != 0;
+    d.ctf_squared = o.ctf_squared != 0;
+    d.range_start = o.fftfreq_start;
+    d.range_stop = o.fftfreq_stop;
+    d.endpoint = o.endpoint != 0;
+    d.ctf_f64 = f64;
+    d.ctf_batched = ctf->batched != 0;
+    if (d.ctf_batched) d.ctf_data = ctf->data;
+    else std::memcpy(d.ctf_single, ctf->data, ctf_bytes);
+
+    device = device_of(out->data, nullptr);
+    NOA_CHECK(device >= 0, INVALID, "output: the pointer is not device memory");
+    if (has_input)
+        NOA_CHECK(device_of(in->data, nullptr) == device, INVALID,
+                  "The input and output arrays must be on the same device");
+    if (ctf->batched)
+        NOA_CHECK(device_of(ctf->data, nullptr) == device, INVALID, "The ctf and output arrays must be on the same device");
+    return d;
+}
+
 int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr) {
     g_last_error.clear();
     if (!args) {
@@ -1065,6 +1201,17 @@ int noa_hip_ctf_anisotropic_2d(const noa_hip_view* input, const noa_hip_view* ou
     using namespace noa_hip;
     g_last_error.clear();
     int device = -1;
+    const bool full = remap == NOA_HIP_F2F || remap == NOA_HIP_FC2F || remap == NOA_HIP_F2FC || remap == NOA_HIP_FC2FC;
+    if (full) {   // full layouts: the general operator
+        CtfNdDesc d;
+        try {
+            d = validate_ctf_nd(input, output, shape, remap, ctf, options, false, device);
+        } catch (const Fail& f) {
+            g_last_error = f.msg;
+            return f.code;
+        }
+        return guarded(device, [&](std::string& err) { return launch_ctf_nd(d, static_cast<hipStream_t>(stream), err); });
+    }
     CtfDesc d;
     try {
         d = validate_ctf(input, output, nullptr, shape, remap, ctf, options, false, device);
@@ -1073,6 +1220,20 @@ int noa_hip_ctf_anisotropic_2d(const noa_hip_view* input, const noa_hip_view* ou
         return f.code;
     }
     return guarded(device, [&](std::string& err) { return launch_ctf(d, static_cast<hipStream_t>(stream), err); });
+}
+int noa_hip_ctf_isotropic(const noa_hip_view* input, const noa_hip_view* output, const int64_t shape[4],
+                          int32_t remap, const noa_hip_ctf* ctf, const noa_hip_ctf_options* options, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    int device = -1;
+    CtfNdDesc d;
+    try {
+        d = validate_ctf_nd(input, output, shape, remap, ctf, options, true, device);
+    } catch (const Fail& f) {
+        g_last_error = f.msg;
+        return f.code;
+    }
+    return guarded(device, [&](std::string& err) { return launch_ctf_nd(d, static_cast<hipStream_t>(stream), err); });
 }
 int noa_hip_ctf_anisotropic_2d_weighted(const noa_hip_view* input, const noa_hip_view* output,
                                         const noa_hip_view* output_weight, const int64_t shape[4], int32_t remap,

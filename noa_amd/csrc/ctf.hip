@@ -1,11 +1,13 @@
 // noa::signal::ctf_anisotropic for 2D rfft layouts (src/noa/signal/CTF.hpp:14-125 operator, :430-500 public
 // function) and the CTFAnisotropic<T> model it evaluates (src/noa/signal/core/CTF.hpp:291-575), plus a fused
 // form that writes input * CTF and CTF^2 in one pass: the prologue of the CTF-weighted insertion, where noa calls
-// the operator twice (once on the slices, once to write the squared CTF as the weights).
+// the operator twice (once on the slices, once to write the squared CTF as the weights). Then the general
+// operator: ctf_isotropic over 1, 2 or 3 dimensions and both CTFs on full layouts.
 //
-// Elementwise and HBM-bound: one lane per rfft element. The CTF is evaluated once per element, in the CTF's
-// precision, with noa's expressions in noa's order.
+// Elementwise and HBM-bound: one lane per element. The CTF is evaluated once per element, in the CTF's precision,
+// with noa's expressions in noa's order.
 #include <cstring>
+#include <type_traits>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -170,6 +172,187 @@ int dispatch(const CtfDesc& d, hipStream_t stream, std::string& err) {
 
 int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err) {
     return d.ctf_f64 ? dispatch<double>(d, stream, err) : dispatch<float>(d, stream, err);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// The general operator (signal/CTF.hpp:14-125) over 1, 2 or 3 dimensions, half or full layouts, isotropic
+// (CTFIsotropic, core/CTF.hpp:30-272) or anisotropic CTFs: ctf_isotropic (:216-421) and the full layouts of
+// ctf_anisotropic (:430-500). The host has already put the dimensions in noa's order.
+// ------------------------------------------------------------------------------------------------------------------
+
+// The members of CTFIsotropic<T> in declaration order (core/CTF.hpp:259-271).
+template<class T>
+struct CtfIso {
+    T pixel_size, defocus, phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+};
+static_assert(sizeof(CtfIso<float>) == 44 && sizeof(CtfIso<double>) == 88, "noa's CTFIsotropic layout");
+
+// CTFIsotropic::value_at (core/CTF.hpp:204-212).
+template<class T>
+__device__ __forceinline__ T ctf_value_at(const CtfIso<T>& c, T fftfreq) {
+    const T r1 = fftfreq / c.pixel_size;
+    const T r2 = r1 * r1;
+    const T r4 = r2 * r2;
+    const T phase = (c.k1 * r2 * c.defocus) + (c.k2 * r4) - c.phase_shift - c.k3;
+    T ctf = -sin(phase);
+    if (c.bfactor_forth != 0) ctf *= exp(c.bfactor_forth * r2);
+    return ctf * c.scale;
+}
+
+template<class T, class C, class In, class Out>
+struct CtfNdParams {
+    const In* in; i64 is[4];
+    Out* out; i64 os[4];
+    const C* ctfs;                  // batched (device), or null: `single`
+    C single;
+    i32 dims[3];                    // logical sizes of the iterated dimensions, outermost first
+    u32 iter[3];                    // iterated extents (the last one halved for a half layout)
+    u32 chunks;                     // blocks per innermost line
+    T step[3];
+    T start;
+    bool half, src_centered, dst_centered, ctf_abs, ctf_squared;
+};
+
+// One block of 256 lanes per stretch of an innermost line: the line's outer indices are decoded once per block,
+// the lanes run along the innermost dimension (contiguous for a row-major output).
+template<int N, bool HAS_INPUT, class T, class C, class In, class Out>
+__global__ __launch_bounds__(256) void ctf_nd_kernel(const CtfNdParams<T, C, In, Out> p) {
+    constexpr bool ISO = !std::is_same<C, CtfAniso<T>>::value;
+    using RIn = typename vt<In>::real;
+    const u32 line = blockIdx.x / p.chunks;
+    const u32 x = (blockIdx.x - line * p.chunks) * 256u + threadIdx.x;
+    if (x >= p.iter[N - 1]) return;
+    u32 idx[3];
+    idx[N - 1] = x;
+    u32 r = line;
+    for (int i = N - 2; i >= 0; --i) {
+        idx[i] = r % p.iter[i];
+        r /= p.iter[i];
+    }
+    const u32 b = r;
+    const C c = p.ctfs ? p.ctfs[b] : p.single;
+    T f[3];
+    i64 in_off = static_cast<i64>(b) * p.is[0], out_off = static_cast<i64>(b) * p.os[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool halfdim = p.half && i == N - 1;
+        const i32 j = static_cast<i32>(idx[i]);
+        const i32 freq = halfdim ? j : index2frequency(p.dst_centered, j, p.dims[i]);
+        const i32 src = halfdim ? j : frequency2index(p.src_centered, freq, p.dims[i]);   // remap_indices
+        f[i] = static_cast<T>(freq) * p.step[i];
+        if constexpr (HAS_INPUT) in_off += static_cast<i64>(src) * p.is[i + 1];
+        out_off += static_cast<i64>(j) * p.os[i + 1];
+    }
+    T ctf;
+    if constexpr (ISO) {
+        T arg;
+        if constexpr (N == 1) arg = f[0] + p.start;
+        else if constexpr (N == 2) arg = sqrt(f[0] * f[0] + f[1] * f[1]);
+        else arg = sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]);
+        ctf = ctf_value_at(c, arg);
+    } else {
+        ctf = ctf_value_at(c, f[0], f[N - 1]);
+    }
+    if (p.ctf_squared) ctf *= ctf;
+    else if (p.ctf_abs) ctf = fabs(ctf);
+    if constexpr (HAS_INPUT) {
+        p.out[out_off] = vcast<Out>(vmul(p.in[in_off], static_cast<RIn>(ctf)));
+    } else {
+        Out value = vzero<Out>();
+        if constexpr (vt<Out>::cx) value.x = static_cast<typename vt<Out>::real>(ctf);
+        else value = static_cast<Out>(ctf);
+        p.out[out_off] = value;
+    }
+}
+
+namespace {
+
+template<class T, class C, class In, class Out>
+int run_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err) {
+    CtfNdParams<T, C, In, Out> p{};
+    p.in = static_cast<const In*>(d.input.data);
+    p.out = static_cast<Out*>(d.output.data);
+    for (int i = 0; i < 4; ++i) { p.is[i] = d.input.s[i]; p.os[i] = d.output.s[i]; }
+    p.ctfs = d.ctf_batched ? static_cast<const C*>(d.ctf_data) : nullptr;
+    if (!d.ctf_batched) std::memcpy(&p.single, d.ctf_single, sizeof(C));
+    const int n = d.n_dims;
+    i64 lines = d.batch;
+    for (int i = 0; i < n; ++i) {
+        const i64 it = (d.half && i == n - 1) ? d.dims[i] / 2 + 1 : d.dims[i];
+        if (d.dims[i] > 0x7fffffffLL) { err = "ctf: a dimension exceeds 2^31"; return NOA_HIP_ERROR_UNSUPPORTED; }
+        p.dims[i] = static_cast<i32>(d.dims[i]);
+        p.iter[i] = static_cast<u32>(it);
+        p.step[i] = fftfreq_step<T>(d.dims[i], d.range_start, d.range_stop, d.endpoint);
+        if (i < n - 1) lines *= it;
+    }
+    p.start = static_cast<T>(d.range_start);
+    p.chunks = (p.iter[n - 1] + 255u) / 256u;
+    const i64 blocks = lines * p.chunks;
+    if (blocks == 0) return NOA_HIP_OK;
+    if (blocks > 0xffffffffLL) { err = "ctf: more than 2^32 blocks in one call"; return NOA_HIP_ERROR_UNSUPPORTED; }
+    p.half = d.half;
+    p.src_centered = d.src_centered;
+    p.dst_centered = d.dst_centered;
+    p.ctf_abs = d.ctf_abs;
+    p.ctf_squared = d.ctf_squared;
+    const dim3 grid(static_cast<u32>(blocks)), block(256);
+    auto launch = [&](auto n_tag) {
+        constexpr int NN = decltype(n_tag)::value;
+        if constexpr (NN == 2 || !std::is_same<C, CtfAniso<T>>::value) {
+            if (p.in) {
+                if constexpr (vt<In>::cx || !vt<Out>::cx)
+                    hipLaunchKernelGGL((ctf_nd_kernel<NN, true, T, C, In, Out>), grid, block, 0, stream, p);
+            } else {
+                hipLaunchKernelGGL((ctf_nd_kernel<NN, false, T, C, In, Out>), grid, block, 0, stream, p);
+            }
+        }
+    };
+    if (n == 1) launch(std::integral_constant<int, 1>{});
+    else if (n == 2) launch(std::integral_constant<int, 2>{});
+    else launch(std::integral_constant<int, 3>{});
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+    return NOA_HIP_OK;
+}
+
+template<class T, class C>
+int dispatch_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err) {
+    const int in = d.input.data ? d.input.dtype : NOA_HIP_DTYPE_NONE, out = d.output.dtype;
+    switch (in) {
+        case NOA_HIP_C32:
+            if (out == NOA_HIP_C32) return run_nd<T, C, float2, float2>(d, stream, err);
+            if (out == NOA_HIP_F32) return run_nd<T, C, float2, float>(d, stream, err);
+            break;
+        case NOA_HIP_F32:
+            if (out == NOA_HIP_F32) return run_nd<T, C, float, float>(d, stream, err);
+            break;
+        case NOA_HIP_C64:
+            if (out == NOA_HIP_C64) return run_nd<T, C, double2, double2>(d, stream, err);
+            if (out == NOA_HIP_F64) return run_nd<T, C, double2, double>(d, stream, err);
+            break;
+        case NOA_HIP_F64:
+            if (out == NOA_HIP_F64) return run_nd<T, C, double, double>(d, stream, err);
+            break;
+        default:
+            if (out == NOA_HIP_F32) return run_nd<T, C, float, float>(d, stream, err);
+            if (out == NOA_HIP_C32) return run_nd<T, C, float, float2>(d, stream, err);
+            if (out == NOA_HIP_F64) return run_nd<T, C, double, double>(d, stream, err);
+            if (out == NOA_HIP_C64) return run_nd<T, C, double, double2>(d, stream, err);
+            break;
+    }
+    err = "ctf: unsupported input/output value types";
+    return NOA_HIP_ERROR_UNSUPPORTED;
+}
+
+} // namespace
+
+int launch_ctf_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err) {
+    if (d.isotropic)
+        return d.ctf_f64 ? dispatch_nd<double, CtfIso<double>>(d, stream, err)
+                         : dispatch_nd<float, CtfIso<float>>(d, stream, err);
+    if (d.n_dims != 2) { err = "ctf_anisotropic: 2D spectra only"; return NOA_HIP_ERROR_UNSUPPORTED; }
+    return d.ctf_f64 ? dispatch_nd<double, CtfAniso<double>>(d, stream, err)
+                     : dispatch_nd<float, CtfAniso<float>>(d, stream, err);
 }
 
 } // namespace noa_hip

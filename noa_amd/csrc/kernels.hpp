@@ -102,6 +102,23 @@ struct Arr4 {
     i64 s[4]{};
     int dtype{};
 };
+// A CTF over 1, 2 or 3 dimensions (noa's general CTF operator), after noa's reordering of the dimensions: dims are
+// the iterated dimensions, outermost first, the last one being the half dimension of a half layout.
+struct CtfNdDesc {
+    Arr4 input, output;        // s[0]: batch stride, s[1 + i]: stride of dims[i]; input.data == nullptr: the CTF
+    int n_dims{};              // 1, 2 or 3
+    i64 batch{};
+    i64 dims[3]{};             // logical sizes
+    bool half{}, src_centered{}, dst_centered{};
+    bool isotropic{};          // CTFIsotropic objects, else CTFAnisotropic (2 dimensions)
+    bool ctf_abs{}, ctf_squared{};
+    double range_start{}, range_stop{-1};
+    bool endpoint{true};
+    bool ctf_f64{}, ctf_batched{};
+    const void* ctf_data{};    // device array of CTF objects when batched
+    alignas(16) unsigned char ctf_single[112]{};
+};
+
 struct RemapDesc {
     Arr4 input, output;
     i64 shape[4]{};          // logical BDHW
@@ -167,6 +184,7 @@ int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err)
 int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
 int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err);
+int launch_ctf_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err);
 int launch_bspline_prefilter(const BSplineDesc& d, hipStream_t stream, std::string& err);
 int launch_fft(const FftDesc& d, hipStream_t stream, std::string& err);
 int launch_mrc_decode(const MrcChunk& c, hipStream_t stream, std::string& err);

@@ -4,6 +4,8 @@
   constructor's arithmetic in T; :func:`ctf_anisotropic` is ``noa::signal::ctf_anisotropic<REMAP>``
   (src/noa/signal/CTF.hpp:430-500) for 2D rfft layouts; :func:`ctf_anisotropic_weighted` writes the CTF-multiplied
   slices and the squared CTF in one pass (the two calls that precede a CTF-weighted insertion).
+- :class:`CTFIsotropic` builds noa's ``CTFIsotropic<T>`` (signal/core/CTF.hpp:30-272); :func:`ctf_isotropic` is
+  ``noa::signal::ctf_isotropic<REMAP>`` (signal/CTF.hpp:216-421) for 1D, 2D and 3D spectra, half and full layouts.
 - :func:`phase_shift_2d` is ``noa::signal::phase_shift_2d`` (re-exported from :mod:`noa_amd.xform`).
 
 Everything is computed by the HIP library through its C ABI; nothing here computes a result on the host.
@@ -85,6 +87,70 @@ class CTFAnisotropic:
         return bytes(self.to_struct())
 
 
+class CTFIsotropic:
+    """noa::signal::CTFIsotropic<T> (T = float32 or float64).
+
+    pixel_size in A/px; defocus in um (positive underfocus); voltage in kV; amplitude fraction; cs in mm;
+    phase_shift in rad; bfactor in A^2; scale. The stored members follow noa's constructor (:61-82): defocus in
+    angstroms with the sign flipped, cs in angstroms, voltage in volts, bfactor / 4, k1, k2, k3 as for
+    CTFAnisotropic. ``CTFIsotropic.from_anisotropic(ctf)`` is noa's conversion (:95-106): mean pixel size, the
+    defocus value without astigmatism.
+    """
+
+    def __init__(self, pixel_size, defocus, voltage, amplitude, cs, phase_shift=0.0, bfactor=0.0, scale=1.0,
+                 dtype=np.float32):
+        t = np.dtype(dtype).type
+        if t not in (np.float32, np.float64):
+            raise NoaError("CTFIsotropic: float32 or float64")
+        self.dtype = np.dtype(t)
+        self.pixel_size = t(pixel_size)
+        self.defocus = -t(defocus) * t(1e4)
+        self.phase_shift = t(phase_shift)
+        self.cs = t(cs) * t(1e7)
+        self.voltage = t(voltage) * t(1e3)
+        self.amplitude = t(amplitude)
+        self.scale = t(scale)
+        self.bfactor_forth = t(bfactor) / t(4)
+        lam = relativistic_electron_wavelength(float(self.voltage)) * 1e10
+        self.k1 = t(math.pi * lam)
+        self.k2 = t(math.pi * 0.5 * float(self.cs) * lam * lam * lam)
+        a = self.amplitude
+        self.k3 = t(np.arctan(a / np.sqrt(t(1) - a * a)))
+
+    @classmethod
+    def from_anisotropic(cls, ctf: "CTFAnisotropic") -> "CTFIsotropic":
+        t = ctf.dtype.type   # the getters of core/CTF.hpp:384-397 (mean pixel size, defocus value, cs, voltage)
+        return cls((ctf.pixel_size[0] + ctf.pixel_size[1]) / t(2), -ctf.defocus_value * t(1e-4),
+                   ctf.voltage * t(1e-3), ctf.amplitude, ctf.cs * t(1e-7), ctf.phase_shift, ctf.bfactor_forth * t(4),
+                   ctf.scale, dtype=ctf.dtype)
+
+    def to_struct(self):
+        cls = abi.CtfIsotropicF64 if self.dtype == np.float64 else abi.CtfIsotropicF32
+        s = cls()
+        for name, _ in cls._fields_:
+            setattr(s, name, float(getattr(self, name)))
+        return s
+
+    def to_bytes(self) -> bytes:
+        return bytes(self.to_struct())
+
+
+_CTF_LAYOUTS = {"f2f": abi.F2F, "f": abi.F2F, "fc2fc": abi.FC2FC, "fc": abi.FC2FC, "fc2f": abi.FC2F,
+                "f2fc": abi.F2FC}
+
+
+def ctf_layout(remap) -> int:
+    """noa::fft::Layout::from_string restricted to the layouts of the CTF functions (hx2hx or fx2fx)."""
+    if isinstance(remap, int):
+        if remap not in (abi.H2H, abi.HC2H, abi.H2HC, abi.HC2HC, abi.F2F, abi.FC2F, abi.F2FC, abi.FC2FC):
+            raise NoaError(f"invalid remap value {remap}")
+        return remap
+    key = remap.strip().lower()
+    if key in _CTF_LAYOUTS:
+        return _CTF_LAYOUTS[key]
+    return layout(remap)
+
+
 def ctf_array(ctfs: Sequence[CTFAnisotropic], device) -> torch.Tensor:
     """A device array of noa CTF objects (one per batch), as a uint8 tensor of the objects' bytes."""
     if not ctfs:
@@ -96,9 +162,11 @@ def ctf_array(ctfs: Sequence[CTFAnisotropic], device) -> torch.Tensor:
     return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
 
 
-def _ctf_arg(ctf, count: int, device: torch.device, keep: list) -> abi.Ctf:
+def _ctf_arg(ctf, count: int, device: torch.device, keep: list, kind=None) -> abi.Ctf:
+    kind = kind or CTFAnisotropic
     c = abi.Ctf()
-    if isinstance(ctf, CTFAnisotropic):
+    sizes = (44, 88) if kind is CTFIsotropic else (56, 112)
+    if isinstance(ctf, kind):
         s = ctf.to_struct()
         keep.append(s)
         c.data = ctypes.addressof(s)
@@ -107,8 +175,8 @@ def _ctf_arg(ctf, count: int, device: torch.device, keep: list) -> abi.Ctf:
         c.count = 1
         return c
     if isinstance(ctf, (list, tuple)):
-        if not all(isinstance(v, CTFAnisotropic) for v in ctf):
-            raise NoaError("ctf: expected CTFAnisotropic objects")
+        if not all(isinstance(v, kind) for v in ctf):
+            raise NoaError(f"ctf: expected {kind.__name__} objects")
         t = ctf_array(ctf, device)
         keep.append(t)
         c.data = t.data_ptr()
@@ -117,15 +185,16 @@ def _ctf_arg(ctf, count: int, device: torch.device, keep: list) -> abi.Ctf:
         c.count = len(ctf)
         return c
     if isinstance(ctf, torch.Tensor):
-        # A device array of objects built elsewhere: (count, 56) or (count, 112) uint8.
-        if ctf.dtype != torch.uint8 or ctf.ndim != 2 or ctf.shape[1] not in (56, 112):
-            raise NoaError("ctf: expected a (count, 56|112) uint8 device tensor of CTFAnisotropic objects")
+        # A device array of objects built elsewhere: (count, bytes per object) uint8.
+        if ctf.dtype != torch.uint8 or ctf.ndim != 2 or ctf.shape[1] not in sizes:
+            raise NoaError(f"ctf: expected a (count, {sizes[0]}|{sizes[1]}) uint8 device tensor of {kind.__name__} "
+                           "objects")
         if ctf.device != device:
             raise NoaError("The ctf and output arrays must be on the same device")
         t = ctf.contiguous()
         keep.append(t)
         c.data = t.data_ptr()
-        c.dtype = abi.F64 if ctf.shape[1] == 112 else abi.F32
+        c.dtype = abi.F64 if ctf.shape[1] == sizes[1] else abi.F32
         c.batched = 1
         c.count = ctf.shape[0]
         return c
@@ -141,9 +210,10 @@ def _options(options: CTFOptions | None) -> abi.CtfOptions:
 
 
 def ctf_anisotropic(remap, input, output, shape, ctf, options: CTFOptions | None = None) -> None:
-    """noa::signal::ctf_anisotropic<REMAP> (signal/CTF.hpp:430-500) on (B, 1, H, W/2+1) rfft arrays: output =
-    input * CTF (|.|^2 for complex -> real), or the CTF when `input` is None. `ctf`: one CTFAnisotropic
-    (broadcast), a list of them (one per batch) or a (B, 56|112) uint8 device tensor of noa CTF objects."""
+    """noa::signal::ctf_anisotropic<REMAP> (signal/CTF.hpp:430-500) on (B, 1, H, W) spectra, half ((B, 1, H, W/2+1)
+    arrays) or full layouts: output = input * CTF (|.|^2 for complex -> real), or the CTF when `input` is None.
+    `ctf`: one CTFAnisotropic (broadcast), a list of them (one per batch) or a (B, 56|112) uint8 device tensor of
+    noa CTF objects."""
     shape = _shape4(shape, "shape")
     if not isinstance(output, torch.Tensor) or output.device.type != "cuda":
         raise NoaError("ctf_anisotropic: the output should be a device tensor")
@@ -154,7 +224,7 @@ def ctf_anisotropic(remap, input, output, shape, ctf, options: CTFOptions | None
     opts = _options(options)
     c_shape = (ctypes.c_int64 * 4)(*shape)
     stream = torch.cuda.current_stream(output.device).cuda_stream
-    status = lib().noa_hip_ctf_anisotropic_2d(ctypes.byref(in_v), ctypes.byref(out_v), c_shape, layout(remap),
+    status = lib().noa_hip_ctf_anisotropic_2d(ctypes.byref(in_v), ctypes.byref(out_v), c_shape, ctf_layout(remap),
                                               ctypes.byref(c), ctypes.byref(opts), ctypes.c_void_p(stream))
     if status != abi.OK:
         raise NoaError(last_error())
@@ -179,5 +249,26 @@ def ctf_anisotropic_weighted(remap, input, output, output_weight, shape, ctf,
     status = lib().noa_hip_ctf_anisotropic_2d_weighted(ctypes.byref(in_v), ctypes.byref(out_v), ctypes.byref(w_v),
                                                        c_shape, layout(remap), ctypes.byref(c), ctypes.byref(opts),
                                                        ctypes.c_void_p(stream))
+    if status != abi.OK:
+        raise NoaError(last_error())
+
+
+def ctf_isotropic(remap, input, output, shape, ctf, options: CTFOptions | None = None) -> None:
+    """noa::signal::ctf_isotropic<REMAP> (signal/CTF.hpp:216-421) on 1D, 2D or 3D spectra (the dimensionality of the
+    logical BDHW `shape`), half (last dimension W/2+1) or full layouts: output = input * CTF(|f|) (|.|^2 for
+    complex -> real), or the CTF when `input` is None. 1D spectra may start the fftfreq range above 0. `ctf`: one
+    CTFIsotropic (broadcast), a list of them (one per batch) or a (B, 44|88) uint8 device tensor of noa objects."""
+    shape = _shape4(shape, "shape")
+    if not isinstance(output, torch.Tensor) or output.device.type != "cuda":
+        raise NoaError("ctf_isotropic: the output should be a device tensor")
+    keep: list = []
+    out_v = _view(output, keep, abi.F32, abi.C32, False, "output")
+    in_v = _view(input, keep, abi.F32, abi.C32, False, "input") if input is not None else abi.View()
+    c = _ctf_arg(ctf, shape[0], output.device, keep, CTFIsotropic)
+    opts = _options(options)
+    c_shape = (ctypes.c_int64 * 4)(*shape)
+    stream = torch.cuda.current_stream(output.device).cuda_stream
+    status = lib().noa_hip_ctf_isotropic(ctypes.byref(in_v), ctypes.byref(out_v), c_shape, ctf_layout(remap),
+                                         ctypes.byref(c), ctypes.byref(opts), ctypes.c_void_p(stream))
     if status != abi.OK:
         raise NoaError(last_error())

@@ -43,6 +43,7 @@
 #include <string>
 #include <algorithm>
 #include <limits>
+#include <type_traits>
 #include <omp.h>
 
 #include "../include/noa_hip.h"
@@ -1519,6 +1520,139 @@ void cubic_bspline_prefilter(const T* input, const i64 is_[4], T* output, const 
     }
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// noa::signal::CTFIsotropic<T> (src/noa/signal/core/CTF.hpp:30-272) and the CTF operator over 1, 2 or 3
+// dimensions for every half and full layout (src/noa/signal/CTF.hpp:14-125), with the dispatch of the public
+// functions (ctf_isotropic :216-421, ctf_anisotropic :430-500): the dimensions are put in the rightmost order of
+// the output first, the input's (broadcast) strides following.
+// ------------------------------------------------------------------------------------------------------------
+template<class T>
+struct CtfIso {   // members in declaration order (:259-271)
+    T pixel_size, defocus, phase_shift, cs, voltage, amplitude, bfactor_forth, scale, k1, k2, k3;
+};
+
+// The constructor from physical parameters (:61-82): pixel size (A/px), defocus (um, positive underfocus),
+// voltage (kV), amplitude fraction, cs (mm), phase shift (rad), bfactor (A^2), scale.
+template<class T>
+CtfIso<T> make_ctf_iso(T pixel_size, T defocus, T voltage, T amplitude, T cs, T phase_shift, T bfactor, T scale) {
+    CtfIso<T> c{};
+    c.pixel_size = pixel_size;
+    c.defocus = -defocus * static_cast<T>(1e4);
+    c.phase_shift = phase_shift;
+    c.cs = cs * static_cast<T>(1e7);
+    c.voltage = voltage * static_cast<T>(1e3);
+    c.amplitude = amplitude;
+    c.scale = scale;
+    c.bfactor_forth = bfactor / 4;                               // set_bfactor (:147-150)
+    const double lambda = relativistic_electron_wavelength(static_cast<double>(c.voltage)) * 1e10;   // (:246-252)
+    const double PI = 3.1415926535897932384626433832795;
+    c.k1 = static_cast<T>(PI * lambda);
+    c.k2 = static_cast<T>(PI * 0.5 * static_cast<double>(c.cs) * lambda * lambda * lambda);
+    c.k3 = std::atan(c.amplitude / std::sqrt(1 - c.amplitude * c.amplitude));   // (:254-256)
+    return c;
+}
+
+// value_at (:204-212).
+template<class T>
+T ctf_value_at(const CtfIso<T>& c, T fftfreq) {
+    const T r1 = fftfreq / c.pixel_size;
+    const T r2 = r1 * r1;
+    const T r4 = r2 * r2;
+    const T phase = (c.k1 * r2 * c.defocus) + (c.k2 * r4) - c.phase_shift - c.k3;
+    T ctf = -std::sin(phase);
+    if (c.bfactor_forth != 0) ctf *= std::exp(c.bfactor_forth * r2);
+    return ctf * c.scale;
+}
+
+// Linspace{start, stop <= 0 ? highest_fftfreq(n) : stop, endpoint}.for_size(n / 2 + 1).step (:65-77).
+template<class T>
+T ctf_fftfreq_step(i64 n, double start, double stop, bool endpoint) {
+    const i64 size = n / 2 + 1;
+    const T t_start = static_cast<T>(start), t_stop_opt = static_cast<T>(stop);
+    const T t_stop = t_stop_opt <= 0 ? static_cast<T>(n / 2) / static_cast<T>(n) : t_stop_opt;
+    const i64 count = std::max<i64>(1, size - (endpoint ? 1 : 0));
+    return (t_stop - t_start) / static_cast<T>(count);
+}
+
+// Returns 0, or 1 when the reordered iteration would leave the output (noa itself would write out of bounds: an
+// rfft output whose half dimension is not the innermost one).
+template<class T, class In, class Out, class Ctf, bool HAS_INPUT>
+int ctf_general(const In* input, const i64 is_[4], Out* output, const i64 os_[4], const i64 out_shape[4],
+                const i64 shape_[4], int remap, const Ctf* ctfs, bool batched, double start, double stop,
+                bool endpoint, bool ctf_abs, bool ctf_squared) {
+    constexpr bool ISO = std::is_same<Ctf, CtfIso<T>>::value;
+    const bool rfft = (remap & 2) == 0, src_c = (remap & 4) != 0, dst_c = (remap & 8) != 0;
+    i64 is[4], os[4], sh[4], ext[4];   // ext: the output's extent along each (reordered) dimension
+    for (int i = 0; i < 4; ++i) { is[i] = is_[i]; os[i] = os_[i]; sh[i] = shape_[i]; ext[i] = out_shape[i]; }
+    const int nd = ndim4(sh);
+    int N = nd;
+    i64 dims[3]{}, ist[3]{}, ost[3]{}, dext[3]{};
+    if (nd == 1) {                       // non_empty_dhw_dimension (runtime/core/Utils.hpp:30-35)
+        int index = 3;
+        for (int i = 1; i < 4; ++i) if (sh[i] > 1) { index = i; break; }
+        dims[0] = sh[index]; ist[0] = is[index]; ost[0] = os[index]; dext[0] = ext[index];
+    } else if (nd == 2) {                // H and W swapped unless rightmost (:270-276)
+        const i64 k2 = sh[2] <= 1 ? std::numeric_limits<i64>::max() : os[2];
+        const i64 k3 = sh[3] <= 1 ? std::numeric_limits<i64>::max() : os[3];
+        if (!(k2 >= k3)) { std::swap(is[2], is[3]); std::swap(os[2], os[3]); std::swap(sh[2], sh[3]); std::swap(ext[2], ext[3]); }
+        for (int i = 0; i < 2; ++i) { dims[i] = sh[i + 2]; ist[i] = is[i + 2]; ost[i] = os[i + 2]; dext[i] = ext[i + 2]; }
+    } else {                             // permute_all to the rightmost order (:298-303)
+        int order[3] = {0, 1, 2};
+        i64 key[3];
+        for (int i = 0; i < 3; ++i) key[i] = sh[i + 1] <= 1 ? std::numeric_limits<i64>::max() : os[i + 1];
+        std::stable_sort(order, order + 3, [&](int a, int b) { return key[a] > key[b]; });
+        for (int i = 0; i < 3; ++i) {
+            dims[i] = sh[order[i] + 1]; ist[i] = is[order[i] + 1]; ost[i] = os[order[i] + 1]; dext[i] = ext[order[i] + 1];
+        }
+    }
+    T step[3];
+    for (int i = 0; i < N; ++i) step[i] = ctf_fftfreq_step<T>(dims[i], start, stop, endpoint);
+    i64 iter[3];
+    for (int i = 0; i < N; ++i) iter[i] = (rfft && i == N - 1) ? dims[i] / 2 + 1 : dims[i];
+    for (int i = 0; i < N; ++i) if (iter[i] > dext[i]) return 1;
+    const T t_start = static_cast<T>(start);
+    using RIn = typename real_of<In>::type;
+    i64 idx[3] = {0, 0, 0};
+    const i64 total = iter[0] * (N > 1 ? iter[1] : 1) * (N > 2 ? iter[2] : 1);
+    for (i64 b = 0; b < sh[0]; ++b) {
+        const Ctf& c = ctfs[batched ? b : 0];
+        for (i64 t = 0; t < total; ++t) {
+            i64 r = t;
+            for (int i = N - 1; i >= 0; --i) { idx[i] = r % iter[i]; r /= iter[i]; }
+            T f[3];
+            i64 in_off = b * is[0], out_off = b * os[0];
+            for (int i = 0; i < N; ++i) {
+                const bool half = rfft && i == N - 1;
+                const i64 freq = half ? idx[i] : index2frequency(dst_c, idx[i], dims[i]);
+                const i64 src = half ? idx[i] : frequency2index(src_c, freq, dims[i]);   // remap_indices
+                f[i] = static_cast<T>(freq) * step[i];
+                in_off += src * ist[i];
+                out_off += idx[i] * ost[i];
+            }
+            T ctf;
+            if constexpr (ISO) {
+                T arg;
+                if (N == 1) arg = f[0] + t_start;
+                else if (N == 2) arg = std::sqrt(f[0] * f[0] + f[1] * f[1]);
+                else arg = std::sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]);
+                ctf = ctf_value_at(c, arg);
+            } else {
+                ctf = ctf_value_at(c, f[0], f[1]);
+            }
+            if (ctf_squared) ctf *= ctf;
+            else if (ctf_abs) ctf = std::abs(ctf);
+            Out& o = output[out_off];
+            if constexpr (HAS_INPUT) {
+                o = cast_or_abs_squared<Out>(input[in_off] * static_cast<RIn>(ctf));
+            } else {
+                if constexpr (is_cx<Out>::value) o = Out{static_cast<typename real_of<Out>::type>(ctf), 0};
+                else o = static_cast<Out>(ctf);
+            }
+        }
+    }
+    return 0;
+}
+
 } // namespace oracle
 
 extern "C" {
@@ -1665,6 +1799,68 @@ void oracle_ctf_anisotropic_make(int f64, const double pixel_size[2], double def
     }
 }
 double oracle_relativistic_electron_wavelength(double volts) { return oracle::relativistic_electron_wavelength(volts); }
+
+/* CTFIsotropic<T> from physical parameters, written to `out` (44 or 88 bytes). */
+void oracle_ctf_isotropic_make(int f64, double pixel_size, double defocus, double voltage, double amplitude, double cs,
+                               double phase_shift, double bfactor, double scale, void* out) {
+    using namespace oracle;
+    if (f64) {
+        const auto c = make_ctf_iso<double>(pixel_size, defocus, voltage, amplitude, cs, phase_shift, bfactor, scale);
+        std::memcpy(out, &c, sizeof(c));
+    } else {
+        const auto c = make_ctf_iso<float>(static_cast<float>(pixel_size), static_cast<float>(defocus),
+                                           static_cast<float>(voltage), static_cast<float>(amplitude),
+                                           static_cast<float>(cs), static_cast<float>(phase_shift),
+                                           static_cast<float>(bfactor), static_cast<float>(scale));
+        std::memcpy(out, &c, sizeof(c));
+    }
+}
+
+/* noa::signal::ctf_isotropic (isotropic != 0: CTFIsotropic<T> objects over 1, 2 or 3 dimensions) or ctf_anisotropic
+ * (CTFAnisotropic<T>, 2D) on strided arrays, every half and full layout. shape: the logical BDHW shape; out_shape:
+ * the output's shape (rfft or full); strides in elements (the input's already broadcast to the output). dtypes: 1 f32,
+ * 2 f64, 3 c32, 4 c64 (input 0: none). Returns 1 if the reordered iteration would leave the output. */
+int oracle_ctf(int isotropic, const void* input, int in_dtype, const long long in_strides[4], void* output,
+               int out_dtype, const long long out_strides[4], const long long out_shape[4], const long long shape[4],
+               int remap, const void* ctfs, int ctf_f64, int batched, double start, double stop, int endpoint,
+               int ctf_abs, int ctf_squared) {
+    using namespace oracle;
+    i64 is[4], os[4], osh[4], sh[4];
+    for (int i = 0; i < 4; ++i) { is[i] = in_strides[i]; os[i] = out_strides[i]; osh[i] = out_shape[i]; sh[i] = shape[i]; }
+    auto go = [&](auto t_tag, auto ctf_tag) -> int {
+        using T = decltype(t_tag);
+        using Ctf = decltype(ctf_tag);
+        const auto* c = static_cast<const Ctf*>(ctfs);
+        auto run = [&](auto in_tag, auto out_tag) -> int {
+            using In = decltype(in_tag);
+            using Out = decltype(out_tag);
+            if (in_dtype == 0)
+                return ctf_general<T, In, Out, Ctf, false>(nullptr, is, static_cast<Out*>(output), os, osh, sh, remap, c,
+                                                           batched != 0, start, stop, endpoint != 0, ctf_abs != 0,
+                                                           ctf_squared != 0);
+            if constexpr (is_cx<In>::value || !is_cx<Out>::value)
+                return ctf_general<T, In, Out, Ctf, true>(static_cast<const In*>(input), is, static_cast<Out*>(output), os,
+                                                          osh, sh, remap, c, batched != 0, start, stop, endpoint != 0,
+                                                          ctf_abs != 0, ctf_squared != 0);
+            return NOA_HIP_ERROR_UNSUPPORTED;
+        };
+        if (in_dtype == 3 && out_dtype == 3) return run(Cx<float>{}, Cx<float>{});
+        if (in_dtype == 3 && out_dtype == 1) return run(Cx<float>{}, float{});
+        if (in_dtype == 1 && out_dtype == 1) return run(float{}, float{});
+        if (in_dtype == 4 && out_dtype == 4) return run(Cx<double>{}, Cx<double>{});
+        if (in_dtype == 4 && out_dtype == 2) return run(Cx<double>{}, double{});
+        if (in_dtype == 2 && out_dtype == 2) return run(double{}, double{});
+        if (in_dtype == 0 && out_dtype == 1) return run(float{}, float{});
+        if (in_dtype == 0 && out_dtype == 2) return run(double{}, double{});
+        if (in_dtype == 0 && out_dtype == 3) return run(float{}, Cx<float>{});
+        if (in_dtype == 0 && out_dtype == 4) return run(double{}, Cx<double>{});
+        g_error = "oracle: unsupported CTF value types";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    };
+    if (isotropic) return ctf_f64 ? go(double{}, CtfIso<double>{}) : go(float{}, CtfIso<float>{});
+    if (ndim4(sh) != 2) { g_error = "oracle: anisotropic CTFs are 2D"; return NOA_HIP_ERROR_UNSUPPORTED; }
+    return ctf_f64 ? go(double{}, CtfAniso<double>{}) : go(float{}, CtfAniso<float>{});
+}
 
 /* interpolation_weights of a window of 4 to 8 taps (CUBIC, CUBIC_BSPLINE, LANCZOS4/6/8) at fraction f; returns the
  * window size. */

@@ -434,6 +434,59 @@ def ctf_anisotropic_2d(remap, input, output, shape, ctfs, *, fftfreq_range=(0.0,
         raise OracleError(lib().oracle_last_error().decode())
 
 
+CTF_LAYOUTS = {"h2h": 0, "hc2h": 4, "h2hc": 8, "hc2hc": 12, "f2f": 3, "fc2f": 7, "f2fc": 11, "fc2fc": 15}
+
+
+def make_ctf_isotropic(pixel_size, defocus, voltage, amplitude, cs, phase_shift=0.0, bfactor=0.0, scale=1.0,
+                       f64=False):
+    """noa::signal::CTFIsotropic<T> from physical parameters (signal/core/CTF.hpp:61-82), as raw bytes."""
+    L = lib()
+    L.oracle_ctf_isotropic_make.argtypes = [ctypes.c_int] + [c_double] * 8 + [c_void_p]
+    L.oracle_ctf_isotropic_make.restype = None
+    buf = ctypes.create_string_buffer(88 if f64 else 44)
+    L.oracle_ctf_isotropic_make(int(f64), pixel_size, defocus, voltage, amplitude, cs, phase_shift, bfactor, scale,
+                                ctypes.addressof(buf))
+    return bytes(buf.raw)
+
+
+def ctf(remap, input, output, shape, ctfs, *, isotropic=True, fftfreq_range=(0.0, -1.0, True), ctf_abs=False,
+        ctf_squared=False):
+    """noa::signal::ctf_isotropic / ctf_anisotropic<REMAP> (signal/CTF.hpp:14-125, :216-500) on strided numpy
+    arrays, every half and full layout: output = input * CTF (|.|^2 for complex -> real), or the CTF when `input` is
+    None. `input` is broadcast to the output's shape. ctfs: one CTF (bytes) or a list of B of them."""
+    out = output
+    if not isinstance(out, np.ndarray):
+        raise OracleError("ctf: numpy output expected")
+    b = shape[0]
+    items = ctfs if isinstance(ctfs, (list, tuple)) else [ctfs]
+    size = len(items[0])
+    f64 = size in (88, 112)
+    raw = ctypes.create_string_buffer(b"".join(items), len(items) * size)
+    inp = None
+    if input is not None:
+        inp = np.broadcast_to(input, out.shape)
+    el = lambda a: (ctypes.c_longlong * 4)(*[st // a.itemsize for st in a.strides])
+    L = lib()
+    L.oracle_ctf.argtypes = [ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p,
+                             c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, ctypes.c_int, c_double, c_double,
+                             ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.oracle_ctf.restype = ctypes.c_int
+    in_strides = el(inp) if inp is not None else (ctypes.c_longlong * 4)()
+    out_strides = el(out)
+    out_shape = (ctypes.c_longlong * 4)(*out.shape)
+    c_shape = (ctypes.c_longlong * 4)(*shape)
+    status = L.oracle_ctf(
+        int(bool(isotropic)), None if inp is None else inp.ctypes.data, 0 if inp is None else _CTF_DT[inp.dtype],
+        ctypes.addressof(in_strides), out.ctypes.data, _CTF_DT[out.dtype], ctypes.addressof(out_strides),
+        ctypes.addressof(out_shape), ctypes.addressof(c_shape), CTF_LAYOUTS[remap.lower()], ctypes.addressof(raw), int(f64), int(len(items) > 1), float(fftfreq_range[0]),
+        float(fftfreq_range[1]), int(bool(fftfreq_range[2]) if len(fftfreq_range) > 2 else 1), int(bool(ctf_abs)),
+        int(bool(ctf_squared)))
+    if status == 1:
+        raise OracleError("ctf: the reordered iteration would leave the output")
+    if status != 0:
+        raise OracleError(lib().oracle_last_error().decode())
+
+
 def window_weights(interp, f, f32=False):
     """interpolation_weights of CUBIC, CUBIC_BSPLINE, LANCZOS4/6/8 at fraction f (Interpolation.hpp:81-141)."""
     L = lib()

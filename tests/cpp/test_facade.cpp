@@ -39,6 +39,8 @@ static void check_layouts() {
     EXPECT(nx::Layout("h2hc").bits == NOA_HIP_H2HC);
     EXPECT(nx::Layout("hc2hc").bits == NOA_HIP_HC2HC);
     EXPECT(nx::Layout("hc").bits == NOA_HIP_HC2HC);
+    EXPECT(nx::Layout("f2f").bits == NOA_HIP_F2F && nx::Layout("fc2fc").bits == NOA_HIP_FC2FC);
+    EXPECT(nx::Layout("fc2f").bits == NOA_HIP_FC2F && nx::Layout("f2fc").bits == NOA_HIP_F2FC);
     std::printf("ok layouts\n");
 }
 
@@ -346,6 +348,23 @@ static void check_ctf_object() {
     static_assert(sizeof(f) == 56 && alignof(noa_hip::signal::CTFAnisotropic<double>) == 16);
     const auto d = noa_hip::signal::details::ctf(f);
     EXPECT(d.dtype == NOA_HIP_F32 && d.batched == 0 && d.data == &f);
+    const noa_hip::signal::CTFIsotropic<double> i(1.2, 1.5, 300.0, 0.07, 2.7, 0.2, -20.0, 2.0);
+    EXPECT(i.defocus == -15000.0 && i.cs == 2.7e7 && i.voltage == 3e5 && i.bfactor_forth == -5.0);
+    EXPECT(i.k1 == c.k1 && i.k2 == c.k2 && i.k3 == c.k3 && i.pixel_size == 1.2 && i.scale == 2.0);
+    static_assert(sizeof(noa_hip::signal::CTFIsotropic<float>) == 44);
+    const auto di = noa_hip::signal::details::ctf(i);
+    EXPECT(di.dtype == NOA_HIP_F64 && di.batched == 0);
+    // ctf_check_parameters (signal/CTF.hpp:143-189): the output's shape is checked before any memory is touched.
+    const View<float> out(reinterpret_cast<float*>(0x10000000), Shape4(2, 8, 8, 8));
+    bool thrown = false;
+    try {
+        noa_hip::signal::ctf_isotropic<"h2h">(View<float>{}, out, Shape4(2, 8, 8, 8), i);
+    } catch (const noa_hip::Exception& e) {
+        thrown = true;
+        std::printf("ok error: %s\n", e.what());
+        EXPECT(std::strstr(e.what(), "expected shape") != nullptr);
+    }
+    EXPECT(thrown);
     std::printf("ok ctf object\n");
 }
 

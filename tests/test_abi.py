@@ -55,6 +55,8 @@ def test_struct_layout_matches_the_c_compiler(tmp_path):
         "noa_hip_shift2": ("Shift2", [f[0] for f in abi.Shift2._fields_]),
         "noa_hip_ctf_anisotropic_f32": ("CtfAnisotropicF32", [f[0] for f in abi.CtfAnisotropicF32._fields_]),
         "noa_hip_ctf_anisotropic_f64": ("CtfAnisotropicF64", [f[0] for f in abi.CtfAnisotropicF64._fields_]),
+        "noa_hip_ctf_isotropic_f32": ("CtfIsotropicF32", [f[0] for f in abi.CtfIsotropicF32._fields_]),
+        "noa_hip_ctf_isotropic_f64": ("CtfIsotropicF64", [f[0] for f in abi.CtfIsotropicF64._fields_]),
         "noa_hip_ctf": ("Ctf", [f[0] for f in abi.Ctf._fields_]),
         "noa_hip_ctf_options": ("CtfOptions", [f[0] for f in abi.CtfOptions._fields_]),
     }
@@ -89,6 +91,10 @@ def test_enum_values_match_noa(lib):
         assert m and int(m.group(1)) == value, name
     # noa's Layout bits (fft/core/Layout.hpp:39-46): SRC_FULL=1, DST_FULL=2, SRC_CENTERED=4, DST_CENTERED=8.
     assert (abi.H2H, abi.HC2H, abi.H2HC, abi.HC2HC) == (0, 4, 8, 12)
+    assert (abi.F2F, abi.FC2F, abi.F2FC, abi.FC2FC) == (3, 7, 11, 15)
+    for name, value in [("NOA_HIP_F2F", 3), ("NOA_HIP_FC2F", 7), ("NOA_HIP_F2FC", 11), ("NOA_HIP_FC2FC", 15)]:
+        m = re.search(rf"\b{name}\s*=\s*(\d+)", text)
+        assert m and int(m.group(1)) == value, name
     assert lib.noa_hip_abi_version() == abi.ABI_VERSION
 
 
@@ -274,6 +280,52 @@ def test_ctf_validation_errors(lib, mutate, message):
     err = lib.noa_hip_last_error().decode()
     assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
     assert message in err, err
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda k: k.__setitem__("remap", 1), "should be h2h, hc2h, h2hc, hc2hc, f2f"),
+    (lambda k: k["shape"].__setitem__(3, 18), "doesn't match the expected shape"),
+    (lambda k: setattr(k["ctf"], "count", 3), "contiguous vector"),
+    (lambda k: setattr(k["opt"], "fftfreq_start", 0.1), "starting fftfreq"),
+    (lambda k: setattr(k["ctf"], "dtype", 3), "CTFIsotropic"),
+    (lambda k: k.__setitem__("input", np.zeros((2, 3, 16, 9), np.float32)), "Cannot broadcast"),
+    (lambda k: k.__setitem__("out", np.zeros((9, 16, 4, 2), np.float32).transpose(3, 2, 1, 0)), "innermost"),
+    (lambda k: None, "not device memory"),
+])
+def test_ctf_isotropic_validation_errors(lib, mutate, message):
+    """ctf_check_parameters (signal/CTF.hpp:143-189) and the layout rules, on host buffers (rejected before any
+    device access; a valid call reaches the device check)."""
+    from noa_amd import abi
+    b, d, h, w = 2, 4, 16, 16
+    ctfs = np.zeros((b, 11), np.float32)
+    c = abi.Ctf(ctfs.ctypes.data, abi.F32, 1, b)
+    k = {"shape": (ctypes.c_int64 * 4)(b, d, h, w), "remap": abi.HC2HC, "ctf": c, "opt": abi.CtfOptions(0, -1, 1, 0, 0),
+         "out": np.zeros((b, d, h, w // 2 + 1), np.float32), "input": None}
+    mutate(k)
+    out_v = _host_view(k["out"], abi.F32)
+    in_v = _host_view(k["input"], abi.F32) if k["input"] is not None else None
+    status = lib.noa_hip_ctf_isotropic(None if in_v is None else ctypes.byref(in_v), ctypes.byref(out_v), k["shape"],
+                                       k["remap"], ctypes.byref(k["ctf"]), ctypes.byref(k["opt"]), None)
+    err = lib.noa_hip_last_error().decode()
+    assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
+    assert message in err, err
+
+
+def test_ctf_isotropic_1d_allows_a_start_and_anisotropic_full_layouts_are_2d(lib):
+    from noa_amd import abi
+    line = np.zeros((3, 1, 1, 33), np.float32)
+    ctfs = np.zeros((3, 11), np.float32)
+    c = abi.Ctf(ctfs.ctypes.data, abi.F32, 1, 3)
+    v = _host_view(line, abi.F32)
+    status = lib.noa_hip_ctf_isotropic(None, ctypes.byref(v), (ctypes.c_int64 * 4)(3, 1, 1, 64), abi.H2H,
+                                       ctypes.byref(c), ctypes.byref(abi.CtfOptions(0.1, 0.4, 1, 0, 0)), None)
+    assert "not device memory" in lib.noa_hip_last_error().decode() and status == abi.ERROR_INVALID_ARGUMENT
+    vol = np.zeros((1, 4, 8, 8), np.float32)
+    a = np.zeros((1, 14), np.float32)
+    ca = abi.Ctf(a.ctypes.data, abi.F32, 1, 1)
+    status = lib.noa_hip_ctf_anisotropic_2d(None, ctypes.byref(_host_view(vol, abi.F32)),
+                                            (ctypes.c_int64 * 4)(1, 4, 8, 8), abi.F2F, ctypes.byref(ca), None, None)
+    assert "2d arrays" in lib.noa_hip_last_error().decode() and status == abi.ERROR_INVALID_ARGUMENT
 
 
 def _host_view(arr, dtype):

@@ -891,6 +891,151 @@ def test_ctf_options_layouts_and_input():
     np.testing.assert_array_equal(part[0, 0], full[0, 0][np.where(fy < 0, fy + 128, fy)][:, :33])
 
 
+# noa::signal::CTFIsotropic and ctf_isotropic over 1, 2 and 3 dimensions, half and full layouts, strided outputs
+# (signal/core/CTF.hpp:30-272, signal/CTF.hpp:14-125, :216-421), and the oracle's general CTF operator.
+ISO = dict(pixel_size=1.3, defocus=2.1, voltage=200, amplitude=0.1, cs=2.0, phase_shift=0.25, bfactor=-15.0,
+           scale=0.7)
+
+
+def _iso_physics(kk, pixel_size, defocus, voltage, amplitude, cs, phase_shift, bfactor, scale):
+    """The textbook CTF at |k| = kk (fftfreq units), float64 (see _ctf_physics)."""
+    lam = O.relativistic_electron_wavelength(voltage * 1e3) * 1e10
+    k2 = (kk / pixel_size) ** 2
+    g = np.pi * lam * defocus * 1e4 * k2 - np.pi / 2 * (cs * 1e7) * lam ** 3 * k2 ** 2 + phase_shift
+    return (np.sqrt(1 - amplitude ** 2) * np.sin(g) + amplitude * np.cos(g)) * np.exp(bfactor / 4 * k2) * scale
+
+
+def _grid_fftfreq(dims, half, centered):
+    """|fftfreq| of every element of a (D, H, W) or (H, W) or (W,) spectrum: the fftfreq step is 1/n per axis."""
+    axes = []
+    for i, n in enumerate(dims):
+        if half and i == len(dims) - 1:
+            axes.append(np.arange(n // 2 + 1) / n)
+        else:
+            axes.append(freq_grid(n, centered) / n)
+    grids = np.meshgrid(*axes, indexing="ij")
+    return np.sqrt(sum(g ** 2 for g in grids))
+
+
+@pytest.mark.parametrize("shape, remap", [
+    ((2, 1, 1, 64), "h2h"), ((1, 1, 1, 33), "f2f"), ((1, 1, 1, 34), "fc2fc"), ((3, 1, 40, 1), "h2h"),
+    ((2, 1, 48, 40), "hc2hc"), ((1, 1, 33, 35), "f2f"), ((1, 1, 32, 30), "fc2fc"),
+    ((1, 24, 20, 18), "h2h"), ((2, 15, 16, 17), "fc2fc"), ((1, 9, 10, 11), "hc2hc"),
+])
+def test_ctf_isotropic_matches_the_physics(shape, remap):
+    """Every dimensionality and layout against the textbook formula at |f| (f64). A wrong axis in the norm, a missing
+    dimension, a wrong centring or a wrong step moves the oscillations by many samples."""
+    b = shape[0]
+    half = remap.startswith("h")
+    c = O.make_ctf_isotropic(**ISO, f64=True)
+    out_shape = shape[:3] + ((shape[3] // 2 + 1) if half else shape[3],)
+    out = np.zeros(out_shape, np.float64)
+    O.ctf(remap, None, out, shape, c)
+    nd = 3 if shape[1] > 1 else (2 if shape[2] > 1 and shape[3] > 1 else 1)
+    centered = remap.endswith("c")
+    if nd == 1:
+        axis = 2 if shape[2] > 1 else 3
+        n = shape[axis]
+        k = (np.arange(n // 2 + 1) if half else freq_grid(n, centered)) / n
+        want = _iso_physics(np.abs(k), **ISO)
+        got = out[:, 0, : (n // 2 + 1 if half else n), 0] if axis == 2 else out[:, 0, 0, :]
+        np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), rtol=1e-9, atol=1e-9)
+        if axis == 2 and half:   # a column vector: only its first n/2+1 elements are written (iwise over (B, n/2+1))
+            assert np.all(out[:, 0, n // 2 + 1:, 0] == 0)
+    else:
+        dims = shape[1:] if nd == 3 else shape[2:]
+        want = _iso_physics(_grid_fftfreq(dims, half, centered), **ISO)
+        got = out if nd == 3 else out[:, 0]
+        np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), rtol=1e-9, atol=1e-9)
+
+
+def test_ctf_general_operator_equals_the_2d_anisotropic_restatement():
+    """The general operator with CTFAnisotropic objects is bit-identical to the separately written 2D operator, on
+    every half layout, with and without input (two restatements of the same noa code)."""
+    rng = np.random.default_rng(51)
+    b, h, w = 3, 30, 26
+    ctfs = [O.make_ctf((1.0, 1.1), (0.8 + i, 0.06 * i, 0.3 * i), 300, 0.07, 2.7, 0.1 * i, -10.0 * i) for i in range(b)]
+    x = random_spectrum((b, 1, h, w // 2 + 1), rng)
+    for remap in ("h2h", "hc2h", "h2hc", "hc2hc"):
+        for inp, dt in ((None, np.float32), (x, np.complex64), (x, np.float32)):
+            a = np.zeros((b, 1, h, w // 2 + 1), dt)
+            g = np.zeros_like(a)
+            O.ctf_anisotropic_2d(remap, inp, a, (b, 1, h, w), ctfs, ctf_abs=inp is None)
+            O.ctf(remap, inp, g, (b, 1, h, w), ctfs, isotropic=False, ctf_abs=inp is None)
+            np.testing.assert_array_equal(g, a)
+
+
+def test_ctf_isotropic_equals_the_anisotropic_without_astigmatism():
+    """CTFIsotropic == CTFAnisotropic with equal pixel sizes and no astigmatism (f64: the two compute the radius in
+    different orders, so to 1e-12)."""
+    b, h, w = 2, 36, 40
+    iso = O.make_ctf_isotropic(1.2, 1.5, 300, 0.07, 2.7, 0.2, -25.0, 1.1, f64=True)
+    ani = O.make_ctf((1.2, 1.2), (1.5, 0.0, 0.7), 300, 0.07, 2.7, 0.2, -25.0, 1.1, f64=True)
+    for remap in ("h2h", "fc2fc"):
+        wo = w // 2 + 1 if remap == "h2h" else w
+        a = np.zeros((b, 1, h, wo), np.float64)
+        g = np.zeros_like(a)
+        O.ctf(remap, None, a, (b, 1, h, w), ani, isotropic=False)
+        O.ctf(remap, None, g, (b, 1, h, w), iso)
+        np.testing.assert_allclose(g, a, rtol=1e-12, atol=1e-12)
+
+
+def test_ctf_isotropic_layouts_input_range_and_strides():
+    rng = np.random.default_rng(52)
+    b, d, h, w = 2, 12, 14, 16
+    ctfs = [O.make_ctf_isotropic(1.0, 0.8 + i, 300, 0.07, 2.7) for i in range(b)]
+    full = np.zeros((b, d, h, w), np.float32)
+    O.ctf("f2f", None, full, (b, d, h, w), ctfs)
+    # Half layout == the non-redundant columns of the full one; centred == fftshift of the non-centred.
+    half = np.zeros((b, d, h, w // 2 + 1), np.float32)
+    O.ctf("h2h", None, half, (b, d, h, w), ctfs)
+    np.testing.assert_array_equal(half, full[..., : w // 2 + 1])
+    cen = np.zeros_like(full)
+    O.ctf("fc2fc", None, cen, (b, d, h, w), ctfs)
+    np.testing.assert_array_equal(cen, np.fft.fftshift(full, axes=(1, 2, 3)))
+    # Input read through the remap; complex -> real is the power spectrum; a batch of 1 broadcasts.
+    x = random_spectrum((b, d, h, w), rng)
+    y = np.zeros_like(x)
+    O.ctf("fc2f", np.fft.fftshift(x, axes=(1, 2, 3)), y, (b, d, h, w), ctfs)
+    np.testing.assert_array_equal(y, x * full)
+    y1 = np.zeros_like(x)
+    O.ctf("f2f", x[:1], y1, (b, d, h, w), ctfs)
+    np.testing.assert_array_equal(y1, x[:1] * full)
+    p = np.zeros_like(full)
+    O.ctf("f2f", x, p, (b, d, h, w), ctfs)
+    np.testing.assert_allclose(p, np.abs(x * full) ** 2, rtol=1e-6)
+    # Batched == one call per CTF; squared and abs.
+    one = np.zeros((1, d, h, w), np.float32)
+    O.ctf("f2f", None, one, (1, d, h, w), ctfs[1])
+    np.testing.assert_array_equal(one[0], full[1])
+    sq = np.zeros_like(full)
+    O.ctf("f2f", None, sq, (b, d, h, w), ctfs, ctf_squared=True)
+    np.testing.assert_array_equal(sq, full * full)
+    # A permuted (column-major) full output holds the same values: the dimensions are reordered before the loop, so
+    # only the order of the sum of squares changes (f64 here: in f32 one ulp of |f| moves a phase of ~100 rad by
+    # ~1e-5).
+    c64 = [O.make_ctf_isotropic(1.0, 0.8 + i, 300, 0.07, 2.7, f64=True) for i in range(b)]
+    row64 = np.zeros((b, d, h, w), np.float64)
+    O.ctf("f2f", None, row64, (b, d, h, w), c64)
+    colmaj = np.zeros((w, h, d, b), np.float64).transpose(3, 2, 1, 0)
+    O.ctf("f2f", None, colmaj, (b, d, h, w), c64)
+    np.testing.assert_allclose(colmaj, row64, rtol=1e-12, atol=1e-12)
+    # A half output whose half dimension is not innermost: noa's reordered loop would leave the array.
+    bad = np.zeros((w // 2 + 1, h, d, b), np.float32).transpose(3, 2, 1, 0)
+    with pytest.raises(O.OracleError, match="leave the output"):
+        O.ctf("h2h", None, bad, (b, d, h, w), ctfs)
+    # 1D range: fftfreq = start + k * (stop - start) / (n/2) from a non-zero start (only 1D allows it).
+    c = O.make_ctf_isotropic(**ISO, f64=True)
+    line = np.zeros((1, 1, 1, 33), np.float64)
+    O.ctf("h2h", None, line, (1, 1, 1, 64), c, fftfreq_range=(0.1, 0.4, True))
+    np.testing.assert_allclose(line[0, 0, 0], _iso_physics(0.1 + np.arange(33) * 0.3 / 32, **ISO), rtol=1e-9,
+                               atol=1e-9)
+    line2 = np.zeros((1, 1, 1, 33), np.float64)
+    O.ctf("h2h", None, line2, (1, 1, 1, 64), c, fftfreq_range=(0.1, 0.4, False))
+    np.testing.assert_allclose(line2[0, 0, 0], _iso_physics(0.1 + np.arange(33) * 0.3 / 33, **ISO), rtol=1e-9,
+                               atol=1e-9)
+
+
 # ------------------------------------------------------------------------------------------------ windows > 2
 # CUBIC, CUBIC_BSPLINE and LANCZOS spectrum interpolation (Interpolation.hpp:81-141, :435-526).
 CUBIC, BSPLINE, LANCZOS4, LANCZOS6, LANCZOS8 = 2, 3, 4, 5, 6
