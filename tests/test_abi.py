@@ -208,6 +208,8 @@ def _args(lib, n=16, b=2, remap=0):
     (lambda a: setattr(a, "remap", 1), "emap"),
     (lambda a: setattr(a.rotation, "count", 3), "rotation"),
     (lambda a: setattr(a, "interp", 9), "nterp"),
+    (lambda a: (a.output_shape.__setitem__(0, 0), a.output.shape.__setitem__(0, 0)), "Empty array detected"),
+    (lambda a: a.output.strides.__setitem__(2, 1 << 31), "exceed 32-bit indexing"),
 ])
 def test_validation_errors_before_any_device_access(lib, mutate, message):
     from noa_amd import abi
