@@ -3,7 +3,7 @@
 Bars: extraction (LINEAR/NEAREST, no w-window) is bit-exact with the oracle (no transcendental functions,
 no fused multiply-add on either side). Interpolated insertion sums in the reference's slice order and differs
 only through sin/cos ulps: rel-L2 <= 1e-6. Rasterisation accumulates with float atomics in an unspecified
-order: rel-L2 <= 1e-6 against the oracle's serial order (bounded and stated in DESIGN.md).
+order: rel-L2 <= 1e-6 against the oracle's serial order (the bound on the atomic-order nondeterminism).
 """
 import numpy as np
 import pytest
