@@ -211,10 +211,12 @@ __global__ __launch_bounds__(256) void extract_kernel(const ExtractParams<C> p) 
 // work that the general kernel does for every pixel: the integer division of the pixel index (here a float
 // reciprocal, exact for slices below 2^21 pixels, checked on the host), the runtime selection between the plain and
 // the scaled/EWS transforms, the weight paths, and the runtime choice of the shift precision.
-// Copy of a c32 volume (unit x stride) into the block layout of BrickLayout: one lane per element of the stored
-// (nz, ny, wr) volume; the block padding (sizes not multiples of the block) is never read.
-__global__ __launch_bounds__(256) void brick_volume_kernel(const float2* __restrict__ in, i32 vs0, i32 vs1,
-                                                           float2* __restrict__ out, i32 ny, i32 wr, i64 total,
+// Copy of a complex volume (unit x stride; c32 or a pair of 16-bit floats) into the block layout of BrickLayout:
+// one lane per element of the stored (nz, ny, wr) volume; the block padding (sizes not multiples of the block) is
+// never read.
+template<class S>
+__global__ __launch_bounds__(256) void brick_volume_kernel(const S* __restrict__ in, i32 vs0, i32 vs1,
+                                                           S* __restrict__ out, i32 ny, i32 wr, i64 total,
                                                            BrickLayout L) {
     const i64 i = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= total) return;
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void brick_volume_kernel(const float2* __restr
     out[L.z(z) + L.y(y) + L.x(x)] = in[static_cast<i64>(z) * vs0 + static_cast<i64>(y) * vs1 + x];
 }
 
-template<int SHIFT, bool BRICK>
+template<int SHIFT, bool BRICK, class S>
 __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<float> p) {
     const u32 lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const i32 b = static_cast<i32>(lb / static_cast<u32>(p.tiles));
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<fl
         sy = p.shift_batched ? s[2 * b] : p.shift_f[0];
         sx = p.shift_batched ? s[2 * b + 1] : p.shift_f[1];
     }
-    const float2* __restrict__ vol = static_cast<const float2*>(p.vol);
+    const S* __restrict__ vol = static_cast<const S*>(p.vol);
     float2* __restrict__ out = static_cast<float2*>(p.out) + static_cast<i64>(b) * p.os_b;
 
     constexpr i32 MAXP = 4;
@@ -267,10 +269,10 @@ __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<fl
         inside[k] = live && dot3(f) <= p.cutoff_sqd;
         const float fz = f.z * p.target[0], fyv = f.y * p.target[1], fx = f.x * p.target[2];
         if constexpr (BRICK)
-            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{vol}, BrickLayout{p.vs0, p.vs1}, bd,
+            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<S>{vol}, BrickLayout{p.vs0, p.vs1}, bd,
                                                           p.vol_centered, fz, fyv, fx, inside[k]);
         else
-            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{vol}, p.vs0, p.vs1, 1, bd, p.vol_centered,
+            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<S>{vol}, p.vs0, p.vs1, 1, bd, p.vol_centered,
                                                           fz, fyv, fx, inside[k]);
     }
 #pragma unroll
@@ -360,7 +362,11 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     if (blocks > 0x7fffffffLL) { err = "too many slices for one launch"; return NOA_HIP_ERROR_UNSUPPORTED; }
 
     const dim3 grid(static_cast<unsigned>(blocks)), block(256);
-    if constexpr (std::is_same_v<C, float> && std::is_same_v<S, float2> && std::is_same_v<Out, float2>) {
+    // c32 volumes and the 16-bit pair storages (FP32 math after the load, as in the general kernel).
+    constexpr bool LEAN_TYPES = std::is_same_v<C, float> && std::is_same_v<Out, float2> && std::is_same_v<M, float2> &&
+                                (std::is_same_v<S, float2> || std::is_same_v<S, __half2> ||
+                                 std::is_same_v<S, __hip_bfloat162>);
+    if constexpr (LEAN_TYPES) {
         const bool lean = d.interp == NOA_HIP_INTERP_LINEAR && !p.general && p.window == 1 && !p.has_wvol &&
                           !p.has_wout && p.vs2 == 1 && p.pixels < (1 << 21) &&
                           (p.shift_mode < 0 || p.shift_mode == 1 || p.shift_mode == 2 || !p.shift_f64);
@@ -380,28 +386,28 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             bricks = bricks && brick_elems < 0x7fffffffLL;
             void* brick = nullptr;
             if (bricks) {
-                if (hipMallocAsync(&brick, static_cast<size_t>(brick_elems) * sizeof(float2), stream) != hipSuccess) {
+                if (hipMallocAsync(&brick, static_cast<size_t>(brick_elems) * sizeof(S), stream) != hipSuccess) {
                     (void)hipGetLastError();
                     brick = nullptr;   // not enough memory for the copy: gather from the row-major volume
                 }
             }
             if (brick) {
                 const BrickLayout L{static_cast<i32>(bz), static_cast<i32>(by)};
-                hipLaunchKernelGGL(brick_volume_kernel, dim3(static_cast<u32>((stored + 255) / 256)), dim3(256), 0,
-                                   stream, static_cast<const float2*>(p.vol), p.vs0, p.vs1, static_cast<float2*>(brick),
+                hipLaunchKernelGGL((brick_volume_kernel<S>), dim3(static_cast<u32>((stored + 255) / 256)), dim3(256), 0,
+                                   stream, static_cast<const S*>(p.vol), p.vs0, p.vs1, static_cast<S*>(brick),
                                    ny, wr, stored, L);
                 ExtractParams<C> q = p;
                 q.vol = brick;
                 q.vs0 = L.bz;
                 q.vs1 = L.by;
-                if (q.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, true>), grid, block, 0, stream, q);
-                else if (q.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, true>), grid, block, 0, stream, q);
-                else hipLaunchKernelGGL((extract_c32_kernel<0, true>), grid, block, 0, stream, q);
+                if (q.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, true, S>), grid, block, 0, stream, q);
+                else if (q.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, true, S>), grid, block, 0, stream, q);
+                else hipLaunchKernelGGL((extract_c32_kernel<0, true, S>), grid, block, 0, stream, q);
                 (void)hipFreeAsync(brick, stream);
             } else {
-                if (p.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, false>), grid, block, 0, stream, p);
-                else if (p.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, false>), grid, block, 0, stream, p);
-                else hipLaunchKernelGGL((extract_c32_kernel<0, false>), grid, block, 0, stream, p);
+                if (p.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, false, S>), grid, block, 0, stream, p);
+                else if (p.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, false, S>), grid, block, 0, stream, p);
+                else hipLaunchKernelGGL((extract_c32_kernel<0, false, S>), grid, block, 0, stream, p);
             }
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }

@@ -32,9 +32,15 @@ def _extract_gpu(vol, n, rot):
     return out.cpu().numpy()
 
 
+@pytest.mark.parametrize("layout", ["default", "bricks", "linear"])
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
-def test_half_storage_equals_fp32_math_on_the_rounded_volume(storage):
+def test_half_storage_equals_fp32_math_on_the_rounded_volume(monkeypatch, storage, layout):
+    """Bit-identical to the c32 extraction of the rounded volume, with the gather from the row-major volume or from
+    its block copy (forced either way)."""
     import noa_amd.xform as nx
+    print(f"case half storage {storage} layout={layout}", flush=True)
+    if layout != "default":
+        monkeypatch.setenv("NOA_HIP_EXTRACT_LAYOUT", layout)
     rng = np.random.default_rng(31)
     n = 48
     vol = torch.from_numpy(_volume(n, rng)).cuda()
