@@ -209,7 +209,10 @@ def _args(lib, n=16, b=2, remap=0):
     (lambda a: setattr(a.rotation, "count", 3), "rotation"),
     (lambda a: setattr(a, "interp", 9), "nterp"),
     (lambda a: (a.output_shape.__setitem__(0, 0), a.output.shape.__setitem__(0, 0)), "Empty array detected"),
-    (lambda a: a.output.strides.__setitem__(2, 1 << 31), "exceed 32-bit indexing"),
+    # (the volume placed 1 MiB below the output, so that the output's huge extent cannot overlap it; host addresses
+    # are never dereferenced by the checks)
+    (lambda a: (a.output.strides.__setitem__(2, 1 << 31), setattr(a.input, "data", a.output.data - (1 << 20))),
+     "exceed 32-bit indexing"),
 ])
 def test_validation_errors_before_any_device_access(lib, mutate, message):
     from noa_amd import abi
