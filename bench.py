@@ -269,6 +269,18 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
     log(f"virtual volume: {t * 1e3:.3f} ms, {pairs / t / 1e9:.1f} G pixel-slice pairs/s")
     del out, wout
 
+    # Isotropic CTF of a reconstructed 256^3 rfft volume (h2h): the volume multiplied by CTF(|f|) in place.
+    spec = torch.randn((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    ciso = ns.CTFIsotropic(1.0, 1.5, 300, 0.07, 2.7)
+    t = timed(lambda: ns.ctf_isotropic("h2h", spec, spec, (1, n, n, n), ciso))
+    nbytes = 2 * spec.numel() * 8
+    rep["ctf_isotropic_volume"] = {
+        "workload": f"ctf_isotropic of a {n}^3 c32 rfft volume (h2h) in place, CTFIsotropic<f32>",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
+    log(f"ctf isotropic {n}^3: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del spec
+
     # Cubic B-spline prefilter of a 256^3 f32 volume in place (before CUBIC_BSPLINE interpolation): three passes
     # (x, y, z), each reading and writing the volume once.
     vol = torch.randn((1, n, n, n), device=dev, dtype=torch.float32)
