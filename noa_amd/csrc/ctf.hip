@@ -52,12 +52,14 @@ struct CtfParams {
     bool src_centered, dst_centered, ctf_abs, ctf_squared;
 };
 
-// One lane per element, x fastest: loads and stores are contiguous along the rows.
+// Elements per lane: each block covers CTF_EPT * 256 consecutive elements (fewer, longer-lived blocks than one
+// element per lane, whose dispatch rate bounded the launch).
+constexpr u32 CTF_EPT = 4;
+
+// Element t (x fastest: loads and stores are contiguous along the rows).
 template<class T, class In, class Out, class W, bool HAS_INPUT>
-__global__ __launch_bounds__(256) void ctf_kernel(const CtfParams<T, In, Out, W> p) {
+__device__ __forceinline__ void ctf_element(const CtfParams<T, In, Out, W>& p, u32 t) {
     using RIn = typename vt<In>::real;
-    const u32 t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= p.n) return;
     const u32 row = t / p.wr, x = t - row * p.wr;
     const u32 b = row / p.h, y = row - b * p.h;
     const CtfAniso<T> c = p.ctfs ? p.ctfs[b] : p.single;
@@ -88,6 +90,16 @@ __global__ __launch_bounds__(256) void ctf_kernel(const CtfParams<T, In, Out, W>
         if constexpr (vt<Out>::cx) value.x = static_cast<typename vt<Out>::real>(ctf);
         else value = static_cast<Out>(ctf);
         *o = value;
+    }
+}
+
+template<class T, class In, class Out, class W, bool HAS_INPUT>
+__global__ __launch_bounds__(256) void ctf_kernel(const CtfParams<T, In, Out, W> p) {
+    const u32 base = blockIdx.x * (256u * CTF_EPT) + threadIdx.x;
+#pragma unroll
+    for (u32 e = 0; e < CTF_EPT; ++e) {
+        const u32 t = base + e * 256u;
+        if (t < p.n) ctf_element<T, In, Out, W, HAS_INPUT>(p, t);
     }
 }
 
@@ -126,7 +138,7 @@ int run(const CtfDesc& d, hipStream_t stream, std::string& err) {
     p.dst_centered = d.dst_centered;
     p.ctf_abs = d.ctf_abs;
     p.ctf_squared = d.ctf_squared;
-    const i64 blocks = (n + 255) / 256;
+    const i64 blocks = (n + 256 * CTF_EPT - 1) / (256 * CTF_EPT);
     if (blocks == 0) return NOA_HIP_OK;
     const dim3 grid(static_cast<u32>(blocks)), block(256);
     if (p.in) {
@@ -213,17 +225,15 @@ struct CtfNdParams {
     bool half, src_centered, dst_centered, ctf_abs, ctf_squared;
 };
 
-// One block of 256 lanes per stretch of an innermost line: the line's outer indices are decoded once per block,
-// the lanes run along the innermost dimension (contiguous for a row-major output).
+// One block of 256 lanes per stretch of CTF_EPT * 256 elements of an innermost line: the line's outer indices are
+// decoded once per block, the lanes run along the innermost dimension (contiguous for a row-major output).
 template<int N, bool HAS_INPUT, class T, class C, class In, class Out>
 __global__ __launch_bounds__(256) void ctf_nd_kernel(const CtfNdParams<T, C, In, Out> p) {
     constexpr bool ISO = !std::is_same<C, CtfAniso<T>>::value;
     using RIn = typename vt<In>::real;
     const u32 line = blockIdx.x / p.chunks;
-    const u32 x = (blockIdx.x - line * p.chunks) * 256u + threadIdx.x;
-    if (x >= p.iter[N - 1]) return;
+    const u32 x0 = (blockIdx.x - line * p.chunks) * (256u * CTF_EPT) + threadIdx.x;
     u32 idx[3];
-    idx[N - 1] = x;
     u32 r = line;
     for (int i = N - 2; i >= 0; --i) {
         idx[i] = r % p.iter[i];
@@ -231,37 +241,43 @@ __global__ __launch_bounds__(256) void ctf_nd_kernel(const CtfNdParams<T, C, In,
     }
     const u32 b = r;
     const C c = p.ctfs ? p.ctfs[b] : p.single;
-    T f[3];
-    i64 in_off = static_cast<i64>(b) * p.is[0], out_off = static_cast<i64>(b) * p.os[0];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const bool halfdim = p.half && i == N - 1;
-        const i32 j = static_cast<i32>(idx[i]);
-        const i32 freq = halfdim ? j : index2frequency(p.dst_centered, j, p.dims[i]);
-        const i32 src = halfdim ? j : frequency2index(p.src_centered, freq, p.dims[i]);   // remap_indices
-        f[i] = static_cast<T>(freq) * p.step[i];
-        if constexpr (HAS_INPUT) in_off += static_cast<i64>(src) * p.is[i + 1];
-        out_off += static_cast<i64>(j) * p.os[i + 1];
-    }
-    T ctf;
-    if constexpr (ISO) {
-        T arg;
-        if constexpr (N == 1) arg = f[0] + p.start;
-        else if constexpr (N == 2) arg = sqrt(f[0] * f[0] + f[1] * f[1]);
-        else arg = sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]);
-        ctf = ctf_value_at(c, arg);
-    } else {
-        ctf = ctf_value_at(c, f[0], f[N - 1]);
-    }
-    if (p.ctf_squared) ctf *= ctf;
-    else if (p.ctf_abs) ctf = fabs(ctf);
-    if constexpr (HAS_INPUT) {
-        p.out[out_off] = vcast<Out>(vmul(p.in[in_off], static_cast<RIn>(ctf)));
-    } else {
-        Out value = vzero<Out>();
-        if constexpr (vt<Out>::cx) value.x = static_cast<typename vt<Out>::real>(ctf);
-        else value = static_cast<Out>(ctf);
-        p.out[out_off] = value;
+    for (u32 e = 0; e < CTF_EPT; ++e) {
+        const u32 x = x0 + e * 256u;
+        if (x >= p.iter[N - 1]) return;
+        idx[N - 1] = x;
+        T f[3];
+        i64 in_off = static_cast<i64>(b) * p.is[0], out_off = static_cast<i64>(b) * p.os[0];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const bool halfdim = p.half && i == N - 1;
+            const i32 j = static_cast<i32>(idx[i]);
+            const i32 freq = halfdim ? j : index2frequency(p.dst_centered, j, p.dims[i]);
+            const i32 src = halfdim ? j : frequency2index(p.src_centered, freq, p.dims[i]);   // remap_indices
+            f[i] = static_cast<T>(freq) * p.step[i];
+            if constexpr (HAS_INPUT) in_off += static_cast<i64>(src) * p.is[i + 1];
+            out_off += static_cast<i64>(j) * p.os[i + 1];
+        }
+        T ctf;
+        if constexpr (ISO) {
+            T arg;
+            if constexpr (N == 1) arg = f[0] + p.start;
+            else if constexpr (N == 2) arg = sqrt(f[0] * f[0] + f[1] * f[1]);
+            else arg = sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]);
+            ctf = ctf_value_at(c, arg);
+        } else {
+            ctf = ctf_value_at(c, f[0], f[N - 1]);
+        }
+        if (p.ctf_squared) ctf *= ctf;
+        else if (p.ctf_abs) ctf = fabs(ctf);
+        if constexpr (HAS_INPUT) {
+            p.out[out_off] = vcast<Out>(vmul(p.in[in_off], static_cast<RIn>(ctf)));
+        } else {
+            Out value = vzero<Out>();
+            if constexpr (vt<Out>::cx) value.x = static_cast<typename vt<Out>::real>(ctf);
+            else value = static_cast<Out>(ctf);
+            p.out[out_off] = value;
+        }
     }
 }
 
@@ -286,7 +302,7 @@ int run_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err) {
         if (i < n - 1) lines *= it;
     }
     p.start = static_cast<T>(d.range_start);
-    p.chunks = (p.iter[n - 1] + 255u) / 256u;
+    p.chunks = (p.iter[n - 1] + 256u * CTF_EPT - 1) / (256u * CTF_EPT);
     const i64 blocks = lines * p.chunks;
     if (blocks == 0) return NOA_HIP_OK;
     if (blocks > 0xffffffffLL) { err = "ctf: more than 2^32 blocks in one call"; return NOA_HIP_ERROR_UNSUPPORTED; }
