@@ -40,6 +40,7 @@ class Request:
     result_weight: torch.Tensor | None = None
     admitted: float | None = None
     done: float | None = None
+    ready: torch.cuda.Event | None = None       # recorded at submit() on the stream that produced the inputs
 
     @property
     def count(self) -> int:
@@ -96,7 +97,22 @@ class Server:
         return time.perf_counter() - self.t0
 
     def submit(self, req: Request) -> None:
+        """Queues a request. Its inputs may still be in flight on the caller's current stream: an event recorded
+        there now orders every launch that reads them after their producer."""
+        req.ready = torch.cuda.Event()
+        req.ready.record(torch.cuda.current_stream(self.device))
         self.pending.append(req)
+
+    @staticmethod
+    def _inputs_on(stream: torch.cuda.Stream, reqs) -> None:
+        """Makes `stream` wait for each request's inputs, and ties their memory to `stream` so that the caching
+        allocator does not reuse it while the launch may still read it."""
+        for r in reqs:
+            if r.ready is not None:
+                stream.wait_event(r.ready)
+            for t in (r.rotations, r.slices, r.weights):
+                if t is not None and t.is_cuda:
+                    t.record_stream(stream)
 
     def idle(self) -> bool:
         return not self.pending and not self.admitted and not self.running
@@ -147,6 +163,7 @@ class Server:
     def _launch_extract(self, box: int, vid: int, reqs: list, n: int) -> None:
         wr = box // 2 + 1
         stream = self._stream()
+        self._inputs_on(stream, reqs)
         with torch.cuda.stream(stream):
             rot = torch.cat([r.rotations for r in reqs]) if len(reqs) > 1 else reqs[0].rotations
             out = torch.empty((n, 1, box, wr), dtype=torch.complex64, device=self.device)
@@ -165,6 +182,7 @@ class Server:
     def _launch_insert(self, req: Request) -> None:
         box, wr = req.box, req.box // 2 + 1
         stream = self._stream()
+        self._inputs_on(stream, [req])
         with torch.cuda.stream(stream):
             vol = torch.zeros((1, box, box, wr), dtype=torch.complex64, device=self.device)
             wvol = torch.zeros((1, box, box, wr), dtype=torch.float32, device=self.device)

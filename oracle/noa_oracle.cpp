@@ -777,14 +777,25 @@ struct InsertOp {
     int interp{};
 
     void operator()(i64 oz, i64 oy, i64 ox, bool) const {
+        In value = zero<In>();
+        W weights{};
+        if (!sum_at(oz, oy, ox, value, weights))
+            return;
+        Out& out = volume(oz, oy, ox);
+        out = add(out, cast_or_abs_squared<Out>(value));
+        if (has_wvolume) wvolume(oz, oy, ox) += weights;
+    }
+
+    // The sums over the slices of one voxel (:478-515); false (and nothing summed) outside the cutoff.
+    bool sum_at(i64 oz, i64 oy, i64 ox, In& value, W& weights) const {
         V3<Coord> f;
         f[0] = static_cast<Coord>(index2frequency(volume_centered, oz, grid[0])) / f_target[0];
         f[1] = static_cast<Coord>(index2frequency(volume_centered, oy, grid[1])) / f_target[1];
         f[2] = static_cast<Coord>(ox) / f_target[2];
         if (dot3(f, f) > cutoff_sqd)
-            return;
-        In value = zero<In>();
-        W weights{};
+            return false;
+        value = zero<In>();
+        weights = W{};
         for (i64 i = 0; i < n_slices; ++i) {
             Coord fz;
             V2<Coord> f2;
@@ -809,9 +820,7 @@ struct InsertOp {
             value = add(value, i_value);
             if (has_wvolume) weights += i_weights;
         }
-        Out& out = volume(oz, oy, ox);
-        out = add(out, cast_or_abs_squared<Out>(value));
-        if (has_wvolume) wvolume(oz, oy, ox) += weights;
+        return true;
     }
 
     template<class T, class R> static T mul(T a, R s) { if constexpr (is_cx<T>::value) return a * s; else return a * s; }
@@ -1007,7 +1016,7 @@ int run_rasterize(const noa_hip_project_args& a, int n_threads, bool force) {
 }
 
 template<class Coord, class In, class Out, class W>
-int run_insert(const noa_hip_project_args& a, int n_threads, bool force) {
+InsertOp<Coord, In, Out, W> make_insert_op(const noa_hip_project_args& a) {
     InsertOp<Coord, In, Out, W> op;
     op.slices = reader_interp2<In>(a.input);
     op.has_wslices = a.input_weight.data != nullptr || a.input_weight.is_value;
@@ -1032,7 +1041,32 @@ int run_insert(const noa_hip_project_args& a, int n_threads, bool force) {
     op.slices_centered = remap_src_centered(a.remap);
     op.volume_centered = remap_dst_centered(a.remap);
     op.interp = a.interp;
+    return op;
+}
+
+template<class Coord, class In, class Out, class W>
+int run_insert(const noa_hip_project_args& a, int n_threads, bool force) {
+    const auto op = make_insert_op<Coord, In, Out, W>(a);
     iwise3(a.output_shape[1], a.output_shape[2], a.output_shape[3] / 2 + 1, op, n_threads, force);
+    return 0;
+}
+
+// The insertion's per-voxel sums (value and weight, before they are added to the volume) at a list of voxels
+// (z, y, x) of the stored half-volume, in parallel over the list. A voxel outside the cutoff gets 0 and flag 0.
+template<class Coord, class In, class Out, class W>
+int run_insert_voxels(const noa_hip_project_args& a, const long long* zyx, long long n, void* values,
+                      void* weights, signed char* inside, int n_threads) {
+    const auto op = make_insert_op<Coord, In, Out, W>(a);
+    In* vo = static_cast<In*>(values);
+    W* wo = static_cast<W*>(weights);
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 16)
+    for (long long i = 0; i < n; ++i) {
+        In v = zero<In>();
+        W w{};
+        inside[i] = op.sum_at(zyx[3 * i], zyx[3 * i + 1], zyx[3 * i + 2], v, w) ? 1 : 0;
+        vo[i] = v;
+        wo[i] = w;
+    }
     return 0;
 }
 
@@ -1682,6 +1716,25 @@ int oracle_rasterize_central_slices_3d(const noa_hip_project_args* a, int n_thre
 }
 int oracle_insert_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
     return oracle::dispatch(oracle::INSERT, a, n_threads, force != 0);
+}
+// Per-voxel sums of the interpolating insertion at a list of (z, y, x) voxels (values: the input's value type,
+// weights: its real type); `inside` receives 1 for voxels inside the cutoff. Single-precision or double rotations.
+int oracle_insert_central_slices_3d_voxels(const noa_hip_project_args* a, const long long* zyx, long long n,
+                                           void* values, void* weights, signed char* inside, int n_threads) {
+    using namespace oracle;
+    g_error.clear();
+    const bool f64 = a->rotation.kind == NOA_HIP_MAT33_F64 || a->rotation.kind == NOA_HIP_QUAT_F64;
+    const int in = a->input.dtype;
+    auto go = [&](auto coord) -> int {
+        using Coord = decltype(coord);
+        if (in == NOA_HIP_C32) return run_insert_voxels<Coord, Cx<float>, Cx<float>, float>(*a, zyx, n, values, weights, inside, n_threads);
+        if (in == NOA_HIP_F32) return run_insert_voxels<Coord, float, float, float>(*a, zyx, n, values, weights, inside, n_threads);
+        if (in == NOA_HIP_C64) return run_insert_voxels<Coord, Cx<double>, Cx<double>, double>(*a, zyx, n, values, weights, inside, n_threads);
+        if (in == NOA_HIP_F64) return run_insert_voxels<Coord, double, double, double>(*a, zyx, n, values, weights, inside, n_threads);
+        g_error = "oracle: unsupported value type";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    };
+    return f64 ? go(double{}) : go(float{});
 }
 int oracle_extract_central_slices_3d(const noa_hip_project_args* a, int n_threads, int force) {
     return oracle::dispatch(oracle::EXTRACT, a, n_threads, force != 0);

@@ -215,6 +215,38 @@ def insert_central_slices_3d(remap, slice, slice_weight, slice_shape, volume, vo
     _run("oracle_insert_central_slices_3d", a, n_threads, force_threads)
 
 
+def insert_central_slices_3d_voxels(remap, slice, slice_weight, slice_shape, volume_shape, inv_rotation, voxels, *,
+                                    fwd_scaling=None, interp=1, fftfreq_sinc=-1.0, fftfreq_blackman=-1.0,
+                                    fftfreq_cutoff=0.5, target_shape=(0, 0, 0, 0), ews_radius=(0.0, 0.0),
+                                    quaternion=False, n_threads=1):
+    """noa's per-voxel insertion sums (FourierProject.hpp:478-515) at the (z, y, x) voxels of `voxels` ((n, 3)
+    integers), before they are added to a volume: returns (values, weights, inside), `inside` False (and both sums
+    0) for voxels beyond the cutoff, which noa leaves untouched. Runs in parallel over the voxels."""
+    vox = np.ascontiguousarray(np.asarray(voxels, dtype=np.int64).reshape(-1, 3))
+    n = vox.shape[0]
+    sl = np.asarray(slice)
+    vdt = sl.dtype
+    rdt = np.float64 if vdt in (np.float64, np.complex128) else np.float32
+    dummy_v = np.zeros((1, 1, 1, 1), vdt)
+    dummy_w = np.zeros((1, 1, 1, 1), rdt)
+    keep = []
+    a = _args(remap, slice, slice_weight, slice_shape, dummy_v, dummy_w, volume_shape, inv_rotation, fwd_scaling,
+              fftfreq_cutoff, target_shape, ews_radius, fftfreq_sinc, fftfreq_blackman, interp, keep, quaternion)
+    values = np.zeros(n, vdt)
+    weights = np.zeros(n, rdt)
+    inside = np.zeros(n, np.int8)
+    L = lib()
+    fn = L.oracle_insert_central_slices_3d_voxels
+    fn.argtypes = [ctypes.POINTER(ProjectArgs), ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    status = fn(ctypes.byref(a), vox.ctypes.data, n, values.ctypes.data, weights.ctypes.data, inside.ctypes.data,
+                n_threads)
+    if status != 0:
+        raise OracleError(L.oracle_last_error().decode())
+    return values, weights, inside.astype(bool)
+
+
 def extract_central_slices_3d(remap, volume, volume_weight, volume_shape, slice, slice_weight, slice_shape,
                               inv_scaling=None, fwd_rotation=None, *, interp=1, fftfreq_sinc=-1.0,
                               fftfreq_blackman=-1.0, fftfreq_cutoff=0.5, target_shape=(0, 0, 0, 0),

@@ -56,6 +56,42 @@ def test_results_are_the_same_alone_and_in_the_mix():
             assert torch.equal(r.result_weight, alone.result_weight)
 
 
+def test_inputs_produced_just_before_submit_are_waited_for():
+    """Requests whose inputs are still being written on the caller's stream when they are submitted (no
+    synchronisation in between) give the same results as the same requests run alone on settled inputs."""
+    from noa_amd import workloads
+    from noa_amd.serve import Server, run_alone
+    dev = torch.device("cuda")
+    vols = {b: v[:2] for b, v in workloads.serving_volumes(dev, per_box=2).items() if b in (64, 128, 192)}
+    reqs = _mix(dev, n=12, seed=8)
+    torch.cuda.synchronize()
+    want = [run_alone(Server(vols, budget_bytes=256 << 20, n_streams=1), _clone(r)) for r in reqs]
+    # Overwrite every input with a long chain of work on the current stream, ending in the same values, and submit
+    # right away: a launch that does not wait would read the intermediate values.
+    server = Server(vols, budget_bytes=256 << 20, n_streams=3)
+    fresh = []
+    for r in reqs:
+        c = _clone(r)
+        rot = c.rotations.clone()
+        c.rotations = torch.full_like(rot, float("nan"))
+        for _ in range(20):
+            c.rotations.mul_(1.0)
+        c.rotations.copy_(rot)
+        if c.kind == "insert":
+            sl, w = c.slices.clone(), c.weights.clone()
+            c.slices = torch.zeros_like(sl)
+            c.weights = torch.zeros_like(w)
+            c.slices.copy_(sl)
+            c.weights.copy_(w)
+        server.submit(c)
+        fresh.append(c)
+    while not server.idle():
+        server.step()
+    torch.cuda.synchronize()
+    for got, ref in zip(fresh, want):
+        assert torch.equal(got.result, ref.result), f"request {got.rid} ({got.kind}) read unsettled inputs"
+
+
 def test_budget_is_respected_and_large_requests_still_run():
     from noa_amd import workloads
     from noa_amd.serve import Server, run_open_loop
