@@ -8,23 +8,21 @@
 // through the voxel contribute (about 0.6 % of the voxel-slice pairs at 256^3 with the default 1-voxel window);
 // the others add exactly zero. This file keeps noa's per-voxel expressions and skips the zero terms.
 //
-// Staged path (no scaling/EWS, array inputs, single-precision values):
-//   - Work item = (16^3 brick of the volume, range of slices). A workgroup owns the brick's 4096 voxels in an LDS
-//     accumulator: no atomics.
-//   - Each wave scans the item's slices 64 at a time: lane l tests whether the slab of slice s0+l intersects the
-//     brick box; a ballot yields the passing slices, processed in increasing slice order. The rotations of the
-//     next 64 slices are loaded while the current ones are consumed.
-//   - For a passing slice, the brick's footprint on the slice (the projection of the brick box, plus its Hermitian
-//     mirror when it crosses x=0) is copied to LDS with coalesced row loads, zero outside the slice's frequency
-//     bounds. The footprint of the next passing slice is loaded into registers while the current one is used.
-//   - The brick voxels near the slab are enumerated along the volume axis closest to the slab normal: each lane of
-//     a wave counts the candidates of one column of the 16x16 face, a wave-wide prefix sum packs the candidates of
-//     the wave's columns onto its 64 lanes, and each candidate runs noa's exact per-voxel tests; a hit interpolates
-//     the slice from LDS and accumulates in LDS (one owner per voxel per slice; barriers order the slices).
-//   - Bricks near the origin meet most slices; their slice range is cut into segments so that work items are
-//     balanced. A brick with one segment adds its sum to the volume directly; a split brick writes one partial
-//     sum per segment, and a second kernel adds the partials of each brick in segment order. The result is
-//     deterministic; it differs from noa's strict slice order only by this regrouping of the sum.
+// Culled paths (array inputs, single-precision values, no scaling/EWS):
+//   - Work item = (8x8x16 sub-brick of the volume, range of slices), one wave per item, the sub-brick's
+//     accumulator in LDS: no global atomics.
+//   - The wave scans the item's slices 64 at a time: lane l tests whether the slab of slice s0+l meets the
+//     sub-brick box; a ballot yields the passing slices, processed in increasing slice order.
+//   - For a passing slice, the voxels near the slab are enumerated along the volume axis closest to the slab normal
+//     and packed onto the wave's lanes; each candidate runs noa's exact per-voxel tests and accumulates in LDS. Each
+//     voxel is met at most once per slice.
+//   - Sub-bricks near the origin meet most slices; their slice range is cut into segments so that work items are
+//     balanced. A sub-brick with one segment adds its sum to the volume directly; a split sub-brick writes one
+//     partial sum per segment, and a second kernel adds the partials of each sub-brick in segment order. The
+//     result is deterministic; it differs from noa's strict slice order only by this regrouping of the sum.
+//   - insert_slab_kernel serves the common configuration (one power-of-two size, Mat33<f32>, LINEAR, cutoff
+//     <= 0.5) with per-slice records and a leaner per-voxel path; insert_wave_kernel serves the others
+//     (quaternions, NEAREST, other sizes and cutoffs).
 // General path (scaling or EWS, constant-value inputs, double precision): noa's per-voxel gather over every
 // slice, one lane per voxel.
 #include <type_traits>
@@ -42,11 +40,6 @@
 
 namespace noa_hip {
 
-constexpr i32 BE = 16;                 // brick edge (voxels)
-constexpr i32 AP = 17;                 // LDS accumulator pitch along x (bank-conflict padding)
-constexpr i32 ACC = BE * BE * AP;      // accumulator slots
-constexpr i32 RS = 32;                 // staged region capacity along y and x (pixels), LDS row pitch
-constexpr i32 NCOL = BE * BE;          // columns of a brick face
 constexpr i32 NWIN = 12;               // degree of the window polynomial
 constexpr i32 NT_REDUCE = 256;         // threads per workgroup of the partial-sum reduction
 
@@ -111,8 +104,6 @@ __device__ __forceinline__ void rot_matrix(const Rot<C>& r, C m[9]) {
     m[6] = txz + twy; m[7] = txy - twz; m[8] = 1 - (tyy + tzz);
 }
 
-__device__ __forceinline__ i32 acc_index(i32 lz, i32 ly, i32 lx) { return (lz * BE + ly) * AP + lx; }
-
 // Windowed sinc of the staged kernel in single precision: a polynomial of degree NWIN in t = 2 (fz/f_b)^2 - 1
 // (the window is even in fz), fitted on the host per (f_s, f_b) and accepted only when its error, evaluated
 // in single precision, stays below 4e-7 on [-f_b, f_b] (noa's FP32 expression itself carries ~2e-7 of rounding).
@@ -122,63 +113,6 @@ __device__ __forceinline__ float window_poly(float fz, float inv_fb, const float
     float r = a[NWIN];
 #pragma unroll
     for (i32 k = NWIN - 1; k >= 0; --k) r = fmaf(r, t, a[k]);
-    return r;
-}
-
-// Footprint of a brick box on one slice. Region A covers px >= 0; region B the mirrored part px < 0 (its x0 is
-// 0). Each region is at most RS x RS pixels when `staged`.
-struct Footprint {
-    i32 ay0, ax0, ah, aw;
-    i32 by0, bh, bw;
-    bool use_a, use_b, staged;
-};
-
-template<class C>
-__device__ __forceinline__ Footprint footprint(const C m[9], const C cen[3], const C half[3], C hs, C ws) {
-    const C pyc = (m[3] * cen[0] + m[4] * cen[1] + m[5] * cen[2]) * hs;
-    const C pxc = (m[6] * cen[0] + m[7] * cen[1] + m[8] * cen[2]) * ws;
-    const C hy = (fabs(m[3]) * half[0] + fabs(m[4]) * half[1] + fabs(m[5]) * half[2]) * hs;
-    const C hx = (fabs(m[6]) * half[0] + fabs(m[7]) * half[1] + fabs(m[8]) * half[2]) * ws;
-    // One extra pixel on each side absorbs the rounding difference with the exact per-voxel positions.
-    Footprint fp;
-    fp.ay0 = static_cast<i32>(floor(pyc - hy)) - 1;
-    fp.ah = static_cast<i32>(floor(pyc + hy)) + 3 - fp.ay0;
-    fp.ax0 = max(0, static_cast<i32>(floor(pxc - hx)) - 1);
-    fp.aw = static_cast<i32>(floor(pxc + hx)) + 3 - fp.ax0;
-    fp.by0 = static_cast<i32>(floor(-(pyc + hy))) - 1;
-    fp.bh = static_cast<i32>(floor(-(pyc - hy))) + 3 - fp.by0;
-    fp.bw = max(1, static_cast<i32>(floor(-(pxc - hx))) + 3);
-    fp.use_a = pxc + hx >= C(-0.5);
-    fp.use_b = pxc - hx < C(0.5);
-    fp.staged = (!fp.use_a || (fp.ah <= RS && fp.aw <= RS)) && (!fp.use_b || (fp.bh <= RS && fp.bw <= RS));
-    return fp;
-}
-
-// What a (brick, slice) pair needs besides the rotation: the footprint, and the enumeration along the volume axis A
-// closest to the slab normal (al = normal in index units; B and Cc: the face axes, B < Cc).
-template<class C>
-struct PairInfo {
-    Footprint fp;
-    i32 flags;            // 1: use_a, 2: use_b, 4: staged, bits 3-4: axis A, 32: candidate range from the slab
-    C inv_alA, alB, alC;
-};
-
-template<class C>
-__device__ __forceinline__ PairInfo<C> pair_info(const C m[9], const C cen[3], const C half[3], C hs, C ws, C it0,
-                                                 C it1, C it2, const bool contiguous[3]) {
-    PairInfo<C> r;
-    r.fp = footprint(m, cen, half, hs, ws);
-    const C al0 = m[0] * it0, al1 = m[1] * it1, al2 = m[2] * it2;
-    i32 ax = 0;
-    if (fabs(al1) > fabs(al0)) ax = 1;
-    if (fabs(al2) > fabs(ax == 0 ? al0 : al1)) ax = 2;
-    const C alA = ax == 0 ? al0 : (ax == 1 ? al1 : al2);
-    r.alB = ax == 0 ? al1 : al0;
-    r.alC = ax == 2 ? al1 : al2;
-    const bool contig = ax == 0 ? contiguous[0] : (ax == 1 ? contiguous[1] : contiguous[2]);
-    const bool ranged = contig && fabs(alA) > C(1e-20);
-    r.inv_alA = ranged ? C(1) / alA : C(0);
-    r.flags = (r.fp.use_a ? 1 : 0) | (r.fp.use_b ? 2 : 0) | (r.fp.staged ? 4 : 0) | (ax << 3) | (ranged ? 32 : 0);
     return r;
 }
 
@@ -195,382 +129,6 @@ __device__ __forceinline__ i32 wave_inclusive_sum(i32 x) {
     return x;
 }
 
-
-template<class C, class In, class Out, class W, int INTERP, int NT>
-__global__ __launch_bounds__(NT) void insert_staged_kernel(const InsertParams<C, In, W> p) {
-    using RIn = typename vt<In>::real;
-    constexpr i32 SLOTS = RS * RS / NT;            // staged elements per thread and region
-    constexpr i32 NWAVE = NT / 64;
-    constexpr i32 CPW = NCOL / NWAVE;              // face columns per wave
-    static_assert(CPW <= 64 && NCOL % NWAVE == 0 && (RS * RS) % NT == 0, "launch shape");
-    __shared__ In acc_v[ACC];
-    __shared__ W acc_w[ACC];
-    __shared__ In st_v[2][RS * RS];
-    __shared__ W st_w[2][RS * RS];
-
-    const i32 tid = static_cast<i32>(threadIdx.x);
-    const i32 lane = tid & 63;
-    const i32 wave = tid >> 6;
-    // The plan is built for a power-of-two slice count: clamp the item's slice range to the actual count (an
-    // empty range still flushes, so a split brick's partial slot is always written).
-    int4 item = p.items[blockIdx.x];
-    item.z = min(item.z, p.n_slices);
-    item.y = min(item.y, item.z);
-    const i32 brick = item.x;
-    const i32 bxi = brick % p.nb[2];
-    const i32 rem = brick / p.nb[2];
-    const i32 byi = rem % p.nb[1];
-    const i32 bzi = rem / p.nb[1];
-    const i32 org[3] = {bzi * BE, byi * BE, bxi * BE};
-    const i32 grid[3] = {p.grid[0], p.grid[1], p.grid[2]};
-    const i32 ext[3] = {p.grid[0], p.grid[1], p.wr};
-    const bool vc = p.vol_centered;
-
-    for (i32 v = tid; v < ACC; v += NT) {
-        acc_v[v] = vzero<In>();
-        acc_w[v] = W(0);
-    }
-
-    // Brick box in normalised frequencies, and whether each axis is free of the non-centred wrap (then the
-    // frequencies of the brick along it are first-frequency + k).
-    C cen[3], half[3];
-    i32 f0[3], n_in[3];
-    bool contiguous[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        i32 lo = 0x7fffffff, hi = -0x7fffffff, n = 0;
-        bool contig = true;
-        const i32 first = axis_freq(a == 2, org[a], vc, grid[a]);
-        for (i32 k = 0; k < BE; ++k) {
-            const i32 idx = org[a] + k;
-            if (idx >= ext[a]) break;
-            const i32 f = axis_freq(a == 2, idx, vc, grid[a]);
-            contig = contig && (f == first + k);
-            lo = min(lo, f);
-            hi = max(hi, f);
-            ++n;
-        }
-        f0[a] = first;
-        n_in[a] = n;
-        contiguous[a] = contig;
-        cen[a] = (static_cast<C>(lo) + static_cast<C>(hi)) * C(0.5) / p.target[a];
-        half[a] = (static_cast<C>(hi) - static_cast<C>(lo)) * C(0.5) / p.target[a];
-    }
-    const C slack = C(1e-5);
-    const i32 Hs = p.slice_shape[0], Ws = p.slice_shape[1];
-    const i32 lo_y = -Hs / 2, hi_y = (Hs - 1) / 2, hi_x = Ws / 2;
-
-    // Values of the hot loop, held in vector registers.
-    const C fb = pin(p.fftfreq_blackman);
-    const C fs = pin(p.fftfreq_sinc);
-    const C cutoff_sqd = pin(p.cutoff_sqd);
-    const C hs_f = pin(p.slice_f[0]), ws_f = pin(p.slice_f[1]);
-    const C t0 = pin(p.target[0]), t1 = pin(p.target[1]), t2 = pin(p.target[2]);
-    const C it0 = pin(p.inv_target[0]), it1 = pin(p.inv_target[1]), it2 = pin(p.inv_target[2]);
-    const bool pow2 = p.pow2;
-    const bool wpoly = p.wpoly;
-    const bool has_wvol = p.has_wvol;
-    const bool win_present = p.win_present;
-    float wa[NWIN + 1];
-#pragma unroll
-    for (int k = 0; k <= NWIN; ++k) wa[k] = pin(p.wa[k]);
-    const float inv_fb = pin(p.inv_fb);
-
-    // Iterator over the passing slices of the item, in increasing order. Every wave scans the same 64 slices:
-    // lane l holds slice chunk0+l, tests its slab against the brick box and prepares what the pair (brick, slice)
-    // needs (footprint, enumeration axis); a passing slice's values are then broadcast from that lane. The next
-    // chunk's rotations are loaded one chunk ahead.
-    const bool quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
-    auto scan_load = [&](i32 s0, Rot<C>& r) {
-        const i32 s = s0 + lane;
-        if (s < item.z) r = load_rotation(p.rot, s);
-    };
-    i32 chunk0 = item.y;
-    Rot<C> chunk{}, ahead{};
-    chunk.quat = quat;
-    ahead.quat = quat;
-    PairInfo<C> mine{};       // this lane's slice of the current chunk
-    unsigned long long mask = 0;
-    auto scan_prepare = [&]() {
-        bool pass = false;
-        if (chunk0 + lane < item.z) {
-            C m[9];
-            rot_matrix(chunk, m);
-            const C d = fabs(m[0] * cen[0] + m[1] * cen[1] + m[2] * cen[2]);
-            const C bound = fb + fabs(m[0]) * half[0] + fabs(m[1]) * half[1] + fabs(m[2]) * half[2];
-            pass = d <= bound + slack;
-            mine = pair_info(m, cen, half, hs_f, ws_f, it0, it1, it2, contiguous);
-        }
-        mask = __ballot(pass);
-    };
-    scan_load(chunk0, chunk);
-    scan_load(chunk0 + 64, ahead);
-    scan_prepare();
-    auto next_slice = [&](Rot<C>& rot_out, PairInfo<C>& out) -> i32 {
-        while (mask == 0) {
-            chunk0 += 64;
-            if (chunk0 >= item.z) return -1;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) chunk.m[q] = ahead.m[q];
-            scan_load(chunk0 + 64, ahead);
-            scan_prepare();
-        }
-        const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
-        mask &= mask - 1;
-        rot_out.quat = quat;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) rot_out.m[q] = bcast(chunk.m[q], k);
-        out.fp.ay0 = __builtin_amdgcn_readlane(mine.fp.ay0, k);
-        out.fp.ax0 = __builtin_amdgcn_readlane(mine.fp.ax0, k);
-        out.fp.ah = __builtin_amdgcn_readlane(mine.fp.ah, k);
-        out.fp.aw = __builtin_amdgcn_readlane(mine.fp.aw, k);
-        out.fp.by0 = __builtin_amdgcn_readlane(mine.fp.by0, k);
-        out.fp.bh = __builtin_amdgcn_readlane(mine.fp.bh, k);
-        out.fp.bw = __builtin_amdgcn_readlane(mine.fp.bw, k);
-        const i32 flags = __builtin_amdgcn_readlane(mine.flags, k);
-        out.flags = flags;
-        out.fp.use_a = flags & 1;
-        out.fp.use_b = flags & 2;
-        out.fp.staged = flags & 4;
-        out.inv_alA = bcast(mine.inv_alA, k);
-        out.alB = bcast(mine.alB, k);
-        out.alC = bcast(mine.alC, k);
-        return chunk0 + k;
-    };
-
-    // Loads the staged elements of slice i into registers (issued ahead of their use). Thread t owns column
-    // t % RS of rows t / RS + (NT / RS) j of each region.
-    In rv[2][SLOTS];
-    W rw[2][SLOTS];
-    const i32 my_col = tid % RS, my_row = tid / RS;
-    auto load_region = [&](const In* __restrict__ sl, const W* __restrict__ wsl, i32 y0, i32 x0, i32 h, i32 w,
-                           In (&v)[SLOTS], W (&wv)[SLOTS]) {
-        const i32 xf = x0 + my_col;
-        const bool col_ok = my_col < w && xf <= hi_x;
-#pragma unroll
-        for (int j = 0; j < SLOTS; ++j) {
-            const i32 row = my_row + j * (NT / RS);
-            const i32 yf = y0 + row;
-            v[j] = vzero<In>();
-            wv[j] = W(0);
-            if (col_ok && row < h && yf >= lo_y && yf <= hi_y) {
-                const i32 off = frequency2index(p.slices_centered, yf, Hs);
-                v[j] = sl[off * p.is_y + xf * p.is_x];
-                if (wsl) wv[j] = wsl[off * p.wis_y + xf * p.wis_x];
-            }
-        }
-    };
-    auto load_footprint = [&](i32 i, const Footprint& f) {
-        if (!f.staged) return;
-        const In* __restrict__ sl = p.in + static_cast<i64>(i) * p.is_b;
-        const W* __restrict__ wsl = win_present ? p.win + static_cast<i64>(i) * p.wis_b : nullptr;
-        if (f.use_a) load_region(sl, wsl, f.ay0, f.ax0, f.ah, f.aw, rv[0], rw[0]);
-        if (f.use_b) load_region(sl, wsl, f.by0, 0, f.bh, f.bw, rv[1], rw[1]);
-    };
-
-    __shared__ i32 cand[NWAVE][64];    // packed candidates of one pass of each wave: column-in-wave << 4 | k
-    i32* const my_cand = cand[wave];
-
-    Rot<C> rot{};
-    PairInfo<C> pi{};
-    i32 cur = next_slice(rot, pi);
-    if (cur >= 0) load_footprint(cur, pi.fp);
-    while (cur >= 0) {
-        const Footprint fp = pi.fp;
-        // Publish the staged footprint of the current slice.
-        if (fp.staged) {
-#pragma unroll
-            for (int j = 0; j < SLOTS; ++j) {
-                if (fp.use_a) {
-                    st_v[0][tid + j * NT] = rv[0][j];
-                    st_w[0][tid + j * NT] = rw[0][j];
-                }
-                if (fp.use_b) {
-                    st_v[1][tid + j * NT] = rv[1][j];
-                    st_w[1][tid + j * NT] = rw[1][j];
-                }
-            }
-        }
-        __syncthreads();
-
-        // Prefetch the next passing slice while the current one is processed.
-        Rot<C> rot_n{};
-        PairInfo<C> pi_n{};
-        const i32 nxt = next_slice(rot_n, pi_n);
-        if (nxt >= 0) load_footprint(nxt, pi_n.fp);
-
-        // Enumeration axis A: the volume axis closest to the slab normal (in index units). B and Cc span the face.
-        const i32 ax = (pi.flags >> 3) & 3;
-        const i32 orgB = ax == 0 ? org[1] : org[0];
-        const i32 orgC = ax == 2 ? org[1] : org[2];
-        const i32 gridB = ax == 0 ? grid[1] : grid[0];
-        const i32 gridC = grid[1];                                  // unused when Cc is x
-        const bool xC = ax != 2;
-        const i32 extB = ax == 0 ? ext[1] : ext[0];
-        const i32 extC = ax == 2 ? ext[1] : ext[2];
-        const i32 f0A = ax == 0 ? f0[0] : (ax == 1 ? f0[1] : f0[2]);
-        const i32 nA = ax == 0 ? n_in[0] : (ax == 1 ? n_in[1] : n_in[2]);
-        const C tA = ax == 0 ? t0 : (ax == 1 ? t1 : t2);
-        const bool ranged = pi.flags & 32;
-
-        // Candidates of this lane's column: |alA * f + beta| <= f_b in exact arithmetic, widened by eps samples to
-        // cover the rounding of the exact per-voxel test (a few ulps of |f| <= 1, divided by |alA| >= 1/(sqrt(3) T)).
-        i32 cnt = 0, k_lo = 0;
-        if (lane < CPW) {
-            const i32 col = wave * CPW + lane;
-            const i32 gb = orgB + col / BE, gc = orgC + col % BE;
-            if (gb < extB && gc < extC) {
-                i32 k_hi = nA - 1;
-                if (ranged) {
-                    const i32 fbq = index2frequency(vc, gb, gridB);
-                    const i32 fcq = axis_freq(xC, gc, vc, gridC);
-                    const C beta = pi.alB * static_cast<C>(fbq) + pi.alC * static_cast<C>(fcq);
-                    const C eps = C(1e-3) + C(8e-6) * tA;
-                    C lo = (-fb - beta) * pi.inv_alA;
-                    C hi = (fb - beta) * pi.inv_alA;
-                    if (lo > hi) { const C t = lo; lo = hi; hi = t; }
-                    const C klo = ceil(lo - eps) - static_cast<C>(f0A);
-                    const C khi = floor(hi + eps) - static_cast<C>(f0A);
-                    k_lo = static_cast<i32>(fmin(fmax(klo, C(0)), static_cast<C>(nA)));
-                    k_hi = static_cast<i32>(fmax(fmin(khi, static_cast<C>(nA - 1)), C(-1)));
-                }
-                cnt = max(0, k_hi - k_lo + 1);
-            }
-        }
-        const i32 incl = wave_inclusive_sum(cnt);
-        const i32 total = __builtin_amdgcn_readlane(incl, 63);
-        const i32 excl = incl - cnt;
-
-        const In* __restrict__ sl = p.in + static_cast<i64>(cur) * p.is_b;
-        const W* __restrict__ wsl = win_present ? p.win + static_cast<i64>(cur) * p.wis_b : nullptr;
-        for (i32 base = 0; base < total; base += 64) {
-            // Pack the candidates [base, base + 64) of the wave into the list, one per lane.
-            __builtin_amdgcn_wave_barrier();
-            {
-                const i32 i0 = max(0, base - excl), i1 = min(cnt, base + 64 - excl);
-                for (i32 i = i0; i < i1; ++i) my_cand[excl + i - base] = (lane << 4) | (k_lo + i);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const i32 e = my_cand[lane];
-            if (base + lane >= total) continue;
-            const i32 k = e & 15;
-            const i32 col = wave * CPW + ((e >> 4) & 63);
-            const i32 lb = col / BE, lc = col % BE;
-            const i32 lz = ax == 0 ? k : lb;
-            const i32 ly = ax == 0 ? lb : (ax == 1 ? k : lc);
-            const i32 lx = ax == 2 ? k : lc;
-            const i32 frz = index2frequency(vc, org[0] + lz, grid[0]);
-            const i32 fry = index2frequency(vc, org[1] + ly, grid[1]);
-            const i32 frx = org[2] + lx;
-            // The normalised frequency as noa computes it: index2frequency / target (:479-484).
-            vec3<C> f;
-            if (pow2) {
-                f.z = static_cast<C>(frz) * it0; f.y = static_cast<C>(fry) * it1; f.x = static_cast<C>(frx) * it2;
-            } else {
-                f.z = static_cast<C>(frz) / t0; f.y = static_cast<C>(fry) / t1; f.x = static_cast<C>(frx) / t2;
-            }
-            // The exact tests of noa (:479-494).
-            if (dot3(f) > cutoff_sqd) continue;
-            Scale<C> sc;
-            sc.identity = true;
-            vec2<C> f2;
-            const C fz = grid2slice<false>(f, sc, rot, C(0), C(0), f2);
-            if (!(fabs(fz) <= fb)) continue;
-            C window;
-            if constexpr (sizeof(C) == 4) {
-                window = wpoly ? window_poly(fz, inv_fb, wa) : windowed_sinc<C>(fz, fs, fb);
-            } else {
-                window = windowed_sinc<C>(fz, fs, fb);
-            }
-            C py = f2.y * hs_f, px = f2.x * ws_f;
-            In v;
-            W wv = W(0);
-            if (fp.staged) {
-                RIn conj = 1;
-                const bool mirror = px < 0;
-                if (mirror) { py *= -1; px *= -1; conj = -1; }
-                const i32 gy0 = mirror ? fp.by0 : fp.ay0;
-                const i32 gx0 = mirror ? 0 : fp.ax0;
-                const In* __restrict__ sv = mirror ? st_v[1] : st_v[0];
-                const W* __restrict__ sw = mirror ? st_w[1] : st_w[0];
-                if constexpr (INTERP == NOA_HIP_INTERP_NEAREST) {
-                    const i32 yy = static_cast<i32>(round(py)) - gy0, xx = static_cast<i32>(round(px)) - gx0;
-                    const i32 o = min(max(yy * RS + xx, 0), RS * RS - 1);
-                    v = vadd(vzero<In>(), vmul(sv[o], RIn(1)));
-                    wv = W(0) + sw[o] * W(1);
-                    (void)conj;   // NEAREST does not conjugate after the flip (Interpolation.hpp:479-484)
-                } else {
-                    const C fly = floor(py), flx = floor(px);
-                    const i32 y0 = static_cast<i32>(fly) - gy0, x0 = static_cast<i32>(flx) - gx0;
-                    const RIn ty = static_cast<RIn>(py - fly), tx = static_cast<RIn>(px - flx);
-                    const RIn wy[2] = {1 - ty, ty}, wx[2] = {1 - tx, tx};
-                    // The margins of the footprint keep every tap inside the region; the clamp only guards the
-                    // LDS index.
-                    const i32 o0 = min(max(y0 * RS + x0, 0), RS * RS - RS - 2);
-                    v = vzero<In>();
-#pragma unroll
-                    for (int yy = 0; yy < 2; ++yy) {
-                        In vy = vzero<In>();
-                        W wy_ = W(0);
-#pragma unroll
-                        for (int xx = 0; xx < 2; ++xx) {
-                            const i32 o = o0 + yy * RS + xx;
-                            vy = vadd(vy, vmul(sv[o], wx[xx]));
-                            wy_ = wy_ + sw[o] * static_cast<W>(wx[xx]);
-                        }
-                        v = vadd(v, vmul(vy, wy[yy]));
-                        wv = wv + wy_ * static_cast<W>(wy[yy]);
-                    }
-                    vimag_mul(v, conj);
-                }
-            } else {
-                v = interp2<INTERP, In>(sl, p.is_y, p.is_x, Hs, Ws, p.slices_centered, py, px);
-                if (wsl) wv = interp2<INTERP, W>(wsl, p.wis_y, p.wis_x, Hs, Ws, p.slices_centered, py, px);
-            }
-            const i32 lv = acc_index(lz, ly, lx);
-            acc_v[lv] = vadd(acc_v[lv], vmul(v, static_cast<RIn>(window)));
-            if (has_wvol)
-                acc_w[lv] = acc_w[lv] + (win_present ? wv * static_cast<W>(window) : static_cast<W>(window));
-        }
-        __syncthreads();
-        cur = nxt;
-        if (cur >= 0) {
-            pi = pi_n;
-            rot = rot_n;
-        }
-    }
-
-    if (item.w >= 0) {
-        // Split brick: store this segment's sums (every slot; the reduction applies the cutoff).
-        In* pv = reinterpret_cast<In*>(p.partials + static_cast<i64>(item.w) * ACC * 3);
-        W* pw = reinterpret_cast<W*>(p.partials + static_cast<i64>(item.w) * ACC * 3 + ACC * 2);
-        for (i32 v = tid; v < ACC; v += NT) {
-            pv[v] = acc_v[v];
-            pw[v] = acc_w[v];
-        }
-        return;
-    }
-    // value and weights of every in-cutoff voxel are added to the volume (:516-518).
-    for (i32 v = tid; v < BE * BE * BE; v += NT) {
-        const i32 lz = v / (BE * BE), ly = (v / BE) % BE, lx = v % BE;
-        const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
-        if (z >= ext[0] || y >= ext[1] || x >= ext[2]) continue;
-        vec3<C> f;
-        f.z = static_cast<C>(index2frequency(vc, z, grid[0])) / p.target[0];
-        f.y = static_cast<C>(index2frequency(vc, y, grid[1])) / p.target[1];
-        f.x = static_cast<C>(x) / p.target[2];
-        if (dot3(f) > p.cutoff_sqd) continue;
-        const i32 a = acc_index(lz, ly, lx);
-        Out* o = static_cast<Out*>(p.vol) + (z * p.vs0 + y * p.vs1 + x * p.vs2);
-        *o = vadd(*o, vcast<Out>(acc_v[a]));
-        if (p.has_wvol) {
-            W* w = p.wvol + (z * p.wvs0 + y * p.wvs1 + x * p.wvs2);
-            *w = *w + acc_w[a];
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------------------------------------
 // One-wave path: a work item is (8x8x16 sub-brick, range of slices) processed by one wave with no barrier
@@ -626,8 +184,10 @@ struct SliceBuf {
     __device__ __forceinline__ SliceBuf(const void* base, u32 bytes)
         : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
     template<class T> __device__ __forceinline__ T load(u32 elem) const {
+        return load_bytes<T>(elem * static_cast<u32>(sizeof(T)));
+    }
+    template<class T> __device__ __forceinline__ T load_bytes(u32 off) const {
         T v;
-        const u32 off = elem * static_cast<u32>(sizeof(T));
         if constexpr (sizeof(T) == 4) {
             const u32 r = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
             __builtin_memcpy(&v, &r, 4);
@@ -1076,6 +636,389 @@ __global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertPa
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Slab kernel: the common single-precision configuration (Mat33<f32> rotations, c32 or f32 slices with f32 or no
+// weights, LINEAR, one power-of-two size n for the slices, the volume and the target, cutoff <= 0.5, no scaling
+// or Ewald sphere). Same work items and sub-bricks as the one-wave kernel, with less work per pair and per voxel:
+//   - A prologue kernel writes one record per slice: the rotation, the volume axis A closest to the slab normal
+//     and the slab's centre and half-width along A as linear functions of the two other coordinates.
+//   - Per (sub-brick, slice) pair, each lane takes one or two columns of the sub-brick face across A, clamps the
+//     slab range of its column to the sub-brick and to the cutoff sphere (exact integer test, precomputed per
+//     column), and the candidates are packed into an LDS list by ballots (one per candidate rank j).
+//   - Per candidate, the coordinates are noa's expressions in sample units: with n a power of two,
+//     R * (index / n) * n rounds exactly like R * index, so pz, py, px are noa's fz*n, f2y*n, f2x*n bit for bit.
+//     The window is a degree-9 polynomial in t = 2 (pz/fbT)^2 - 1 (fitted per window, error < 4e-7), folded
+//     into the bilinear weights; taps beyond noa's frequency bounds get weight 0 and never load outside the slice
+//     (buffer resources). Sums are added to the LDS accumulator with float atomics; a voxel is met at most once
+//     per slice and slices are processed in order, so the result is deterministic.
+// ---------------------------------------------------------------------------------------------------------
+constexpr i32 SLAB_NWIN = 9;        // degree of the window polynomial
+constexpr i32 SLAB_MAXC = 1024;     // candidates of one pair: at most the sub-brick's 1024 voxels
+
+template<class In, class W>
+struct SlabParams {
+    const In* in; i64 is_b;             // slices, unit x stride
+    const W* win; i64 wis_b;            // weights with the slices' row pitch, or null (weight = window)
+    i32 pitch;                          // row pitch of the slices and weights (elements)
+    u32 slice_bytes, wslice_bytes;      // byte extent of one slice and of one weight slice
+    void* vol; i32 vs0, vs1, vs2;
+    W* wvol; i32 wvs0, wvs1, wvs2;      // null: no output weights
+    i32 n;                              // logical size of every axis of the slices, the volume and the target
+    i32 nb[3];                          // sub-bricks along z, y, x
+    const int4* items;                  // {sub-brick, first slice, end slice, partial slot or -1}
+    float* partials;
+    const float4* rec;                  // 4 float4 per slice
+    i32 n_slices;
+    float fbT;                          // window half-width f_b * n, in samples
+    float tk;                           // 2 / fbT^2
+    float wa[SLAB_NWIN + 1];
+    float cut2;                         // cutoff^2 * n^2: a voxel is inside when iz^2 + iy^2 + ix^2 <= cut2
+    bool vol_centered;
+};
+
+// One record per slice: {R00, R01, R02, -}, {R10, R11, R12, ub}, {R20, R21, R22, uc}, {hw, A, -, -}. With n = the
+// first row (pz = n . v in samples), A = argmax |n_a|, B < C the other axes: the slab |pz| <= fbT runs along A
+// around ub * v_B + uc * v_C (ub = -n_B / n_A, uc = -n_C / n_A) with half-width fbT / |n_A|, widened by 2e-3
+// samples to cover the rounding of the exact per-voxel test (a few 1e-5 samples).
+__global__ __launch_bounds__(256) void slab_records_kernel(const XformArg<float> rot, i32 n_slices, float fbT,
+                                                           float4* rec) {
+    const i32 s = static_cast<i32>(blockIdx.x) * 256 + static_cast<i32>(threadIdx.x);
+    if (s >= n_slices) return;
+    const Rot<float> r = load_rotation(rot, s);
+    const float* m = r.m;
+    const float a0 = fabsf(m[0]), a1 = fabsf(m[1]), a2 = fabsf(m[2]);
+    i32 ax = 0;
+    if (a1 > a0) ax = 1;
+    if (a2 > (ax == 0 ? a0 : a1)) ax = 2;
+    const float nA = m[ax];
+    const float nB = ax == 0 ? m[1] : m[0];
+    const float nC = ax == 2 ? m[1] : m[2];
+    const float inv = 1.f / nA;
+    float4* o = rec + 4 * static_cast<i64>(s);
+    o[0] = make_float4(m[0], m[1], m[2], 0.f);
+    o[1] = make_float4(m[3], m[4], m[5], -nB * inv);
+    o[2] = make_float4(m[6], m[7], m[8], -nC * inv);
+    o[3] = make_float4(fbT * fabsf(inv) + 2e-3f, __int_as_float(ax), 0.f, 0.f);
+}
+
+// Largest integer m >= 0 with m^2 + r2 <= cut2 (all values exact in single precision), or -1 if none.
+__device__ __forceinline__ float sphere_half_extent(float r2, float cut2) {
+    const float room = cut2 - r2;
+    if (room < 0.f) return -1.f;
+    float m = floorf(sqrtf(room));
+    if ((m + 1.f) * (m + 1.f) + r2 <= cut2) m += 1.f;
+    if (m * m + r2 > cut2) m -= 1.f;
+    return m;
+}
+
+// Column data of one face orientation for one lane slot: face coordinates (lb, lc), the candidate word of k = 0 and
+// the range of k inside the stored volume and the cutoff sphere.
+struct SlabColumn {
+    float lb, lc, klo, khi;
+    u32 word0;
+};
+
+// Candidate word: lx | ly << 8 | lz << 16 | (voxel index in the sub-brick) << 22 (lz < 8 stays below bit 22).
+__device__ __forceinline__ u32 slab_word(i32 lz, i32 ly, i32 lx) {
+    return static_cast<u32>(lx) | (static_cast<u32>(ly) << 8) | (static_cast<u32>(lz) << 16) |
+           (static_cast<u32>((lz * WY + ly) * WX + lx) << 22);
+}
+
+template<int AX>
+__device__ __forceinline__ SlabColumn slab_column(i32 lb, i32 lc, const i32 f0[3], const i32 nin[3], float cut2) {
+    // AX 0: k along z, columns (y, x); AX 1: k along y, columns (z, x); AX 2: k along x, columns (z, y).
+    SlabColumn c;
+    c.lb = static_cast<float>(lb);
+    c.lc = static_cast<float>(lc);
+    constexpr i32 B = AX == 0 ? 1 : 0;
+    constexpr i32 CC = AX == 2 ? 1 : 2;
+    const i32 lz = AX == 0 ? 0 : lb, ly = AX == 0 ? lb : (AX == 1 ? 0 : lc), lx = AX == 2 ? 0 : lc;
+    c.word0 = slab_word(lz, ly, lx);
+    const bool inside = lb < nin[B] && lc < nin[CC];
+    const float fb = static_cast<float>(f0[B] + lb), fc = static_cast<float>(f0[CC] + lc);
+    const float m = sphere_half_extent(fb * fb + fc * fc, cut2);
+    const float fa0 = static_cast<float>(f0[AX]);
+    c.klo = fmaxf(-m - fa0, 0.f);
+    c.khi = fminf(m - fa0, static_cast<float>(nin[AX] - 1));
+    if (!inside || m < 0.f) { c.klo = 1.f; c.khi = 0.f; }
+    return c;
+}
+
+__device__ __forceinline__ u32 lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<u32>(m), 0u));
+}
+
+// Packs the candidates of one face orientation into `cand` (ordered by rank j, then slot, then lane); returns their
+// number. `u0`: the slab centre along A at face coordinates (0, 0), relative to the sub-brick's first sample.
+template<int AX, int NSLOT>
+__device__ __forceinline__ u32 slab_candidates(u32* cand, const SlabColumn (&col)[2], float u0, float ub, float uc,
+                                               float hw) {
+    constexpr u32 STEP = AX == 0 ? ((1u << 16) | (128u << 22)) : (AX == 1 ? ((1u << 8) | (16u << 22)) : (1u | (1u << 22)));
+    float cnt[NSLOT];
+    u32 w[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        const float centre = fmaf(ub, col[s].lb, fmaf(uc, col[s].lc, u0));
+        const float kl = fmaxf(ceilf(centre - hw), col[s].klo);
+        const float kh = fminf(floorf(centre + hw), col[s].khi);
+        cnt[s] = kh - kl;                                   // candidates - 1 (negative: none)
+        w[s] = col[s].word0 + static_cast<u32>(static_cast<i32>(fmaxf(kl, 0.f))) * STEP;
+    }
+    u32 off = 0;
+    for (i32 j = 0; j < 16; ++j) {
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) {
+            const bool has = cnt[s] >= static_cast<float>(j);
+            const unsigned long long m = __ballot(has);
+            if (m == 0) continue;
+            any = true;
+            if (has) cand[off + lanes_below(m)] = w[s] + static_cast<u32>(j) * STEP;
+            off += static_cast<u32>(__popcll(m));
+        }
+        if (!any) break;
+    }
+    return off;
+}
+
+template<class In, class W>
+struct SlabTaps {
+    In v00, v01, v10, v11;
+    W q00, q01, q10, q11;
+    float ty, tx, win;
+    u32 lv;
+    bool hit, flip, oky0, oky1, okx1;
+};
+
+template<class In, class Out, class W, bool SC, bool PAIRED>
+__global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W> p) {
+    constexpr bool CX = vt<In>::cx;
+    constexpr i32 NCH = CX ? 3 : 2;                    // accumulator channels: re (, im), weight
+    __shared__ float acc[NCH * WVOX];
+    __shared__ u32 cand[SLAB_MAXC];
+
+    const i32 lane = static_cast<i32>(threadIdx.x);
+    int4 item = p.items[blockIdx.x];
+    item.z = min(item.z, p.n_slices);
+    item.y = min(item.y, item.z);
+    const i32 brick = item.x;
+    const i32 bxi = brick % p.nb[2];
+    const i32 rem = brick / p.nb[2];
+    const i32 byi = rem % p.nb[1];
+    const i32 bzi = rem / p.nb[1];
+    const i32 n = p.n, half = n / 2, wr = half + 1;
+    const i32 org[3] = {bzi * WZ, byi * WY, bxi * WX};
+    const bool vc = p.vol_centered;
+    // First frequency of each axis of the sub-brick: frequencies are contiguous inside a sub-brick because n/2 is
+    // a multiple of the sub-brick sizes.
+    i32 f0[3], nin[3];
+    f0[0] = vc ? org[0] - half : (org[0] < half ? org[0] : org[0] - n);
+    f0[1] = vc ? org[1] - half : (org[1] < half ? org[1] : org[1] - n);
+    f0[2] = org[2];
+    nin[0] = WZ;
+    nin[1] = WY;
+    nin[2] = min(WX, wr - org[2]);
+
+    for (i32 v = lane; v < NCH * WVOX; v += 64) acc[v] = 0.f;
+
+    // Column data of the three face orientations (slot 1 of AX 0 and 1: the columns lane + 64).
+    SlabColumn c0[2], c1[2], c2[2];
+    c0[0] = slab_column<0>(lane >> 4, lane & 15, f0, nin, p.cut2);
+    c0[1] = slab_column<0>((lane >> 4) + 4, lane & 15, f0, nin, p.cut2);
+    c1[0] = slab_column<1>(lane >> 4, lane & 15, f0, nin, p.cut2);
+    c1[1] = slab_column<1>((lane >> 4) + 4, lane & 15, f0, nin, p.cut2);
+    c2[0] = slab_column<2>(lane >> 3, lane & 7, f0, nin, p.cut2);
+    c2[1] = c2[0];
+
+    // Culling of a slice against the sub-brick box, in samples: |n . centre| <= fbT + |n| . half + slack.
+    const float cz = static_cast<float>(f0[0]) + 0.5f * static_cast<float>(nin[0] - 1);
+    const float cy = static_cast<float>(f0[1]) + 0.5f * static_cast<float>(nin[1] - 1);
+    const float cx = static_cast<float>(f0[2]) + 0.5f * static_cast<float>(nin[2] - 1);
+    const float hz = 0.5f * static_cast<float>(nin[0] - 1), hy = 0.5f * static_cast<float>(nin[1] - 1);
+    const float hx = 0.5f * static_cast<float>(nin[2] - 1);
+    const float fbT = p.fbT;
+    const float f0zf = static_cast<float>(f0[0]), f0yf = static_cast<float>(f0[1]), f0xf = static_cast<float>(f0[2]);
+
+    // Records of the current chunk of 64 slices (lane l: slice chunk0 + l) and of the next one.
+    auto load_rec = [&](i32 s, float4 (&r)[4]) {
+        if (s < item.z) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = p.rec[4 * static_cast<i64>(s) + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    float4 cur[4], nxt[4];
+    i32 chunk0 = item.y;
+    load_rec(chunk0 + lane, cur);
+    load_rec(chunk0 + 64 + lane, nxt);
+    auto scan = [&]() -> unsigned long long {
+        const float d = fabsf((cur[0].x * cz + cur[0].y * cy) + cur[0].z * cx);
+        const float bound = fbT + fabsf(cur[0].x) * hz + fabsf(cur[0].y) * hy + fabsf(cur[0].z) * hx;
+        return __ballot(chunk0 + lane < item.z && d <= bound + 2e-3f);
+    };
+    unsigned long long mask = scan();
+
+    const float wa0 = p.wa[0], wa1 = p.wa[1], wa2 = p.wa[2], wa3 = p.wa[3], wa4 = p.wa[4];
+    const float wa5 = p.wa[5], wa6 = p.wa[6], wa7 = p.wa[7], wa8 = p.wa[8], wa9 = p.wa[9];
+    const float tk = p.tk;
+    const i32 pitch = p.pitch;
+
+    while (true) {
+        while (mask == 0) {
+            chunk0 += 64;
+            if (chunk0 >= item.z) break;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
+            load_rec(chunk0 + 64 + lane, nxt);
+            mask = scan();
+        }
+        if (mask == 0) break;
+        const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
+        mask &= mask - 1;
+        const i32 s = chunk0 + k;
+        // The slice's record, broadcast from the lane that scanned it.
+        const float R00 = bcast(cur[0].x, k), R01 = bcast(cur[0].y, k), R02 = bcast(cur[0].z, k);
+        const float R10 = bcast(cur[1].x, k), R11 = bcast(cur[1].y, k), R12 = bcast(cur[1].z, k);
+        const float R20 = bcast(cur[2].x, k), R21 = bcast(cur[2].y, k), R22 = bcast(cur[2].z, k);
+        const float ub = bcast(cur[1].w, k), uc = bcast(cur[2].w, k), hw = bcast(cur[3].x, k);
+        const i32 ax = __builtin_amdgcn_readlane(__float_as_int(cur[3].y), k);
+
+        __syncthreads();   // the previous pair's candidate reads are done before the list is rewritten
+        u32 total;
+        if (ax == 0) {
+            const float u0 = fmaf(ub, static_cast<float>(f0[1]), fmaf(uc, static_cast<float>(f0[2]), -f0zf));
+            total = slab_candidates<0, 2>(cand, c0, u0, ub, uc, hw);
+        } else if (ax == 1) {
+            const float u0 = fmaf(ub, static_cast<float>(f0[0]), fmaf(uc, static_cast<float>(f0[2]), -f0yf));
+            total = slab_candidates<1, 2>(cand, c1, u0, ub, uc, hw);
+        } else {
+            const float u0 = fmaf(ub, static_cast<float>(f0[0]), fmaf(uc, static_cast<float>(f0[1]), -f0xf));
+            total = slab_candidates<2, 1>(cand, c2, u0, ub, uc, hw);
+        }
+        __syncthreads();
+
+        const SliceBuf sb(p.in + static_cast<i64>(s) * p.is_b, p.slice_bytes);
+        const SliceBuf wb(p.win ? static_cast<const void*>(p.win + static_cast<i64>(s) * p.wis_b)
+                                : static_cast<const void*>(p.in), p.win ? p.wslice_bytes : 0u);
+        for (u32 base = 0; base < total; base += 128) {
+            SlabTaps<In, W> t[2];
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                t[g].hit = false;
+                if (base + 64u * g >= total) continue;            // uniform
+                const u32 pos = base + 64u * g + static_cast<u32>(lane);
+                const bool real = pos < total;
+                const u32 e = cand[real ? pos : 0u];
+                const float iz = f0zf + static_cast<float>((e >> 16) & 63u);
+                const float iy = f0yf + static_cast<float>((e >> 8) & 255u);
+                const float ix = f0xf + static_cast<float>(e & 255u);
+                t[g].lv = e >> 22;
+                // noa's grid2slice in samples (FourierProject.hpp:71-95 with R * (index / n) * n == R * index).
+                const float pz = (R00 * iz + R01 * iy) + R02 * ix;
+                float py = (R10 * iz + R11 * iy) + R12 * ix;
+                float px = (R20 * iz + R21 * iy) + R22 * ix;
+                t[g].hit = real && fabsf(pz) <= fbT;
+                const float u = pz * pz;
+                const float tt = fmaf(u, tk, -1.f);
+                float wv = wa9;
+                wv = fmaf(wv, tt, wa8); wv = fmaf(wv, tt, wa7); wv = fmaf(wv, tt, wa6); wv = fmaf(wv, tt, wa5);
+                wv = fmaf(wv, tt, wa4); wv = fmaf(wv, tt, wa3); wv = fmaf(wv, tt, wa2); wv = fmaf(wv, tt, wa1);
+                wv = fmaf(wv, tt, wa0);
+                t[g].win = wv;
+                // Early flip of LINEAR spectrum interpolation (Interpolation.hpp:459-463).
+                t[g].flip = px < 0.f;
+                py = t[g].flip ? -py : py;
+                px = fabsf(px);
+                const float fy = floorf(py), fx = floorf(px);
+                t[g].ty = py - fy;
+                t[g].tx = px - fx;
+                const i32 y0 = static_cast<i32>(fy), x0 = static_cast<i32>(fx);
+                // Bounds of the rfft slice (Frequency.hpp:258-288): y in [-n/2, n/2 - 1], x in [0, n/2].
+                t[g].oky0 = static_cast<u32>(y0 + half) < static_cast<u32>(n);
+                t[g].oky1 = static_cast<u32>(y0 + half) < static_cast<u32>(n - 1);
+                t[g].okx1 = x0 < half;
+                const i32 row0 = SC ? y0 + half : (y0 & (n - 1));
+                const i32 row1 = SC ? row0 + 1 : ((y0 + 1) & (n - 1));
+                // Element offsets of the two tap rows (24-bit products: rows and pitch are below 2^24), then byte
+                // offsets; the x + 1 taps use the loads' immediate offset.
+                const u32 e0 = __umul24(static_cast<u32>(row0), static_cast<u32>(pitch)) + static_cast<u32>(x0);
+                const u32 e1 = __umul24(static_cast<u32>(row1), static_cast<u32>(pitch)) + static_cast<u32>(x0);
+                // PAIRED lets the compiler merge the two x taps of a row into one load; otherwise the second offset
+                // is hidden from it and each tap is its own load.
+                const u32 b0 = e0 * static_cast<u32>(sizeof(In)), b1 = e1 * static_cast<u32>(sizeof(In));
+                const u32 b0x = PAIRED ? b0 + static_cast<u32>(sizeof(In)) : pin(b0 + static_cast<u32>(sizeof(In)));
+                const u32 b1x = PAIRED ? b1 + static_cast<u32>(sizeof(In)) : pin(b1 + static_cast<u32>(sizeof(In)));
+                t[g].v00 = sb.template load_bytes<In>(b0);
+                t[g].v01 = sb.template load_bytes<In>(b0x);
+                t[g].v10 = sb.template load_bytes<In>(b1);
+                t[g].v11 = sb.template load_bytes<In>(b1x);
+                if (p.win) {
+                    const u32 c0 = e0 * 4u, c1 = e1 * 4u;
+                    const u32 c0x = PAIRED ? c0 + 4u : pin(c0 + 4u), c1x = PAIRED ? c1 + 4u : pin(c1 + 4u);
+                    t[g].q00 = wb.template load_bytes<W>(c0);
+                    t[g].q01 = wb.template load_bytes<W>(c0x);
+                    t[g].q10 = wb.template load_bytes<W>(c1);
+                    t[g].q11 = wb.template load_bytes<W>(c1x);
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                if (base + 64u * g >= total) continue;
+                if (!t[g].hit) continue;
+                const float a0 = t[g].oky0 ? (1.f - t[g].ty) * t[g].win : 0.f;
+                const float a1 = t[g].oky1 ? t[g].ty * t[g].win : 0.f;
+                const float bx0 = 1.f - t[g].tx, bx1 = t[g].okx1 ? t[g].tx : 0.f;
+                const float w00 = a0 * bx0, w01 = a0 * bx1, w10 = a1 * bx0, w11 = a1 * bx1;
+                float re, im = 0.f;
+                if constexpr (CX) {
+                    re = fmaf(w11, t[g].v11.x, fmaf(w10, t[g].v10.x, fmaf(w01, t[g].v01.x, w00 * t[g].v00.x)));
+                    im = fmaf(w11, t[g].v11.y, fmaf(w10, t[g].v10.y, fmaf(w01, t[g].v01.y, w00 * t[g].v00.y)));
+                    im = t[g].flip ? -im : im;
+                } else {
+                    re = fmaf(w11, t[g].v11, fmaf(w10, t[g].v10, fmaf(w01, t[g].v01, w00 * t[g].v00)));
+                }
+                const float wt = p.win ? fmaf(w11, t[g].q11, fmaf(w10, t[g].q10, fmaf(w01, t[g].q01, w00 * t[g].q00)))
+                                       : t[g].win;
+                atomicAdd(&acc[t[g].lv], re);
+                if constexpr (CX) atomicAdd(&acc[WVOX + t[g].lv], im);
+                atomicAdd(&acc[(NCH - 1) * WVOX + t[g].lv], wt);
+            }
+        }
+    }
+    __syncthreads();
+
+    if (item.w >= 0) {
+        // Split sub-brick: this segment's sums, in the layout of insert_reduce_kernel's partial slots.
+        In* pv = reinterpret_cast<In*>(p.partials + static_cast<i64>(item.w) * WVOX * 3);
+        W* pw = reinterpret_cast<W*>(p.partials + static_cast<i64>(item.w) * WVOX * 3 + WVOX * 2);
+        for (i32 v = lane; v < WVOX; v += 64) {
+            if constexpr (CX) pv[v] = make_float2(acc[v], acc[WVOX + v]);
+            else pv[v] = acc[v];
+            pw[v] = acc[(NCH - 1) * WVOX + v];
+        }
+        return;
+    }
+    // Sums of the voxels inside the stored volume and the cutoff are added to the volume (:516-518).
+    for (i32 v = lane; v < WVOX; v += 64) {
+        const i32 lz = v / (WY * WX), ly = (v / WX) % WY, lx = v % WX;
+        if (lx >= nin[2]) continue;
+        const float iz = f0zf + static_cast<float>(lz), iy = f0yf + static_cast<float>(ly), ix = f0xf + static_cast<float>(lx);
+        if ((iz * iz + iy * iy) + ix * ix > p.cut2) continue;
+        const i32 z = org[0] + lz, y = org[1] + ly, x = org[2] + lx;
+        In sum;
+        if constexpr (CX) sum = make_float2(acc[v], acc[WVOX + v]);
+        else sum = acc[v];
+        Out* o = static_cast<Out*>(p.vol) + (z * p.vs0 + y * p.vs1 + x * p.vs2);
+        *o = vadd(*o, vcast<Out>(sum));
+        if (p.wvol) {
+            W* w = p.wvol + (z * p.wvs0 + y * p.wvs1 + x * p.wvs2);
+            *w = *w + acc[(NCH - 1) * WVOX + v];
+        }
+    }
+}
+
 // General path: one lane per voxel, every slice in order (noa's operator as is).
 template<class C, class In, class Out, class W, int INTERP, bool GENERAL>
 __global__ __launch_bounds__(256) void insert_general_kernel(const InsertParams<C, In, W> p) {
@@ -1314,25 +1257,87 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
     return NOA_HIP_OK;
 }
 
-// Culled kernel of the insertion: the one-wave sub-brick kernel (default) or the staged workgroup kernel
-// (NOA_HIP_INSERT_KERNEL=staged, with NOA_HIP_INSERT_THREADS=256|512), and the slice count per item.
+// Culled kernel of the insertion: the slab kernel where the configuration allows it, else the one-wave kernel
+// (NOA_HIP_INSERT_KERNEL=wave forces the one-wave kernel), and the expected passing slices per item.
 struct KernelChoice {
-    bool staged;
-    int threads;
+    bool force_wave;
+    bool paired;          // slab kernel: one 16-byte (8-byte) load for the two x taps of a slice (weight) row
     double list_target;
 };
 KernelChoice kernel_choice() {
-    static const KernelChoice c = [] {
+    const KernelChoice c = [] {
         KernelChoice k{};
         const char* e = std::getenv("NOA_HIP_INSERT_KERNEL");
-        k.staged = e && std::string(e) == "staged";
-        const char* t = std::getenv("NOA_HIP_INSERT_THREADS");
-        k.threads = t && std::atoi(t) == 512 ? 512 : 256;
+        k.force_wave = e && std::string(e) == "wave";
+        const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED");
+        k.paired = pr && std::string(pr) == "1";
         const char* l = std::getenv("NOA_HIP_INSERT_LIST");
-        k.list_target = l ? std::max(16.0, std::atof(l)) : (k.staged ? 384.0 : 256.0);
+        k.list_target = l ? std::max(16.0, std::atof(l)) : 256.0;
         return k;
     }();
     return c;
+}
+
+// Window of the slab kernel: windowed_sinc(pz / n, fs, fb) (FourierProject.hpp:99-110) as a polynomial of degree
+// SLAB_NWIN in t = 2 (pz / fbT)^2 - 1, pz in samples (fbT = fb * n): Chebyshev interpolation at the Chebyshev nodes,
+// converted to monomials in t and rounded to single precision. Accepted when the kernel's own evaluation (u = pz^2,
+// t = fma(u, 2 / fbT^2, -1), Horner with fma, reproduced here) stays within 4e-7 of the double-precision window.
+struct SlabWindow {
+    float a[SLAB_NWIN + 1];
+    float fbT, tk;
+    bool ok;
+};
+std::map<std::tuple<float, float, i32>, SlabWindow> g_slab_windows;
+
+SlabWindow fit_slab_window(float fs, float fb, i32 n) {
+    std::lock_guard<std::mutex> lock(g_window_mutex);
+    const auto key = std::make_tuple(fs, fb, n);
+    auto it = g_slab_windows.find(key);
+    if (it != g_slab_windows.end()) return it->second;
+    const double PI = 3.141592653589793238462643383279502884;
+    auto w = [&](double f) {
+        const double x = PI * f / fs;
+        const double s = x == 0 ? 1.0 : std::sin(x) / x;
+        const double b = PI * f / fb;
+        return s * (0.42 + 0.5 * std::cos(b) + 0.08 * std::cos(2 * b));
+    };
+    SlabWindow sw{};
+    sw.fbT = fb * static_cast<float>(n);
+    sw.tk = 2.f / (sw.fbT * sw.fbT);
+    constexpr i32 deg = SLAB_NWIN, m = SLAB_NWIN + 1;
+    double c[SLAB_NWIN + 1] = {};
+    for (i32 k = 0; k <= deg; ++k) {
+        double sum = 0;
+        for (i32 j = 0; j < m; ++j) {
+            const double th = PI * (j + 0.5) / m;
+            const double u = 0.5 * (std::cos(th) + 1);                 // (pz / fbT)^2
+            sum += w(static_cast<double>(fb) * std::sqrt(u)) * std::cos(k * th);
+        }
+        c[k] = 2.0 * sum / m;
+    }
+    c[0] *= 0.5;
+    double tkm[SLAB_NWIN + 1][SLAB_NWIN + 1] = {};
+    tkm[0][0] = 1;
+    tkm[1][1] = 1;
+    for (i32 k = 2; k <= deg; ++k)
+        for (i32 j = 0; j <= k; ++j) tkm[k][j] = (j > 0 ? 2 * tkm[k - 1][j - 1] : 0.0) - tkm[k - 2][j];
+    for (i32 j = 0; j <= deg; ++j) {
+        double a = 0;
+        for (i32 k = j; k <= deg; ++k) a += c[k] * tkm[k][j];
+        sw.a[j] = static_cast<float>(a);
+    }
+    double err = 0;
+    for (i32 j = -4000; j <= 4000; ++j) {
+        const float pz = sw.fbT * (static_cast<float>(j) / 4000.f);
+        const float u = pz * pz;
+        const float t = std::fmaf(u, sw.tk, -1.f);
+        float r = sw.a[deg];
+        for (i32 k = deg - 1; k >= 0; --k) r = std::fmaf(r, t, sw.a[k]);
+        err = std::max(err, std::abs(static_cast<double>(r) - w(static_cast<double>(pz) / n)));
+    }
+    sw.ok = err <= 4e-7;
+    g_slab_windows[key] = sw;
+    return sw;
 }
 
 template<class C, class In, class Out, class W>
@@ -1360,10 +1365,10 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     }
     p.wr = p.grid[2] / 2 + 1;
     const KernelChoice kc = kernel_choice();
-    p.bd[0] = kc.staged ? BE : WZ;
-    p.bd[1] = kc.staged ? BE : WY;
-    p.bd[2] = kc.staged ? BE : WX;
-    p.apitch = kc.staged ? AP : WX;
+    p.bd[0] = WZ;
+    p.bd[1] = WY;
+    p.bd[2] = WX;
+    p.apitch = WX;
     p.nb[0] = (p.grid[0] + p.bd[0] - 1) / p.bd[0];
     p.nb[1] = (p.grid[1] + p.bd[1] - 1) / p.bd[1];
     p.nb[2] = (p.wr + p.bd[2] - 1) / p.bd[2];
@@ -1411,7 +1416,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
 
     constexpr bool single = sizeof(typename vt<In>::real) == 4 && sizeof(W) == 4;
     if constexpr (single) {
-        if (!p.general && !p.in_is_value && !p.win_is_value && (buffers_ok || kc.staged) && window_size(d.interp) == 2) {
+        if (!p.general && !p.in_is_value && !p.win_is_value && buffers_ok && window_size(d.interp) == 2) {
             int device = 0;
             (void)hipGetDevice(&device);
             Plan plan;
@@ -1443,7 +1448,65 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                                   p.vol_centered == p.slices_centered && p.is_x == 1 && p.wis_x == 1 &&
                                   p.is_y == p.wis_y && p.slice_bytes < 0x7ffffff0u && p.wslice_bytes < 0x7ffffff0u;
             const int hot = hot_opts ? (p.vol_centered ? 2 : 1) : 0;
-            if (!kc.staged) {
+            // The slab kernel: Mat33<f32> rotations, LINEAR, one power-of-two size n >= 32 for the slices, the
+            // volume and the target, cutoff <= 0.5, a window of at most ~1.15 samples (at most 4 candidates per column)
+            // that the degree-9 polynomial fits, unit x strides and one row pitch for the slices and their weights.
+            bool slab = false;
+            SlabWindow sw{};
+            if constexpr (std::is_same_v<C, float>) {
+                const i64 nn = d.target[0];
+                const bool cubic = d.target[1] == nn && d.target[2] == nn && p.grid[0] == nn && p.grid[1] == nn &&
+                                   p.grid[2] == nn && p.slice_shape[0] == nn && p.slice_shape[1] == nn;
+                if (!kc.force_wave && d.interp == NOA_HIP_INTERP_LINEAR && cubic && p.pow2 && nn >= 32 &&
+                    p.rot.kind == NOA_HIP_MAT33_F32 && cutoff <= C(0.5) && p.is_x == 1 &&
+                    (!p.win_present || (p.wis_x == 1 && p.wis_y == p.is_y)) && p.slice_bytes < 0x7ffffff0u &&
+                    p.wslice_bytes < 0x7ffffff0u) {
+                    sw = fit_slab_window(static_cast<float>(p.fftfreq_sinc), static_cast<float>(p.fftfreq_blackman),
+                                         static_cast<i32>(nn));
+                    slab = sw.ok && sw.fbT <= 1.15f;
+                }
+            }
+            void* records = nullptr;
+            if (slab) {
+                if (hipMallocAsync(&records, static_cast<size_t>(p.n_slices) * 4 * sizeof(float4), stream) != hipSuccess) {
+                    if (partials) (void)hipFreeAsync(partials, stream);
+                    err = "insert: could not allocate the slice records";
+                    return NOA_HIP_ERROR_RUNTIME;
+                }
+                if constexpr (std::is_same_v<C, float>) {
+                    hipLaunchKernelGGL(slab_records_kernel, dim3(static_cast<unsigned>((p.n_slices + 255) / 256)),
+                                       dim3(256), 0, stream, p.rot, p.n_slices, sw.fbT, static_cast<float4*>(records));
+                    SlabParams<In, W> sp{};
+                    sp.in = p.in; sp.is_b = p.is_b;
+                    sp.win = p.win_present ? p.win : nullptr; sp.wis_b = p.wis_b;
+                    sp.pitch = p.is_y;
+                    sp.slice_bytes = p.slice_bytes; sp.wslice_bytes = p.wslice_bytes;
+                    sp.vol = p.vol; sp.vs0 = p.vs0; sp.vs1 = p.vs1; sp.vs2 = p.vs2;
+                    sp.wvol = p.has_wvol ? p.wvol : nullptr; sp.wvs0 = p.wvs0; sp.wvs1 = p.wvs1; sp.wvs2 = p.wvs2;
+                    sp.n = p.grid[0];
+                    for (int i = 0; i < 3; ++i) sp.nb[i] = p.nb[i];
+                    sp.items = plan.items;
+                    sp.partials = p.partials;
+                    sp.rec = static_cast<const float4*>(records);
+                    sp.n_slices = p.n_slices;
+                    sp.fbT = sw.fbT;
+                    sp.tk = sw.tk;
+                    for (int k = 0; k <= SLAB_NWIN; ++k) sp.wa[k] = sw.a[k];
+                    sp.cut2 = static_cast<float>(p.cutoff_sqd) * static_cast<float>(p.grid[0]) * static_cast<float>(p.grid[0]);
+                    sp.vol_centered = p.vol_centered;
+                    if (kc.paired) {
+                        if (p.slices_centered)
+                            hipLaunchKernelGGL((insert_slab_kernel<In, Out, W, true, true>), grid_items, dim3(64), 0, stream, sp);
+                        else
+                            hipLaunchKernelGGL((insert_slab_kernel<In, Out, W, false, true>), grid_items, dim3(64), 0, stream, sp);
+                    } else {
+                        if (p.slices_centered)
+                            hipLaunchKernelGGL((insert_slab_kernel<In, Out, W, true, false>), grid_items, dim3(64), 0, stream, sp);
+                        else
+                            hipLaunchKernelGGL((insert_slab_kernel<In, Out, W, false, false>), grid_items, dim3(64), 0, stream, sp);
+                    }
+                }
+            } else {
                 if (nearest) {
                     hipLaunchKernelGGL((insert_wave_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST>), grid_items, dim3(64), 0, stream, p);
                 } else if constexpr (std::is_same_v<C, float>) {
@@ -1453,18 +1516,13 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                 } else {
                     hipLaunchKernelGGL((insert_wave_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR>), grid_items, dim3(64), 0, stream, p);
                 }
-            } else if (kc.threads == 512) {
-                if (nearest) hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, 512>), grid_items, dim3(512), 0, stream, p);
-                else hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, 512>), grid_items, dim3(512), 0, stream, p);
-            } else {
-                if (nearest) hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_NEAREST, 256>), grid_items, dim3(256), 0, stream, p);
-                else hipLaunchKernelGGL((insert_staged_kernel<C, In, Out, W, NOA_HIP_INTERP_LINEAR, 256>), grid_items, dim3(256), 0, stream, p);
             }
             if (plan.n_reduce > 0)
                 hipLaunchKernelGGL((insert_reduce_kernel<C, In, Out, W>), dim3(plan.n_reduce), dim3(NT_REDUCE), 0, stream, p);
             hipError_t e = hipGetLastError();
-            if (partials) {
-                const hipError_t fe = hipFreeAsync(partials, stream);
+            for (void* buf : {partials, records}) {
+                if (!buf) continue;
+                const hipError_t fe = hipFreeAsync(buf, stream);
                 if (e == hipSuccess) e = fe;
             }
             if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }

@@ -21,21 +21,33 @@
  *
  * Compiled with -ffp-contract=off so that a*b+c is never fused, as in the reference's x86-64 build.
  *
- * How each function is pinned (tests/test_oracle_*.py):
- *   - index/frequency/bounds: closed forms and the worked examples of Frequency.hpp:34-35, :57-58, :82-83,
+ * How each function is pinned (tests/test_oracle_pins.py unless stated; tests/test_poses.py for the poses):
+ *   - index/frequency/bounds: numpy's fftfreq and the worked examples of Frequency.hpp:34-35, :57-58, :82-83,
  *     :117-118, :253-256.
- *   - rotations (euler2matrix, quaternions): rotate_z(90deg) maps x to y; scipy's independent ZYZ rotation;
- *     quaternion <-> matrix round trips; orthonormality.
- *   - windowed sinc: closed form, w(0)=1, window-size examples of FourierProject.hpp:118-122, :140-141.
- *   - interpolate_spectrum: exact copies at integer frequencies, exact reproduction of a linear field
- *     inside the bounds, Hermitian symmetry, zeros outside the bounds, and an independent numpy restatement.
- *   - rasterize: identity-pose closed form incl. the x=0 Hermitian duplicate, mass conservation, the
- *     reference's remap and weights invariants (TestXformFourierInsertRasterize.cpp:72-152), and an FP64
- *     brute force written separately in numpy.
- *   - insert (interpolate): identity pose closed form; equality with an independent per-slice scatter in
- *     numpy; matrix == quaternion.
- *   - extract: identity pose closed form; rasterize->extract round trip on integer grids; the reference's
- *     remap invariant (TestXformFourierExtract.cpp:116-153); an independent numpy restatement.
+ *   - poses (noa_amd.xform.euler2matrix, the workloads' rotations): scipy's independent Rotation for the 12 axis
+ *     sequences, intrinsic and extrinsic, both handednesses; a round trip through scipy's inverse;
+ *     rotate_z(90deg) maps x to y; quaternion == matrix; orthonormality.
+ *   - windowed sinc: numpy's sinc times scipy's Blackman window, w(0)=1, the window-size examples of
+ *     FourierProject.hpp:118-122, :140-141.
+ *   - interpolate_spectrum: exact copies at integer frequencies, exact reproduction of a linear Hermitian field
+ *     inside the bounds, the flip and out-of-bounds rules, NEAREST without the conjugate; the windows of 4-8 taps
+ *     against numpy's textbook kernels.
+ *   - rasterize: identity-pose closed form incl. the x=0 Hermitian duplicate, the adjoint of extraction in FP64,
+ *     mass conservation (closed form of the total weight), an FP64 restatement written separately in numpy
+ *     (test_rasterize_fp64_brute_force_in_numpy), negation of flipped values, and the reference's remap and
+ *     weights invariants (TestXformFourierInsertRasterize.cpp:72-152).
+ *   - insert (interpolate): identity-pose closed form, a linear field at random poses, linearity, the EWS term
+ *     against the paraboloid, matrix == quaternion, weights == values, and an FP64 sweep of every voxel one slice
+ *     at a time written separately in numpy (test_insert_fp64_per_slice_scatter_in_numpy).
+ *   - extract: identity-pose closed form, a linear field at random poses, the reference's remap invariant
+ *     (TestXformFourierExtract.cpp:116-153), EWS/scaling/target/w-window closed forms, and an FP64 restatement
+ *     written separately in numpy (test_extract_fp64_restatement_in_numpy).
+ *   - insert's per-voxel sums at a voxel list (oracle_insert_central_slices_3d_voxels) run the same operator
+ *     object as the whole-volume entry point; they agree bit for bit on whole volumes.
+ *
+ * One deviation from noa, shared with the HIP kernels: rasterisation skips taps at x > n/2. noa bounds x by the
+ * logical width there (FourierProject.hpp:330-332) and, for a contiguous array, writes such taps into the next row;
+ * their weight is zero for cutoffs <= 0.5 and target shapes equal to the grid.
  */
 #include <cmath>
 #include <cstdint>

@@ -1229,3 +1229,221 @@ def test_fft_oracle_normalisation():
     for norm in ("forward", "backward", "ortho"):
         assert np.allclose(O.fft_c2r(O.fft_r2c(x, norm), shape, norm), x, atol=1e-12)
     assert np.allclose(O.fft_c2r(O.fft_r2c(x, "none"), shape, "none"), n * x, atol=1e-10)
+
+
+# ------------------------------------------------------------------------ independent FP64 restatements (numpy)
+def _np_f2i(centered, f, n):
+    """frequency2index (Frequency.hpp:115-122), written independently."""
+    return f + n // 2 if centered else (f + n if f < 0 else f)
+
+
+def _np_rasterize(slices, rot, n, src_c, dst_c, cutoff):
+    """FourierInsertRasterize (FourierProject.hpp:238-373) in float64 loops: slice pixel (v, u) -> (0, v/n, u/n),
+    forward rotation, cutoff, flip of x < 0 (the value is negated), trilinear spread over the 2x2x2 cube with
+    index bounds, and the conjugate duplicate on the x = 0 plane. Taps at x > n/2 are left out (noa bounds x by the
+    logical n there; they carry a zero weight at cutoffs <= 0.5)."""
+    vol = np.zeros((n, n, n // 2 + 1), np.complex128)
+    wv = np.zeros((n, n, n // 2 + 1), np.float64)
+    for b in range(slices.shape[0]):
+        for y in range(n):
+            v = y - n // 2 if src_c else (y if y < (n + 1) // 2 else y - n)
+            for u in range(n // 2 + 1):
+                f3 = rot[b] @ np.array([0.0, v / n, u / n])
+                if f3 @ f3 > cutoff * cutoff:
+                    continue
+                val = complex(slices[b, 0, y, u])
+                if f3[2] < 0:
+                    f3, val = -f3, -val
+                freq = f3 * n
+                base = np.floor(freq).astype(np.int64)
+                frac = [1 - (freq - base), freq - base]
+                for w in (0, 1):
+                    for vv in (0, 1):
+                        for uu in (0, 1):
+                            k = frac[w][0] * frac[vv][1] * frac[uu][2]
+                            iz = _np_f2i(dst_c, base[0] + w, n)
+                            iy = _np_f2i(dst_c, base[1] + vv, n)
+                            ix = base[2] + uu
+                            if 0 <= iz < n and 0 <= iy < n and 0 <= ix <= n // 2:
+                                vol[iz, iy, ix] += val * k
+                                wv[iz, iy, ix] += k
+                if base[2] == 0:
+                    for w in (0, 1):
+                        for vv in (0, 1):
+                            k = frac[w][0] * frac[vv][1] * frac[0][2]
+                            iz = _np_f2i(dst_c, -(base[0] + w), n)
+                            iy = _np_f2i(dst_c, -(base[1] + vv), n)
+                            if 0 <= iz < n and 0 <= iy < n:
+                                vol[iz, iy, 0] += np.conj(val) * k
+                                wv[iz, iy, 0] += k
+    return vol, wv
+
+
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h", "h2hc"])
+def test_rasterize_fp64_brute_force_in_numpy(remap):
+    rng = np.random.default_rng(61)
+    n, b = 8, 3
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng, np.complex128)
+    rot = random_rotations(b, rng, dtype=np.float64)
+    vol, wvol = _rasterize(remap, s, n, rot, vol_dtype=np.complex128, wout=True, cutoff=0.5)
+    src_c, dst_c = remap in ("hc2hc", "hc2h"), remap in ("hc2hc", "h2hc")
+    want_v, want_w = _np_rasterize(s, rot, n, src_c, dst_c, 0.5)
+    np.testing.assert_allclose(vol[0], want_v, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(wvol[0], want_w, rtol=0, atol=1e-12)
+
+
+def test_rasterize_mass_conservation():
+    """With unit weights, each in-cutoff pixel spreads a total weight of 1 over its cube (every corner in bounds at
+    cutoff 0.3), plus (1 - x) on the mirrored x = 0 column when its cube touches x = 0 (0 <= x < 1 samples)."""
+    rng = np.random.default_rng(62)
+    n, b = 32, 5
+    rot = random_rotations(b, rng, dtype=np.float64)
+    s = np.ones((b, 1, n, n // 2 + 1), np.float64)
+    _, wvol = _rasterize("h2h", s, n, rot, vol_dtype=np.float64, wout=True, cutoff=0.3)
+    v = freq_grid(n, False)[:, None] / n
+    u = np.arange(n // 2 + 1)[None, :] / n
+    total = 0.0
+    for r in rot:
+        f3 = r[:, 1, None, None] * v + r[:, 2, None, None] * u      # R . (0, v, u)
+        inside = (f3 ** 2).sum(0) <= 0.09
+        x = np.abs(f3[2]) * n
+        total += inside.sum() + np.where(inside & (x < 1), 1 - x, 0).sum()
+    np.testing.assert_allclose(wvol.sum(), total, rtol=1e-12)
+
+
+def _np_interp2_linear(sl, py, px, n, centered):
+    """LINEAR interpolate_spectrum of one 2D rfft slice (Interpolation.hpp:435-526) at float64 sample coordinates:
+    flip of x < 0 with the conjugate, bounds y in [-n/2, (n-1)/2] and x in [0, n/2], taps outside add 0."""
+    flip = px < 0
+    py = np.where(flip, -py, py)
+    px = np.where(flip, -px, px)
+    y0 = np.floor(py).astype(np.int64)
+    x0 = np.floor(px).astype(np.int64)
+    ty, tx = py - y0, px - x0
+    out = np.zeros(py.shape, np.complex128)
+    for dy, wy in ((0, 1 - ty), (1, ty)):
+        for dx, wx in ((0, 1 - tx), (1, tx)):
+            yy, xx = y0 + dy, x0 + dx
+            ok = (yy >= -(n // 2)) & (yy <= (n - 1) // 2) & (xx >= 0) & (xx <= n // 2)
+            iy = np.where(centered, yy + n // 2, np.where(yy < 0, yy + n, yy))
+            val = sl[np.clip(iy, 0, n - 1), np.clip(xx, 0, n // 2)]
+            out += np.where(ok, val, 0) * wy * wx
+    return np.where(flip, np.conj(out), out)
+
+
+def _np_insert(slices, weights, inv, n, src_c, dst_c, fb, cutoff):
+    """FourierInsertInterpolate (FourierProject.hpp:478-519) in float64, one slice at a time over every voxel:
+    (fz, f2) = R_inv . f, window sinc(fz/fb) * blackman(fz/fb) where |fz| <= fb, interpolated values and weights."""
+    fr = (np.arange(n) - n // 2) if dst_c else np.fft.fftfreq(n, 1.0 / n).round()
+    f = np.stack(np.broadcast_arrays(fr[:, None, None] / n, fr[None, :, None] / n,
+                                     np.arange(n // 2 + 1)[None, None, :] / n), -1)
+    inside = (f ** 2).sum(-1) <= cutoff * cutoff
+    vol = np.zeros(f.shape[:3], np.complex128)
+    wv = np.zeros(f.shape[:3], np.float64)
+    for b in range(slices.shape[0]):
+        g = f @ inv[b].T
+        fz = g[..., 0]
+        use = inside & (np.abs(fz) <= fb)
+        win = np.sinc(fz / fb) * (0.42 + 0.5 * np.cos(np.pi * fz / fb) + 0.08 * np.cos(2 * np.pi * fz / fb))
+        val = _np_interp2_linear(slices[b, 0], g[..., 1] * n, g[..., 2] * n, n, src_c)
+        wgt = _np_interp2_linear(weights[b, 0].astype(np.complex128), g[..., 1] * n, g[..., 2] * n, n, src_c).real
+        vol += np.where(use, val * win, 0)
+        wv += np.where(use, wgt * win, 0)
+    return vol, wv
+
+
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h"])
+def test_insert_fp64_per_slice_scatter_in_numpy(remap):
+    """The oracle's per-voxel gather over slices equals an independent numpy sweep of every voxel one slice at a
+    time (FP64, a wider window so that several voxels per column contribute)."""
+    rng = np.random.default_rng(63)
+    n, b = 16, 5
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng, np.complex128)
+    w = rng.uniform(0.2, 1.0, (b, 1, n, n // 2 + 1))
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng, dtype=np.float64), (0, 2, 1)))
+    fb = 2.0 / n
+    vol = np.zeros((1, n, n, n // 2 + 1), np.complex128)
+    wvol = np.zeros((1, n, n, n // 2 + 1), np.float64)
+    O.insert_central_slices_3d(remap, s, w, (b, 1, n, n), vol, wvol, (1, n, n, n), None, inv,
+                               fftfreq_sinc=fb, fftfreq_blackman=fb)
+    want_v, want_w = _np_insert(s, w, inv, n, remap in ("hc2hc", "hc2h"), remap in ("hc2hc", "h2hc"), fb, 0.5)
+    assert np.abs(want_w).max() > 0.5
+    np.testing.assert_allclose(vol[0], want_v, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(wvol[0], want_w, rtol=0, atol=1e-12)
+
+
+def test_insert_weights_equal_values():
+    """Real slices equal to their weights: the value and weight volumes are identical (same interpolation, same
+    window, same sum order)."""
+    rng = np.random.default_rng(64)
+    n, b = 32, 12
+    w = rng.uniform(0.1, 2.0, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    vol, wvol = _insert("hc2hc", w, n, inv, weights=w.copy(), wout=True, vol_dtype=np.float32)
+    assert np.abs(wvol).max() > 0
+    np.testing.assert_array_equal(vol, wvol)
+
+
+def _np_extract(vol, rot, n, src_c, dst_c, cutoff):
+    """FourierExtract 3D (FourierProject.hpp:632-657) in float64: output pixel (v, u) -> R_fwd . (0, v/n, u/n),
+    zero beyond the cutoff, else LINEAR 3D interpolation of the volume (flip of x < 0 with the conjugate, taps outside
+    [-n/2, (n-1)/2]^2 x [0, n/2] add 0)."""
+    b = rot.shape[0]
+    out = np.zeros((b, n, n // 2 + 1), np.complex128)
+    fr = (np.arange(n) - n // 2) if dst_c else np.fft.fftfreq(n, 1.0 / n).round()
+    for i in range(b):
+        f2 = np.stack(np.broadcast_arrays(np.zeros((n, n // 2 + 1)), fr[:, None] / n,
+                                          np.arange(n // 2 + 1)[None, :] / n), -1)
+        f = f2 @ rot[i].T
+        inside = (f ** 2).sum(-1) <= cutoff * cutoff
+        p = f * n
+        flip = p[..., 2] < 0
+        p = np.where(flip[..., None], -p, p)
+        base = np.floor(p).astype(np.int64)
+        t = p - base
+        acc = np.zeros(p.shape[:2], np.complex128)
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    zz, yy, xx = base[..., 0] + dz, base[..., 1] + dy, base[..., 2] + dx
+                    ok = ((zz >= -(n // 2)) & (zz <= (n - 1) // 2) & (yy >= -(n // 2)) & (yy <= (n - 1) // 2)
+                          & (xx >= 0) & (xx <= n // 2))
+                    iz = np.where(src_c, zz + n // 2, np.where(zz < 0, zz + n, zz))
+                    iy = np.where(src_c, yy + n // 2, np.where(yy < 0, yy + n, yy))
+                    val = vol[np.clip(iz, 0, n - 1), np.clip(iy, 0, n - 1), np.clip(xx, 0, n // 2)]
+                    wgt = ((t[..., 0] if dz else 1 - t[..., 0]) * (t[..., 1] if dy else 1 - t[..., 1])
+                           * (t[..., 2] if dx else 1 - t[..., 2]))
+                    acc += np.where(ok, val, 0) * wgt
+        acc = np.where(flip, np.conj(acc), acc)
+        out[i] = np.where(inside, acc, 0)
+    return out
+
+
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc", "hc2h", "h2hc"])
+def test_extract_fp64_restatement_in_numpy(remap):
+    rng = np.random.default_rng(65)
+    n, b = 16, 6
+    vol = random_spectrum((1, n, n, n // 2 + 1), rng, np.complex128)
+    rot = random_rotations(b, rng, dtype=np.float64)
+    got = _extract(remap, vol[0], n, rot, out_dtype=np.complex128, cutoff=0.5)
+    want = _np_extract(vol[0], rot, n, remap in ("hc2hc", "hc2h"), remap in ("hc2hc", "h2hc"), 0.5)
+    np.testing.assert_allclose(got[:, 0], want, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("remap", ["h2h", "hc2hc"])
+def test_insert_voxel_list_equals_the_whole_volume(remap):
+    """oracle_insert_central_slices_3d_voxels (the checker of the 256^3 insertion on the GPU) gives, at every voxel,
+    the sums the whole-volume entry point adds, bit for bit, and flags the voxels beyond the cutoff."""
+    rng = np.random.default_rng(66)
+    n, b = 24, 30
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0, 1, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    vol, wvol = _insert(remap, s, n, inv, weights=w, wout=True)
+    zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n // 2 + 1), indexing="ij")
+    vox = np.stack([zz.ravel(), yy.ravel(), xx.ravel()], -1)
+    v, ww, inside = O.insert_central_slices_3d_voxels(remap, s, w, (b, 1, n, n), (1, n, n, n), inv, vox, n_threads=4)
+    np.testing.assert_array_equal(v.reshape(vol.shape[1:]), vol[0])
+    np.testing.assert_array_equal(ww.reshape(vol.shape[1:]), wvol[0])
+    assert 0.3 < inside.mean() < 0.7
+    assert not np.any(v[~inside])
