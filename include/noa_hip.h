@@ -259,6 +259,60 @@ int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* str
 /* Replaces the GPU branch of noa::xform::extract_central_slices_3d (FourierProject.hpp:1869-1914). */
 int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream);
 
+/* ---- Batched calls (no counterpart in noa, whose only asynchrony is Stream::enqueue,
+ * runtime/cuda/Stream.hpp:205-239) --------------------------------------------------------------------------------
+ * Several extraction (or insertion) calls, each with its own arrays, box size, layout, slice count and cutoff, run
+ * as ONE persistent launch whose grid does not depend on the batch, so a launch can be captured once in a HIP graph
+ * and replayed for other batches. Each call is described by the same noa_hip_project_args as a single call and
+ * checked by the same rules; results equal those of the calls made one by one. A batch is made in two steps:
+ *   1. *_batch_prepare (host only, nothing launched): writes the batch's device image (header, per-call parameters,
+ *      work items) into host memory `image` of `capacity` bytes and its size into *bytes (with image == NULL or a
+ *      small capacity: only the size, and NOA_HIP_ERROR_INVALID_ARGUMENT).
+ *   2. the caller copies the image to device memory (aligned to 256 bytes), then *_batch_launch enqueues the
+ *      kernel(s) on `stream`; the image must stay unchanged until they have run.
+ * Batched extraction: complex FP32 volumes and slices (unit x strides), LINEAR, no weights, scaling, Ewald sphere,
+ * w window or shifts. Batched insertion: complex FP32 slice arrays (+ optional FP32 weight arrays) into complex FP32
+ * volumes (+ optional FP32 weights), LINEAR, no scaling or Ewald sphere; calls that write the same volume race.
+ * Grid sizes and pixels per lane come from the launch tuning (NOA_HIP_OP_EXTRACT_BATCH, NOA_HIP_OP_INSERT_BATCH). */
+int noa_hip_extract_batch_prepare(const noa_hip_project_args* calls, int32_t n_calls, void* image, int64_t capacity,
+                                  int64_t* bytes);
+int noa_hip_extract_batch_launch(const void* device_image, void* stream);
+int noa_hip_insert_batch_prepare(const noa_hip_project_args* calls, int32_t n_calls, void* image, int64_t capacity,
+                                 int64_t* bytes);
+int noa_hip_insert_batch_launch(const void* device_image, void* stream);
+
+/* ---- Launch tuning (replaces noa's compile-time launch options: IwiseOptions, runtime/Iwise.hpp:14-60, and the
+ * fixed 32x16 block reshaping of the CUDA iwise kernels, runtime/cuda/Iwise.cuh:94-175) ---------------------------
+ * A table of launch parameters keyed by (operator, box, layout, interpolation, volume type), filled by an autotuner
+ * (noa_amd.autotune) or by hand. A key field of -1 (box: 0) matches anything; the entry with the most exact matches
+ * wins, the later one on a tie. Values not set (0) take the defaults.
+ *   NOA_HIP_OP_EXTRACT:       values = {volume layout: 0 auto, 1 row-major gather, 2 block-copy gather, 3 LDS tiles;
+ *                                       expected crossing slices per tile work item (default 128); -; -}
+ *   NOA_HIP_OP_INSERT:        values = {kernel: 0 auto, 1 one-wave, 2 slab; expected slices per work item (default
+ *                                       256); 1: one load for both x taps of a slab row; -}
+ *   NOA_HIP_OP_EXTRACT_BATCH: values = {pixels per lane, 1-4 (default 3); workgroups (default 2048); -; -}
+ *   NOA_HIP_OP_INSERT_BATCH:  values = {workgroups (default 3072); -; -; -}
+ * Environment variables (NOA_HIP_EXTRACT_LAYOUT, NOA_HIP_EXTRACT_PAIRS, NOA_HIP_INSERT_KERNEL, NOA_HIP_INSERT_LIST,
+ * NOA_HIP_INSERT_PAIRED) override the table. */
+enum {
+    NOA_HIP_OP_EXTRACT = 1,
+    NOA_HIP_OP_INSERT = 2,
+    NOA_HIP_OP_EXTRACT_BATCH = 3,
+    NOA_HIP_OP_INSERT_BATCH = 4
+};
+typedef struct noa_hip_tuning {
+    int32_t op;          /* NOA_HIP_OP_* */
+    int32_t box;         /* logical size of the volume's first axis; 0 = any */
+    int32_t remap;       /* NOA_HIP_H2H, ...; -1 = any */
+    int32_t interp;      /* NOA_HIP_INTERP_*; -1 = any */
+    int32_t dtype;       /* value type of the volume; -1 = any */
+    int32_t values[4];
+} noa_hip_tuning;
+/* Replaces the table (count == 0 clears it). */
+int noa_hip_set_tuning(const noa_hip_tuning* entries, int32_t count);
+/* The values the table gives a key (zeros where nothing matches); returns 1 if an entry matched, else 0. */
+int noa_hip_get_tuning(int32_t op, int32_t box, int32_t remap, int32_t interp, int32_t dtype, int32_t values[4]);
+
 /* Replaces the GPU branch of noa::xform::insert_and_extract_central_slices_3d (FourierProject.hpp:1916-2049,
  * launcher :1489-1598, operator FourierInsertExtract :727-1006). Without a w window (w_fftfreq_blackman giving a
  * window of one sample) the output slices are OVERWRITTEN (0 beyond the cutoff) or, with add_to_output, added to;

@@ -197,6 +197,18 @@ class Shift2(ctypes.Structure):
     ]
 
 
+# Launch tuning (noa_hip_tuning): operators and the table entry.
+OP_EXTRACT = 1
+OP_INSERT = 2
+OP_EXTRACT_BATCH = 3
+OP_INSERT_BATCH = 4
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("op", c_int32), ("box", c_int32), ("remap", c_int32), ("interp", c_int32), ("dtype", c_int32),
+                ("values", c_int32 * 4)]
+
+
 def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     """Set the argument/return types of the exported functions."""
     P = ctypes.POINTER(ProjectArgs)
@@ -205,6 +217,18 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
         f = getattr(lib, name)
         f.argtypes = [P, c_void_p]
         f.restype = ctypes.c_int
+    for name in ("noa_hip_extract_batch_prepare", "noa_hip_insert_batch_prepare"):
+        f = getattr(lib, name)
+        f.argtypes = [P, c_int32, c_void_p, c_int64, ctypes.POINTER(c_int64)]
+        f.restype = ctypes.c_int
+    for name in ("noa_hip_extract_batch_launch", "noa_hip_insert_batch_launch"):
+        f = getattr(lib, name)
+        f.argtypes = [c_void_p, c_void_p]
+        f.restype = ctypes.c_int
+    lib.noa_hip_set_tuning.argtypes = [ctypes.POINTER(Tuning), c_int32]
+    lib.noa_hip_set_tuning.restype = ctypes.c_int
+    lib.noa_hip_get_tuning.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, ctypes.POINTER(c_int32)]
+    lib.noa_hip_get_tuning.restype = ctypes.c_int
     lib.noa_hip_last_error.argtypes = []
     lib.noa_hip_last_error.restype = ctypes.c_char_p
     lib.noa_hip_abi_version.argtypes = []

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <vector>
 #include <exception>
 #include <functional>
 #include <limits>
@@ -1085,7 +1087,8 @@ CtfNdDesc validate_ctf_nd(const noa_hip_view* in, const noa_hip_view* out, const
     return d;
 }
 
-int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr) {
+int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hip_shift2* shifts = nullptr,
+         bool no_alloc = false) {
     g_last_error.clear();
     if (!args) {
         g_last_error = "null arguments";
@@ -1120,7 +1123,7 @@ int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hi
     const hipStream_t s = static_cast<hipStream_t>(stream);
     switch (kind) {
         case Kind::RASTERIZE: status = launch_rasterize(d, s, err); break;
-        case Kind::INSERT: status = launch_insert(d, s, err); break;
+        case Kind::INSERT: status = launch_insert(d, s, err, no_alloc); break;
         case Kind::EXTRACT: status = launch_extract(d, s, err); break;
     }
     if (current != device) (void)hipSetDevice(current);
@@ -1128,7 +1131,52 @@ int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hi
     return status;
 }
 
+// Validates every call of a batch with the single-call rules; all calls on one device.
+int validate_batch(Kind kind, const noa_hip_project_args* calls, int32_t n_calls, std::vector<ProjectDesc>& descs,
+                   int& device) {
+    if (n_calls < 0 || (n_calls > 0 && !calls)) {
+        g_last_error = "batch: null calls or a negative count";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    descs.resize(static_cast<size_t>(n_calls));
+    device = -1;
+    for (int32_t k = 0; k < n_calls; ++k) {
+        try {
+            int dev = -1;
+            descs[static_cast<size_t>(k)] = validate(kind, calls[k], dev);
+            NOA_CHECK(device < 0 || dev == device, INVALID, "the calls of a batch must be on one device");
+            device = dev;
+        } catch (const Fail& f) {
+            g_last_error = "batch call " + std::to_string(k) + ": " + f.msg;
+            return f.code;
+        }
+    }
+    if (device < 0) (void)hipGetDevice(&device);
+    return NOA_HIP_OK;
+}
+
 } // namespace
+namespace {
+std::mutex g_tuning_mutex;
+std::vector<noa_hip_tuning> g_tuning;
+} // namespace
+
+bool tuning_lookup(int op, i64 box, int remap, int interp, int dtype, i32 values[4]) {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    int best = -1;
+    const noa_hip_tuning* found = nullptr;
+    for (const noa_hip_tuning& t : g_tuning) {
+        if (t.op != op) continue;
+        int score = 0;
+        if (t.box != 0) { if (t.box != box) continue; ++score; }
+        if (t.remap != -1) { if (t.remap != remap) continue; ++score; }
+        if (t.interp != -1) { if (t.interp != interp) continue; ++score; }
+        if (t.dtype != -1) { if (t.dtype != dtype) continue; ++score; }
+        if (score >= best) { best = score; found = &t; }
+    }
+    for (int i = 0; i < 4; ++i) values[i] = found ? found->values[i] : 0;
+    return found != nullptr;
+}
 } // namespace noa_hip
 
 extern "C" {
@@ -1164,6 +1212,83 @@ int noa_hip_rasterize_central_slices_3d(const noa_hip_project_args* args, void* 
 }
 int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* stream) {
     return noa_hip::call(noa_hip::Kind::INSERT, args, stream);
+}
+int noa_hip_extract_batch_prepare(const noa_hip_project_args* calls, int32_t n_calls, void* image, int64_t capacity,
+                                  int64_t* bytes) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!bytes) { g_last_error = "batch: null size pointer"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    *bytes = extract_batch_bytes(std::max<int32_t>(n_calls, 0));
+    std::vector<ProjectDesc> descs;
+    int device = -1;
+    const int st = validate_batch(Kind::EXTRACT, calls, n_calls, descs, device);
+    if (st != NOA_HIP_OK) return st;
+    i32 v[4];
+    tuning_lookup(NOA_HIP_OP_EXTRACT_BATCH, 0, -1, -1, -1, v);
+    const i32 ppt = v[0] > 0 ? v[0] : 3;
+    std::string err;
+    i64 b = 0;
+    const int status = extract_batch_prepare(descs.data(), n_calls, image, capacity, ppt, b, err);
+    *bytes = b;
+    if (status != NOA_HIP_OK) g_last_error = err;
+    return status;
+}
+int noa_hip_extract_batch_launch(const void* device_image, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!device_image) { g_last_error = "batch: null image"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    i32 v[4];
+    tuning_lookup(NOA_HIP_OP_EXTRACT_BATCH, 0, -1, -1, -1, v);
+    std::string err;
+    const int status = extract_batch_launch(device_image, v[1] > 0 ? v[1] : 2048, static_cast<hipStream_t>(stream), err);
+    if (status != NOA_HIP_OK) g_last_error = err;
+    return status;
+}
+int noa_hip_insert_batch_prepare(const noa_hip_project_args* calls, int32_t n_calls, void* image, int64_t capacity,
+                                 int64_t* bytes) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!bytes) { g_last_error = "batch: null size pointer"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    *bytes = 0;
+    std::vector<ProjectDesc> descs;
+    int device = -1;
+    const int st = validate_batch(Kind::INSERT, calls, n_calls, descs, device);
+    if (st != NOA_HIP_OK) return st;
+    return guarded(device, [&](std::string& err) {
+        i64 b = 0;
+        const int status = insert_batch_prepare(descs.data(), n_calls, image, capacity, b, err);
+        *bytes = b;
+        return status;
+    });
+}
+int noa_hip_insert_batch_launch(const void* device_image, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!device_image) { g_last_error = "batch: null image"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    i32 v[4];
+    tuning_lookup(NOA_HIP_OP_INSERT_BATCH, 0, -1, -1, -1, v);
+    std::string err;
+    const int status = insert_batch_launch(device_image, v[0] > 0 ? v[0] : 3072, static_cast<hipStream_t>(stream), err);
+    if (status != NOA_HIP_OK) g_last_error = err;
+    return status;
+}
+int noa_hip_set_tuning(const noa_hip_tuning* entries, int32_t count) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (count < 0 || (count > 0 && !entries)) { g_last_error = "tuning: null entries or a negative count"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    for (int32_t k = 0; k < count; ++k) {
+        if (entries[k].op < NOA_HIP_OP_EXTRACT || entries[k].op > NOA_HIP_OP_INSERT_BATCH) {
+            g_last_error = "tuning: unknown operator in entry " + std::to_string(k);
+            return NOA_HIP_ERROR_INVALID_ARGUMENT;
+        }
+    }
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    g_tuning.assign(entries, entries + count);
+    return NOA_HIP_OK;
+}
+int noa_hip_get_tuning(int32_t op, int32_t box, int32_t remap, int32_t interp, int32_t dtype, int32_t values[4]) {
+    if (!values) return 0;
+    return noa_hip::tuning_lookup(op, box, remap, interp, dtype, values) ? 1 : 0;
 }
 int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream) {
     return noa_hip::call(noa_hip::Kind::EXTRACT, args, stream);

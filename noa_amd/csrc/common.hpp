@@ -89,6 +89,19 @@ template<class C> __device__ __forceinline__ C bcast(C v, i32 lane) {
     }
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: shifts by 1, 2, 4, 8 inside each row of 16
+// lanes (lanes whose source falls outside the row add 0), then the last lane of rows 0 and 2 is added to rows 1
+// and 3 (row_bcast:15), and lane 31 to rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ i32 wave_inclusive_sum(i32 x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Index <-> frequency (fft/core/Frequency.hpp:80-87, :115-122).
 // ---------------------------------------------------------------------------------------------------------

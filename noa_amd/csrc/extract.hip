@@ -21,7 +21,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
 namespace noa_hip {
 
@@ -288,6 +292,356 @@ __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<fl
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Tiled extraction, for many slices from one volume (cubic volume and slices, Mat33<f32> rotations, no target
+// shape, cutoff <= 0.5; the lean path's types and options otherwise). The sample positions (floor of the flipped
+// coordinates, in samples) are cut into tiles of 16^3; with cutoff <= 0.5 every pixel inside the cutoff has its
+// floor within [lo - 1, hi + 1] along z and y and [0, n/2] along x, which the tiles cover. A work item is (tile,
+// range of slices), one workgroup of four waves:
+//   - The tile's 17^3 values (taps reach floor + 1; out-of-bounds positions hold 0, which is what noa's bounds
+//     checks add for them) are staged in LDS once.
+//   - Each wave takes 64 of the item's slices at a time; lane l tests whether slice l's plane crosses the tile, and
+//     the crossing slices are processed one at a time.
+//   - For a crossing slice, the pixels whose sample falls in the tile (directly, or mirrored by the x < 0 flip) are
+//     enumerated row by row: lanes 0-31 take rows of the direct set and lanes 32-63 rows of the mirrored one, each
+//     computes its row's pixel range from the tile's box (with a margin far above the rounding), a prefix sum packs
+//     the candidates into the wave's LDS list, and every candidate runs noa's expressions: the pixel is taken only
+//     where its own floor lies in this tile, so each pixel inside the cutoff is computed by exactly one item with
+//     the same arithmetic as the gather kernels (bit-identical), its taps read from LDS.
+//   - The pixels outside the cutoff (value 0, times the phase shift) are written by extract_outside_kernel.
+// The volume is read about once (plus the tiles' halo), instead of once per slice footprint.
+// ---------------------------------------------------------------------------------------------------------
+constexpr i32 XT = 16;                  // tile extent in floor positions along each axis
+constexpr i32 XTT = XT + 1;             // LDS extent along each axis
+constexpr i32 XT_LIST = 768;            // candidate pixels per wave and round
+constexpr i32 XT_WAVES = 4;
+
+struct TileParams {
+    const void* vol; i32 vs0, vs1;      // stored volume, unit x stride
+    float2* out; i64 os_b; i32 os_y;    // output slices, unit x stride
+    XformArg<float> rot;
+    const int4* items;                  // {tile, first slice, end slice, 0}
+    i32 n;                              // logical size of the volume's and the slices' axes
+    i32 ntz, nty, ntx;                  // tiles along z, y, x
+    i32 base;                           // floor position of the first tile along z and y
+    i32 n_slices, pixels, tiles_px;     // pixels per slice; workgroups of 256 pixels per slice (zero pass)
+    float inv_wr;
+    float cutoff_sqd, margin;
+    bool vol_centered, slice_centered;
+    const float* shifts;                // per-slice (y, x) shifts, or null (broadcast shift_f)
+    float shift_f[2], norm_f[2], cutoff_sqd_f;
+};
+
+template<int SHIFT>
+__device__ __forceinline__ float2 tile_shift(const TileParams& p, i32 b, i32 fy, i32 u, float2 value) {
+    if constexpr (SHIFT == 0) {
+        return value;
+    } else {
+        float sy = 0.f, sx = 0.f;
+        if constexpr (SHIFT == 1) {
+            sy = p.shifts ? p.shifts[2 * b] : p.shift_f[0];
+            sx = p.shifts ? p.shifts[2 * b + 1] : p.shift_f[1];
+        }
+        return cmul(value, phase_shift_at<float2, float>(SHIFT == 1 ? 0 : 2, sy, sx, fy, u, p.norm_f[0], p.norm_f[1],
+                                                          p.cutoff_sqd_f));
+    }
+}
+
+// Candidate word: (row frequency + 512) | pixel x << 10 | mirrored << 31.
+__device__ __forceinline__ u32 tile_word(i32 fy, i32 u, bool mirrored) {
+    return static_cast<u32>(fy + 512) | (static_cast<u32>(u) << 10) | (mirrored ? 0x80000000u : 0u);
+}
+
+template<int SHIFT, class S>
+__global__ __launch_bounds__(256) void extract_tile_kernel(const TileParams p) {
+    __shared__ float2 tile[XTT * XTT * XTT];
+    __shared__ u32 list[XT_WAVES][XT_LIST];
+
+    const int4 item = p.items[blockIdx.x];
+    const i32 first = item.y, end = min(item.z, p.n_slices);
+    const i32 tx = item.x % p.ntx, tzy = item.x / p.ntx;
+    const i32 ty = tzy % p.nty, tz = tzy / p.nty;
+    const i32 Z0 = p.base + tz * XT, Y0 = p.base + ty * XT, X0 = tx * XT;
+    const i32 n = p.n, lo = -n / 2, hi = (n - 1) / 2, hix = n / 2;
+
+    // Stage the tile: positions [Z0, Z0 + 16] x [Y0, Y0 + 16] x [X0, X0 + 16], zero outside the stored frequencies.
+    const S* __restrict__ vol = static_cast<const S*>(p.vol);
+    for (i32 i = static_cast<i32>(threadIdx.x); i < XTT * XTT * XTT; i += 256) {
+        const i32 dz = i / (XTT * XTT), r = i - dz * (XTT * XTT);
+        const i32 dy = r / XTT, dx = r - dy * XTT;
+        const i32 z = Z0 + dz, y = Y0 + dy, x = X0 + dx;
+        float2 v = make_float2(0.f, 0.f);
+        if (z >= lo && z <= hi && y >= lo && y <= hi && x <= hix)
+            v = load_as<float2>(vol + (frequency2index(p.vol_centered, z, n) * p.vs0 +
+                                       frequency2index(p.vol_centered, y, n) * p.vs1 + x));
+        tile[i] = v;
+    }
+    __syncthreads();
+
+    const i32 wave = static_cast<i32>(threadIdx.x) >> 6, lane = static_cast<i32>(threadIdx.x) & 63;
+    u32* wl = list[wave];
+    const float slice_f = static_cast<float>(n), target = static_cast<float>(n);
+    const float M = p.margin;
+    // Tile centre and half-extent in samples (floor positions [Z0, Z0 + 16) are samples in [Z0, Z0 + 16)).
+    const float cz = static_cast<float>(Z0) + 0.5f * XT, cy = static_cast<float>(Y0) + 0.5f * XT;
+    const float cx = static_cast<float>(X0) + 0.5f * XT;
+    Scale<float> sc;
+    sc.identity = true;
+
+    for (i32 c0 = first + wave * 64; c0 < end; c0 += 64 * XT_WAVES) {
+        // Does the plane of slice c0 + lane (spanned by the rotation's 2nd and 3rd columns) cross the tile box? The
+        // plane is symmetric through the origin, so this also decides the mirrored box.
+        bool cross = false;
+        if (c0 + lane < end) {
+            const Rot<float> r = load_rotation(p.rot, c0 + lane);
+            const float nz = r.m[4] * r.m[8] - r.m[7] * r.m[5];
+            const float ny = r.m[7] * r.m[2] - r.m[1] * r.m[8];
+            const float nx = r.m[1] * r.m[5] - r.m[4] * r.m[2];
+            const float d = fabsf((nz * cz + ny * cy) + nx * cx);
+            const float reach = (0.5f * XT) * (fabsf(nz) + fabsf(ny) + fabsf(nx));
+            cross = d <= reach + M * (fabsf(nz) + fabsf(ny) + fabsf(nx)) + 1e-6f;
+        }
+        unsigned long long mask = __ballot(cross);
+        while (mask) {
+            const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
+            mask &= mask - 1;
+            const i32 b = c0 + k;
+            const Rot<float> r = load_rotation(p.rot, b);
+            float2* __restrict__ out = p.out + static_cast<i64>(b) * p.os_b;
+
+            // Evaluates one candidate pixel: noa's expressions (as extract_c32_kernel), kept only when its floor is
+            // in this tile and its flip matches the set it was enumerated from.
+            auto evaluate = [&](i32 fy, i32 u, bool mirrored) {
+                vec2<float> f2;
+                f2.y = static_cast<float>(fy) / slice_f;
+                f2.x = static_cast<float>(u) / slice_f;
+                const vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
+                if (!(dot3(f) <= p.cutoff_sqd)) return;
+                float fz = f.z * target, fyv = f.y * target, fx = f.x * target;
+                Taps3<float2, float> t;
+                t.conj = 1;
+                const bool flip = fx < 0;
+                if (flip != mirrored) return;
+                if (flip) { fz *= -1; fyv *= -1; fx *= -1; t.conj = -1; }
+                const float flz = floorf(fz), fly = floorf(fyv), flx = floorf(fx);
+                const i32 lz = static_cast<i32>(flz) - Z0, ly = static_cast<i32>(fly) - Y0, lx = static_cast<i32>(flx) - X0;
+                if (static_cast<u32>(lz) >= static_cast<u32>(XT) || static_cast<u32>(ly) >= static_cast<u32>(XT) ||
+                    static_cast<u32>(lx) >= static_cast<u32>(XT))
+                    return;
+                t.tz = fz - flz;
+                t.ty = fyv - fly;
+                t.tx = fx - flx;
+                t.ok = 0xffu;
+                const i32 o = (lz * XTT + ly) * XTT + lx;
+#pragma unroll
+                for (int z = 0; z < 2; ++z)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+#pragma unroll
+                        for (int x = 0; x < 2; ++x) t.v[z][y][x] = tile[o + (z * XTT + y) * XTT + x];
+                float2 value = combine3<NOA_HIP_INTERP_LINEAR>(t);
+                value = tile_shift<SHIFT>(p, b, fy, u, value);
+                out[frequency2index(p.slice_centered, fy, n) * p.os_y + u] = value;
+            };
+
+            // Plane coordinates: F = Y a + U c (a, c: the 2nd and 3rd columns); on the plane Y = g . F with g the
+            // first row of (G^T G)^-1 G^T, G = [a c].
+            const float a[3] = {r.m[1], r.m[4], r.m[7]}, c[3] = {r.m[2], r.m[5], r.m[8]};
+            const float aa = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+            const float ac = (a[0] * c[0] + a[1] * c[1]) + a[2] * c[2];
+            const float cc = (c[0] * c[0] + c[1] * c[1]) + c[2] * c[2];
+            const float det = aa * cc - ac * ac;
+            if (!(det > 1e-6f * aa * cc)) {
+                // Degenerate transform (the plane collapses): every pixel of the slice is a candidate.
+                for (i32 pix = lane; pix < p.pixels; pix += 64) {
+                    const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
+                    const i32 u = pix - yq * (hix + 1);
+                    const i32 fy = index2frequency(p.slice_centered, yq, n);
+                    evaluate(fy, u, false);
+                    evaluate(fy, u, true);
+                }
+                continue;
+            }
+            const float idet = 1.f / det;
+            const float g[3] = {(cc * a[0] - ac * c[0]) * idet, (cc * a[1] - ac * c[1]) * idet,
+                                (cc * a[2] - ac * c[2]) * idet};
+            const bool mirrored = lane >= 32;
+            // The box of this lane's set, widened by the margin: direct F in [P0, P0 + 16), mirrored F in
+            // (-P0 - 16, -P0].
+            const float P0[3] = {static_cast<float>(Z0), static_cast<float>(Y0), static_cast<float>(X0)};
+            float blo[3], bhi[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                blo[i] = (mirrored ? -(P0[i] + XT) : P0[i]) - M;
+                bhi[i] = (mirrored ? -P0[i] : P0[i] + XT) + M;
+            }
+            float ymin = 0.f, ymax = 0.f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ymin += g[i] >= 0.f ? g[i] * blo[i] : g[i] * bhi[i];
+                ymax += g[i] >= 0.f ? g[i] * bhi[i] : g[i] * blo[i];
+            }
+            const i32 yfirst = max(static_cast<i32>(ceilf(fmaxf(ymin, static_cast<float>(lo) - 1.f))), lo);
+            const i32 ylast = min(static_cast<i32>(floorf(fminf(ymax, static_cast<float>(hi) + 1.f))), hi);
+            for (i32 rb = 0;; rb += 32) {
+                const i32 fy = yfirst + rb + (lane & 31);
+                const bool row = fy <= ylast;
+                if (__ballot(row) == 0) break;
+                // Pixel range of the row: U with Y a + U c inside the box.
+                float ulo = 0.f, uhi = static_cast<float>(hix);
+                bool any = row;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float base = a[i] * static_cast<float>(fy);
+                    if (fabsf(c[i]) > 1e-12f) {
+                        const float t1 = (blo[i] - base) / c[i], t2 = (bhi[i] - base) / c[i];
+                        ulo = fmaxf(ulo, fminf(t1, t2));
+                        uhi = fminf(uhi, fmaxf(t1, t2));
+                    } else if (base < blo[i] || base > bhi[i]) {
+                        any = false;
+                    }
+                }
+                i32 u0 = 0, cnt = 0;
+                if (any && ulo <= uhi) {
+                    u0 = static_cast<i32>(ceilf(ulo));
+                    cnt = max(0, static_cast<i32>(floorf(uhi)) - u0 + 1);
+                }
+                const i32 incl = wave_inclusive_sum(cnt), excl = incl - cnt;
+                const i32 total = __builtin_amdgcn_readlane(incl, 63);
+                for (i32 done = 0; done < total;) {
+                    // Whole rows into the list, as many as fit (rows are short: at most n/2 + 1 pixels).
+                    const bool fits = cnt > 0 && excl >= done && incl <= done + XT_LIST;
+                    const unsigned long long fm = __ballot(fits);
+                    const i32 last = 63 - __clzll(static_cast<long long>(fm));
+                    const i32 upto = __builtin_amdgcn_readlane(incl, last);
+                    if (fits)
+                        for (i32 q = 0; q < cnt; ++q) wl[excl - done + q] = tile_word(fy, u0 + q, mirrored);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (i32 e = lane; e < upto - done; e += 64) {
+                        const u32 w = wl[e];
+                        evaluate(static_cast<i32>(w & 1023u) - 512, static_cast<i32>((w >> 10) & 1023u), (w >> 31) != 0u);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    done = upto;
+                }
+            }
+        }
+    }
+}
+
+// The pixels outside the cutoff of the tiled extraction: 0, times the phase shift as the gather kernels compute it.
+template<int SHIFT>
+__global__ __launch_bounds__(256) void extract_outside_kernel(const TileParams p) {
+    const u32 bid = blockIdx.x;
+    const i32 b = static_cast<i32>(bid / static_cast<u32>(p.tiles_px));
+    const i32 pix = (static_cast<i32>(bid) - b * p.tiles_px) * 256 + static_cast<i32>(threadIdx.x);
+    if (pix >= p.pixels) return;
+    const i32 n = p.n, wr = n / 2 + 1;
+    const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
+    const i32 u = pix - yq * wr;
+    const i32 fy = index2frequency(p.slice_centered, yq, n);
+    const Rot<float> r = load_rotation(p.rot, b);
+    Scale<float> sc;
+    sc.identity = true;
+    vec2<float> f2;
+    f2.y = static_cast<float>(fy) / static_cast<float>(n);
+    f2.x = static_cast<float>(u) / static_cast<float>(n);
+    const vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
+    if (dot3(f) <= p.cutoff_sqd) return;   // computed by the tile kernel
+    p.out[static_cast<i64>(b) * p.os_b + yq * p.os_y + u] = tile_shift<SHIFT>(p, b, fy, u, vzero<float2>());
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Batched extraction: several extraction calls (each its own volume, box size, layout, slice count and cutoff) in
+// one persistent launch. The device image of a batch holds a header and one ExtractBatchJob per call; a call's
+// slices are cut into units of ppt * 256 consecutive pixels of one slice (ppt from the header), numbered across
+// the calls. The grid is fixed (any size works, so a launch can be captured once in a graph and replayed for
+// other batches): workgroup g takes the contiguous units [U g / G, U (g + 1) / G), the workgroups of one XCD
+// neighbouring ranges (mostly one volume per XCD L2). Each pixel runs the same expressions as extract_c32_kernel,
+// so the results are those of the calls made one by one.
+// ---------------------------------------------------------------------------------------------------------
+struct ExtractBatchJob {
+    const float2* vol; float2* out;
+    XformArg<float> rot;
+    i64 first_unit;                     // index of the call's first unit in the batch
+    i64 os_b; i32 os_y;                 // output strides (unit x stride)
+    i32 vs0, vs1;                       // volume strides (unit x stride)
+    i32 grid[3];                        // logical D, H, W of the volume
+    i32 slice_h, slice_wr, pixels, tiles;
+    float target[3], slice_f[2];
+    float cutoff_sqd, inv_wr;
+    i32 vol_centered, slice_centered;
+};
+struct ExtractBatchHeader {
+    i64 n_units;
+    i32 n_jobs, ppt;
+    i32 pad[12];                        // jobs start at byte 64
+};
+static_assert(sizeof(ExtractBatchHeader) == 64, "batch header");
+
+__global__ __launch_bounds__(256) void extract_batch_kernel(const unsigned char* __restrict__ image) {
+    const ExtractBatchHeader* h = reinterpret_cast<const ExtractBatchHeader*>(image);
+    const ExtractBatchJob* __restrict__ jobs = reinterpret_cast<const ExtractBatchJob*>(image + sizeof(ExtractBatchHeader));
+    const i64 U = h->n_units;
+    const i32 J = h->n_jobs, ppt = h->ppt;
+    const u32 g = xcd_swizzle(blockIdx.x, gridDim.x);
+    const i64 u_begin = U * g / gridDim.x, u_end = U * (g + 1) / gridDim.x;
+    if (u_begin >= u_end) return;
+    i32 lo = 0, hi = J - 1;                              // last job whose first unit is <= u_begin
+    while (lo < hi) {
+        const i32 mid = (lo + hi + 1) / 2;
+        if (jobs[mid].first_unit <= u_begin) lo = mid;
+        else hi = mid - 1;
+    }
+    i32 j = lo;
+    constexpr i32 MAXP = 4;
+    for (i64 u = u_begin; u < u_end; ++u) {
+        while (j + 1 < J && jobs[j + 1].first_unit <= u) ++j;
+        const ExtractBatchJob& q = jobs[j];
+        const i32 local = static_cast<i32>(u - q.first_unit);
+        const i32 b = local / q.tiles, tile = local - b * q.tiles;
+        const Rot<float> rot = load_rotation(q.rot, b);
+        Scale<float> sc;
+        sc.identity = true;
+        const Bounds3 bd = make_bounds3(q.grid[0], q.grid[1], q.grid[2]);
+        const bool vc = q.vol_centered != 0, scn = q.slice_centered != 0;
+        float2* __restrict__ out = q.out + static_cast<i64>(b) * q.os_b;
+        Taps3<float2, float> tv[MAXP];
+        bool inside[MAXP];
+        i32 ys[MAXP], us[MAXP];
+        const i32 first = tile * ppt * 256 + static_cast<i32>(threadIdx.x);
+#pragma unroll
+        for (i32 k = 0; k < MAXP; ++k) {
+            const i32 pix = first + k * 256;
+            const bool live = k < ppt && pix < q.pixels;
+            const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * q.inv_wr);
+            const i32 y = live ? yq : 0;
+            const i32 uu = live ? pix - yq * q.slice_wr : 0;
+            ys[k] = live ? y : -1;
+            us[k] = uu;
+            vec2<float> f2;
+            const i32 fy = index2frequency(scn, y, q.slice_h);
+            f2.y = static_cast<float>(fy) / q.slice_f[0];
+            f2.x = static_cast<float>(uu) / q.slice_f[1];
+            const vec3<float> f = slice2grid<false>(f2, sc, rot, 0.f, 0.f);
+            inside[k] = live && dot3(f) <= q.cutoff_sqd;
+            const float fz = f.z * q.target[0], fyv = f.y * q.target[1], fx = f.x * q.target[2];
+            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{q.vol}, q.vs0, q.vs1, 1, bd, vc, fz, fyv, fx,
+                                                          inside[k]);
+        }
+#pragma unroll
+        for (i32 k = 0; k < MAXP; ++k) {
+            if (ys[k] < 0) continue;
+            const float2 value = inside[k] ? combine3<NOA_HIP_INTERP_LINEAR>(tv[k]) : vzero<float2>();
+            out[ys[k] * q.os_y + us[k]] = value;
+        }
+    }
+}
+
 namespace {
 
 template<class C>
@@ -298,6 +652,71 @@ XformArg<C> to_arg(const Xform& x) {
     a.batched = x.batched ? 1 : 0;
     for (int i = 0; i < 9; ++i) a.single[i] = x.single[i];
     return a;
+}
+
+// Work items of the tiled extraction for one geometry: every tile that the cutoff sphere reaches, cut into ranges
+// of slices so that each item expects about `pairs` crossing slices (a plane through the origin with a uniformly
+// drawn normal crosses a box of half-extent h at distance r with probability min(1, 1.5 h / r)); items sorted by
+// expected work, largest first. Kept per (device, size, cutoff, slice count, pairs) for the process lifetime.
+struct TilePlan {
+    int4* items{};
+    i32 n_items{}, ntz{}, nty{}, ntx{}, base{};
+};
+std::mutex g_tile_mutex;
+std::map<std::tuple<int, i32, double, i32, i32>, TilePlan> g_tile_plans;
+
+int make_tile_plan(int device, i32 n, double cutoff, i32 n_slices, i32 pairs, TilePlan& plan, std::string& err) {
+    const auto key = std::make_tuple(device, n, cutoff, n_slices, pairs);
+    std::lock_guard<std::mutex> lock(g_tile_mutex);
+    auto it = g_tile_plans.find(key);
+    if (it != g_tile_plans.end()) {
+        plan = it->second;
+        return NOA_HIP_OK;
+    }
+    TilePlan pl;
+    const i32 lo = -n / 2, hi = (n - 1) / 2;
+    pl.base = lo - 1;
+    pl.ntz = pl.nty = (hi + 1 - (lo - 1) + 1 + XT - 1) / XT;
+    pl.ntx = (n / 2 + 1 + XT - 1) / XT;
+    const double rc = cutoff * n + 1.0;
+    struct Entry { double work; int4 item; };
+    std::vector<Entry> entries;
+    for (i32 tz = 0; tz < pl.ntz; ++tz)
+        for (i32 ty = 0; ty < pl.nty; ++ty)
+            for (i32 tx = 0; tx < pl.ntx; ++tx) {
+                const double lo3[3] = {double(pl.base + tz * XT), double(pl.base + ty * XT), double(tx * XT)};
+                double closest = 0, r2 = 0;
+                for (int a = 0; a < 3; ++a) {
+                    const double l = lo3[a], h = lo3[a] + XT;
+                    const double dd = l > 0 ? l : (h < 0 ? -h : 0.0);
+                    closest += dd * dd;
+                    const double c = l + 0.5 * XT;
+                    r2 += c * c;
+                }
+                if (closest > rc * rc) continue;
+                const double frac = std::min(1.0, 1.5 * 0.5 * XT / std::max(std::sqrt(r2), 1e-9));
+                const double expected = frac * n_slices;
+                const i32 n_seg = std::max<i32>(1, static_cast<i32>(std::ceil(expected / pairs)));
+                const i32 len = (n_slices + n_seg - 1) / n_seg;
+                const i32 t = (tz * pl.nty + ty) * pl.ntx + tx;
+                for (i32 sgm = 0; sgm < n_seg; ++sgm) {
+                    const i32 b = sgm * len, e = std::min(n_slices, b + len);
+                    if (b < e) entries.push_back({expected / n_seg + (e - b) / 64.0, make_int4(t, b, e, 0)});
+                }
+            }
+    std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
+    std::vector<int4> items(entries.size());
+    for (size_t k = 0; k < entries.size(); ++k) items[k] = entries[k].item;
+    pl.n_items = static_cast<i32>(items.size());
+    if (!items.empty() &&
+        (hipMalloc(&pl.items, items.size() * sizeof(int4)) != hipSuccess ||
+         hipMemcpy(pl.items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)) {
+        err = "extract: could not upload the tile plan";
+        return NOA_HIP_ERROR_RUNTIME;
+    }
+    g_tile_plans[key] = pl;
+    plan = pl;
+    return NOA_HIP_OK;
 }
 
 template<class C, class S, class M, class Out, class W>
@@ -372,6 +791,75 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                           (p.shift_mode < 0 || p.shift_mode == 1 || p.shift_mode == 2 || !p.shift_f64);
         if (lean) {
             p.inv_wr = 1.f / static_cast<float>(p.slice_wr);
+            // Tiled path (extract_tile_kernel): cubic volume and slices, no target shape, cutoff <= 0.5, Mat33<f32>
+            // rotations, and slices holding at least twice the volume's elements. NOA_HIP_EXTRACT_LAYOUT=tiles forces
+            // it where the configuration allows it, linear or bricks disables it.
+            // Launch tuning: volume layout and tile work-item size; NOA_HIP_EXTRACT_LAYOUT overrides the table.
+            i32 tv[4];
+            tuning_lookup(NOA_HIP_OP_EXTRACT, d.in_shape[1], (d.src_centered ? 4 : 0) + (d.dst_centered ? 8 : 0),
+                          d.interp, d.input.dtype, tv);
+            static const char* const LAYOUTS[4] = {nullptr, "linear", "bricks", "tiles"};
+            const char* layout_env = std::getenv("NOA_HIP_EXTRACT_LAYOUT");
+            if (!layout_env && tv[0] >= 1 && tv[0] <= 3) layout_env = LAYOUTS[tv[0]];
+            const i64 nn = p.grid[0];
+            const bool cubic = p.grid[1] == nn && p.grid[2] == nn && d.out_shape[2] == nn && d.out_shape[3] == nn &&
+                               d.target[0] == nn && d.target[1] == nn && d.target[2] == nn;
+            const i64 stored_elems = nn * nn * (nn / 2 + 1);
+            bool tiled = cubic && nn >= 16 && nn <= 1000 && static_cast<double>(cutoff) <= 0.5 && p.os_x == 1 &&
+                         p.rot.kind == NOA_HIP_MAT33_F32 && stored_elems < 0x7fffffffLL &&
+                         n_slices * static_cast<i64>(p.pixels) >= 2 * stored_elems;
+            if (layout_env && std::strcmp(layout_env, "tiles") != 0) tiled = false;
+            if (layout_env && std::strcmp(layout_env, "tiles") == 0)
+                tiled = cubic && nn >= 16 && nn <= 1000 && static_cast<double>(cutoff) <= 0.5 && p.os_x == 1 &&
+                        p.rot.kind == NOA_HIP_MAT33_F32 && stored_elems < 0x7fffffffLL;
+            const i64 tiles_px = (p.pixels + 255) / 256;
+            if (tiled && n_slices * tiles_px <= 0x7fffffffLL) {
+                int device = 0;
+                (void)hipGetDevice(&device);
+                // Plans are kept per slice count rounded up to m * 2^k, 8 <= m < 16 (items are clamped to the
+                // actual count in the kernel).
+                i64 step = 1;
+                while (n_slices > 16 * step) step *= 2;
+                const i32 plan_slices = static_cast<i32>((n_slices + step - 1) / step * step);
+                const char* pe = std::getenv("NOA_HIP_EXTRACT_PAIRS");
+                const i32 pairs = pe ? std::max(8, std::atoi(pe)) : (tv[1] > 0 ? std::max(8, tv[1]) : 128);
+                TilePlan plan;
+                const int st = make_tile_plan(device, static_cast<i32>(nn), static_cast<double>(cutoff), plan_slices,
+                                              pairs, plan, err);
+                if (st != NOA_HIP_OK) return st;
+                TileParams tp{};
+                tp.vol = p.vol; tp.vs0 = p.vs0; tp.vs1 = p.vs1;
+                tp.out = static_cast<float2*>(p.out); tp.os_b = p.os_b; tp.os_y = p.os_y;
+                tp.rot = p.rot;
+                tp.items = plan.items;
+                tp.n = static_cast<i32>(nn);
+                tp.ntz = plan.ntz; tp.nty = plan.nty; tp.ntx = plan.ntx; tp.base = plan.base;
+                tp.n_slices = static_cast<i32>(n_slices);
+                tp.pixels = p.pixels;
+                tp.tiles_px = static_cast<i32>(tiles_px);
+                tp.inv_wr = p.inv_wr;
+                tp.cutoff_sqd = static_cast<float>(p.cutoff_sqd);
+                tp.margin = 1e-3f + 1e-5f * static_cast<float>(nn);
+                tp.vol_centered = p.vol_centered;
+                tp.slice_centered = p.slice_centered;
+                tp.shifts = p.shift_mode == 0 && p.shift_batched ? static_cast<const float*>(p.shifts) : nullptr;
+                for (int i = 0; i < 2; ++i) { tp.shift_f[i] = p.shift_f[i]; tp.norm_f[i] = p.norm_f[i]; }
+                tp.cutoff_sqd_f = p.cutoff_sqd_f;
+                const dim3 gi(static_cast<u32>(plan.n_items)), go(static_cast<u32>(n_slices * tiles_px));
+                if (p.shift_mode == 0) {
+                    if (plan.n_items > 0) hipLaunchKernelGGL((extract_tile_kernel<1, S>), gi, block, 0, stream, tp);
+                    hipLaunchKernelGGL((extract_outside_kernel<1>), go, block, 0, stream, tp);
+                } else if (p.shift_mode == 2) {
+                    if (plan.n_items > 0) hipLaunchKernelGGL((extract_tile_kernel<2, S>), gi, block, 0, stream, tp);
+                    hipLaunchKernelGGL((extract_outside_kernel<2>), go, block, 0, stream, tp);
+                } else {
+                    if (plan.n_items > 0) hipLaunchKernelGGL((extract_tile_kernel<0, S>), gi, block, 0, stream, tp);
+                    hipLaunchKernelGGL((extract_outside_kernel<0>), go, block, 0, stream, tp);
+                }
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+                return NOA_HIP_OK;
+            }
             // Many slices from one volume: gather from a block copy of it (BrickLayout) (one read and one write of the
             // volume, then every trilinear gather touches fewer cache lines). Worth it once the slices hold about
             // twice the volume's elements; NOA_HIP_EXTRACT_LAYOUT=linear|bricks forces either.
@@ -379,7 +867,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             const i64 stored = static_cast<i64>(nz) * ny * wr;
             i64 bz, by, brick_elems;
             BrickLayout::strides(nz, ny, wr, bz, by, brick_elems);
-            const char* env = std::getenv("NOA_HIP_EXTRACT_LAYOUT");
+            const char* env = layout_env;
             bool bricks = n_slices * static_cast<i64>(p.pixels) >= 2 * stored;
             if (env && std::strcmp(env, "linear") == 0) bricks = false;
             if (env && std::strcmp(env, "bricks") == 0) bricks = true;
@@ -445,6 +933,81 @@ int dispatch(const ProjectDesc& d, hipStream_t stream, std::string& err) {
 
 int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     return d.coord_f64 ? dispatch<double>(d, stream, err) : dispatch<float>(d, stream, err);
+}
+
+i64 extract_batch_bytes(i32 n_jobs) {
+    return static_cast<i64>(sizeof(ExtractBatchHeader)) + static_cast<i64>(n_jobs) * static_cast<i64>(sizeof(ExtractBatchJob));
+}
+
+int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64 capacity, i32 ppt, i64& bytes,
+                          std::string& err) {
+    bytes = extract_batch_bytes(n_jobs);
+    if (ppt < 1 || ppt > 4) { err = "extract batch: pixels per lane must be 1 to 4"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    if (!image || capacity < bytes) {
+        err = "extract batch: the image needs " + std::to_string(bytes) + " bytes";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    std::vector<unsigned char> buf(static_cast<size_t>(bytes), 0);
+    auto* h = reinterpret_cast<ExtractBatchHeader*>(buf.data());
+    auto* jobs = reinterpret_cast<ExtractBatchJob*>(buf.data() + sizeof(ExtractBatchHeader));
+    i64 units = 0;
+    for (i32 k = 0; k < n_jobs; ++k) {
+        const ProjectDesc& d = descs[k];
+        const std::string at = "extract batch, call " + std::to_string(k) + ": ";
+        if (d.input.dtype != NOA_HIP_C32 || d.output.dtype != NOA_HIP_C32 || d.coord_f64 || d.general ||
+            d.interp != NOA_HIP_INTERP_LINEAR || d.input_weight.data || d.output_weight.data || d.input.s2 != 1 ||
+            d.output.s2 != 1 || d.shift.present) {
+            err = at + "only complex FP32 volumes and slices with unit x strides, LINEAR interpolation, single-precision "
+                       "transforms, no weights, scaling, Ewald sphere or shifts are batched";
+            return NOA_HIP_ERROR_UNSUPPORTED;
+        }
+        const float t0 = static_cast<float>(d.target[0]);
+        const float fb = std::max(static_cast<float>(d.fftfreq_blackman), 1 / t0);
+        if (blackman_window_size(static_cast<double>(fb), static_cast<double>(t0)) != 1) {
+            err = at + "a w-windowed sinc wider than one sample is not batched";
+            return NOA_HIP_ERROR_UNSUPPORTED;
+        }
+        ExtractBatchJob& q = jobs[k];
+        q.vol = static_cast<const float2*>(d.input.data);
+        q.out = static_cast<float2*>(d.output.data);
+        q.rot = to_arg<float>(d.rot);
+        q.first_unit = units;
+        q.os_b = d.output.s0;
+        q.os_y = static_cast<i32>(d.output.s1);
+        q.vs0 = static_cast<i32>(d.input.s0);
+        q.vs1 = static_cast<i32>(d.input.s1);
+        for (int i = 0; i < 3; ++i) {
+            q.grid[i] = static_cast<i32>(d.in_shape[i + 1]);
+            q.target[i] = static_cast<float>(d.target[i]);
+        }
+        q.slice_h = static_cast<i32>(d.out_shape[2]);
+        q.slice_wr = static_cast<i32>(d.out_shape[3] / 2 + 1);
+        q.pixels = q.slice_h * q.slice_wr;
+        if (q.pixels >= (1 << 21)) { err = at + "slices above 2^21 pixels are not batched"; return NOA_HIP_ERROR_UNSUPPORTED; }
+        q.tiles = (q.pixels + 256 * ppt - 1) / (256 * ppt);
+        q.slice_f[0] = static_cast<float>(d.out_shape[2]);
+        q.slice_f[1] = static_cast<float>(d.out_shape[3]);
+        const float cutoff = std::max(static_cast<float>(d.cutoff), 0.f);
+        q.cutoff_sqd = cutoff * cutoff;
+        q.inv_wr = 1.f / static_cast<float>(q.slice_wr);
+        q.vol_centered = d.src_centered ? 1 : 0;
+        q.slice_centered = d.dst_centered ? 1 : 0;
+        units += d.out_shape[0] * q.tiles;
+    }
+    h->n_units = units;
+    h->n_jobs = n_jobs;
+    h->ppt = ppt;
+    std::memcpy(image, buf.data(), buf.size());
+    return NOA_HIP_OK;
+}
+
+int extract_batch_launch(const void* image, i32 grid, hipStream_t stream, std::string& err) {
+    if (grid < 1) { err = "extract batch: the grid needs at least one workgroup"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    hipLaunchKernelGGL(extract_batch_kernel, dim3(static_cast<u32>(grid)), dim3(256), 0, stream,
+                       static_cast<const unsigned char*>(image));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+    return NOA_HIP_OK;
 }
 
 } // namespace noa_hip

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -116,18 +117,6 @@ __device__ __forceinline__ float window_poly(float fz, float inv_fb, const float
     return r;
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: shifts by 1, 2, 4, 8 inside each row of 16
-// lanes (lanes whose source falls outside the row add 0), then the last lane of rows 0 and 2 is added to rows 1
-// and 3 (row_bcast:15), and lane 31 to rows 2 and 3 (row_bcast:31).
-__device__ __forceinline__ i32 wave_inclusive_sum(i32 x) {
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
-    return x;
-}
 
 
 // ---------------------------------------------------------------------------------------------------------
@@ -305,16 +294,12 @@ __device__ __forceinline__ void combine_taps(const Taps<In, W, C>& t, bool has_w
 // polynomial window; 1: h2h, 2: hc2hc), which frees scalar registers, removes the uniform branches of the hot loop
 // and shares one row offset between the value and weight taps (same results).
 template<class C, class In, class Out, class W, int INTERP, int HOT = 0>
-__global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, In, W> p) {
+__device__ __forceinline__ void insert_wave_item(const InsertParams<C, In, W>& p, int4 item, In* __restrict__ acc_v,
+                                                 W* __restrict__ acc_w, i32* __restrict__ cand) {
     using RIn = typename vt<In>::real;
-    __shared__ In acc_v[WVOX];
-    __shared__ W acc_w[WVOX];
-    __shared__ i32 cand[64 * WG];          // packed candidates of one pass: column << 4 | k
-
     const i32 lane = static_cast<i32>(threadIdx.x);
-    // The plan is built for a power-of-two slice count: clamp the item's slice range to the actual count (an
-    // empty range still flushes, so a split brick's partial slot is always written).
-    int4 item = p.items[blockIdx.x];
+    // The plan is built for a rounded-up slice count: clamp the item's slice range to the actual count (an empty
+    // range still flushes, so a split brick's partial slot is always written).
     item.z = min(item.z, p.n_slices);
     item.y = min(item.y, item.z);
     const i32 brick = item.x;
@@ -596,6 +581,45 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
             W* w = p.wvol + (z * p.wvs0 + y * p.wvs1 + x * p.wvs2);
             *w = *w + acc_w[v];
         }
+    }
+}
+
+template<class C, class In, class Out, class W, int INTERP, int HOT = 0>
+__global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, In, W> p) {
+    __shared__ In acc_v[WVOX];
+    __shared__ W acc_w[WVOX];
+    __shared__ i32 cand[64 * WG];          // packed candidates of one pass: column << 4 | k
+    insert_wave_item<C, In, Out, W, INTERP, HOT>(p, p.items[blockIdx.x], acc_v, acc_w, cand);
+}
+
+// Batched insertion: several insertion calls (each its own volume, box size, layout, slice count and cutoff) in one
+// persistent launch of the one-wave kernel. The device image of a batch holds a header, one parameter block per call
+// and the work items of every call ({sub-brick, first slice, end slice, call}; the calls' slice ranges are not split,
+// so no partial sums); workgroup g takes the items g, g + G, ... (items of each call sorted by expected work, calls
+// interleaved by the host). Each item runs the one-wave kernel's code with its call's parameters: the results are
+// those of the calls made one by one (same per-voxel sums, in slice order).
+struct InsertBatchHeader {
+    i32 n_items, n_jobs;
+    i32 jobs_offset, items_offset;       // bytes from the start of the image
+    i32 pad[12];
+};
+static_assert(sizeof(InsertBatchHeader) == 64, "batch header");
+
+__global__ __launch_bounds__(64) void insert_batch_kernel(const unsigned char* __restrict__ image) {
+    using P = InsertParams<float, float2, float>;
+    __shared__ float2 acc_v[WVOX];
+    __shared__ float acc_w[WVOX];
+    __shared__ i32 cand[64 * WG];
+    const InsertBatchHeader* h = reinterpret_cast<const InsertBatchHeader*>(image);
+    const P* __restrict__ jobs = reinterpret_cast<const P*>(image + h->jobs_offset);
+    const int4* __restrict__ items = reinterpret_cast<const int4*>(image + h->items_offset);
+    const i32 n_items = h->n_items;
+    for (i32 i = static_cast<i32>(blockIdx.x); i < n_items; i += static_cast<i32>(gridDim.x)) {
+        int4 item = items[i];
+        const P& p = jobs[item.w];
+        item.w = -1;
+        __syncthreads();   // the previous item's flush has read the accumulator
+        insert_wave_item<float, float2, float2, float, NOA_HIP_INTERP_LINEAR, 0>(p, item, acc_v, acc_w, cand);
     }
 }
 
@@ -1165,22 +1189,50 @@ WindowFit fit_window(float fs, float fb) {
 // slabs, t = f_b + mean projected half-extent. Items are sorted by expected work, largest first. Kept per (device,
 // geometry, brick shape, slice count, window) for the process lifetime.
 struct Plan {
-    int4* items{};
+    int4* items{};                      // device copies, uploaded on first use by the allocating entry point
     int4* reduce{};
     i32 n_items{}, n_reduce{}, n_partials{};
+    std::vector<int4> h_items, h_reduce;   // host copies (the allocation-free entry point copies them to its scratch)
+    std::vector<float> h_work;             // expected passing slices of each item
 };
 using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double, i32, i32, i32, double>;
 std::mutex g_plan_mutex;
 std::map<PlanKey, Plan> g_plans;
 
+// Device copies of a plan's items and reductions (once per plan; called with g_plan_mutex held).
+int upload_plan(Plan& pl, std::string& err) {
+    if (!pl.h_items.empty() && !pl.items) {
+        if (hipMalloc(&pl.items, pl.h_items.size() * sizeof(int4)) != hipSuccess ||
+            hipMemcpy(pl.items, pl.h_items.data(), pl.h_items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "insert: could not upload the work plan";
+            return NOA_HIP_ERROR_RUNTIME;
+        }
+    }
+    if (!pl.h_reduce.empty() && !pl.reduce) {
+        if (hipMalloc(&pl.reduce, pl.h_reduce.size() * sizeof(int4)) != hipSuccess ||
+            hipMemcpy(pl.reduce, pl.h_reduce.data(), pl.h_reduce.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "insert: could not upload the work plan";
+            return NOA_HIP_ERROR_RUNTIME;
+        }
+    }
+    return NOA_HIP_OK;
+}
+
+// The plan of one geometry, built on first use and kept (map nodes are stable, so `plan` stays valid). `upload`:
+// also make the device copies (the allocation-free entry point only reads the host copies).
 int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff, bool centered, i32 n_slices,
-              double fftfreq_blackman, const i32 bd[3], double list_target, Plan& plan, std::string& err) {
+              double fftfreq_blackman, const i32 bd[3], double list_target, bool upload, const Plan*& plan,
+              std::string& err) {
     const PlanKey key{device, grid[0], grid[1], grid[2], target[0], target[1], target[2], cutoff, centered, n_slices,
                       fftfreq_blackman, bd[0], bd[1], bd[2], list_target};
     std::lock_guard<std::mutex> lock(g_plan_mutex);
     auto it = g_plans.find(key);
     if (it != g_plans.end()) {
-        plan = it->second;
+        if (upload) {
+            const int st = upload_plan(it->second, err);
+            if (st != NOA_HIP_OK) return st;
+        }
+        plan = &it->second;
         return NOA_HIP_OK;
     }
     const i32 ext[3] = {grid[0], grid[1], grid[2] / 2 + 1};
@@ -1233,28 +1285,20 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
             }
     std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
     std::vector<int4> items(entries.size());
-    for (size_t k = 0; k < entries.size(); ++k) items[k] = entries[k].item;
-    Plan pl;
+    std::vector<float> work(entries.size());
+    for (size_t k = 0; k < entries.size(); ++k) {
+        items[k] = entries[k].item;
+        work[k] = static_cast<float>(entries[k].work);
+    }
+    Plan& pl = g_plans[key];
     pl.n_items = static_cast<i32>(items.size());
     pl.n_reduce = static_cast<i32>(reduce.size());
     pl.n_partials = n_partials;
-    if (!items.empty()) {
-        if (hipMalloc(&pl.items, items.size() * sizeof(int4)) != hipSuccess ||
-            hipMemcpy(pl.items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
-            err = "insert: could not upload the work plan";
-            return NOA_HIP_ERROR_RUNTIME;
-        }
-    }
-    if (!reduce.empty()) {
-        if (hipMalloc(&pl.reduce, reduce.size() * sizeof(int4)) != hipSuccess ||
-            hipMemcpy(pl.reduce, reduce.data(), reduce.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
-            err = "insert: could not upload the work plan";
-            return NOA_HIP_ERROR_RUNTIME;
-        }
-    }
-    g_plans[key] = pl;
-    plan = pl;
-    return NOA_HIP_OK;
+    pl.h_items = std::move(items);
+    pl.h_work = std::move(work);
+    pl.h_reduce = std::move(reduce);
+    plan = &pl;
+    return upload ? upload_plan(pl, err) : NOA_HIP_OK;
 }
 
 // Culled kernel of the insertion: the slab kernel where the configuration allows it, else the one-wave kernel
@@ -1264,18 +1308,18 @@ struct KernelChoice {
     bool paired;          // slab kernel: one 16-byte (8-byte) load for the two x taps of a slice (weight) row
     double list_target;
 };
-KernelChoice kernel_choice() {
-    const KernelChoice c = [] {
-        KernelChoice k{};
-        const char* e = std::getenv("NOA_HIP_INSERT_KERNEL");
-        k.force_wave = e && std::string(e) == "wave";
-        const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED");
-        k.paired = pr && std::string(pr) == "1";
-        const char* l = std::getenv("NOA_HIP_INSERT_LIST");
-        k.list_target = l ? std::max(16.0, std::atof(l)) : 256.0;
-        return k;
-    }();
-    return c;
+KernelChoice kernel_choice(const ProjectDesc& d) {
+    KernelChoice k{};
+    i32 v[4];
+    const int remap = (d.src_centered ? 4 : 0) + (d.dst_centered ? 8 : 0);
+    tuning_lookup(NOA_HIP_OP_INSERT, d.out_shape[1], remap, d.interp, d.output.dtype, v);
+    k.force_wave = v[0] == 1;
+    k.list_target = v[1] > 0 ? std::max(16.0, static_cast<double>(v[1])) : 256.0;
+    k.paired = v[2] == 1;
+    if (const char* e = std::getenv("NOA_HIP_INSERT_KERNEL")) k.force_wave = std::string(e) == "wave";
+    if (const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED")) k.paired = std::string(pr) == "1";
+    if (const char* l = std::getenv("NOA_HIP_INSERT_LIST")) k.list_target = std::max(16.0, std::atof(l));
+    return k;
 }
 
 // Window of the slab kernel: windowed_sinc(pz / n, fs, fb) (FourierProject.hpp:99-110) as a polynomial of degree
@@ -1340,9 +1384,18 @@ SlabWindow fit_slab_window(float fs, float fb, i32 n) {
     return sw;
 }
 
-template<class C, class In, class Out, class W>
-int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
-    InsertParams<C, In, W> p{};
+// Device scratch of the allocation-free entry point: the work plan, the slice records and the partial sums of one
+// call are carved from it (256-byte aligned pieces) instead of being allocated.
+struct Scratch {
+    char* base;
+    size_t bytes;
+};
+
+// The kernel parameters of one call (everything but the work plan and the partial sums); returns whether the slice
+// extents fit the 32-bit buffer loads of the culled kernels.
+template<class C, class In, class W>
+bool fill_params(const ProjectDesc& d, InsertParams<C, In, W>& p) {
+    p = InsertParams<C, In, W>{};
     p.in = static_cast<const In*>(d.input.data);
     p.is_b = d.input.s0; p.is_y = (i32)d.input.s1; p.is_x = (i32)d.input.s2;
     if (d.in_shape[0] == 1) p.is_b = 0;   // batch broadcast of the interpolator input (xform/Utils.hpp:25-28)
@@ -1364,7 +1417,6 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
         p.inv_target[i] = C(1) / p.target[i];
     }
     p.wr = p.grid[2] / 2 + 1;
-    const KernelChoice kc = kernel_choice();
     p.bd[0] = WZ;
     p.bd[1] = WY;
     p.bd[2] = WX;
@@ -1377,7 +1429,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     p.slice_shape[0] = static_cast<i32>(d.in_shape[2]);
     p.slice_shape[1] = static_cast<i32>(d.in_shape[3]);
     p.n_slices = static_cast<i32>(d.in_shape[0]);
-    C cutoff = std::max(static_cast<C>(d.cutoff), C(0));
+    const C cutoff = std::max(static_cast<C>(d.cutoff), C(0));
     p.cutoff_sqd = cutoff * cutoff;
     // Clamp the windowed sinc to at least one pixel (:471-474).
     const C max_output_size = static_cast<C>(std::min({d.target[0], d.target[1], d.target[2]}));
@@ -1400,8 +1452,6 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     }
     p.slice_begin = 0;
     p.slice_end = p.n_slices;
-    if (p.n_slices == 0) return NOA_HIP_OK;
-    const bool nearest = d.interp == NOA_HIP_INTERP_NEAREST;
 
     // Byte extents of one slice and of its weights, for the buffer loads of the one-wave kernel.
     auto extent = [&](i64 sy, i64 sx, size_t elem) -> u64 {
@@ -1413,29 +1463,74 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
     const bool buffers_ok = sb < 0xffffffffULL && wb < 0xffffffffULL;
     p.slice_bytes = buffers_ok ? static_cast<u32>(sb) : 0;
     p.wslice_bytes = buffers_ok ? static_cast<u32>(wb) : 0;
+    return buffers_ok;
+}
+
+template<class C, class In, class Out, class W>
+int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratch* scratch) {
+    InsertParams<C, In, W> p;
+    const bool buffers_ok = fill_params(d, p);
+    if (p.n_slices == 0) return NOA_HIP_OK;
+    const KernelChoice kc = kernel_choice(d);
+    const C cutoff = std::max(static_cast<C>(d.cutoff), C(0));
+    const bool nearest = d.interp == NOA_HIP_INTERP_NEAREST;
 
     constexpr bool single = sizeof(typename vt<In>::real) == 4 && sizeof(W) == 4;
     if constexpr (single) {
         if (!p.general && !p.in_is_value && !p.win_is_value && buffers_ok && window_size(d.interp) == 2) {
             int device = 0;
             (void)hipGetDevice(&device);
-            Plan plan;
-            // Plans are kept per power-of-two slice count (items are clamped to the actual count in the kernel), so
-            // that callers with varying batch sizes reuse a handful of plans.
-            i32 plan_slices = 64;
-            while (plan_slices < p.n_slices) plan_slices *= 2;
+            const Plan* planp = nullptr;
+            // Plans are kept per slice count rounded up to m * 2^k (8 <= m < 16, at least 64; items are clamped to
+            // the actual count in the kernel), so that callers with varying batch sizes reuse a few plans and at most
+            // 1/8 of a plan's slice range is empty.
+            i32 step = 8;
+            while (p.n_slices > 16 * step) step *= 2;
+            const i32 plan_slices = std::max<i32>(64, (p.n_slices + step - 1) / step * step);
             const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
-                                     plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target, plan,
-                                     err);
+                                     plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target,
+                                     scratch == nullptr, planp, err);
             if (st != NOA_HIP_OK) return st;
+            const Plan& plan = *planp;
             if (plan.n_items == 0) return NOA_HIP_OK;
-            p.items = plan.items;
-            p.reduce = plan.reduce;
+            const size_t partial_bytes = static_cast<size_t>(plan.n_partials) * p.bd[0] * p.bd[1] * p.apitch * 3 *
+                                         sizeof(float);
+            const size_t record_bytes = static_cast<size_t>(p.n_slices) * 4 * sizeof(float4);
             void* partials = nullptr;
-            if (plan.n_partials > 0) {
-                const size_t bytes = static_cast<size_t>(plan.n_partials) * p.bd[0] * p.bd[1] * p.apitch * 3 *
-                                     sizeof(float);
-                if (hipMallocAsync(&partials, bytes, stream) != hipSuccess) {
+            void* records = nullptr;     // slab kernel only
+            if (scratch) {
+                // Allocation-free: plan, records and partial sums in the caller-independent scratch, in that order.
+                auto round = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+                const size_t items_bytes = round(plan.h_items.size() * sizeof(int4));
+                const size_t reduce_bytes = round(plan.h_reduce.size() * sizeof(int4));
+                const size_t need = items_bytes + reduce_bytes + round(record_bytes) + round(partial_bytes);
+                if (need > scratch->bytes) {
+                    err = "insert (no allocation): this call needs " + std::to_string(need) +
+                          " bytes of scratch, more than the " + std::to_string(scratch->bytes) +
+                          "-byte maximum; insert fewer slices per call";
+                    return NOA_HIP_ERROR_UNSUPPORTED;
+                }
+                char* at = scratch->base;
+                int4* items = reinterpret_cast<int4*>(at);
+                at += items_bytes;
+                int4* reduce = plan.h_reduce.empty() ? nullptr : reinterpret_cast<int4*>(at);
+                at += reduce_bytes;
+                records = at;
+                at += round(record_bytes);
+                partials = plan.n_partials > 0 ? at : nullptr;
+                p.items = items;
+                p.reduce = reduce;
+                if (hipMemcpyAsync(items, plan.h_items.data(), plan.h_items.size() * sizeof(int4),
+                                   hipMemcpyHostToDevice, stream) != hipSuccess ||
+                    (reduce && hipMemcpyAsync(reduce, plan.h_reduce.data(), plan.h_reduce.size() * sizeof(int4),
+                                              hipMemcpyHostToDevice, stream) != hipSuccess)) {
+                    err = "insert (no allocation): could not copy the work plan to the scratch";
+                    return NOA_HIP_ERROR_RUNTIME;
+                }
+            } else {
+                p.items = plan.items;
+                p.reduce = plan.reduce;
+                if (plan.n_partials > 0 && hipMallocAsync(&partials, partial_bytes, stream) != hipSuccess) {
                     err = "insert: could not allocate the partial sums";
                     return NOA_HIP_ERROR_RUNTIME;
                 }
@@ -1466,9 +1561,8 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                     slab = sw.ok && sw.fbT <= 1.15f;
                 }
             }
-            void* records = nullptr;
             if (slab) {
-                if (hipMallocAsync(&records, static_cast<size_t>(p.n_slices) * 4 * sizeof(float4), stream) != hipSuccess) {
+                if (!scratch && hipMallocAsync(&records, record_bytes, stream) != hipSuccess) {
                     if (partials) (void)hipFreeAsync(partials, stream);
                     err = "insert: could not allocate the slice records";
                     return NOA_HIP_ERROR_RUNTIME;
@@ -1485,7 +1579,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                     sp.wvol = p.has_wvol ? p.wvol : nullptr; sp.wvs0 = p.wvs0; sp.wvs1 = p.wvs1; sp.wvs2 = p.wvs2;
                     sp.n = p.grid[0];
                     for (int i = 0; i < 3; ++i) sp.nb[i] = p.nb[i];
-                    sp.items = plan.items;
+                    sp.items = p.items;
                     sp.partials = p.partials;
                     sp.rec = static_cast<const float4*>(records);
                     sp.n_slices = p.n_slices;
@@ -1521,7 +1615,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                 hipLaunchKernelGGL((insert_reduce_kernel<C, In, Out, W>), dim3(plan.n_reduce), dim3(NT_REDUCE), 0, stream, p);
             hipError_t e = hipGetLastError();
             for (void* buf : {partials, records}) {
-                if (!buf) continue;
+                if (!buf || scratch) continue;
                 const hipError_t fe = hipFreeAsync(buf, stream);
                 if (e == hipSuccess) e = fe;
             }
@@ -1547,22 +1641,97 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
 }
 
 template<class C>
-int dispatch(const ProjectDesc& d, hipStream_t stream, std::string& err) {
+int dispatch(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratch* sc) {
     const int in = d.input.dtype, out = d.output.dtype;
-    if (in == NOA_HIP_C32 && out == NOA_HIP_C32) return run<C, float2, float2, float>(d, stream, err);
-    if (in == NOA_HIP_C32 && out == NOA_HIP_F32) return run<C, float2, float, float>(d, stream, err);
-    if (in == NOA_HIP_F32 && out == NOA_HIP_F32) return run<C, float, float, float>(d, stream, err);
-    if (in == NOA_HIP_C64 && out == NOA_HIP_C64) return run<C, double2, double2, double>(d, stream, err);
-    if (in == NOA_HIP_C64 && out == NOA_HIP_F64) return run<C, double2, double, double>(d, stream, err);
-    if (in == NOA_HIP_F64 && out == NOA_HIP_F64) return run<C, double, double, double>(d, stream, err);
+    if (in == NOA_HIP_C32 && out == NOA_HIP_C32) return run<C, float2, float2, float>(d, stream, err, sc);
+    if (in == NOA_HIP_C32 && out == NOA_HIP_F32) return run<C, float2, float, float>(d, stream, err, sc);
+    if (in == NOA_HIP_F32 && out == NOA_HIP_F32) return run<C, float, float, float>(d, stream, err, sc);
+    if (in == NOA_HIP_C64 && out == NOA_HIP_C64) return run<C, double2, double2, double>(d, stream, err, sc);
+    if (in == NOA_HIP_C64 && out == NOA_HIP_F64) return run<C, double2, double, double>(d, stream, err, sc);
+    if (in == NOA_HIP_F64 && out == NOA_HIP_F64) return run<C, double, double, double>(d, stream, err, sc);
     err = "insert: unsupported input/output value types";
     return NOA_HIP_ERROR_UNSUPPORTED;
 }
 
 } // namespace
 
-int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err) {
-    return d.coord_f64 ? dispatch<double>(d, stream, err) : dispatch<float>(d, stream, err);
+int insert_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64 capacity, i64& bytes,
+                         std::string& err) {
+    using P = InsertParams<float, float2, float>;
+    auto round = [](i64 b) { return (b + 255) & ~static_cast<i64>(255); };
+    std::vector<P> params(static_cast<size_t>(n_jobs));
+    struct Item { float work; int4 item; };
+    std::vector<Item> items;
+    int device = 0;
+    (void)hipGetDevice(&device);
+    for (i32 k = 0; k < n_jobs; ++k) {
+        const ProjectDesc& d = descs[k];
+        const std::string at = "insert batch, call " + std::to_string(k) + ": ";
+        const bool weights_ok = !d.input_weight.present() || (d.input_weight.dtype == NOA_HIP_F32 && !d.input_weight.is_value);
+        const bool wvol_ok = !d.output_weight.data || d.output_weight.dtype == NOA_HIP_F32;
+        if (d.input.dtype != NOA_HIP_C32 || d.input.is_value || d.output.dtype != NOA_HIP_C32 || !weights_ok ||
+            !wvol_ok || d.coord_f64 || d.general || d.interp != NOA_HIP_INTERP_LINEAR) {
+            err = at + "only complex FP32 slices (arrays) into complex FP32 volumes, FP32 weights, LINEAR "
+                       "interpolation and single-precision transforms without scaling or Ewald sphere are batched";
+            return NOA_HIP_ERROR_UNSUPPORTED;
+        }
+        P& p = params[static_cast<size_t>(k)];
+        if (!fill_params(d, p)) { err = at + "the slices exceed the 4 GiB of a buffer load"; return NOA_HIP_ERROR_UNSUPPORTED; }
+        if (p.n_slices == 0) continue;
+        i32 step = 8;
+        while (p.n_slices > 16 * step) step *= 2;
+        const i32 plan_slices = std::max<i32>(64, (p.n_slices + step - 1) / step * step);
+        const Plan* plan = nullptr;
+        // One segment per sub-brick (no partial sums): list target above any slice count.
+        const int st = make_plan(device, p.grid, d.target, static_cast<double>(std::max(static_cast<float>(d.cutoff), 0.f)),
+                                 p.vol_centered, plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, 1e30, false,
+                                 plan, err);
+        if (st != NOA_HIP_OK) return st;
+        for (size_t i = 0; i < plan->h_items.size(); ++i) {
+            int4 it = plan->h_items[i];
+            it.w = k;
+            items.push_back({plan->h_work[i], it});
+        }
+    }
+    // Largest expected work first, across the calls.
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.work > b.work; });
+    const i64 jobs_offset = round(static_cast<i64>(sizeof(InsertBatchHeader)));
+    const i64 items_offset = round(jobs_offset + static_cast<i64>(sizeof(P)) * n_jobs);
+    bytes = items_offset + static_cast<i64>(items.size()) * static_cast<i64>(sizeof(int4));
+    if (!image || capacity < bytes) {
+        err = "insert batch: the image needs " + std::to_string(bytes) + " bytes";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    if (items_offset > 0x7fffffffLL || items.size() > 0x7fffffffULL) { err = "insert batch: too many calls or items"; return NOA_HIP_ERROR_UNSUPPORTED; }
+    unsigned char* out = static_cast<unsigned char*>(image);
+    std::memset(out, 0, static_cast<size_t>(items_offset));
+    InsertBatchHeader h{};
+    h.n_items = static_cast<i32>(items.size());
+    h.n_jobs = n_jobs;
+    h.jobs_offset = static_cast<i32>(jobs_offset);
+    h.items_offset = static_cast<i32>(items_offset);
+    std::memcpy(out, &h, sizeof(h));
+    if (n_jobs > 0) std::memcpy(out + jobs_offset, params.data(), sizeof(P) * static_cast<size_t>(n_jobs));
+    int4* dst = reinterpret_cast<int4*>(out + items_offset);
+    for (size_t i = 0; i < items.size(); ++i) dst[i] = items[i].item;
+    return NOA_HIP_OK;
+}
+
+int insert_batch_launch(const void* image, i32 grid, hipStream_t stream, std::string& err) {
+    if (grid < 1) { err = "insert batch: the grid needs at least one workgroup"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    hipLaunchKernelGGL(insert_batch_kernel, dim3(static_cast<u32>(grid)), dim3(64), 0, stream,
+                       static_cast<const unsigned char*>(image));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+    return NOA_HIP_OK;
+}
+
+int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err, bool no_alloc) {
+    if (no_alloc) {
+        err = "insert: no allocation-free path";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    }
+    return d.coord_f64 ? dispatch<double>(d, stream, err, nullptr) : dispatch<float>(d, stream, err, nullptr);
 }
 
 } // namespace noa_hip

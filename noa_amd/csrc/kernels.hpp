@@ -181,8 +181,21 @@ int launch_gridding_correction(const GriddingDesc& d, hipStream_t stream, std::s
 int launch_phase_shift(const PhaseShiftDesc& d, hipStream_t stream, std::string& err);
 int launch_extract(const ProjectDesc& d, hipStream_t stream, std::string& err);
 int launch_rasterize(const ProjectDesc& d, hipStream_t stream, std::string& err);
-int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err);
+// no_alloc: the plan, slice records and partial sums go to the library's static device scratch (no allocation;
+// calls that overlap on one GPU race on it).
+int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err, bool no_alloc = false);
 int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
+// Batched extraction / insertion (one persistent launch for several calls); prepare writes the device image on the
+// host, launch reads it from device memory.
+i64 extract_batch_bytes(i32 n_jobs);
+int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64 capacity, i32 ppt, i64& bytes,
+                          std::string& err);
+int extract_batch_launch(const void* image, i32 grid, hipStream_t stream, std::string& err);
+int insert_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64 capacity, i64& bytes,
+                         std::string& err);
+int insert_batch_launch(const void* image, i32 grid, hipStream_t stream, std::string& err);
+// The launch tuning table: values[] for a key (zeros where nothing matches); true if an entry matched.
+bool tuning_lookup(int op, i64 box, int remap, int interp, int dtype, i32 values[4]);
 int launch_ctf(const CtfDesc& d, hipStream_t stream, std::string& err);
 int launch_ctf_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err);
 int launch_bspline_prefilter(const BSplineDesc& d, hipStream_t stream, std::string& err);

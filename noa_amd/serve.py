@@ -1,14 +1,18 @@
 """Continuous-batching request loop for central-slice projection (extraction) and backprojection (insertion).
 
-Reference volumes stay resident in HBM. Requests arrive over time; the loop admits them while an HBM budget for
-their outputs allows, merges the admitted extraction requests that read the same volume into one launch, gives each
-insertion request its own accumulator, and launches on several streams so that independent work overlaps on the
-device. A request completes when the event recorded after its launch has passed; its latency is completion time
-minus arrival time. Requests that arrive while others run are admitted as soon as budget frees up: batching is
-continuous, not a queue drained in rounds.
+Reference volumes stay resident in HBM. Requests arrive over time. Each request's output (extracted slices, or the
+accumulator volume and weights of an insertion) is carved from one preallocated device pool: the pool is the HBM
+budget, and a request is admitted when its output fits (a request larger than the whole pool runs alone, with its
+own allocation, when nothing else is in flight). Every step, the admitted requests that are not running yet become
+one batch: all their extraction calls run as one persistent launch and all their insertion calls as another
+(noa_amd.batch), replayed from a HIP graph captured once per slot. Several slots (each its own stream, images and
+graph) keep batches in flight while the next one is formed, so requests that arrive while others are on the GPU
+are admitted as soon as memory and a slot free up: batching is continuous, not a queue drained in rounds. Nothing
+is allocated on the device per request or per batch (outputs come from the pool, accumulators are cleared there).
 
-Results do not depend on the mix: an extraction slice is computed independently of the other slices of its launch,
-and an insertion request accumulates into its own volume with a deterministic kernel.
+Results do not depend on the mix: an extracted slice is computed independently of the other work of its launch,
+and an insertion request accumulates into its own volume with a deterministic kernel. A request completes when the
+event recorded after its batch has passed; its latency is completion minus arrival.
 """
 from __future__ import annotations
 
@@ -18,20 +22,24 @@ from dataclasses import dataclass, field
 
 import torch
 
+from . import batch as nb
 from . import xform as nx
+
+ALIGN = 256
 
 
 @dataclass
 class Request:
     """One projection ("extract") or backprojection ("insert") request.
 
-    rotations: (K, 3, 3) float32 on the device; forward rotations for extraction, inverse rotations for insertion.
-    slices, weights: insertion inputs, (K, 1, box, box//2+1) complex64 and float32 on the device ("h" layout).
+    rotations: (K, 3, 3) float32 on the device (or a Mat33Array); forward rotations for extraction, inverse
+    rotations for insertion. slices, weights: insertion inputs, (K, 1, box, box//2+1) complex64 and float32 on the
+    device ("h" layout).
     """
     rid: int
     kind: str
     box: int
-    rotations: torch.Tensor
+    rotations: torch.Tensor | nx.Mat33Array
     volume: int = 0
     slices: torch.Tensor | None = None
     weights: torch.Tensor | None = None
@@ -41,23 +49,64 @@ class Request:
     admitted: float | None = None
     done: float | None = None
     ready: torch.cuda.Event | None = None       # recorded at submit() on the stream that produced the inputs
+    offset: int | None = None                   # pool offset of the output (None: dedicated allocation)
 
     @property
     def count(self) -> int:
-        return int(self.rotations.shape[0])
+        r = self.rotations
+        return int(r.count if isinstance(r, nx.Mat33Array) else r.shape[0])
 
     def output_bytes(self) -> int:
         wr = self.box // 2 + 1
         if self.kind == "extract":
-            return self.count * self.box * wr * 8
-        return self.box * self.box * wr * (8 + 4)
+            return _aligned(self.count * self.box * wr * 8)
+        return _aligned(self.box * self.box * wr * 8) + _aligned(self.box * self.box * wr * 4)
+
+
+def _aligned(n: int) -> int:
+    return (n + ALIGN - 1) // ALIGN * ALIGN
+
+
+class DevicePool:
+    """First-fit allocator over one preallocated device buffer (offsets aligned to 256 bytes)."""
+
+    def __init__(self, nbytes: int, device: torch.device):
+        self.size = _aligned(int(nbytes))
+        self.buf = torch.empty(self.size, dtype=torch.uint8, device=device)
+        self.free = [(0, self.size)]
+
+    def alloc(self, nbytes: int) -> int | None:
+        for i, (off, size) in enumerate(self.free):
+            if size >= nbytes:
+                if size == nbytes:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + nbytes, size - nbytes)
+                return off
+        return None
+
+    def release(self, off: int, nbytes: int) -> None:
+        self.free.append((off, nbytes))
+        self.free.sort()
+        merged = []
+        for o, s in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + s)
+            else:
+                merged.append((o, s))
+        self.free = merged
+
+    def view(self, off: int, shape, dtype: torch.dtype) -> torch.Tensor:
+        nbytes = torch.Size(shape).numel() * torch.empty((), dtype=dtype).element_size()
+        return self.buf[off:off + nbytes].view(dtype).view(shape)
 
 
 @dataclass
 class _Launch:
     requests: list
     event: torch.cuda.Event
-    nbytes: int
+    slot: nb.BatchSlot
+    calls: list                                  # keeps the calls' host objects alive until the batch has run
 
 
 @dataclass
@@ -66,6 +115,7 @@ class ServeStats:
     slices: int = 0
     launches: int = 0
     merged: int = 0
+    batch_sizes: list = field(default_factory=list)
 
     def percentile(self, q: float) -> float:
         if not self.latencies_ms:
@@ -76,20 +126,27 @@ class ServeStats:
 
 
 class Server:
-    """volumes: {box: [complex64 (1, box, box, box//2+1) "hc" volumes on the device]}."""
+    """volumes: {box: [complex64 (1, box, box, box//2+1) "hc" volumes on the device]}.
 
-    def __init__(self, volumes: dict, budget_bytes: int, n_streams: int = 3, max_merge_slices: int = 4096,
-                 device: torch.device | str = "cuda"):
+    budget_bytes: size of the output pool. n_streams: batch slots (each with its own stream and graph).
+    keep_results: copy each finished request's outputs out of the pool (into req.result / req.result_weight) before
+    the pool space is reused; otherwise the outputs are only valid inside `on_done`.
+    """
+
+    def __init__(self, volumes: dict, budget_bytes: int, n_streams: int = 3, device: torch.device | str = "cuda",
+                 keep_results: bool = True, on_done=None, graphs: bool = True):
         self.device = torch.device(device)
         self.volumes = volumes
         self.budget = int(budget_bytes)
+        self.pool = DevicePool(self.budget, self.device)
         self.in_use = 0
-        self.max_merge = max_merge_slices
-        self.streams = [torch.cuda.Stream(self.device) for _ in range(n_streams)]   # default priority
-        self.next_stream = 0
+        self.slots = [nb.BatchSlot(self.device, graph=graphs) for _ in range(max(1, n_streams))]
+        self.free_slots = deque(self.slots)
         self.pending: deque[Request] = deque()
         self.admitted: list[Request] = []
         self.running: list[_Launch] = []
+        self.keep_results = keep_results
+        self.on_done = on_done
         self.stats = ServeStats()
         self.t0 = time.perf_counter()
 
@@ -98,101 +155,94 @@ class Server:
 
     def submit(self, req: Request) -> None:
         """Queues a request. Its inputs may still be in flight on the caller's current stream: an event recorded
-        there now orders every launch that reads them after their producer."""
+        there now orders the batch that reads them after their producer."""
+        if not isinstance(req.rotations, nx.Mat33Array):
+            req.rotations = nx.Mat33Array(req.rotations, device=self.device)
         req.ready = torch.cuda.Event()
         req.ready.record(torch.cuda.current_stream(self.device))
         self.pending.append(req)
 
-    @staticmethod
-    def _inputs_on(stream: torch.cuda.Stream, reqs) -> None:
-        """Makes `stream` wait for each request's inputs, and ties their memory to `stream` so that the caching
-        allocator does not reuse it while the launch may still read it."""
-        for r in reqs:
-            if r.ready is not None:
-                stream.wait_event(r.ready)
-            for t in (r.rotations, r.slices, r.weights):
-                if t is not None and t.is_cuda:
-                    t.record_stream(stream)
-
     def idle(self) -> bool:
         return not self.pending and not self.admitted and not self.running
 
-    # -- admission: FIFO while the budget allows; a request larger than the budget runs alone.
+    # -- admission: FIFO while the output fits in the pool; a request larger than the pool runs alone.
     def _admit(self) -> None:
         while self.pending:
             req = self.pending[0]
             need = req.output_bytes()
-            if self.in_use + need > self.budget and (self.in_use > 0 or self.running):
-                break
+            if need > self.pool.size:
+                if self.in_use > 0 or self.running or self.admitted:
+                    break
+                req.offset = None
+            else:
+                off = self.pool.alloc(need)
+                if off is None:
+                    break
+                req.offset = off
             self.pending.popleft()
             self.in_use += need
             req.admitted = self.now()
             self.admitted.append(req)
 
-    def _stream(self) -> torch.cuda.Stream:
-        s = self.streams[self.next_stream]
-        self.next_stream = (self.next_stream + 1) % len(self.streams)
-        return s
+    def _outputs(self, req: Request):
+        wr = req.box // 2 + 1
+        if req.kind == "extract":
+            shape = (req.count, 1, req.box, wr)
+            if req.offset is None:
+                return torch.empty(shape, dtype=torch.complex64, device=self.device), None
+            return self.pool.view(req.offset, shape, torch.complex64), None
+        shape = (1, req.box, req.box, wr)
+        vbytes = _aligned(req.box * req.box * wr * 8)
+        if req.offset is None:
+            return (torch.zeros(shape, dtype=torch.complex64, device=self.device),
+                    torch.zeros(shape, dtype=torch.float32, device=self.device))
+        return (self.pool.view(req.offset, shape, torch.complex64),
+                self.pool.view(req.offset + vbytes, shape, torch.float32))
 
-    # -- launches: extraction requests on the same volume are merged (up to max_merge slices).
+    # -- launches: every admitted request joins one batch on a free slot.
     def _launch(self) -> None:
-        if not self.admitted:
+        if not self.admitted or not self.free_slots:
             return
-        groups: dict = {}
-        inserts = []
-        for req in self.admitted:
-            if req.kind == "extract":
-                groups.setdefault((req.box, req.volume), []).append(req)
-            else:
-                inserts.append(req)
-        self.admitted = []
-        for (box, vid), reqs in groups.items():
-            batch: list[Request] = []
-            n = 0
-            for req in reqs + [None]:
-                if req is not None and (n + req.count <= self.max_merge or not batch):
-                    batch.append(req)
-                    n += req.count
-                    continue
-                if batch:
-                    self._launch_extract(box, vid, batch, n)
-                batch, n = ([req], req.count) if req is not None else ([], 0)
-        for req in inserts:
-            self._launch_insert(req)
-
-    def _launch_extract(self, box: int, vid: int, reqs: list, n: int) -> None:
-        wr = box // 2 + 1
-        stream = self._stream()
-        self._inputs_on(stream, reqs)
-        with torch.cuda.stream(stream):
-            rot = torch.cat([r.rotations for r in reqs]) if len(reqs) > 1 else reqs[0].rotations
-            out = torch.empty((n, 1, box, wr), dtype=torch.complex64, device=self.device)
-            nx.extract_central_slices_3d("hc2hc", self.volumes[box][vid], None, (1, box, box, box), out, None,
-                                         (n, 1, box, box), None, rot)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        off = 0
-        for r in reqs:
-            r.result = out[off:off + r.count]
-            off += r.count
-        self.running.append(_Launch(reqs, ev, sum(r.output_bytes() for r in reqs)))
+        slot = self.free_slots.popleft()
+        reqs, self.admitted = self.admitted, []
+        ext, ins = [], []
+        with torch.cuda.stream(slot.stream):
+            for r in reqs:
+                if r.ready is not None:
+                    slot.stream.wait_event(r.ready)
+                out, wout = self._outputs(r)
+                r.result, r.result_weight = out, wout
+                box = r.box
+                if r.kind == "extract":
+                    ext.append(nb.extract_call("hc2hc", self.volumes[box][r.volume], (1, box, box, box), out,
+                                               (r.count, 1, box, box), r.rotations))
+                else:
+                    out.zero_()
+                    wout.zero_()
+                    ins.append(nb.insert_call("h2h", r.slices, r.weights, (r.count, 1, box, box), out, wout,
+                                              (1, box, box, box), r.rotations))
+        slot.prepare(ext, ins)
+        ev = slot.run()
+        self.running.append(_Launch(reqs, ev, slot, ext + ins))
         self.stats.launches += 1
         self.stats.merged += len(reqs) - 1
+        self.stats.batch_sizes.append(len(reqs))
 
-    def _launch_insert(self, req: Request) -> None:
-        box, wr = req.box, req.box // 2 + 1
-        stream = self._stream()
-        self._inputs_on(stream, [req])
-        with torch.cuda.stream(stream):
-            vol = torch.zeros((1, box, box, wr), dtype=torch.complex64, device=self.device)
-            wvol = torch.zeros((1, box, box, wr), dtype=torch.float32, device=self.device)
-            nx.insert_central_slices_3d("h2h", req.slices, req.weights, (req.count, 1, box, box), vol, wvol,
-                                        (1, box, box, box), None, req.rotations)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        req.result, req.result_weight = vol, wvol
-        self.running.append(_Launch([req], ev, req.output_bytes()))
-        self.stats.launches += 1
+    def _finish(self, r: Request) -> None:
+        if self.on_done is not None:
+            self.on_done(r)
+        if r.offset is not None:
+            if self.keep_results:
+                # Copied before the pool space can be handed to a later batch on another stream.
+                r.result = r.result.clone()
+                if r.result_weight is not None:
+                    r.result_weight = r.result_weight.clone()
+                torch.cuda.current_stream(self.device).synchronize()
+            else:
+                r.result = r.result_weight = None
+            self.pool.release(r.offset, r.output_bytes())
+            r.offset = None
+        self.in_use -= r.output_bytes()
 
     def _poll(self) -> int:
         finished = 0
@@ -204,7 +254,8 @@ class Server:
                     r.done = t
                     self.stats.latencies_ms.append((t - r.arrival) * 1e3)
                     self.stats.slices += r.count
-                self.in_use -= launch.nbytes
+                    self._finish(r)
+                self.free_slots.append(launch.slot)
                 finished += len(launch.requests)
             else:
                 still.append(launch)

@@ -59,6 +59,7 @@ def test_struct_layout_matches_the_c_compiler(tmp_path):
         "noa_hip_ctf_isotropic_f64": ("CtfIsotropicF64", [f[0] for f in abi.CtfIsotropicF64._fields_]),
         "noa_hip_ctf": ("Ctf", [f[0] for f in abi.Ctf._fields_]),
         "noa_hip_ctf_options": ("CtfOptions", [f[0] for f in abi.CtfOptions._fields_]),
+        "noa_hip_tuning": ("Tuning", [f[0] for f in abi.Tuning._fields_]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "noa_hip.h"', "int main(void) {"]
     for c_name, (_, names) in fields.items():
@@ -437,3 +438,72 @@ def test_fft_validation_errors(lib, kind, case, message):
     err = lib.noa_hip_last_error().decode()
     assert status in (abi.ERROR_INVALID_ARGUMENT, abi.ERROR_UNSUPPORTED), err
     assert message in err, err
+
+
+def test_tuning_table_most_specific_entry_wins(lib):
+    """The launch tuning table: exact key fields beat wildcards, the later entry wins a tie, count 0 clears."""
+    T = abi_tuning()
+    entries = (T * 4)()
+    rows = [(2, 0, -1, -1, -1, (1, 100, 0, 0)),      # insert, any key
+            (2, 256, -1, -1, -1, (2, 200, 0, 0)),    # insert, box 256
+            (2, 256, 0, -1, -1, (2, 300, 1, 0)),     # insert, box 256, h2h
+            (2, 256, 0, -1, -1, (1, 400, 0, 0))]     # same key, later: wins the tie
+    for e, (op, box, remap, interp, dtype, vals) in zip(entries, rows):
+        e.op, e.box, e.remap, e.interp, e.dtype = op, box, remap, interp, dtype
+        for i in range(4):
+            e.values[i] = vals[i]
+    assert lib.noa_hip_set_tuning(entries, 4) == 0
+    v = (ctypes.c_int32 * 4)()
+    assert lib.noa_hip_get_tuning(2, 256, 0, 1, 3, v) == 1 and tuple(v) == (1, 400, 0, 0)
+    assert lib.noa_hip_get_tuning(2, 256, 12, 1, 3, v) == 1 and tuple(v) == (2, 200, 0, 0)
+    assert lib.noa_hip_get_tuning(2, 128, 12, 1, 3, v) == 1 and tuple(v) == (1, 100, 0, 0)
+    assert lib.noa_hip_get_tuning(1, 128, 12, 1, 3, v) == 0 and tuple(v) == (0, 0, 0, 0)
+    bad = (T * 1)()
+    bad[0].op = 9
+    assert lib.noa_hip_set_tuning(bad, 1) != 0
+    assert "unknown operator" in lib.noa_hip_last_error().decode()
+    assert lib.noa_hip_set_tuning(entries, 0) == 0
+    assert lib.noa_hip_get_tuning(2, 256, 0, 1, 3, v) == 0
+
+
+def abi_tuning():
+    from noa_amd import abi
+    return abi.Tuning
+
+
+def test_autotune_table_round_trip(lib, tmp_path):
+    from noa_amd import autotune
+    try:
+        autotune.apply([autotune.Entry(op="extract", box=256, values=(3, 96, 0, 0)),
+                        autotune.Entry(op="insert_batch", values=(1536, 0, 0, 0), ms=0.5)])
+        assert autotune.lookup("extract", 256) == (3, 96, 0, 0)
+        assert autotune.lookup("extract", 128) == (0, 0, 0, 0)
+        assert autotune.lookup("insert_batch") == (1536, 0, 0, 0)
+        autotune.save(tmp_path / "t.json")
+        autotune.apply([])
+        assert autotune.lookup("insert_batch") == (0, 0, 0, 0)
+        autotune.load(tmp_path / "t.json")
+        assert autotune.lookup("extract", 256) == (3, 96, 0, 0)
+    finally:
+        autotune.apply([])
+
+
+def test_batch_prepare_argument_errors(lib):
+    """Argument errors of the batch calls are reported before any device access."""
+    from noa_amd import abi
+    size = ctypes.c_int64(-1)
+    calls = (abi.ProjectArgs * 1)()
+    assert lib.noa_hip_extract_batch_prepare(calls, -1, None, 0, ctypes.byref(size)) == abi.ERROR_INVALID_ARGUMENT
+    assert "negative count" in lib.noa_hip_last_error().decode()
+    assert lib.noa_hip_insert_batch_prepare(calls, 1, None, 0, None) == abi.ERROR_INVALID_ARGUMENT
+    assert "null size" in lib.noa_hip_last_error().decode()
+    # Zero calls: a valid empty batch; a too-small image reports the size needed.
+    assert lib.noa_hip_extract_batch_prepare(calls, 0, None, 0, ctypes.byref(size)) == abi.ERROR_INVALID_ARGUMENT
+    assert size.value == 64
+    img = (ctypes.c_uint8 * 64)()
+    assert lib.noa_hip_extract_batch_prepare(calls, 0, img, 64, ctypes.byref(size)) == 0
+    assert lib.noa_hip_extract_batch_launch(None, None) == abi.ERROR_INVALID_ARGUMENT
+    # A first call whose own rules fail names the call.
+    lib.noa_hip_project_args_init(calls)
+    assert lib.noa_hip_insert_batch_prepare(calls, 1, None, 0, ctypes.byref(size)) != 0
+    assert lib.noa_hip_last_error().decode().startswith("batch call 0: ")
