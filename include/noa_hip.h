@@ -256,6 +256,19 @@ int noa_hip_rasterize_central_slices_3d(const noa_hip_project_args* args, void* 
 /* Replaces the GPU branch of noa::xform::insert_central_slices_3d (FourierProject.hpp:1770-1815). */
 int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* stream);
 
+/* Size of the device scratch array of noa_hip_insert_central_slices_3d_noalloc (one per GPU, held until the process
+ * ends). */
+#define NOA_HIP_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in noa: noa_hip_insert_central_slices_3d for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation). Same arguments, rules and results
+ * (bit-identical). The work plan, the per-slice records and the partial sums of split sub-bricks are carved from a
+ * static device array of NOA_HIP_NOALLOC_SCRATCH_BYTES per GPU instead of being allocated; a call whose work needs
+ * more returns NOA_HIP_ERROR_UNSUPPORTED with the size in noa_hip_last_error() (at 256^3, up to about 2,048 slices
+ * fit). Every call on a GPU shares that GPU's array: calls that overlap on one GPU (different streams, or
+ * threads) race and must be ordered by the caller. A geometry's first call builds its plan on the host. */
+int noa_hip_insert_central_slices_3d_noalloc(const noa_hip_project_args* args, void* stream);
+
 /* Replaces the GPU branch of noa::xform::extract_central_slices_3d (FourierProject.hpp:1869-1914). */
 int noa_hip_extract_central_slices_3d(const noa_hip_project_args* args, void* stream);
 

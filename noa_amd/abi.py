@@ -213,7 +213,7 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     """Set the argument/return types of the exported functions."""
     P = ctypes.POINTER(ProjectArgs)
     for name in ("noa_hip_rasterize_central_slices_3d", "noa_hip_insert_central_slices_3d",
-                 "noa_hip_extract_central_slices_3d"):
+                 "noa_hip_insert_central_slices_3d_noalloc", "noa_hip_extract_central_slices_3d"):
         f = getattr(lib, name)
         f.argtypes = [P, c_void_p]
         f.restype = ctypes.c_int

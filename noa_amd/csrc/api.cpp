@@ -1213,6 +1213,9 @@ int noa_hip_rasterize_central_slices_3d(const noa_hip_project_args* args, void* 
 int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* stream) {
     return noa_hip::call(noa_hip::Kind::INSERT, args, stream);
 }
+int noa_hip_insert_central_slices_3d_noalloc(const noa_hip_project_args* args, void* stream) {
+    return noa_hip::call(noa_hip::Kind::INSERT, args, stream, nullptr, true);
+}
 int noa_hip_extract_batch_prepare(const noa_hip_project_args* calls, int32_t n_calls, void* image, int64_t capacity,
                                   int64_t* bytes) {
     using namespace noa_hip;

@@ -1653,6 +1653,31 @@ int dispatch(const ProjectDesc& d, hipStream_t stream, std::string& err, const S
     return NOA_HIP_ERROR_UNSUPPORTED;
 }
 
+// Scratch of the allocation-free entry point: one array per GPU, resident for the life of the process. Host code
+// never uses the symbol itself as an address: the device address of each GPU's copy is looked up once, with that
+// GPU current, and kept.
+__device__ __align__(256) unsigned char scratch[NOA_HIP_NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mutex;
+std::vector<void*> g_scratch_ptr;   // per device ordinal; null until looked up
+
+int scratch_address(int device, void*& ptr, std::string& err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (static_cast<int>(g_scratch_ptr.size()) <= device) g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (!g_scratch_ptr[static_cast<size_t>(device)]) {
+        void* found = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&found, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !found) {
+            (void)hipGetLastError();
+            err = std::string("insert (no allocation): could not find the scratch array on device ") +
+                  std::to_string(device) + ": " + hipGetErrorString(e);
+            return NOA_HIP_ERROR_RUNTIME;
+        }
+        g_scratch_ptr[static_cast<size_t>(device)] = found;
+    }
+    ptr = g_scratch_ptr[static_cast<size_t>(device)];
+    return NOA_HIP_OK;
+}
+
 } // namespace
 
 int insert_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64 capacity, i64& bytes,
@@ -1727,11 +1752,17 @@ int insert_batch_launch(const void* image, i32 grid, hipStream_t stream, std::st
 }
 
 int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err, bool no_alloc) {
-    if (no_alloc) {
-        err = "insert: no allocation-free path";
-        return NOA_HIP_ERROR_UNSUPPORTED;
+    if (!no_alloc) return d.coord_f64 ? dispatch<double>(d, stream, err, nullptr) : dispatch<float>(d, stream, err, nullptr);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) {
+        err = "insert (no allocation): no current device";
+        return NOA_HIP_ERROR_RUNTIME;
     }
-    return d.coord_f64 ? dispatch<double>(d, stream, err, nullptr) : dispatch<float>(d, stream, err, nullptr);
+    void* base = nullptr;
+    const int st = scratch_address(device, base, err);
+    if (st != NOA_HIP_OK) return st;
+    const Scratch sc{static_cast<char*>(base), NOA_HIP_NOALLOC_SCRATCH_BYTES};
+    return d.coord_f64 ? dispatch<double>(d, stream, err, &sc) : dispatch<float>(d, stream, err, &sc);
 }
 
 } // namespace noa_hip
