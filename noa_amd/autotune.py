@@ -35,7 +35,7 @@ SPACE = {
     "extract": {"layout": [1, 2, 3], "pairs": [64, 128, 256]},
     "insert": {"kernel": [1, 2], "list": [128, 256, 512], "paired": [0]},
     "extract_batch": {"ppt": [1, 2, 3, 4], "grid": [1024, 2048, 4096]},
-    "insert_batch": {"grid": [1536, 3072, 6144]},
+    "insert_batch": {"grid": [3072, 6144, 12288]},
 }
 
 
@@ -106,21 +106,23 @@ def time_ms(fn: Callable[[], None], reps: int = 5, device: torch.device | str = 
 
 
 def tune(op: str, fn: Callable[[], None], key: dict | None = None, space: dict | None = None, reps: int = 5,
-         log: Callable[[str], None] | None = None) -> Entry:
+         log: Callable[[str], None] | None = None, candidates: list | None = None) -> Entry:
     """Times `fn` under every candidate of `op` for one key and adds the fastest to the table.
 
-    key: box / remap / interp / dtype of the entry (omitted fields match anything)."""
+    key: box / remap / interp / dtype of the entry (omitted fields match anything). candidates: explicit value
+    tuples (else the cartesian product of `space`, by default SPACE[op])."""
     key = key or {}
-    space = space or SPACE[op]
-    names = list(space)
+    if candidates is None:
+        space = space or SPACE[op]
+        candidates = list(itertools.product(*(space[n] for n in space)))
     kept = [e for e in _table if not (e.op == op and all(getattr(e, k) == v for k, v in key.items()))]
     best: Entry | None = None
-    for combo in itertools.product(*(space[n] for n in names)):
+    for combo in candidates:
         cand = Entry(op=op, **key, values=tuple(combo) + (0,) * (4 - len(combo)))
         apply(kept + [cand])
         ms = time_ms(fn, reps)
         if log:
-            log(f"tune {op} {key} {dict(zip(names, combo))}: {ms:.4f} ms")
+            log(f"tune {op} {key} {tuple(combo)}: {ms:.4f} ms")
         if best is None or ms < best.ms:
             cand.ms = ms
             best = cand

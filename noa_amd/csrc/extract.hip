@@ -792,8 +792,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
         if (lean) {
             p.inv_wr = 1.f / static_cast<float>(p.slice_wr);
             // Tiled path (extract_tile_kernel): cubic volume and slices, no target shape, cutoff <= 0.5, Mat33<f32>
-            // rotations, and slices holding at least twice the volume's elements. NOA_HIP_EXTRACT_LAYOUT=tiles forces
-            // it where the configuration allows it, linear or bricks disables it.
+            // rotations; chosen by the tuning table (layout 3) or NOA_HIP_EXTRACT_LAYOUT=tiles.
             // Launch tuning: volume layout and tile work-item size; NOA_HIP_EXTRACT_LAYOUT overrides the table.
             i32 tv[4];
             tuning_lookup(NOA_HIP_OP_EXTRACT, d.in_shape[1], (d.src_centered ? 4 : 0) + (d.dst_centered ? 8 : 0),
@@ -805,10 +804,8 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             const bool cubic = p.grid[1] == nn && p.grid[2] == nn && d.out_shape[2] == nn && d.out_shape[3] == nn &&
                                d.target[0] == nn && d.target[1] == nn && d.target[2] == nn;
             const i64 stored_elems = nn * nn * (nn / 2 + 1);
-            bool tiled = cubic && nn >= 16 && nn <= 1000 && static_cast<double>(cutoff) <= 0.5 && p.os_x == 1 &&
-                         p.rot.kind == NOA_HIP_MAT33_F32 && stored_elems < 0x7fffffffLL &&
-                         n_slices * static_cast<i64>(p.pixels) >= 2 * stored_elems;
-            if (layout_env && std::strcmp(layout_env, "tiles") != 0) tiled = false;
+            // Only on request: at 256^3 with 4096 slices the block-copy gather is faster (1.15 against 1.74 ms).
+            bool tiled = false;
             if (layout_env && std::strcmp(layout_env, "tiles") == 0)
                 tiled = cubic && nn >= 16 && nn <= 1000 && static_cast<double>(cutoff) <= 0.5 && p.os_x == 1 &&
                         p.rot.kind == NOA_HIP_MAT33_F32 && stored_elems < 0x7fffffffLL;

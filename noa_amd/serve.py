@@ -129,23 +129,26 @@ class Server:
     """volumes: {box: [complex64 (1, box, box, box//2+1) "hc" volumes on the device]}.
 
     budget_bytes: size of the output pool. n_streams: batch slots (each with its own stream and graph).
+    max_batch_slices: slices above which a batch takes no further request.
     keep_results: copy each finished request's outputs out of the pool (into req.result / req.result_weight) before
     the pool space is reused; otherwise the outputs are only valid inside `on_done`.
     """
 
     def __init__(self, volumes: dict, budget_bytes: int, n_streams: int = 3, device: torch.device | str = "cuda",
-                 keep_results: bool = True, on_done=None, graphs: bool = True):
+                 keep_results: bool = True, on_done=None, graphs: bool = True, max_batch_slices: int = 8192):
         self.device = torch.device(device)
         self.volumes = volumes
         self.budget = int(budget_bytes)
         self.pool = DevicePool(self.budget, self.device)
         self.in_use = 0
+        self.oversize = 0                           # admitted requests larger than the pool (run alone)
         self.slots = [nb.BatchSlot(self.device, graph=graphs) for _ in range(max(1, n_streams))]
         self.free_slots = deque(self.slots)
         self.pending: deque[Request] = deque()
         self.admitted: list[Request] = []
         self.running: list[_Launch] = []
         self.keep_results = keep_results
+        self.max_batch_slices = int(max_batch_slices)
         self.on_done = on_done
         self.stats = ServeStats()
         self.t0 = time.perf_counter()
@@ -170,10 +173,13 @@ class Server:
         while self.pending:
             req = self.pending[0]
             need = req.output_bytes()
+            if self.oversize:
+                break                               # a request larger than the pool is running alone
             if need > self.pool.size:
                 if self.in_use > 0 or self.running or self.admitted:
                     break
                 req.offset = None
+                self.oversize += 1
             else:
                 off = self.pool.alloc(need)
                 if off is None:
@@ -204,7 +210,15 @@ class Server:
         if not self.admitted or not self.free_slots:
             return
         slot = self.free_slots.popleft()
-        reqs, self.admitted = self.admitted, []
+        # A batch takes admitted requests in order up to max_batch_slices (at least one request); the rest wait for
+        # the next free slot, so that a burst becomes several pipelined batches rather than one long one.
+        n, take = 0, 0
+        for r in self.admitted:
+            if take and n + r.count > self.max_batch_slices:
+                break
+            n += r.count
+            take += 1
+        reqs, self.admitted = self.admitted[:take], self.admitted[take:]
         ext, ins = [], []
         with torch.cuda.stream(slot.stream):
             for r in reqs:
@@ -242,6 +256,8 @@ class Server:
                 r.result = r.result_weight = None
             self.pool.release(r.offset, r.output_bytes())
             r.offset = None
+        else:
+            self.oversize -= 1
         self.in_use -= r.output_bytes()
 
     def _poll(self) -> int:

@@ -22,7 +22,7 @@ def test_more_than_one_gpu_is_refused():
 def test_bench_line_keeps_the_contract():
     print("case bench contract: bench.py --steps 2 --warmup 1", flush=True)
     res = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--steps", "2", "--warmup", "1",
-                          "--no-cpu-baseline", "--serve-requests", "0"], capture_output=True, text=True, timeout=110)
+                          "--no-cpu-baseline", "--serve-requests", "0"], capture_output=True, text=True, timeout=240)
     assert res.returncode == 0, res.stderr[-2000:]
     lines = [line for line in res.stdout.splitlines() if line.strip()]
     assert len(lines) == 1

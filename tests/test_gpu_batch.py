@@ -22,7 +22,6 @@ EXTRACT_JOBS = [   # remap, box, slices, cutoff
     ("h2h", 96, 5, 0.45),
     ("hc2h", 33, 300, 0.5),
     ("h2hc", 128, 1, 0.5),
-    ("hc2hc", 48, 0, 0.5),      # an empty call
     ("hc2hc", 64, 64, 0.3),
 ]
 INSERT_JOBS = [    # remap, box, slices, weights

@@ -192,7 +192,18 @@ def test_insert_256_10000_ctf_weighted_slices():
     print(f"  rel-L2 values {e_v:.3g}, weights {e_w:.3g} over {int(inside.sum())} voxels", flush=True)
     assert e_v < 1e-5
     assert e_w < 1e-5
-    # The origin neighbourhood meets most slices: a dropped slice segment there shows in the weights.
-    near = np.abs(np.fft.fftfreq(n, 1.0 / n).round()[vox[:, 0]]) + np.abs(np.fft.fftfreq(n, 1.0 / n).round()[vox[:, 1]]) + vox[:, 2] <= 3
-    assert rel_l2(got_w[near & inside], want_w[near & inside]) < 1e-5
-    assert want_w[near & inside].min() > 0
+    # The origin neighbourhood meets most slices: a dropped slice segment there shows in the weights. Its voxels sum
+    # ~10,000 terms, where the oracle's own FP32 running sum (noa's order) drifts by ~2e-5: both are compared with the
+    # same sums in FP64.
+    fr = np.fft.fftfreq(n, 1.0 / n).round()
+    near = (np.abs(fr[vox[:, 0]]) + np.abs(fr[vox[:, 1]]) + vox[:, 2] <= 3) & inside
+    ref_v, ref_w, _ = O.insert_central_slices_3d_voxels(
+        "h2h", ins.slices.cpu().numpy().astype(np.complex128), ins.weights.cpu().numpy().astype(np.float64),
+        ins.slice_shape, ins.volume_shape, inv.astype(np.float64), vox[near], n_threads=THREADS)
+    e_gpu = rel_l2(got_w[near], ref_w)
+    e_oracle = rel_l2(want_w[near], ref_w)
+    print(f"  near the origin ({int(near.sum())} voxels), weights vs FP64: GPU {e_gpu:.3g}, FP32 oracle {e_oracle:.3g}; "
+          f"values vs FP64: GPU {rel_l2(got_v[near], ref_v):.3g}", flush=True)
+    assert e_gpu < 1e-5
+    assert rel_l2(got_v[near], ref_v) < 1e-5
+    assert ref_w.min() > 0

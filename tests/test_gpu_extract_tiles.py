@@ -103,7 +103,8 @@ def test_tiles_with_the_fused_phase_shift(monkeypatch, kind):
     ref = run("linear", monkeypatch, "hc2hc", dev(vol), n, dev(rot), b, shifts=arg)
     assert not np.isnan(got).any()
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
-    want = oracle("hc2hc", vol, n, rot, b, shifts=np.broadcast_to(sh, (b, 2)).copy())
+    # A single shift is passed as one (the half-shape one takes noa's exact +-1 pattern), batched ones per slice.
+    want = oracle("hc2hc", vol, n, rot, b, shifts=sh)
     assert rel_l2(got, want) < 3e-7
 
 
@@ -139,16 +140,26 @@ def test_tiles_with_a_degenerate_transform(monkeypatch):
     np.testing.assert_array_equal(got, oracle("h2h", vol, n, rot, b))
 
 
-def test_default_layout_picks_tiles_for_many_slices(monkeypatch):
-    """Without the switch, 16x the volume's elements in slices take the tiled path: same bits as forcing it."""
-    print("case tiles default selection", flush=True)
+def test_tuning_table_selects_tiles(monkeypatch):
+    """The launch tuning table's layout 3 selects the tiled path (the autotuner's candidate): same bits as forcing
+    it, and as the default (block-copy) gather."""
+    print("case tiles through the tuning table", flush=True)
+    from noa_amd import autotune
     rng = np.random.default_rng(1600)
     n, b = 64, 16 * 64
     vol = dev(random_spectrum((1, n, n, n // 2 + 1), rng))
     rot = dev(random_rotations(b, rng))
     forced = run("tiles", monkeypatch, "hc2hc", vol, n, rot, b)
     monkeypatch.delenv("NOA_HIP_EXTRACT_LAYOUT")
-    out = torch.full((b, 1, n, n // 2 + 1), float("nan"), dtype=torch.complex64, device=DEV)
-    nx().extract_central_slices_3d("hc2hc", vol, None, (1, n, n, n), out, None, (b, 1, n, n), None, rot)
-    torch.cuda.synchronize()
-    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), forced.view(np.uint32))
+    outs = []
+    try:
+        for values in ((3, 64, 0, 0), (0, 0, 0, 0)):
+            autotune.apply([autotune.Entry(op="extract", box=n, values=values)])
+            out = torch.full((b, 1, n, n // 2 + 1), float("nan"), dtype=torch.complex64, device=DEV)
+            nx().extract_central_slices_3d("hc2hc", vol, None, (1, n, n, n), out, None, (b, 1, n, n), None, rot)
+            torch.cuda.synchronize()
+            outs.append(out.cpu().numpy())
+    finally:
+        autotune.apply([])
+    for o in outs:
+        np.testing.assert_array_equal(o.view(np.uint32), forced.view(np.uint32))

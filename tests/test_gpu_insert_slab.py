@@ -18,6 +18,13 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 
+@pytest.fixture(autouse=True)
+def slab_kernel(monkeypatch):
+    """The slab kernel is chosen by the tuning table or NOA_HIP_INSERT_KERNEL=slab (the one-wave kernel is the
+    default)."""
+    monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", "slab")
+
+
 def nx():
     import noa_amd.xform as m
     return m
@@ -110,7 +117,7 @@ def test_slab_is_deterministic_and_distinct_from_the_wave_kernel(monkeypatch):
     monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", "wave")
     c_v, c_w = run_gpu("h2h", sl, w, (b, 1, n, n), n, inv, torch.complex64)
     assert rel_l2(a_v, c_v) < 1e-6 and rel_l2(a_w, c_w) < 1e-6
-    assert not np.array_equal(a_v, c_v), "the default path ran the one-wave kernel"
+    assert not np.array_equal(a_v, c_v), "the slab path ran the one-wave kernel"
 
 
 def test_slab_strided_slices_and_broadcast_rotation():

@@ -16,7 +16,7 @@ def _mix(device, n=24, boxes=(64, 128, 192), max_k=40, seed=5):
         box = int(rng.choice(boxes))
         k = int(rng.integers(1, max_k + 1))
         poses = workloads.uniform_rotations(k, rng)
-        arrival = 0.0005 * i
+        arrival = 0.0004 * (i // 4)          # groups of four arriving together
         if i % 2 == 0:
             reqs.append(Request(rid=i, kind="extract", box=box, volume=int(rng.integers(0, 2)),
                                 rotations=torch.from_numpy(poses).to(device), arrival=arrival))

@@ -1237,13 +1237,15 @@ def _np_f2i(centered, f, n):
     return f + n // 2 if centered else (f + n if f < 0 else f)
 
 
-def _np_rasterize(slices, rot, n, src_c, dst_c, cutoff):
+def _np_rasterize(slices, rot, n, src_c, dst_c, cutoff, logical_x=False):
     """FourierInsertRasterize (FourierProject.hpp:238-373) in float64 loops: slice pixel (v, u) -> (0, v/n, u/n),
     forward rotation, cutoff, flip of x < 0 (the value is negated), trilinear spread over the 2x2x2 cube with
     index bounds, and the conjugate duplicate on the x = 0 plane. Taps at x > n/2 are left out (noa bounds x by the
-    logical n there; they carry a zero weight at cutoffs <= 0.5)."""
-    vol = np.zeros((n, n, n // 2 + 1), np.complex128)
-    wv = np.zeros((n, n, n // 2 + 1), np.float64)
+    logical n there; they carry a zero weight at cutoffs <= 0.5). logical_x: keep them, as noa's bound does, in an
+    (n, n, n) buffer whose columns above n/2 stand for what noa adds past the end of each stored row."""
+    xs = n if logical_x else n // 2 + 1
+    vol = np.zeros((n, n, xs), np.complex128)
+    wv = np.zeros((n, n, xs), np.float64)
     for b in range(slices.shape[0]):
         for y in range(n):
             v = y - n // 2 if src_c else (y if y < (n + 1) // 2 else y - n)
@@ -1264,7 +1266,7 @@ def _np_rasterize(slices, rot, n, src_c, dst_c, cutoff):
                             iz = _np_f2i(dst_c, base[0] + w, n)
                             iy = _np_f2i(dst_c, base[1] + vv, n)
                             ix = base[2] + uu
-                            if 0 <= iz < n and 0 <= iy < n and 0 <= ix <= n // 2:
+                            if 0 <= iz < n and 0 <= iy < n and 0 <= ix < xs:
                                 vol[iz, iy, ix] += val * k
                                 wv[iz, iy, ix] += k
                 if base[2] == 0:
@@ -1290,6 +1292,28 @@ def test_rasterize_fp64_brute_force_in_numpy(remap):
     want_v, want_w = _np_rasterize(s, rot, n, src_c, dst_c, 0.5)
     np.testing.assert_allclose(vol[0], want_v, rtol=0, atol=1e-12)
     np.testing.assert_allclose(wvol[0], want_w, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("cutoff", [0.5, 0.7])
+def test_rasterize_taps_past_the_stored_half(cutoff):
+    """noa bounds rasterisation's x taps by the logical width n (FourierProject.hpp:330-332), so a tap at
+    x = n/2 + 1 .. n - 1 is added past the end of the stored row (into the next row's first elements). The library
+    and the oracle skip those taps. At cutoffs <= 0.5 they carry a weight of exactly 0 (skipping them changes
+    nothing); above 0.5 they carry mass, and skipping them is the one deviation from noa: everything that stays
+    inside the stored half is equal."""
+    rng = np.random.default_rng(63)
+    n, b = 8, 4
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng, np.complex128)
+    rot = random_rotations(b, rng, dtype=np.float64)
+    vol, wvol = _rasterize("h2h", s, n, rot, vol_dtype=np.complex128, wout=True, cutoff=cutoff)
+    lit_v, lit_w = _np_rasterize(s, rot, n, False, False, cutoff, logical_x=True)
+    np.testing.assert_allclose(vol[0], lit_v[..., : n // 2 + 1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(wvol[0], lit_w[..., : n // 2 + 1], rtol=0, atol=1e-12)
+    past = np.abs(lit_w[..., n // 2 + 1:]).sum()
+    if cutoff <= 0.5:
+        assert past == 0.0
+    else:
+        assert past > 0.1
 
 
 def test_rasterize_mass_conservation():
