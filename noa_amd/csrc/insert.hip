@@ -673,8 +673,9 @@ __global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertPa
 //     R * (index / n) * n rounds exactly like R * index, so pz, py, px are noa's fz*n, f2y*n, f2x*n bit for bit.
 //     The window is a degree-9 polynomial in t = 2 (pz/fbT)^2 - 1 (fitted per window, error < 4e-7), folded
 //     into the bilinear weights; taps beyond noa's frequency bounds get weight 0 and never load outside the slice
-//     (buffer resources). Sums are added to the LDS accumulator with float atomics; a voxel is met at most once
-//     per slice and slices are processed in order, so the result is deterministic.
+//     (buffer resources). Sums are added to the LDS accumulator (plain read-modify-write: a voxel is met at most
+//     once per slice and slices are processed in order, so the result is deterministic). The tap loads of the next
+//     passing slice are issued before the current slice's sums are added.
 // ---------------------------------------------------------------------------------------------------------
 constexpr i32 SLAB_NWIN = 9;        // degree of the window polynomial
 constexpr i32 SLAB_MAXC = 1024;     // candidates of one pair: at most the sub-brick's 1024 voxels
@@ -889,6 +890,36 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
     const float tk = p.tk;
     const i32 pitch = p.pitch;
 
+    // Phase 2 of a group of candidates: interpolation and accumulation (groups of non-hits skip).
+    auto finish = [&](const SlabTaps<In, W> (&t)[2]) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            if (!t[g].hit) continue;
+            const float a0 = t[g].oky0 ? (1.f - t[g].ty) * t[g].win : 0.f;
+            const float a1 = t[g].oky1 ? t[g].ty * t[g].win : 0.f;
+            const float bx0 = 1.f - t[g].tx, bx1 = t[g].okx1 ? t[g].tx : 0.f;
+            const float w00 = a0 * bx0, w01 = a0 * bx1, w10 = a1 * bx0, w11 = a1 * bx1;
+            float re, im = 0.f;
+            if constexpr (CX) {
+                re = fmaf(w11, t[g].v11.x, fmaf(w10, t[g].v10.x, fmaf(w01, t[g].v01.x, w00 * t[g].v00.x)));
+                im = fmaf(w11, t[g].v11.y, fmaf(w10, t[g].v10.y, fmaf(w01, t[g].v01.y, w00 * t[g].v00.y)));
+                im = t[g].flip ? -im : im;
+            } else {
+                re = fmaf(w11, t[g].v11, fmaf(w10, t[g].v10, fmaf(w01, t[g].v01, w00 * t[g].v00)));
+            }
+            const float wt = p.win ? fmaf(w11, t[g].q11, fmaf(w10, t[g].q10, fmaf(w01, t[g].q01, w00 * t[g].q00)))
+                                   : t[g].win;
+            // A voxel is a candidate at most once per slice, and one wave adds its slices in order: plain
+            // read-modify-write.
+            acc[t[g].lv] += re;
+            if constexpr (CX) acc[WVOX + t[g].lv] += im;
+            acc[(NCH - 1) * WVOX + t[g].lv] += wt;
+        }
+    };
+    // Cross-pair pipeline: the taps of the last pair whose sums are not added yet.
+    SlabTaps<In, W> pend[2];
+    bool have = false;
+
     while (true) {
         while (mask == 0) {
             chunk0 += 64;
@@ -926,8 +957,8 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
         const SliceBuf sb(p.in + static_cast<i64>(s) * p.is_b, p.slice_bytes);
         const SliceBuf wb(p.win ? static_cast<const void*>(p.win + static_cast<i64>(s) * p.wis_b)
                                 : static_cast<const void*>(p.in), p.win ? p.wslice_bytes : 0u);
-        for (u32 base = 0; base < total; base += 128) {
-            SlabTaps<In, W> t[2];
+        // Phase 1 of candidates [base, base + 128): coordinates, window, and every tap load issued.
+        auto issue = [&](u32 base, SlabTaps<In, W> (&t)[2]) {
 #pragma unroll
             for (int g = 0; g < 2; ++g) {
                 t[g].hit = false;
@@ -987,30 +1018,26 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
                     t[g].q11 = wb.template load_bytes<W>(c1x);
                 }
             }
-#pragma unroll
-            for (int g = 0; g < 2; ++g) {
-                if (base + 64u * g >= total) continue;
-                if (!t[g].hit) continue;
-                const float a0 = t[g].oky0 ? (1.f - t[g].ty) * t[g].win : 0.f;
-                const float a1 = t[g].oky1 ? t[g].ty * t[g].win : 0.f;
-                const float bx0 = 1.f - t[g].tx, bx1 = t[g].okx1 ? t[g].tx : 0.f;
-                const float w00 = a0 * bx0, w01 = a0 * bx1, w10 = a1 * bx0, w11 = a1 * bx1;
-                float re, im = 0.f;
-                if constexpr (CX) {
-                    re = fmaf(w11, t[g].v11.x, fmaf(w10, t[g].v10.x, fmaf(w01, t[g].v01.x, w00 * t[g].v00.x)));
-                    im = fmaf(w11, t[g].v11.y, fmaf(w10, t[g].v10.y, fmaf(w01, t[g].v01.y, w00 * t[g].v00.y)));
-                    im = t[g].flip ? -im : im;
-                } else {
-                    re = fmaf(w11, t[g].v11, fmaf(w10, t[g].v10, fmaf(w01, t[g].v01, w00 * t[g].v00)));
-                }
-                const float wt = p.win ? fmaf(w11, t[g].q11, fmaf(w10, t[g].q10, fmaf(w01, t[g].q01, w00 * t[g].q00)))
-                                       : t[g].win;
-                atomicAdd(&acc[t[g].lv], re);
-                if constexpr (CX) atomicAdd(&acc[WVOX + t[g].lv], im);
-                atomicAdd(&acc[(NCH - 1) * WVOX + t[g].lv], wt);
+        };
+        SlabTaps<In, W> now[2];
+        issue(0u, now);
+        // The previous pair's sums are added while this pair's loads are in flight; per voxel, slices still add in
+        // order.
+        if (have) finish(pend);
+        if (total > 128u) {
+            finish(now);
+            for (u32 base = 128; base < total; base += 128) {
+                issue(base, now);
+                finish(now);
             }
+            have = false;
+        } else {
+#pragma unroll
+            for (int g = 0; g < 2; ++g) pend[g] = now[g];
+            have = true;
         }
     }
+    if (have) finish(pend);
     __syncthreads();
 
     if (item.w >= 0) {
