@@ -392,6 +392,21 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
     log(f"virtual volume: {t * 1e3:.3f} ms, {pairs / t / 1e9:.1f} G pixel-slice pairs/s")
     del out, wout
 
+    # Rasterisation (trilinear scatter with float atomics) of 4096 of the insertion workload's slices into a 256^3
+    # volume + weights, forward rotations (h2h, cutoff 0.5).
+    b_r = min(4096, b)
+    rvol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    rw = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=dev)
+    fwd = torch.from_numpy(np.ascontiguousarray(ins.poses[:b_r])).to(dev)
+    t = timed(lambda: nx.rasterize_central_slices_3d("h2h", ins.slices[:b_r], None, (b_r, 1, n, n), rvol, rw,
+                                                     ins.volume_shape, None, fwd))
+    px = b_r * n * (n // 2 + 1)
+    rep["rasterize"] = {
+        "workload": f"rasterize_central_slices_3d of {b_r} slices 256^2 into a 256^3 c32 volume + f32 weights (h2h, "
+                    "cutoff 0.5)", "avg_ms": t * 1e3, "slices_per_s": b_r / t, "pixels_per_s": px / t}
+    log(f"rasterize {b_r} slices: {t * 1e3:.3f} ms")
+    del rvol, rw, fwd
+
     # Isotropic CTF of a reconstructed 256^3 rfft volume (h2h): the volume multiplied by CTF(|f|) in place.
     spec = torch.randn((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
     ciso = ns.CTFIsotropic(1.0, 1.5, 300, 0.07, 2.7)
@@ -480,6 +495,8 @@ def main():
     ap.add_argument("--n-insert", type=int, default=10_000)
     ap.add_argument("--n-extract", type=int, default=4096)
     ap.add_argument("--probe", action="store_true", help="one launch of each operator, no timing (counter runs)")
+    ap.add_argument("--probe-extract", default="bricks", help="extraction layouts of --probe (linear,bricks,tiles)")
+    ap.add_argument("--probe-insert", default="wave", help="insertion kernels of --probe (wave,slab)")
     ap.add_argument("--serve-qps", default="200,1000,3000,max",
                     help="offered loads of the serving sweep (requests/s, comma-separated; max = all at once)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests in the serving run (0: skip)")
@@ -533,10 +550,16 @@ def main():
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
-        extract()
-        insert()
-        torch.cuda.synchronize()
-        log("probe done: copy 1 GiB, extract, insert")
+        # One launch per requested variant (comma-separated), in order.
+        for layout in args.probe_extract.split(","):
+            os.environ["NOA_HIP_EXTRACT_LAYOUT"] = layout
+            extract()
+            torch.cuda.synchronize()
+        for kernel in args.probe_insert.split(","):
+            os.environ["NOA_HIP_INSERT_KERNEL"] = kernel
+            insert()
+            torch.cuda.synchronize()
+        log(f"probe done: copy 1 GiB, extract ({args.probe_extract}), insert ({args.probe_insert})")
         return
 
     tuning = None

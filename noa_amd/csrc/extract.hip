@@ -411,37 +411,56 @@ __global__ __launch_bounds__(256) void extract_tile_kernel(const TileParams p) {
 
             // Evaluates one candidate pixel: noa's expressions (as extract_c32_kernel), kept only when its floor is
             // in this tile and its flip matches the set it was enumerated from.
-            auto evaluate = [&](i32 fy, i32 u, bool mirrored) {
+            // A candidate in two phases, so that a lane can have the LDS reads of two candidates in flight: fetch runs
+            // noa's expressions (as extract_c32_kernel) and reads the taps, and keeps the candidate only when its
+            // floor is in this tile and its flip matches the set it was enumerated from; store combines and writes.
+            struct Cand {
+                Taps3<float2, float> t;
+                i32 fy, u;
+                bool ok;
+            };
+            auto fetch = [&](i32 fy, i32 u, bool mirrored, Cand& c) {
+                c.ok = false;
+                c.fy = fy;
+                c.u = u;
                 vec2<float> f2;
                 f2.y = static_cast<float>(fy) / slice_f;
                 f2.x = static_cast<float>(u) / slice_f;
                 const vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
                 if (!(dot3(f) <= p.cutoff_sqd)) return;
                 float fz = f.z * target, fyv = f.y * target, fx = f.x * target;
-                Taps3<float2, float> t;
-                t.conj = 1;
+                c.t.conj = 1;
                 const bool flip = fx < 0;
                 if (flip != mirrored) return;
-                if (flip) { fz *= -1; fyv *= -1; fx *= -1; t.conj = -1; }
+                if (flip) { fz *= -1; fyv *= -1; fx *= -1; c.t.conj = -1; }
                 const float flz = floorf(fz), fly = floorf(fyv), flx = floorf(fx);
                 const i32 lz = static_cast<i32>(flz) - Z0, ly = static_cast<i32>(fly) - Y0, lx = static_cast<i32>(flx) - X0;
                 if (static_cast<u32>(lz) >= static_cast<u32>(XT) || static_cast<u32>(ly) >= static_cast<u32>(XT) ||
                     static_cast<u32>(lx) >= static_cast<u32>(XT))
                     return;
-                t.tz = fz - flz;
-                t.ty = fyv - fly;
-                t.tx = fx - flx;
-                t.ok = 0xffu;
+                c.t.tz = fz - flz;
+                c.t.ty = fyv - fly;
+                c.t.tx = fx - flx;
+                c.t.ok = 0xffu;
                 const i32 o = (lz * XTT + ly) * XTT + lx;
 #pragma unroll
                 for (int z = 0; z < 2; ++z)
 #pragma unroll
                     for (int y = 0; y < 2; ++y)
 #pragma unroll
-                        for (int x = 0; x < 2; ++x) t.v[z][y][x] = tile[o + (z * XTT + y) * XTT + x];
-                float2 value = combine3<NOA_HIP_INTERP_LINEAR>(t);
-                value = tile_shift<SHIFT>(p, b, fy, u, value);
-                out[frequency2index(p.slice_centered, fy, n) * p.os_y + u] = value;
+                        for (int x = 0; x < 2; ++x) c.t.v[z][y][x] = tile[o + (z * XTT + y) * XTT + x];
+                c.ok = true;
+            };
+            auto store = [&](const Cand& c) {
+                if (!c.ok) return;
+                float2 value = combine3<NOA_HIP_INTERP_LINEAR>(c.t);
+                value = tile_shift<SHIFT>(p, b, c.fy, c.u, value);
+                out[frequency2index(p.slice_centered, c.fy, n) * p.os_y + c.u] = value;
+            };
+            auto evaluate = [&](i32 fy, i32 u, bool mirrored) {
+                Cand c;
+                fetch(fy, u, mirrored, c);
+                store(c);
             };
 
             // Plane coordinates: F = Y a + U c (a, c: the 2nd and 3rd columns); on the plane Y = g . F with g the
@@ -519,9 +538,20 @@ __global__ __launch_bounds__(256) void extract_tile_kernel(const TileParams p) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    for (i32 e = lane; e < upto - done; e += 64) {
-                        const u32 w = wl[e];
-                        evaluate(static_cast<i32>(w & 1023u) - 512, static_cast<i32>((w >> 10) & 1023u), (w >> 31) != 0u);
+                    const i32 len = upto - done;
+                    for (i32 e = lane; e < len; e += 128) {
+                        Cand c0, c1;
+                        const u32 w0 = wl[e];
+                        fetch(static_cast<i32>(w0 & 1023u) - 512, static_cast<i32>((w0 >> 10) & 1023u), (w0 >> 31) != 0u,
+                              c0);
+                        c1.ok = false;
+                        if (e + 64 < len) {
+                            const u32 w1 = wl[e + 64];
+                            fetch(static_cast<i32>(w1 & 1023u) - 512, static_cast<i32>((w1 >> 10) & 1023u),
+                                  (w1 >> 31) != 0u, c1);
+                        }
+                        store(c0);
+                        store(c1);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -534,25 +564,54 @@ __global__ __launch_bounds__(256) void extract_tile_kernel(const TileParams p) {
 }
 
 // The pixels outside the cutoff of the tiled extraction: 0, times the phase shift as the gather kernels compute it.
+// Four pixels per lane (consecutive groups of 256). The transform stretches a slice frequency f by at most
+// sqrt(lambda), lambda the largest eigenvalue of G^T G (G: its 2nd and 3rd columns); a pixel with lambda |f|^2 inside
+// the cutoff by a margin far above the rounding is inside after the transform too and is skipped.
 template<int SHIFT>
 __global__ __launch_bounds__(256) void extract_outside_kernel(const TileParams p) {
     const u32 bid = blockIdx.x;
     const i32 b = static_cast<i32>(bid / static_cast<u32>(p.tiles_px));
-    const i32 pix = (static_cast<i32>(bid) - b * p.tiles_px) * 256 + static_cast<i32>(threadIdx.x);
-    if (pix >= p.pixels) return;
+    const i32 first = (static_cast<i32>(bid) - b * p.tiles_px) * 1024 + static_cast<i32>(threadIdx.x);
     const i32 n = p.n, wr = n / 2 + 1;
-    const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
-    const i32 u = pix - yq * wr;
-    const i32 fy = index2frequency(p.slice_centered, yq, n);
+    const float inv_n = 1.f / static_cast<float>(n);
     const Rot<float> r = load_rotation(p.rot, b);
+    const float aa = (r.m[1] * r.m[1] + r.m[4] * r.m[4]) + r.m[7] * r.m[7];
+    const float cc = (r.m[2] * r.m[2] + r.m[5] * r.m[5]) + r.m[8] * r.m[8];
+    const float ac = (r.m[1] * r.m[2] + r.m[4] * r.m[5]) + r.m[7] * r.m[8];
+    const float hd = 0.5f * (aa - cc);
+    const float lambda = 0.5f * (aa + cc) + sqrtf(hd * hd + ac * ac);
+    const float inner = p.cutoff_sqd * (1.f - 1e-4f) / fmaxf(lambda * (1.f + 1e-4f), 1e-30f);
+    bool any = false;
+    i32 yqs[4], us[4], fys[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const i32 pix = first + k * 256;
+        yqs[k] = -1;
+        if (pix >= p.pixels) continue;
+        const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
+        const i32 u = pix - yq * wr;
+        const i32 fy = index2frequency(p.slice_centered, yq, n);
+        const float gy = static_cast<float>(fy) * inv_n, gx = static_cast<float>(u) * inv_n;
+        if (gy * gy + gx * gx < inner) continue;
+        yqs[k] = yq;
+        us[k] = u;
+        fys[k] = fy;
+        any = true;
+    }
+    if (!any) return;
     Scale<float> sc;
     sc.identity = true;
-    vec2<float> f2;
-    f2.y = static_cast<float>(fy) / static_cast<float>(n);
-    f2.x = static_cast<float>(u) / static_cast<float>(n);
-    const vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
-    if (dot3(f) <= p.cutoff_sqd) return;   // computed by the tile kernel
-    p.out[static_cast<i64>(b) * p.os_b + yq * p.os_y + u] = tile_shift<SHIFT>(p, b, fy, u, vzero<float2>());
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (yqs[k] < 0) continue;
+        vec2<float> f2;
+        f2.y = static_cast<float>(fys[k]) / static_cast<float>(n);
+        f2.x = static_cast<float>(us[k]) / static_cast<float>(n);
+        const vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
+        if (dot3(f) <= p.cutoff_sqd) continue;   // computed by the tile kernel
+        p.out[static_cast<i64>(b) * p.os_b + yqs[k] * p.os_y + us[k]] =
+            tile_shift<SHIFT>(p, b, fys[k], us[k], vzero<float2>());
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -809,7 +868,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
             if (layout_env && std::strcmp(layout_env, "tiles") == 0)
                 tiled = cubic && nn >= 16 && nn <= 1000 && static_cast<double>(cutoff) <= 0.5 && p.os_x == 1 &&
                         p.rot.kind == NOA_HIP_MAT33_F32 && stored_elems < 0x7fffffffLL;
-            const i64 tiles_px = (p.pixels + 255) / 256;
+            const i64 tiles_px = (p.pixels + 1023) / 1024;   // outside kernel: 1024 pixels per workgroup
             if (tiled && n_slices * tiles_px <= 0x7fffffffLL) {
                 int device = 0;
                 (void)hipGetDevice(&device);

@@ -1222,7 +1222,7 @@ struct Plan {
     std::vector<int4> h_items, h_reduce;   // host copies (the allocation-free entry point copies them to its scratch)
     std::vector<float> h_work;             // expected passing slices of each item
 };
-using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double, i32, i32, i32, double>;
+using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double, i32, i32, i32, double, int>;
 std::mutex g_plan_mutex;
 std::map<PlanKey, Plan> g_plans;
 
@@ -1247,11 +1247,13 @@ int upload_plan(Plan& pl, std::string& err) {
 
 // The plan of one geometry, built on first use and kept (map nodes are stable, so `plan` stays valid). `upload`:
 // also make the device copies (the allocation-free entry point only reads the host copies).
+// order: 0 = items by expected work, largest first; 1 = by first slice, then by expected work (items running at the
+// same time read neighbouring slices: their footprints are shared in the L2).
 int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff, bool centered, i32 n_slices,
               double fftfreq_blackman, const i32 bd[3], double list_target, bool upload, const Plan*& plan,
-              std::string& err) {
+              std::string& err, int order = 0) {
     const PlanKey key{device, grid[0], grid[1], grid[2], target[0], target[1], target[2], cutoff, centered, n_slices,
-                      fftfreq_blackman, bd[0], bd[1], bd[2], list_target};
+                      fftfreq_blackman, bd[0], bd[1], bd[2], list_target, order};
     std::lock_guard<std::mutex> lock(g_plan_mutex);
     auto it = g_plans.find(key);
     if (it != g_plans.end()) {
@@ -1310,7 +1312,12 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
                     n_partials += made;
                 }
             }
-    std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
+    if (order == 1)
+        std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) {
+            return a.item.y != b.item.y ? a.item.y < b.item.y : a.work > b.work;
+        });
+    else
+        std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
     std::vector<int4> items(entries.size());
     std::vector<float> work(entries.size());
     for (size_t k = 0; k < entries.size(); ++k) {
@@ -1333,6 +1340,7 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
 struct KernelChoice {
     bool force_wave;
     bool paired;          // slab kernel: one 16-byte (8-byte) load for the two x taps of a slice (weight) row
+    int order;            // order of the work items (make_plan)
     double list_target;
 };
 KernelChoice kernel_choice(const ProjectDesc& d) {
@@ -1344,6 +1352,8 @@ KernelChoice kernel_choice(const ProjectDesc& d) {
     k.force_wave = v[0] != 2;
     k.list_target = v[1] > 0 ? std::max(16.0, static_cast<double>(v[1])) : 256.0;
     k.paired = v[2] == 1;
+    k.order = v[3];
+    if (const char* o = std::getenv("NOA_HIP_INSERT_ORDER")) k.order = std::atoi(o);
     if (const char* e = std::getenv("NOA_HIP_INSERT_KERNEL")) k.force_wave = std::string(e) != "slab";
     if (const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED")) k.paired = std::string(pr) == "1";
     if (const char* l = std::getenv("NOA_HIP_INSERT_LIST")) k.list_target = std::max(16.0, std::atof(l));
@@ -1517,7 +1527,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
             const i32 plan_slices = std::max<i32>(64, (p.n_slices + step - 1) / step * step);
             const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
                                      plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target,
-                                     scratch == nullptr, planp, err);
+                                     scratch == nullptr, planp, err, kc.order);
             if (st != NOA_HIP_OK) return st;
             const Plan& plan = *planp;
             if (plan.n_items == 0) return NOA_HIP_OK;
