@@ -9,10 +9,16 @@
 // (global_atomic_add_f32) on the coarse-grained volume; the volume is small enough to stay in L2/MALL, where
 // the atomics execute. Summation order across pixels is not fixed: the result is deterministic only up to
 // float reassociation, as with noa's CUDA and OpenMP paths.
+//
+// Many slices into one cubic volume (FP32 values and weights, Mat33<f32> rotations, cutoff <= 0.5) take the tiled
+// path instead, rasterize_tile_kernel: the accumulation happens in LDS and reaches the volume once per work item.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 namespace noa_hip {
 
@@ -127,6 +133,196 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const RasterParams<C, In
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Tiled rasterisation. The floor positions of the (flipped) pixel coordinates are cut into tiles of 16^3 (as for the
+// tiled extraction, tiles.hpp). A work item is (tile, range of slices), one workgroup of four waves:
+//   - LDS holds the tile's accumulators over 17^3 positions (the trilinear taps reach floor + 1): real and imaginary
+//     parts of the value and the weight, zeroed at the start.
+//   - Each wave takes 64 of the item's slices at a time, keeps those whose plane crosses the tile and enumerates
+//     their candidate pixels (tile_candidates). A candidate runs noa's expressions of rasterize_kernel and is kept
+//     only where its floor lies in the tile, so every pixel inside the cutoff is rasterised by exactly one item; its
+//     eight taps are added to LDS with the LDS float atomic (ds_add_f32).
+//   - The x = 0 duplicates (conjugate at (-w, -v, 0)) go straight to the volume with device-wide atomics, as in
+//     rasterize_kernel: they land anywhere in the x = 0 plane and are a thin set.
+//   - After the item's slices, the 17^3 sums are added to the volume with device-wide atomics at noa's indices:
+//     frequency2index of the position, dropped outside [0, D) x [0, H) x [0, W/2+1) (the same bounds checks
+//     rasterize_kernel applies per tap, so the wrapped taps of the non-centred layout land where they land there).
+//     Neighbouring tiles overlap by one position; both add theirs.
+// Device-wide atomics drop from 24 per pixel to about 14,700 per item; summation order is not fixed (LDS and
+// device atomics), as with rasterize_kernel.
+// ---------------------------------------------------------------------------------------------------------
+template<class In, class Out>
+struct RasterTileParams {
+    const In* in; i64 is_b; i32 is_y, is_x; bool in_is_value; In in_value;
+    const float* win; i64 wis_b; i32 wis_y, wis_x; bool win_present, win_is_value; float win_value;
+    Out* vol; i32 vs0, vs1, vs2;
+    float* wvol; i32 wvs0, wvs1, wvs2; bool has_wvol;
+    XformArg<float> rot;
+    const int4* items;                  // {tile, first slice, end slice, 0}
+    i32 n, nty, ntx, base, n_slices, pixels;
+    float inv_wr, cutoff_sqd, margin;
+    bool vol_centered, slice_centered;
+};
+
+template<class In, class Out>
+__global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTileParams<In, Out> p) {
+    using RIn = typename vt<In>::real;
+    constexpr bool CX = vt<Out>::cx;
+    constexpr i32 NT = XTT * XTT * XTT;
+    __shared__ float acc_re[NT];
+    __shared__ float acc_im[CX ? NT : 1];
+    __shared__ float acc_w[NT];
+    __shared__ u32 list[XT_WAVES][XT_LIST];
+
+    const int4 item = p.items[blockIdx.x];
+    const i32 first = item.y, end = min(item.z, p.n_slices);
+    const i32 tx = item.x % p.ntx, tzy = item.x / p.ntx;
+    const i32 ty = tzy % p.nty, tz = tzy / p.nty;
+    const i32 Z0 = p.base + tz * XT, Y0 = p.base + ty * XT, X0 = tx * XT;
+    const i32 n = p.n, width = n / 2 + 1;
+
+    for (i32 i = static_cast<i32>(threadIdx.x); i < NT; i += 256) {
+        acc_re[i] = 0.f;
+        if constexpr (CX) acc_im[i] = 0.f;
+        acc_w[i] = 0.f;
+    }
+    __syncthreads();
+
+    const i32 wave = static_cast<i32>(threadIdx.x) >> 6, lane = static_cast<i32>(threadIdx.x) & 63;
+    u32* wl = list[wave];
+    const float slice_f = static_cast<float>(n), target = static_cast<float>(n);
+    const float M = p.margin;
+    const float cz = static_cast<float>(Z0) + 0.5f * XT, cy = static_cast<float>(Y0) + 0.5f * XT;
+    const float cx = static_cast<float>(X0) + 0.5f * XT;
+    Scale<float> sc;
+    sc.identity = true;
+
+    for (i32 c0 = first + wave * 64; c0 < end; c0 += 64 * XT_WAVES) {
+        const bool cross = c0 + lane < end && tile_crossed(load_rotation(p.rot, c0 + lane), cz, cy, cx, M);
+        unsigned long long mask = __ballot(cross);
+        while (mask) {
+            const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
+            mask &= mask - 1;
+            const i32 b = c0 + k;
+            const Rot<float> r = load_rotation(p.rot, b);
+
+            // fetch: noa's expressions up to the fractions and the input value and weight (the loads of two
+            // candidates in flight per lane); store: the eight taps into LDS, and the x = 0 duplicates.
+            struct Cand {
+                Out value;
+                float weight, tz, ty, tx;
+                i32 lz, ly, lx;
+                bool ok;
+            };
+            auto fetch = [&](i32 fy, i32 u, bool mirrored, Cand& c) {
+                c.ok = false;
+                vec2<float> f2;
+                f2.y = static_cast<float>(fy) / slice_f;
+                f2.x = static_cast<float>(u) / slice_f;
+                vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
+                if (dot3(f) > p.cutoff_sqd) return;
+                const bool flip = f.x < 0;
+                if (flip != mirrored) return;
+                RIn conjugate = 1;
+                if (flip) {
+                    f.z = -f.z; f.y = -f.y; f.x = -f.x;
+                    if constexpr (vt<In>::cx) conjugate = -1;
+                }
+                const float fz = f.z * target, fyv = f.y * target, fx = f.x * target;
+                const float flz = floorf(fz), fly = floorf(fyv), flx = floorf(fx);
+                const i32 lz = static_cast<i32>(flz) - Z0, ly = static_cast<i32>(fly) - Y0;
+                const i32 lx = static_cast<i32>(flx) - X0;
+                if (static_cast<u32>(lz) >= static_cast<u32>(XT) || static_cast<u32>(ly) >= static_cast<u32>(XT) ||
+                    static_cast<u32>(lx) >= static_cast<u32>(XT))
+                    return;
+                const i32 y = frequency2index(p.slice_centered, fy, n);
+                const In raw = p.in_is_value ? p.in_value : p.in[static_cast<i64>(b) * p.is_b + y * p.is_y + u * p.is_x];
+                if constexpr (vt<In>::cx && CX) c.value = vcast<Out>(vmul(raw, conjugate));
+                else c.value = vcast<Out>(raw);
+                c.weight = 0.f;
+                if (p.has_wvol) {
+                    if (!p.win_present) c.weight = 1.f;
+                    else c.weight = p.win_is_value ? p.win_value
+                                                   : p.win[static_cast<i64>(b) * p.wis_b + y * p.wis_y + u * p.wis_x];
+                }
+                c.tz = fz - flz;
+                c.ty = fyv - fly;
+                c.tx = fx - flx;
+                c.lz = lz; c.ly = ly; c.lx = lx;
+                c.ok = true;
+            };
+            auto store = [&](const Cand& c) {
+                if (!c.ok) return;
+                float fr[2][3];
+                fr[1][0] = c.tz; fr[1][1] = c.ty; fr[1][2] = c.tx;
+                fr[0][0] = 1 - c.tz; fr[0][1] = 1 - c.ty; fr[0][2] = 1 - c.tx;
+                const i32 o = (c.lz * XTT + c.ly) * XTT + c.lx;
+#pragma unroll
+                for (int w = 0; w < 2; ++w)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            const float kk = fr[w][0] * fr[v][1] * fr[uu][2];
+                            const i32 at = o + (w * XTT + v) * XTT + uu;
+                            const Out add = vmul(c.value, kk);
+                            if constexpr (CX) {
+                                atomicAdd(acc_re + at, add.x);
+                                atomicAdd(acc_im + at, add.y);
+                            } else {
+                                atomicAdd(acc_re + at, add);
+                            }
+                            if (p.has_wvol) atomicAdd(acc_w + at, c.weight * kk);
+                        }
+                // The x = 0 plane is redundant: the conjugate at (-w, -v, 0) with the same fraction (:347-372).
+                if (X0 + c.lx == 0) {
+                    Out value = c.value;
+                    if constexpr (CX) value.y = -value.y;
+                    const i32 b0z = Z0 + c.lz, b0y = Y0 + c.ly;
+#pragma unroll
+                    for (int w = 0; w < 2; ++w) {
+                        const i32 iw = frequency2index(p.vol_centered, -(b0z + w), n);
+                        if (iw < 0 || iw >= n) continue;
+#pragma unroll
+                        for (int v = 0; v < 2; ++v) {
+                            const i32 iv = frequency2index(p.vol_centered, -(b0y + v), n);
+                            if (iv < 0 || iv >= n) continue;
+                            const float kk = fr[w][0] * fr[v][1] * fr[0][2];
+                            atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1), vmul(value, kk));
+                            if (p.has_wvol) atomic_add(p.wvol + (iw * p.wvs0 + iv * p.wvs1), c.weight * kk);
+                        }
+                    }
+                }
+            };
+
+            tile_candidates<Cand>(r, Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered, lane, wl, fetch, store);
+        }
+    }
+    __syncthreads();
+
+    // Flush: positions [Z0, Z0 + 16] x [Y0, Y0 + 16] x [X0, X0 + 16] at noa's indices, inside the stored volume.
+    for (i32 i = static_cast<i32>(threadIdx.x); i < NT; i += 256) {
+        const i32 dz = i / (XTT * XTT), rem = i - dz * (XTT * XTT);
+        const i32 dy = rem / XTT, dx = rem - dy * XTT;
+        const i32 iu = X0 + dx;
+        if (iu >= width) continue;
+        const i32 iw = frequency2index(p.vol_centered, Z0 + dz, n);
+        const i32 iv = frequency2index(p.vol_centered, Y0 + dy, n);
+        if (iw < 0 || iw >= n || iv < 0 || iv >= n) continue;
+        if constexpr (CX) {
+            const float re = acc_re[i], im = acc_im[i];
+            if (re != 0.f || im != 0.f) atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1 + iu * p.vs2), make_float2(re, im));
+        } else {
+            const float re = acc_re[i];
+            if (re != 0.f) atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1 + iu * p.vs2), re);
+        }
+        if (p.has_wvol) {
+            const float w = acc_w[i];
+            if (w != 0.f) atomic_add(p.wvol + (iw * p.wvs0 + iv * p.wvs1 + iu * p.wvs2), w);
+        }
+    }
+}
+
 namespace {
 
 template<class C>
@@ -146,6 +342,59 @@ template<class T> T value_of(const Arr& a) {
     } else {
         return static_cast<T>(a.value[0]);
     }
+}
+
+constexpr int NOT_TILED = -1000;
+
+// The tiled path (rasterize_tile_kernel) where it applies: cubic volume and slices, no target shape, no scaling or
+// Ewald sphere, cutoff <= 0.5, Mat33<f32> rotations. NOA_HIP_RASTERIZE_LAYOUT=direct|tiles forces either; by default
+// the tiles take launches of at least 64 slices. NOA_HIP_RASTERIZE_PAIRS: expected crossing slices per work item.
+template<class In, class Out, class C, class W>
+int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStream_t stream, std::string& err) {
+    const i64 nn = p.grid[0], n_slices = d.in_shape[0];
+    const bool cubic = p.grid[1] == nn && p.grid[2] == nn && d.in_shape[2] == nn && d.in_shape[3] == nn &&
+                       d.target[0] == nn && d.target[1] == nn && d.target[2] == nn;
+    const i64 stored = nn * nn * (nn / 2 + 1);
+    const bool eligible = cubic && !p.general && nn >= 16 && nn <= 1000 && d.cutoff <= 0.5 &&
+                          p.rot.kind == NOA_HIP_MAT33_F32 && stored < 0x7fffffffLL && n_slices <= 0x7fffffffLL;
+    if (!eligible) return NOT_TILED;
+    const char* env = std::getenv("NOA_HIP_RASTERIZE_LAYOUT");
+    const bool want = env ? std::strcmp(env, "tiles") == 0 : n_slices >= 64;
+    if (!want) return NOT_TILED;
+    int device = 0;
+    (void)hipGetDevice(&device);
+    i64 step = 1;
+    while (n_slices > 16 * step) step *= 2;
+    const i32 plan_slices = static_cast<i32>((n_slices + step - 1) / step * step);
+    const char* pe = std::getenv("NOA_HIP_RASTERIZE_PAIRS");
+    const i32 pairs = pe ? std::max(8, std::atoi(pe)) : 512;
+    TilePlan plan;
+    const int st = make_tile_plan(device, static_cast<i32>(nn), std::max(d.cutoff, 0.0), plan_slices, pairs, plan, err);
+    if (st != NOA_HIP_OK) return st;
+    if (plan.n_items == 0) return NOA_HIP_OK;
+    RasterTileParams<In, Out> q{};
+    q.in = p.in; q.is_b = p.is_b; q.is_y = p.is_y; q.is_x = p.is_x; q.in_is_value = p.in_is_value;
+    q.in_value = p.in_value;
+    q.win = p.win; q.wis_b = p.wis_b; q.wis_y = p.wis_y; q.wis_x = p.wis_x; q.win_present = p.win_present;
+    q.win_is_value = p.win_is_value; q.win_value = p.win_value;
+    q.vol = p.vol; q.vs0 = p.vs0; q.vs1 = p.vs1; q.vs2 = p.vs2;
+    q.wvol = p.wvol; q.wvs0 = p.wvs0; q.wvs1 = p.wvs1; q.wvs2 = p.wvs2; q.has_wvol = p.has_wvol;
+    q.rot = p.rot;
+    q.items = plan.items;
+    q.n = static_cast<i32>(nn);
+    q.nty = plan.nty; q.ntx = plan.ntx; q.base = plan.base;
+    q.n_slices = static_cast<i32>(n_slices);
+    q.pixels = p.pixels;
+    q.inv_wr = 1.f / static_cast<float>(p.slice_wr);
+    q.cutoff_sqd = p.cutoff_sqd;
+    q.margin = 1e-3f + 1e-5f * static_cast<float>(nn);
+    q.vol_centered = p.vol_centered;
+    q.slice_centered = p.slices_centered;
+    hipLaunchKernelGGL((rasterize_tile_kernel<In, Out>), dim3(static_cast<u32>(plan.n_items)), dim3(256), 0, stream,
+                       q);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+    return NOA_HIP_OK;
 }
 
 template<class C, class In, class Out, class W>
@@ -185,6 +434,12 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
 
     const i64 blocks = d.in_shape[0] * p.tiles;
     if (blocks == 0) return NOA_HIP_OK;
+    if constexpr (std::is_same_v<C, float> && std::is_same_v<W, float> &&
+                  (std::is_same_v<In, float2> || std::is_same_v<In, float>) &&
+                  (std::is_same_v<Out, float2> || std::is_same_v<Out, float>)) {
+        const int st = run_tiles<In, Out>(d, p, stream, err);
+        if (st != NOT_TILED) return st;
+    }
     if (blocks > 0x7fffffffLL) { err = "too many slices for one launch"; return NOA_HIP_ERROR_UNSUPPORTED; }
     hipLaunchKernelGGL((rasterize_kernel<C, In, Out, W>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, p);
     const hipError_t e = hipGetLastError();
