@@ -135,22 +135,34 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const RasterParams<C, In
 
 // ---------------------------------------------------------------------------------------------------------
 // Tiled rasterisation. The floor positions of the (flipped) pixel coordinates are cut into tiles of 16^3 (as for the
-// tiled extraction, tiles.hpp). A work item is (tile, range of slices), one workgroup of four waves:
-//   - LDS holds the tile's accumulators over 17^3 positions (the trilinear taps reach floor + 1): real and imaginary
-//     parts of the value and the weight, zeroed at the start.
-//   - Each wave takes 64 of the item's slices at a time, keeps those whose plane crosses the tile and enumerates
-//     their candidate pixels (tile_candidates). A candidate runs noa's expressions of rasterize_kernel and is kept
-//     only where its floor lies in the tile, so every pixel inside the cutoff is rasterised by exactly one item; its
-//     eight taps are added to LDS with the LDS float atomic (ds_add_f32).
+// tiled extraction, tiles.hpp). A work item is (tile, range of slices), one workgroup of four waves; wave w owns the
+// slab of floor positions z in [Z0 + 4w, Z0 + 4w + 4) of the tile:
+//   - LDS holds each wave's own accumulators over its slab and the layer above it (the trilinear taps reach
+//     floor + 1), 5 x 17 x 17 positions of value and weight, zeroed at the start. No other wave touches them.
+//   - The wave visits every slice of the item whose plane crosses its slab and enumerates the slab's candidate
+//     pixels (tile_candidates). A candidate runs noa's expressions of rasterize_kernel and is kept only where its
+//     floor lies in the slab, so every pixel inside the cutoff is rasterised by exactly one wave of one item.
+//   - The eight taps are added with plain LDS read-modify-write, tap by tap in program order (LDS executes one
+//     wave's instructions in order; compiler barriers keep the order). Lanes whose candidates share a floor
+//     position would update the same words in one instruction: each lane writes its id into a tag byte of its
+//     floor position and reads it back, the lane that reads its own id does the read-modify-write and the others
+//     add with the LDS float atomic (ds_add_f32) after it. Two pixels share a floor position only when they are
+//     neighbours in the slice, so these lanes are few; the float atomic costs about two clocks per lane.
 //   - The x = 0 duplicates (conjugate at (-w, -v, 0)) go straight to the volume with device-wide atomics, as in
 //     rasterize_kernel: they land anywhere in the x = 0 plane and are a thin set.
-//   - After the item's slices, the 17^3 sums are added to the volume with device-wide atomics at noa's indices:
-//     frequency2index of the position, dropped outside [0, D) x [0, H) x [0, W/2+1) (the same bounds checks
+//   - After the item's slices, each wave adds its 5 x 17 x 17 sums to the volume with device-wide atomics at noa's
+//     indices: frequency2index of the position, dropped outside [0, D) x [0, H) x [0, W/2+1) (the bounds checks
 //     rasterize_kernel applies per tap, so the wrapped taps of the non-centred layout land where they land there).
-//     Neighbouring tiles overlap by one position; both add theirs.
-// Device-wide atomics drop from 24 per pixel to about 14,700 per item; summation order is not fixed (LDS and
-// device atomics), as with rasterize_kernel.
+//     Neighbouring slabs and tiles overlap by one layer; both add theirs.
+// Summation order is not fixed (the device atomics), as with rasterize_kernel.
 // ---------------------------------------------------------------------------------------------------------
+constexpr i32 RS_Z = 4;                             // floor layers per wave
+constexpr i32 RS_L = RS_Z + 1;                      // LDS layers per wave
+constexpr i32 RS_CELLS = RS_L * XTT * XTT;          // accumulator positions per wave
+constexpr i32 RS_TAGS = RS_Z * XT * XT;             // floor positions per wave
+constexpr i32 RS_LIST = 384;                        // candidate pixels per wave and round
+static_assert(RS_Z * XT_WAVES == XT, "the waves' slabs cover the tile");
+
 template<class In, class Out>
 struct RasterTileParams {
     const In* in; i64 is_b; i32 is_y, is_x; bool in_is_value; In in_value;
@@ -164,41 +176,44 @@ struct RasterTileParams {
     bool vol_centered, slice_centered;
 };
 
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
 template<class In, class Out>
 __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTileParams<In, Out> p) {
     using RIn = typename vt<In>::real;
     constexpr bool CX = vt<Out>::cx;
-    constexpr i32 NT = XTT * XTT * XTT;
-    __shared__ float acc_re[NT];
-    __shared__ float acc_im[CX ? NT : 1];
-    __shared__ float acc_w[NT];
-    __shared__ u32 list[XT_WAVES][XT_LIST];
+    __shared__ Out vacc_all[XT_WAVES][RS_CELLS];
+    __shared__ float wacc_all[XT_WAVES][RS_CELLS];
+    __shared__ unsigned char tags_all[XT_WAVES][RS_TAGS];
+    __shared__ u32 list[XT_WAVES][RS_LIST];
 
     const int4 item = p.items[blockIdx.x];
     const i32 first = item.y, end = min(item.z, p.n_slices);
     const i32 tx = item.x % p.ntx, tzy = item.x / p.ntx;
     const i32 ty = tzy % p.nty, tz = tzy / p.nty;
-    const i32 Z0 = p.base + tz * XT, Y0 = p.base + ty * XT, X0 = tx * XT;
-    const i32 n = p.n, width = n / 2 + 1;
-
-    for (i32 i = static_cast<i32>(threadIdx.x); i < NT; i += 256) {
-        acc_re[i] = 0.f;
-        if constexpr (CX) acc_im[i] = 0.f;
-        acc_w[i] = 0.f;
-    }
-    __syncthreads();
-
     const i32 wave = static_cast<i32>(threadIdx.x) >> 6, lane = static_cast<i32>(threadIdx.x) & 63;
+    const i32 Z0 = p.base + tz * XT + wave * RS_Z, Y0 = p.base + ty * XT, X0 = tx * XT;   // this wave's slab
+    const i32 n = p.n, width = n / 2 + 1;
+    Out* vacc = vacc_all[wave];
+    float* wacc = wacc_all[wave];
+    unsigned char* tags = tags_all[wave];
     u32* wl = list[wave];
+
+    for (i32 i = lane; i < RS_CELLS; i += 64) {
+        vacc[i] = vzero<Out>();
+        wacc[i] = 0.f;
+    }
+    lds_order();
+
     const float slice_f = static_cast<float>(n), target = static_cast<float>(n);
     const float M = p.margin;
-    const float cz = static_cast<float>(Z0) + 0.5f * XT, cy = static_cast<float>(Y0) + 0.5f * XT;
+    const float cz = static_cast<float>(Z0) + 0.5f * RS_Z, cy = static_cast<float>(Y0) + 0.5f * XT;
     const float cx = static_cast<float>(X0) + 0.5f * XT;
     Scale<float> sc;
     sc.identity = true;
 
-    for (i32 c0 = first + wave * 64; c0 < end; c0 += 64 * XT_WAVES) {
-        const bool cross = c0 + lane < end && tile_crossed(load_rotation(p.rot, c0 + lane), cz, cy, cx, M);
+    for (i32 c0 = first; c0 < end; c0 += 64) {
+        const bool cross = c0 + lane < end && slab_crossed<RS_Z>(load_rotation(p.rot, c0 + lane), cz, cy, cx, M);
         unsigned long long mask = __ballot(cross);
         while (mask) {
             const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
@@ -232,7 +247,7 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                 const float flz = floorf(fz), fly = floorf(fyv), flx = floorf(fx);
                 const i32 lz = static_cast<i32>(flz) - Z0, ly = static_cast<i32>(fly) - Y0;
                 const i32 lx = static_cast<i32>(flx) - X0;
-                if (static_cast<u32>(lz) >= static_cast<u32>(XT) || static_cast<u32>(ly) >= static_cast<u32>(XT) ||
+                if (static_cast<u32>(lz) >= static_cast<u32>(RS_Z) || static_cast<u32>(ly) >= static_cast<u32>(XT) ||
                     static_cast<u32>(lx) >= static_cast<u32>(XT))
                     return;
                 const i32 y = frequency2index(p.slice_centered, fy, n);
@@ -252,7 +267,13 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                 c.ok = true;
             };
             auto store = [&](const Cand& c) {
-                if (!c.ok) return;
+                // One lane per floor position does the plain read-modify-write, the others the atomic add.
+                const i32 cell = (c.lz * XT + c.ly) * XT + c.lx;
+                if (c.ok) tags[cell] = static_cast<unsigned char>(lane);
+                lds_order();
+                const bool own = c.ok && tags[cell] == static_cast<unsigned char>(lane);
+                const bool shared = c.ok && !own;
+                lds_order();
                 float fr[2][3];
                 fr[1][0] = c.tz; fr[1][1] = c.ty; fr[1][2] = c.tx;
                 fr[0][0] = 1 - c.tz; fr[0][1] = 1 - c.ty; fr[0][2] = 1 - c.tx;
@@ -266,16 +287,28 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                             const float kk = fr[w][0] * fr[v][1] * fr[uu][2];
                             const i32 at = o + (w * XTT + v) * XTT + uu;
                             const Out add = vmul(c.value, kk);
-                            if constexpr (CX) {
-                                atomicAdd(acc_re + at, add.x);
-                                atomicAdd(acc_im + at, add.y);
-                            } else {
-                                atomicAdd(acc_re + at, add);
+                            const float wadd = c.weight * kk;
+                            if (own) {
+                                // Value and weight together (the weight is 0 without a weight volume).
+                                const Out v0 = vacc[at];
+                                const float w0 = wacc[at];
+                                vacc[at] = vadd(v0, add);
+                                wacc[at] = w0 + wadd;
                             }
-                            if (p.has_wvol) atomicAdd(acc_w + at, c.weight * kk);
+                            lds_order();
+                            if (shared) {
+                                if constexpr (CX) {
+                                    atomicAdd(&vacc[at].x, add.x);
+                                    atomicAdd(&vacc[at].y, add.y);
+                                } else {
+                                    atomicAdd(vacc + at, add);
+                                }
+                                if (p.has_wvol) atomicAdd(wacc + at, wadd);
+                            }
+                            lds_order();
                         }
                 // The x = 0 plane is redundant: the conjugate at (-w, -v, 0) with the same fraction (:347-372).
-                if (X0 + c.lx == 0) {
+                if (c.ok && X0 + c.lx == 0) {
                     Out value = c.value;
                     if constexpr (CX) value.y = -value.y;
                     const i32 b0z = Z0 + c.lz, b0y = Y0 + c.ly;
@@ -295,13 +328,14 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                 }
             };
 
-            tile_candidates<Cand>(r, Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered, lane, wl, fetch, store);
+            tile_candidates<Cand, RS_Z, RS_LIST>(r, Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered, lane, wl,
+                                                 fetch, store);
         }
     }
-    __syncthreads();
+    lds_order();
 
-    // Flush: positions [Z0, Z0 + 16] x [Y0, Y0 + 16] x [X0, X0 + 16] at noa's indices, inside the stored volume.
-    for (i32 i = static_cast<i32>(threadIdx.x); i < NT; i += 256) {
+    // Flush: positions [Z0, Z0 + 4] x [Y0, Y0 + 16] x [X0, X0 + 16] at noa's indices, inside the stored volume.
+    for (i32 i = lane; i < RS_CELLS; i += 64) {
         const i32 dz = i / (XTT * XTT), rem = i - dz * (XTT * XTT);
         const i32 dy = rem / XTT, dx = rem - dy * XTT;
         const i32 iu = X0 + dx;
@@ -309,15 +343,14 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
         const i32 iw = frequency2index(p.vol_centered, Z0 + dz, n);
         const i32 iv = frequency2index(p.vol_centered, Y0 + dy, n);
         if (iw < 0 || iw >= n || iv < 0 || iv >= n) continue;
+        const Out v = vacc[i];
         if constexpr (CX) {
-            const float re = acc_re[i], im = acc_im[i];
-            if (re != 0.f || im != 0.f) atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1 + iu * p.vs2), make_float2(re, im));
+            if (v.x != 0.f || v.y != 0.f) atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1 + iu * p.vs2), v);
         } else {
-            const float re = acc_re[i];
-            if (re != 0.f) atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1 + iu * p.vs2), re);
+            if (v != 0.f) atomic_add(p.vol + (iw * p.vs0 + iv * p.vs1 + iu * p.vs2), v);
         }
         if (p.has_wvol) {
-            const float w = acc_w[i];
+            const float w = wacc[i];
             if (w != 0.f) atomic_add(p.wvol + (iw * p.wvs0 + iv * p.wvs1 + iu * p.wvs2), w);
         }
     }
@@ -355,7 +388,8 @@ int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStr
     const bool cubic = p.grid[1] == nn && p.grid[2] == nn && d.in_shape[2] == nn && d.in_shape[3] == nn &&
                        d.target[0] == nn && d.target[1] == nn && d.target[2] == nn;
     const i64 stored = nn * nn * (nn / 2 + 1);
-    const bool eligible = cubic && !p.general && nn >= 16 && nn <= 1000 && d.cutoff <= 0.5 &&
+    // The candidate list of a wave must hold a whole slice row (n/2 + 1 pixels) for any matrix.
+    const bool eligible = cubic && !p.general && nn >= 16 && nn / 2 + 1 <= RS_LIST && d.cutoff <= 0.5 &&
                           p.rot.kind == NOA_HIP_MAT33_F32 && stored < 0x7fffffffLL && n_slices <= 0x7fffffffLL;
     if (!eligible) return NOT_TILED;
     const char* env = std::getenv("NOA_HIP_RASTERIZE_LAYOUT");
@@ -367,7 +401,7 @@ int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStr
     while (n_slices > 16 * step) step *= 2;
     const i32 plan_slices = static_cast<i32>((n_slices + step - 1) / step * step);
     const char* pe = std::getenv("NOA_HIP_RASTERIZE_PAIRS");
-    const i32 pairs = pe ? std::max(8, std::atoi(pe)) : 512;
+    const i32 pairs = pe ? std::max(8, std::atoi(pe)) : 256;
     TilePlan plan;
     const int st = make_tile_plan(device, static_cast<i32>(nn), std::max(d.cutoff, 0.0), plan_slices, pairs, plan, err);
     if (st != NOA_HIP_OK) return st;

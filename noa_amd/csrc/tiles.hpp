@@ -44,13 +44,27 @@ __device__ __forceinline__ bool tile_crossed(const Rot<float>& r, float cz, floa
     return d <= (0.5f * XT) * sum + M * sum + 1e-6f;
 }
 
-// The candidate pixels of one (tile, crossing slice) pair. Cand: the caller's candidate type with a `bool ok`;
-// fetch(fy, u, mirrored, Cand&) evaluates one candidate, store(const Cand&) finishes it. `wl`: the wave's list of
-// XT_LIST words in LDS. A degenerate transform (collapsed plane) makes every pixel of the slice a candidate.
-template<class Cand, class Fetch, class Store>
+// The same test for a box of EZ x 16 x 16 positions (a slab of a tile), centre c.
+template<int EZ>
+__device__ __forceinline__ bool slab_crossed(const Rot<float>& r, float cz, float cy, float cx, float M) {
+    const float nz = r.m[4] * r.m[8] - r.m[7] * r.m[5];
+    const float ny = r.m[7] * r.m[2] - r.m[1] * r.m[8];
+    const float nx = r.m[1] * r.m[5] - r.m[4] * r.m[2];
+    const float d = fabsf((nz * cz + ny * cy) + nx * cx);
+    const float sum = fabsf(nz) + fabsf(ny) + fabsf(nx);
+    return d <= (0.5f * EZ) * fabsf(nz) + (0.5f * XT) * (fabsf(ny) + fabsf(nx)) + M * sum + 1e-6f;
+}
+
+// The candidate pixels of one (box, crossing slice) pair; the box is [Z0, Z0 + EZ) x [Y0, Y0 + 16) x [X0, X0 + 16)
+// (a whole tile when EZ = 16). Cand: the caller's candidate type with a `bool ok`; fetch(fy, u, mirrored, Cand&)
+// evaluates one candidate, store(const Cand&) finishes it. `wl`: the wave's list of LIST words in LDS, which must
+// hold a whole row (n/2 + 1 <= LIST: rows enter the list whole). A degenerate transform (collapsed plane) makes every
+// pixel of the slice a candidate.
+template<class Cand, int EZ = XT, int LIST = XT_LIST, class Fetch, class Store>
 __device__ __forceinline__ void tile_candidates(const Rot<float>& r, i32 Z0, i32 Y0, i32 X0, i32 n, float M, i32 pixels,
                                                 float inv_wr, bool slice_centered, i32 lane, u32* wl, Fetch& fetch,
                                                 Store& store) {
+    constexpr i32 EXT[3] = {EZ, XT, XT};
     const i32 lo = -n / 2, hi = (n - 1) / 2, hix = n / 2;
     // Plane coordinates: F = Y a + U c (a, c: the 2nd and 3rd columns); on the plane Y = g . F with g the
     // first row of (G^T G)^-1 G^T, G = [a c].
@@ -77,14 +91,14 @@ __device__ __forceinline__ void tile_candidates(const Rot<float>& r, i32 Z0, i32
     const float g[3] = {(cc * a[0] - ac * c[0]) * idet, (cc * a[1] - ac * c[1]) * idet,
                         (cc * a[2] - ac * c[2]) * idet};
     const bool mirrored = lane >= 32;
-    // The box of this lane's set, widened by the margin: direct F in [P0, P0 + 16), mirrored F in
-    // (-P0 - 16, -P0].
+    // The box of this lane's set, widened by the margin: direct F in [P0, P0 + EXT), mirrored F in
+    // (-P0 - EXT, -P0].
     const float P0[3] = {static_cast<float>(Z0), static_cast<float>(Y0), static_cast<float>(X0)};
     float blo[3], bhi[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        blo[i] = (mirrored ? -(P0[i] + XT) : P0[i]) - M;
-        bhi[i] = (mirrored ? -P0[i] : P0[i] + XT) + M;
+        blo[i] = (mirrored ? -(P0[i] + EXT[i]) : P0[i]) - M;
+        bhi[i] = (mirrored ? -P0[i] : P0[i] + EXT[i]) + M;
     }
     float ymin = 0.f, ymax = 0.f;
 #pragma unroll
@@ -121,7 +135,7 @@ __device__ __forceinline__ void tile_candidates(const Rot<float>& r, i32 Z0, i32
         const i32 total = __builtin_amdgcn_readlane(incl, 63);
         for (i32 done = 0; done < total;) {
             // Whole rows into the list, as many as fit (rows are short: at most n/2 + 1 pixels).
-            const bool fits = cnt > 0 && excl >= done && incl <= done + XT_LIST;
+            const bool fits = cnt > 0 && excl >= done && incl <= done + LIST;
             const unsigned long long fm = __ballot(fits);
             const i32 last = 63 - __clzll(static_cast<long long>(fm));
             const i32 upto = __builtin_amdgcn_readlane(incl, last);
