@@ -341,8 +341,9 @@ def precision_report(ext, nx, torch, n_ref=128):
 
 def neighbours_report(torch, dev, nx, ins, n, reps=5):
     """The steps on either side of the hot path, timed alone (not part of the step): the CTF prologue of the
-    insertion (slice * CTF and CTF^2 in one pass over the insertion workload's raw spectra) and an extraction from a
-    virtual volume (a 61-view +-60 degree tilt series of 256^2 slices reprojected at 61 tilts offset by 1 degree)."""
+    insertion (slice * CTF and CTF^2 in one pass over the insertion workload's raw spectra), an extraction from a
+    virtual volume (a 61-view +-60 degree tilt series of 256^2 slices reprojected at 61 tilts offset by 1 degree),
+    the rasterisation of 4096 slices and the isotropic CTF of a volume."""
     import numpy as np
     import noa_amd.signal as ns
     rep = {}
@@ -350,7 +351,7 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
     vals = torch.empty_like(ins.raw)
     wts = torch.empty((b, 1, n, n // 2 + 1), dtype=torch.float32, device=dev)
 
-    def timed(fn):
+    def timed(fn, reps=reps):
         fn()
         torch.cuda.synchronize()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -392,19 +393,31 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
     log(f"virtual volume: {t * 1e3:.3f} ms, {pairs / t / 1e9:.1f} G pixel-slice pairs/s")
     del out, wout
 
-    # Rasterisation (trilinear scatter with float atomics) of 4096 of the insertion workload's slices into a 256^3
-    # volume + weights, forward rotations (h2h, cutoff 0.5).
+    # Rasterisation (trilinear scatter) of 4096 of the insertion workload's slices into a 256^3 volume + weights,
+    # forward rotations (h2h, cutoff 0.5): the tiled kernel (per-wave LDS slabs, one device-atomic flush per work
+    # item), and once the one-lane-per-pixel kernel with device-wide float atomics for comparison.
     b_r = min(4096, b)
     rvol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
     rw = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=dev)
     fwd = torch.from_numpy(np.ascontiguousarray(ins.poses[:b_r])).to(dev)
     t = timed(lambda: nx.rasterize_central_slices_3d("h2h", ins.slices[:b_r], None, (b_r, 1, n, n), rvol, rw,
                                                      ins.volume_shape, None, fwd))
+    prev = os.environ.get("NOA_HIP_RASTERIZE_LAYOUT")
+    os.environ["NOA_HIP_RASTERIZE_LAYOUT"] = "direct"
+    try:
+        t_direct = timed(lambda: nx.rasterize_central_slices_3d("h2h", ins.slices[:b_r], None, (b_r, 1, n, n), rvol,
+                                                                rw, ins.volume_shape, None, fwd), reps=1)
+    finally:
+        if prev is None:
+            os.environ.pop("NOA_HIP_RASTERIZE_LAYOUT")
+        else:
+            os.environ["NOA_HIP_RASTERIZE_LAYOUT"] = prev
     px = b_r * n * (n // 2 + 1)
     rep["rasterize"] = {
         "workload": f"rasterize_central_slices_3d of {b_r} slices 256^2 into a 256^3 c32 volume + f32 weights (h2h, "
-                    "cutoff 0.5)", "avg_ms": t * 1e3, "slices_per_s": b_r / t, "pixels_per_s": px / t}
-    log(f"rasterize {b_r} slices: {t * 1e3:.3f} ms")
+                    "cutoff 0.5)", "avg_ms": t * 1e3, "slices_per_s": b_r / t, "pixels_per_s": px / t,
+        "kernel": "tiled (LDS slabs per wave)", "direct_kernel_ms": t_direct * 1e3}
+    log(f"rasterize {b_r} slices: {t * 1e3:.3f} ms (direct kernel {t_direct * 1e3:.1f} ms)")
     del rvol, rw, fwd
 
     # Isotropic CTF of a reconstructed 256^3 rfft volume (h2h): the volume multiplied by CTF(|f|) in place.

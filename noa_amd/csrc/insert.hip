@@ -589,7 +589,9 @@ __global__ __launch_bounds__(64) void insert_wave_kernel(const InsertParams<C, I
     __shared__ In acc_v[WVOX];
     __shared__ W acc_w[WVOX];
     __shared__ i32 cand[64 * WG];          // packed candidates of one pass: column << 4 | k
-    insert_wave_item<C, In, Out, W, INTERP, HOT>(p, p.items[blockIdx.x], acc_v, acc_w, cand);
+    const int4 item = p.items[blockIdx.x];
+    if (item.x < 0) return;                 // padding of an XCD-ordered plan
+    insert_wave_item<C, In, Out, W, INTERP, HOT>(p, item, acc_v, acc_w, cand);
 }
 
 // Batched insertion: several insertion calls (each its own volume, box size, layout, slice count and cutoff) in one
@@ -824,6 +826,7 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
 
     const i32 lane = static_cast<i32>(threadIdx.x);
     int4 item = p.items[blockIdx.x];
+    if (item.x < 0) return;                 // padding of an XCD-ordered plan
     item.z = min(item.z, p.n_slices);
     item.y = min(item.y, item.z);
     const i32 brick = item.x;
@@ -1248,7 +1251,7 @@ int upload_plan(Plan& pl, std::string& err) {
 // The plan of one geometry, built on first use and kept (map nodes are stable, so `plan` stays valid). `upload`:
 // also make the device copies (the allocation-free entry point only reads the host copies).
 // order: 0 = items by expected work, largest first; 1 = by first slice, then by expected work (items running at the
-// same time read neighbouring slices: their footprints are shared in the L2).
+// same time read neighbouring slices: their footprints are shared in the L2); 2 = XCD-aware spatial order (below).
 int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff, bool centered, i32 n_slices,
               double fftfreq_blackman, const i32 bd[3], double list_target, bool upload, const Plan*& plan,
               std::string& err, int order = 0) {
@@ -1312,12 +1315,47 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
                     n_partials += made;
                 }
             }
-    if (order == 1)
+    if (order == 1) {
         std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) {
             return a.item.y != b.item.y ? a.item.y < b.item.y : a.work > b.work;
         });
-    else
+    } else if (order == 2) {
+        // XCD-aware: workgroup i runs on XCD i % 8. The sub-bricks are put in Morton order and cut into 8 contiguous
+        // runs of equal expected work; item i of the array comes from run i % 8, in order, so each XCD walks one
+        // compact region and the items resident on it at a time are neighbours reading the same slices (their
+        // footprints shared in that XCD's L2). Runs shorter than the longest are padded with no-op items (x = -1).
+        auto morton = [&](i32 brick) {
+            const i32 bx = brick % nb[2], r = brick / nb[2], by = r % nb[1], bz = r / nb[1];
+            unsigned long long c = 0;
+            for (int k = 0; k < 11; ++k)
+                c |= (static_cast<unsigned long long>((bz >> k) & 1) << (3 * k + 2)) |
+                     (static_cast<unsigned long long>((by >> k) & 1) << (3 * k + 1)) |
+                     (static_cast<unsigned long long>((bx >> k) & 1) << (3 * k));
+            return c;
+        };
+        std::stable_sort(entries.begin(), entries.end(), [&](const Entry& a, const Entry& b) {
+            const unsigned long long ma = morton(a.item.x), mb = morton(b.item.x);
+            return ma != mb ? ma < mb : a.item.y < b.item.y;
+        });
+        double total = 0;
+        for (const Entry& e : entries) total += e.work;
+        std::vector<Entry> runs[8];
+        double cum = 0;
+        for (const Entry& e : entries) {
+            const int r = total > 0 ? std::min(7, static_cast<int>(8 * (cum + 0.5 * e.work) / total)) : 0;
+            runs[r].push_back(e);
+            cum += e.work;
+        }
+        size_t longest = 0;
+        for (const auto& r : runs) longest = std::max(longest, r.size());
+        std::vector<Entry> out;
+        out.reserve(8 * longest);
+        for (size_t j = 0; j < longest; ++j)
+            for (const auto& r : runs) out.push_back(j < r.size() ? r[j] : Entry{0.0, make_int4(-1, 0, 0, -1)});
+        entries.swap(out);
+    } else {
         std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
+    }
     std::vector<int4> items(entries.size());
     std::vector<float> work(entries.size());
     for (size_t k = 0; k < entries.size(); ++k) {

@@ -211,6 +211,30 @@ def test_insert_parity(remap):
     assert rel_l2(cpu(gw), wvol) < 1e-6
 
 
+@pytest.mark.parametrize("kernel", ["wave", "slab"])
+def test_insert_work_orders_give_the_same_bits(monkeypatch, kernel):
+    """The work plan's item order (0: expected work, 1: first slice, 2: XCD-aware Morton runs padded with no-op items)
+    only changes when each item runs: every voxel's sum, and the split sub-bricks' reduction, are the same bits."""
+    print(f"case insert work orders {kernel}", flush=True)
+    monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", kernel)
+    monkeypatch.setenv("NOA_HIP_INSERT_LIST", "32")            # many split sub-bricks
+    rng = np.random.default_rng(122)
+    n, b = 64, 400
+    s = to_dev(random_spectrum((b, 1, n, n // 2 + 1), rng))
+    w = to_dev(rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32))
+    inv = to_dev(np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1))))
+    outs = []
+    for order in ("0", "1", "2"):
+        monkeypatch.setenv("NOA_HIP_INSERT_ORDER", order)
+        gv = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+        gw = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+        nx().insert_central_slices_3d("h2h", s, w, (b, 1, n, n), gv, gw, (1, n, n, n), None, inv)
+        outs.append((cpu(gv), cpu(gw)))
+    for v, wt in outs[1:]:
+        np.testing.assert_array_equal(v, outs[0][0])
+        np.testing.assert_array_equal(wt, outs[0][1])
+
+
 def test_insert_defaults_value_input_single_slice_broadcast():
     print("case insert value/broadcast", flush=True)
     rng = np.random.default_rng(121)

@@ -113,3 +113,41 @@ def test_budget_is_respected_and_large_requests_still_run():
     assert len(stats.latencies_ms) == len(reqs)
     largest = max(r.output_bytes() for r in reqs)
     assert peak <= max(budget, largest)
+
+
+def test_full_serving_mix_alone_and_in_the_mix():
+    """The serving workload's whole box range (64-384) and slice counts up to 512: requests of every box, extraction
+    and insertion, including K = 512 at 384^3, give the same bits alone and in the mix."""
+    from noa_amd import workloads
+    from noa_amd.serve import Request, Server, run_alone, run_open_loop
+    dev = torch.device("cuda")
+    vols = workloads.serving_volumes(dev, per_box=1)
+    rng = np.random.default_rng(21)
+    g = torch.Generator(device=dev).manual_seed(21)
+    reqs = []
+    plan = [(b, kind) for b in workloads.SERVING_BOXES for kind in ("extract", "insert")]
+    for i, (box, kind) in enumerate(plan):
+        k = 512 if box == 384 else int(rng.integers(1, 513))
+        poses = workloads.uniform_rotations(k, rng)
+        if kind == "extract":
+            reqs.append(Request(rid=i, kind="extract", box=box, volume=0, rotations=torch.from_numpy(poses).to(dev),
+                                arrival=0.0))
+        else:
+            spec = torch.fft.rfft2(torch.randn((k, box, box), generator=g, device=dev), norm="forward")
+            w = torch.rand((k, box, box // 2 + 1), generator=g, device=dev)
+            inv = np.ascontiguousarray(np.transpose(poses, (0, 2, 1)))
+            reqs.append(Request(rid=i, kind="insert", box=box, rotations=torch.from_numpy(inv).to(dev),
+                                slices=spec.reshape(k, 1, box, box // 2 + 1).contiguous(),
+                                weights=w.reshape(k, 1, box, box // 2 + 1).contiguous(), arrival=0.0))
+    print(f"case full serving mix: {len(reqs)} requests, boxes {sorted({r.box for r in reqs})}", flush=True)
+    torch.cuda.synchronize()
+    server = Server(vols, budget_bytes=4 << 30, n_streams=3)
+    stats = run_open_loop(server, reqs, timeout_s=120)
+    torch.cuda.synchronize()
+    assert len(stats.latencies_ms) == len(reqs)
+    assert max(stats.batch_sizes) > 1
+    for r in reqs:
+        alone = run_alone(Server(vols, budget_bytes=4 << 30, n_streams=1), _clone(r))
+        assert torch.equal(r.result, alone.result), f"request {r.rid} ({r.kind}, box {r.box}) differs"
+        if r.kind == "insert":
+            assert torch.equal(r.result_weight, alone.result_weight)
