@@ -301,12 +301,13 @@ int noa_hip_insert_batch_launch(const void* device_image, void* stream);
  * wins, the later one on a tie. Values not set (0) take the defaults.
  *   NOA_HIP_OP_EXTRACT:       values = {volume layout: 0 auto, 1 row-major gather, 2 block-copy gather, 3 LDS tiles;
  *                                       expected crossing slices per tile work item (default 128); -; -}
- *   NOA_HIP_OP_INSERT:        values = {kernel: 0 auto, 1 one-wave, 2 slab; expected slices per work item (default
- *                                       256); 1: one load for both x taps of a slab row; -}
+ *   NOA_HIP_OP_INSERT:        values = {kernel: 0 auto (slab where it applies), 1 one-wave, 2 slab; expected
+ *                                       slices per work item (default 256); x taps of a slab row: 0 auto (one load),
+ *                                       1 one load, 2 two loads; item order: 0 by work, 1 by first slice, 2 XCD-aware}
  *   NOA_HIP_OP_EXTRACT_BATCH: values = {pixels per lane, 1-4 (default 3); workgroups (default 2048); -; -}
  *   NOA_HIP_OP_INSERT_BATCH:  values = {workgroups (default 3072); -; -; -}
  * Environment variables (NOA_HIP_EXTRACT_LAYOUT, NOA_HIP_EXTRACT_PAIRS, NOA_HIP_INSERT_KERNEL, NOA_HIP_INSERT_LIST,
- * NOA_HIP_INSERT_PAIRED) override the table. */
+ * NOA_HIP_INSERT_PAIRED, NOA_HIP_INSERT_ORDER) override the table. */
 enum {
     NOA_HIP_OP_EXTRACT = 1,
     NOA_HIP_OP_INSERT = 2,

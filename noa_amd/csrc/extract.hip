@@ -40,6 +40,7 @@ struct ExtractParams {
     C target[3];
     C slice_f[2];
     i32 slice_h, slice_wr, pixels, tiles, ppt;
+    i32 patch_wgs, patch_pc;            // extract_c32_kernel<PATCH>: workgroups of 8x8 patches per slice, patches per row
     float inv_wr;          // 1 / slice_wr (lean kernel's pixel -> row)
     C cutoff_sqd;
     C ews[2];
@@ -231,7 +232,11 @@ __global__ __launch_bounds__(256) void brick_volume_kernel(const S* __restrict__
     out[L.z(z) + L.y(y) + L.x(x)] = in[static_cast<i64>(z) * vs0 + static_cast<i64>(y) * vs1 + x];
 }
 
-template<int SHIFT, bool BRICK, class S>
+// PATCH: each wave takes an 8 x 8 patch of pixels (rows y0..y0+7, columns u0..u0+7) instead of 64 consecutive
+// pixels of a row: the patch samples a compact piece of the plane, so each tap load touches fewer cache lines of the
+// volume. Workgroups [0, patch_wgs) of a slice hold four patches each (columns 0..wr-2); the next ones take the last
+// column (u = wr - 1), 256 rows per workgroup.
+template<int SHIFT, bool BRICK, class S, bool PATCH = false>
 __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<float> p) {
     const u32 lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const i32 b = static_cast<i32>(lb / static_cast<u32>(p.tiles));
@@ -250,20 +255,37 @@ __global__ __launch_bounds__(256) void extract_c32_kernel(const ExtractParams<fl
     const S* __restrict__ vol = static_cast<const S*>(p.vol);
     float2* __restrict__ out = static_cast<float2*>(p.out) + static_cast<i64>(b) * p.os_b;
 
-    constexpr i32 MAXP = 4;
+    constexpr i32 MAXP = PATCH ? 1 : 4;
     Taps3<float2, float> tv[MAXP];
     bool inside[MAXP];
     i32 ys[MAXP], us[MAXP];
     const i32 first = tile * p.ppt * 256 + static_cast<i32>(threadIdx.x);
 #pragma unroll
     for (i32 k = 0; k < MAXP; ++k) {
-        const i32 pix = first + k * 256;
-        const bool live = k < p.ppt && pix < p.pixels;
-        // pix / wr from a float reciprocal: (pix + 0.5) / wr is at least 0.5 / wr away from an integer, far above
-        // the rounding error for pix < 2^21.
-        const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
-        const i32 y = live ? yq : 0;
-        const i32 u = live ? pix - yq * p.slice_wr : 0;
+        bool live;
+        i32 y, u;
+        if constexpr (PATCH) {
+            const i32 wave = static_cast<i32>(threadIdx.x) >> 6, lane = static_cast<i32>(threadIdx.x) & 63;
+            if (tile < p.patch_wgs) {
+                const i32 q = tile * 4 + wave, pr = q / p.patch_pc, pc = q - pr * p.patch_pc;
+                y = pr * 8 + (lane >> 3);
+                u = pc * 8 + (lane & 7);
+                live = true;
+            } else {
+                y = (tile - p.patch_wgs) * 256 + static_cast<i32>(threadIdx.x);
+                u = p.slice_wr - 1;
+                live = y < p.slice_h;
+                y = live ? y : 0;
+            }
+        } else {
+            const i32 pix = first + k * 256;
+            live = k < p.ppt && pix < p.pixels;
+            // pix / wr from a float reciprocal: (pix + 0.5) / wr is at least 0.5 / wr away from an integer, far
+            // above the rounding error for pix < 2^21.
+            const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * p.inv_wr);
+            y = live ? yq : 0;
+            u = live ? pix - yq * p.slice_wr : 0;
+        }
         ys[k] = live ? y : -1;
         us[k] = u;
         vec2<float> f2;
@@ -753,6 +775,13 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                     brick = nullptr;   // not enough memory for the copy: gather from the row-major volume
                 }
             }
+            // Pixel patches (extract_c32_kernel<PATCH>): rows and the columns but the last in blocks of 8, four
+            // patches per workgroup.
+            const i32 pr = p.slice_h / 8, pc = (p.slice_wr - 1) / 8;
+            bool patch = p.slice_h % 8 == 0 && (p.slice_wr - 1) % 8 == 0 && (pr * pc) % 4 == 0 && pr * pc > 0;
+            if (const char* pe = std::getenv("NOA_HIP_EXTRACT_PATCH")) patch = patch && pe[0] != '0';
+            const i64 patch_tiles = pr * pc / 4 + (p.slice_h + 255) / 256;
+            patch = patch && n_slices * patch_tiles <= 0x7fffffffLL;
             if (brick) {
                 const BrickLayout L{static_cast<i32>(bz), static_cast<i32>(by)};
                 hipLaunchKernelGGL((brick_volume_kernel<S>), dim3(static_cast<u32>((stored + 255) / 256)), dim3(256), 0,
@@ -762,9 +791,20 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err) {
                 q.vol = brick;
                 q.vs0 = L.bz;
                 q.vs1 = L.by;
-                if (q.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, true, S>), grid, block, 0, stream, q);
-                else if (q.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, true, S>), grid, block, 0, stream, q);
-                else hipLaunchKernelGGL((extract_c32_kernel<0, true, S>), grid, block, 0, stream, q);
+                if (patch) {
+                    q.ppt = 1;
+                    q.patch_wgs = pr * pc / 4;
+                    q.patch_pc = pc;
+                    q.tiles = static_cast<i32>(patch_tiles);
+                    const dim3 pgrid(static_cast<u32>(n_slices * patch_tiles));
+                    if (q.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, true, S, true>), pgrid, block, 0, stream, q);
+                    else if (q.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, true, S, true>), pgrid, block, 0, stream, q);
+                    else hipLaunchKernelGGL((extract_c32_kernel<0, true, S, true>), pgrid, block, 0, stream, q);
+                } else {
+                    if (q.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, true, S>), grid, block, 0, stream, q);
+                    else if (q.shift_mode == 2) hipLaunchKernelGGL((extract_c32_kernel<2, true, S>), grid, block, 0, stream, q);
+                    else hipLaunchKernelGGL((extract_c32_kernel<0, true, S>), grid, block, 0, stream, q);
+                }
                 (void)hipFreeAsync(brick, stream);
             } else {
                 if (p.shift_mode == 0) hipLaunchKernelGGL((extract_c32_kernel<1, false, S>), grid, block, 0, stream, p);

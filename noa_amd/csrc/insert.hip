@@ -851,10 +851,12 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
 
     // Column data of the three face orientations (slot 1 of AX 0 and 1: the columns lane + 64).
     SlabColumn c0[2], c1[2], c2[2];
-    c0[0] = slab_column<0>(lane >> 4, lane & 15, f0, nin, p.cut2);
-    c0[1] = slab_column<0>((lane >> 4) + 4, lane & 15, f0, nin, p.cut2);
-    c1[0] = slab_column<1>(lane >> 4, lane & 15, f0, nin, p.cut2);
-    c1[1] = slab_column<1>((lane >> 4) + 4, lane & 15, f0, nin, p.cut2);
+    // 8 x 8 patches of columns per slot: the 64 lanes of a candidate group project onto a compact patch of the
+    // slice, whatever its in-plane rotation (fewer cache lines per tap load than 4 x 16 strips).
+    c0[0] = slab_column<0>(lane >> 3, lane & 7, f0, nin, p.cut2);
+    c0[1] = slab_column<0>(lane >> 3, (lane & 7) + 8, f0, nin, p.cut2);
+    c1[0] = slab_column<1>(lane >> 3, lane & 7, f0, nin, p.cut2);
+    c1[1] = slab_column<1>(lane >> 3, (lane & 7) + 8, f0, nin, p.cut2);
     c2[0] = slab_column<2>(lane >> 3, lane & 7, f0, nin, p.cut2);
     c2[1] = c2[0];
 
@@ -1386,14 +1388,15 @@ KernelChoice kernel_choice(const ProjectDesc& d) {
     i32 v[4];
     const int remap = (d.src_centered ? 4 : 0) + (d.dst_centered ? 8 : 0);
     tuning_lookup(NOA_HIP_OP_INSERT, d.out_shape[1], remap, d.interp, d.output.dtype, v);
-    // The one-wave kernel unless the table asks for the slab kernel (measured slower at 256^3: 9.2 against 6.0 ms).
-    k.force_wave = v[0] != 2;
+    // The slab kernel where it applies (5.5 against 6.0 ms at 256^3 with 10,000 slices), unless the table asks for
+    // the one-wave kernel (1); its two x taps of a row as one load unless the table says 2.
+    k.force_wave = v[0] == 1;
     k.list_target = v[1] > 0 ? std::max(16.0, static_cast<double>(v[1])) : 256.0;
-    k.paired = v[2] == 1;
+    k.paired = v[2] != 2;
     k.order = v[3];
     if (const char* o = std::getenv("NOA_HIP_INSERT_ORDER")) k.order = std::atoi(o);
     if (const char* e = std::getenv("NOA_HIP_INSERT_KERNEL")) k.force_wave = std::string(e) != "slab";
-    if (const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED")) k.paired = std::string(pr) == "1";
+    if (const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED")) k.paired = std::string(pr) != "0";
     if (const char* l = std::getenv("NOA_HIP_INSERT_LIST")) k.list_target = std::max(16.0, std::atof(l));
     return k;
 }

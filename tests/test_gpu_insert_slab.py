@@ -20,8 +20,8 @@ DEV = "cuda"
 
 @pytest.fixture(autouse=True)
 def slab_kernel(monkeypatch):
-    """The slab kernel is chosen by the tuning table or NOA_HIP_INSERT_KERNEL=slab (the one-wave kernel is the
-    default)."""
+    """The slab kernel is the default where it applies; NOA_HIP_INSERT_KERNEL=slab makes these tests independent of
+    the tuning table."""
     monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", "slab")
 
 
