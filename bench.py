@@ -420,6 +420,34 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
     log(f"rasterize {b_r} slices: {t * 1e3:.3f} ms (direct kernel {t_direct * 1e3:.1f} ms)")
     del rvol, rw, fwd
 
+    # Insertion of a +-60 degree single-axis tilt series (10,000 slices, random in-plane angles; the same slices and
+    # weights as the step): with the work plan of the uniform-orientation model, and with the pose-adaptive plan
+    # (item order 3: passing slices counted on the device first).
+    rng = np.random.default_rng(11)
+    ang = np.stack([np.zeros(b), np.deg2rad(rng.uniform(-60, 60, b)), rng.uniform(0, 2 * np.pi, b)], -1)
+    tilt = nx.euler2matrix(ang, axes="zyz", dtype=np.float32)
+    tinv = torch.from_numpy(np.ascontiguousarray(np.transpose(tilt, (0, 2, 1)))).to(dev)
+    tv = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    tw = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=dev)
+    tilt_ms = {}
+    prev = os.environ.get("NOA_HIP_INSERT_ORDER")
+    try:
+        for name, order in (("model_plan", "0"), ("pose_adaptive_plan", "3")):
+            os.environ["NOA_HIP_INSERT_ORDER"] = order
+            tilt_ms[name] = timed(lambda: nx.insert_central_slices_3d("h2h", ins.slices, ins.weights, ins.slice_shape,
+                                                                      tv, tw, ins.volume_shape, None, tinv)) * 1e3
+    finally:
+        if prev is None:
+            os.environ.pop("NOA_HIP_INSERT_ORDER")
+        else:
+            os.environ["NOA_HIP_INSERT_ORDER"] = prev
+    rep["insert_tilt_series"] = {
+        "workload": f"insert_central_slices_3d of {b} CTF-weighted 256^2 slices of a +-60 deg single-axis tilt series "
+                    "(random in-plane angles) into 256^3 + weights (h2h)", **{k + "_ms": v for k, v in tilt_ms.items()}}
+    log(f"tilt-series insertion: {tilt_ms['model_plan']:.3f} ms (model plan), "
+        f"{tilt_ms['pose_adaptive_plan']:.3f} ms (pose-adaptive plan)")
+    del tv, tw, tinv
+
     # Isotropic CTF of a reconstructed 256^3 rfft volume (h2h): the volume multiplied by CTF(|f|) in place.
     spec = torch.randn((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
     ciso = ns.CTFIsotropic(1.0, 1.5, 300, 0.07, 2.7)

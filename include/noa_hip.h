@@ -303,7 +303,9 @@ int noa_hip_insert_batch_launch(const void* device_image, void* stream);
  *                                       expected crossing slices per tile work item (default 128); -; -}
  *   NOA_HIP_OP_INSERT:        values = {kernel: 0 auto (slab where it applies), 1 one-wave, 2 slab; expected
  *                                       slices per work item (default 256); x taps of a slab row: 0 auto (one load),
- *                                       1 one load, 2 two loads; item order: 0 by work, 1 by first slice, 2 XCD-aware}
+ *                                       1 one load, 2 two loads; item order: 0 by work, 1 by first slice, 2 XCD-aware,
+ *                                       3 pose-adaptive (passing slices counted on the device first; the call then
+ *                                       waits for the count, so it is not for stream capture)}
  *   NOA_HIP_OP_EXTRACT_BATCH: values = {pixels per lane, 1-4 (default 3); workgroups (default 2048); -; -}
  *   NOA_HIP_OP_INSERT_BATCH:  values = {workgroups (default 3072); -; -; -}
  * Environment variables (NOA_HIP_EXTRACT_LAYOUT, NOA_HIP_EXTRACT_PAIRS, NOA_HIP_INSERT_KERNEL, NOA_HIP_INSERT_LIST,

@@ -10,7 +10,8 @@ tuning never changes an output bit.
 Operators and their parameters (``values``):
   OP_EXTRACT        volume layout (1 row-major gather, 2 block-copy gather, 3 LDS tiles), crossing slices per tile item
   OP_INSERT         kernel (1 one-wave, 2 slab where allowed), expected slices per work item, x taps of a slab row
-                    (0 auto: one load, 2: two loads), item order (0 by work, 1 by first slice, 2 XCD-aware)
+                    (0 auto: one load, 2: two loads), item order (0 by work, 1 by first slice, 2 XCD-aware, 3
+                    pose-adaptive: segments cut from the passing slices counted on the device)
   OP_EXTRACT_BATCH  pixels per lane (1-4), workgroups of the persistent grid
   OP_INSERT_BATCH   workgroups of the persistent grid
 """
@@ -34,7 +35,7 @@ OPS = {"extract": abi.OP_EXTRACT, "insert": abi.OP_INSERT, "extract_batch": abi.
 # Candidate values per operator (cartesian products).
 SPACE = {
     "extract": {"layout": [1, 2, 3], "pairs": [64, 128, 256]},
-    "insert": {"kernel": [1, 2], "list": [128, 256, 512], "paired": [0], "order": [0, 2]},
+    "insert": {"kernel": [1, 2], "list": [128, 256, 512], "paired": [0], "order": [0, 2, 3]},
     "extract_batch": {"ppt": [1, 2, 3, 4], "grid": [1024, 2048, 4096]},
     "insert_batch": {"grid": [3072, 6144, 12288]},
 }

@@ -1075,6 +1075,90 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Pose-adaptive work plan (item order 3): how many slices pass each active sub-brick, in bins of the slice range
+// (count_bin_width slices each, at most CNT_BINS), counted with the one-wave kernel's culling test
+// (|n . centre| <= f_b + |n| . half-extent, normalised frequencies: a superset of the slices with hits). The host
+// cuts each sub-brick's range at bin boundaries into segments of about the target number of passing slices, so that
+// a sub-brick crossed by far more slices than the uniform-orientation model expects (tilt series, preferred
+// orientations) is split instead of becoming one long item.
+// ---------------------------------------------------------------------------------------------------------
+constexpr i32 CNT_BINS = 64;
+
+template<class C>
+struct CountParams {
+    XformArg<C> rot;
+    const i32* bricks;                  // active sub-bricks
+    i32* counts;                        // CNT_BINS per active sub-brick
+    i32 n_slices, n_bricks;
+    i32 bin_w;                          // slices per bin: a multiple of 64, so that 64 consecutive slices share a bin
+    i32 range_w;                        // slices per workgroup range (blockIdx.y), a multiple of 64
+    i32 grid[3], nb[3], bd[3];
+    C target[3], fb;
+    bool vol_centered;
+};
+
+// Slices per bin of the pass counts: 64 * ceil(n / (64 * CNT_BINS)).
+inline i32 count_bin_width(i32 n_slices) {
+    return 64 * std::max<i32>(1, (n_slices + 64 * CNT_BINS - 1) / (64 * CNT_BINS));
+}
+
+constexpr i32 CNT_BRICKS = 16;      // sub-bricks per wave of the count kernel (each rotation read once for all)
+
+template<class C>
+__global__ __launch_bounds__(64) void insert_count_kernel(const CountParams<C> p) {
+    __shared__ i32 bins[CNT_BRICKS][CNT_BINS];
+    __shared__ C box[CNT_BRICKS][6];                   // centre and half-extent of each sub-brick
+    const i32 lane = static_cast<i32>(threadIdx.x);
+    const i32 j0 = static_cast<i32>(blockIdx.x) * CNT_BRICKS;
+    const i32 nj = min(CNT_BRICKS, p.n_bricks - j0);
+    for (i32 i = lane; i < CNT_BRICKS * CNT_BINS; i += 64) bins[i / CNT_BINS][i % CNT_BINS] = 0;
+    if (lane < nj) {
+        const i32 brick = p.bricks[j0 + lane];
+        const i32 bx = brick % p.nb[2], r = brick / p.nb[2], by = r % p.nb[1], bz = r / p.nb[1];
+        const i32 org[3] = {bz * p.bd[0], by * p.bd[1], bx * p.bd[2]};
+        const i32 ext[3] = {p.grid[0], p.grid[1], p.grid[2] / 2 + 1};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            i32 lo = 0x7fffffff, hi = -0x7fffffff;
+            for (i32 k = 0; k < p.bd[a] && org[a] + k < ext[a]; ++k) {
+                const i32 f = axis_freq(a == 2, org[a] + k, p.vol_centered, p.grid[a]);
+                lo = min(lo, f);
+                hi = max(hi, f);
+            }
+            box[lane][a] = (static_cast<C>(lo) + static_cast<C>(hi)) * C(0.5) / p.target[a];
+            box[lane][3 + a] = (static_cast<C>(hi) - static_cast<C>(lo)) * C(0.5) / p.target[a];
+        }
+    }
+    __syncthreads();
+    // This workgroup's range of slices, 64 consecutive slices per step, all in bin s0 / bin_w (bin_w and range_w are
+    // multiples of 64).
+    const i32 r0 = static_cast<i32>(blockIdx.y) * p.range_w, r1 = min(p.n_slices, r0 + p.range_w);
+    for (i32 s0 = r0; s0 < r1; s0 += 64) {
+        const i32 s = s0 + lane;
+        const bool live = s < p.n_slices;
+        C m[9] = {};
+        if (live) rot_matrix(load_rotation(p.rot, s), m);
+        const i32 bin = s0 / p.bin_w;
+        for (i32 j = 0; j < nj; ++j) {
+            const C d = fabs(m[0] * box[j][0] + m[1] * box[j][1] + m[2] * box[j][2]);
+            const C bound = p.fb + fabs(m[0]) * box[j][3] + fabs(m[1]) * box[j][4] + fabs(m[2]) * box[j][5];
+            const unsigned long long pass = __ballot(live && d <= bound + C(1e-5));
+            if (lane == 0 && pass != 0) bins[j][bin] += __popcll(pass);
+        }
+    }
+    __syncthreads();
+    for (i32 i = lane; i < nj * CNT_BINS; i += 64) {
+        const i32 c = bins[i / CNT_BINS][i % CNT_BINS];
+        if (c != 0) atomicAdd(p.counts + (static_cast<i64>(j0 + i / CNT_BINS) * CNT_BINS + i % CNT_BINS), c);
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_i32_kernel(i32* __restrict__ a, i64 n) {
+    const i64 i = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
 // General path: one lane per voxel, every slice in order (noa's operator as is).
 template<class C, class In, class Out, class W, int INTERP, bool GENERAL>
 __global__ __launch_bounds__(256) void insert_general_kernel(const InsertParams<C, In, W> p) {
@@ -1226,6 +1310,10 @@ struct Plan {
     i32 n_items{}, n_reduce{}, n_partials{};
     std::vector<int4> h_items, h_reduce;   // host copies (the allocation-free entry point copies them to its scratch)
     std::vector<float> h_work;             // expected passing slices of each item
+    std::vector<i32> h_bricks;             // active sub-bricks (pose-adaptive plans count their passing slices)
+    std::vector<i32> h_nseg;               // segments of each active sub-brick in this plan
+    i32* bricks{};                         // device copy of h_bricks, uploaded on first use
+    bool owned{};                          // device arrays freed with the plan (pose-adaptive plans, one call)
 };
 using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double, i32, i32, i32, double, int>;
 std::mutex g_plan_mutex;
@@ -1364,7 +1452,13 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
         items[k] = entries[k].item;
         work[k] = static_cast<float>(entries[k].work);
     }
+    std::map<i32, i32> segments_of;
+    for (const int4& it : items) ++segments_of[it.x];
     Plan& pl = g_plans[key];
+    for (const auto& kv : segments_of) {
+        pl.h_bricks.push_back(kv.first);
+        pl.h_nseg.push_back(kv.second);
+    }
     pl.n_items = static_cast<i32>(items.size());
     pl.n_reduce = static_cast<i32>(reduce.size());
     pl.n_partials = n_partials;
@@ -1373,6 +1467,70 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
     pl.h_reduce = std::move(reduce);
     plan = &pl;
     return upload ? upload_plan(pl, err) : NOA_HIP_OK;
+}
+
+// The pose-adaptive plan of one call: for each active sub-brick (counts: CNT_BINS passing-slice counts over the slice
+// range), segments cut at bin boundaries with about `list_target` passing slices each; sub-bricks crossed by no slice
+// get no item. Items sorted by passing slices, largest first; partial slots and reductions as make_plan's.
+void make_pose_plan(const std::vector<i32>& bricks, const std::vector<i32>& counts, i32 n_slices, double list_target,
+                    Plan& pl) {
+    struct Entry { double work; int4 item; };
+    std::vector<Entry> entries;
+    entries.reserve(bricks.size() * 2);
+    std::vector<int4> reduce;
+    i32 n_partials = 0;
+    const i32 bin_w = count_bin_width(n_slices);
+    auto bin_start = [&](i32 b) { return std::min<i32>(n_slices, b * bin_w); };
+    i32 seg_lo[CNT_BINS], seg_hi[CNT_BINS];
+    i64 seg_work[CNT_BINS];
+    for (size_t j = 0; j < bricks.size(); ++j) {
+        const i32* c = counts.data() + j * CNT_BINS;
+        i64 total = 0;
+        for (i32 b = 0; b < CNT_BINS; ++b) total += c[b];
+        if (total == 0) continue;
+        const i64 n_seg = std::max<i64>(1, static_cast<i64>(std::ceil(static_cast<double>(total) / list_target)));
+        // Segments cut where the running count passes k / n_seg of the total, each trimmed to its first and last
+        // non-empty bins.
+        i32 nk = 0;
+        i64 acc = 0, prev = 0;
+        i32 lo = -1, hi = -1;
+        for (i32 b = 0; b < CNT_BINS; ++b) {
+            if (c[b] > 0) {
+                if (lo < 0) lo = b;
+                hi = b;
+            }
+            acc += c[b];
+            const bool cut = b == CNT_BINS - 1 || (nk < n_seg - 1 && acc * n_seg >= total * (nk + 1));
+            if (cut && lo >= 0) {
+                seg_lo[nk] = lo;
+                seg_hi[nk] = hi + 1;
+                seg_work[nk] = acc - prev;
+                ++nk;
+                prev = acc;
+                lo = hi = -1;
+            }
+        }
+        const bool split = nk > 1;
+        for (i32 k = 0; k < nk; ++k)
+            entries.push_back({static_cast<double>(seg_work[k]),
+                               make_int4(bricks[j], bin_start(seg_lo[k]), bin_start(seg_hi[k]),
+                                         split ? n_partials + k : -1)});
+        if (split) {
+            reduce.push_back(make_int4(bricks[j], n_partials, nk, 0));
+            n_partials += nk;
+        }
+    }
+    std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
+    pl.h_items.resize(entries.size());
+    pl.h_work.resize(entries.size());
+    for (size_t k = 0; k < entries.size(); ++k) {
+        pl.h_items[k] = entries[k].item;
+        pl.h_work[k] = static_cast<float>(entries[k].work);
+    }
+    pl.h_reduce = std::move(reduce);
+    pl.n_items = static_cast<i32>(pl.h_items.size());
+    pl.n_reduce = static_cast<i32>(pl.h_reduce.size());
+    pl.n_partials = n_partials;
 }
 
 // Culled kernel of the insertion: the slab kernel where the configuration allows it, else the one-wave kernel
@@ -1570,7 +1728,91 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
                                      plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target,
                                      scratch == nullptr, planp, err, kc.order);
             if (st != NOA_HIP_OK) return st;
+            // Pose-adaptive plan (item order 3; not for the allocation-free entry point): the passing slices of each
+            // active sub-brick are counted on the device, and the segments are cut from the counts.
+            Plan pose;
+            if (kc.order == 3 && scratch == nullptr && !planp->h_bricks.empty()) {
+                Plan& shared = const_cast<Plan&>(*planp);
+                {
+                    std::lock_guard<std::mutex> lock(g_plan_mutex);
+                    if (!shared.bricks &&
+                        (hipMalloc(&shared.bricks, shared.h_bricks.size() * sizeof(i32)) != hipSuccess ||
+                         hipMemcpy(shared.bricks, shared.h_bricks.data(), shared.h_bricks.size() * sizeof(i32),
+                                   hipMemcpyHostToDevice) != hipSuccess)) {
+                        err = "insert: could not upload the active sub-bricks";
+                        return NOA_HIP_ERROR_RUNTIME;
+                    }
+                }
+                const i32 nbk = static_cast<i32>(shared.h_bricks.size());
+                i32* dcounts = nullptr;
+                if (hipMallocAsync(&dcounts, static_cast<size_t>(nbk) * CNT_BINS * sizeof(i32), stream) != hipSuccess) {
+                    err = "insert: could not allocate the pass counts";
+                    return NOA_HIP_ERROR_RUNTIME;
+                }
+                CountParams<C> cp{};
+                cp.rot = p.rot;
+                cp.bricks = shared.bricks;
+                cp.counts = dcounts;
+                cp.n_slices = p.n_slices;
+                cp.n_bricks = nbk;
+                cp.bin_w = count_bin_width(p.n_slices);
+                constexpr i32 RANGES = 8;
+                cp.range_w = 64 * std::max<i32>(1, ((p.n_slices + 63) / 64 + RANGES - 1) / RANGES);
+                const i64 n_counts = static_cast<i64>(nbk) * CNT_BINS;
+                hipLaunchKernelGGL(zero_i32_kernel, dim3(static_cast<u32>((n_counts + 255) / 256)), dim3(256), 0, stream,
+                                   dcounts, n_counts);
+                for (int a = 0; a < 3; ++a) {
+                    cp.grid[a] = p.grid[a];
+                    cp.nb[a] = p.nb[a];
+                    cp.bd[a] = p.bd[a];
+                    cp.target[a] = p.target[a];
+                }
+                cp.fb = p.fftfreq_blackman;
+                cp.vol_centered = p.vol_centered;
+                hipLaunchKernelGGL((insert_count_kernel<C>),
+                                   dim3(static_cast<u32>((nbk + CNT_BRICKS - 1) / CNT_BRICKS),
+                                        static_cast<u32>((p.n_slices + cp.range_w - 1) / cp.range_w)),
+                                   dim3(64), 0, stream, cp);
+                std::vector<i32> counts(static_cast<size_t>(nbk) * CNT_BINS);
+                const bool ok = hipMemcpyAsync(counts.data(), dcounts, counts.size() * sizeof(i32), hipMemcpyDeviceToHost,
+                                               stream) == hipSuccess &&
+                                hipStreamSynchronize(stream) == hipSuccess;
+                (void)hipFreeAsync(dcounts, stream);
+                if (!ok) { err = "insert: could not read the pass counts"; return NOA_HIP_ERROR_RUNTIME; }
+                // The model plan stays when no sub-brick needs more than twice its segments.
+                bool skewed = false;
+                for (i32 j = 0; j < nbk && !skewed; ++j) {
+                    i64 total = 0;
+                    for (i32 b = 0; b < CNT_BINS; ++b) total += counts[static_cast<size_t>(j) * CNT_BINS + b];
+                    skewed = static_cast<double>(total) > 2.0 * kc.list_target * shared.h_nseg[static_cast<size_t>(j)];
+                }
+                if (skewed) make_pose_plan(shared.h_bricks, counts, p.n_slices, kc.list_target, pose);
+            }
+            if (pose.n_items > 0) {
+                if (hipMallocAsync(&pose.items, pose.h_items.size() * sizeof(int4), stream) != hipSuccess ||
+                    hipMemcpyAsync(pose.items, pose.h_items.data(), pose.h_items.size() * sizeof(int4),
+                                   hipMemcpyHostToDevice, stream) != hipSuccess ||
+                    (!pose.h_reduce.empty() &&
+                     (hipMallocAsync(&pose.reduce, pose.h_reduce.size() * sizeof(int4), stream) != hipSuccess ||
+                      hipMemcpyAsync(pose.reduce, pose.h_reduce.data(), pose.h_reduce.size() * sizeof(int4),
+                                     hipMemcpyHostToDevice, stream) != hipSuccess))) {
+                    err = "insert: could not upload the pose-adaptive plan";
+                    return NOA_HIP_ERROR_RUNTIME;
+                }
+                pose.owned = true;
+                planp = &pose;
+            }
             const Plan& plan = *planp;
+            // The pose-adaptive plan's device arrays are released after this call's kernels (stream-ordered).
+            struct PoseRelease {
+                Plan& pl;
+                hipStream_t st;
+                ~PoseRelease() {
+                    if (!pl.owned) return;
+                    if (pl.items) (void)hipFreeAsync(pl.items, st);
+                    if (pl.reduce) (void)hipFreeAsync(pl.reduce, st);
+                }
+            } pose_release{pose, stream};
             if (plan.n_items == 0) return NOA_HIP_OK;
             const size_t partial_bytes = static_cast<size_t>(plan.n_partials) * p.bd[0] * p.bd[1] * p.apitch * 3 *
                                          sizeof(float);
