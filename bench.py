@@ -189,7 +189,7 @@ def _serving_calls(server, reqs):
         if r.kind == "extract":
             out = server.pool.view(off, (r.count, 1, box, wr), torch_mod().complex64)
             off += r.output_bytes()
-            ext.append(nb.extract_call("hc2hc", server.volumes[box][r.volume], (1, box, box, box), out,
+            ext.append(nb.extract_call("hc2hc", server.sources[box][r.volume], (1, box, box, box), out,
                                        (r.count, 1, box, box), r.rotations))
         else:
             v = server.pool.view(off, (1, box, box, wr), torch_mod().complex64)

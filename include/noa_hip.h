@@ -60,7 +60,10 @@ enum {
     NOA_HIP_C32 = 3,      /* Complex<f32> */
     NOA_HIP_C64 = 4,      /* Complex<f64> */
     NOA_HIP_C16 = 5,      /* {f16 real, f16 imag}: storage-only volume format for extraction (FP32 math) */
-    NOA_HIP_CBF16 = 6     /* {bf16 real, bf16 imag}: storage-only volume format for extraction (FP32 math) */
+    NOA_HIP_CBF16 = 6,    /* {bf16 real, bf16 imag}: storage-only volume format for extraction (FP32 math) */
+    NOA_HIP_C32_BLOCKS = 7 /* Complex<f32> volume in the 4x2x4-block layout of noa_hip_volume_blocks: accepted as the
+                            * volume of batched extractions only; the view's shape is the volume's rfft shape and its
+                            * strides those of a contiguous volume (the layout fixes the element offsets) */
 };
 
 /* noa::fft::Layout values (src/noa/fft/core/Layout.hpp:39-68): the half-to-half layouts, and the full-to-full ones
@@ -293,6 +296,16 @@ int noa_hip_extract_batch_launch(const void* device_image, void* stream);
 int noa_hip_insert_batch_prepare(const noa_hip_project_args* calls, int32_t n_calls, void* image, int64_t capacity,
                                  int64_t* bytes);
 int noa_hip_insert_batch_launch(const void* device_image, void* stream);
+
+/* ---- Prepared volumes (no counterpart in noa's projection API; like noa's Texture, xform/Texture.hpp:62-300, a
+ * copy made once and read by many calls). For callers that extract from the same resident volume many times, such
+ * as a serving loop: the 4x2x4-block copy that the single-call extraction makes on every call, made once, then
+ * passed to noa_hip_extract_batch_prepare as a NOA_HIP_C32_BLOCKS view. ---------------------------------------- */
+/* Bytes of the block copy of a c32 rfft volume of rfft shape (1, D, H, W/2+1). */
+int64_t noa_hip_volume_blocks_bytes(const int64_t rfft_shape[4]);
+/* Writes the block copy of `volume` (c32, rfft shape (1, D, H, W/2+1), unit x stride) into `blocks` (device memory
+ * of at least noa_hip_volume_blocks_bytes bytes), asynchronously on `stream`. */
+int noa_hip_volume_blocks(const noa_hip_view* volume, void* blocks, int64_t capacity, void* stream);
 
 /* ---- Launch tuning (replaces noa's compile-time launch options: IwiseOptions, runtime/Iwise.hpp:14-60, and the
  * fixed 32x16 block reshaping of the CUDA iwise kernels, runtime/cuda/Iwise.cuh:94-175) ---------------------------

@@ -23,6 +23,7 @@ C32 = 3
 C64 = 4
 C16 = 5
 CBF16 = 6
+C32_BLOCKS = 7      # c32 volume in the 4x2x4-block layout of noa_hip_volume_blocks (batched extraction only)
 
 # Remaps (noa::fft::Layout bits, src/noa/fft/core/Layout.hpp:39-68).
 H2H = 0
@@ -217,6 +218,10 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
         f = getattr(lib, name)
         f.argtypes = [P, c_void_p]
         f.restype = ctypes.c_int
+    lib.noa_hip_volume_blocks_bytes.argtypes = [ctypes.POINTER(c_int64)]
+    lib.noa_hip_volume_blocks_bytes.restype = c_int64
+    lib.noa_hip_volume_blocks.argtypes = [ctypes.POINTER(View), c_void_p, c_int64, c_void_p]
+    lib.noa_hip_volume_blocks.restype = ctypes.c_int
     for name in ("noa_hip_extract_batch_prepare", "noa_hip_insert_batch_prepare"):
         f = getattr(lib, name)
         f.argtypes = [P, c_int32, c_void_p, c_int64, ctypes.POINTER(c_int64)]

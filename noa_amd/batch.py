@@ -19,6 +19,7 @@ import torch
 from . import abi
 from . import xform as nx
 from ._lib import last_error, lib
+from .xform import VolumeBlocks  # noqa: F401  (a prepared volume accepted by extract_call)
 
 
 @dataclass
@@ -31,7 +32,7 @@ class Call:
 def extract_call(remap, volume, volume_shape, slices, slice_shape, fwd_rotation,
                  options: nx.ExtractCentralSlicesOptions | None = None) -> Call:
     """The descriptor of extract_central_slices_3d(remap, volume, None, volume_shape, slices, None, slice_shape,
-    None, fwd_rotation, options)."""
+    None, fwd_rotation, options). `volume` may be a VolumeBlocks (the volume's block copy, made once)."""
     opts = options or nx.ExtractCentralSlicesOptions()
     keep: list = []
     args = abi.default_args()

@@ -36,13 +36,15 @@ std::string shape_str(const int64_t* s) {
 
 bool is_real(int dtype) { return dtype == NOA_HIP_F32 || dtype == NOA_HIP_F64; }
 bool is_complex(int dtype) {
-    return dtype == NOA_HIP_C32 || dtype == NOA_HIP_C64 || dtype == NOA_HIP_C16 || dtype == NOA_HIP_CBF16;
+    return dtype == NOA_HIP_C32 || dtype == NOA_HIP_C64 || dtype == NOA_HIP_C16 || dtype == NOA_HIP_CBF16 ||
+           dtype == NOA_HIP_C32_BLOCKS;
 }
 size_t dtype_size(int dtype) {
     switch (dtype) {
         case NOA_HIP_F32: return 4;
         case NOA_HIP_F64: return 8;
         case NOA_HIP_C32: return 8;
+        case NOA_HIP_C32_BLOCKS: return 8;
         case NOA_HIP_C64: return 16;
         case NOA_HIP_C16: return 4;
         case NOA_HIP_CBF16: return 4;
@@ -232,6 +234,11 @@ ProjectDesc validate(Kind kind, const noa_hip_project_args& a, int& device) {
               INVALID, "Invalid input/output types: %d -> %d", in, out);
     if (in == NOA_HIP_C16 || in == NOA_HIP_CBF16)
         NOA_CHECK(extract && !a.input.is_value, UNSUPPORTED, "FP16/BF16 storage is only supported for the extracted volume");
+    if (in == NOA_HIP_C32_BLOCKS)
+        NOA_CHECK(extract && !a.input.is_value && out == NOA_HIP_C32, UNSUPPORTED,
+                  "A block-copy volume is only accepted as the c32 volume of an extraction");
+    NOA_CHECK(out != NOA_HIP_C32_BLOCKS && a.input_weight.dtype != NOA_HIP_C32_BLOCKS &&
+              a.output_weight.dtype != NOA_HIP_C32_BLOCKS, UNSUPPORTED, "A block-copy volume is only accepted as the input of an extraction");
 
     // Transforms (:1166-1178).
     const int64_t required = extract ? a.output_shape[0] : a.input_shape[0];
@@ -1096,6 +1103,10 @@ int call(Kind kind, const noa_hip_project_args* args, void* stream, const noa_hi
     }
     int device = -1;
     ProjectDesc d;
+    if (args->input.dtype == NOA_HIP_C32_BLOCKS) {
+        g_last_error = "A block-copy volume (NOA_HIP_C32_BLOCKS) is accepted by the batched extraction only";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    }
     try {
         if (shifts) {
             // Host-only checks of the shifts first, then the extraction's own checks (device checks last).
@@ -1235,6 +1246,29 @@ int noa_hip_extract_batch_prepare(const noa_hip_project_args* calls, int32_t n_c
     *bytes = b;
     if (status != NOA_HIP_OK) g_last_error = err;
     return status;
+}
+int64_t noa_hip_volume_blocks_bytes(const int64_t rfft_shape[4]) {
+    if (!rfft_shape) return -1;
+    return noa_hip::volume_blocks_bytes(rfft_shape);
+}
+int noa_hip_volume_blocks(const noa_hip_view* volume, void* blocks, int64_t capacity, void* stream) {
+    using namespace noa_hip;
+    g_last_error.clear();
+    if (!volume || !blocks) { g_last_error = "volume blocks: null argument"; return NOA_HIP_ERROR_INVALID_ARGUMENT; }
+    const noa_hip_view& v = *volume;
+    if (v.dtype != NOA_HIP_C32 || v.is_value || !v.data || v.shape[0] != 1 || v.strides[3] != 1) {
+        g_last_error = "volume blocks: expected a c32 rfft volume (1, D, H, W/2+1) with a unit x stride";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    const int64_t need = volume_blocks_bytes(v.shape);
+    if (capacity < need) {
+        g_last_error = "volume blocks: the copy needs " + std::to_string(need) + " bytes";
+        return NOA_HIP_ERROR_INVALID_ARGUMENT;
+    }
+    std::string err;
+    const int st = volume_blocks(v.data, v.strides[1], v.strides[2], v.shape, blocks, static_cast<hipStream_t>(stream), err);
+    if (st != NOA_HIP_OK) g_last_error = err;
+    return st;
 }
 int noa_hip_extract_batch_launch(const void* device_image, void* stream) {
     using namespace noa_hip;

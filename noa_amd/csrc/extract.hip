@@ -539,6 +539,7 @@ struct ExtractBatchJob {
     float target[3], slice_f[2];
     float cutoff_sqd, inv_wr;
     i32 vol_centered, slice_centered;
+    i32 blocks;                         // the volume is a block copy (noa_hip_volume_blocks): vs0, vs1 are its strides
 };
 struct ExtractBatchHeader {
     i64 n_units;
@@ -594,8 +595,12 @@ __global__ __launch_bounds__(256) void extract_batch_kernel(const unsigned char*
             const vec3<float> f = slice2grid<false>(f2, sc, rot, 0.f, 0.f);
             inside[k] = live && dot3(f) <= q.cutoff_sqd;
             const float fz = f.z * q.target[0], fyv = f.y * q.target[1], fx = f.x * q.target[2];
-            tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{q.vol}, q.vs0, q.vs1, 1, bd, vc, fz, fyv, fx,
-                                                          inside[k]);
+            if (q.blocks)
+                tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{q.vol}, BrickLayout{q.vs0, q.vs1}, bd, vc,
+                                                              fz, fyv, fx, inside[k]);
+            else
+                tv[k] = fetch3<NOA_HIP_INTERP_LINEAR, float2>(GlobalLoad<float2>{q.vol}, q.vs0, q.vs1, 1, bd, vc, fz, fyv,
+                                                              fx, inside[k]);
         }
 #pragma unroll
         for (i32 k = 0; k < MAXP; ++k) {
@@ -868,7 +873,8 @@ int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64
     for (i32 k = 0; k < n_jobs; ++k) {
         const ProjectDesc& d = descs[k];
         const std::string at = "extract batch, call " + std::to_string(k) + ": ";
-        if (d.input.dtype != NOA_HIP_C32 || d.output.dtype != NOA_HIP_C32 || d.coord_f64 || d.general ||
+        if ((d.input.dtype != NOA_HIP_C32 && d.input.dtype != NOA_HIP_C32_BLOCKS) || d.output.dtype != NOA_HIP_C32 ||
+            d.coord_f64 || d.general ||
             d.interp != NOA_HIP_INTERP_LINEAR || d.input_weight.data || d.output_weight.data || d.input.s2 != 1 ||
             d.output.s2 != 1 || d.shift.present) {
             err = at + "only complex FP32 volumes and slices with unit x strides, LINEAR interpolation, single-precision "
@@ -890,6 +896,14 @@ int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64
         q.os_y = static_cast<i32>(d.output.s1);
         q.vs0 = static_cast<i32>(d.input.s0);
         q.vs1 = static_cast<i32>(d.input.s1);
+        q.blocks = d.input.dtype == NOA_HIP_C32_BLOCKS ? 1 : 0;
+        if (q.blocks) {
+            i64 bz, by, count;
+            BrickLayout::strides(d.in_shape[1], d.in_shape[2], d.in_shape[3] / 2 + 1, bz, by, count);
+            if (count >= 0x7fffffffLL) { err = at + "the block copy exceeds 32-bit indexing"; return NOA_HIP_ERROR_UNSUPPORTED; }
+            q.vs0 = static_cast<i32>(bz);
+            q.vs1 = static_cast<i32>(by);
+        }
         for (int i = 0; i < 3; ++i) {
             q.grid[i] = static_cast<i32>(d.in_shape[i + 1]);
             q.target[i] = static_cast<float>(d.target[i]);
@@ -912,6 +926,32 @@ int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64
     h->n_jobs = n_jobs;
     h->ppt = ppt;
     std::memcpy(image, buf.data(), buf.size());
+    return NOA_HIP_OK;
+}
+
+i64 volume_blocks_bytes(const int64_t rfft_shape[4]) {
+    i64 bz, by, count;
+    BrickLayout::strides(rfft_shape[1], rfft_shape[2], rfft_shape[3], bz, by, count);
+    return count * static_cast<i64>(sizeof(float2));
+}
+
+int volume_blocks(const void* volume, i64 s_z, i64 s_y, const int64_t rfft_shape[4], void* blocks, hipStream_t stream,
+                  std::string& err) {
+    const i64 nz = rfft_shape[1], ny = rfft_shape[2], wr = rfft_shape[3];
+    i64 bz, by, count;
+    BrickLayout::strides(nz, ny, wr, bz, by, count);
+    const i64 stored = nz * ny * wr;
+    if (count >= 0x7fffffffLL || s_z * (nz - 1) + s_y * (ny - 1) + wr >= 0x7fffffffLL) {
+        err = "volume blocks: the volume exceeds 32-bit indexing";
+        return NOA_HIP_ERROR_UNSUPPORTED;
+    }
+    if (stored == 0) return NOA_HIP_OK;
+    const BrickLayout L{static_cast<i32>(bz), static_cast<i32>(by)};
+    hipLaunchKernelGGL((brick_volume_kernel<float2>), dim3(static_cast<u32>((stored + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const float2*>(volume), static_cast<i32>(s_z), static_cast<i32>(s_y),
+                       static_cast<float2*>(blocks), static_cast<i32>(ny), static_cast<i32>(wr), stored, L);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
     return NOA_HIP_OK;
 }
 

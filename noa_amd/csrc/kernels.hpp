@@ -187,6 +187,9 @@ int launch_insert(const ProjectDesc& d, hipStream_t stream, std::string& err, bo
 int launch_insert_extract(const InsertExtractDesc& d, hipStream_t stream, std::string& err);
 // Batched extraction / insertion (one persistent launch for several calls); prepare writes the device image on the
 // host, launch reads it from device memory.
+i64 volume_blocks_bytes(const int64_t rfft_shape[4]);
+int volume_blocks(const void* volume, i64 s_z, i64 s_y, const int64_t rfft_shape[4], void* blocks, hipStream_t stream,
+                  std::string& err);
 i64 extract_batch_bytes(i32 n_jobs);
 int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64 capacity, i32 ppt, i64& bytes,
                           std::string& err);

@@ -130,14 +130,20 @@ class Server:
 
     budget_bytes: size of the output pool. n_streams: batch slots (each with its own stream and graph).
     max_batch_slices: slices above which a batch takes no further request.
+    volume_blocks: extract from block copies of the resident volumes (noa_amd.xform.VolumeBlocks, made once here:
+    the gather reads fewer cache lines than from the row-major volumes, with the same results).
     keep_results: copy each finished request's outputs out of the pool (into req.result / req.result_weight) before
     the pool space is reused; otherwise the outputs are only valid inside `on_done`.
     """
 
     def __init__(self, volumes: dict, budget_bytes: int, n_streams: int = 3, device: torch.device | str = "cuda",
-                 keep_results: bool = True, on_done=None, graphs: bool = True, max_batch_slices: int = 8192):
+                 keep_results: bool = True, on_done=None, graphs: bool = True, max_batch_slices: int = 8192,
+                 volume_blocks: bool = True):
         self.device = torch.device(device)
         self.volumes = volumes
+        # What the extraction calls read: the volumes' block copies, or the volumes themselves.
+        self.sources = ({box: [nx.VolumeBlocks(v) for v in vols] for box, vols in volumes.items()}
+                        if volume_blocks else volumes)
         self.budget = int(budget_bytes)
         self.pool = DevicePool(self.budget, self.device)
         self.in_use = 0
@@ -228,7 +234,7 @@ class Server:
                 r.result, r.result_weight = out, wout
                 box = r.box
                 if r.kind == "extract":
-                    ext.append(nb.extract_call("hc2hc", self.volumes[box][r.volume], (1, box, box, box), out,
+                    ext.append(nb.extract_call("hc2hc", self.sources[box][r.volume], (1, box, box, box), out,
                                                (r.count, 1, box, box), r.rotations))
                 else:
                     out.zero_()

@@ -136,6 +136,38 @@ class Texture:
         return self.array.shape
 
 
+class VolumeBlocks:
+    """A c32 rfft volume (1, D, H, W/2+1) copied once into the 4x2x4-block layout that the extraction gathers from
+    (noa_hip_volume_blocks). The single-call extraction makes this copy on every call; a caller that extracts many
+    times from the same resident volume (a serving loop) makes it once and passes the copy to the batched extraction
+    (noa_amd.batch.extract_call). Like noa's Texture it is a copy: `update` copies a new volume in."""
+
+    def __init__(self, volume: torch.Tensor):
+        self._check(volume)
+        self.shape = tuple(volume.shape)
+        self.device = volume.device
+        shape = (ctypes.c_int64 * 4)(*self.shape)
+        nbytes = int(lib().noa_hip_volume_blocks_bytes(shape))
+        self.blocks = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=volume.device)
+        self.update(volume)
+
+    @staticmethod
+    def _check(volume):
+        if (not isinstance(volume, torch.Tensor) or volume.device.type != "cuda" or volume.dtype != torch.complex64
+                or volume.dim() != 4 or volume.shape[0] != 1 or volume.stride(-1) != 1):
+            raise NoaError("VolumeBlocks: expected a complex64 device volume (1, D, H, W/2+1) with a unit x stride")
+
+    def update(self, volume: torch.Tensor) -> None:
+        self._check(volume)
+        if tuple(volume.shape) != self.shape or volume.device != self.device:
+            raise NoaError(f"VolumeBlocks.update: expected shape {self.shape} on {self.device}")
+        keep: list = []
+        v = _view(volume, keep, abi.F32, abi.C32, False, "volume")
+        stream = torch.cuda.current_stream(volume.device).cuda_stream
+        if lib().noa_hip_volume_blocks(ctypes.byref(v), self.blocks.data_ptr(), self.blocks.numel(), stream) != abi.OK:
+            raise NoaError(last_error())
+
+
 class ComplexBF16:
     """A complex array stored as {bf16 real, bf16 imag} pairs: a bfloat16 tensor whose last dimension is 2."""
 
@@ -295,6 +327,16 @@ def _view(obj: Any, keep: list, real_dtype: int, cx_dtype: int, allow_value: boo
         v.value[1] = c.imag
         for i in range(4):
             v.shape[i] = 1
+        return v
+    if isinstance(obj, VolumeBlocks):
+        # The block copy: the volume's rfft shape with contiguous strides (the layout fixes the offsets).
+        d, h, w = obj.shape[1], obj.shape[2], obj.shape[3]
+        v.data = obj.blocks.data_ptr()
+        v.dtype = abi.C32_BLOCKS
+        for i, (n_, s_) in enumerate(zip(obj.shape, (d * h * w, h * w, w, 1))):
+            v.shape[i] = n_
+            v.strides[i] = s_
+        keep.append(obj)
         return v
     if isinstance(obj, ComplexBF16):
         t = obj.pairs

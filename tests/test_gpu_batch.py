@@ -204,3 +204,42 @@ def test_tuning_changes_no_result(monkeypatch, op, values):
         np.testing.assert_array_equal(o.cpu().numpy().view(np.uint32), w.cpu().numpy().view(np.uint32))
     for (wv, ww), (ov, ow) in zip(iwant, iouts):
         np.testing.assert_array_equal(ov.cpu().numpy(), wv.cpu().numpy())
+
+
+def test_batched_extraction_from_block_copies(monkeypatch):
+    """Batched extraction from VolumeBlocks (the 4x2x4-block copy made once) gives the bits of the single calls on
+    the plain volumes; after `update` with another volume, those of the other volume; a single call refuses it."""
+    print("case batch extract from block copies", flush=True)
+    monkeypatch.setenv("NOA_HIP_EXTRACT_LAYOUT", "linear")
+    rng = np.random.default_rng(2500)
+    jobs = make_extract(rng, EXTRACT_JOBS)
+    want = [extract_eager(j) for j in jobs]
+    blocks = [nx().VolumeBlocks(j["vol"]) for j in jobs]
+    outs = [torch.full_like(w, float("nan")) for w in want]
+    calls = [nb().extract_call(j["remap"], vb, (1, j["n"], j["n"], j["n"]), o, (j["b"], 1, j["n"], j["n"]), j["rot"],
+                               nx().ExtractCentralSlicesOptions(fftfreq_cutoff=j["cut"]))
+             for j, vb, o in zip(jobs, blocks, outs)]
+    nb().run_batch(calls, []).synchronize()
+    for w, o in zip(want, outs):
+        np.testing.assert_array_equal(o.cpu().numpy().view(np.uint32), w.cpu().numpy().view(np.uint32))
+
+    # A new volume copied into the same blocks.
+    j = jobs[1]
+    j2 = dict(j, vol=dev(random_spectrum(tuple(j["vol"].shape), rng)))
+    blocks[1].update(j2["vol"])
+    want2 = extract_eager(j2)
+    out2 = torch.full_like(want2, float("nan"))
+    call = nb().extract_call(j["remap"], blocks[1], (1, j["n"], j["n"], j["n"]), out2, (j["b"], 1, j["n"], j["n"]),
+                             j["rot"], nx().ExtractCentralSlicesOptions(fftfreq_cutoff=j["cut"]))
+    nb().run_batch([call], []).synchronize()
+    np.testing.assert_array_equal(out2.cpu().numpy().view(np.uint32), want2.cpu().numpy().view(np.uint32))
+
+    # Refusals, all raised on the host before any launch.
+    with pytest.raises(nx().NoaError, match="batched extraction only"):
+        nx().extract_central_slices_3d(j["remap"], blocks[1], None, (1, j["n"], j["n"], j["n"]), out2, None,
+                                       (j["b"], 1, j["n"], j["n"]), None, j["rot"])
+    with pytest.raises(nx().NoaError):
+        nb().run_batch([nb().extract_call(j["remap"], blocks[1], (1, 64, 64, 64), out2, (j["b"], 1, j["n"], j["n"]),
+                                          j["rot"])], [])
+    with pytest.raises(nx().NoaError, match="expected shape"):
+        blocks[1].update(jobs[0]["vol"])

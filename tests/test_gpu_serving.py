@@ -151,3 +151,19 @@ def test_full_serving_mix_alone_and_in_the_mix():
         assert torch.equal(r.result, alone.result), f"request {r.rid} ({r.kind}, box {r.box}) differs"
         if r.kind == "insert":
             assert torch.equal(r.result_weight, alone.result_weight)
+
+
+def test_block_copy_volumes_change_no_result():
+    """Extraction requests served from the resident volumes' block copies (the default) and from the volumes
+    themselves give the same bits."""
+    from noa_amd import workloads
+    from noa_amd.serve import Server, run_alone
+    dev = torch.device("cuda")
+    vols = {b: v[:2] for b, v in workloads.serving_volumes(dev, per_box=2).items() if b in (64, 128, 192)}
+    reqs = [r for r in _mix(dev, n=12, seed=9) if r.kind == "extract"]
+    print(f"case serving block copies: {len(reqs)} extraction requests", flush=True)
+    torch.cuda.synchronize()
+    for r in reqs:
+        a = run_alone(Server(vols, budget_bytes=256 << 20, n_streams=1), _clone(r))
+        b = run_alone(Server(vols, budget_bytes=256 << 20, n_streams=1, volume_blocks=False), _clone(r))
+        assert torch.equal(a.result, b.result), f"request {r.rid} differs with and without block copies"
