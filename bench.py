@@ -616,7 +616,7 @@ def main():
         be = autotune.tune("extract", extract, key=ek, reps=3, log=log,
                            candidates=[(1, 0), (2, 0), (3, 128)])
         bi = autotune.tune("insert", insert, key=ik, reps=3, log=log,
-                           candidates=[(1, 256), (2, 256)])
+                           candidates=[(1, 256), (2, 512)])
         tuning = {"extract": {"layout": {1: "row-major gather", 2: "block-copy gather", 3: "LDS tiles"}[be.values[0]],
                               "tile_pairs": be.values[1], "ms": be.ms},
                   "insert": {"kernel": {1: "one-wave", 2: "slab"}[bi.values[0]], "slices_per_item": bi.values[1],

@@ -267,7 +267,7 @@ int noa_hip_insert_central_slices_3d(const noa_hip_project_args* args, void* str
  * HIP graph capture, tight serving loops, harnesses that forbid allocation). Same arguments, rules and results
  * (bit-identical). The work plan, the per-slice records and the partial sums of split sub-bricks are carved from a
  * static device array of NOA_HIP_NOALLOC_SCRATCH_BYTES per GPU instead of being allocated; a call whose work needs
- * more returns NOA_HIP_ERROR_UNSUPPORTED with the size in noa_hip_last_error() (at 256^3, up to about 2,048 slices
+ * more returns NOA_HIP_ERROR_UNSUPPORTED with the size in noa_hip_last_error() (at 256^3, up to about 5,000 slices
  * fit). Every call on a GPU shares that GPU's array: calls that overlap on one GPU (different streams, or
  * threads) race and must be ordered by the caller. A geometry's first call builds its plan on the host. */
 int noa_hip_insert_central_slices_3d_noalloc(const noa_hip_project_args* args, void* stream);
@@ -315,7 +315,7 @@ int noa_hip_volume_blocks(const noa_hip_view* volume, void* blocks, int64_t capa
  *   NOA_HIP_OP_EXTRACT:       values = {volume layout: 0 auto, 1 row-major gather, 2 block-copy gather, 3 LDS tiles;
  *                                       expected crossing slices per tile work item (default 128); -; -}
  *   NOA_HIP_OP_INSERT:        values = {kernel: 0 auto (slab where it applies), 1 one-wave, 2 slab; expected
- *                                       slices per work item (default 256); x taps of a slab row: 0 auto (one load),
+ *                                       slices per work item (default 512 for the slab kernel, 256 for the one-wave kernel); x taps of a slab row: 0 auto (one load),
  *                                       1 one load, 2 two loads; item order: 0 by work, 1 by first slice, 2 XCD-aware,
  *                                       3 pose-adaptive (passing slices counted on the device first; the call then
  *                                       waits for the count, so it is not for stream capture)}

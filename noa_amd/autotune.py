@@ -35,7 +35,7 @@ OPS = {"extract": abi.OP_EXTRACT, "insert": abi.OP_INSERT, "extract_batch": abi.
 # Candidate values per operator (cartesian products).
 SPACE = {
     "extract": {"layout": [1, 2, 3], "pairs": [64, 128, 256]},
-    "insert": {"kernel": [1, 2], "list": [128, 256, 512], "paired": [0], "order": [0, 2, 3]},
+    "insert": {"kernel": [1, 2], "list": [256, 512, 1024], "paired": [0], "order": [0, 2, 3]},
     "extract_batch": {"ppt": [1, 2, 3, 4], "grid": [1024, 2048, 4096]},
     "insert_batch": {"grid": [3072, 6144, 12288]},
 }

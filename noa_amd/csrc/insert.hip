@@ -680,7 +680,10 @@ __global__ __launch_bounds__(NT_REDUCE) void insert_reduce_kernel(const InsertPa
 //     passing slice are issued before the current slice's sums are added.
 // ---------------------------------------------------------------------------------------------------------
 constexpr i32 SLAB_NWIN = 9;        // degree of the window polynomial
-constexpr i32 SLAB_MAXC = 1024;     // candidates of one pair: at most the sub-brick's 1024 voxels
+// Candidates of one pair: at most 4 per column (the slab kernel runs for fbT <= 1.15 samples, so a column's slab
+// range is shorter than 2 * (1.15 * sqrt(3) + 2e-3) < 4 samples) over at most 128 columns (the 8 x 16 faces).
+constexpr i32 SLAB_MAXC = 512;
+static_assert(WZ == 8 && WY == 8 && WX == 16, "slab candidate words decode lz = v >> 7, ly = (v >> 4) & 7, lx = v & 15");
 
 template<class In, class W>
 struct SlabParams {
@@ -745,10 +748,10 @@ struct SlabColumn {
     u32 word0;
 };
 
-// Candidate word: lx | ly << 8 | lz << 16 | (voxel index in the sub-brick) << 22 (lz < 8 stays below bit 22).
+// Candidate word: the voxel's index in the sub-brick, (lz * WY + ly) * WX + lx (a 16-bit list entry keeps the
+// workgroup's LDS at 13 KB: 12 resident waves per CU).
 __device__ __forceinline__ u32 slab_word(i32 lz, i32 ly, i32 lx) {
-    return static_cast<u32>(lx) | (static_cast<u32>(ly) << 8) | (static_cast<u32>(lz) << 16) |
-           (static_cast<u32>((lz * WY + ly) * WX + lx) << 22);
+    return static_cast<u32>((lz * WY + ly) * WX + lx);
 }
 
 template<int AX>
@@ -778,9 +781,9 @@ __device__ __forceinline__ u32 lanes_below(unsigned long long m) {
 // Packs the candidates of one face orientation into `cand` (ordered by rank j, then slot, then lane); returns their
 // number. `u0`: the slab centre along A at face coordinates (0, 0), relative to the sub-brick's first sample.
 template<int AX, int NSLOT>
-__device__ __forceinline__ u32 slab_candidates(u32* cand, const SlabColumn (&col)[2], float u0, float ub, float uc,
-                                               float hw) {
-    constexpr u32 STEP = AX == 0 ? ((1u << 16) | (128u << 22)) : (AX == 1 ? ((1u << 8) | (16u << 22)) : (1u | (1u << 22)));
+__device__ __forceinline__ u32 slab_candidates(unsigned short* cand, const SlabColumn (&col)[2], float u0, float ub,
+                                               float uc, float hw) {
+    constexpr u32 STEP = AX == 0 ? WY * WX : (AX == 1 ? WX : 1);
     float cnt[NSLOT];
     u32 w[NSLOT];
 #pragma unroll
@@ -800,7 +803,7 @@ __device__ __forceinline__ u32 slab_candidates(u32* cand, const SlabColumn (&col
             const unsigned long long m = __ballot(has);
             if (m == 0) continue;
             any = true;
-            if (has) cand[off + lanes_below(m)] = w[s] + static_cast<u32>(j) * STEP;
+            if (has) cand[off + lanes_below(m)] = static_cast<unsigned short>(w[s] + static_cast<u32>(j) * STEP);
             off += static_cast<u32>(__popcll(m));
         }
         if (!any) break;
@@ -822,7 +825,7 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
     constexpr bool CX = vt<In>::cx;
     constexpr i32 NCH = CX ? 3 : 2;                    // accumulator channels: re (, im), weight
     __shared__ float acc[NCH * WVOX];
-    __shared__ u32 cand[SLAB_MAXC];
+    __shared__ unsigned short cand[SLAB_MAXC];
 
     const i32 lane = static_cast<i32>(threadIdx.x);
     int4 item = p.items[blockIdx.x];
@@ -971,10 +974,10 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
                 const u32 pos = base + 64u * g + static_cast<u32>(lane);
                 const bool real = pos < total;
                 const u32 e = cand[real ? pos : 0u];
-                const float iz = f0zf + static_cast<float>((e >> 16) & 63u);
-                const float iy = f0yf + static_cast<float>((e >> 8) & 255u);
-                const float ix = f0xf + static_cast<float>(e & 255u);
-                t[g].lv = e >> 22;
+                const float iz = f0zf + static_cast<float>(e >> 7);           // WX * WY = 128 voxels per z
+                const float iy = f0yf + static_cast<float>((e >> 4) & 7u);
+                const float ix = f0xf + static_cast<float>(e & 15u);
+                t[g].lv = e;
                 // noa's grid2slice in samples (FourierProject.hpp:71-95 with R * (index / n) * n == R * index).
                 const float pz = (R00 * iz + R01 * iy) + R02 * ix;
                 float py = (R10 * iz + R11 * iy) + R12 * ix;
@@ -1539,7 +1542,7 @@ struct KernelChoice {
     bool force_wave;
     bool paired;          // slab kernel: one 16-byte (8-byte) load for the two x taps of a slice (weight) row
     int order;            // order of the work items (make_plan)
-    double list_target;
+    double list_target;   // expected passing slices per work item; 0: the kernel's default
 };
 KernelChoice kernel_choice(const ProjectDesc& d) {
     KernelChoice k{};
@@ -1549,7 +1552,7 @@ KernelChoice kernel_choice(const ProjectDesc& d) {
     // The slab kernel where it applies (5.5 against 6.0 ms at 256^3 with 10,000 slices), unless the table asks for
     // the one-wave kernel (1); its two x taps of a row as one load unless the table says 2.
     k.force_wave = v[0] == 1;
-    k.list_target = v[1] > 0 ? std::max(16.0, static_cast<double>(v[1])) : 256.0;
+    k.list_target = v[1] > 0 ? std::max(16.0, static_cast<double>(v[1])) : 0.0;   // 0: the kernel's default
     k.paired = v[2] != 2;
     k.order = v[3];
     if (const char* o = std::getenv("NOA_HIP_INSERT_ORDER")) k.order = std::atoi(o);
@@ -1724,8 +1727,30 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
             i32 step = 8;
             while (p.n_slices > 16 * step) step *= 2;
             const i32 plan_slices = std::max<i32>(64, (p.n_slices + step - 1) / step * step);
+            // The slab kernel: Mat33<f32> rotations, LINEAR, one power-of-two size n >= 32 for the slices, the
+            // volume and the target, cutoff <= 0.5, a window of at most ~1.15 samples (at most 4 candidates per column)
+            // that the degree-9 polynomial fits, unit x strides and one row pitch for the slices and their weights.
+            bool slab = false;
+            SlabWindow sw{};
+            if constexpr (std::is_same_v<C, float>) {
+                const i64 nn = d.target[0];
+                const bool cubic = d.target[1] == nn && d.target[2] == nn && p.grid[0] == nn && p.grid[1] == nn &&
+                                   p.grid[2] == nn && p.slice_shape[0] == nn && p.slice_shape[1] == nn;
+                if (!kc.force_wave && d.interp == NOA_HIP_INTERP_LINEAR && cubic && p.pow2 && nn >= 32 &&
+                    p.rot.kind == NOA_HIP_MAT33_F32 && cutoff <= C(0.5) && p.is_x == 1 &&
+                    (!p.win_present || (p.wis_x == 1 && p.wis_y == p.is_y)) && p.slice_bytes < 0x7ffffff0u &&
+                    p.wslice_bytes < 0x7ffffff0u) {
+                    sw = fit_slab_window(static_cast<float>(p.fftfreq_sinc), static_cast<float>(p.fftfreq_blackman),
+                                         static_cast<i32>(nn));
+                    slab = sw.ok && sw.fbT <= 1.15f;
+                }
+            }
+            // Expected passing slices per work item: the table's value, else 512 for the slab kernel (12 resident
+            // waves per CU: 4.6-4.7 ms against 5.0 ms at 256 for 10,000 slices at 256^3) and 256 for the one-wave
+            // kernel.
+            const double list_target = kc.list_target > 0 ? kc.list_target : (slab ? 512.0 : 256.0);
             const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
-                                     plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, kc.list_target,
+                                     plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, list_target,
                                      scratch == nullptr, planp, err, kc.order);
             if (st != NOA_HIP_OK) return st;
             // Pose-adaptive plan (item order 3; not for the allocation-free entry point): the passing slices of each
@@ -1784,9 +1809,9 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
                 for (i32 j = 0; j < nbk && !skewed; ++j) {
                     i64 total = 0;
                     for (i32 b = 0; b < CNT_BINS; ++b) total += counts[static_cast<size_t>(j) * CNT_BINS + b];
-                    skewed = static_cast<double>(total) > 2.0 * kc.list_target * shared.h_nseg[static_cast<size_t>(j)];
+                    skewed = static_cast<double>(total) > 2.0 * list_target * shared.h_nseg[static_cast<size_t>(j)];
                 }
-                if (skewed) make_pose_plan(shared.h_bricks, counts, p.n_slices, kc.list_target, pose);
+                if (skewed) make_pose_plan(shared.h_bricks, counts, p.n_slices, list_target, pose);
             }
             if (pose.n_items > 0) {
                 if (hipMallocAsync(&pose.items, pose.h_items.size() * sizeof(int4), stream) != hipSuccess ||
@@ -1864,24 +1889,6 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
                                   p.vol_centered == p.slices_centered && p.is_x == 1 && p.wis_x == 1 &&
                                   p.is_y == p.wis_y && p.slice_bytes < 0x7ffffff0u && p.wslice_bytes < 0x7ffffff0u;
             const int hot = hot_opts ? (p.vol_centered ? 2 : 1) : 0;
-            // The slab kernel: Mat33<f32> rotations, LINEAR, one power-of-two size n >= 32 for the slices, the
-            // volume and the target, cutoff <= 0.5, a window of at most ~1.15 samples (at most 4 candidates per column)
-            // that the degree-9 polynomial fits, unit x strides and one row pitch for the slices and their weights.
-            bool slab = false;
-            SlabWindow sw{};
-            if constexpr (std::is_same_v<C, float>) {
-                const i64 nn = d.target[0];
-                const bool cubic = d.target[1] == nn && d.target[2] == nn && p.grid[0] == nn && p.grid[1] == nn &&
-                                   p.grid[2] == nn && p.slice_shape[0] == nn && p.slice_shape[1] == nn;
-                if (!kc.force_wave && d.interp == NOA_HIP_INTERP_LINEAR && cubic && p.pow2 && nn >= 32 &&
-                    p.rot.kind == NOA_HIP_MAT33_F32 && cutoff <= C(0.5) && p.is_x == 1 &&
-                    (!p.win_present || (p.wis_x == 1 && p.wis_y == p.is_y)) && p.slice_bytes < 0x7ffffff0u &&
-                    p.wslice_bytes < 0x7ffffff0u) {
-                    sw = fit_slab_window(static_cast<float>(p.fftfreq_sinc), static_cast<float>(p.fftfreq_blackman),
-                                         static_cast<i32>(nn));
-                    slab = sw.ok && sw.fbT <= 1.15f;
-                }
-            }
             if (slab) {
                 if (!scratch && hipMallocAsync(&records, record_bytes, stream) != hipSuccess) {
                     if (partials) (void)hipFreeAsync(partials, stream);

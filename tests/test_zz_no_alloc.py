@@ -73,11 +73,11 @@ def test_noalloc_equals_the_allocating_entry_point_twice_in_a_row(monkeypatch, r
 
 
 def test_noalloc_refuses_work_beyond_the_scratch():
-    """256^3 with 4096 slices needs ~128 MB of partial sums: refused with the size, the volume untouched."""
-    print("case noalloc refusal 256^3 x 4096", flush=True)
+    """256^3 with 8192 slices needs ~120 MB of partial sums: refused with the size, the volume untouched."""
+    print("case noalloc refusal 256^3 x 8192", flush=True)
     m = nx()
     rng = np.random.default_rng(950)
-    n, b = 256, 4096
+    n, b = 256, 8192
     sl = torch.zeros((b, 1, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
     inv = torch.from_numpy(np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))).to(DEV)
     vol = torch.full((1, n, n, n // 2 + 1), 1 + 1j, dtype=torch.complex64, device=DEV)
