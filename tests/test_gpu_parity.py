@@ -124,7 +124,7 @@ def test_extract_odd_sizes_strided_output_single_rotation():
 @pytest.mark.parametrize("remap,n,h,shifts", [("hc2hc", 48, 48, False), ("h2h", 33, 35, True), ("hc2h", 31, 27, False),
                                               ("h2hc", 24, 50, True)])
 def test_extract_volume_layouts(monkeypatch, remap, n, h, shifts):
-    """The c32 extraction gathers from a 4x2x2-block copy of the volume when many slices are extracted; forced both
+    """The c32 extraction gathers from a 4x2x4-block copy of the volume when many slices are extracted; forced both
     ways here: the same bits in either layout, equal to the oracle (bit-exact without shifts, within the sin/cos
     rounding of the fused phase shift with them), odd and even sizes, slices larger or smaller than the volume."""
     print(f"case extract layouts {remap} n={n} h={h} shifts={shifts}", flush=True)
