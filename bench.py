@@ -698,6 +698,31 @@ def main():
         "kernels": kernels,
         "tuning": tuning,
     }
+    # The same step with the two operators on two streams (independent inputs and outputs): reported beside the
+    # sequential step that `value` and the per-kernel times come from.
+    side = torch.cuda.Stream(device=dev)
+    main_stream = torch.cuda.current_stream(dev)
+
+    def two_streams():
+        side.wait_stream(main_stream)
+        with torch.cuda.stream(side):
+            extract()
+        insert()
+        main_stream.wait_stream(side)
+
+    for _ in range(args.warmup):
+        two_streams()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        two_streams()
+    torch.cuda.synchronize()
+    wall2 = time.perf_counter() - t0
+    result["two_streams"] = {"ms_per_step": wall2 / args.steps * 1e3,
+                             "slices_per_s": slices_per_step / (wall2 / args.steps),
+                             "note": "extraction on a second stream, concurrent with the insertion; not `value`"}
+    log(f"two streams: {wall2 / args.steps * 1e3:.3f} ms per step, "
+        f"{slices_per_step / (wall2 / args.steps) / 1e6:.2f} M slices/s")
     if not args.no_neighbours:
         log("prologue / virtual volume ...")
         result["neighbours"] = neighbours_report(torch, dev, nx, ins, n)

@@ -36,6 +36,7 @@ def test_bench_line_keeps_the_contract():
     r = b["roofline"]
     assert r["bound"] in ("hbm", "mfma") and r["unit"] == "GB/s" and r["peak"] == 8000.0
     assert r["achieved"] > 0 and abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-9
+    assert b["two_streams"]["ms_per_step"] > 0 and b["two_streams"]["slices_per_s"] > 0
 
 
 def test_counter_summaries_describe_the_current_kernels():
