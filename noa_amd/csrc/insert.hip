@@ -706,10 +706,12 @@ struct SlabParams {
     bool vol_centered;
 };
 
-// One record per slice: {R00, R01, R02, -}, {R10, R11, R12, ub}, {R20, R21, R22, uc}, {hw, A, -, -}. With n = the
-// first row (pz = n . v in samples), A = argmax |n_a|, B < C the other axes: the slab |pz| <= fbT runs along A
+// One record per slice: {R00, R01, R02, -}, {R10, R11, R12, ub}, {R20, R21, R22, uc}, {hw, A, passes, -}. With n =
+// the first row (pz = n . v in samples), A = argmax |n_a|, B < C the other axes: the slab |pz| <= fbT runs along A
 // around ub * v_B + uc * v_C (ub = -n_B / n_A, uc = -n_C / n_A) with half-width fbT / |n_A|, widened by 2e-3
-// samples to cover the rounding of the exact per-voxel test (a few 1e-5 samples).
+// samples to cover the rounding of the exact per-voxel test (a few 1e-5 samples). A column holds at most
+// floor(2 hw) + 1 candidates: at most 4 when the row is a unit vector (|n_A| >= 1/sqrt(3), fbT <= 1.15), more when
+// it is shorter; `passes` = the packing passes of 4 ranks each that the slice needs (1 for rotations).
 __global__ __launch_bounds__(256) void slab_records_kernel(const XformArg<float> rot, i32 n_slices, float fbT,
                                                            float4* rec) {
     const i32 s = static_cast<i32>(blockIdx.x) * 256 + static_cast<i32>(threadIdx.x);
@@ -728,7 +730,9 @@ __global__ __launch_bounds__(256) void slab_records_kernel(const XformArg<float>
     o[0] = make_float4(m[0], m[1], m[2], 0.f);
     o[1] = make_float4(m[3], m[4], m[5], -nB * inv);
     o[2] = make_float4(m[6], m[7], m[8], -nC * inv);
-    o[3] = make_float4(fbT * fabsf(inv) + 2e-3f, __int_as_float(ax), 0.f, 0.f);
+    const float hw = fbT * fabsf(inv) + 2e-3f;
+    const i32 passes = (static_cast<i32>(fminf(2.f * hw, 15.f)) + 4) >> 2;
+    o[3] = make_float4(hw, __int_as_float(ax), __int_as_float(passes), 0.f);
 }
 
 // Largest integer m >= 0 with m^2 + r2 <= cut2 (all values exact in single precision), or -1 if none.
@@ -778,28 +782,31 @@ __device__ __forceinline__ u32 lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<u32>(m), 0u));
 }
 
-// Packs the candidates of one face orientation into `cand` (ordered by rank j, then slot, then lane); returns their
-// number. `u0`: the slab centre along A at face coordinates (0, 0), relative to the sub-brick's first sample.
+// Packs the candidates of ranks j0, j0 + 1, ... (at most `jn` ranks) of one face orientation into `cand` (ordered by
+// rank, then slot, then lane); returns their number. `u0`: the slab centre along A at face coordinates (0, 0),
+// relative to the sub-brick's first sample. Rotations: j0 = 0 and every rank (at most 4 per column, so at most
+// 4 x 128 = SLAB_MAXC candidates); thicker slabs: 4 ranks per pass (jn = 4).
 template<int AX, int NSLOT>
 __device__ __forceinline__ u32 slab_candidates(unsigned short* cand, const SlabColumn (&col)[2], float u0, float ub,
-                                               float uc, float hw) {
+                                               float uc, float hw, i32 j0, i32 jn) {
     constexpr u32 STEP = AX == 0 ? WY * WX : (AX == 1 ? WX : 1);
     float cnt[NSLOT];
     u32 w[NSLOT];
+    const float fj0 = static_cast<float>(j0);
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         const float centre = fmaf(ub, col[s].lb, fmaf(uc, col[s].lc, u0));
         const float kl = fmaxf(ceilf(centre - hw), col[s].klo);
         const float kh = fminf(floorf(centre + hw), col[s].khi);
-        cnt[s] = kh - kl;                                   // candidates - 1 (negative: none)
-        w[s] = col[s].word0 + static_cast<u32>(static_cast<i32>(fmaxf(kl, 0.f))) * STEP;
+        cnt[s] = (kh - kl) - fj0;                           // candidates from rank j0 on, - 1 (negative: none)
+        w[s] = col[s].word0 + static_cast<u32>(static_cast<i32>(fmaxf(kl, 0.f)) + j0) * STEP;
     }
     u32 off = 0;
-    for (i32 j = 0; j < 16; ++j) {
+    for (i32 j = 0; j < 16; ++j) {     // one exit and a constant bound: the compiler unrolls it (faster than a loop)
         bool any = false;
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s) {
-            const bool has = cnt[s] >= static_cast<float>(j);
+            const bool has = j < jn && cnt[s] >= static_cast<float>(j);
             const unsigned long long m = __ballot(has);
             if (m == 0) continue;
             any = true;
@@ -927,6 +934,7 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
     // Cross-pair pipeline: the taps of the last pair whose sums are not added yet.
     SlabTaps<In, W> pend[2];
     bool have = false;
+    i32 pass = 0;          // pass of the current slice (more than one only for rows that are not unit vectors)
 
     while (true) {
         while (mask == 0) {
@@ -947,18 +955,27 @@ __global__ __launch_bounds__(64) void insert_slab_kernel(const SlabParams<In, W>
         const float R20 = bcast(cur[2].x, k), R21 = bcast(cur[2].y, k), R22 = bcast(cur[2].z, k);
         const float ub = bcast(cur[1].w, k), uc = bcast(cur[2].w, k), hw = bcast(cur[3].x, k);
         const i32 ax = __builtin_amdgcn_readlane(__float_as_int(cur[3].y), k);
+        // Rotations pack a slice's candidates in one pass (at most 4 per column). Rows that are not unit vectors
+        // thicken the slab: such a slice takes one pass per 4 ranks (a voxel is a candidate at most once per slice,
+        // so the passes add distinct voxels); its bit of the mask is set again until the last pass.
+        const i32 passes = __builtin_amdgcn_readlane(__float_as_int(cur[3].z), k);
+        const i32 j0 = 4 * pass, jn = passes > 1 ? 4 : 16;
+        if (passes > 1) {
+            if (++pass < passes) mask |= 1ull << k;
+            else pass = 0;
+        }
 
         __syncthreads();   // the previous pair's candidate reads are done before the list is rewritten
         u32 total;
         if (ax == 0) {
             const float u0 = fmaf(ub, static_cast<float>(f0[1]), fmaf(uc, static_cast<float>(f0[2]), -f0zf));
-            total = slab_candidates<0, 2>(cand, c0, u0, ub, uc, hw);
+            total = slab_candidates<0, 2>(cand, c0, u0, ub, uc, hw, j0, jn);
         } else if (ax == 1) {
             const float u0 = fmaf(ub, static_cast<float>(f0[0]), fmaf(uc, static_cast<float>(f0[2]), -f0yf));
-            total = slab_candidates<1, 2>(cand, c1, u0, ub, uc, hw);
+            total = slab_candidates<1, 2>(cand, c1, u0, ub, uc, hw, j0, jn);
         } else {
             const float u0 = fmaf(ub, static_cast<float>(f0[0]), fmaf(uc, static_cast<float>(f0[1]), -f0xf));
-            total = slab_candidates<2, 1>(cand, c2, u0, ub, uc, hw);
+            total = slab_candidates<2, 1>(cand, c2, u0, ub, uc, hw, j0, jn);
         }
         __syncthreads();
 

@@ -355,6 +355,26 @@ def test_insert_many_slices_split_bricks(remap):
     np.testing.assert_array_equal(w1, w2)
 
 
+@pytest.mark.parametrize("kernel", ["slab", "wave"])
+def test_insert_rows_that_are_not_unit_vectors(monkeypatch, kernel):
+    # Scaled matrices (rows of norm 0.45 and 0.8): the slab |pz| <= f_b is thicker in samples than for a rotation, so
+    # a column of a sub-brick holds more than four candidates and the slab kernel packs a slice's candidates in
+    # several passes. Against the oracle, which evaluates noa's per-voxel expressions with the matrices as given.
+    print(f"case insert scaled matrices {kernel}", flush=True)
+    monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", kernel)
+    rng = np.random.default_rng(134)
+    n, b = 64, 60
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    scale = np.where(np.arange(b) % 2 == 0, 0.45, 0.8).astype(np.float32)[:, None, None]
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)) * scale).astype(np.float32)
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("h2h", s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv)
+    v, wv = _insert_dev("h2h", s, w, (b, 1, n, n), n, inv)
+    assert rel_l2(v, want) < 1e-6 and rel_l2(wv, wwant) < 1e-6
+
+
 def test_insert_oversampled_and_odd_sizes():
     # Slices larger than the volume (footprints exceed the LDS staging: global-read fallback), odd volume size,
     # explicit target shape and a cutoff above 0.5.
