@@ -289,10 +289,10 @@ def _insert_dev(remap, s, w, shape, n, inv, scale=None, opts=None, out_dtype=tor
 
 @pytest.mark.parametrize("remap,interp,fb", [("h2h", 1, -1.0), ("hc2hc", 1, 0.05), ("hc2h", 0, -1.0),
                                              ("h2hc", 1, 0.02)])
-def test_insert_staged_equals_general_path(remap, interp, fb):
+def test_insert_culled_equals_general_path(remap, interp, fb):
     # A batched identity scaling selects the general (one lane per voxel, every slice) kernel; no scaling selects
-    # the staged brick kernel. Both sum the same terms in the same order.
-    print(f"case insert staged-vs-general {remap} interp={interp} fb={fb}", flush=True)
+    # the culled one-wave kernel (48 is not a power of two). Both sum the same terms in the same order.
+    print(f"case insert culled-vs-general {remap} interp={interp} fb={fb}", flush=True)
     rng = np.random.default_rng(130)
     n, b = 48, 40
     s = random_spectrum((b, 1, n, n // 2 + 1), rng)
@@ -310,8 +310,8 @@ def test_insert_staged_equals_general_path(remap, interp, fb):
     assert rel_l2(v1, want) < 1e-6 and rel_l2(w1, wwant) < 1e-6
 
 
-def test_insert_staged_quaternion_real_and_abs2_outputs():
-    print("case insert staged quaternion / f32 / c32->f32", flush=True)
+def test_insert_culled_quaternion_real_and_abs2_outputs():
+    print("case insert culled quaternion / f32 / c32->f32", flush=True)
     rng = np.random.default_rng(131)
     n, b = 40, 25
     fwd = random_rotations(b, rng, dtype=np.float64)
