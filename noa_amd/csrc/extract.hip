@@ -535,9 +535,11 @@ struct ExtractBatchJob {
     i64 os_b; i32 os_y;                 // output strides (unit x stride)
     i32 vs0, vs1;                       // volume strides (unit x stride)
     i32 grid[3];                        // logical D, H, W of the volume
-    i32 slice_h, slice_wr, pixels, tiles;
+    i32 slice_h, slice_wr;
+    i32 pixels, tiles;                  // pixel slots of a slice (8 x 8 patches, then the last column) and units
+    i32 patch_cols, patch_pixels;       // patches per row of patches (columns 0 .. wr - 2), and their pixel slots
     float target[3], slice_f[2];
-    float cutoff_sqd, inv_wr;
+    float cutoff_sqd;
     i32 vol_centered, slice_centered;
     i32 blocks;                         // the volume is a block copy (noa_hip_volume_blocks): vs0, vs1 are its strides
 };
@@ -581,11 +583,25 @@ __global__ __launch_bounds__(256) void extract_batch_kernel(const unsigned char*
         const i32 first = tile * ppt * 256 + static_cast<i32>(threadIdx.x);
 #pragma unroll
         for (i32 k = 0; k < MAXP; ++k) {
+            // Pixel slots: 8 x 8 patches of columns 0 .. wr - 2 (a wave's 64 slots are one patch, a compact piece of
+            // the plane: fewer cache lines per tap load than 64 pixels of a row), then the last column.
             const i32 pix = first + k * 256;
-            const bool live = k < ppt && pix < q.pixels;
-            const i32 yq = static_cast<i32>((static_cast<float>(pix) + 0.5f) * q.inv_wr);
-            const i32 y = live ? yq : 0;
-            const i32 uu = live ? pix - yq * q.slice_wr : 0;
+            const bool in_range = k < ppt && pix < q.pixels;
+            i32 y, uu;
+            bool live;
+            if (pix < q.patch_pixels) {                  // uniform per wave: patch_pixels is a multiple of 64
+                const i32 patch = pix >> 6, l = pix & 63;
+                const i32 pr = patch / q.patch_cols, pc = patch - pr * q.patch_cols;
+                y = pr * 8 + (l >> 3);
+                uu = pc * 8 + (l & 7);
+                live = in_range && y < q.slice_h && uu < q.slice_wr - 1;
+            } else {
+                y = pix - q.patch_pixels;
+                uu = q.slice_wr - 1;
+                live = in_range && y < q.slice_h;
+            }
+            y = live ? y : 0;
+            uu = live ? uu : 0;
             ys[k] = live ? y : -1;
             us[k] = uu;
             vec2<float> f2;
@@ -910,14 +926,18 @@ int extract_batch_prepare(const ProjectDesc* descs, i32 n_jobs, void* image, i64
         }
         q.slice_h = static_cast<i32>(d.out_shape[2]);
         q.slice_wr = static_cast<i32>(d.out_shape[3] / 2 + 1);
-        q.pixels = q.slice_h * q.slice_wr;
-        if (q.pixels >= (1 << 21)) { err = at + "slices above 2^21 pixels are not batched"; return NOA_HIP_ERROR_UNSUPPORTED; }
+        if (static_cast<i64>(q.slice_h) * q.slice_wr >= (1 << 21)) {
+            err = at + "slices above 2^21 pixels are not batched";
+            return NOA_HIP_ERROR_UNSUPPORTED;
+        }
+        q.patch_cols = (q.slice_wr - 1 + 7) / 8;
+        q.patch_pixels = (q.slice_h + 7) / 8 * q.patch_cols * 64;
+        q.pixels = q.patch_pixels + q.slice_h;
         q.tiles = (q.pixels + 256 * ppt - 1) / (256 * ppt);
         q.slice_f[0] = static_cast<float>(d.out_shape[2]);
         q.slice_f[1] = static_cast<float>(d.out_shape[3]);
         const float cutoff = std::max(static_cast<float>(d.cutoff), 0.f);
         q.cutoff_sqd = cutoff * cutoff;
-        q.inv_wr = 1.f / static_cast<float>(q.slice_wr);
         q.vol_centered = d.src_centered ? 1 : 0;
         q.slice_centered = d.dst_centered ? 1 : 0;
         units += d.out_shape[0] * q.tiles;

@@ -68,7 +68,7 @@ struct InsertParams {
     bool general, slices_centered, vol_centered;
     C inv_target[3];      // 1 / target
     bool pow2;            // every target is a power of two (then x / t == x * (1 / t) exactly)
-    float wa[NWIN + 1];   // windowed sinc as a polynomial in t = 2 (fz/f_b)^2 - 1 (single-precision staged path)
+    float wa[NWIN + 1];   // windowed sinc as a polynomial in t = 2 (fz/f_b)^2 - 1 (single-precision culled paths)
     bool wpoly;           // use wa; otherwise evaluate noa's expression
     float inv_fb;         // 1 / fftfreq_blackman
     u32 slice_bytes, wslice_bytes;   // byte extent of one slice and of its weights (buffer loads)
@@ -105,7 +105,7 @@ __device__ __forceinline__ void rot_matrix(const Rot<C>& r, C m[9]) {
     m[6] = txz + twy; m[7] = txy - twz; m[8] = 1 - (tyy + tzz);
 }
 
-// Windowed sinc of the staged kernel in single precision: a polynomial of degree NWIN in t = 2 (fz/f_b)^2 - 1
+// Windowed sinc of the culled kernels in single precision: a polynomial of degree NWIN in t = 2 (fz/f_b)^2 - 1
 // (the window is even in fz), fitted on the host per (f_s, f_b) and accepted only when its error, evaluated
 // in single precision, stays below 4e-7 on [-f_b, f_b] (noa's FP32 expression itself carries ~2e-7 of rounding).
 __device__ __forceinline__ float window_poly(float fz, float inv_fb, const float (&a)[NWIN + 1]) {
