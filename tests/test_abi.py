@@ -507,3 +507,38 @@ def test_batch_prepare_argument_errors(lib):
     lib.noa_hip_project_args_init(calls)
     assert lib.noa_hip_insert_batch_prepare(calls, 1, None, 0, ctypes.byref(size)) != 0
     assert lib.noa_hip_last_error().decode().startswith("batch call 0: ")
+
+
+def test_volume_blocks_sizes_and_argument_errors(lib):
+    """The block copy's size (4x2x4 blocks of 32 c32 values, each extent rounded up to the block's) and the argument
+    checks of noa_hip_volume_blocks, all made before any device access; a block-copy view is refused by the single
+    extraction call."""
+    from noa_amd import abi
+    shape = (ctypes.c_int64 * 4)(1, 256, 256, 129)
+    assert lib.noa_hip_volume_blocks_bytes(shape) == 256 * 256 * 132 * 8
+    odd = (ctypes.c_int64 * 4)(1, 33, 33, 17)
+    assert lib.noa_hip_volume_blocks_bytes(odd) == 36 * 34 * 20 * 8
+    vol = np.zeros((1, 16, 16, 9), np.complex64)
+    v = abi.View()
+    v.data = vol.ctypes.data
+    for i in range(4):
+        v.shape[i] = vol.shape[i]
+        v.strides[i] = vol.strides[i] // vol.itemsize
+    v.dtype = abi.C32
+    need = lib.noa_hip_volume_blocks_bytes((ctypes.c_int64 * 4)(*vol.shape))
+    dst = np.zeros(need // 8, np.complex64)
+    assert lib.noa_hip_volume_blocks(None, dst.ctypes.data, need, None) == abi.ERROR_INVALID_ARGUMENT
+    assert "null" in lib.noa_hip_last_error().decode()
+    assert lib.noa_hip_volume_blocks(ctypes.byref(v), dst.ctypes.data, need - 1, None) == abi.ERROR_INVALID_ARGUMENT
+    assert f"needs {need} bytes" in lib.noa_hip_last_error().decode()
+    v.dtype = abi.F32
+    assert lib.noa_hip_volume_blocks(ctypes.byref(v), dst.ctypes.data, need, None) == abi.ERROR_INVALID_ARGUMENT
+    assert "c32" in lib.noa_hip_last_error().decode()
+    v.dtype = abi.C32
+    v.shape[0] = 2
+    assert lib.noa_hip_volume_blocks(ctypes.byref(v), dst.ctypes.data, need, None) == abi.ERROR_INVALID_ARGUMENT
+    a, keep = _args(lib)
+    a.input.dtype = abi.C32_BLOCKS
+    assert lib.noa_hip_extract_central_slices_3d(ctypes.byref(a), None) == abi.ERROR_UNSUPPORTED
+    assert "batched extraction only" in lib.noa_hip_last_error().decode()
+    del keep
