@@ -322,7 +322,8 @@ int noa_hip_volume_blocks(const noa_hip_view* volume, void* blocks, int64_t capa
  *   NOA_HIP_OP_EXTRACT_BATCH: values = {pixels per lane, 1-4 (default 3); workgroups (default 2048); -; -}
  *   NOA_HIP_OP_INSERT_BATCH:  values = {workgroups (default 3072); -; -; -}
  * Environment variables (NOA_HIP_EXTRACT_LAYOUT, NOA_HIP_EXTRACT_PAIRS, NOA_HIP_INSERT_KERNEL, NOA_HIP_INSERT_LIST,
- * NOA_HIP_INSERT_PAIRED, NOA_HIP_INSERT_ORDER) override the table. */
+ * NOA_HIP_INSERT_PAIRED, NOA_HIP_INSERT_ORDER) override the table; NOA_HIP_INSERT_SPAN sets the most slices of one
+ * insertion work item (default 1024, 0: no bound; not applied by the allocation-free entry point or item order 3). */
 enum {
     NOA_HIP_OP_EXTRACT = 1,
     NOA_HIP_OP_INSERT = 2,

@@ -62,6 +62,9 @@ def both(remap, sl, w, inv, b, n):
 def test_noalloc_equals_the_allocating_entry_point_twice_in_a_row(monkeypatch, remap, n, b1, b2, kernel):
     print(f"case noalloc {remap} n={n} b={b1} then b={b2} kernel={kernel}", flush=True)
     monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", kernel)
+    # The allocation-free entry point does not bound the slices per work item (its partial sums stay in the
+    # scratch); with the same bound off, the allocating entry point builds the same plan: the same bits.
+    monkeypatch.setenv("NOA_HIP_INSERT_SPAN", "0")
     rng = np.random.default_rng(900 + n)
     first = inputs(rng, n, b1)
     second = inputs(rng, n, b2)
@@ -70,6 +73,18 @@ def test_noalloc_equals_the_allocating_entry_point_twice_in_a_row(monkeypatch, r
         assert np.abs(ref_v).max() > 0 and np.abs(ref_w).max() > 0
         np.testing.assert_array_equal(got_v, ref_v)
         np.testing.assert_array_equal(got_w, ref_w)
+
+
+def test_noalloc_matches_the_allocating_default_plan():
+    """With its default plan (at most 1,024 slices per work item) the allocating entry point regroups the partial
+    sums differently: the same result within rel-L2 1e-6."""
+    print("case noalloc vs default plan h2h n=128 b=3000", flush=True)
+    rng = np.random.default_rng(960)
+    n, b = 128, 3000
+    sl, w, inv = inputs(rng, n, b)
+    (ref_v, ref_w), (got_v, got_w) = both("h2h", sl, w, inv, b, n)
+    rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    assert rel(got_v, ref_v) < 1e-6 and rel(got_w, ref_w) < 1e-6
 
 
 def test_noalloc_refuses_work_beyond_the_scratch():
