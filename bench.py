@@ -82,7 +82,7 @@ def cpu_cores() -> int:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))   # at most the CPU share of one GPU box
+    return max(1, min(n, 16))   # at most 16 threads: the host cores that serve one GPU
 
 
 def cpu_model() -> str:
