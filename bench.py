@@ -470,7 +470,39 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
         "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
         "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
     log(f"bspline prefilter: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+
+    # Gridding correction of the reconstructed 256^3 real volume in place (divided by sinc^2), and the h2hc remap of
+    # the 256^3 rfft volume (out of place): one read and one write of the volume each.
+    t = timed(lambda: nx.fourier_interpolation_correction(vol, vol))
+    nbytes = 2 * vol.numel() * 4
+    rep["gridding_correction"] = {
+        "workload": f"fourier_interpolation_correction of a {n}^3 f32 volume in place (LINEAR, post-correction)",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
+    log(f"gridding correction: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
     del vol
+    spec = torch.randn((1, n, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    spec_c = torch.empty_like(spec)
+    t = timed(lambda: nx.remap("h2hc", spec, spec_c, (1, n, n, n)))
+    nbytes = 2 * spec.numel() * 8
+    rep["remap_volume"] = {
+        "workload": f"remap h2hc of a {n}^3 c32 rfft volume (out of place)",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
+    log(f"remap {n}^3: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del spec, spec_c
+
+    # The standalone phase shift of 4096 extracted 256^2 slices (h2h) in place, one shift per slice.
+    sl = torch.randn((4096, 1, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    sh = torch.from_numpy(np.random.default_rng(12).uniform(-8, 8, (4096, 2)).astype(np.float32)).to(dev)
+    t = timed(lambda: nx.phase_shift_2d("h2h", sl, sl, (4096, 1, n, n), sh))
+    nbytes = 2 * sl.numel() * 8 + sh.numel() * 4
+    rep["phase_shift_slices"] = {
+        "workload": f"phase_shift_2d of 4096 {n}^2 rfft slices (h2h) in place, one shift per slice",
+        "avg_ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+        "frac_hbm_peak": nbytes / t / 1e9 / HBM_PEAK_GBS}
+    log(f"phase shift 4096 slices: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s")
+    del sl, sh
 
     # FFTs on either side of the path (rocFFT plans cached as in noa): the c2r of the reconstructed 256^3 rfft
     # volume, and the r2c of a stack of 1024 real 256^2 images into "h" slices.

@@ -27,6 +27,70 @@ __global__ __launch_bounds__(256) void remap_kernel(const T* in, T* out, i64 is0
     out[b * os0 + oz * os1 + oy * os2 + x * os3] = in[b * is0 + z * is1 + y * is2 + x * is3];
 }
 
+// The same two operators over the flattened elements, four per lane with every load issued before any store (the
+// row-per-block kernels above leave lanes idle on rows shorter than 256 and keep one load in flight per lane); for
+// launches below 2^31 elements. Same per-element expressions.
+constexpr i32 SPEC_EPT = 4;
+
+template<class T>
+__global__ __launch_bounds__(256) void remap_flat_kernel(const T* in, T* out, i64 is0, i64 is1, i64 is2, i64 is3,
+                                                         i64 os0, i64 os1, i64 os2, i64 os3, u32 d, u32 h, u32 wr,
+                                                         u32 n, bool src_c, bool dst_c) {
+    const u32 base = blockIdx.x * (256u * SPEC_EPT) + threadIdx.x;
+    T v[SPEC_EPT];
+    i64 o[SPEC_EPT];
+#pragma unroll
+    for (i32 e = 0; e < SPEC_EPT; ++e) {
+        const u32 t = min(base + static_cast<u32>(e) * 256u, n - 1);
+        const u32 row = t / wr, x = t - row * wr;
+        const u32 bz = row / h, y = row - bz * h;
+        const u32 b = bz / d, z = bz - b * d;
+        const i32 oz = frequency2index(dst_c, index2frequency(src_c, static_cast<i32>(z), static_cast<i32>(d)),
+                                       static_cast<i32>(d));
+        const i32 oy = frequency2index(dst_c, index2frequency(src_c, static_cast<i32>(y), static_cast<i32>(h)),
+                                       static_cast<i32>(h));
+        v[e] = in[b * is0 + z * is1 + y * is2 + x * is3];
+        o[e] = b * os0 + oz * os1 + oy * os2 + x * os3;
+    }
+#pragma unroll
+    for (i32 e = 0; e < SPEC_EPT; ++e)
+        if (base + static_cast<u32>(e) * 256u < n) out[o[e]] = v[e];
+}
+
+template<class In, class Out, class C>
+__global__ __launch_bounds__(256) void gridding_flat_kernel(const In* in, Out* out, i64 is0, i64 is1, i64 is2,
+                                                            i64 is3, i64 os0, i64 os1, i64 os2, i64 os3, u32 d, u32 h,
+                                                            u32 w, u32 n, C fs0, C fs1, C fs2, C half0, C half1,
+                                                            C half2, bool post) {
+    const u32 base = blockIdx.x * (256u * SPEC_EPT) + threadIdx.x;
+    In v[SPEC_EPT];
+    u32 js[SPEC_EPT], ks[SPEC_EPT], ls[SPEC_EPT], bs[SPEC_EPT];
+#pragma unroll
+    for (i32 e = 0; e < SPEC_EPT; ++e) {
+        const u32 t = min(base + static_cast<u32>(e) * 256u, n - 1);
+        const u32 row = t / w, l = t - row * w;
+        const u32 bj = row / h, k = row - bj * h;
+        const u32 b = bj / d, j = bj - b * d;
+        v[e] = in[b * is0 + j * is1 + k * is2 + l * is3];
+        js[e] = j; ks[e] = k; ls[e] = l; bs[e] = b;
+    }
+#pragma unroll
+    for (i32 e = 0; e < SPEC_EPT; ++e) {
+        if (base + static_cast<u32>(e) * 256u >= n) continue;
+        // GriddingCorrection::operator() (:1037-1054).
+        C dz = static_cast<C>(static_cast<i32>(js[e])), dy = static_cast<C>(static_cast<i32>(ks[e]));
+        C dx = static_cast<C>(static_cast<i32>(ls[e]));
+        dz -= half0; dy -= half1; dx -= half2;
+        dz /= fs0; dy /= fs1; dx /= fs2;
+        const C PI = static_cast<C>(3.141592653589793238462643383279502884);
+        const C radius = sqrt((dz * dz + dy * dy) + dx * dx);
+        const C xpi = PI * radius;
+        const C sinc = xpi == 0 ? C(1) : sin(xpi) / xpi;
+        const In sinc2 = static_cast<In>(sinc * sinc);
+        out[bs[e] * os0 + js[e] * os1 + ks[e] * os2 + ls[e] * os3] = static_cast<Out>(post ? v[e] / sinc2 : v[e] * sinc2);
+    }
+}
+
 template<class In, class Out, class C>
 __global__ __launch_bounds__(256) void gridding_kernel(const In* in, Out* out, i64 is0, i64 is1, i64 is2,
                                                        i64 is3, i64 os0, i64 os1, i64 os2, i64 os3, i32 d, i32 h,
@@ -71,6 +135,17 @@ int run_remap(const RemapDesc& dd, hipStream_t stream, std::string& err) {
     const i32 tiles_x = (wr + 255) / 256;
     const i64 blocks = dd.shape[0] * d * static_cast<i64>(h) * tiles_x;
     if (blocks == 0) return NOA_HIP_OK;
+    const i64 n = dd.shape[0] * d * static_cast<i64>(h) * wr;
+    if (n < (1LL << 31)) {
+        hipLaunchKernelGGL((remap_flat_kernel<T>), dim3(static_cast<u32>((n + 256 * SPEC_EPT - 1) / (256 * SPEC_EPT))),
+                           dim3(256), 0, stream, static_cast<const T*>(dd.input.data), static_cast<T*>(dd.output.data),
+                           dd.input.s[0], dd.input.s[1], dd.input.s[2], dd.input.s[3], dd.output.s[0], dd.output.s[1],
+                           dd.output.s[2], dd.output.s[3], static_cast<u32>(d), static_cast<u32>(h),
+                           static_cast<u32>(wr), static_cast<u32>(n), dd.src_centered, dd.dst_centered);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+        return NOA_HIP_OK;
+    }
     if (blocks > 0xffffffffLL) { err = "remap: too many elements for one launch"; return NOA_HIP_ERROR_UNSUPPORTED; }
     hipLaunchKernelGGL((remap_kernel<T>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
                        static_cast<const T*>(dd.input.data), static_cast<T*>(dd.output.data), dd.input.s[0],
@@ -91,6 +166,18 @@ int run_gridding(const GriddingDesc& g, hipStream_t stream, std::string& err) {
     // m_f_shape and m_half = shape / 2 * (shape != 1) (:1027-1033).
     const C fs0 = static_cast<C>(d), fs1 = static_cast<C>(h), fs2 = static_cast<C>(w);
     const C h0 = fs0 / 2 * static_cast<C>(d != 1), h1 = fs1 / 2 * static_cast<C>(h != 1), h2 = fs2 / 2 * static_cast<C>(w != 1);
+    const i64 n = g.shape[0] * d * static_cast<i64>(h) * w;
+    if (n < (1LL << 31)) {
+        hipLaunchKernelGGL((gridding_flat_kernel<In, Out, C>),
+                           dim3(static_cast<u32>((n + 256 * SPEC_EPT - 1) / (256 * SPEC_EPT))), dim3(256), 0, stream,
+                           static_cast<const In*>(g.input.data), static_cast<Out*>(g.output.data), g.input.s[0],
+                           g.input.s[1], g.input.s[2], g.input.s[3], g.output.s[0], g.output.s[1], g.output.s[2],
+                           g.output.s[3], static_cast<u32>(d), static_cast<u32>(h), static_cast<u32>(w),
+                           static_cast<u32>(n), fs0, fs1, fs2, h0, h1, h2, g.post);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { err = std::string("kernel launch failed: ") + hipGetErrorString(e); return NOA_HIP_ERROR_RUNTIME; }
+        return NOA_HIP_OK;
+    }
     hipLaunchKernelGGL((gridding_kernel<In, Out, C>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
                        static_cast<const In*>(g.input.data), static_cast<Out*>(g.output.data), g.input.s[0],
                        g.input.s[1], g.input.s[2], g.input.s[3], g.output.s[0], g.output.s[1], g.output.s[2],
