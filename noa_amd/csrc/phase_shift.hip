@@ -21,25 +21,43 @@ struct PhaseShiftParams {
     int mode;
 };
 
+// PS_EPT elements of one slice per lane (tiles of 256 * PS_EPT elements), the input loads issued before any phase
+// is evaluated.
+constexpr i32 PS_EPT = 4;
+
 template<class V, class R>
 __global__ __launch_bounds__(256) void phase_shift_kernel(const PhaseShiftParams<V, R> p) {
     const i32 b = static_cast<i32>(blockIdx.x) / p.tiles;
-    const i32 e = (static_cast<i32>(blockIdx.x) - b * p.tiles) * 256 + static_cast<i32>(threadIdx.x);
-    if (e >= p.h * p.wr) return;
-    const i32 y = e / p.wr, x = e - y * p.wr;
-    const i32 fy = index2frequency(p.src_centered, y, p.h);
-    const i32 oy = frequency2index(p.dst_centered, fy, p.h);
-    V* o = p.out + static_cast<i64>(b) * p.os_b + oy * p.os_y + x * p.os_x;
-    if (p.mode == 1) {
-        if (p.in) *o = p.in[static_cast<i64>(b) * p.is_b + y * p.is_y + x * p.is_x];
-        else { V one; one.x = 1; one.y = 0; *o = one; }
-        return;
+    const i32 first = (static_cast<i32>(blockIdx.x) - b * p.tiles) * (256 * PS_EPT) + static_cast<i32>(threadIdx.x);
+    const i32 count = p.h * p.wr;
+    V v[PS_EPT];
+    i32 ys[PS_EPT], xs[PS_EPT];
+#pragma unroll
+    for (i32 k = 0; k < PS_EPT; ++k) {
+        const i32 e = min(first + k * 256, count - 1);   // lanes past the end repeat the last element, not stored
+        const i32 y = e / p.wr, x = e - y * p.wr;
+        ys[k] = y;
+        xs[k] = x;
+        if (p.in) v[k] = p.in[static_cast<i64>(b) * p.is_b + y * p.is_y + x * p.is_x];
     }
     const R sy = p.batched ? p.shifts[2 * b] : p.single[0];
     const R sx = p.batched ? p.shifts[2 * b + 1] : p.single[1];
-    const V phase = phase_shift_at<V, R>(p.mode, sy, sx, fy, x, p.norm_y, p.norm_x, p.cutoff_sqd);
-    if (p.in) *o = cmul(p.in[static_cast<i64>(b) * p.is_b + y * p.is_y + x * p.is_x], phase);
-    else *o = phase;
+#pragma unroll
+    for (i32 k = 0; k < PS_EPT; ++k) {
+        if (first + k * 256 >= count) continue;
+        const i32 y = ys[k], x = xs[k];
+        const i32 fy = index2frequency(p.src_centered, y, p.h);
+        const i32 oy = frequency2index(p.dst_centered, fy, p.h);
+        V* o = p.out + static_cast<i64>(b) * p.os_b + oy * p.os_y + x * p.os_x;
+        if (p.mode == 1) {
+            if (p.in) *o = v[k];
+            else { V one; one.x = 1; one.y = 0; *o = one; }
+            continue;
+        }
+        const V phase = phase_shift_at<V, R>(p.mode, sy, sx, fy, x, p.norm_y, p.norm_x, p.cutoff_sqd);
+        if (p.in) *o = cmul(v[k], phase);
+        else *o = phase;
+    }
 }
 
 namespace {
@@ -54,7 +72,7 @@ int run(const PhaseShiftDesc& d, hipStream_t stream, std::string& err) {
     p.h = static_cast<i32>(d.h);
     p.w = static_cast<i32>(d.w);
     p.wr = p.w / 2 + 1;
-    p.tiles = (p.h * p.wr + 255) / 256;
+    p.tiles = (p.h * p.wr + 256 * PS_EPT - 1) / (256 * PS_EPT);
     p.src_centered = d.src_centered;
     p.dst_centered = d.dst_centered;
     p.shifts = static_cast<const R*>(d.shift.data);
