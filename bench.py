@@ -352,7 +352,8 @@ def neighbours_report(torch, dev, nx, ins, n, reps=5):
     wts = torch.empty((b, 1, n, n // 2 + 1), dtype=torch.float32, device=dev)
 
     def timed(fn, reps=reps):
-        fn()
+        for _ in range(3):      # warm-up: plans, caches and the clocks reach their steady state
+            fn()
         torch.cuda.synchronize()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record()

@@ -318,6 +318,73 @@ __global__ __launch_bounds__(256) void ctf_nd_kernel(const CtfNdParams<T, C, In,
     }
 }
 
+// The same over the flattened elements (lines shorter than 256 * CTF_EPT leave most of the per-line kernel's lanes
+// idle): CTF_EPT elements per lane, the input loads issued before any CTF is evaluated; for fewer than 2^32 elements.
+template<int N, bool HAS_INPUT, class T, class C, class In, class Out>
+__global__ __launch_bounds__(256) void ctf_nd_flat_kernel(const CtfNdParams<T, C, In, Out> p, u32 total) {
+    constexpr bool ISO = !std::is_same<C, CtfAniso<T>>::value;
+    using RIn = typename vt<In>::real;
+    const u32 base = blockIdx.x * (256u * CTF_EPT) + threadIdx.x;
+    In v[CTF_EPT];
+    i64 oo[CTF_EPT];
+    T fs[CTF_EPT][3];
+    u32 bs[CTF_EPT];
+#pragma unroll
+    for (u32 e = 0; e < CTF_EPT; ++e) {
+        const u32 t = min(base + e * 256u, total - 1);   // lanes past the end repeat the last element, not stored
+        u32 idx[3];
+        const u32 line = t / p.iter[N - 1];
+        idx[N - 1] = t - line * p.iter[N - 1];
+        u32 r = line;
+#pragma unroll
+        for (int i = N - 2; i >= 0; --i) {
+            idx[i] = r % p.iter[i];
+            r /= p.iter[i];
+        }
+        const u32 b = r;
+        i64 in_off = static_cast<i64>(b) * p.is[0], out_off = static_cast<i64>(b) * p.os[0];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const bool halfdim = p.half && i == N - 1;
+            const i32 j = static_cast<i32>(idx[i]);
+            const i32 freq = halfdim ? j : index2frequency(p.dst_centered, j, p.dims[i]);
+            const i32 src = halfdim ? j : frequency2index(p.src_centered, freq, p.dims[i]);   // remap_indices
+            fs[e][i] = static_cast<T>(freq) * p.step[i];
+            if constexpr (HAS_INPUT) in_off += static_cast<i64>(src) * p.is[i + 1];
+            out_off += static_cast<i64>(j) * p.os[i + 1];
+        }
+        if constexpr (HAS_INPUT) v[e] = p.in[in_off];
+        oo[e] = out_off;
+        bs[e] = b;
+    }
+#pragma unroll
+    for (u32 e = 0; e < CTF_EPT; ++e) {
+        if (base + e * 256u >= total) continue;
+        const C c = p.ctfs ? p.ctfs[bs[e]] : p.single;
+        const T* f = fs[e];
+        T ctf;
+        if constexpr (ISO) {
+            T arg;
+            if constexpr (N == 1) arg = f[0] + p.start;
+            else if constexpr (N == 2) arg = sqrt(f[0] * f[0] + f[1] * f[1]);
+            else arg = sqrt((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]);
+            ctf = ctf_value_at(c, arg);
+        } else {
+            ctf = ctf_value_at(c, f[0], f[N - 1]);
+        }
+        if (p.ctf_squared) ctf *= ctf;
+        else if (p.ctf_abs) ctf = fabs(ctf);
+        if constexpr (HAS_INPUT) {
+            p.out[oo[e]] = vcast<Out>(vmul(v[e], static_cast<RIn>(ctf)));
+        } else {
+            Out value = vzero<Out>();
+            if constexpr (vt<Out>::cx) value.x = static_cast<typename vt<Out>::real>(ctf);
+            else value = static_cast<Out>(ctf);
+            p.out[oo[e]] = value;
+        }
+    }
+}
+
 namespace {
 
 template<class T, class C, class In, class Out>
@@ -349,12 +416,23 @@ int run_nd(const CtfNdDesc& d, hipStream_t stream, std::string& err) {
     p.ctf_abs = d.ctf_abs;
     p.ctf_squared = d.ctf_squared;
     const dim3 grid(static_cast<u32>(blocks)), block(256);
+    const i64 total = lines * static_cast<i64>(p.iter[n - 1]);
+    const bool flat = total < (1LL << 32);
+    const dim3 fgrid(static_cast<u32>((total + 256 * CTF_EPT - 1) / (256 * CTF_EPT)));
     auto launch = [&](auto n_tag) {
         constexpr int NN = decltype(n_tag)::value;
         if constexpr (NN == 2 || !std::is_same<C, CtfAniso<T>>::value) {
             if (p.in) {
-                if constexpr (vt<In>::cx || !vt<Out>::cx)
-                    hipLaunchKernelGGL((ctf_nd_kernel<NN, true, T, C, In, Out>), grid, block, 0, stream, p);
+                if constexpr (vt<In>::cx || !vt<Out>::cx) {
+                    if (flat)
+                        hipLaunchKernelGGL((ctf_nd_flat_kernel<NN, true, T, C, In, Out>), fgrid, block, 0, stream, p,
+                                           static_cast<u32>(total));
+                    else
+                        hipLaunchKernelGGL((ctf_nd_kernel<NN, true, T, C, In, Out>), grid, block, 0, stream, p);
+                }
+            } else if (flat) {
+                hipLaunchKernelGGL((ctf_nd_flat_kernel<NN, false, T, C, In, Out>), fgrid, block, 0, stream, p,
+                                   static_cast<u32>(total));
             } else {
                 hipLaunchKernelGGL((ctf_nd_kernel<NN, false, T, C, In, Out>), grid, block, 0, stream, p);
             }
