@@ -158,6 +158,24 @@ struct Rot {
     }
 };
 
+// The 3x3 matrix of a rotation (for quaternions: Quaternion::to_matrix, Quaternion.hpp:69-94); only used to
+// bound and enumerate (plane geometry), never for the exact per-sample expressions.
+template<class C>
+__device__ __forceinline__ void rot_matrix(const Rot<C>& r, C m[9]) {
+    if (!r.quat) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) m[i] = r.m[i];
+        return;
+    }
+    const C z = r.m[0], y = r.m[1], x = r.m[2], w = r.m[3];
+    const C tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const C twx = tx * w, twy = ty * w, twz = tz * w;
+    const C txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    m[0] = 1 - (txx + tyy); m[1] = tyz + twx; m[2] = txz - twy;
+    m[3] = tyz - twx; m[4] = 1 - (txx + tzz); m[5] = txy + twz;
+    m[6] = txz + twy; m[7] = txy - twz; m[8] = 1 - (tyy + tzz);
+}
+
 template<class C>
 __device__ __forceinline__ Rot<C> load_rotation(const XformArg<C>& a, i64 b) {
     Rot<C> r;

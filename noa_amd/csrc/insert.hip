@@ -87,24 +87,6 @@ __device__ __forceinline__ i32 axis_freq(bool is_x, i32 i, bool centered, i32 n)
     return is_x ? i : index2frequency(centered, i, n);
 }
 
-// The 3x3 matrix of a rotation (for quaternions: Quaternion::to_matrix, Quaternion.hpp:69-94); only used to
-// bound and enumerate, never for the exact per-voxel expressions.
-template<class C>
-__device__ __forceinline__ void rot_matrix(const Rot<C>& r, C m[9]) {
-    if (!r.quat) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) m[i] = r.m[i];
-        return;
-    }
-    const C z = r.m[0], y = r.m[1], x = r.m[2], w = r.m[3];
-    const C tx = 2 * x, ty = 2 * y, tz = 2 * z;
-    const C twx = tx * w, twy = ty * w, twz = tz * w;
-    const C txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y, tyz = tz * y, tzz = tz * z;
-    m[0] = 1 - (txx + tyy); m[1] = tyz + twx; m[2] = txz - twy;
-    m[3] = tyz - twx; m[4] = 1 - (txx + tzz); m[5] = txy + twz;
-    m[6] = txz + twy; m[7] = txy - twz; m[8] = 1 - (tyy + tzz);
-}
-
 // Windowed sinc of the culled kernels in single precision: a polynomial of degree NWIN in t = 2 (fz/f_b)^2 - 1
 // (the window is even in fz), fitted on the host per (f_s, f_b) and accepted only when its error, evaluated
 // in single precision, stays below 4e-7 on [-f_b, f_b] (noa's FP32 expression itself carries ~2e-7 of rounding).

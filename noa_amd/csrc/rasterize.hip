@@ -178,6 +178,15 @@ struct RasterTileParams {
 
 __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 
+// The plane geometry of the tiled path (crossing test, candidate enumeration) takes the rotation as a matrix; the
+// per-pixel expressions keep the rotation as given (noa's quaternion rotation for quaternions).
+__device__ __forceinline__ Rot<float> as_matrix(const Rot<float>& r) {
+    Rot<float> g;
+    rot_matrix(r, g.m);
+    g.quat = false;
+    return g;
+}
+
 template<class In, class Out>
 __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTileParams<In, Out> p) {
     using RIn = typename vt<In>::real;
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
     sc.identity = true;
 
     for (i32 c0 = first; c0 < end; c0 += 64) {
-        const bool cross = c0 + lane < end && slab_crossed<RS_Z>(load_rotation(p.rot, c0 + lane), cz, cy, cx, M);
+        const bool cross = c0 + lane < end && slab_crossed<RS_Z>(as_matrix(load_rotation(p.rot, c0 + lane)), cz, cy, cx, M);
         unsigned long long mask = __ballot(cross);
         while (mask) {
             const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
@@ -328,8 +337,8 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                 }
             };
 
-            tile_candidates<Cand, RS_Z, RS_LIST>(r, Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered, lane, wl,
-                                                 fetch, store);
+            tile_candidates<Cand, RS_Z, RS_LIST>(as_matrix(r), Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered,
+                                                 lane, wl, fetch, store);
         }
     }
     lds_order();
@@ -390,7 +399,8 @@ int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStr
     const i64 stored = nn * nn * (nn / 2 + 1);
     // The candidate list of a wave must hold a whole slice row (n/2 + 1 pixels) for any matrix.
     const bool eligible = cubic && !p.general && nn >= 16 && nn / 2 + 1 <= RS_LIST && d.cutoff <= 0.5 &&
-                          p.rot.kind == NOA_HIP_MAT33_F32 && stored < 0x7fffffffLL && n_slices <= 0x7fffffffLL;
+                          (p.rot.kind == NOA_HIP_MAT33_F32 || p.rot.kind == NOA_HIP_QUAT_F32) &&
+                          stored < 0x7fffffffLL && n_slices <= 0x7fffffffLL;
     if (!eligible) return NOT_TILED;
     const char* env = std::getenv("NOA_HIP_RASTERIZE_LAYOUT");
     const bool want = env ? std::strcmp(env, "tiles") == 0 : n_slices >= 64;

@@ -182,3 +182,24 @@ def test_default_selection_by_slice_count(monkeypatch):
         torch.cuda.synchronize()
         want, _ = oracle("hc2hc", sl, None, b, n, rot, wvol=False)
         assert rel_l2(vol.cpu().numpy(), want) < 1e-6
+
+
+@pytest.mark.parametrize("remap,n,b", [("h2h", 64, 200), ("hc2hc", 48, 96)])
+def test_tiles_take_quaternions(monkeypatch, remap, n, b):
+    """Quaternion<f32> rotations on the tiled path (the plane geometry from their matrix, noa's quaternion rotation
+    per pixel): against the oracle with the same quaternions and against the direct kernel, rel-L2 1e-6."""
+    print(f"case rasterize tiles quaternions {remap} n={n} b={b}", flush=True)
+    rng = np.random.default_rng(1800 + n)
+    sl = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.1, 1.0, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    rot = random_rotations(b, rng)
+    rot[: len(axis_poses())] = axis_poses()
+    q = nx().quaternion_from_matrix(rot.astype(np.float64)).astype(np.float32)
+    want_v = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    want_w = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.rasterize_central_slices_3d(remap, sl, w, (b, 1, n, n), want_v, want_w, (1, n, n, n), None, q, quaternion=True,
+                                  n_threads=8)
+    got_v, got_w = gpu("tiles", monkeypatch, remap, dev(sl), dev(w), b, n, nx().Quaternions(dev(q)))
+    dir_v, dir_w = gpu("direct", monkeypatch, remap, dev(sl), dev(w), b, n, nx().Quaternions(dev(q)))
+    assert rel_l2(got_v, want_v) < 1e-6 and rel_l2(got_w, want_w) < 1e-6
+    assert rel_l2(got_v, dir_v) < 1e-6 and rel_l2(got_w, dir_w) < 1e-6
