@@ -698,8 +698,8 @@ __global__ __launch_bounds__(256) void slab_records_kernel(const XformArg<float>
                                                            float4* rec) {
     const i32 s = static_cast<i32>(blockIdx.x) * 256 + static_cast<i32>(threadIdx.x);
     if (s >= n_slices) return;
-    const Rot<float> r = load_rotation(rot, s);
-    const float* m = r.m;
+    float m[9];
+    rot_matrix(load_rotation(rot, s), m);      // a quaternion's matrix: its rotation up to rounding
     const float a0 = fabsf(m[0]), a1 = fabsf(m[1]), a2 = fabsf(m[2]);
     i32 ax = 0;
     if (a1 > a0) ax = 1;
@@ -1736,7 +1736,8 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
             i32 step = 8;
             while (p.n_slices > 16 * step) step *= 2;
             const i32 plan_slices = std::max<i32>(64, (p.n_slices + step - 1) / step * step);
-            // The slab kernel: Mat33<f32> rotations, LINEAR, one power-of-two size n >= 32 for the slices, the
+            // The slab kernel: Mat33<f32> rotations (or Quaternion<f32>, through their matrix: the coordinates are
+            // noa's quaternion rotation up to rounding, within the insertion's tolerance), LINEAR, one power-of-two size n >= 32 for the slices, the
             // volume and the target, cutoff <= 0.5, a window of at most ~1.15 samples (at most 4 candidates per column)
             // that the degree-9 polynomial fits, unit x strides and one row pitch for the slices and their weights.
             bool slab = false;
@@ -1746,7 +1747,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
                 const bool cubic = d.target[1] == nn && d.target[2] == nn && p.grid[0] == nn && p.grid[1] == nn &&
                                    p.grid[2] == nn && p.slice_shape[0] == nn && p.slice_shape[1] == nn;
                 if (!kc.force_wave && d.interp == NOA_HIP_INTERP_LINEAR && cubic && p.pow2 && nn >= 32 &&
-                    p.rot.kind == NOA_HIP_MAT33_F32 && cutoff <= C(0.5) && p.is_x == 1 &&
+                    (p.rot.kind == NOA_HIP_MAT33_F32 || p.rot.kind == NOA_HIP_QUAT_F32) && cutoff <= C(0.5) && p.is_x == 1 &&
                     (!p.win_present || (p.wis_x == 1 && p.wis_y == p.is_y)) && p.slice_bytes < 0x7ffffff0u &&
                     p.wslice_bytes < 0x7ffffff0u) {
                     sw = fit_slab_window(static_cast<float>(p.fftfreq_sinc), static_cast<float>(p.fftfreq_blackman),

@@ -375,6 +375,50 @@ def test_insert_rows_that_are_not_unit_vectors(monkeypatch, kernel):
     assert rel_l2(v, want) < 1e-6 and rel_l2(wv, wwant) < 1e-6
 
 
+@pytest.mark.parametrize("kernel", ["slab", "wave"])
+def test_insert_quaternions_on_both_culled_kernels(monkeypatch, kernel):
+    # A power-of-two size: the slab kernel takes Quaternion<f32> through the quaternions' matrices (coordinates equal
+    # to noa's quaternion rotation up to rounding), the one-wave kernel with noa's quaternion expressions.
+    print(f"case insert quaternions {kernel}", flush=True)
+    monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", kernel)
+    rng = np.random.default_rng(135)
+    n, b = 64, 300
+    fwd = random_rotations(b, rng, dtype=np.float64)
+    inv = np.ascontiguousarray(np.transpose(fwd, (0, 2, 1)))
+    q = nx().quaternion_from_matrix(inv).astype(np.float32)
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d("hc2hc", s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, q, quaternion=True,
+                               n_threads=8)
+    got, gotw = _insert_dev("hc2hc", s, w, (b, 1, n, n), n, q, quat=True)
+    if kernel == "wave":
+        assert rel_l2(got, want) < 1e-6 and rel_l2(gotw, wwant) < 1e-6
+        return
+    # The slab kernel equals the oracle fed the quaternions' float32 matrices; the oracle's own quaternion and matrix
+    # paths for one rotation differ by rounding (rel-L2 4.2e-6 for these inputs, on the CPU), hence 1e-5 against it.
+    mat = _quaternion_matrices(q)
+    mwant = np.zeros_like(want)
+    mwwant = np.zeros_like(wwant)
+    O.insert_central_slices_3d("hc2hc", s, w, (b, 1, n, n), mwant, mwwant, (1, n, n, n), None, mat, n_threads=8)
+    assert rel_l2(got, mwant) < 1e-6 and rel_l2(gotw, mwwant) < 1e-6
+    assert rel_l2(got, want) < 1e-5 and rel_l2(gotw, wwant) < 1e-5
+
+
+def _quaternion_matrices(q):
+    """Rotation matrices of (z, y, x, w) unit quaternions in float32, in the order of operations of the kernels'
+    conversion; the rotation itself is checked against the quaternion oracle."""
+    f = np.float32
+    z, y, x, w = (q[:, i].astype(f) for i in range(4))
+    tx, ty, tz = f(2) * x, f(2) * y, f(2) * z
+    twx, twy, twz = tx * w, ty * w, tz * w
+    txx, txy, txz, tyy, tyz, tzz = tx * x, ty * x, tz * x, ty * y, tz * y, tz * z
+    m = np.stack([f(1) - (txx + tyy), tyz + twx, txz - twy, tyz - twx, f(1) - (txx + tzz), txy + twz,
+                  txz + twy, txy - twz, f(1) - (tyy + tzz)], 1)
+    return np.ascontiguousarray(m.reshape(-1, 3, 3).astype(f))
+
+
 def test_insert_oversampled_and_odd_sizes():
     # Slices larger than the volume (footprints exceed the LDS staging: global-read fallback), odd volume size,
     # explicit target shape and a cutoff above 0.5.
