@@ -169,11 +169,11 @@ struct RasterTileParams {
     const float* win; i64 wis_b; i32 wis_y, wis_x; bool win_present, win_is_value; float win_value;
     Out* vol; i32 vs0, vs1, vs2;
     float* wvol; i32 wvs0, wvs1, wvs2; bool has_wvol;
-    XformArg<float> rot;
+    XformArg<float> rot, scale;         // scale: Mat22 inverse scaling, or none
     const int4* items;                  // {tile, first slice, end slice, 0}
     i32 n, nty, ntx, base, n_slices, pixels;
     float inv_wr, cutoff_sqd, margin;
-    bool vol_centered, slice_centered;
+    bool vol_centered, slice_centered, scaled;
 };
 
 __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
@@ -185,6 +185,27 @@ __device__ __forceinline__ Rot<float> as_matrix(const Rot<float>& r) {
     rot_matrix(r, g.m);
     g.quat = false;
     return g;
+}
+
+// With an inverse scaling S (applied to the 2D frequency before the rotation), pixel (y, x) lands at
+// y (s0 a + s2 c) + x (s1 a + s3 c), with a and c the rotation's 2nd and 3rd columns: the same plane, other columns.
+__device__ __forceinline__ Rot<float> plane_matrix(const Rot<float>& r, const Scale<float>& s) {
+    Rot<float> g = as_matrix(r);
+    if (s.identity) return g;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float a = g.m[3 * i + 1], c = g.m[3 * i + 2];
+        g.m[3 * i + 1] = s.m[0] * a + s.m[2] * c;
+        g.m[3 * i + 2] = s.m[1] * a + s.m[3] * c;
+    }
+    return g;
+}
+
+__device__ __forceinline__ Scale<float> slice_scaling(const XformArg<float>& a, bool scaled, i64 b) {
+    Scale<float> s;
+    s.identity = true;
+    if (scaled) s = load_scaling(a, b);
+    return s;
 }
 
 template<class In, class Out>
@@ -218,17 +239,18 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
     const float M = p.margin;
     const float cz = static_cast<float>(Z0) + 0.5f * RS_Z, cy = static_cast<float>(Y0) + 0.5f * XT;
     const float cx = static_cast<float>(X0) + 0.5f * XT;
-    Scale<float> sc;
-    sc.identity = true;
-
     for (i32 c0 = first; c0 < end; c0 += 64) {
-        const bool cross = c0 + lane < end && slab_crossed<RS_Z>(as_matrix(load_rotation(p.rot, c0 + lane)), cz, cy, cx, M);
+        const bool cross = c0 + lane < end &&
+                           slab_crossed<RS_Z>(plane_matrix(load_rotation(p.rot, c0 + lane),
+                                                           slice_scaling(p.scale, p.scaled, c0 + lane)),
+                                              cz, cy, cx, M);
         unsigned long long mask = __ballot(cross);
         while (mask) {
             const i32 k = __ffsll(static_cast<long long>(mask)) - 1;
             mask &= mask - 1;
             const i32 b = c0 + k;
             const Rot<float> r = load_rotation(p.rot, b);
+            const Scale<float> sc = slice_scaling(p.scale, p.scaled, b);
 
             // fetch: noa's expressions up to the fractions and the input value and weight (the loads of two
             // candidates in flight per lane); store: the eight taps into LDS, and the x = 0 duplicates.
@@ -243,7 +265,8 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                 vec2<float> f2;
                 f2.y = static_cast<float>(fy) / slice_f;
                 f2.x = static_cast<float>(u) / slice_f;
-                vec3<float> f = slice2grid<false>(f2, sc, r, 0.f, 0.f);
+                // With a scaling: rasterize_kernel's general expression, without Ewald sphere (0 curvature).
+                vec3<float> f = p.scaled ? slice2grid<true>(f2, sc, r, 0.f, 0.f) : slice2grid<false>(f2, sc, r, 0.f, 0.f);
                 if (dot3(f) > p.cutoff_sqd) return;
                 const bool flip = f.x < 0;
                 if (flip != mirrored) return;
@@ -337,7 +360,7 @@ __global__ __launch_bounds__(256) void rasterize_tile_kernel(const RasterTilePar
                 }
             };
 
-            tile_candidates<Cand, RS_Z, RS_LIST>(as_matrix(r), Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered,
+            tile_candidates<Cand, RS_Z, RS_LIST>(plane_matrix(r, sc), Z0, Y0, X0, n, M, p.pixels, p.inv_wr, p.slice_centered,
                                                  lane, wl, fetch, store);
         }
     }
@@ -388,8 +411,8 @@ template<class T> T value_of(const Arr& a) {
 
 constexpr int NOT_TILED = -1000;
 
-// The tiled path (rasterize_tile_kernel) where it applies: cubic volume and slices, no target shape, no scaling or
-// Ewald sphere, cutoff <= 0.5, Mat33<f32> rotations. NOA_HIP_RASTERIZE_LAYOUT=direct|tiles forces either; by default
+// The tiled path (rasterize_tile_kernel) where it applies: cubic volume and slices, no target shape, no Ewald
+// sphere, cutoff <= 0.5, Mat33<f32> or Quaternion<f32> rotations, Mat22 inverse scalings or none. NOA_HIP_RASTERIZE_LAYOUT=direct|tiles forces either; by default
 // the tiles take launches of at least 64 slices. NOA_HIP_RASTERIZE_PAIRS: expected crossing slices per work item.
 template<class In, class Out, class C, class W>
 int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStream_t stream, std::string& err) {
@@ -398,7 +421,8 @@ int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStr
                        d.target[0] == nn && d.target[1] == nn && d.target[2] == nn;
     const i64 stored = nn * nn * (nn / 2 + 1);
     // The candidate list of a wave must hold a whole slice row (n/2 + 1 pixels) for any matrix.
-    const bool eligible = cubic && !p.general && nn >= 16 && nn / 2 + 1 <= RS_LIST && d.cutoff <= 0.5 &&
+    const bool flat = p.ews[0] == 0 && p.ews[1] == 0;
+    const bool eligible = cubic && flat && nn >= 16 && nn / 2 + 1 <= RS_LIST && d.cutoff <= 0.5 &&
                           (p.rot.kind == NOA_HIP_MAT33_F32 || p.rot.kind == NOA_HIP_QUAT_F32) &&
                           stored < 0x7fffffffLL && n_slices <= 0x7fffffffLL;
     if (!eligible) return NOT_TILED;
@@ -424,6 +448,8 @@ int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStr
     q.vol = p.vol; q.vs0 = p.vs0; q.vs1 = p.vs1; q.vs2 = p.vs2;
     q.wvol = p.wvol; q.wvs0 = p.wvs0; q.wvs1 = p.wvs1; q.wvs2 = p.wvs2; q.has_wvol = p.has_wvol;
     q.rot = p.rot;
+    q.scale = p.scale;
+    q.scaled = p.general && p.scale.kind != NOA_HIP_XFORM_NONE;
     q.items = plan.items;
     q.n = static_cast<i32>(nn);
     q.nty = plan.nty; q.ntx = plan.ntx; q.base = plan.base;

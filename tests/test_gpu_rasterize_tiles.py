@@ -1,8 +1,8 @@
 """The tiled rasterisation (LDS accumulators per 16^3 volume tile, each tile visited by the slices whose plane
 crosses it, one device-wide atomic flush per work item).
 
-It takes launches of 64 slices or more into a cubic volume with FP32 values and weights, Mat33<f32> rotations and
-cutoff <= 0.5; NOA_HIP_RASTERIZE_LAYOUT=tiles|direct forces either path. Bars: rel-L2 1e-6 of values and weights
+It takes launches of 64 slices or more into a cubic volume with FP32 values and weights, Mat33<f32> or
+Quaternion<f32> rotations, Mat22 inverse scalings or none, no Ewald curvature and cutoff <= 0.5; NOA_HIP_RASTERIZE_LAYOUT=tiles|direct forces either path. Bars: rel-L2 1e-6 of values and weights
 against the oracle (noa's per-pixel scatter; the sums are regrouped, by LDS and device atomics in both paths) and
 against the one-lane-per-pixel kernel.
 """
@@ -203,3 +203,45 @@ def test_tiles_take_quaternions(monkeypatch, remap, n, b):
     dir_v, dir_w = gpu("direct", monkeypatch, remap, dev(sl), dev(w), b, n, nx().Quaternions(dev(q)))
     assert rel_l2(got_v, want_v) < 1e-6 and rel_l2(got_w, want_w) < 1e-6
     assert rel_l2(got_v, dir_v) < 1e-6 and rel_l2(got_w, dir_w) < 1e-6
+
+
+def _scalings(b, rng):
+    """Anisotropic inverse scalings (magnification 0.9-1.1 along a random in-plane axis): R(t) diag(s) R(-t)."""
+    t = rng.uniform(0, np.pi, b)
+    s = rng.uniform(0.9, 1.1, (b, 2))
+    c, d = np.cos(t), np.sin(t)
+    rt = np.stack([np.stack([c, -d], -1), np.stack([d, c], -1)], -2)
+    return np.ascontiguousarray(rt @ (s[:, :, None] * np.transpose(rt, (0, 2, 1)))).astype(np.float32)
+
+
+@pytest.mark.parametrize("remap,n,b,quat", [("h2h", 64, 200, False), ("hc2hc", 48, 96, False), ("h2h", 64, 120, True)])
+def test_tiles_take_inverse_scalings(monkeypatch, remap, n, b, quat):
+    """Mat22 inverse scalings on the tiled path (the plane's pixel columns scaled, noa's general expression per
+    pixel without Ewald curvature): against the oracle with the same scalings and against the direct kernel,
+    rel-L2 1e-6."""
+    print(f"case rasterize tiles scalings {remap} n={n} b={b} quat={quat}", flush=True)
+    rng = np.random.default_rng(1900 + n + b)
+    sl = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.1, 1.0, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    rot = random_rotations(b, rng)
+    rot[: len(axis_poses())] = axis_poses()
+    scale = _scalings(b, rng)
+    if quat:
+        rot = nx().quaternion_from_matrix(rot.astype(np.float64)).astype(np.float32)
+    want_v = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    want_w = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.rasterize_central_slices_3d(remap, sl, w, (b, 1, n, n), want_v, want_w, (1, n, n, n), scale, rot,
+                                  quaternion=quat, n_threads=8)
+    got, direct = {}, {}
+    for layout, out in (("tiles", got), ("direct", direct)):
+        monkeypatch.setenv("NOA_HIP_RASTERIZE_LAYOUT", layout)
+        vol = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.complex64, device=DEV)
+        wv = torch.zeros((1, n, n, n // 2 + 1), dtype=torch.float32, device=DEV)
+        r = nx().Quaternions(dev(rot)) if quat else dev(rot)
+        nx().rasterize_central_slices_3d(remap, dev(sl), dev(w), (b, 1, n, n), vol, wv, (1, n, n, n), dev(scale), r,
+                                         nx().RasterizeCentralSlicesOptions())
+        torch.cuda.synchronize()
+        out["v"], out["w"] = vol.cpu().numpy(), wv.cpu().numpy()
+    assert np.abs(want_v).max() > 0
+    assert rel_l2(got["v"], want_v) < 1e-6 and rel_l2(got["w"], want_w) < 1e-6
+    assert rel_l2(got["v"], direct["v"]) < 1e-6 and rel_l2(got["w"], direct["w"]) < 1e-6
