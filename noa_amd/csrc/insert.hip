@@ -8,7 +8,7 @@
 // through the voxel contribute (about 0.6 % of the voxel-slice pairs at 256^3 with the default 1-voxel window);
 // the others add exactly zero. This file keeps noa's per-voxel expressions and skips the zero terms.
 //
-// Culled paths (array inputs, single-precision values, no scaling/EWS):
+// Culled paths (array inputs, single-precision values, no Ewald curvature):
 //   - Work item = (8x8x16 sub-brick of the volume, range of slices), one wave per item, the sub-brick's
 //     accumulator in LDS: no global atomics.
 //   - The wave scans the item's slices 64 at a time: lane l tests whether the slab of slice s0+l meets the
@@ -20,11 +20,11 @@
 //     balanced. A sub-brick with one segment adds its sum to the volume directly; a split sub-brick writes one
 //     partial sum per segment, and a second kernel adds the partials of each sub-brick in segment order. The
 //     result is deterministic; it differs from noa's strict slice order only by this regrouping of the sum.
-//   - insert_slab_kernel serves the common configuration (one power-of-two size, Mat33<f32>, LINEAR, cutoff
-//     <= 0.5) with per-slice records and a leaner per-voxel path; insert_wave_kernel serves the others
-//     (quaternions, NEAREST, other sizes and cutoffs).
-// General path (scaling or EWS, constant-value inputs, double precision): noa's per-voxel gather over every
-// slice, one lane per voxel.
+//   - insert_slab_kernel serves the common configuration (one power-of-two size, Mat33<f32> or Quaternion<f32>,
+//     LINEAR, cutoff <= 0.5, no scaling) with per-slice records and a leaner per-voxel path; insert_wave_kernel
+//     serves the others (inverse scalings, NEAREST, other sizes and cutoffs).
+// General path (EWS, constant-value inputs, double precision): noa's per-voxel gather over every slice, one lane
+// per voxel.
 #include <type_traits>
 
 #include "common.hpp"
@@ -395,9 +395,14 @@ __device__ __forceinline__ void insert_wave_item(const InsertParams<C, In, W>& p
         return chunk0 + k;
     };
 
+    // Inverse scalings (no Ewald curvature on this path): the 2D frequency of grid2slice is scaled, fz is not.
+    const bool scaled = HOT == 0 && p.general;
     Rot<C> rot{};
     AxisInfo<C> ai{};
     for (i32 cur = next_slice(rot, ai); cur >= 0; cur = next_slice(rot, ai)) {
+        Scale<C> sc;
+        sc.identity = true;
+        if (scaled) sc = load_scaling(p.scale, cur);
         // Face of axis A: B (z, or y when A is z; 8 voxels) and Cc (x, or y when A is x). Columns: 8 x FC.
         const i32 ax = ai.flags & 3;
         const bool ranged = ai.flags & 16;
@@ -507,10 +512,9 @@ __device__ __forceinline__ void insert_wave_item(const InsertParams<C, In, W>& p
                 } else {
                     f.z = static_cast<C>(frz) / t0; f.y = static_cast<C>(fry) / t1; f.x = static_cast<C>(frx) / t2;
                 }
-                Scale<C> sc;
-                sc.identity = true;
                 vec2<C> f2;
-                const C fz = grid2slice<false>(f, sc, rot, C(0), C(0), f2);
+                const C fz = scaled ? grid2slice<true>(f, sc, rot, C(0), C(0), f2)
+                                    : grid2slice<false>(f, sc, rot, C(0), C(0), f2);
                 valid[g] = base + g * 64 + lane < total && !(dot3(f) > cutoff_sqd) && fabs(fz) <= fb;
                 if constexpr (sizeof(C) == 4) {
                     window[g] = wpoly ? window_poly(fz, inv_fb, wa) : windowed_sinc<C>(fz, fs, fb);
@@ -1542,6 +1546,7 @@ void make_pose_plan(const std::vector<i32>& bricks, const std::vector<i32>& coun
 // (NOA_HIP_INSERT_KERNEL=wave forces the one-wave kernel), and the expected passing slices per item.
 struct KernelChoice {
     bool force_wave;
+    bool force_general;   // NOA_HIP_INSERT_KERNEL=general: noa's per-voxel gather over every slice
     bool paired;          // slab kernel: one 16-byte (8-byte) load for the two x taps of a slice (weight) row
     int order;            // order of the work items (make_plan)
     double list_target;   // expected passing slices per work item; 0: the kernel's default
@@ -1560,7 +1565,10 @@ KernelChoice kernel_choice(const ProjectDesc& d) {
     k.paired = v[2] != 2;
     k.order = v[3];
     if (const char* o = std::getenv("NOA_HIP_INSERT_ORDER")) k.order = std::atoi(o);
-    if (const char* e = std::getenv("NOA_HIP_INSERT_KERNEL")) k.force_wave = std::string(e) != "slab";
+    if (const char* e = std::getenv("NOA_HIP_INSERT_KERNEL")) {
+        k.force_wave = std::string(e) != "slab";
+        k.force_general = std::string(e) == "general";
+    }
     if (const char* pr = std::getenv("NOA_HIP_INSERT_PAIRED")) k.paired = std::string(pr) != "0";
     if (const char* l = std::getenv("NOA_HIP_INSERT_LIST")) k.list_target = std::max(16.0, std::atof(l));
     // At most 1,024 slices per work item: a sub-brick crossed by many more slices than the uniform model expects
@@ -1726,7 +1734,10 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
 
     constexpr bool single = sizeof(typename vt<In>::real) == 4 && sizeof(W) == 4;
     if constexpr (single) {
-        if (!p.general && !p.in_is_value && !p.win_is_value && buffers_ok && window_size(d.interp) == 2) {
+        // An inverse scaling without Ewald curvature leaves fz (the culling) as it is and scales the 2D frequency.
+        const bool flat = p.ews[0] == 0 && p.ews[1] == 0;
+        if ((!p.general || flat) && !kc.force_general && !p.in_is_value && !p.win_is_value && buffers_ok &&
+            window_size(d.interp) == 2) {
             int device = 0;
             (void)hipGetDevice(&device);
             const Plan* planp = nullptr;
@@ -1746,7 +1757,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
                 const i64 nn = d.target[0];
                 const bool cubic = d.target[1] == nn && d.target[2] == nn && p.grid[0] == nn && p.grid[1] == nn &&
                                    p.grid[2] == nn && p.slice_shape[0] == nn && p.slice_shape[1] == nn;
-                if (!kc.force_wave && d.interp == NOA_HIP_INTERP_LINEAR && cubic && p.pow2 && nn >= 32 &&
+                if (!kc.force_wave && !p.general && d.interp == NOA_HIP_INTERP_LINEAR && cubic && p.pow2 && nn >= 32 &&
                     (p.rot.kind == NOA_HIP_MAT33_F32 || p.rot.kind == NOA_HIP_QUAT_F32) && cutoff <= C(0.5) && p.is_x == 1 &&
                     (!p.win_present || (p.wis_x == 1 && p.wis_y == p.is_y)) && p.slice_bytes < 0x7ffffff0u &&
                     p.wslice_bytes < 0x7ffffff0u) {
@@ -1896,7 +1907,7 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
             const dim3 grid_items(static_cast<unsigned>(plan.n_items));
             // The compile-time configurations: both layouts centred or both not, and the options HOT fixes.
             const bool quat = p.rot.kind == NOA_HIP_QUAT_F32 || p.rot.kind == NOA_HIP_QUAT_F64;
-            const bool hot_opts = p.pow2 && !quat && p.win_present && p.has_wvol && p.wpoly &&
+            const bool hot_opts = p.pow2 && !quat && !p.general && p.win_present && p.has_wvol && p.wpoly &&
                                   p.vol_centered == p.slices_centered && p.is_x == 1 && p.wis_x == 1 &&
                                   p.is_y == p.wis_y && p.slice_bytes < 0x7ffffff0u && p.wslice_bytes < 0x7ffffff0u;
             const int hot = hot_opts ? (p.vol_centered ? 2 : 1) : 0;

@@ -289,9 +289,10 @@ def _insert_dev(remap, s, w, shape, n, inv, scale=None, opts=None, out_dtype=tor
 
 @pytest.mark.parametrize("remap,interp,fb", [("h2h", 1, -1.0), ("hc2hc", 1, 0.05), ("hc2h", 0, -1.0),
                                              ("h2hc", 1, 0.02)])
-def test_insert_culled_equals_general_path(remap, interp, fb):
-    # A batched identity scaling selects the general (one lane per voxel, every slice) kernel; no scaling selects
-    # the culled one-wave kernel (48 is not a power of two). Both sum the same terms in the same order.
+def test_insert_culled_equals_general_path(monkeypatch, remap, interp, fb):
+    # NOA_HIP_INSERT_KERNEL=general selects the general (one lane per voxel, every slice) kernel; by default the
+    # culled one-wave kernel runs (48 is not a power of two), also with a batched identity scaling. All sum the same
+    # terms in the same order.
     print(f"case insert culled-vs-general {remap} interp={interp} fb={fb}", flush=True)
     rng = np.random.default_rng(130)
     n, b = 48, 40
@@ -301,8 +302,12 @@ def test_insert_culled_equals_general_path(remap, interp, fb):
     opts = nx().InsertCentralSlicesOptions(interp=nx().Interp(interp), windowed_sinc=nx().WindowedSinc(-1, fb))
     eye = torch.eye(2, device=DEV).repeat(b, 1, 1)
     v1, w1 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, None, opts)
-    v2, w2 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, eye, opts)
+    v3, w3 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, eye, opts)
+    monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", "general")
+    v2, w2 = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, None, opts)
     assert rel_l2(v1, v2) < 1e-6 and rel_l2(w1, w2) < 1e-6
+    np.testing.assert_array_equal(v3, v1)
+    np.testing.assert_array_equal(w3, w1)
     want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
     wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
     O.insert_central_slices_3d(remap, s, w, (b, 1, n, n), want, wwant, (1, n, n, n), None, inv, interp=interp,
@@ -417,6 +422,38 @@ def _quaternion_matrices(q):
     m = np.stack([f(1) - (txx + tyy), tyz + twx, txz - twy, tyz - twx, f(1) - (txx + tzz), txy + twz,
                   txz + twy, txy - twz, f(1) - (tyy + tzz)], 1)
     return np.ascontiguousarray(m.reshape(-1, 3, 3).astype(f))
+
+
+def _scalings(b, rng):
+    """Anisotropic inverse scalings (0.9-1.1 along a random in-plane axis): R(t) diag(s) R(-t)."""
+    t = rng.uniform(0, np.pi, b)
+    sv = rng.uniform(0.9, 1.1, (b, 2))
+    c, d = np.cos(t), np.sin(t)
+    rt = np.stack([np.stack([c, -d], -1), np.stack([d, c], -1)], -2)
+    return np.ascontiguousarray(rt @ (sv[:, :, None] * np.transpose(rt, (0, 2, 1)))).astype(np.float32)
+
+
+@pytest.mark.parametrize("remap,n,b,interp", [("h2h", 64, 200, 1), ("hc2hc", 48, 120, 1), ("hc2h", 40, 80, 0)])
+def test_insert_inverse_scalings_on_the_culled_kernel(monkeypatch, remap, n, b, interp):
+    # Mat22 inverse scalings: the one-wave kernel (fz and its culling are those of the rotation alone; the 2D
+    # frequency is scaled) against the oracle and against the general kernel, rel-L2 1e-6.
+    print(f"case insert scalings {remap} n={n} b={b} interp={interp}", flush=True)
+    rng = np.random.default_rng(1700 + n)
+    inv = np.ascontiguousarray(np.transpose(random_rotations(b, rng), (0, 2, 1)))
+    scale = _scalings(b, rng)
+    s = random_spectrum((b, 1, n, n // 2 + 1), rng)
+    w = rng.uniform(0.5, 1.5, (b, 1, n, n // 2 + 1)).astype(np.float32)
+    opts = nx().InsertCentralSlicesOptions(interp=nx().Interp(interp))
+    want = np.zeros((1, n, n, n // 2 + 1), np.complex64)
+    wwant = np.zeros((1, n, n, n // 2 + 1), np.float32)
+    O.insert_central_slices_3d(remap, s, w, (b, 1, n, n), want, wwant, (1, n, n, n), scale, inv, interp=interp,
+                               n_threads=8)
+    got, gotw = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, torch.from_numpy(scale).to(DEV), opts)
+    monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", "general")
+    gen, genw = _insert_dev(remap, s, w, (b, 1, n, n), n, inv, torch.from_numpy(scale).to(DEV), opts)
+    assert np.abs(want).max() > 0
+    assert rel_l2(got, want) < 1e-6 and rel_l2(gotw, wwant) < 1e-6
+    assert rel_l2(got, gen) < 1e-6 and rel_l2(gotw, genw) < 1e-6
 
 
 def test_insert_oversampled_and_odd_sizes():
