@@ -315,7 +315,8 @@ int noa_hip_volume_blocks(const noa_hip_view* volume, void* blocks, int64_t capa
  *   NOA_HIP_OP_EXTRACT:       values = {volume layout: 0 auto, 1 row-major gather, 2 block-copy gather, 3 LDS tiles;
  *                                       expected crossing slices per tile work item (default 128); -; -}
  *   NOA_HIP_OP_INSERT:        values = {kernel: 0 auto (slab where it applies), 1 one-wave, 2 slab; expected
- *                                       slices per work item (default 512 for the slab kernel, 256 for the one-wave kernel); x taps of a slab row: 0 auto (one load),
+ *                                       slices per work item (default: at most 512 for the slab kernel, 256 for the one-wave kernel,
+ *                                       fewer for calls whose expected work fills fewer than 4,096 items); x taps of a slab row: 0 auto (one load),
  *                                       1 one load, 2 two loads; item order: 0 by work, 1 by first slice, 2 XCD-aware,
  *                                       3 pose-adaptive (passing slices counted on the device first; the call then
  *                                       waits for the count, so it is not for stream capture)}

@@ -1320,7 +1320,13 @@ struct Plan {
     std::vector<i32> h_nseg;               // segments of each active sub-brick in this plan
     i32* bricks{};                         // device copy of h_bricks, uploaded on first use
     bool owned{};                          // device arrays freed with the plan (pose-adaptive plans, one call)
+    double target{};                       // expected passing slices per item the plan was cut for
 };
+
+// The default work-item size adapts to the call: at most the kernel's value, and small enough that the call's
+// expected passing (sub-brick, slice) pairs fill about this many items (a few per resident wave of the GPU), so that
+// calls of a few hundred or thousand slices keep every CU busy instead of a few long items.
+constexpr double PLAN_MIN_ITEMS = 4096;
 using PlanKey = std::tuple<int, i32, i32, i32, i64, i64, i64, double, bool, i32, double, i32, i32, i32, double, int, i32>;
 std::mutex g_plan_mutex;
 std::map<PlanKey, Plan> g_plans;
@@ -1367,6 +1373,8 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
     const i32 nb[3] = {(ext[0] + bd[0] - 1) / bd[0], (ext[1] + bd[1] - 1) / bd[1], (ext[2] + bd[2] - 1) / bd[2]};
     const double c2 = std::max(cutoff, 0.0) * std::max(cutoff, 0.0);
     struct Entry { double work; int4 item; };
+    struct Active { i32 brick; double expected; };
+    std::vector<Active> active;
     std::vector<Entry> entries;
     std::vector<int4> reduce;
     i32 n_partials = 0;
@@ -1393,13 +1401,23 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
                     t += 0.25 * (hi[a] - lo[a]);   // E|n_a| = 1/2 times the half-extent
                 }
                 if (closest > c2 * (1 + 1e-6) + 1e-12) continue;
-                const i32 brick = (bz * nb[1] + by) * nb[2] + bx;
                 const double density = std::min(1.0, t / std::max(std::sqrt(r2), 1e-12));
-                const double expected = density * n_slices;
+                active.push_back({(bz * nb[1] + by) * nb[2] + bx, density * n_slices});
+            }
+    // A negative list_target is the default: at most -list_target slices per item, fewer for small calls.
+    double per_item = list_target;
+    if (list_target < 0) {
+        double total = 0;
+        for (const Active& a : active) total += a.expected;
+        per_item = std::min(-list_target, std::max(16.0, total / PLAN_MIN_ITEMS));
+    }
+    for (const Active& a : active) {
+                const i32 brick = a.brick;
+                const double expected = a.expected;
                 // At most max_span slices per segment (0: no bound): a sub-brick crossed by many more slices than
                 // the model expects (orientation sets far from uniform) then still gets several items.
                 const i32 span_segs = max_span > 0 ? (n_slices + max_span - 1) / max_span : 1;
-                const i32 n_seg = std::max<i32>(span_segs, static_cast<i32>(std::ceil(expected / list_target)));
+                const i32 n_seg = std::max<i32>(span_segs, static_cast<i32>(std::ceil(expected / per_item)));
                 const i32 len = (n_slices + n_seg - 1) / n_seg;
                 const i32 first_slot = n_seg > 1 ? n_partials : -1;
                 i32 made = 0;
@@ -1464,6 +1482,7 @@ int make_plan(int device, const i32 grid[3], const i64 target[3], double cutoff,
     std::map<i32, i32> segments_of;
     for (const int4& it : items) ++segments_of[it.x];
     Plan& pl = g_plans[key];
+    pl.target = per_item;
     for (const auto& kv : segments_of) {
         pl.h_bricks.push_back(kv.first);
         pl.h_nseg.push_back(kv.second);
@@ -1769,7 +1788,10 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
             // Expected passing slices per work item: the table's value, else 512 for the slab kernel (12 resident
             // waves per CU: 4.6-4.7 ms against 5.0 ms at 256 for 10,000 slices at 256^3) and 256 for the one-wave
             // kernel.
-            const double list_target = kc.list_target > 0 ? kc.list_target : (slab ? 512.0 : 256.0);
+            // The default adapts to the call's work (make_plan); the allocation-free entry point keeps the fixed
+            // value, whose partial sums fit its scratch.
+            const double cap = slab ? 512.0 : 256.0;
+            const double list_target = kc.list_target > 0 ? kc.list_target : (scratch == nullptr ? -cap : cap);
             const int st = make_plan(device, p.grid, d.target, static_cast<double>(cutoff), p.vol_centered,
                                      plan_slices, static_cast<double>(p.fftfreq_blackman), p.bd, list_target,
                                      scratch == nullptr, planp, err, kc.order,
@@ -1831,9 +1853,9 @@ int run(const ProjectDesc& d, hipStream_t stream, std::string& err, const Scratc
                 for (i32 j = 0; j < nbk && !skewed; ++j) {
                     i64 total = 0;
                     for (i32 b = 0; b < CNT_BINS; ++b) total += counts[static_cast<size_t>(j) * CNT_BINS + b];
-                    skewed = static_cast<double>(total) > 2.0 * list_target * shared.h_nseg[static_cast<size_t>(j)];
+                    skewed = static_cast<double>(total) > 2.0 * shared.target * shared.h_nseg[static_cast<size_t>(j)];
                 }
-                if (skewed) make_pose_plan(shared.h_bricks, counts, p.n_slices, list_target, pose);
+                if (skewed) make_pose_plan(shared.h_bricks, counts, p.n_slices, shared.target, pose);
             }
             if (pose.n_items > 0) {
                 if (hipMallocAsync(&pose.items, pose.h_items.size() * sizeof(int4), stream) != hipSuccess ||

@@ -62,9 +62,11 @@ def both(remap, sl, w, inv, b, n):
 def test_noalloc_equals_the_allocating_entry_point_twice_in_a_row(monkeypatch, remap, n, b1, b2, kernel):
     print(f"case noalloc {remap} n={n} b={b1} then b={b2} kernel={kernel}", flush=True)
     monkeypatch.setenv("NOA_HIP_INSERT_KERNEL", kernel)
-    # The allocation-free entry point does not bound the slices per work item (its partial sums stay in the
-    # scratch); with the same bound off, the allocating entry point builds the same plan: the same bits.
+    # The allocation-free entry point does not bound the slices per work item and keeps the kernel's fixed work-item
+    # size (its partial sums stay in the scratch); with the same bound off and the same size, the allocating entry
+    # point builds the same plan: the same bits.
     monkeypatch.setenv("NOA_HIP_INSERT_SPAN", "0")
+    monkeypatch.setenv("NOA_HIP_INSERT_LIST", "512" if kernel == "slab" else "256")
     rng = np.random.default_rng(900 + n)
     first = inputs(rng, n, b1)
     second = inputs(rng, n, b2)
@@ -76,8 +78,8 @@ def test_noalloc_equals_the_allocating_entry_point_twice_in_a_row(monkeypatch, r
 
 
 def test_noalloc_matches_the_allocating_default_plan():
-    """With its default plan (at most 1,024 slices per work item) the allocating entry point regroups the partial
-    sums differently: the same result within rel-L2 1e-6."""
+    """With its default plan (at most 1,024 slices per work item, work items sized to the call) the allocating
+    entry point regroups the partial sums differently: the same result within rel-L2 1e-6."""
     print("case noalloc vs default plan h2h n=128 b=3000", flush=True)
     rng = np.random.default_rng(960)
     n, b = 128, 3000
