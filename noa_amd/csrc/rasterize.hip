@@ -435,7 +435,7 @@ int run_tiles(const ProjectDesc& d, const RasterParams<C, In, Out, W>& p, hipStr
     while (n_slices > 16 * step) step *= 2;
     const i32 plan_slices = static_cast<i32>((n_slices + step - 1) / step * step);
     const char* pe = std::getenv("NOA_HIP_RASTERIZE_PAIRS");
-    const i32 pairs = pe ? std::max(8, std::atoi(pe)) : 256;
+    const i32 pairs = pe ? std::max(8, std::atoi(pe)) : -256;   // default: at most 256, fewer for small calls
     TilePlan plan;
     const int st = make_tile_plan(device, static_cast<i32>(nn), std::max(d.cutoff, 0.0), plan_slices, pairs, plan, err);
     if (st != NOA_HIP_OK) return st;

@@ -178,6 +178,10 @@ struct TilePlan {
 inline std::mutex g_tile_mutex;
 inline std::map<std::tuple<int, i32, double, i32, i32>, TilePlan> g_tile_plans;
 
+// A negative `pairs` is a default: at most -pairs expected crossing slices per item, fewer when the call's expected
+// crossings would fill fewer than TILE_MIN_ITEMS items (small calls: more, shorter items keep every CU busy).
+constexpr double TILE_MIN_ITEMS = 512;
+
 inline int make_tile_plan(int device, i32 n, double cutoff, i32 n_slices, i32 pairs, TilePlan& plan, std::string& err) {
     const auto key = std::make_tuple(device, n, cutoff, n_slices, pairs);
     std::lock_guard<std::mutex> lock(g_tile_mutex);
@@ -193,6 +197,8 @@ inline int make_tile_plan(int device, i32 n, double cutoff, i32 n_slices, i32 pa
     pl.ntx = (n / 2 + 1 + XT - 1) / XT;
     const double rc = cutoff * n + 1.0;
     struct Entry { double work; int4 item; };
+    struct Active { i32 tile; double expected; };
+    std::vector<Active> active;
     std::vector<Entry> entries;
     for (i32 tz = 0; tz < pl.ntz; ++tz)
         for (i32 ty = 0; ty < pl.nty; ++ty)
@@ -208,15 +214,22 @@ inline int make_tile_plan(int device, i32 n, double cutoff, i32 n_slices, i32 pa
                 }
                 if (closest > rc * rc) continue;
                 const double frac = std::min(1.0, 1.5 * 0.5 * XT / std::max(std::sqrt(r2), 1e-9));
-                const double expected = frac * n_slices;
-                const i32 n_seg = std::max<i32>(1, static_cast<i32>(std::ceil(expected / pairs)));
-                const i32 len = (n_slices + n_seg - 1) / n_seg;
-                const i32 t = (tz * pl.nty + ty) * pl.ntx + tx;
-                for (i32 sgm = 0; sgm < n_seg; ++sgm) {
-                    const i32 b = sgm * len, e = std::min(n_slices, b + len);
-                    if (b < e) entries.push_back({expected / n_seg + (e - b) / 64.0, make_int4(t, b, e, 0)});
-                }
+                active.push_back({(tz * pl.nty + ty) * pl.ntx + tx, frac * n_slices});
             }
+    double per_item = pairs;
+    if (pairs < 0) {
+        double total = 0;
+        for (const Active& a : active) total += a.expected;
+        per_item = std::min(static_cast<double>(-pairs), std::max(8.0, total / TILE_MIN_ITEMS));
+    }
+    for (const Active& a : active) {
+        const i32 n_seg = std::max<i32>(1, static_cast<i32>(std::ceil(a.expected / per_item)));
+        const i32 len = (n_slices + n_seg - 1) / n_seg;
+        for (i32 sgm = 0; sgm < n_seg; ++sgm) {
+            const i32 b = sgm * len, e = std::min(n_slices, b + len);
+            if (b < e) entries.push_back({a.expected / n_seg + (e - b) / 64.0, make_int4(a.tile, b, e, 0)});
+        }
+    }
     std::stable_sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) { return a.work > b.work; });
     std::vector<int4> items(entries.size());
     for (size_t k = 0; k < entries.size(); ++k) items[k] = entries[k].item;
